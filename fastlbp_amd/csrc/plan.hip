@@ -1,0 +1,464 @@
+/*
+ * plan.hip -- the plain PARALL solve on one GPU: device layout, the two
+ * sweep kernels, and the chunked run loop with a device-side stop test.
+ *
+ * Device layout (all arrays owned by the plan, sized once at creation):
+ *   fac_off  u32[F+1]   factor-major edge ranges (h_row_ptr_vf)
+ *   fac_meta u32[F]     type | head-clamped bit | head slot << 8
+ *   fac_p    f64[F]     prob(); fac_q f64[F] prob_default()
+ *   fac_mask f64x2[F]   head_mask, read only for head-clamped factors
+ *   var_off  u32[V+1]   var-major edge ranges (h_row_ptr_fv)
+ *   var_edge u32[E]     factor-major slot of each var-major edge
+ *   m_vf     f64x2[E]   v->f messages, factor-major, (state0, state1) pairs
+ *   m_fv     f64x2[E]   f->v messages, factor-major
+ *   belief   f64[V]     P(x=1) of the last sweep (the reference's _oldBeliefsV)
+ *   ctl      u64[chunk] per-sweep max |b - b_old| as IEEE bits
+ *
+ * Replaces the reference's per-iteration sequence in csrc/ext/causal_bp.cu:
+ * calcNewMessageFusedAll (409-444), calcBeliefsV (483-515), the dist/max
+ * reduce with its blocking D->H read (879-886) and the two per-iteration
+ * device_vector allocations (876-877).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fastlbp.h"
+#include "flbp_internal.h"
+#include "plan_internal.h"
+#include "sweep_kernels.h"
+
+using namespace flbp::dev;
+
+namespace {
+
+/* one lane per factor; damping fused into the store */
+template <bool kDamp>
+__global__ __launch_bounds__(kBlock) void factor_pass(
+    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
+    const double *__restrict__ fac_p, const double *__restrict__ fac_q,
+    const double2 *__restrict__ fac_mask, const double2 *__restrict__ m_vf,
+    double2 *__restrict__ m_fv, uint32_t F, double damp, double keep,
+    const unsigned long long *__restrict__ ctl, int k, double tol) {
+    if (chunk_stopped(ctl, k, tol)) return;
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= F) return;
+    const uint32_t s = fac_off[f];
+    const int d = (int)(fac_off[f + 1] - s);
+    const uint32_t meta = fac_meta[f];
+    const uint32_t type = meta & 3u;
+    const bool clamped = (meta & 4u) != 0;
+    const int h = (int)(meta >> 8);
+    const double p = fac_p[f];
+    const double q = type ? fac_q[f] : 0.0;
+    double mask0 = 1.0, mask1 = 1.0;
+    if (clamped) {
+        const double2 mk = fac_mask[f];
+        mask0 = mk.x;
+        mask1 = mk.y;
+    }
+    auto store = [&](int t, double2 r) __attribute__((always_inline)) {
+        if (kDamp) {
+            const double2 old = m_fv[s + t];
+            r.x = keep * r.x + damp * old.x;
+            r.y = keep * r.y + damp * old.y;
+        }
+        m_fv[s + t] = r;
+    };
+    if (d <= kFacRegDeg) {
+        RegIn<kFacRegDeg> in;
+#pragma unroll
+        for (int j = 0; j < kFacRegDeg; j++) in.m[j] = j < d ? m_vf[s + j] : make_double2(0.0, 0.0);
+        const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
+        for (int t = 0; t < d; t++)
+            store(t, fv_msg<kFacRegDeg>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
+    } else {
+        GlobalIn in{m_vf + s};
+        const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
+        for (int t = 0; t < d; t++)
+            store(t, fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
+    }
+}
+
+/* one lane per variable: product, belief, max-diff, leave-one-out scatter */
+__global__ __launch_bounds__(kBlock) void var_pass(
+    const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
+    const double2 *__restrict__ m_fv, double2 *__restrict__ m_vf, double *__restrict__ belief,
+    uint32_t V, unsigned long long *__restrict__ ctl, int k, double tol) {
+    if (chunk_stopped(ctl, k, tol)) return;
+    __shared__ unsigned long long red[kBlock / 64];
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    unsigned long long diff_bits = 0;
+    if (v < V) {
+        const uint32_t s = var_off[v];
+        const int d = (int)(var_off[v + 1] - s);
+        double prod_0 = 1.0, prod_1 = 1.0;
+        uint32_t z0 = 0, z1 = 0;
+        uint32_t slot[kVarRegDeg];
+        double2 msg[kVarRegDeg];
+        auto accumulate = [&](double2 m) __attribute__((always_inline)) {
+            if (m.x == 0) z0++; else prod_0 *= m.x;
+            if (m.y == 0) z1++; else prod_1 *= m.y;
+            scale_pair(prod_0, prod_1);
+        };
+        if (d <= kVarRegDeg) {
+#pragma unroll
+            for (int j = 0; j < kVarRegDeg; j++) {
+                if (j < d) {
+                    slot[j] = var_edge[s + j];
+                    msg[j] = m_fv[slot[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kVarRegDeg; j++)
+                if (j < d) accumulate(msg[j]);
+        } else {
+            for (int j = 0; j < d; j++) accumulate(m_fv[var_edge[s + j]]);
+        }
+        /* getBeliefsVKernel + norm_functor + dist_functor */
+        const double b0 = prod_0 * (double)(z0 == 0);
+        const double b1 = prod_1 * (double)(z1 == 0);
+        const double b = b1 / (b0 + b1);
+        const double diff = fabs(belief[v] - b);
+        belief[v] = b;
+        diff_bits = (unsigned long long)__double_as_longlong(diff);
+        if (d <= kVarRegDeg) {
+#pragma unroll
+            for (int j = 0; j < kVarRegDeg; j++)
+                if (j < d) m_vf[slot[j]] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
+        } else {
+            for (int j = 0; j < d; j++) {
+                const uint32_t sl = var_edge[s + j];
+                m_vf[sl] = vf_msg(prod_0, prod_1, z0, z1, m_fv[sl]);
+            }
+        }
+    }
+    /* max over non-negative doubles == max over their bit patterns; a NaN
+     * (positive after fabs) wins, which stops the loop as NaN > tol is false */
+    diff_bits = wave_max_u64(diff_bits);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = diff_bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = red[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; w++) m = red[w] > m ? red[w] : m;
+        if (m) atomicMax(&ctl[k], m);
+    }
+}
+
+/* zero the per-sweep slots of a chunk (a kernel, so the sequence can be
+ * captured in a graph) */
+__global__ void clear_u64(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void fill_pairs(double2 *p, uint64_t n, double c) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double2(c, c);
+}
+
+/* beliefs straight after init with no sweep: prod = (c, c), zero counts 0 */
+__global__ void fill_f64(double *p, uint64_t n, double c) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = c;
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace
+
+namespace flbp {
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(FLBP_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+int Plan::launch_sweep(int k, double tol, double damping) {
+    const double keep = 1.0 - damping;
+    if (F) {
+        if (damping != 0.0)
+            hipLaunchKernelGGL(factor_pass<true>, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, stream,
+                               fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv, (uint32_t)F,
+                               damping, keep, ctl, k, tol);
+        else
+            hipLaunchKernelGGL(factor_pass<false>, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, stream,
+                               fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv, (uint32_t)F,
+                               damping, keep, ctl, k, tol);
+    }
+    if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
+    if (V)
+        hipLaunchKernelGGL(var_pass, dim3(grid_for(V, kBlock)), dim3(kBlock), 0, stream, var_off,
+                           var_edge, m_fv, m_vf, belief, (uint32_t)V, ctl, k, tol);
+    if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 2], stream));
+    HIP_TRY(hipGetLastError());
+    return FLBP_OK;
+}
+
+/* Enqueue n sweeps (n <= kMaxChunk) and return how many ran before the stop
+ * test fired, with the max diff of the last one that ran. */
+int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_diff) {
+    hipLaunchKernelGGL(clear_u64, dim3(1), dim3(kMaxChunk), 0, stream, ctl, kMaxChunk);
+    if (profiling) HIP_TRY(hipEventRecord(ev[0], stream));
+    for (int k = 0; k < n; k++) {
+        int rc = launch_sweep(k, tol, damping);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(host_ctl, ctl, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    ran = n;
+    for (int k = 0; k < n; k++) {
+        double dk;
+        std::memcpy(&dk, &host_ctl[k], sizeof dk);
+        last_diff = dk;
+        if (!(dk > tol)) {
+            ran = k + 1;
+            break;
+        }
+    }
+    if (profiling) {
+        for (int k = 0; k < ran; k++) {
+            float a = 0, b = 0;
+            HIP_TRY(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
+            HIP_TRY(hipEventElapsedTime(&b, ev[2 * k + 1], ev[2 * k + 2]));
+            prof_fac_ms += a;
+            prof_var_ms += b;
+        }
+        prof_fac_n += F ? ran : 0;
+        prof_var_n += V ? ran : 0;
+    }
+    return FLBP_OK;
+}
+
+Plan::~Plan() {
+    if (device >= 0) (void)hipSetDevice(device);
+    for (void *p : allocs) (void)hipFree(p);
+    if (host_ctl) (void)hipHostFree(host_ctl);
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+}  // namespace flbp
+
+using flbp::Plan;
+using flbp::hip_fail;
+
+extern "C" {
+
+int flbp_plan_create(const flbp_csr *c, int device, flbp_plan **out) {
+    if (!c || !out) return flbp::fail(FLBP_EINVAL, "flbp_plan_create: null argument");
+    const uint64_t V = c->n_vars, F = c->n_factors, E = c->n_edges;
+    if (V >= (1ull << 32) - 1 || F >= (1ull << 32) - 1 || E >= (1ull << 32) - 1)
+        return flbp::fail(FLBP_ERANGE, "flbp_plan_create: V, F and E must each be below 2^32-1");
+    int rc = flbp::validate_csr(*c);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return flbp::fail(FLBP_ENODEV, "flbp_plan_create: no HIP device");
+    if (device < 0 || device >= ndev) return flbp::fail(FLBP_EINVAL, "flbp_plan_create: bad device");
+    HIP_TRY(hipSetDevice(device));
+    auto *P = new (std::nothrow) Plan();
+    if (!P) return flbp::fail(FLBP_ENOMEM, "flbp_plan_create: out of memory");
+    P->device = device;
+    P->V = V;
+    P->F = F;
+    P->E = E;
+    rc = P->upload(*c);
+    if (rc) {
+        delete P;
+        return rc;
+    }
+    *out = reinterpret_cast<flbp_plan *>(P);
+    return FLBP_OK;
+}
+
+void flbp_plan_destroy(flbp_plan *plan) { delete reinterpret_cast<Plan *>(plan); }
+
+int flbp_plan_init(flbp_plan *plan, int logdomain) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_init: null plan");
+    HIP_TRY(hipSetDevice(P->device));
+    const double c = logdomain ? 0.0 : 1.0;
+    if (P->E) {
+        hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_vf, P->E, c);
+        hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_fv, P->E, c);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->init_value = c;
+    P->swept_since_init = false;
+    return FLBP_OK;
+}
+
+int flbp_plan_run(flbp_plan *plan, const flbp_run_opts *o, flbp_run_result *res) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || !o) return flbp::fail(FLBP_EINVAL, "flbp_plan_run: null argument");
+    if (!(o->damping >= 0.0 && o->damping < 1.0))
+        return flbp::fail(FLBP_EINVAL, "flbp_plan_run: damping must be in [0, 1)");
+    HIP_TRY(hipSetDevice(P->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    P->reset_profile();
+    double maxdiff = INFINITY;
+    uint64_t sweeps = 0;
+    int chunk = o->chunk ? (int)std::min<uint32_t>(o->chunk, flbp::kMaxChunk) : 4;
+    const bool adaptive = o->chunk == 0;
+    while (P->iters < o->maxiter && maxdiff > o->tol && elapsed() < o->maxtime) {
+        const int n = (int)std::min<uint64_t>((uint64_t)chunk, o->maxiter - P->iters);
+        int ran = 0;
+        double last = maxdiff;
+        int rc = P->run_chunk(n, o->tol, o->damping, ran, last);
+        if (rc) return rc;
+        P->iters += ran;
+        sweeps += ran;
+        maxdiff = last;
+        P->swept_since_init = true;
+        if (adaptive) chunk = std::min(chunk * 2, flbp::kMaxChunk);
+    }
+    P->last_maxdiff = maxdiff;
+    if (res) {
+        res->iters = P->iters;
+        res->sweeps = sweeps;
+        res->maxdiff = maxdiff;
+        res->seconds = elapsed();
+    }
+    return FLBP_OK;
+}
+
+int flbp_plan_run_fixed(flbp_plan *plan, uint64_t n_sweeps, double damping, flbp_run_result *res) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_run_fixed: null plan");
+    if (!(damping >= 0.0 && damping < 1.0))
+        return flbp::fail(FLBP_EINVAL, "flbp_plan_run_fixed: damping must be in [0, 1)");
+    HIP_TRY(hipSetDevice(P->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    P->reset_profile();
+    uint64_t done = 0;
+    double last = INFINITY;
+    while (done < n_sweeps) {
+        const int n = (int)std::min<uint64_t>(flbp::kMaxChunk, n_sweeps - done);
+        int ran = 0;
+        /* tol = -inf: the stop test never fires */
+        int rc = P->run_chunk(n, -INFINITY, damping, ran, last);
+        if (rc) return rc;
+        done += ran;
+        P->swept_since_init = true;
+    }
+    if (n_sweeps == 0 && !P->swept_since_init && P->V) {
+        /* run(tol, 0, hist) still computes beliefs from the init products:
+         * (c, c) with no zeros, i.e. c / (c + c) */
+        const double c = P->init_value;
+        const double b = c / (c + c);
+        hipLaunchKernelGGL(fill_f64, dim3(grid_for(P->V, 256)), dim3(256), 0, P->stream, P->belief, P->V, b);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(P->stream));
+    }
+    P->iters += n_sweeps;
+    P->last_maxdiff = last;
+    if (res) {
+        res->iters = P->iters;
+        res->sweeps = n_sweeps;
+        res->maxdiff = last;
+        res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return FLBP_OK;
+}
+
+int flbp_plan_beliefs(flbp_plan *plan, double *out, uint64_t n) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || (n && !out)) return flbp::fail(FLBP_EINVAL, "flbp_plan_beliefs: null argument");
+    if (n > P->V) return flbp::fail(FLBP_EINVAL, "flbp_plan_beliefs: n > n_vars");
+    HIP_TRY(hipSetDevice(P->device));
+    if (n) HIP_TRY(hipMemcpyAsync(out, P->belief, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return FLBP_OK;
+}
+
+int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facmajor) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_get_messages: null plan");
+    HIP_TRY(hipSetDevice(P->device));
+    std::vector<double> tmp(2 * P->E);
+    if (fv_varmajor && P->E) {
+        HIP_TRY(hipMemcpy(tmp.data(), P->m_fv, P->E * sizeof(double2), hipMemcpyDeviceToHost));
+        for (uint64_t k = 0; k < P->E; k++) {
+            const uint64_t s = P->h_var_edge[k];
+            fv_varmajor[2 * k] = tmp[2 * s];
+            fv_varmajor[2 * k + 1] = tmp[2 * s + 1];
+        }
+    }
+    if (vf_facmajor && P->E)
+        HIP_TRY(hipMemcpy(vf_facmajor, P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
+    return FLBP_OK;
+}
+
+int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const double *vf_facmajor) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_messages: null plan");
+    HIP_TRY(hipSetDevice(P->device));
+    if (fv_varmajor && P->E) {
+        std::vector<double> tmp(2 * P->E);
+        for (uint64_t k = 0; k < P->E; k++) {
+            const uint64_t s = P->h_var_edge[k];
+            tmp[2 * s] = fv_varmajor[2 * k];
+            tmp[2 * s + 1] = fv_varmajor[2 * k + 1];
+        }
+        HIP_TRY(hipMemcpy(P->m_fv, tmp.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    if (vf_facmajor && P->E)
+        HIP_TRY(hipMemcpy(P->m_vf, vf_facmajor, P->E * sizeof(double2), hipMemcpyHostToDevice));
+    return FLBP_OK;
+}
+
+int flbp_plan_set_p(flbp_plan *plan, const double *p, uint64_t n) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || (n && !p)) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_p: null argument");
+    if (n != P->F) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_p: n must equal n_factors");
+    HIP_TRY(hipSetDevice(P->device));
+    if (n) HIP_TRY(hipMemcpy(P->fac_p, p, n * sizeof(double), hipMemcpyHostToDevice));
+    return FLBP_OK;
+}
+
+int flbp_plan_profile(flbp_plan *plan, int enable) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_profile: null plan");
+    HIP_TRY(hipSetDevice(P->device));
+    if (enable && !P->ev[0]) {
+        for (auto &e : P->ev) HIP_TRY(hipEventCreate(&e));
+    }
+    P->profiling = enable != 0;
+    return FLBP_OK;
+}
+
+int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms, uint64_t *fac_n,
+                          uint64_t *var_n) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_last_timing: null plan");
+    if (fac_ms) *fac_ms = P->prof_fac_ms;
+    if (var_ms) *var_ms = P->prof_var_ms;
+    if (fac_n) *fac_n = P->prof_fac_n;
+    if (var_n) *var_n = P->prof_var_n;
+    return FLBP_OK;
+}
+
+uint64_t flbp_plan_device_bytes(const flbp_plan *plan) {
+    const Plan *P = reinterpret_cast<const Plan *>(plan);
+    return P ? P->device_bytes : 0;
+}
+
+}  // extern "C"

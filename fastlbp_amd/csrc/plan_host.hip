@@ -1,0 +1,118 @@
+/*
+ * plan_host.hip -- CSR validation and the upload of a graph into a plan's
+ * device layout (the H->D half of CausalBP::construct,
+ * csrc/ext/causal_bp.cu:368-369, 815-816).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "fastlbp.h"
+#include "flbp_internal.h"
+#include "plan_internal.h"
+#include "sweep_kernels.h"
+
+namespace flbp {
+
+int validate_csr(const flbp_csr &c) {
+    const uint64_t V = c.n_vars, F = c.n_factors, E = c.n_edges;
+    auto bad = [](const std::string &m) { return fail(FLBP_EINVAL, "invalid CSR: " + m); };
+    if (!c.fac_off || !c.var_off || (E && (!c.edge_var || !c.var_edge)) ||
+        (F && (!c.fac_type || !c.head_slot || !c.head_clamped || !c.p || !c.q || !c.mask0 || !c.mask1)))
+        return bad("null array");
+    if (c.fac_off[0] != 0 || c.fac_off[F] != E) return bad("fac_off must run from 0 to n_edges");
+    if (c.var_off[0] != 0 || c.var_off[V] != E) return bad("var_off must run from 0 to n_edges");
+    for (uint64_t I = 0; I < F; I++) {
+        if (c.fac_off[I + 1] <= c.fac_off[I]) return bad("every factor needs at least one variable");
+        if (c.fac_type[I] < 0 || c.fac_type[I] > 2) return bad("factor type must be 0, 1 or 2");
+        if (c.head_slot[I] < c.fac_off[I] || c.head_slot[I] >= c.fac_off[I + 1])
+            return bad("head slot outside its factor");
+        if (c.fac_off[I + 1] - c.fac_off[I] >= (1u << 24)) return fail(FLBP_ERANGE, "factor degree >= 2^24");
+    }
+    for (uint64_t i = 0; i < V; i++)
+        if (c.var_off[i + 1] < c.var_off[i]) return bad("var_off not monotone");
+    std::vector<uint8_t> seen(E, 0);
+    for (uint64_t i = 0; i < V; i++) {
+        for (uint64_t k = c.var_off[i]; k < c.var_off[i + 1]; k++) {
+            const uint64_t s = c.var_edge[k];
+            if (s >= E || seen[s]) return bad("var_edge is not a permutation of the slots");
+            if (c.edge_var[s] != i) return bad("var_edge and edge_var disagree");
+            seen[s] = 1;
+        }
+    }
+    return FLBP_OK;
+}
+
+template <class T>
+int Plan::alloc(T *&p, uint64_t n) {
+    const size_t bytes = std::max<uint64_t>(n, 1) * sizeof(T);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FLBP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                     hipGetErrorString(e));
+    }
+    allocs.push_back(q);
+    device_bytes += bytes;
+    p = static_cast<T *>(q);
+    return FLBP_OK;
+}
+
+#define TRY(expr)          \
+    do {                   \
+        int rc_ = (expr);  \
+        if (rc_) return rc_; \
+    } while (0)
+#define HIP_TRY(expr)                                     \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+int Plan::upload(const flbp_csr &c) {
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    TRY(alloc(fac_off, F + 1));
+    TRY(alloc(fac_meta, F));
+    TRY(alloc(fac_p, F));
+    TRY(alloc(fac_q, F));
+    TRY(alloc(fac_mask, F));
+    TRY(alloc(var_off, V + 1));
+    TRY(alloc(var_edge, E));
+    TRY(alloc(m_vf, E));
+    TRY(alloc(m_fv, E));
+    TRY(alloc(belief, V));
+    TRY(alloc(ctl, kMaxChunk));
+    HIP_TRY(hipHostMalloc((void **)&host_ctl, sizeof(unsigned long long) * kMaxChunk, hipHostMallocDefault));
+
+    std::vector<uint32_t> u32(std::max(std::max(F, V), E) + 1);
+    for (uint64_t I = 0; I <= F; I++) u32[I] = (uint32_t)c.fac_off[I];
+    HIP_TRY(hipMemcpy(fac_off, u32.data(), (F + 1) * 4, hipMemcpyHostToDevice));
+    for (uint64_t I = 0; I < F; I++)
+        u32[I] = dev::pack_meta((uint32_t)c.fac_type[I], c.head_clamped[I] != 0,
+                                (uint32_t)(c.head_slot[I] - c.fac_off[I]));
+    if (F) HIP_TRY(hipMemcpy(fac_meta, u32.data(), F * 4, hipMemcpyHostToDevice));
+    for (uint64_t i = 0; i <= V; i++) u32[i] = (uint32_t)c.var_off[i];
+    HIP_TRY(hipMemcpy(var_off, u32.data(), (V + 1) * 4, hipMemcpyHostToDevice));
+    for (uint64_t k = 0; k < E; k++) u32[k] = (uint32_t)c.var_edge[k];
+    if (E) HIP_TRY(hipMemcpy(var_edge, u32.data(), E * 4, hipMemcpyHostToDevice));
+    if (F) {
+        HIP_TRY(hipMemcpy(fac_p, c.p, F * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(fac_q, c.q, F * 8, hipMemcpyHostToDevice));
+        std::vector<double2> mk(F);
+        for (uint64_t I = 0; I < F; I++) mk[I] = make_double2(c.mask0[I], c.mask1[I]);
+        HIP_TRY(hipMemcpy(fac_mask, mk.data(), F * 16, hipMemcpyHostToDevice));
+    }
+    /* old beliefs are value-initialised to 0.0 (causal_bp.cu:371) */
+    HIP_TRY(hipMemset(belief, 0, std::max<uint64_t>(V, 1) * 8));
+    /* messages are zero until init(), as the reference's freshly resized
+     * device_vectors are */
+    HIP_TRY(hipMemset(m_vf, 0, std::max<uint64_t>(E, 1) * 16));
+    HIP_TRY(hipMemset(m_fv, 0, std::max<uint64_t>(E, 1) * 16));
+    h_var_edge.assign(c.var_edge, c.var_edge + E);
+    return FLBP_OK;
+}
+
+}  // namespace flbp

@@ -1,0 +1,61 @@
+/*
+ * plan_internal.h -- the plain-solve plan (device state of one graph).
+ */
+#ifndef FLBP_PLAN_INTERNAL_H
+#define FLBP_PLAN_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "fastlbp.h"
+
+namespace flbp {
+
+constexpr int kMaxChunk = 64;   /* sweeps enqueued between host checks */
+
+int hip_fail(hipError_t e, const char *what);
+int validate_csr(const flbp_csr &c);
+
+struct Plan {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    uint64_t V = 0, F = 0, E = 0;
+
+    uint32_t *fac_off = nullptr, *fac_meta = nullptr, *var_off = nullptr, *var_edge = nullptr;
+    double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
+    double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
+    unsigned long long *ctl = nullptr;
+    unsigned long long *host_ctl = nullptr;        /* pinned */
+    std::vector<void *> allocs;
+    uint64_t device_bytes = 0;
+
+    std::vector<uint64_t> h_var_edge;              /* for message export/import */
+
+    uint64_t iters = 0;                            /* CausalBP::_iters */
+    double last_maxdiff = INFINITY;
+    double init_value = 1.0;
+    bool swept_since_init = false;
+
+    bool profiling = false;
+    hipEvent_t ev[2 * kMaxChunk + 1] = {};
+    double prof_fac_ms = 0, prof_var_ms = 0;
+    uint64_t prof_fac_n = 0, prof_var_n = 0;
+
+    void reset_profile() {
+        prof_fac_ms = prof_var_ms = 0;
+        prof_fac_n = prof_var_n = 0;
+    }
+    int upload(const flbp_csr &c);
+    int launch_sweep(int k, double tol, double damping);
+    int run_chunk(int n, double tol, double damping, int &ran, double &last_diff);
+    ~Plan();
+
+    template <class T>
+    int alloc(T *&p, uint64_t n);
+};
+
+}  // namespace flbp
+
+#endif

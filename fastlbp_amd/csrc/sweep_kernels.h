@@ -1,0 +1,248 @@
+/*
+ * sweep_kernels.h -- device code of one PARALL sweep (gfx950).
+ *
+ * One sweep of the reference (csrc/ext/causal_bp.cu:409-444 + 483-515 +
+ * 879-886) is three per-edge/per-variable kernels, two belief kernels, a
+ * thrust transform and a thrust reduce.  Here it is two kernels:
+ *
+ *   factor pass  -- one lane per factor: loads the factor's v->f messages
+ *                   (contiguous, factor-major) into registers and writes every
+ *                   f->v message of the factor (contiguous, factor-major),
+ *                   damping fused into the store.
+ *   var pass     -- one lane per variable: gathers its f->v messages through
+ *                   the var-major -> factor-major slot map, forms the product
+ *                   with zero counts, the belief and |b - b_old|, then
+ *                   scatters the leave-one-out v->f messages through the same
+ *                   map; the max |b - b_old| is reduced in the wave (DPP
+ *                   shuffles), across the block in LDS, and across the grid
+ *                   with one 64-bit atomic max per block.
+ *
+ * Every message is computed with the reference's own operation order (the
+ * per-output loops of calcMessageFVFusedAllKernel, csrc/kernel/
+ * causal_bp_parall.cu:94-398, and calcProdAllKernel/calcMessageVFAllKernel,
+ * 20-92), so results are bit-identical to the reference's FP64 arithmetic.
+ * The one rewrite is scale() (causal_bp_parall.cu:14-18), which divides both
+ * values by their max: the max divided by itself is exactly 1 for any finite
+ * nonzero value, so only the other value is divided (half the FP64 divides);
+ * zero, infinite and NaN maxima keep the reference's NaN results.
+ */
+#ifndef FLBP_SWEEP_KERNELS_H
+#define FLBP_SWEEP_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flbp {
+namespace dev {
+
+constexpr int kFacRegDeg = 8;     /* factors up to this degree run from registers */
+constexpr int kVarRegDeg = 8;     /* variables up to this degree keep messages in registers */
+constexpr int kBlock = 256;
+
+/* factor meta word: [1:0] type, [2] head clamped, [31:8] head slot in factor */
+__host__ __device__ inline uint32_t pack_meta(uint32_t type, bool clamped, uint32_t hslot) {
+    return (type & 3u) | (clamped ? 4u : 0u) | (hslot << 8);
+}
+
+__device__ __forceinline__ void scale_pair(double &x, double &y) {
+    const double m = fmax(x, y);
+    const bool x_is_max = (x == m);
+    const double r = (x_is_max ? y : x) / m;
+    const double one = (m != 0.0 && __builtin_isfinite(m)) ? 1.0 : __builtin_nan("");
+    x = x_is_max ? one : r;
+    y = x_is_max ? r : one;
+}
+
+/* Messages of one factor, either held in registers or read from global
+ * memory.  get(j) is only called with indices that are compile-time
+ * constants after unrolling; pick(j) fetches a run-time slot with an AND/OR
+ * select over bit patterns, which the compiler cannot turn back into an
+ * indexed load (that would push the array into scratch memory). */
+template <int N>
+struct RegIn {
+    double2 m[N];
+    __device__ __forceinline__ double2 get(int j) const { return m[j]; }
+    __device__ __forceinline__ double2 pick(int j) const {
+        unsigned long long x = 0, y = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const unsigned long long sel = 0ull - (unsigned long long)(j == k);
+            x |= sel & (unsigned long long)__double_as_longlong(m[k].x);
+            y |= sel & (unsigned long long)__double_as_longlong(m[k].y);
+        }
+        return make_double2(__longlong_as_double((long long)x), __longlong_as_double((long long)y));
+    }
+};
+struct GlobalIn {
+    const double2 *__restrict__ m;
+    __device__ __forceinline__ double2 get(int j) const { return m[j]; }
+    __device__ __forceinline__ double2 pick(int j) const { return m[j]; }
+};
+
+/* Runs f(j) for j = 0..d-1 in order; unrolled when the degree bound N is a
+ * compile-time register count, a plain loop when N == 0. */
+template <int N, class F>
+__device__ __forceinline__ void for_slots(int d, F &&f) {
+    if constexpr (N > 0) {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (j < d) f(j);
+    } else {
+        for (int j = 0; j < d; j++) f(j);
+    }
+}
+
+/* The head-target message (codes 1-4) before masking: returns
+ * u = p*X + q*R and w = (1-p)*X + (1-q)*R where X is the product of the
+ * state the CPD singles out (state 1 for AND, state 0 for OR); AND gives
+ * (marg0, marg1) = (w, u), OR gives (u, w).  causal_bp_parall.cu:113-143,
+ * 152-182. */
+template <int N, class In>
+__device__ __forceinline__ void head_uw(const In &in, int d, int t, double2 m_other, bool is_and,
+                                        double p, double q, double &u, double &w) {
+    if (d == 1) {
+        u = p;
+        w = 1 - p;
+        return;
+    }
+    if (d == 2) {
+        const double2 m = m_other;   /* slot (t == 0 ? 1 : 0) */
+        const double x = is_and ? m.y : m.x, y = is_and ? m.x : m.y;
+        u = p * x + q * y;
+        w = (1 - p) * x + (1 - q) * y;
+        return;
+    }
+    double px = 1.0, p10 = 1.0, e1 = 0;
+    for_slots<N>(d, [&](int j) __attribute__((always_inline)) {
+        if (j != t) {
+            const double2 m = in.get(j);
+            const double a0 = m.x + m.y;
+            const double x = is_and ? m.y : m.x;
+            const double delta = is_and ? m.x : m.y;
+            px *= x;
+            p10 *= a0;
+            scale_pair(px, p10);
+            if (x != 0 && a0 == x && delta != 0) e1 += delta / x;
+        }
+    });
+    const double R = e1 * p10 + (p10 - px);
+    u = p * px + q * R;
+    w = (1 - p) * px + (1 - q) * R;
+}
+
+/* The body-target message (codes 5-8): causal_bp_parall.cu:289-393.
+ * Unclamped factors use masks (1, 1), which multiply exactly. */
+template <int N, class In>
+__device__ __forceinline__ double2 body_msg(const In &in, int d, int t, int h, double2 mh,
+                                            bool is_and, double p, double q, double mask0,
+                                            double mask1) {
+    const double hx = is_and ? mh.y : mh.x, hy = is_and ? mh.x : mh.y;
+    const double mx = is_and ? mask1 : mask0, my = is_and ? mask0 : mask1;
+    double P = (p - q) * (hx * mx - hy * my);
+    double Q = q * hx * mx + (1 - q) * hy * my;
+    scale_pair(P, Q);
+    if (Q != 0) {
+        for_slots<N>(d, [&](int j) __attribute__((always_inline)) {
+            if (j != t && j != h) {
+                const double2 m = in.get(j);
+                Q *= m.x + m.y;
+                P *= is_and ? m.y : m.x;
+                scale_pair(P, Q);
+            }
+        });
+    }
+    const double QP = Q + P;
+    const double marg = Q + QP;
+    double2 r;
+    if (marg != 0) {
+        const double a = Q / marg, b = QP / marg;
+        r.x = is_and ? a : b;
+        r.y = is_and ? b : a;
+    } else {
+        r.x = 0.5;
+        r.y = 0.5;
+    }
+    return r;
+}
+
+/* The f->v message to slot t of a factor (type, degree d, head slot h).
+ * mh = message of the head slot, m_other = message of slot (h == 0 ? 1 : 0)
+ * (the only other slot of a degree-2 factor). */
+template <int N, class In>
+__device__ __forceinline__ double2 fv_msg(const In &in, uint32_t type, bool clamped, int d, int t,
+                                          int h, double2 mh, double2 m_other, double p, double q,
+                                          double mask0, double mask1) {
+    double2 r;
+    if (type == 0) {
+        r.x = 1 - p;
+        r.y = p;
+        return r;
+    }
+    const bool is_and = (type == 1);
+    if (t != h) return body_msg<N>(in, d, t, h, mh, is_and, p, q, mask0, mask1);
+    double u, w;
+    head_uw<N>(in, d, t, m_other, is_and, p, q, u, w);
+    double marg0 = is_and ? w : u, marg1 = is_and ? u : w;
+    if (clamped) {
+        marg0 *= mask0;
+        marg1 *= mask1;
+    }
+    const double marg = marg0 + marg1;
+    if (marg != 0) {
+        r.x = marg0 / marg;
+        r.y = marg1 / marg;
+    } else if (clamped && mask0 == 0) {
+        r.x = 0.0;
+        r.y = 1.0;
+    } else if (clamped && mask1 == 0) {
+        r.x = 1.0;
+        r.y = 0.0;
+    } else {
+        r.x = 0.5;
+        r.y = 0.5;
+    }
+    return r;
+}
+
+/* leave-one-out v->f message, calcMessageVFAllKernel causal_bp_parall.cu:68-90 */
+__device__ __forceinline__ double2 vf_msg(double prod_0, double prod_1, uint32_t z0, uint32_t z1,
+                                          double2 m) {
+    if ((z0 >= 2) || (z0 == 1 && m.x != 0)) {
+        prod_0 = 0.0;
+    } else if (m.x != 0) {
+        prod_0 = prod_0 / m.x;
+    }
+    if ((z1 >= 2) || (z1 == 1 && m.y != 0)) {
+        prod_1 = 0.0;
+    } else if (m.y != 0) {
+        prod_1 = prod_1 / m.y;
+    }
+    if (prod_0 == 0 && prod_1 == 0) prod_0 = prod_1 = 0.5;
+    const double s = prod_0 + prod_1;
+    double2 r;
+    r.x = prod_0 / s;
+    r.y = prod_1 / s;
+    return r;
+}
+
+/* stop test: sweep k of a chunk runs only if sweep k-1 left max diff > tol
+ * (a slot that never ran reads 0 and stops every later sweep) */
+__device__ __forceinline__ bool chunk_stopped(const unsigned long long *ctl, int k, double tol) {
+    if (k <= 0) return false;
+    const double prev = __longlong_as_double((long long)__atomic_load_n(&ctl[k - 1], __ATOMIC_RELAXED));
+    return !(prev > tol);
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+}  // namespace dev
+}  // namespace flbp
+
+#endif

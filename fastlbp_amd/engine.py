@@ -1,0 +1,309 @@
+"""Python verification harness over libfastlbp.so.
+
+Thin numpy wrappers around the C ABI (include/fastlbp.h): factor lists,
+graphs in the reference's CSR orders, the synthetic generator, clamping,
+and the GPU plan.  Every call goes through the native library; nothing here
+computes BP itself.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from ._native import (CsrC, FactorListC, GenParamsC, RunOptsC, RunResultC, call, f64p, i32p,
+                      lib, u8p, u64p)
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _arr(ptr, n: int, dtype) -> np.ndarray:
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+@dataclass
+class FactorList:
+    """Flat std::vector<CausalFactor> (include/ext/causal_factor.h)."""
+
+    type: np.ndarray          # int32 [F]
+    head: np.ndarray          # uint64 [F]
+    body_off: np.ndarray      # uint64 [F+1]
+    body: np.ndarray          # uint64 [body_off[-1]]
+    p: np.ndarray             # float64 [F]
+    q: np.ndarray             # float64 [F]
+    head_clamped: np.ndarray  # uint8 [F]
+    mask0: np.ndarray         # float64 [F]
+    mask1: np.ndarray         # float64 [F]
+
+    def __post_init__(self):
+        self.type = np.ascontiguousarray(self.type, dtype=np.int32)
+        self.head = np.ascontiguousarray(self.head, dtype=np.uint64)
+        self.body_off = np.ascontiguousarray(self.body_off, dtype=np.uint64)
+        self.body = np.ascontiguousarray(self.body, dtype=np.uint64)
+        self.p = np.ascontiguousarray(self.p, dtype=np.float64)
+        self.q = np.ascontiguousarray(self.q, dtype=np.float64)
+        self.head_clamped = np.ascontiguousarray(self.head_clamped, dtype=np.uint8)
+        self.mask0 = np.ascontiguousarray(self.mask0, dtype=np.float64)
+        self.mask1 = np.ascontiguousarray(self.mask1, dtype=np.float64)
+        if self.body.size == 0:
+            self.body = np.zeros(1, dtype=np.uint64)[:0]
+
+    @property
+    def n_factors(self) -> int:
+        return int(self.type.shape[0])
+
+    @staticmethod
+    def build(factors: Sequence[tuple]) -> "FactorList":
+        """From (type, head, body, p[, q[, clamped, mask0, mask1]]) tuples."""
+        t, h, off, body, p, q, cl, m0, m1 = [], [], [0], [], [], [], [], [], []
+        for f in factors:
+            ty, hd, bd, pp = f[0], f[1], list(f[2]), f[3]
+            qq = f[4] if len(f) > 4 else 0.0
+            c, a, b = (f[5], f[6], f[7]) if len(f) > 5 else (0, 1.0, 1.0)
+            t.append(ty); h.append(hd); body.extend(bd); off.append(len(body))
+            p.append(pp); q.append(qq); cl.append(c); m0.append(a); m1.append(b)
+        return FactorList(np.array(t), np.array(h, dtype=np.uint64), np.array(off, dtype=np.uint64),
+                          np.array(body, dtype=np.uint64), np.array(p), np.array(q),
+                          np.array(cl, dtype=np.uint8), np.array(m0), np.array(m1))
+
+    def as_c(self) -> FactorListC:
+        return FactorListC(self.n_factors, _ptr(self.type, C.c_int32), _ptr(self.head, C.c_uint64),
+                           _ptr(self.body_off, C.c_uint64), _ptr(self.body, C.c_uint64),
+                           _ptr(self.p, C.c_double), _ptr(self.q, C.c_double),
+                           _ptr(self.head_clamped, C.c_uint8), _ptr(self.mask0, C.c_double),
+                           _ptr(self.mask1, C.c_double))
+
+    @staticmethod
+    def from_c(v: FactorListC) -> "FactorList":
+        F = int(v.n_factors)
+        off = _arr(v.body_off, F + 1, np.uint64)
+        nb = int(off[-1]) if F else 0
+        return FactorList(_arr(v.type, F, np.int32), _arr(v.head, F, np.uint64), off,
+                          _arr(v.body, nb, np.uint64), _arr(v.p, F, np.float64),
+                          _arr(v.q, F, np.float64), _arr(v.head_clamped, F, np.uint8),
+                          _arr(v.mask0, F, np.float64), _arr(v.mask1, F, np.float64))
+
+    def with_p(self, p: np.ndarray) -> "FactorList":
+        return FactorList(self.type, self.head, self.body_off, self.body, np.asarray(p, np.float64),
+                          self.q, self.head_clamped, self.mask0, self.mask1)
+
+
+def _take_flist(handle: C.c_void_p) -> FactorList:
+    v = FactorListC()
+    call("flbp_flist_view", handle, C.byref(v))
+    try:
+        return FactorList.from_c(v)
+    finally:
+        lib().flbp_flist_free(handle)
+
+
+def read_causal_fg(path: str) -> FactorList:
+    h = C.c_void_p()
+    call("flbp_read_causal_fg", path.encode(), C.byref(h))
+    return _take_flist(h)
+
+
+def write_causal_fg(fl: FactorList, path: str) -> None:
+    c = fl.as_c()
+    call("flbp_write_causal_fg", C.byref(c), path.encode())
+
+
+def clamp(fl: FactorList, labels: Sequence[int], values: Sequence[int]) -> FactorList:
+    """CausalFactorGraph::clamp per (label, value), in ascending label order."""
+    lab = np.ascontiguousarray(labels, dtype=np.uint64)
+    val = np.ascontiguousarray(values, dtype=np.uint8)
+    c = fl.as_c()
+    h = C.c_void_p()
+    call("flbp_clamp", C.byref(c), len(lab), _ptr(lab, C.c_uint64), _ptr(val, C.c_uint8), C.byref(h))
+    return _take_flist(h)
+
+
+@dataclass
+class Generated:
+    factors: FactorList
+    alarms: np.ndarray   # uint64 labels
+    rule: np.ndarray     # int32 per factor, -1 = untied
+
+
+def gen_params(**kw) -> GenParamsC:
+    prm = GenParamsC()
+    lib().flbp_gen_defaults(C.byref(prm))
+    for k, v in kw.items():
+        if not hasattr(prm, k):
+            raise TypeError(f"unknown generator parameter {k}")
+        setattr(prm, k, v)
+    return prm
+
+
+def generate(n_vars: int, seed: int, **kw) -> Generated:
+    prm = gen_params(n_vars=n_vars, seed=seed, **kw)
+    h = C.c_void_p()
+    call("flbp_generate", C.byref(prm), C.byref(h))
+    try:
+        v = FactorListC()
+        call("flbp_generated_view", h, C.byref(v))
+        fl = FactorList.from_c(v)
+        na = C.c_uint64()
+        ap = u64p()
+        rp = i32p()
+        call("flbp_generated_extra", h, C.byref(na), C.byref(ap), C.byref(rp))
+        alarms = _arr(ap, int(na.value), np.uint64)
+        rule = _arr(rp, fl.n_factors, np.int32)
+    finally:
+        lib().flbp_generated_free(h)
+    return Generated(fl, alarms, rule)
+
+
+class Graph:
+    """A factor graph in the reference's CSR orders (flbp_graph_build)."""
+
+    def __init__(self, fl: FactorList):
+        h = C.c_void_p()
+        c = fl.as_c()
+        call("flbp_graph_build", C.byref(c), C.byref(h))
+        try:
+            v = CsrC()
+            call("flbp_graph_csr", h, C.byref(v))
+            V, F, E = int(v.n_vars), int(v.n_factors), int(v.n_edges)
+            self.n_vars, self.n_factors, self.n_edges = V, F, E
+            self.var_label = _arr(v.var_label, V, np.uint64)
+            self.fac_off = _arr(v.fac_off, F + 1, np.uint64)
+            self.edge_var = _arr(v.edge_var, E, np.uint64)
+            self.var_off = _arr(v.var_off, V + 1, np.uint64)
+            self.var_edge = _arr(v.var_edge, E, np.uint64)
+            self.fac_type = _arr(v.fac_type, F, np.int32)
+            self.head_slot = _arr(v.head_slot, F, np.uint64)
+            self.head_clamped = _arr(v.head_clamped, F, np.uint8)
+            self.p = _arr(v.p, F, np.float64)
+            self.q = _arr(v.q, F, np.float64)
+            self.mask0 = _arr(v.mask0, F, np.float64)
+            self.mask1 = _arr(v.mask1, F, np.float64)
+        finally:
+            lib().flbp_graph_free(h)
+
+    def csr_c(self) -> CsrC:
+        return CsrC(self.n_vars, self.n_factors, self.n_edges,
+                    _ptr(self.var_label, C.c_uint64), _ptr(self.fac_off, C.c_uint64),
+                    _ptr(self.edge_var, C.c_uint64), _ptr(self.var_off, C.c_uint64),
+                    _ptr(self.var_edge, C.c_uint64), _ptr(self.fac_type, C.c_int32),
+                    _ptr(self.head_slot, C.c_uint64), _ptr(self.head_clamped, C.c_uint8),
+                    _ptr(self.p, C.c_double), _ptr(self.q, C.c_double),
+                    _ptr(self.mask0, C.c_double), _ptr(self.mask1, C.c_double))
+
+    def index_of(self, labels) -> np.ndarray:
+        labels = np.asarray(labels, dtype=np.uint64)
+        idx = np.searchsorted(self.var_label, labels)
+        if np.any(idx >= self.n_vars) or np.any(self.var_label[np.minimum(idx, self.n_vars - 1)] != labels):
+            raise KeyError("label not in graph")
+        return idx
+
+
+@dataclass
+class RunResult:
+    iters: int
+    sweeps: int
+    maxdiff: float
+    seconds: float
+
+
+class Plan:
+    """A graph resident on one GPU (flbp_plan_*)."""
+
+    def __init__(self, graph: Graph, device: int = 0):
+        self.graph = graph
+        self._h = C.c_void_p()
+        c = graph.csr_c()
+        call("flbp_plan_create", C.byref(c), device, C.byref(self._h))
+
+    def close(self):
+        if self._h:
+            lib().flbp_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, logdomain: bool = False) -> None:
+        call("flbp_plan_init", self._h, 1 if logdomain else 0)
+
+    def run(self, tol: float, maxiter: int, damping: float = 0.0, maxtime: float = math.inf,
+            chunk: int = 0) -> RunResult:
+        o = RunOptsC(tol, maxiter, maxtime, damping, chunk)
+        r = RunResultC()
+        call("flbp_plan_run", self._h, C.byref(o), C.byref(r))
+        return RunResult(int(r.iters), int(r.sweeps), float(r.maxdiff), float(r.seconds))
+
+    def run_fixed(self, n: int, damping: float = 0.0) -> RunResult:
+        r = RunResultC()
+        call("flbp_plan_run_fixed", self._h, n, damping, C.byref(r))
+        return RunResult(int(r.iters), int(r.sweeps), float(r.maxdiff), float(r.seconds))
+
+    def beliefs(self) -> np.ndarray:
+        out = np.empty(self.graph.n_vars, dtype=np.float64)
+        call("flbp_plan_beliefs", self._h, _ptr(out, C.c_double), out.shape[0])
+        return out
+
+    def messages(self):
+        E = self.graph.n_edges
+        fv = np.empty(2 * max(E, 1), dtype=np.float64)
+        vf = np.empty(2 * max(E, 1), dtype=np.float64)
+        call("flbp_plan_get_messages", self._h, _ptr(fv, C.c_double), _ptr(vf, C.c_double))
+        return fv[:2 * E].reshape(E, 2), vf[:2 * E].reshape(E, 2)
+
+    def set_messages(self, fv_varmajor: Optional[np.ndarray], vf_facmajor: Optional[np.ndarray]):
+        a = None if fv_varmajor is None else np.ascontiguousarray(fv_varmajor, dtype=np.float64).ravel()
+        b = None if vf_facmajor is None else np.ascontiguousarray(vf_facmajor, dtype=np.float64).ravel()
+        call("flbp_plan_set_messages", self._h, None if a is None else _ptr(a, C.c_double),
+             None if b is None else _ptr(b, C.c_double))
+
+    def set_p(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        call("flbp_plan_set_p", self._h, _ptr(p, C.c_double), p.shape[0])
+
+    def profile(self, enable: bool = True) -> None:
+        call("flbp_plan_profile", self._h, 1 if enable else 0)
+
+    def last_timing(self):
+        a, b = C.c_double(), C.c_double()
+        na, nb = C.c_uint64(), C.c_uint64()
+        call("flbp_plan_last_timing", self._h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb))
+        return float(a.value), float(b.value), int(na.value), int(nb.value)
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().flbp_plan_device_bytes(self._h))
+
+
+def rank_alarms(labels, marginals) -> np.ndarray:
+    """Alarm indices by marginal descending, ties by label ascending."""
+    labels = np.ascontiguousarray(labels, dtype=np.uint64)
+    marg = np.ascontiguousarray(marginals, dtype=np.float64)
+    out = np.empty(labels.shape[0], dtype=np.uint64)
+    call("flbp_rank_alarms", labels.shape[0], _ptr(labels, C.c_uint64), _ptr(marg, C.c_double),
+         _ptr(out, C.c_uint64))
+    return out.astype(np.int64)
+
+
+def algorithmic_bytes_per_sweep(n_vars: int, n_factors: int, n_edges: int, damping: float) -> int:
+    """HBM bytes one sweep must move at minimum: 68 E + 24 F + 20 V
+    (+16 E when damped).
+
+    Per edge: read v->f pair (16), write f->v pair (16), gather f->v pair
+    (16), scatter v->f pair (16), 4-byte slot map.  Per factor: offset (4),
+    meta word (4), p and q (16).  Per variable: offset (4), old belief read
+    and new belief write (16).  Damping reads the previous f->v pair (16).
+    """
+    b = 68 * n_edges + 24 * n_factors + 20 * n_vars
+    if damping != 0.0:
+        b += 16 * n_edges
+    return b

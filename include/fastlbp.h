@@ -1,0 +1,214 @@
+/*
+ * fastlbp.h -- C ABI of the MI355X loopy-BP engine (libfastlbp.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status:
+ * 0 on success, a negative FLBP_E* code on failure, with a message available
+ * from flbp_last_error() (per calling thread).  Nothing in the library calls
+ * exit().
+ *
+ * The engine replaces the reference's device-side PARALL sweep, which is
+ * reached through the C++ free functions
+ *     lbp::kernel::calcMessageFVFusedAll  include/kernel/causal_bp_parall.h:37-44
+ *     lbp::kernel::calcProdAll            include/kernel/causal_bp_parall.h:13-22
+ *     lbp::kernel::calcMessageVFAll       include/kernel/causal_bp_parall.h:24-35
+ *     lbp::kernel::getBeliefsV            include/kernel/causal_bp_seq.h:22-27
+ * and the device-state management of lbp::CausalBP (csrc/ext/causal_bp.cu)
+ * that drives them.  Paths are relative to the reference repository root.
+ * The C++ drop-in classes (include/ext/causal_bp.h and friends) sit on top
+ * of this ABI.
+ */
+#ifndef FASTLBP_H
+#define FASTLBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLBP_OK 0
+#define FLBP_EINVAL (-1)     /* bad argument / malformed graph */
+#define FLBP_ENOMEM (-2)     /* host or device allocation failed */
+#define FLBP_EHIP (-3)       /* a HIP runtime call failed */
+#define FLBP_ENODEV (-4)     /* no usable GPU */
+#define FLBP_ERANGE (-5)     /* size beyond what the engine supports */
+#define FLBP_EIO (-6)        /* file could not be read / parsed */
+
+/* ------------------------------------------------------------------ */
+/* Factor lists and graphs (host side, no GPU needed)                  */
+/* ------------------------------------------------------------------ */
+
+/* A list of causal factors in list order: the flat form of the reference's
+ * std::vector<CausalFactor> (include/ext/causal_factor.h:14-76).
+ * type: 0 Singleton, 1 DefiniteAnd, 2 DefiniteOr (CausalFactor::type()).
+ * p = prob(), q = prob_default(); head_clamped/mask0/mask1 = the public
+ * head_clamped/head_mask members. */
+typedef struct flbp_factor_list {
+    uint64_t n_factors;
+    const int32_t *type;
+    const uint64_t *head;
+    const uint64_t *body_off;     /* [F+1] */
+    const uint64_t *body;         /* [body_off[F]] labels */
+    const double *p;
+    const double *q;
+    const uint8_t *head_clamped;
+    const double *mask0;
+    const double *mask1;
+} flbp_factor_list;
+
+/* The CSR form of a factor graph in the reference's orders, i.e. what
+ * CausalBP::construct (csrc/ext/causal_bp.cu:310-407) and
+ * putParallUpdateSeqToKernelFused (724-817) derive on the host:
+ *   var i        = rank of its label (causal_fg.cpp:50-68)
+ *   fac_off      = h_row_ptr_vf; slots of factor I in nbF(I) order, i.e.
+ *                  ascending label
+ *   edge_var[s]  = var index of factor-major slot s
+ *   var_off      = h_row_ptr_fv; edges of var i in nbV(i) order, i.e.
+ *                  ascending factor index
+ *   var_edge[k]  = factor-major slot of var-major edge k (Index6Real4.src_ind)
+ *   head_slot[I] = factor-major slot of I's head (h_head) */
+typedef struct flbp_csr {
+    uint64_t n_vars, n_factors, n_edges;
+    const uint64_t *var_label;    /* [V] */
+    const uint64_t *fac_off;      /* [F+1] */
+    const uint64_t *edge_var;     /* [E] */
+    const uint64_t *var_off;      /* [V+1] */
+    const uint64_t *var_edge;     /* [E] */
+    const int32_t *fac_type;      /* [F] */
+    const uint64_t *head_slot;    /* [F] */
+    const uint8_t *head_clamped;  /* [F] */
+    const double *p, *q, *mask0, *mask1;  /* [F] */
+} flbp_csr;
+
+typedef struct flbp_graph flbp_graph;   /* owns a flbp_csr */
+typedef struct flbp_flist flbp_flist;   /* owns a factor list */
+
+/* CausalFactorGraph(const std::vector<CausalFactor>&) (csrc/ext/causal_fg.cpp:50-68)
+ * followed by the CSR derivation of CausalBP::construct. */
+int flbp_graph_build(const flbp_factor_list *fl, flbp_graph **out);
+int flbp_graph_csr(const flbp_graph *g, flbp_csr *view);
+void flbp_graph_free(flbp_graph *g);
+
+/* The .causal_fg reader (csrc/ext/causal_fg.cpp:118-279), implicit
+ * Singleton(0.5) factors included. */
+int flbp_read_causal_fg(const char *path, flbp_flist **out);
+/* Writes a factor list in the .causal_fg text format the reader accepts. */
+int flbp_write_causal_fg(const flbp_factor_list *fl, const char *path);
+
+/* CausalFactorGraph::clamp (causal_fg.cpp:17-26) for each (label, value),
+ * applied in ascending label order as examples/fl_wrapper.cu:26-45 does,
+ * returning the factor list to rebuild from (fl_wrapper.cu:45). */
+int flbp_clamp(const flbp_factor_list *fl, uint64_t n_clamps, const uint64_t *labels,
+               const uint8_t *values, flbp_flist **out);
+int flbp_flist_view(const flbp_flist *l, flbp_factor_list *view);
+void flbp_flist_free(flbp_flist *l);
+
+/* Seeded synthetic causal graphs (not in the reference): a layered
+ * derivation graph.  Layer 0 holds leaves with Singleton priors; every
+ * head of layers 1..n_layers-1 is a deterministic OR over k clause
+ * variables, each clause an AND factor over b body variables from earlier
+ * layers, plus with probability back_frac one body variable from a later
+ * layer (a loop). */
+typedef struct flbp_gen_params {
+    uint64_t n_vars;
+    uint64_t seed;
+    uint32_t n_layers;           /* >= 2; layer 0 = leaves */
+    uint32_t k_min, k_max;       /* clauses per head */
+    uint32_t b_min, b_max;       /* body variables per clause */
+    uint32_t n_rules;            /* 0: no tying; else rule ids 0..n_rules-1 */
+    double leaf_frac;
+    double p_and_lo, p_and_hi;
+    double p_leaf_lo, p_leaf_hi;
+    double back_frac;
+    uint64_t n_alarms;
+} flbp_gen_params;
+
+typedef struct flbp_generated flbp_generated;
+
+void flbp_gen_defaults(flbp_gen_params *p);
+int flbp_generate(const flbp_gen_params *params, flbp_generated **out);
+int flbp_generated_view(const flbp_generated *g, flbp_factor_list *view);
+/* alarm labels [n_alarms] and rule id per factor [F] (-1: none) */
+int flbp_generated_extra(const flbp_generated *g, uint64_t *n_alarms,
+                         const uint64_t **alarms, const int32_t **rule_of_factor);
+void flbp_generated_free(flbp_generated *g);
+
+/* Alarm ranking (not in the reference): indices of alarms sorted by
+ * marginal descending, ties (bitwise-equal marginals) by label ascending. */
+int flbp_rank_alarms(uint64_t n_alarms, const uint64_t *labels, const double *marginals,
+                     uint64_t *order_out);
+
+/* ------------------------------------------------------------------ */
+/* Plain solve on the GPU                                              */
+/* ------------------------------------------------------------------ */
+
+typedef struct flbp_plan flbp_plan;
+
+/* CausalBP::construct's device half (csrc/ext/causal_bp.cu:310-407, the H->D
+ * copies at 368-369 and 815-816): copies the CSR into device memory owned by
+ * the plan, in the engine's layout.  device = HIP device ordinal. */
+int flbp_plan_create(const flbp_csr *csr, int device, flbp_plan **out);
+void flbp_plan_destroy(flbp_plan *plan);
+
+/* CausalBP::init (causal_bp.cu:824-856): every message (1,1) -- (0,0) when
+ * logdomain is set -- and zero counts 0.  The belief array and the
+ * iteration counter are left alone, as in the reference. */
+int flbp_plan_init(flbp_plan *plan, int logdomain);
+
+typedef struct flbp_run_opts {
+    double tol;              /* stop when max |b_new - b_old| <= tol */
+    uint64_t maxiter;        /* iteration cap (counted like CausalBP::_iters) */
+    double maxtime;          /* seconds; checked between chunks of sweeps */
+    double damping;          /* 0 = the reference */
+    uint32_t chunk;          /* sweeps enqueued per host check; 0 = default */
+} flbp_run_opts;
+
+typedef struct flbp_run_result {
+    uint64_t iters;          /* the plan's iteration counter after the call */
+    uint64_t sweeps;         /* sweeps done by this call */
+    double maxdiff;          /* last max belief change (INFINITY if none) */
+    double seconds;          /* wall time of the call */
+} flbp_run_result;
+
+/* CausalBP::run() (causal_bp.cu:858-910): the three passes of
+ * calcNewMessageFusedAll (409-444) + calcBeliefsV (483-515) + the max-diff
+ * reduce (879-886) per iteration, fused into two kernels per sweep with a
+ * device-side stop test. */
+int flbp_plan_run(flbp_plan *plan, const flbp_run_opts *opts, flbp_run_result *res);
+
+/* CausalBP::run(tol, maxIters, histLength) for PARALL (causal_bp.cu:912-1004):
+ * exactly n_sweeps sweeps, no convergence test. */
+int flbp_plan_run_fixed(flbp_plan *plan, uint64_t n_sweeps, double damping,
+                        flbp_run_result *res);
+
+/* transferBeliefToHost (causal_bp.cu:552-554): P(x_i = 1) for i < n_vars. */
+int flbp_plan_beliefs(flbp_plan *plan, double *out, uint64_t n);
+
+/* transferMessagesToHost (causal_bp.cu:1145-1161) and its inverse: f->v
+ * messages in var-major (nbV) order, v->f messages in factor-major order,
+ * each as interleaved (state0, state1) pairs [2*E].  NULL skips one. */
+int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facmajor);
+int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor,
+                           const double *vf_facmajor);
+
+/* Replaces p of factor I by p[I] for every I (used by the adjoint's
+ * training loop; the reference rebuilds the whole CausalBP instead). */
+int flbp_plan_set_p(flbp_plan *plan, const double *p, uint64_t n);
+
+/* Kernel timing of the last run call: sum of device time of the factor and
+ * variable passes in milliseconds, and launches of each (0 if not
+ * recorded).  Enabled with flbp_plan_profile(plan, 1). */
+int flbp_plan_profile(flbp_plan *plan, int enable);
+int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
+                          uint64_t *fac_launches, uint64_t *var_launches);
+
+uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
+
+const char *flbp_last_error(void);
+const char *flbp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

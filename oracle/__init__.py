@@ -1,0 +1,265 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The CPU FP64 restatement of the reference's PARALL path (see oracle.cpp for
+the per-function citations).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module; the engine never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DUMP = os.path.join(HERE, "_ref", "ref_dump")
+
+u8p = C.POINTER(C.c_uint8)
+i32p = C.POINTER(C.c_int32)
+u64p = C.POINTER(C.c_uint64)
+f64p = C.POINTER(C.c_double)
+
+
+class OrcFactorList(C.Structure):
+    _fields_ = [("n_factors", C.c_uint64), ("type", i32p), ("head", u64p), ("body_off", u64p),
+                ("body", u64p), ("p", f64p), ("q", f64p), ("head_clamped", u8p),
+                ("mask0", f64p), ("mask1", f64p)]
+
+
+_SIG = {
+    "orc_build": (C.c_int, [C.POINTER(OrcFactorList), C.POINTER(C.c_void_p)]),
+    "orc_free": (None, [C.c_void_p]),
+    "orc_nvars": (C.c_uint64, [C.c_void_p]),
+    "orc_nfactors": (C.c_uint64, [C.c_void_p]),
+    "orc_nedges": (C.c_uint64, [C.c_void_p]),
+    "orc_dump": (C.c_int, [C.c_void_p, u64p, u64p, u64p, u64p, u64p, u64p, i32p]),
+    "orc_init": (C.c_int, [C.c_void_p, C.c_int]),
+    "orc_run": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.c_double, C.c_int, u64p, f64p]),
+    "orc_run_fixed": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.c_int]),
+    "orc_sweep": (C.c_int, [C.c_void_p, C.c_double, C.c_int, f64p]),
+    "orc_beliefs": (C.c_int, [C.c_void_p, f64p]),
+    "orc_iters": (C.c_uint64, [C.c_void_p]),
+    "orc_messages": (C.c_int, [C.c_void_p, f64p, f64p, f64p, f64p]),
+    "orc_set_messages": (C.c_int, [C.c_void_p, f64p, f64p, f64p, f64p]),
+    "orc_fv_message": (None, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double,
+                              C.c_double, C.c_double, f64p, f64p, f64p, f64p]),
+    "orc_vf_message": (None, [C.c_uint64, C.c_uint64, f64p, f64p, f64p, f64p]),
+    "orc_clamp": (C.c_int, [C.POINTER(OrcFactorList), C.c_uint64, u64p, u8p, C.POINTER(C.c_void_p)]),
+    "orc_flist_view": (None, [C.c_void_p, C.POINTER(OrcFactorList)]),
+    "orc_flist_free": (None, [C.c_void_p]),
+    "orc_grad": (C.c_int, [C.POINTER(OrcFactorList), i32p, C.c_uint64, f64p, C.c_uint64, u64p, f64p,
+                           C.c_uint64, C.c_double, f64p, f64p]),
+    "orc_loss": (C.c_int, [C.POINTER(OrcFactorList), i32p, C.c_uint64, f64p, C.c_uint64, u64p, f64p,
+                           C.c_uint64, C.c_double, f64p]),
+    "orc_last_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
+        l = C.CDLL(LIB_PATH)
+        for n, (r, a) in _SIG.items():
+            f = getattr(l, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = l
+    return _lib
+
+
+def _chk(name, rc):
+    if rc != 0:
+        raise RuntimeError(f"{name}: {lib().orc_last_error().decode()}")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class _ListHold:
+    """Keeps the numpy arrays of a factor list alive while C reads them."""
+
+    def __init__(self, fl):
+        self.a = [np.ascontiguousarray(fl.type, np.int32), np.ascontiguousarray(fl.head, np.uint64),
+                  np.ascontiguousarray(fl.body_off, np.uint64), np.ascontiguousarray(fl.body, np.uint64),
+                  np.ascontiguousarray(fl.p, np.float64), np.ascontiguousarray(fl.q, np.float64),
+                  np.ascontiguousarray(fl.head_clamped, np.uint8), np.ascontiguousarray(fl.mask0, np.float64),
+                  np.ascontiguousarray(fl.mask1, np.float64)]
+        if self.a[3].size == 0:
+            self.a[3] = np.zeros(1, np.uint64)
+        t, h, o, b, p, q, c, m0, m1 = self.a
+        self.c = OrcFactorList(t.shape[0], _p(t, C.c_int32), _p(h, C.c_uint64), _p(o, C.c_uint64),
+                               _p(b, C.c_uint64), _p(p, C.c_double), _p(q, C.c_double),
+                               _p(c, C.c_uint8), _p(m0, C.c_double), _p(m1, C.c_double))
+
+
+class OracleGraph:
+    def __init__(self, fl):
+        self._hold = _ListHold(fl)
+        self._h = C.c_void_p()
+        _chk("orc_build", lib().orc_build(C.byref(self._hold.c), C.byref(self._h)))
+        L = lib()
+        self.n_vars = int(L.orc_nvars(self._h))
+        self.n_factors = int(L.orc_nfactors(self._h))
+        self.n_edges = int(L.orc_nedges(self._h))
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().orc_free(self._h)
+        except Exception:
+            pass
+
+    def dump(self):
+        V, F, E = self.n_vars, self.n_factors, self.n_edges
+        d = dict(var_label=np.zeros(V, np.uint64), fac_off=np.zeros(F + 1, np.uint64),
+                 edge_var=np.zeros(E, np.uint64), var_off=np.zeros(V + 1, np.uint64),
+                 var_src=np.zeros(E, np.uint64), head_slot=np.zeros(F, np.uint64),
+                 code=np.zeros(E, np.int32))
+        _chk("orc_dump", lib().orc_dump(
+            self._h, _p(d["var_label"], C.c_uint64), _p(d["fac_off"], C.c_uint64),
+            _p(d["edge_var"], C.c_uint64), _p(d["var_off"], C.c_uint64), _p(d["var_src"], C.c_uint64),
+            _p(d["head_slot"], C.c_uint64), _p(d["code"], C.c_int32)))
+        return d
+
+    def init(self, logdomain=False):
+        _chk("orc_init", lib().orc_init(self._h, 1 if logdomain else 0))
+
+    def run(self, tol, maxiter, damping=0.0, threads=1):
+        it = C.c_uint64()
+        md = C.c_double()
+        _chk("orc_run", lib().orc_run(self._h, tol, maxiter, damping, threads, C.byref(it), C.byref(md)))
+        return int(it.value), float(md.value)
+
+    def run_fixed(self, n, damping=0.0, threads=1):
+        _chk("orc_run_fixed", lib().orc_run_fixed(self._h, n, damping, threads))
+
+    def sweep(self, damping=0.0, threads=1):
+        md = C.c_double()
+        _chk("orc_sweep", lib().orc_sweep(self._h, damping, threads, C.byref(md)))
+        return float(md.value)
+
+    def beliefs(self):
+        out = np.zeros(self.n_vars, np.float64)
+        _chk("orc_beliefs", lib().orc_beliefs(self._h, _p(out, C.c_double)))
+        return out
+
+    @property
+    def iters(self):
+        return int(lib().orc_iters(self._h))
+
+    def messages(self):
+        E = self.n_edges
+        a = [np.zeros(max(E, 1), np.float64) for _ in range(4)]
+        _chk("orc_messages", lib().orc_messages(self._h, *[_p(x, C.c_double) for x in a]))
+        fv = np.stack([a[0][:E], a[1][:E]], axis=1)
+        vf = np.stack([a[2][:E], a[3][:E]], axis=1)
+        return fv, vf
+
+    def set_messages(self, fv, vf):
+        a = [np.ascontiguousarray(fv[:, 0]), np.ascontiguousarray(fv[:, 1]),
+             np.ascontiguousarray(vf[:, 0]), np.ascontiguousarray(vf[:, 1])]
+        _chk("orc_set_messages", lib().orc_set_messages(self._h, *[_p(x, C.c_double) for x in a]))
+
+
+def fv_message(code, m0, m1, src, head, p, q=0.0, mask0=1.0, mask1=1.0):
+    m0 = np.ascontiguousarray(m0, np.float64)
+    m1 = np.ascontiguousarray(m1, np.float64)
+    o0, o1 = C.c_double(), C.c_double()
+    lib().orc_fv_message(code, m0.shape[0], src, head, p, q, mask0, mask1, _p(m0, C.c_double),
+                         _p(m1, C.c_double), C.byref(o0), C.byref(o1))
+    return o0.value, o1.value
+
+
+def vf_message(m0, m1, src):
+    m0 = np.ascontiguousarray(m0, np.float64)
+    m1 = np.ascontiguousarray(m1, np.float64)
+    o0, o1 = C.c_double(), C.c_double()
+    lib().orc_vf_message(m0.shape[0], src, _p(m0, C.c_double), _p(m1, C.c_double), C.byref(o0),
+                         C.byref(o1))
+    return o0.value, o1.value
+
+
+def clamp(fl, labels, values, factory):
+    """Returns factory(type, head, body_off, body, p, q, head_clamped, mask0, mask1)."""
+    hold = _ListHold(fl)
+    lab = np.ascontiguousarray(labels, np.uint64)
+    val = np.ascontiguousarray(values, np.uint8)
+    h = C.c_void_p()
+    _chk("orc_clamp", lib().orc_clamp(C.byref(hold.c), lab.shape[0], _p(lab, C.c_uint64),
+                                       _p(val, C.c_uint8), C.byref(h)))
+    v = OrcFactorList()
+    lib().orc_flist_view(h, C.byref(v))
+    F = int(v.n_factors)
+
+    def arr(ptr, n, dt):
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True) if n else np.zeros(0, dt)
+
+    off = arr(v.body_off, F + 1, np.uint64)
+    out = factory(arr(v.type, F, np.int32), arr(v.head, F, np.uint64), off,
+                  arr(v.body, int(off[-1]), np.uint64), arr(v.p, F, np.float64), arr(v.q, F, np.float64),
+                  arr(v.head_clamped, F, np.uint8), arr(v.mask0, F, np.float64),
+                  arr(v.mask1, F, np.float64))
+    lib().orc_flist_free(h)
+    return out
+
+
+def loss(fl, rule, n_rules, theta, alarm_labels, targets, T, damping):
+    hold = _ListHold(fl)
+    rule = np.ascontiguousarray(rule, np.int32)
+    theta = np.ascontiguousarray(theta, np.float64)
+    al = np.ascontiguousarray(alarm_labels, np.uint64)
+    tg = np.ascontiguousarray(targets, np.float64)
+    out = C.c_double()
+    _chk("orc_loss", lib().orc_loss(C.byref(hold.c), _p(rule, C.c_int32), n_rules, _p(theta, C.c_double),
+                                     al.shape[0], _p(al, C.c_uint64), _p(tg, C.c_double), T, damping,
+                                     C.byref(out)))
+    return out.value
+
+
+def grad(fl, rule, n_rules, theta, alarm_labels, targets, T, damping):
+    hold = _ListHold(fl)
+    rule = np.ascontiguousarray(rule, np.int32)
+    theta = np.ascontiguousarray(theta, np.float64)
+    al = np.ascontiguousarray(alarm_labels, np.uint64)
+    tg = np.ascontiguousarray(targets, np.float64)
+    out = C.c_double()
+    g = np.zeros(n_rules, np.float64)
+    _chk("orc_grad", lib().orc_grad(C.byref(hold.c), _p(rule, C.c_int32), n_rules, _p(theta, C.c_double),
+                                     al.shape[0], _p(al, C.c_uint64), _p(tg, C.c_double), T, damping,
+                                     C.byref(out), _p(g, C.c_double)))
+    return out.value, g
+
+
+def ref_dump(path, observations=None):
+    """Runs oracle/_ref/ref_dump (the reference's own graph model) and parses
+    its output.  Returns None when it has not been built."""
+    if not os.path.exists(REF_DUMP):
+        return None
+    args = [REF_DUMP, path] + ([observations] if observations else [])
+    txt = subprocess.run(args, check=True, capture_output=True, text=True).stdout
+    return parse_dump(txt)
+
+
+def parse_dump(txt):
+    lines = txt.strip("\n").split("\n")
+    V, F, E = map(int, lines[0].split())
+    labels = [int(x) for x in lines[1:1 + V]]
+    facs = []
+    for ln in lines[1 + V:1 + V + F]:
+        t = ln.split()
+        nb = [tuple(map(int, x.split(":"))) for x in t[8:]]
+        facs.append(dict(type=int(t[0]), p=float(t[1]), q=float(t[2]), clamped=int(t[3]),
+                         mask0=float(t[4]), mask1=float(t[5]), head=int(t[6]), nbF=nb))
+    nbv = []
+    for ln in lines[1 + V + F:1 + V + F + V]:
+        t = ln.split()
+        nbv.append([tuple(map(int, x.split(":"))) for x in t[1:]])
+    return dict(V=V, F=F, E=E, labels=labels, factors=facs, nbV=nbv)

@@ -1,0 +1,736 @@
+/*
+ * oracle.cpp -- CPU FP64 restatement of the reference's PARALL CausalBP path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ *
+ * It follows the reference line by line where the floating-point operation
+ * order matters (citations are paths relative to the reference repository):
+ *   - graph ordering      csrc/ext/causal_fg.cpp:28-68, csrc/lbp/bipgraph.cpp:24-42
+ *   - CSR + descriptors   csrc/ext/causal_bp.cu:310-407, 556-581, 724-817
+ *   - f->v messages       csrc/kernel/causal_bp_parall.cu:14-18, 94-398
+ *   - products            csrc/kernel/causal_bp_parall.cu:20-53
+ *   - v->f messages       csrc/kernel/causal_bp_parall.cu:55-92
+ *   - beliefs             csrc/kernel/causal_bp_seq.cu:102-111, csrc/ext/causal_bp.cu:118-128, 483-515
+ *   - run loops           csrc/ext/causal_bp.cu:824-1004
+ *   - clamp + rebuild     csrc/ext/causal_fg.cpp:17-26, csrc/ext/causal_factor.cpp:63-100,
+ *                         examples/fl_wrapper.cu:26-45
+ * Damping is not in the reference; it is defined here as
+ *   m_fv <- (1-d)*m_new + d*m_fv_old   (both states, after normalization)
+ * and is skipped entirely when d == 0, so d == 0 is the reference.
+ *
+ * Parity pins (tests/): every f->v code against brute-force sum-product
+ * enumeration; products/v->f against brute-force leave-one-out; whole-graph
+ * BP on trees against exact joint enumeration; graph ordering and clamping
+ * against the reference's own graph model compiled from its sources
+ * (oracle/_ref).  The adjoint (orc_grad) is pinned by central finite
+ * differences of orc_loss.
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &msg) {
+    g_err = msg;
+    return -1;
+}
+
+/* csrc/kernel/causal_bp_parall.cu:14-18 */
+inline void scale(double &x, double &y) {
+    double m = std::fmax(x, y);
+    x /= m;
+    y /= m;
+}
+
+struct VarMajorDesc {       /* Index6Real4 of causal_bp.cu:724-805 */
+    int code;
+    uint64_t src, start, end, head;
+    double q, p, mask0, mask1;
+};
+
+struct FacMajorDesc {       /* Index3 of causal_bp.cu:806-813 */
+    uint64_t ind, v, src;
+};
+
+}  // namespace
+
+struct orc_graph {
+    uint64_t V = 0, F = 0, E = 0;
+    std::vector<uint64_t> var_label;
+    std::vector<uint64_t> row_ptr_vf;    /* F+1 */
+    std::vector<uint64_t> edge_var;      /* E, factor-major */
+    std::vector<uint64_t> row_ptr_fv;    /* V+1 */
+    std::vector<uint64_t> head_slot;     /* F */
+    std::vector<VarMajorDesc> fv_desc;   /* E, var-major */
+    std::vector<FacMajorDesc> vf_desc;   /* E, factor-major */
+    /* state */
+    std::vector<double> m_fv0, m_fv1, m_vf0, m_vf1;
+    std::vector<double> prod0, prod1;
+    std::vector<uint64_t> nz0, nz1;
+    std::vector<double> old_beliefs;     /* value-initialised to 0.0, causal_bp.cu:371 */
+    uint64_t iters = 0;
+};
+
+struct orc_flist {
+    std::vector<int32_t> type;
+    std::vector<uint64_t> head, body_off, body;
+    std::vector<double> p, q, mask0, mask1;
+    std::vector<uint8_t> head_clamped;
+};
+
+namespace {
+
+/* vars() of a factor: VarSet(body) + head, sorted and unique
+ * (csrc/ext/causal_factor.cpp:12-29, csrc/lbp/smallset.cpp:14-21). */
+void factor_vars(const orc_factor_list *fl, uint64_t I, std::vector<uint64_t> &out) {
+    out.clear();
+    for (uint64_t k = fl->body_off[I]; k < fl->body_off[I + 1]; k++) out.push_back(fl->body[k]);
+    out.push_back(fl->head[I]);
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+}
+
+/* f->v message for one var-major edge: the body of
+ * calcMessageFVFusedAllKernel, csrc/kernel/causal_bp_parall.cu:102-397.
+ * vf0/vf1 point at the factor-major message arrays. */
+void fv_edge(const VarMajorDesc &d, const double *vf0, const double *vf1,
+             double &out0, double &out1) {
+    const uint64_t src_ind = d.src, h = d.head;
+    const uint64_t start_ind = d.start, end_ind = d.end;
+    const double p0 = d.q, p1 = d.p, mask0 = d.mask0, mask1 = d.mask1;
+    const int type = d.code;
+    double marg0 = 0, marg1 = 0;
+    if (type == 0) {                                    /* 110-112 */
+        marg0 = 1 - p1;
+        marg1 = p1;
+    } else if (type == 1 || type == 3) {                /* 113-151, 191-239 */
+        double prod_1 = 1.0, prod_10 = 1.0;
+        if (end_ind - start_ind == 1) {
+            marg0 = 1 - p1;
+            marg1 = p1;
+        } else if (end_ind - start_ind == 2) {
+            uint64_t j = start_ind;
+            if (start_ind == src_ind) j++;
+            double m_0 = vf0[j], m_1 = vf1[j];
+            marg0 = (1 - p1) * m_1 + (1 - p0) * m_0;
+            marg1 = p1 * m_1 + p0 * m_0;
+        } else {
+            double e1 = 0;
+            for (uint64_t j = start_ind; j < end_ind; j++) {
+                if (j != src_ind) {
+                    double m_0 = vf0[j], m_1 = vf1[j];
+                    double a0 = m_0 + m_1, a1 = m_1;
+                    prod_1 *= a1;
+                    prod_10 *= a0;
+                    scale(prod_1, prod_10);
+                    double delta = m_0;
+                    if (a1 != 0 && a0 == a1 && delta != 0) e1 += delta / a1;
+                }
+            }
+            marg0 = (1 - p1) * prod_1 + (1 - p0) * (e1 * prod_10 + (prod_10 - prod_1));
+            marg1 = p1 * prod_1 + p0 * (e1 * prod_10 + (prod_10 - prod_1));
+        }
+        if (type == 3) {
+            marg0 *= mask0;
+            marg1 *= mask1;
+        }
+        double marg = marg0 + marg1;
+        if (marg != 0) {
+            marg0 = marg0 / marg;
+            marg1 = marg1 / marg;
+        } else if (type == 3 && mask0 == 0) {
+            marg0 = 0.0; marg1 = 1.0;
+        } else if (type == 3 && mask1 == 0) {
+            marg0 = 1.0; marg1 = 0.0;
+        } else {
+            marg0 = 0.5; marg1 = 0.5;
+        }
+    } else if (type == 2 || type == 4) {                /* 152-190, 240-288 */
+        double prod_0 = 1.0, prod_10 = 1.0;
+        if (end_ind - start_ind == 1) {
+            marg0 = p1;
+            marg1 = 1 - p1;
+        } else if (end_ind - start_ind == 2) {
+            uint64_t j = start_ind;
+            if (start_ind == src_ind) j++;
+            double m_0 = vf0[j], m_1 = vf1[j];
+            marg0 = p1 * m_0 + p0 * m_1;
+            marg1 = (1 - p1) * m_0 + (1 - p0) * m_1;
+        } else {
+            double e1 = 0;
+            for (uint64_t j = start_ind; j < end_ind; j++) {
+                if (j != src_ind) {
+                    double m_0 = vf0[j], m_1 = vf1[j];
+                    double a0 = m_0 + m_1, a1 = m_0;
+                    prod_0 *= a1;
+                    prod_10 *= a0;
+                    scale(prod_0, prod_10);
+                    double delta = m_1;
+                    if (a1 != 0 && a0 == a1 && delta != 0) e1 += delta / a1;
+                }
+            }
+            marg0 = p1 * prod_0 + p0 * (e1 * prod_10 + (prod_10 - prod_0));
+            marg1 = (1 - p1) * prod_0 + (1 - p0) * (e1 * prod_10 + (prod_10 - prod_0));
+        }
+        if (type == 4) {
+            marg0 *= mask0;
+            marg1 *= mask1;
+        }
+        double marg = marg0 + marg1;
+        if (marg != 0) {
+            marg0 = marg0 / marg;
+            marg1 = marg1 / marg;
+        } else if (type == 4 && mask0 == 0) {
+            marg0 = 0.0; marg1 = 1.0;
+        } else if (type == 4 && mask1 == 0) {
+            marg0 = 1.0; marg1 = 0.0;
+        } else {
+            marg0 = 0.5; marg1 = 0.5;
+        }
+    } else if (type == 5 || type == 7) {                /* 289-314, 341-366 */
+        double m0h = vf0[h], m1h = vf1[h];
+        double prod_1, prod_10;
+        if (type == 5) {
+            prod_1 = (p1 - p0) * (m1h - m0h);
+            prod_10 = p0 * m1h + (1 - p0) * m0h;
+        } else {
+            prod_1 = (p1 - p0) * (m1h * mask1 - m0h * mask0);
+            prod_10 = p0 * m1h * mask1 + (1 - p0) * m0h * mask0;
+        }
+        scale(prod_1, prod_10);
+        if (prod_10 != 0) {
+            for (uint64_t j = start_ind; j < end_ind; j++) {
+                if ((j != src_ind) && (j != h)) {
+                    double m0 = vf0[j], m1 = vf1[j], m = m0 + m1;
+                    prod_10 *= m;
+                    prod_1 *= m1;
+                    scale(prod_1, prod_10);
+                }
+            }
+        }
+        marg0 = prod_10;
+        marg1 = prod_10 + prod_1;
+        double marg = marg0 + marg1;
+        if (marg != 0) {
+            marg0 = marg0 / marg;
+            marg1 = marg1 / marg;
+        } else {
+            marg0 = 0.5; marg1 = 0.5;
+        }
+    } else if (type == 6 || type == 8) {                /* 315-340, 367-393 */
+        double m0h = vf0[h], m1h = vf1[h];
+        double prod_0, prod_10;
+        if (type == 6) {
+            prod_0 = (p1 - p0) * (m0h - m1h);
+            prod_10 = p0 * m0h + (1 - p0) * m1h;
+        } else {
+            prod_0 = (p1 - p0) * (m0h * mask0 - m1h * mask1);
+            prod_10 = p0 * m0h * mask0 + (1 - p0) * m1h * mask1;
+        }
+        scale(prod_0, prod_10);
+        if (prod_10 != 0) {
+            for (uint64_t j = start_ind; j < end_ind; j++) {
+                if ((j != src_ind) && (j != h)) {
+                    double m0 = vf0[j], m1 = vf1[j], m = m0 + m1;
+                    prod_10 *= m;
+                    prod_0 *= m0;
+                    scale(prod_0, prod_10);
+                }
+            }
+        }
+        marg0 = prod_0 + prod_10;
+        marg1 = prod_10;
+        double marg = marg0 + marg1;
+        if (marg != 0) {
+            marg0 = marg0 / marg;
+            marg1 = marg1 / marg;
+        } else {
+            marg0 = 0.5; marg1 = 0.5;
+        }
+    }
+    out0 = marg0;
+    out1 = marg1;
+}
+
+/* calcProdAllKernel body, csrc/kernel/causal_bp_parall.cu:30-52 */
+void prod_var(const double *fv0, const double *fv1, uint64_t start_ind, uint64_t end_ind,
+              double &p0_out, double &p1_out, uint64_t &z0_out, uint64_t &z1_out) {
+    double prod_0 = 1.0, prod_1 = 1.0;
+    uint64_t zero_0 = 0, zero_1 = 0;
+    for (uint64_t j = start_ind; j < end_ind; j++) {
+        double m0 = fv0[j], m1 = fv1[j];
+        if (m0 == 0) zero_0++; else prod_0 *= m0;
+        if (m1 == 0) zero_1++; else prod_1 *= m1;
+        scale(prod_0, prod_1);
+    }
+    p0_out = prod_0;
+    p1_out = prod_1;
+    z0_out = zero_0;
+    z1_out = zero_1;
+}
+
+/* calcMessageVFAllKernel body, csrc/kernel/causal_bp_parall.cu:68-90 */
+void vf_edge(double prod_0, double prod_1, uint64_t zero_cnt_0, uint64_t zero_cnt_1,
+             double m_src_0, double m_src_1, double &out0, double &out1) {
+    if ((zero_cnt_0 >= 2) || (zero_cnt_0 == 1 && m_src_0 != 0)) {
+        prod_0 = 0.0;
+    } else if (m_src_0 != 0) {
+        prod_0 = prod_0 / m_src_0;
+    }
+    if ((zero_cnt_1 >= 2) || (zero_cnt_1 == 1 && m_src_1 != 0)) {
+        prod_1 = 0.0;
+    } else if (m_src_1 != 0) {
+        prod_1 = prod_1 / m_src_1;
+    }
+    if (prod_0 == 0 && prod_1 == 0) prod_0 = prod_1 = 0.5;
+    double prod = prod_0 + prod_1;
+    out0 = prod_0 / prod;
+    out1 = prod_1 / prod;
+}
+
+/* getBeliefsVKernel (causal_bp_seq.cu:102-111) on both states, then
+ * norm_functor (causal_bp.cu:124-128). */
+inline double belief_of(double prod_0, double prod_1, uint64_t nz0, uint64_t nz1) {
+    double b0 = prod_0 * (double)(!nz0);
+    double b1 = prod_1 * (double)(!nz1);
+    return b1 / (b0 + b1);
+}
+
+/* max with NaN propagation: a NaN difference stops the loop (NaN > tol is false). */
+inline double nanmax(double a, double b) {
+    if (std::isnan(a) || std::isnan(b)) return std::numeric_limits<double>::quiet_NaN();
+    return a < b ? b : a;
+}
+
+void set_threads(int threads) {
+#ifdef _OPENMP
+    omp_set_num_threads(threads > 0 ? threads : 1);
+#else
+    (void)threads;
+#endif
+}
+
+/* One PARALL sweep (calcNewMessageFusedAll, causal_bp.cu:409-444) plus the
+ * belief/max-diff step of run() (causal_bp.cu:874-889). */
+double sweep(orc_graph *g, double damping) {
+    const int64_t E = (int64_t)g->E, V = (int64_t)g->V;
+    /* f -> v over var-major edges */
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < E; e++) {
+        double o0, o1;
+        fv_edge(g->fv_desc[e], g->m_vf0.data(), g->m_vf1.data(), o0, o1);
+        if (damping != 0.0) {
+            o0 = (1 - damping) * o0 + damping * g->m_fv0[e];
+            o1 = (1 - damping) * o1 + damping * g->m_fv1[e];
+        }
+        g->m_fv0[e] = o0;
+        g->m_fv1[e] = o1;
+    }
+    /* products over var-major edges */
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < V; v++) {
+        prod_var(g->m_fv0.data(), g->m_fv1.data(), g->row_ptr_fv[v], g->row_ptr_fv[v + 1],
+                 g->prod0[v], g->prod1[v], g->nz0[v], g->nz1[v]);
+    }
+    /* v -> f over factor-major edges */
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < E; e++) {
+        const FacMajorDesc &d = g->vf_desc[e];
+        double o0, o1;
+        vf_edge(g->prod0[d.v], g->prod1[d.v], g->nz0[d.v], g->nz1[d.v],
+                g->m_fv0[d.src], g->m_fv1[d.src], o0, o1);
+        g->m_vf0[d.ind] = o0;
+        g->m_vf1[d.ind] = o1;
+    }
+    /* beliefs + max |old - new|, reduce init 0.0 */
+    double maxdiff = 0.0;
+#pragma omp parallel
+    {
+        double local = 0.0;
+#pragma omp for schedule(static)
+        for (int64_t v = 0; v < V; v++) {
+            double b = belief_of(g->prod0[v], g->prod1[v], g->nz0[v], g->nz1[v]);
+            double diff = std::fabs(g->old_beliefs[v] - b);
+            local = nanmax(local, diff);
+            g->old_beliefs[v] = b;
+        }
+#pragma omp critical
+        maxdiff = nanmax(maxdiff, local);
+    }
+    return maxdiff;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+/* CausalFactorGraph(vector) (causal_fg.cpp:50-68) + constructGraph (28-48)
+ * + BipartiteGraph::addEdge (bipgraph.cpp:24-42) + CausalBP::construct
+ * (causal_bp.cu:310-407) + putParallUpdateSeqToKernelFused (724-817). */
+int orc_build(const orc_factor_list *fl, orc_graph **out) {
+    if (!fl || !out) return fail("null argument");
+    const uint64_t F = fl->n_factors;
+    std::vector<std::vector<uint64_t>> fvars(F);
+    std::set<uint64_t> labels;
+    uint64_t E = 0;
+    for (uint64_t I = 0; I < F; I++) {
+        if (fl->type[I] < 0 || fl->type[I] > 2) return fail("bad factor type");
+        factor_vars(fl, I, fvars[I]);
+        labels.insert(fvars[I].begin(), fvars[I].end());
+        E += fvars[I].size();
+    }
+    orc_graph *g = new orc_graph();
+    g->F = F;
+    g->E = E;
+    g->V = labels.size();
+    g->var_label.assign(labels.begin(), labels.end());
+    std::map<uint64_t, uint64_t> index;
+    for (uint64_t i = 0; i < g->V; i++) index[g->var_label[i]] = i;
+
+    /* nbF(I): vars of I in ascending label; nbV(i): factors in ascending
+     * index, each with its dual position. */
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> nbV(g->V);  /* (I, dual) */
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> nbF(F);     /* (i, dual) */
+    for (uint64_t I = 0; I < F; I++) {
+        for (uint64_t lab : fvars[I]) {
+            uint64_t i = index[lab];
+            uint64_t it1 = nbV[i].size(), it2 = nbF[I].size();
+            nbV[i].push_back({I, it2});
+            nbF[I].push_back({i, it1});
+        }
+    }
+    g->row_ptr_vf.assign(F + 1, 0);
+    for (uint64_t I = 0; I < F; I++) g->row_ptr_vf[I + 1] = g->row_ptr_vf[I] + nbF[I].size();
+    g->row_ptr_fv.assign(g->V + 1, 0);
+    for (uint64_t i = 0; i < g->V; i++) g->row_ptr_fv[i + 1] = g->row_ptr_fv[i] + nbV[i].size();
+    g->edge_var.resize(E);
+    g->head_slot.resize(F);
+    for (uint64_t I = 0; I < F; I++) {
+        for (uint64_t k = 0; k < nbF[I].size(); k++) {
+            uint64_t i = nbF[I][k].first;
+            g->edge_var[g->row_ptr_vf[I] + k] = i;
+            if (g->var_label[i] == fl->head[I]) g->head_slot[I] = g->row_ptr_vf[I] + k;
+        }
+    }
+    g->fv_desc.resize(E);
+    for (uint64_t i = 0; i < g->V; i++) {
+        for (uint64_t k = 0; k < nbV[i].size(); k++) {
+            uint64_t I = nbV[i][k].first, dual = nbV[i][k].second;
+            VarMajorDesc &d = g->fv_desc[g->row_ptr_fv[i] + k];
+            d.src = g->row_ptr_vf[I] + dual;
+            d.start = g->row_ptr_vf[I];
+            d.end = g->row_ptr_vf[I + 1];
+            d.head = g->head_slot[I];
+            d.q = fl->q[I];
+            d.p = fl->p[I];
+            d.mask0 = fl->mask0[I];
+            d.mask1 = fl->mask1[I];
+            bool is_head = fl->head[I] == g->var_label[i];
+            bool clamped = fl->head_clamped[I] != 0;
+            int t = fl->type[I];
+            if (t == 0) d.code = 0;
+            else if (t == 1) d.code = clamped ? (is_head ? 3 : 7) : (is_head ? 1 : 5);
+            else d.code = clamped ? (is_head ? 4 : 8) : (is_head ? 2 : 6);
+        }
+    }
+    g->vf_desc.resize(E);
+    for (uint64_t I = 0; I < F; I++) {
+        for (uint64_t k = 0; k < nbF[I].size(); k++) {
+            uint64_t i = nbF[I][k].first, dual = nbF[I][k].second;
+            g->vf_desc[g->row_ptr_vf[I] + k] = {g->row_ptr_vf[I] + k, i, g->row_ptr_fv[i] + dual};
+        }
+    }
+    g->m_fv0.assign(E, 0.0); g->m_fv1.assign(E, 0.0);
+    g->m_vf0.assign(E, 0.0); g->m_vf1.assign(E, 0.0);
+    g->prod0.assign(g->V, 0.0); g->prod1.assign(g->V, 0.0);
+    g->nz0.assign(g->V, 0); g->nz1.assign(g->V, 0);
+    g->old_beliefs.assign(g->V, 0.0);
+    *out = g;
+    return 0;
+}
+
+void orc_free(orc_graph *g) { delete g; }
+uint64_t orc_nvars(const orc_graph *g) { return g->V; }
+uint64_t orc_nfactors(const orc_graph *g) { return g->F; }
+uint64_t orc_nedges(const orc_graph *g) { return g->E; }
+uint64_t orc_iters(const orc_graph *g) { return g->iters; }
+
+int orc_dump(const orc_graph *g, uint64_t *var_label, uint64_t *fac_off, uint64_t *edge_var,
+             uint64_t *var_off, uint64_t *var_src, uint64_t *head_slot, int32_t *code) {
+    if (var_label) std::copy(g->var_label.begin(), g->var_label.end(), var_label);
+    if (fac_off) std::copy(g->row_ptr_vf.begin(), g->row_ptr_vf.end(), fac_off);
+    if (edge_var) std::copy(g->edge_var.begin(), g->edge_var.end(), edge_var);
+    if (var_off) std::copy(g->row_ptr_fv.begin(), g->row_ptr_fv.end(), var_off);
+    if (head_slot) std::copy(g->head_slot.begin(), g->head_slot.end(), head_slot);
+    for (uint64_t e = 0; e < g->E; e++) {
+        if (var_src) var_src[e] = g->fv_desc[e].src;
+        if (code) code[e] = g->fv_desc[e].code;
+    }
+    return 0;
+}
+
+/* CausalBP::init, causal_bp.cu:824-856 (old beliefs are not reset). */
+int orc_init(orc_graph *g, int logdomain) {
+    double c = logdomain ? 0.0 : 1.0;
+    std::fill(g->m_fv0.begin(), g->m_fv0.end(), c);
+    std::fill(g->m_fv1.begin(), g->m_fv1.end(), c);
+    std::fill(g->m_vf0.begin(), g->m_vf0.end(), c);
+    std::fill(g->m_vf1.begin(), g->m_vf1.end(), c);
+    std::fill(g->prod0.begin(), g->prod0.end(), c);
+    std::fill(g->prod1.begin(), g->prod1.end(), c);
+    std::fill(g->nz0.begin(), g->nz0.end(), 0);
+    std::fill(g->nz1.begin(), g->nz1.end(), 0);
+    return 0;
+}
+
+/* CausalBP::run(), causal_bp.cu:858-910 (maxtime = infinity). */
+int orc_run(orc_graph *g, double tol, uint64_t maxiter, double damping, int threads,
+            uint64_t *iters_out, double *maxdiff_out) {
+    set_threads(threads);
+    double maxDiff = INFINITY;
+    for (; g->iters < maxiter && maxDiff > tol; g->iters++) {
+        maxDiff = sweep(g, damping);
+    }
+    if (iters_out) *iters_out = g->iters;
+    if (maxdiff_out) *maxdiff_out = maxDiff;
+    return 0;
+}
+
+/* CausalBP::run(tol, maxIters, histLength), causal_bp.cu:912-1004 for the
+ * PARALL schedule: exactly maxIters sweeps, then beliefs once. */
+int orc_run_fixed(orc_graph *g, uint64_t n_iters, double damping, int threads) {
+    set_threads(threads);
+    for (uint64_t k = 0; k < n_iters; k++, g->iters++) {
+        (void)sweep(g, damping);
+    }
+    return 0;
+}
+
+int orc_sweep(orc_graph *g, double damping, int threads, double *maxdiff_out) {
+    set_threads(threads);
+    double d = sweep(g, damping);
+    if (maxdiff_out) *maxdiff_out = d;
+    return 0;
+}
+
+int orc_beliefs(const orc_graph *g, double *out) {
+    std::copy(g->old_beliefs.begin(), g->old_beliefs.end(), out);
+    return 0;
+}
+
+int orc_messages(const orc_graph *g, double *fv0, double *fv1, double *vf0, double *vf1) {
+    if (fv0) std::copy(g->m_fv0.begin(), g->m_fv0.end(), fv0);
+    if (fv1) std::copy(g->m_fv1.begin(), g->m_fv1.end(), fv1);
+    if (vf0) std::copy(g->m_vf0.begin(), g->m_vf0.end(), vf0);
+    if (vf1) std::copy(g->m_vf1.begin(), g->m_vf1.end(), vf1);
+    return 0;
+}
+
+int orc_set_messages(orc_graph *g, const double *fv0, const double *fv1, const double *vf0,
+                     const double *vf1) {
+    if (fv0) std::copy(fv0, fv0 + g->E, g->m_fv0.begin());
+    if (fv1) std::copy(fv1, fv1 + g->E, g->m_fv1.begin());
+    if (vf0) std::copy(vf0, vf0 + g->E, g->m_vf0.begin());
+    if (vf1) std::copy(vf1, vf1 + g->E, g->m_vf1.begin());
+    return 0;
+}
+
+void orc_fv_message(int code, uint64_t deg, uint64_t src, uint64_t head, double p, double q,
+                    double mask0, double mask1, const double *m0, const double *m1,
+                    double *out0, double *out1) {
+    VarMajorDesc d;
+    d.code = code;
+    d.src = src;
+    d.start = 0;
+    d.end = deg;
+    d.head = head;
+    d.q = q;
+    d.p = p;
+    d.mask0 = mask0;
+    d.mask1 = mask1;
+    fv_edge(d, m0, m1, *out0, *out1);
+}
+
+void orc_vf_message(uint64_t deg, uint64_t src, const double *m0, const double *m1,
+                    double *out0, double *out1) {
+    double p0, p1;
+    uint64_t z0, z1;
+    prod_var(m0, m1, 0, deg, p0, p1, z0, z1);
+    vf_edge(p0, p1, z0, z1, m0[src], m1[src], *out0, *out1);
+}
+
+/* CausalFactor::gen_clamped, csrc/ext/causal_factor.cpp:63-100, applied by
+ * CausalFactorGraph::clamp (causal_fg.cpp:17-26) to every factor holding the
+ * var, one clamp after another in ascending label order (fl_wrapper.cu:26-45:
+ * std::map<int,bool> iteration order). */
+int orc_clamp(const orc_factor_list *fl, uint64_t n_clamps, const uint64_t *labels,
+              const uint8_t *values, orc_flist **out) {
+    const uint64_t F = fl->n_factors;
+    struct Fac {
+        int32_t type;
+        uint64_t head;
+        std::vector<uint64_t> body;  /* sorted unique */
+        double p, q, mask0, mask1;
+        uint8_t clamped;
+    };
+    std::vector<Fac> facs(F);
+    for (uint64_t I = 0; I < F; I++) {
+        Fac &f = facs[I];
+        f.type = fl->type[I];
+        f.head = fl->head[I];
+        f.body.assign(fl->body + fl->body_off[I], fl->body + fl->body_off[I + 1]);
+        std::sort(f.body.begin(), f.body.end());
+        f.body.erase(std::unique(f.body.begin(), f.body.end()), f.body.end());
+        f.p = fl->p[I]; f.q = fl->q[I];
+        f.mask0 = fl->mask0[I]; f.mask1 = fl->mask1[I];
+        f.clamped = fl->head_clamped[I];
+    }
+    std::map<uint64_t, uint8_t> clamps;
+    for (uint64_t k = 0; k < n_clamps; k++) clamps[labels[k]] = values[k] ? 1 : 0;
+    /* the adjacency used is the original graph's (clamp does not rebuild _G) */
+    std::map<uint64_t, std::vector<uint64_t>> holders;
+    for (uint64_t I = 0; I < F; I++) {
+        std::vector<uint64_t> vs;
+        factor_vars(fl, I, vs);
+        for (uint64_t lab : vs)
+            if (clamps.count(lab)) holders[lab].push_back(I);
+    }
+    for (auto &c : clamps) {
+        const uint64_t lab = c.first;
+        const uint64_t x = c.second;
+        for (uint64_t I : holders[lab]) {
+            Fac &f = facs[I];
+            if (f.type == 0) {
+                if (lab == f.head) f.p = (double)x;
+            } else if (lab == f.head) {
+                double n0 = x == 0 ? 1.0 : 0.0, n1 = x == 1 ? 1.0 : 0.0;
+                if (f.clamped) {
+                    n0 *= f.mask0;
+                    n1 *= f.mask1;
+                }
+                f.clamped = 1;
+                f.mask0 = n0;
+                f.mask1 = n1;
+            } else {
+                auto it = std::lower_bound(f.body.begin(), f.body.end(), lab);
+                if (it != f.body.end() && *it == lab) {
+                    f.body.erase(it);
+                    if ((x == 0 && f.type == 1) || (x > 0 && f.type == 2)) {
+                        f.p = f.q;
+                        f.body.clear();
+                    }
+                }
+            }
+        }
+    }
+    orc_flist *l = new orc_flist();
+    l->body_off.push_back(0);
+    for (auto &f : facs) {
+        l->type.push_back(f.type);
+        l->head.push_back(f.head);
+        l->body.insert(l->body.end(), f.body.begin(), f.body.end());
+        l->body_off.push_back(l->body.size());
+        l->p.push_back(f.p); l->q.push_back(f.q);
+        l->mask0.push_back(f.mask0); l->mask1.push_back(f.mask1);
+        l->head_clamped.push_back(f.clamped);
+    }
+    *out = l;
+    return 0;
+}
+
+void orc_flist_view(const orc_flist *l, orc_factor_list *v) {
+    v->n_factors = l->type.size();
+    v->type = l->type.data();
+    v->head = l->head.data();
+    v->body_off = l->body_off.data();
+    v->body = l->body.data();
+    v->p = l->p.data();
+    v->q = l->q.data();
+    v->head_clamped = l->head_clamped.data();
+    v->mask0 = l->mask0.data();
+    v->mask1 = l->mask1.data();
+}
+
+void orc_flist_free(orc_flist *l) { delete l; }
+
+}  // extern "C"
+
+namespace {
+
+/* p of every AND factor with a rule becomes sigmoid(theta[rule]). */
+int tied_list(const orc_factor_list *fl, const int32_t *rule_of_factor, uint64_t n_rules,
+              const double *theta, std::vector<double> &p_out) {
+    p_out.assign(fl->p, fl->p + fl->n_factors);
+    for (uint64_t I = 0; I < fl->n_factors; I++) {
+        int32_t r = rule_of_factor[I];
+        if (r < 0) continue;
+        if ((uint64_t)r >= n_rules) return fail("rule index out of range");
+        p_out[I] = 1.0 / (1.0 + std::exp(-theta[r]));
+    }
+    return 0;
+}
+
+int alarm_index(const orc_graph *g, uint64_t n_alarms, const uint64_t *alarm_labels,
+                std::vector<uint64_t> &idx) {
+    idx.resize(n_alarms);
+    for (uint64_t a = 0; a < n_alarms; a++) {
+        auto it = std::lower_bound(g->var_label.begin(), g->var_label.end(), alarm_labels[a]);
+        if (it == g->var_label.end() || *it != alarm_labels[a]) return fail("alarm label not in graph");
+        idx[a] = (uint64_t)(it - g->var_label.begin());
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_loss(const orc_factor_list *fl, const int32_t *rule_of_factor, uint64_t n_rules,
+             const double *theta, uint64_t n_alarms, const uint64_t *alarm_labels,
+             const double *targets, uint64_t T, double damping, double *loss_out) {
+    std::vector<double> p;
+    if (tied_list(fl, rule_of_factor, n_rules, theta, p)) return -1;
+    orc_factor_list tied = *fl;
+    tied.p = p.data();
+    orc_graph *g = nullptr;
+    if (orc_build(&tied, &g)) return -1;
+    std::vector<uint64_t> idx;
+    if (alarm_index(g, n_alarms, alarm_labels, idx)) { orc_free(g); return -1; }
+    orc_init(g, 0);
+    orc_run_fixed(g, T, damping, 0);
+    double L = 0.0;
+    for (uint64_t a = 0; a < n_alarms; a++) {
+        double r = g->old_beliefs[idx[a]] - targets[a];
+        L += r * r;
+    }
+    *loss_out = L;
+    orc_free(g);
+    return 0;
+}
+
+int orc_grad(const orc_factor_list *fl, const int32_t *rule_of_factor, uint64_t n_rules,
+             const double *theta, uint64_t n_alarms, const uint64_t *alarm_labels,
+             const double *targets, uint64_t T, double damping, double *loss_out,
+             double *grad_out) {
+    (void)fl; (void)rule_of_factor; (void)n_rules; (void)theta; (void)n_alarms;
+    (void)alarm_labels; (void)targets; (void)T; (void)damping; (void)loss_out; (void)grad_out;
+    return fail("orc_grad: not implemented");
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+"""GPU parity: the HIP engine against the oracle (the reference's FP64
+PARALL arithmetic restated on the CPU) on the same seeded graphs.
+
+The engine keeps the reference's per-message operation order, so marginals
+must agree bit for bit (a stronger form of the 1e-9 bar), iteration counts
+and the final max belief change must be identical, and the alarm ranking
+(marginal descending, label ascending) must match.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import fastlbp_amd as fl
+import oracle
+from tests.helpers import oracle_clamp, random_alarm_clamps, same_bits
+
+pytestmark = pytest.mark.gpu
+
+TOL_MARGINAL = 1e-9   # the stated bar; checked besides bitwise equality
+
+
+def solve_both(flist, tol, maxiter, damping, threads=8):
+    g = fl.Graph(flist)
+    plan = fl.Plan(g)
+    plan.init()
+    r = plan.run(tol, maxiter, damping)
+    bg = plan.beliefs()
+    o = oracle.OracleGraph(flist)
+    o.init()
+    it, md = o.run(tol, maxiter, damping, threads=threads)
+    bo = o.beliefs()
+    return g, plan, r, bg, o, it, md, bo
+
+
+def assert_parity(r, bg, it, md, bo):
+    assert r.iters == it, f"iterations differ: engine {r.iters}, oracle {it}"
+    assert np.max(np.abs(bg - bo), initial=0.0) <= TOL_MARGINAL
+    assert same_bits(bg, bo), "marginals are not bit-identical"
+    assert (math.isinf(md) and math.isinf(r.maxdiff)) or same_bits([r.maxdiff], [md])
+
+
+def test_c1_plain_fp64():
+    """2,000 variables, no damping, tol 1e-6, at most 1000 iterations."""
+    gen = fl.generate(2000, 1)
+    g, plan, r, bg, o, it, md, bo = solve_both(gen.factors, 1e-6, 1000, 0.0)
+    assert_parity(r, bg, it, md, bo)
+    assert r.iters < 1000 and md <= 1e-6
+    idx = g.index_of(gen.alarms)
+    assert np.array_equal(fl.rank_alarms(gen.alarms, bg[idx]), fl.rank_alarms(gen.alarms, bo[idx]))
+
+
+@pytest.mark.parametrize("seed,n_vars,damping,tol", [
+    (2, 5000, 0.3, 1e-9), (3, 20000, 0.0, 1e-9), (4, 777, 0.5, 1e-12), (5, 50000, 0.3, 1e-9)])
+def test_seeded_graphs(seed, n_vars, damping, tol):
+    gen = fl.generate(n_vars, seed)
+    assert_parity(*solve_both(gen.factors, tol, 500, damping)[2:])
+
+
+def test_messages_bitwise_after_fixed_sweeps():
+    """run(tol, maxIters, hist): exactly n sweeps; every message equal."""
+    gen = fl.generate(3000, 11)
+    g = fl.Graph(gen.factors)
+    plan = fl.Plan(g)
+    plan.init()
+    plan.run_fixed(7, 0.2)
+    fv, vf = plan.messages()
+    o = oracle.OracleGraph(gen.factors)
+    o.init()
+    o.run_fixed(7, 0.2)
+    ofv, ovf = o.messages()
+    assert same_bits(fv, ofv) and same_bits(vf, ovf)
+    assert same_bits(plan.beliefs(), o.beliefs())
+
+
+def test_high_degree_paths():
+    """Factors and variables above the register-path degree (8)."""
+    gen = fl.generate(4000, 21, k_min=6, k_max=14, b_min=4, b_max=10, leaf_frac=0.02)
+    g = fl.Graph(gen.factors)
+    assert np.diff(g.fac_off).max() > 8 and np.diff(g.var_off).max() > 8
+    assert_parity(*solve_both(gen.factors, 1e-9, 300, 0.3)[2:])
+
+
+@pytest.mark.parametrize("variant", range(6))
+def test_clamped_variants(variant):
+    """fl_wrapper-style clamps of 1-50 alarms, rebuilt graph."""
+    gen = fl.generate(6000, 31)
+    rng = np.random.default_rng(100 + variant)
+    labels, values = random_alarm_clamps(rng, gen.alarms, 50)
+    if variant == 5:   # head, body and leaf clamps mixed, duplicates included
+        labels = np.concatenate([labels, [0, 1, labels[0]]]).astype(np.uint64)
+        values = np.concatenate([values, [1, 0, values[0]]]).astype(np.uint8)
+    ce = fl.clamp(gen.factors, labels, values)
+    co = oracle_clamp(gen.factors, labels, values)
+    for k in ("type", "head", "body_off", "body", "p", "q", "head_clamped", "mask0", "mask1"):
+        assert np.array_equal(getattr(ce, k), getattr(co, k)), k
+    assert_parity(*solve_both(ce, 1e-9, 500, 0.3 if variant % 2 else 0.0)[2:])
+
+
+def test_contradictory_clamps_and_tiny_graphs():
+    # an AND head clamped true while its only body var is clamped false: zero mass
+    flist = fl.FactorList.build([(0, 1, [], 0.8), (1, 2, [1], 0.9, 0.0), (0, 3, [], 0.5),
+                                 (2, 4, [2, 3], 1.0, 0.0)])
+    c = fl.clamp(flist, [1, 2, 4], [0, 1, 1])
+    assert_parity(*solve_both(c, 1e-9, 50, 0.0)[2:])
+    single = fl.FactorList.build([(0, 5, [], 0.3)])
+    assert_parity(*solve_both(single, 1e-9, 50, 0.0)[2:])
+
+
+def test_run_semantics_counter_and_init():
+    """_iters persists across run() and init(); old beliefs are not reset."""
+    gen = fl.generate(2500, 41)
+    g = fl.Graph(gen.factors)
+    plan = fl.Plan(g)
+    o = oracle.OracleGraph(gen.factors)
+    plan.init(); o.init()
+    r1 = plan.run(1e-4, 1000, 0.0)
+    i1, _ = o.run(1e-4, 1000, 0.0)
+    assert r1.iters == i1
+    plan.init(); o.init()
+    r2 = plan.run(1e-9, 1000, 0.0)
+    i2, _ = o.run(1e-9, 1000, 0.0)
+    assert r2.iters == i2 and same_bits(plan.beliefs(), o.beliefs())
+    # logdomain init (0, 0) messages: the reference's arithmetic, whatever it yields
+    plan.init(logdomain=True); o.init(logdomain=True)
+    plan.run_fixed(3); o.run_fixed(3)
+    assert same_bits(plan.beliefs(), o.beliefs())
+
+
+def test_c2_full_size_parity():
+    """1M variables, damping 0.3, tol 1e-9, at most 500 iterations: the full
+    configuration against the oracle (OpenMP)."""
+    gen = fl.generate(1_000_000, 2)
+    g, plan, r, bg, o, it, md, bo = solve_both(gen.factors, 1e-9, 500, 0.3, threads=16)
+    assert_parity(r, bg, it, md, bo)
+    assert np.all((bg >= 0) & (bg <= 1))
