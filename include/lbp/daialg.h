@@ -1,0 +1,19 @@
+/* lbp/daialg.h -- inference-algorithm mixin (interface of the reference's
+ * include/lbp/daialg.h:14-33): DAIAlg<GRM> is-a GRM holding a copy. */
+#pragma once
+
+namespace lbp {
+
+class InfAlg {
+public:
+    InfAlg() = default;
+};
+
+template <class GRM>
+class DAIAlg : public InfAlg, public GRM {
+public:
+    DAIAlg() = default;
+    DAIAlg(const GRM &grm) : InfAlg(), GRM(grm) {}
+};
+
+}  // namespace lbp

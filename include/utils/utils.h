@@ -1,0 +1,44 @@
+/* utils/utils.h -- basic types and helpers of the drop-in C++ API.
+ * Same names as the reference's include/utils/utils.h:21-22, 109-122. */
+#pragma once
+
+#include <sys/time.h>
+
+#include <cstddef>
+#include <iostream>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+namespace lbp {
+
+typedef double Real;
+typedef size_t Size;
+
+/* wall clock in seconds */
+inline Real toc() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return (Real)tv.tv_sec + (Real)tv.tv_usec / 1000000.0;
+}
+
+template <class T>
+std::ostream &operator<<(std::ostream &os, const std::vector<T> &x) {
+    os << "(";
+    for (size_t i = 0; i < x.size(); i++) os << (i ? ", " : "") << x[i];
+    return os << ")";
+}
+
+template <class T>
+std::ostream &operator<<(std::ostream &os, const std::set<T> &x) {
+    os << "{";
+    bool first = true;
+    for (const auto &v : x) {
+        os << (first ? "" : ", ") << v;
+        first = false;
+    }
+    return os << "}";
+}
+
+}  // namespace lbp
