@@ -33,6 +33,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <set>
 #include <string>
 #include <vector>
@@ -728,9 +729,175 @@ int orc_grad(const orc_factor_list *fl, const int32_t *rule_of_factor, uint64_t 
              const double *theta, uint64_t n_alarms, const uint64_t *alarm_labels,
              const double *targets, uint64_t T, double damping, double *loss_out,
              double *grad_out) {
-    (void)fl; (void)rule_of_factor; (void)n_rules; (void)theta; (void)n_alarms;
-    (void)alarm_labels; (void)targets; (void)T; (void)damping; (void)loss_out; (void)grad_out;
-    return fail("orc_grad: not implemented");
+    std::vector<double> p;
+    if (tied_list(fl, rule_of_factor, n_rules, theta, p)) return -1;
+    orc_factor_list tied = *fl;
+    tied.p = p.data();
+    orc_graph *g = nullptr;
+    if (orc_build(&tied, &g)) return -1;
+    std::unique_ptr<orc_graph> hold(g);
+    std::vector<uint64_t> idx;
+    if (alarm_index(g, n_alarms, alarm_labels, idx)) return -1;
+    const uint64_t E = g->E, V = g->V, F = g->F;
+    orc_init(g, 0);
+    /* forward, every state kept: vf[t] = m_vf after t sweeps (factor-major),
+     * fv[t] = m_fv after t sweeps (var-major) */
+    std::vector<std::vector<double>> vf0(T + 1), vf1(T + 1), fv0(T + 1), fv1(T + 1);
+    vf0[0] = g->m_vf0; vf1[0] = g->m_vf1; fv0[0] = g->m_fv0; fv1[0] = g->m_fv1;
+    for (uint64_t t = 1; t <= T; t++) {
+        (void)sweep(g, damping);
+        vf0[t] = g->m_vf0; vf1[t] = g->m_vf1; fv0[t] = g->m_fv0; fv1[t] = g->m_fv1;
+    }
+    double L = 0.0;
+    std::vector<double> bbar(V, 0.0);
+    for (uint64_t a = 0; a < n_alarms; a++) {
+        double r = g->old_beliefs[idx[a]] - targets[a];
+        L += r * r;
+        bbar[idx[a]] += 2.0 * r;
+    }
+    *loss_out = L;
+    /* factor of each factor-major slot, and each factor's var-major edges */
+    std::vector<uint64_t> fac_of(E);
+    for (uint64_t I = 0; I < F; I++)
+        for (uint64_t s = g->row_ptr_vf[I]; s < g->row_ptr_vf[I + 1]; s++) fac_of[s] = I;
+    std::vector<double> pbar(F, 0.0);
+    std::vector<double> avf0(E, 0.0), avf1(E, 0.0);          /* adjoint of m_vf^t */
+    std::vector<double> carry0(E, 0.0), carry1(E, 0.0);      /* adjoint of m_fv^t via damping */
+    std::vector<double> afv0(E), afv1(E);
+    std::vector<double> nvf0(E), nvf1(E);
+    for (uint64_t t = T; t >= 1; t--) {
+        /* variable pass of sweep t: inputs fv[t] (var-major), outputs vf[t] and beliefs */
+        for (uint64_t v = 0; v < V; v++) {
+            const uint64_t a = g->row_ptr_fv[v], b = g->row_ptr_fv[v + 1], d = b - a;
+            const double *m0 = &fv0[t][a], *m1 = &fv1[t][a];
+            double P0 = 1.0, P1 = 1.0;
+            for (uint64_t k = 0; k < d; k++) { P0 *= m0[k]; P1 *= m1[k]; }
+            double G0 = 0.0, G1 = 0.0;
+            if (t == T && bbar[v] != 0.0 && P0 + P1 != 0.0) {
+                const double den = (P0 + P1) * (P0 + P1);
+                G1 = bbar[v] * P0 / den;
+                G0 = -bbar[v] * P1 / den;
+            }
+            std::vector<double> g0(d, 0.0), g1(d, 0.0);
+            for (uint64_t e = 0; e < d; e++) {
+                double L0 = 1.0, L1 = 1.0;
+                for (uint64_t j = 0; j < d; j++)
+                    if (j != e) { L0 *= m0[j]; L1 *= m1[j]; }
+                const double S = L0 + L1;
+                if (S == 0.0) continue;
+                const uint64_t slot = g->fv_desc[a + e].src;
+                const double o0 = L0 / S, o1 = L1 / S;
+                const double dot = avf0[slot] * o0 + avf1[slot] * o1;
+                g0[e] = (avf0[slot] - dot) / S;
+                g1[e] = (avf1[slot] - dot) / S;
+            }
+            for (uint64_t e2 = 0; e2 < d; e2++) {
+                double s0 = 0.0, s1 = 0.0;
+                for (uint64_t e = 0; e < d; e++) {
+                    if (e == e2) continue;
+                    double q0 = 1.0, q1 = 1.0;
+                    for (uint64_t j = 0; j < d; j++)
+                        if (j != e && j != e2) { q0 *= m0[j]; q1 *= m1[j]; }
+                    s0 += g0[e] * q0;
+                    s1 += g1[e] * q1;
+                }
+                double l0 = 1.0, l1 = 1.0;
+                for (uint64_t j = 0; j < d; j++)
+                    if (j != e2) { l0 *= m0[j]; l1 *= m1[j]; }
+                afv0[a + e2] = carry0[a + e2] + s0 + G0 * l0;
+                afv1[a + e2] = carry1[a + e2] + s1 + G1 * l1;
+            }
+        }
+        /* factor pass of sweep t: m_fv^t = (1-d) u + d m_fv^{t-1}, u = FV(vf[t-1], p) */
+        std::fill(nvf0.begin(), nvf0.end(), 0.0);
+        std::fill(nvf1.begin(), nvf1.end(), 0.0);
+        for (uint64_t e = 0; e < E; e++) {
+            carry0[e] = damping * afv0[e];
+            carry1[e] = damping * afv1[e];
+            const double ub0 = (1 - damping) * afv0[e], ub1 = (1 - damping) * afv1[e];
+            const VarMajorDesc &dsc = g->fv_desc[e];
+            const uint64_t I = fac_of[dsc.src];
+            const double *x0 = vf0[t - 1].data(), *x1 = vf1[t - 1].data();
+            const double pp = dsc.p, qq = dsc.q;
+            const bool clamped = dsc.code == 3 || dsc.code == 4 || dsc.code == 7 || dsc.code == 8;
+            const double mk0 = clamped ? dsc.mask0 : 1.0, mk1 = clamped ? dsc.mask1 : 1.0;
+            double M0, M1;
+            if (dsc.code == 0) {
+                M0 = 1 - pp; M1 = pp;
+                const double S = M0 + M1;
+                const double u0 = M0 / S, u1 = M1 / S, dot = ub0 * u0 + ub1 * u1;
+                pbar[I] += ((ub1 - dot) - (ub0 - dot)) / S;
+                continue;
+            }
+            const bool is_and = dsc.code == 1 || dsc.code == 3 || dsc.code == 5 || dsc.code == 7;
+            const bool head_target = dsc.code >= 1 && dsc.code <= 4;
+            /* x = the state the CPD singles out (1 for AND, 0 for OR), y = the other */
+            auto X = [&](uint64_t j) { return is_and ? x1[j] : x0[j]; };
+            auto Y = [&](uint64_t j) { return is_and ? x0[j] : x1[j]; };
+            auto addX = [&](uint64_t j, double v) { if (is_and) nvf1[j] += v; else nvf0[j] += v; };
+            auto addY = [&](uint64_t j, double v) { if (is_and) nvf0[j] += v; else nvf1[j] += v; };
+            if (head_target) {
+                double pi = 1.0, Z = 1.0;
+                for (uint64_t j = dsc.start; j < dsc.end; j++)
+                    if (j != dsc.src) { pi *= X(j); Z *= X(j) + Y(j); }
+                const double Mu = (pp - qq) * pi + qq * Z, Mw = (qq - pp) * pi + (1 - qq) * Z;
+                M0 = (is_and ? Mw : Mu) * mk0;
+                M1 = (is_and ? Mu : Mw) * mk1;
+                const double S = M0 + M1;
+                if (S == 0.0) continue;
+                const double u0 = M0 / S, u1 = M1 / S, dot = ub0 * u0 + ub1 * u1;
+                const double Mb0 = (ub0 - dot) / S * mk0, Mb1 = (ub1 - dot) / S * mk1;
+                const double Ub = is_and ? Mb1 : Mb0, Wb = is_and ? Mb0 : Mb1;
+                const double pib = (pp - qq) * (Ub - Wb), Zb = qq * Ub + (1 - qq) * Wb;
+                pbar[I] += (Ub - Wb) * pi;
+                for (uint64_t j = dsc.start; j < dsc.end; j++) {
+                    if (j == dsc.src) continue;
+                    double px = 1.0, pz = 1.0;
+                    for (uint64_t k = dsc.start; k < dsc.end; k++)
+                        if (k != dsc.src && k != j) { px *= X(k); pz *= X(k) + Y(k); }
+                    addX(j, pib * px + Zb * pz);
+                    addY(j, Zb * pz);
+                }
+            } else {
+                const uint64_t h = dsc.head;
+                const double hx = is_and ? x1[h] * mk1 : x0[h] * mk0;
+                const double hy = is_and ? x0[h] * mk0 : x1[h] * mk1;
+                const double B = (pp - qq) * (hx - hy), A = qq * hx + (1 - qq) * hy;
+                double pi = 1.0, Z = 1.0;
+                for (uint64_t j = dsc.start; j < dsc.end; j++)
+                    if (j != dsc.src && j != h) { pi *= X(j); Z *= X(j) + Y(j); }
+                const double Ma = A * Z, Mb = A * Z + B * pi;
+                M0 = is_and ? Ma : Mb;
+                M1 = is_and ? Mb : Ma;
+                const double S = M0 + M1;
+                if (S == 0.0) continue;
+                const double u0 = M0 / S, u1 = M1 / S, dot = ub0 * u0 + ub1 * u1;
+                const double Mb0 = (ub0 - dot) / S, Mb1 = (ub1 - dot) / S;
+                const double Mab = is_and ? Mb0 : Mb1, Mbb = is_and ? Mb1 : Mb0;
+                const double Zb = (Mab + Mbb) * A, Ab = (Mab + Mbb) * Z, Bb = Mbb * pi, pib = Mbb * B;
+                pbar[I] += Bb * (hx - hy);
+                const double hxb = Bb * (pp - qq) + Ab * qq, hyb = -Bb * (pp - qq) + Ab * (1 - qq);
+                if (is_and) { nvf1[h] += hxb * mk1; nvf0[h] += hyb * mk0; }
+                else { nvf0[h] += hxb * mk0; nvf1[h] += hyb * mk1; }
+                for (uint64_t j = dsc.start; j < dsc.end; j++) {
+                    if (j == dsc.src || j == h) continue;
+                    double px = 1.0, pz = 1.0;
+                    for (uint64_t k = dsc.start; k < dsc.end; k++)
+                        if (k != dsc.src && k != h && k != j) { px *= X(k); pz *= X(k) + Y(k); }
+                    addX(j, pib * px + Zb * pz);
+                    addY(j, Zb * pz);
+                }
+            }
+        }
+        avf0.swap(nvf0);
+        avf1.swap(nvf1);
+    }
+    for (uint64_t r = 0; r < n_rules; r++) grad_out[r] = 0.0;
+    for (uint64_t I = 0; I < F; I++) {
+        const int32_t r = rule_of_factor[I];
+        if (r >= 0) grad_out[r] += pbar[I] * p[I] * (1.0 - p[I]);
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -61,6 +61,12 @@ def test_fl_wrapper_cli_matches_oracle(tmp_path):
     o.run_fixed(60)
     bo = o.beliefs()
     labs = o.dump()["var_label"]
+    clamps = dict(zip(labels, values))
+    expect = {}
     for l, b in zip(labs, bo):
-        assert marg[int(l)] == b or (math.isnan(b) and int(l) in dict(zip(labels, values)))
+        expect[int(l)] = b
+    for l in range(int(labs.max()) + 1):   # fl_wrapper.cu:80-88
+        b = expect.get(l, -1.0)
+        want = b if 0 <= b <= 1 else (float(clamps[l]) if l in clamps else 0.5)
+        assert marg[l] == want, (l, marg[l], want)
     assert len(marg) == int(labs.max()) + 1
