@@ -120,6 +120,14 @@ SIGNATURES = {
     "flbp_plan_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_plan_last_timing": (C.c_int, [C.c_void_p, f64p, f64p, u64p, u64p]),
     "flbp_plan_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "flbp_adjoint_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.POINTER(C.c_void_p)]),
+    "flbp_adjoint_destroy": (None, [C.c_void_p]),
+    "flbp_adjoint_forward": (C.c_int, [C.c_void_p, f64p]),
+    "flbp_adjoint_backward": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "flbp_adjoint_last_timing": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "flbp_adjoint_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "flbp_tied_p": (C.c_int, [C.c_uint64, f64p, i32p, C.c_uint64, f64p, f64p]),
+    "flbp_rule_grad": (C.c_int, [C.c_uint64, f64p, f64p, i32p, C.c_uint64, f64p]),
     "flbp_last_error": (C.c_char_p, []),
     "flbp_version": (C.c_char_p, []),
 }

@@ -1,0 +1,272 @@
+/*
+ * adjoint.hip -- checkpointed reverse-mode gradient of the damped PARALL
+ * solve with respect to the factors' probabilities p.
+ *
+ * Forward: init, then n sweeps on the plan's buffers; before sweep c*k+1
+ * the message state (m_vf, m_fv) is copied to checkpoint c.
+ * Backward: for segments c = last..0, recompute sweeps c*k+1..c*k+k from
+ * checkpoint c into per-sweep state slots (out of place), then walk the
+ * segment backwards with var_vjp and fac_vjp (adjoint_kernels.h).  Device
+ * memory: ceil(n/k) checkpoints and k state slots of 32 B per edge each,
+ * three 16 B-per-edge adjoint arrays, and a 16 B-per-edge scratch row when
+ * some variable has more than 8 edges.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "adjoint_kernels.h"
+#include "fastlbp.h"
+#include "flbp_internal.h"
+#include "plan_internal.h"
+
+namespace {
+
+__global__ void fill2(double2 *p, uint64_t n, double c) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double2(c, c);
+}
+
+__global__ void fill1(double *p, uint64_t n, double c) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = c;
+}
+
+}  // namespace
+
+namespace flbp {
+
+struct Adjoint {
+    Plan *P = nullptr;
+    uint64_t T = 0;
+    uint32_t k = 1;
+    uint32_t nck = 0;
+    double damp = 0.0;
+    std::vector<double2 *> ck_vf, ck_fv, st_vf, st_fv;
+    double2 *avf = nullptr, *avf2 = nullptr, *afv = nullptr, *gscratch = nullptr;
+    double *pbar = nullptr, *bbar = nullptr, *scratch_belief = nullptr;
+    std::vector<void *> allocs;
+    uint64_t bytes = 0;
+    bool have_forward = false;
+    hipEvent_t ev[4] = {};
+    double fwd_ms = 0, bwd_ms = 0;
+
+    template <class T_>
+    int alloc(T_ *&p, uint64_t n) {
+        void *q = nullptr;
+        const size_t b = std::max<uint64_t>(n, 1) * sizeof(T_);
+        hipError_t e = hipMalloc(&q, b);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(FLBP_ENOMEM, "adjoint: hipMalloc of " + std::to_string(b) + " bytes failed");
+        }
+        allocs.push_back(q);
+        bytes += b;
+        p = static_cast<T_ *>(q);
+        return FLBP_OK;
+    }
+    ~Adjoint() {
+        if (P) (void)hipSetDevice(P->device);
+        for (void *q : allocs) (void)hipFree(q);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+#define TRY(expr)            \
+    do {                     \
+        int rc_ = (expr);    \
+        if (rc_) return rc_; \
+    } while (0)
+#define HIP_TRY(expr)                                     \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+}  // namespace flbp
+
+using flbp::Adjoint;
+using flbp::Plan;
+using flbp::grid_for;
+using flbp::hip_fail;
+
+extern "C" {
+
+int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_every, double damping,
+                        flbp_adjoint **out) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || !out || n_sweeps == 0 || checkpoint_every == 0)
+        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: bad arguments");
+    if (!(damping >= 0.0 && damping < 1.0))
+        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: damping must be in [0, 1)");
+    if (P->max_fac_deg > (uint64_t)kAdjFacDeg)
+        return flbp::fail(FLBP_ERANGE, "flbp_adjoint_create: factor degree above 8");
+    HIP_TRY(hipSetDevice(P->device));
+    auto *A = new (std::nothrow) Adjoint();
+    if (!A) return flbp::fail(FLBP_ENOMEM, "flbp_adjoint_create: out of memory");
+    A->P = P;
+    A->T = n_sweeps;
+    A->k = (uint32_t)std::min<uint64_t>(checkpoint_every, n_sweeps);
+    A->nck = (uint32_t)((n_sweeps + A->k - 1) / A->k);
+    A->damp = damping;
+    const uint64_t E = P->E;
+    int rc = FLBP_OK;
+    A->ck_vf.resize(A->nck);
+    A->ck_fv.resize(A->nck);
+    A->st_vf.resize(A->k);
+    A->st_fv.resize(A->k);
+    for (uint32_t c = 0; c < A->nck && !rc; c++) {
+        rc = A->alloc(A->ck_vf[c], E);
+        if (!rc) rc = A->alloc(A->ck_fv[c], E);
+    }
+    for (uint32_t i = 0; i < A->k && !rc; i++) {
+        rc = A->alloc(A->st_vf[i], E);
+        if (!rc) rc = A->alloc(A->st_fv[i], E);
+    }
+    if (!rc) rc = A->alloc(A->avf, E);
+    if (!rc) rc = A->alloc(A->avf2, E);
+    if (!rc) rc = A->alloc(A->afv, E);
+    if (!rc) rc = A->alloc(A->gscratch, P->max_var_deg > (uint64_t)kAdjVarDeg ? E : 1);
+    if (!rc) rc = A->alloc(A->pbar, P->F);
+    if (!rc) rc = A->alloc(A->bbar, P->V);
+    if (!rc) rc = A->alloc(A->scratch_belief, P->V);
+    for (int i = 0; i < 4 && !rc; i++)
+        if (hipEventCreate(&A->ev[i]) != hipSuccess) rc = flbp::fail(FLBP_EHIP, "hipEventCreate failed");
+    if (rc) {
+        delete A;
+        return rc;
+    }
+    *out = reinterpret_cast<flbp_adjoint *>(A);
+    return FLBP_OK;
+}
+
+void flbp_adjoint_destroy(flbp_adjoint *adj) { delete reinterpret_cast<Adjoint *>(adj); }
+
+int flbp_adjoint_forward(flbp_adjoint *adj, double *beliefs) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    if (!A) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_forward: null adjoint");
+    Plan *P = A->P;
+    HIP_TRY(hipSetDevice(P->device));
+    const uint64_t E = P->E;
+    hipStream_t st = P->stream;
+    HIP_TRY(hipEventRecord(A->ev[0], st));
+    if (E) {
+        hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, P->m_vf, E, 1.0);
+        hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, P->m_fv, E, 1.0);
+    }
+    for (uint64_t t = 1; t <= A->T; t++) {
+        if ((t - 1) % A->k == 0 && E) {
+            const uint64_t c = (t - 1) / A->k;
+            HIP_TRY(hipMemcpyAsync(A->ck_vf[c], P->m_vf, E * sizeof(double2), hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(A->ck_fv[c], P->m_fv, E * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        }
+        TRY(P->launch_sweep_on(0, -INFINITY, A->damp, P->m_vf, P->m_fv, P->m_fv, P->m_vf, P->belief));
+    }
+    HIP_TRY(hipEventRecord(A->ev[1], st));
+    if (beliefs && P->V)
+        HIP_TRY(hipMemcpyAsync(beliefs, P->belief, P->V * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, A->ev[0], A->ev[1]));
+    A->fwd_ms = ms;
+    A->have_forward = true;
+    P->swept_since_init = true;
+    return FLBP_OK;
+}
+
+int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    if (!A || !dl_db || !dl_dp) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: null argument");
+    if (!A->have_forward) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: run the forward first");
+    Plan *P = A->P;
+    HIP_TRY(hipSetDevice(P->device));
+    const uint64_t E = P->E, V = P->V, F = P->F;
+    hipStream_t st = P->stream;
+    if (V) HIP_TRY(hipMemcpyAsync(A->bbar, dl_db, V * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(A->ev[2], st));
+    if (E) {
+        hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, A->avf, E, 0.0);
+        hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, A->afv, E, 0.0);
+    }
+    if (F) hipLaunchKernelGGL(fill1, dim3(grid_for(F, 256)), dim3(256), 0, st, A->pbar, F, 0.0);
+    const double keep = 1.0 - A->damp;
+    for (int64_t c = (int64_t)A->nck - 1; c >= 0; c--) {
+        const uint64_t t0 = (uint64_t)c * A->k, t1 = std::min<uint64_t>(t0 + A->k, A->T);
+        auto vf_state = [&](uint64_t t) -> double2 * {   /* m_vf after t sweeps, t0 <= t <= t1 */
+            return t == t0 ? A->ck_vf[c] : A->st_vf[t - t0 - 1];
+        };
+        auto fv_state = [&](uint64_t t) -> double2 * {
+            return t == t0 ? A->ck_fv[c] : A->st_fv[t - t0 - 1];
+        };
+        for (uint64_t t = t0 + 1; t <= t1; t++)
+            TRY(P->launch_sweep_on(0, -INFINITY, A->damp, vf_state(t - 1), fv_state(t - 1), fv_state(t),
+                                   vf_state(t), A->scratch_belief));
+        for (uint64_t t = t1; t > t0; t--) {
+            if (V)
+                hipLaunchKernelGGL(var_vjp, dim3(grid_for(V, kBlock)), dim3(kBlock), 0, st, P->var_off,
+                                   P->var_edge, fv_state(t), A->avf, t == A->T ? A->bbar : nullptr, A->afv,
+                                   A->gscratch, A->damp, (uint32_t)V);
+            if (F)
+                hipLaunchKernelGGL(fac_vjp, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, st, P->fac_off,
+                                   P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, vf_state(t - 1), A->afv,
+                                   A->avf2, A->pbar, keep, (uint32_t)F);
+            std::swap(A->avf, A->avf2);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(A->ev[3], st));
+    if (F) HIP_TRY(hipMemcpyAsync(dl_dp, A->pbar, F * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, A->ev[2], A->ev[3]));
+    A->bwd_ms = ms;
+    return FLBP_OK;
+}
+
+int flbp_adjoint_last_timing(flbp_adjoint *adj, double *forward_ms, double *backward_ms) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    if (!A) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_last_timing: null adjoint");
+    if (forward_ms) *forward_ms = A->fwd_ms;
+    if (backward_ms) *backward_ms = A->bwd_ms;
+    return FLBP_OK;
+}
+
+uint64_t flbp_adjoint_device_bytes(const flbp_adjoint *adj) {
+    const Adjoint *A = reinterpret_cast<const Adjoint *>(adj);
+    return A ? A->bytes : 0;
+}
+
+int flbp_tied_p(uint64_t n_factors, const double *base_p, const int32_t *rule, uint64_t n_rules,
+                const double *theta, double *p_out) {
+    if (n_factors && (!base_p || !rule || !p_out)) return flbp::fail(FLBP_EINVAL, "flbp_tied_p: null argument");
+    for (uint64_t I = 0; I < n_factors; I++) {
+        const int32_t r = rule[I];
+        if (r < 0) {
+            p_out[I] = base_p[I];
+        } else {
+            if ((uint64_t)r >= n_rules || !theta) return flbp::fail(FLBP_EINVAL, "flbp_tied_p: rule id out of range");
+            p_out[I] = 1.0 / (1.0 + std::exp(-theta[r]));
+        }
+    }
+    return FLBP_OK;
+}
+
+int flbp_rule_grad(uint64_t n_factors, const double *dl_dp, const double *p, const int32_t *rule,
+                   uint64_t n_rules, double *dl_dtheta) {
+    if ((n_factors && (!dl_dp || !p || !rule)) || (n_rules && !dl_dtheta))
+        return flbp::fail(FLBP_EINVAL, "flbp_rule_grad: null argument");
+    for (uint64_t r = 0; r < n_rules; r++) dl_dtheta[r] = 0.0;
+    for (uint64_t I = 0; I < n_factors; I++) {
+        const int32_t r = rule[I];
+        if (r < 0) continue;
+        if ((uint64_t)r >= n_rules) return flbp::fail(FLBP_EINVAL, "flbp_rule_grad: rule id out of range");
+        dl_dtheta[r] += dl_dp[I] * p[I] * (1.0 - p[I]);
+    }
+    return FLBP_OK;
+}
+
+}  // extern "C"

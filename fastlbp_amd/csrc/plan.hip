@@ -37,121 +37,6 @@ using namespace flbp::dev;
 
 namespace {
 
-/* one lane per factor; damping fused into the store */
-template <bool kDamp>
-__global__ __launch_bounds__(kBlock) void factor_pass(
-    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
-    const double *__restrict__ fac_p, const double *__restrict__ fac_q,
-    const double2 *__restrict__ fac_mask, const double2 *__restrict__ m_vf,
-    double2 *__restrict__ m_fv, uint32_t F, double damp, double keep,
-    const unsigned long long *__restrict__ ctl, int k, double tol) {
-    if (chunk_stopped(ctl, k, tol)) return;
-    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
-    if (f >= F) return;
-    const uint32_t s = fac_off[f];
-    const int d = (int)(fac_off[f + 1] - s);
-    const uint32_t meta = fac_meta[f];
-    const uint32_t type = meta & 3u;
-    const bool clamped = (meta & 4u) != 0;
-    const int h = (int)(meta >> 8);
-    const double p = fac_p[f];
-    const double q = type ? fac_q[f] : 0.0;
-    double mask0 = 1.0, mask1 = 1.0;
-    if (clamped) {
-        const double2 mk = fac_mask[f];
-        mask0 = mk.x;
-        mask1 = mk.y;
-    }
-    auto store = [&](int t, double2 r) __attribute__((always_inline)) {
-        if (kDamp) {
-            const double2 old = m_fv[s + t];
-            r.x = keep * r.x + damp * old.x;
-            r.y = keep * r.y + damp * old.y;
-        }
-        m_fv[s + t] = r;
-    };
-    if (d <= kFacRegDeg) {
-        RegIn<kFacRegDeg> in;
-#pragma unroll
-        for (int j = 0; j < kFacRegDeg; j++) in.m[j] = j < d ? m_vf[s + j] : make_double2(0.0, 0.0);
-        const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
-        for (int t = 0; t < d; t++)
-            store(t, fv_msg<kFacRegDeg>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
-    } else {
-        GlobalIn in{m_vf + s};
-        const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
-        for (int t = 0; t < d; t++)
-            store(t, fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
-    }
-}
-
-/* one lane per variable: product, belief, max-diff, leave-one-out scatter */
-__global__ __launch_bounds__(kBlock) void var_pass(
-    const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
-    const double2 *__restrict__ m_fv, double2 *__restrict__ m_vf, double *__restrict__ belief,
-    uint32_t V, unsigned long long *__restrict__ ctl, int k, double tol) {
-    if (chunk_stopped(ctl, k, tol)) return;
-    __shared__ unsigned long long red[kBlock / 64];
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    unsigned long long diff_bits = 0;
-    if (v < V) {
-        const uint32_t s = var_off[v];
-        const int d = (int)(var_off[v + 1] - s);
-        double prod_0 = 1.0, prod_1 = 1.0;
-        uint32_t z0 = 0, z1 = 0;
-        uint32_t slot[kVarRegDeg];
-        double2 msg[kVarRegDeg];
-        auto accumulate = [&](double2 m) __attribute__((always_inline)) {
-            if (m.x == 0) z0++; else prod_0 *= m.x;
-            if (m.y == 0) z1++; else prod_1 *= m.y;
-            scale_pair(prod_0, prod_1);
-        };
-        if (d <= kVarRegDeg) {
-#pragma unroll
-            for (int j = 0; j < kVarRegDeg; j++) {
-                if (j < d) {
-                    slot[j] = var_edge[s + j];
-                    msg[j] = m_fv[slot[j]];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < kVarRegDeg; j++)
-                if (j < d) accumulate(msg[j]);
-        } else {
-            for (int j = 0; j < d; j++) accumulate(m_fv[var_edge[s + j]]);
-        }
-        /* getBeliefsVKernel + norm_functor + dist_functor */
-        const double b0 = prod_0 * (double)(z0 == 0);
-        const double b1 = prod_1 * (double)(z1 == 0);
-        const double b = b1 / (b0 + b1);
-        const double diff = fabs(belief[v] - b);
-        belief[v] = b;
-        diff_bits = (unsigned long long)__double_as_longlong(diff);
-        if (d <= kVarRegDeg) {
-#pragma unroll
-            for (int j = 0; j < kVarRegDeg; j++)
-                if (j < d) m_vf[slot[j]] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
-        } else {
-            for (int j = 0; j < d; j++) {
-                const uint32_t sl = var_edge[s + j];
-                m_vf[sl] = vf_msg(prod_0, prod_1, z0, z1, m_fv[sl]);
-            }
-        }
-    }
-    /* max over non-negative doubles == max over their bit patterns; a NaN
-     * (positive after fabs) wins, which stops the loop as NaN > tol is false */
-    diff_bits = wave_max_u64(diff_bits);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = diff_bits;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long m = red[0];
-#pragma unroll
-        for (int w = 1; w < kBlock / 64; w++) m = red[w] > m ? red[w] : m;
-        if (m) atomicMax(&ctl[k], m);
-    }
-}
-
 /* zero the per-sweep slots of a chunk (a kernel, so the sequence can be
  * captured in a graph) */
 __global__ void clear_u64(unsigned long long *p, int n) {
@@ -170,8 +55,6 @@ __global__ void fill_f64(double *p, uint64_t n, double c) {
     if (i < n) p[i] = c;
 }
 
-inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
-
 }  // namespace
 
 namespace flbp {
@@ -187,21 +70,29 @@ int hip_fail(hipError_t e, const char *what) {
     } while (0)
 
 int Plan::launch_sweep(int k, double tol, double damping) {
+    return launch_sweep_on(k, tol, damping, m_vf, m_fv, m_fv, m_vf, belief);
+}
+
+/* One sweep reading v->f messages from vf_in and the previous f->v
+ * messages from fv_old, writing f->v messages to fv_out and v->f messages
+ * to vf_out (the plain solve passes the same arrays in and out). */
+int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_in,
+                          const double2 *fv_old, double2 *fv_out, double2 *vf_out, double *bel) {
     const double keep = 1.0 - damping;
     if (F) {
         if (damping != 0.0)
             hipLaunchKernelGGL(factor_pass<true>, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, stream,
-                               fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv, (uint32_t)F,
-                               damping, keep, ctl, k, tol);
+                               fac_off, fac_meta, fac_p, fac_q, fac_mask, vf_in, fv_old, fv_out,
+                               (uint32_t)F, damping, keep, ctl, k, tol);
         else
             hipLaunchKernelGGL(factor_pass<false>, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, stream,
-                               fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv, (uint32_t)F,
-                               damping, keep, ctl, k, tol);
+                               fac_off, fac_meta, fac_p, fac_q, fac_mask, vf_in, fv_old, fv_out,
+                               (uint32_t)F, damping, keep, ctl, k, tol);
     }
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
     if (V)
         hipLaunchKernelGGL(var_pass, dim3(grid_for(V, kBlock)), dim3(kBlock), 0, stream, var_off,
-                           var_edge, m_fv, m_vf, belief, (uint32_t)V, ctl, k, tol);
+                           var_edge, fv_out, vf_out, bel, (uint32_t)V, ctl, k, tol);
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 2], stream));
     HIP_TRY(hipGetLastError());
     return FLBP_OK;
@@ -255,6 +146,7 @@ Plan::~Plan() {
 
 using flbp::Plan;
 using flbp::hip_fail;
+using flbp::grid_for;
 
 extern "C" {
 

@@ -112,6 +112,8 @@ int Plan::upload(const flbp_csr &c) {
     HIP_TRY(hipMemset(m_vf, 0, std::max<uint64_t>(E, 1) * 16));
     HIP_TRY(hipMemset(m_fv, 0, std::max<uint64_t>(E, 1) * 16));
     h_var_edge.assign(c.var_edge, c.var_edge + E);
+    for (uint64_t I = 0; I < F; I++) max_fac_deg = std::max<uint64_t>(max_fac_deg, c.fac_off[I + 1] - c.fac_off[I]);
+    for (uint64_t i = 0; i < V; i++) max_var_deg = std::max<uint64_t>(max_var_deg, c.var_off[i + 1] - c.var_off[i]);
     return FLBP_OK;
 }
 

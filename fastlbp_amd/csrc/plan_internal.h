@@ -15,6 +15,8 @@ namespace flbp {
 
 constexpr int kMaxChunk = 64;   /* sweeps enqueued between host checks */
 
+inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
 int hip_fail(hipError_t e, const char *what);
 int validate_csr(const flbp_csr &c);
 
@@ -30,6 +32,7 @@ struct Plan {
     unsigned long long *host_ctl = nullptr;        /* pinned */
     std::vector<void *> allocs;
     uint64_t device_bytes = 0;
+    uint64_t max_fac_deg = 0, max_var_deg = 0;
 
     std::vector<uint64_t> h_var_edge;              /* for message export/import */
 
@@ -49,6 +52,8 @@ struct Plan {
     }
     int upload(const flbp_csr &c);
     int launch_sweep(int k, double tol, double damping);
+    int launch_sweep_on(int k, double tol, double damping, const double2 *vf_in,
+                        const double2 *fv_old, double2 *fv_out, double2 *vf_out, double *bel);
     int run_chunk(int n, double tol, double damping, int &ran, double &last_diff);
     ~Plan();
 

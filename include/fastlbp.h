@@ -205,6 +205,41 @@ int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
 
 uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
 
+/* ------------------------------------------------------------------ */
+/* Adjoint (not in the reference)                                      */
+/* ------------------------------------------------------------------ */
+
+/* Reverse-mode gradient through n_sweeps damped PARALL sweeps from init:
+ * the forward keeps the message state every checkpoint_every sweeps, the
+ * backward recomputes each segment from its checkpoint and runs the
+ * hand-written reverse kernels of the factor and variable passes.  The
+ * adjoint works on its plan's device buffers (the forward overwrites the
+ * plan's messages and beliefs).  Factors of degree above 8 are rejected
+ * (FLBP_ERANGE). */
+typedef struct flbp_adjoint flbp_adjoint;
+
+int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_every,
+                        double damping, flbp_adjoint **out);
+void flbp_adjoint_destroy(flbp_adjoint *adj);
+/* init + n_sweeps sweeps with the plan's current p; beliefs[V] (NULL skips) */
+int flbp_adjoint_forward(flbp_adjoint *adj, double *beliefs);
+/* dl_db[V] = dL/d(belief) of every variable -> dl_dp[F] = dL/dp of every
+ * factor, for the last forward */
+int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp);
+/* device time of the last forward and backward, milliseconds */
+int flbp_adjoint_last_timing(flbp_adjoint *adj, double *forward_ms, double *backward_ms);
+uint64_t flbp_adjoint_device_bytes(const flbp_adjoint *adj);
+
+/* p_out[I] = sigmoid(theta[rule_of_factor[I]]) = 1 / (1 + exp(-theta)) for
+ * rule >= 0, else base_p[I] */
+int flbp_tied_p(uint64_t n_factors, const double *base_p, const int32_t *rule_of_factor,
+                uint64_t n_rules, const double *theta, double *p_out);
+
+/* dL/dtheta[r] = sum over factors I with rule_of_factor[I] == r of
+ * dl_dp[I] * p[I] * (1 - p[I])  (p = sigmoid(theta[rule])), in factor order */
+int flbp_rule_grad(uint64_t n_factors, const double *dl_dp, const double *p,
+                   const int32_t *rule_of_factor, uint64_t n_rules, double *dl_dtheta);
+
 const char *flbp_last_error(void);
 const char *flbp_version(void);
 
