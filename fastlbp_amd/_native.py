@@ -83,6 +83,16 @@ class RunOptsC(C.Structure):
     ]
 
 
+class BatchOptsC(C.Structure):
+    _fields_ = [
+        ("tile_factors", C.c_uint32),
+        ("tile_vars", C.c_uint32),
+        ("grid_blocks", C.c_uint32),
+        ("chunk", C.c_uint32),
+        ("use_graph", C.c_uint32),
+    ]
+
+
 class RunResultC(C.Structure):
     _fields_ = [
         ("iters", C.c_uint64),
@@ -120,6 +130,18 @@ SIGNATURES = {
     "flbp_plan_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_plan_last_timing": (C.c_int, [C.c_void_p, f64p, f64p, u64p, u64p]),
     "flbp_plan_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "flbp_batch_default_opts": (None, [C.POINTER(BatchOptsC)]),
+    "flbp_batch_create": (C.c_int, [C.c_uint64, C.POINTER(C.POINTER(CsrC)), C.c_int, C.POINTER(BatchOptsC),
+                                    C.POINTER(C.c_void_p)]),
+    "flbp_batch_destroy": (None, [C.c_void_p]),
+    "flbp_batch_init": (C.c_int, [C.c_void_p, C.c_int]),
+    "flbp_batch_run": (C.c_int, [C.c_void_p, C.POINTER(RunOptsC), u64p, f64p, C.POINTER(RunResultC)]),
+    "flbp_batch_beliefs": (C.c_int, [C.c_void_p, C.c_uint64, f64p, C.c_uint64]),
+    "flbp_batch_set_opts": (C.c_int, [C.c_void_p, C.POINTER(BatchOptsC)]),
+    "flbp_batch_get_opts": (C.c_int, [C.c_void_p, C.POINTER(BatchOptsC)]),
+    "flbp_batch_reset_counters": (C.c_int, [C.c_void_p]),
+    "flbp_batch_autotune": (C.c_int, [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(BatchOptsC)]),
+    "flbp_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "flbp_adjoint_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.POINTER(C.c_void_p)]),
     "flbp_adjoint_destroy": (None, [C.c_void_p]),
     "flbp_adjoint_forward": (C.c_int, [C.c_void_p, f64p]),

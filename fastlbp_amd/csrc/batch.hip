@@ -1,0 +1,481 @@
+/*
+ * batch.hip -- batched PARALL solve of many graphs (independent graphs or
+ * clamped variants of one graph) in one device state.
+ *
+ * Layout: every per-factor / per-variable / per-edge array of all graphs
+ * concatenated (graph bases in BatchGraph, 64-bit; indices inside a graph
+ * 32-bit).  A chunk of sweeps -- one clear kernel and, per sweep, one
+ * persistent factor kernel and one persistent variable kernel over the
+ * tile list -- is captured once into a hipGraph and replayed; per-graph
+ * stop flags live in device memory, so one replay advances every unfinished
+ * graph by up to `chunk` sweeps with no host round trip.  Tile sizes and
+ * grid size are chosen by flbp_batch_autotune.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batch_kernels.h"
+#include "fastlbp.h"
+#include "flbp_internal.h"
+#include "plan_internal.h"
+
+namespace flbp {
+
+struct Batch {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    uint64_t G = 0, Ftot = 0, Vtot = 0, Etot = 0;
+    std::vector<BatchGraph> h_graphs;
+    BatchGraph *graphs = nullptr;
+    uint32_t *fac_off = nullptr, *fac_meta = nullptr, *var_off = nullptr, *var_edge = nullptr;
+    double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
+    double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
+    Tile *ftiles = nullptr, *vtiles = nullptr;
+    uint32_t n_ftiles = 0, n_vtiles = 0;
+    unsigned long long *diff = nullptr;     /* [kMaxChunk][G] */
+    uint8_t *start_active = nullptr;        /* device [G] */
+    uint32_t *budget = nullptr;             /* device [G] */
+    unsigned long long *h_diff = nullptr;   /* pinned */
+    uint8_t *h_active = nullptr;            /* pinned */
+    uint32_t *h_budget = nullptr;           /* pinned */
+    std::vector<uint64_t> iters;
+    std::vector<double> last_diff;
+    std::vector<void *> allocs;
+    uint64_t bytes = 0;
+    flbp_batch_opts opts{};
+    /* captured chunk */
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int exec_chunk = 0;
+    double exec_damp = -1, exec_tol = 0;
+    uint32_t exec_fgrid = 0, exec_vgrid = 0;
+
+    template <class T>
+    int alloc(T *&p, uint64_t n) {
+        void *q = nullptr;
+        const size_t b = std::max<uint64_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&q, b);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(FLBP_ENOMEM, "batch: hipMalloc of " + std::to_string(b) + " bytes failed");
+        }
+        allocs.push_back(q);
+        bytes += b;
+        p = static_cast<T *>(q);
+        return FLBP_OK;
+    }
+    void drop_exec() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~Batch() {
+        if (device >= 0) (void)hipSetDevice(device);
+        drop_exec();
+        for (void *q : allocs) (void)hipFree(q);
+        if (h_diff) (void)hipHostFree(h_diff);
+        if (h_active) (void)hipHostFree(h_active);
+        if (h_budget) (void)hipHostFree(h_budget);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int build_tiles();
+    int record_chunk(int n, double tol, double damping);
+    int enqueue_chunk(int n, double tol, double damping);
+};
+
+#define TRY(expr)            \
+    do {                     \
+        int rc_ = (expr);    \
+        if (rc_) return rc_; \
+    } while (0)
+#define HIP_TRY(expr)                                     \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+int Batch::build_tiles() {
+    /* a tile covers at least one block's worth of lanes */
+    const uint32_t tf = std::max<uint32_t>(opts.tile_factors, kBlock), tv = std::max<uint32_t>(opts.tile_vars, kBlock);
+    std::vector<Tile> ft, vt;
+    for (uint32_t g = 0; g < G; g++) {
+        for (uint32_t a = 0; a < h_graphs[g].n_fac; a += tf)
+            ft.push_back({g, a, std::min(tf, h_graphs[g].n_fac - a)});
+        for (uint32_t a = 0; a < h_graphs[g].n_var; a += tv)
+            vt.push_back({g, a, std::min(tv, h_graphs[g].n_var - a)});
+    }
+    if (ft.size() >= (1ull << 31) || vt.size() >= (1ull << 31)) return fail(FLBP_ERANGE, "batch: too many tiles");
+    n_ftiles = (uint32_t)ft.size();
+    n_vtiles = (uint32_t)vt.size();
+    /* tiles are re-cut by the autotuner: the arrays are sized once for the
+     * smallest tile (kBlock items) */
+    if (!ftiles) {
+        TRY(alloc(ftiles, Ftot / kBlock + G + 1));
+        TRY(alloc(vtiles, Vtot / kBlock + G + 1));
+    }
+    if (n_ftiles) HIP_TRY(hipMemcpy(ftiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (n_vtiles) HIP_TRY(hipMemcpy(vtiles, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    drop_exec();
+    return FLBP_OK;
+}
+
+int Batch::record_chunk(int n, double tol, double damping) {
+    const double keep = 1.0 - damping;
+    BatchCtl c{diff, start_active, budget, (uint32_t)G, tol};
+    hipLaunchKernelGGL(batch_clear, dim3(std::max<uint64_t>(1, std::min<uint64_t>(1024, (n * G + 255) / 256))),
+                       dim3(256), 0, stream, diff, (uint64_t)n * G);
+    const uint32_t fg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_ftiles));
+    const uint32_t vg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_vtiles));
+    for (int k = 0; k < n; k++) {
+        if (n_ftiles) {
+            if (damping != 0.0)
+                hipLaunchKernelGGL(batch_factor_pass<true>, dim3(fg), dim3(kBlock), 0, stream, graphs, ftiles,
+                                   n_ftiles, fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv, damping,
+                                   keep, c, k);
+            else
+                hipLaunchKernelGGL(batch_factor_pass<false>, dim3(fg), dim3(kBlock), 0, stream, graphs, ftiles,
+                                   n_ftiles, fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv, damping,
+                                   keep, c, k);
+        }
+        if (n_vtiles)
+            hipLaunchKernelGGL(batch_var_pass, dim3(vg), dim3(kBlock), 0, stream, graphs, vtiles, n_vtiles,
+                               var_off, var_edge, m_fv, m_vf, belief, c, k);
+    }
+    return FLBP_OK;
+}
+
+int Batch::enqueue_chunk(int n, double tol, double damping) {
+    if (!opts.use_graph) {
+        TRY(record_chunk(n, tol, damping));
+        HIP_TRY(hipGetLastError());
+        return FLBP_OK;
+    }
+    if (!exec || exec_chunk != n || exec_damp != damping || exec_tol != tol) {
+        drop_exec();
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = record_chunk(n, tol, damping);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        graph = g;
+        HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        exec_chunk = n;
+        exec_damp = damping;
+        exec_tol = tol;
+    }
+    HIP_TRY(hipGraphLaunch(exec, stream));
+    return FLBP_OK;
+}
+
+}  // namespace flbp
+
+using flbp::Batch;
+using flbp::grid_for;
+using flbp::hip_fail;
+
+namespace {
+
+__global__ void bfill2(double2 *p, uint64_t n, double c) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = make_double2(c, c);
+}
+
+int default_grid(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_factor_pass<true>, kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    return prop.multiProcessorCount * per_cu;
+}
+
+}  // namespace
+
+extern "C" {
+
+void flbp_batch_default_opts(flbp_batch_opts *o) {
+    if (!o) return;
+    o->tile_factors = 1024;
+    o->tile_vars = 1024;
+    o->grid_blocks = 0;
+    o->chunk = 32;
+    o->use_graph = 1;
+}
+
+int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device, const flbp_batch_opts *opts,
+                      flbp_batch **out) {
+    if (!csrs || !out || n_graphs == 0) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: bad arguments");
+    if (n_graphs >= (1ull << 31)) return flbp::fail(FLBP_ERANGE, "flbp_batch_create: too many graphs");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return flbp::fail(FLBP_ENODEV, "flbp_batch_create: no HIP device");
+    if (device < 0 || device >= ndev) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: bad device");
+    for (uint64_t g = 0; g < n_graphs; g++) {
+        if (!csrs[g]) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: null graph");
+        const flbp_csr &c = *csrs[g];
+        if (c.n_vars >= (1ull << 32) - 1 || c.n_factors >= (1ull << 32) - 1 || c.n_edges >= (1ull << 32) - 1)
+            return flbp::fail(FLBP_ERANGE, "flbp_batch_create: a graph has V, F or E >= 2^32-1");
+        int rc = flbp::validate_csr(c);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipSetDevice(device));
+    auto *B = new (std::nothrow) Batch();
+    if (!B) return flbp::fail(FLBP_ENOMEM, "flbp_batch_create: out of memory");
+    std::unique_ptr<Batch> hold(B);
+    B->device = device;
+    B->G = n_graphs;
+    if (opts) B->opts = *opts;
+    else flbp_batch_default_opts(&B->opts);
+    if (B->opts.chunk == 0 || B->opts.chunk > (uint32_t)flbp::kMaxChunk) B->opts.chunk = flbp::kMaxChunk;
+    if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(device);
+    B->h_graphs.resize(n_graphs);
+    uint64_t F = 0, V = 0, E = 0;
+    for (uint64_t g = 0; g < n_graphs; g++) {
+        const flbp_csr &c = *csrs[g];
+        B->h_graphs[g] = {F, V, E, F + g, V + g, (uint32_t)c.n_factors, (uint32_t)c.n_vars};
+        F += c.n_factors;
+        V += c.n_vars;
+        E += c.n_edges;
+    }
+    B->Ftot = F;
+    B->Vtot = V;
+    B->Etot = E;
+    HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    TRY(B->alloc(B->graphs, n_graphs));
+    TRY(B->alloc(B->fac_off, F + n_graphs));
+    TRY(B->alloc(B->fac_meta, F));
+    TRY(B->alloc(B->fac_p, F));
+    TRY(B->alloc(B->fac_q, F));
+    TRY(B->alloc(B->fac_mask, F));
+    TRY(B->alloc(B->var_off, V + n_graphs));
+    TRY(B->alloc(B->var_edge, E));
+    TRY(B->alloc(B->m_vf, E));
+    TRY(B->alloc(B->m_fv, E));
+    TRY(B->alloc(B->belief, V));
+    TRY(B->alloc(B->diff, (uint64_t)flbp::kMaxChunk * n_graphs));
+    TRY(B->alloc(B->start_active, n_graphs));
+    TRY(B->alloc(B->budget, n_graphs));
+    HIP_TRY(hipHostMalloc((void **)&B->h_diff, sizeof(unsigned long long) * flbp::kMaxChunk * n_graphs, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&B->h_active, n_graphs, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&B->h_budget, sizeof(uint32_t) * n_graphs, hipHostMallocDefault));
+    HIP_TRY(hipMemcpy(B->graphs, B->h_graphs.data(), n_graphs * sizeof(BatchGraph), hipMemcpyHostToDevice));
+    /* upload graph by graph through one staging vector */
+    std::vector<uint32_t> u32;
+    std::vector<double2> mk;
+    for (uint64_t g = 0; g < n_graphs; g++) {
+        const flbp_csr &c = *csrs[g];
+        const BatchGraph &bg = B->h_graphs[g];
+        u32.resize(std::max(std::max(c.n_factors, c.n_vars), c.n_edges) + 1);
+        for (uint64_t I = 0; I <= c.n_factors; I++) u32[I] = (uint32_t)c.fac_off[I];
+        HIP_TRY(hipMemcpy(B->fac_off + bg.fac_off_base, u32.data(), (c.n_factors + 1) * 4, hipMemcpyHostToDevice));
+        for (uint64_t I = 0; I < c.n_factors; I++)
+            u32[I] = flbp::dev::pack_meta((uint32_t)c.fac_type[I], c.head_clamped[I] != 0,
+                                          (uint32_t)(c.head_slot[I] - c.fac_off[I]));
+        if (c.n_factors) {
+            HIP_TRY(hipMemcpy(B->fac_meta + bg.fac_base, u32.data(), c.n_factors * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(B->fac_p + bg.fac_base, c.p, c.n_factors * 8, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(B->fac_q + bg.fac_base, c.q, c.n_factors * 8, hipMemcpyHostToDevice));
+            mk.resize(c.n_factors);
+            for (uint64_t I = 0; I < c.n_factors; I++) mk[I] = make_double2(c.mask0[I], c.mask1[I]);
+            HIP_TRY(hipMemcpy(B->fac_mask + bg.fac_base, mk.data(), c.n_factors * 16, hipMemcpyHostToDevice));
+        }
+        for (uint64_t i = 0; i <= c.n_vars; i++) u32[i] = (uint32_t)c.var_off[i];
+        HIP_TRY(hipMemcpy(B->var_off + bg.var_off_base, u32.data(), (c.n_vars + 1) * 4, hipMemcpyHostToDevice));
+        for (uint64_t k = 0; k < c.n_edges; k++) u32[k] = (uint32_t)c.var_edge[k];
+        if (c.n_edges)
+            HIP_TRY(hipMemcpy(B->var_edge + bg.edge_base, u32.data(), c.n_edges * 4, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemset(B->belief, 0, std::max<uint64_t>(V, 1) * 8));
+    HIP_TRY(hipMemset(B->m_vf, 0, std::max<uint64_t>(E, 1) * 16));
+    HIP_TRY(hipMemset(B->m_fv, 0, std::max<uint64_t>(E, 1) * 16));
+    B->iters.assign(n_graphs, 0);
+    B->last_diff.assign(n_graphs, INFINITY);
+    TRY(B->build_tiles());
+    *out = reinterpret_cast<flbp_batch *>(hold.release());
+    return FLBP_OK;
+}
+
+void flbp_batch_destroy(flbp_batch *b) { delete reinterpret_cast<Batch *>(b); }
+
+int flbp_batch_init(flbp_batch *b, int logdomain) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B) return flbp::fail(FLBP_EINVAL, "flbp_batch_init: null batch");
+    HIP_TRY(hipSetDevice(B->device));
+    const double c = logdomain ? 0.0 : 1.0;
+    if (B->Etot) {
+        const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(B->Etot, 256), 65536);
+        hipLaunchKernelGGL(bfill2, dim3(grid), dim3(256), 0, B->stream, B->m_vf, B->Etot, c);
+        hipLaunchKernelGGL(bfill2, dim3(grid), dim3(256), 0, B->stream, B->m_fv, B->Etot, c);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    return FLBP_OK;
+}
+
+int flbp_batch_run(flbp_batch *b, const flbp_run_opts *o, uint64_t *iters_out, double *maxdiff_out,
+                   flbp_run_result *total) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B || !o) return flbp::fail(FLBP_EINVAL, "flbp_batch_run: null argument");
+    if (!(o->damping >= 0.0 && o->damping < 1.0))
+        return flbp::fail(FLBP_EINVAL, "flbp_batch_run: damping must be in [0, 1)");
+    HIP_TRY(hipSetDevice(B->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    const uint64_t G = B->G;
+    std::vector<uint8_t> active(G);
+    std::vector<double> md(G, INFINITY);
+    uint64_t n_active = 0, sweeps = 0;
+    for (uint64_t g = 0; g < G; g++) {
+        active[g] = B->iters[g] < o->maxiter;
+        n_active += active[g];
+    }
+    const int chunk = (int)std::min<uint32_t>(o->chunk ? o->chunk : B->opts.chunk, flbp::kMaxChunk);
+    while (n_active && elapsed() < o->maxtime) {
+        for (uint64_t g = 0; g < G; g++) {
+            B->h_active[g] = active[g];
+            const uint64_t left = o->maxiter > B->iters[g] ? o->maxiter - B->iters[g] : 0;
+            B->h_budget[g] = (uint32_t)std::min<uint64_t>(left, 0xffffffffu);
+        }
+        HIP_TRY(hipMemcpyAsync(B->start_active, B->h_active, G, hipMemcpyHostToDevice, B->stream));
+        HIP_TRY(hipMemcpyAsync(B->budget, B->h_budget, G * 4, hipMemcpyHostToDevice, B->stream));
+        TRY(B->enqueue_chunk(chunk, o->tol, o->damping));
+        HIP_TRY(hipMemcpyAsync(B->h_diff, B->diff, sizeof(unsigned long long) * chunk * G, hipMemcpyDeviceToHost,
+                               B->stream));
+        HIP_TRY(hipStreamSynchronize(B->stream));
+        n_active = 0;
+        uint64_t most = 0;
+        for (uint64_t g = 0; g < G; g++) {
+            if (!active[g]) continue;
+            const uint64_t cap = std::min<uint64_t>((uint64_t)chunk, B->h_budget[g]);
+            uint64_t ran = cap;
+            bool stop = false;
+            for (uint64_t k = 0; k < cap; k++) {
+                double dk;
+                std::memcpy(&dk, &B->h_diff[k * G + g], sizeof dk);
+                md[g] = dk;
+                if (!(dk > o->tol)) {
+                    ran = k + 1;
+                    stop = true;
+                    break;
+                }
+            }
+            B->iters[g] += ran;
+            most = std::max(most, ran);
+            if (stop || B->iters[g] >= o->maxiter) active[g] = 0;
+            n_active += active[g];
+        }
+        sweeps += most;
+    }
+    for (uint64_t g = 0; g < G; g++) {
+        B->last_diff[g] = md[g];
+        if (iters_out) iters_out[g] = B->iters[g];
+        if (maxdiff_out) maxdiff_out[g] = md[g];
+    }
+    if (total) {
+        total->iters = sweeps;
+        total->sweeps = sweeps;
+        total->maxdiff = 0;
+        for (uint64_t g = 0; g < G; g++) total->maxdiff = std::max(total->maxdiff, md[g]);
+        total->seconds = elapsed();
+    }
+    return FLBP_OK;
+}
+
+int flbp_batch_beliefs(flbp_batch *b, uint64_t g, double *out, uint64_t n) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B || (n && !out) || g >= (B ? B->G : 0)) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs: bad argument");
+    if (n > B->h_graphs[g].n_var) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs: n > n_vars of the graph");
+    HIP_TRY(hipSetDevice(B->device));
+    if (n)
+        HIP_TRY(hipMemcpyAsync(out, B->belief + B->h_graphs[g].var_base, n * 8, hipMemcpyDeviceToHost, B->stream));
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    return FLBP_OK;
+}
+
+int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *o) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B || !o) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: null argument");
+    HIP_TRY(hipSetDevice(B->device));
+    B->opts = *o;
+    if (B->opts.chunk == 0 || B->opts.chunk > (uint32_t)flbp::kMaxChunk) B->opts.chunk = flbp::kMaxChunk;
+    if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(B->device);
+    return B->build_tiles();
+}
+
+int flbp_batch_get_opts(flbp_batch *b, flbp_batch_opts *o) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B || !o) return flbp::fail(FLBP_EINVAL, "flbp_batch_get_opts: null argument");
+    *o = B->opts;
+    return FLBP_OK;
+}
+
+int flbp_batch_reset_counters(flbp_batch *b) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B) return flbp::fail(FLBP_EINVAL, "flbp_batch_reset_counters: null batch");
+    std::fill(B->iters.begin(), B->iters.end(), 0);
+    HIP_TRY(hipSetDevice(B->device));
+    HIP_TRY(hipMemsetAsync(B->belief, 0, std::max<uint64_t>(B->Vtot, 1) * 8, B->stream));
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    return FLBP_OK;
+}
+
+int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps, flbp_batch_opts *best) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B || trial_sweeps == 0) return flbp::fail(FLBP_EINVAL, "flbp_batch_autotune: bad argument");
+    HIP_TRY(hipSetDevice(B->device));
+    const flbp_batch_opts base = B->opts;
+    const uint32_t g0 = (uint32_t)default_grid(B->device);
+    const uint32_t tiles[] = {256, 1024, 4096};
+    const uint32_t grids[] = {g0, 2 * g0};
+    const int n = (int)std::min<uint32_t>(trial_sweeps, flbp::kMaxChunk);
+    double best_t = INFINITY;
+    flbp_batch_opts best_o = base;
+    for (uint32_t tf : tiles) {
+        for (uint32_t gb : grids) {
+            flbp_batch_opts o = base;
+            o.tile_factors = tf;
+            o.tile_vars = tf;
+            o.grid_blocks = gb;
+            TRY(flbp_batch_set_opts(b, &o));
+            TRY(flbp_batch_init(b, 0));
+            for (uint64_t g = 0; g < B->G; g++) {
+                B->h_active[g] = 1;
+                B->h_budget[g] = (uint32_t)n;
+            }
+            HIP_TRY(hipMemcpyAsync(B->start_active, B->h_active, B->G, hipMemcpyHostToDevice, B->stream));
+            HIP_TRY(hipMemcpyAsync(B->budget, B->h_budget, B->G * 4, hipMemcpyHostToDevice, B->stream));
+            TRY(B->enqueue_chunk(n, -INFINITY, damping));   /* warm: captures the graph */
+            HIP_TRY(hipStreamSynchronize(B->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            TRY(B->enqueue_chunk(n, -INFINITY, damping));
+            HIP_TRY(hipStreamSynchronize(B->stream));
+            const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (t < best_t) {
+                best_t = t;
+                best_o = o;
+            }
+        }
+    }
+    TRY(flbp_batch_set_opts(b, &best_o));
+    TRY(flbp_batch_reset_counters(b));
+    TRY(flbp_batch_init(b, 0));
+    if (best) *best = best_o;
+    return FLBP_OK;
+}
+
+uint64_t flbp_batch_device_bytes(const flbp_batch *b) {
+    const Batch *B = reinterpret_cast<const Batch *>(b);
+    return B ? B->bytes : 0;
+}
+
+}  // extern "C"

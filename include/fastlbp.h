@@ -206,6 +206,47 @@ int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
 uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
 
 /* ------------------------------------------------------------------ */
+/* Batched solve (not in the reference)                                */
+/* ------------------------------------------------------------------ */
+
+/* Many graphs -- independent graphs or clamped variants of one graph --
+ * solved together: one device state, a persistent factor kernel and a
+ * persistent variable kernel per sweep over a tile list of all unfinished
+ * graphs, a chunk of sweeps captured once in a hipGraph and replayed.  Each
+ * graph stops after exactly the sweeps its plain solve would run, with
+ * bit-identical messages and beliefs. */
+typedef struct flbp_batch flbp_batch;
+
+typedef struct flbp_batch_opts {
+    uint32_t tile_factors;   /* factors per tile (one block walks a tile) */
+    uint32_t tile_vars;      /* variables per tile */
+    uint32_t grid_blocks;    /* persistent grid size; 0 = resident blocks */
+    uint32_t chunk;          /* sweeps per hipGraph replay (<= 64) */
+    uint32_t use_graph;      /* 0: launch the chunk's kernels directly */
+} flbp_batch_opts;
+
+void flbp_batch_default_opts(flbp_batch_opts *o);
+int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *graphs, int device,
+                      const flbp_batch_opts *opts, flbp_batch **out);
+void flbp_batch_destroy(flbp_batch *b);
+/* CausalBP::init for every graph */
+int flbp_batch_init(flbp_batch *b, int logdomain);
+/* CausalBP::run() for every graph: iters_out[G] / maxdiff_out[G] per graph
+ * (NULL skips); total->sweeps = sweeps of the longest-running graph */
+int flbp_batch_run(flbp_batch *b, const flbp_run_opts *opts, uint64_t *iters_out,
+                   double *maxdiff_out, flbp_run_result *total);
+int flbp_batch_beliefs(flbp_batch *b, uint64_t graph, double *out, uint64_t n);
+int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *opts);
+int flbp_batch_get_opts(flbp_batch *b, flbp_batch_opts *opts);
+/* iteration counters and beliefs back to their creation values */
+int flbp_batch_reset_counters(flbp_batch *b);
+/* times trial_sweeps sweeps for each candidate tile size and grid size,
+ * keeps the fastest, then resets counters and re-inits */
+int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps,
+                        flbp_batch_opts *best);
+uint64_t flbp_batch_device_bytes(const flbp_batch *b);
+
+/* ------------------------------------------------------------------ */
 /* Adjoint (not in the reference)                                      */
 /* ------------------------------------------------------------------ */
 
