@@ -13,6 +13,8 @@ oracle, an FP64 restatement of the reference's PARALL sweep, timed on a
 bounded number of sweeps of the same graph on this host's cores).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --workload batched   # 512 graphs + 256 clamped variants vs sequential solves
+    python bench.py --workload adjoint   # 100k-variable tied-rule gradient, backward vs forward
 """
 from __future__ import annotations
 
@@ -77,6 +79,110 @@ def cpu_baseline(flist, damping, budget_s=15.0):
                       f"{dt:.1f} s, OpenMP over {threads} threads"}
 
 
+def build_graphs_parallel(specs, workers=16):
+    """specs: callables returning a Graph; run in a thread pool (the native
+    generator and builder release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(lambda f: f(), specs))
+
+
+def bench_batched(args):
+    """512 graphs with V log-uniform in [1k, 200k] plus 256 clamped variants
+    (1-50 alarms clamped to 0/1 each) of one 200k-variable graph; damping 0.3,
+    tol 1e-9, at most 500 sweeps.  Sequential plain solves (one plan per
+    graph, each init + run) against one batched solve (init + run)."""
+    import numpy as np
+    import torch
+    import fastlbp_amd as fl
+    from fastlbp_amd.batch import Batch
+
+    rng = np.random.default_rng(7)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(200000), 512)).astype(np.int64)
+    base = fl.generate(200000, 99_000)
+    clamps = []
+    for v in range(256):
+        n = int(rng.integers(1, 51))
+        idx = rng.choice(len(base.alarms), size=n, replace=False)
+        clamps.append((base.alarms[idx], rng.integers(0, 2, n).astype(np.uint8)))
+    specs = [(lambda n=int(n), g=g: fl.Graph(fl.generate(n, 10_000 + g).factors)) for g, n in enumerate(sizes)]
+    specs += [(lambda c=c: fl.Graph(fl.clamp(base.factors, c[0], c[1]))) for c in clamps]
+    t0 = time.perf_counter()
+    graphs = build_graphs_parallel(specs)
+    log(f"[bench] built {len(graphs)} graphs in {time.perf_counter() - t0:.1f}s; "
+        f"sum V={sum(g.n_vars for g in graphs)} sum E={sum(g.n_edges for g in graphs)}")
+    tol, maxiter, damp = 1e-9, 500, 0.3
+    # sequential plain launches
+    plans = [fl.Plan(g) for g in graphs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq_iters = []
+    for p in plans:
+        p.init()
+        seq_iters.append(p.run(tol, maxiter, damp).iters)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    seq_bel = [plans[i].beliefs() for i in (0, len(plans) // 2, len(plans) - 1)]
+    del plans
+    b = Batch(graphs)
+    best = b.autotune(damp, trial_sweeps=8)
+    log(f"[bench] autotuned batch opts {best}")
+    b.init()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters, md, tot = b.run(tol, maxiter, damp)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    same = list(iters) == seq_iters and all(
+        np.array_equal(b.beliefs(i), bel) for i, bel in zip((0, len(graphs) // 2, len(graphs) - 1), seq_bel))
+    msgs = sum(2.0 * g.n_edges * int(it) for g, it in zip(graphs, iters))
+    out = {"metric": METRIC, "value": msgs / t_bat, "unit": "msg-updates/s", "n_gpus": 1, "steps": 1,
+           "warmup": 0, "ms_per_step": t_bat * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "batched-512-graphs+256-variants", "n_graphs": len(graphs),
+                      "sum_vars": int(sum(g.n_vars for g in graphs)),
+                      "sum_edges": int(sum(g.n_edges for g in graphs)), "damping": damp, "tol": tol,
+                      "maxiter": maxiter, "batch_opts": best},
+           "batched": {"seconds": t_bat, "sequential_seconds": t_seq, "speedup_vs_sequential": t_seq / t_bat,
+                       "sequential_msg_updates_per_s": msgs / t_seq, "iterations_identical": bool(same),
+                       "max_sweeps": int(iters.max()), "device_bytes": b.device_bytes}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """100k variables, 200 tied rules, 1000 alarms, T = 50, damping 0.5,
+    checkpoint every 10: forward and backward device time."""
+    import numpy as np
+    import fastlbp_amd as fl
+    from fastlbp_amd.adjoint import TiedProblem
+
+    gen = fl.generate(100_000, 4, n_rules=200, n_alarms=1000)
+    rng = np.random.default_rng(4)
+    theta = rng.normal(2.5, 0.3, 200)
+    y = rng.integers(0, 2, len(gen.alarms)).astype(np.float64)
+    g = fl.Graph(gen.factors)
+    prob = TiedProblem(g, gen.rule, 200, gen.alarms, 50, 10, 0.5)
+    fw, bw = [], []
+    for i in range(args.warmup + args.steps):
+        b = prob.marginals(theta)
+        prob.grad(2.0 * (b - y))
+        f_ms, b_ms = prob.adj.last_timing()
+        if i >= args.warmup:
+            fw.append(f_ms)
+            bw.append(b_ms)
+    f_ms, b_ms = float(np.median(fw)), float(np.median(bw))
+    msgs = 2.0 * g.n_edges * 50
+    out = {"metric": METRIC, "value": msgs / (f_ms * 1e-3), "unit": "msg-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": f_ms + b_ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "adjoint-100k-T50", "n_vars": g.n_vars, "n_edges": g.n_edges,
+                      "n_rules": 200, "n_alarms": int(len(gen.alarms)), "T": 50, "checkpoint_every": 10,
+                      "damping": 0.5},
+           "adjoint": {"forward_ms": f_ms, "backward_ms": b_ms, "backward_over_forward": b_ms / f_ms,
+                       "device_bytes": prob.adj.device_bytes}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,11 +190,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", choices=["plain", "batched", "adjoint"], default="plain")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
               "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     import torch
     import fastlbp_amd as fl
