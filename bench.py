@@ -118,6 +118,7 @@ def bench_batched(args):
     t0 = time.perf_counter()
     seq_iters = []
     for p in plans:
+        p.reset()
         p.init()
         seq_iters.append(p.run(tol, maxiter, damp).iters)
     torch.cuda.synchronize()
@@ -217,6 +218,7 @@ def main():
         f"device bytes {plan.device_bytes / 1e6:.1f} MB")
 
     def step():
+        plan.reset()   # a fresh solve each step: counter and old beliefs as constructed
         plan.init()
         return plan.run(w["tol"], w["maxiter"], w["damping"])
 

@@ -124,6 +124,7 @@ SIGNATURES = {
     "flbp_plan_run": (C.c_int, [C.c_void_p, C.POINTER(RunOptsC), C.POINTER(RunResultC)]),
     "flbp_plan_run_fixed": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(RunResultC)]),
     "flbp_plan_beliefs": (C.c_int, [C.c_void_p, f64p, C.c_uint64]),
+    "flbp_plan_reset": (C.c_int, [C.c_void_p]),
     "flbp_plan_get_messages": (C.c_int, [C.c_void_p, f64p, f64p]),
     "flbp_plan_set_messages": (C.c_int, [C.c_void_p, f64p, f64p]),
     "flbp_plan_set_p": (C.c_int, [C.c_void_p, f64p, C.c_uint64]),

@@ -8,8 +8,8 @@
  * checkpoint c into per-sweep state slots (out of place), then walk the
  * segment backwards with var_vjp and fac_vjp (adjoint_kernels.h).  Device
  * memory: ceil(n/k) checkpoints and k state slots of 32 B per edge each,
- * three 16 B-per-edge adjoint arrays, and a 16 B-per-edge scratch row when
- * some variable has more than 8 edges.
+ * three 16 B-per-edge adjoint arrays, and a 64 B-per-edge scratch row when
+ * some variable has more than 4 edges.
  */
 #include <hip/hip_runtime.h>
 
@@ -41,6 +41,7 @@ namespace flbp {
 
 struct Adjoint {
     Plan *P = nullptr;
+    int device = -1;          /* kept here: the plan may be destroyed first */
     uint64_t T = 0;
     uint32_t k = 1;
     uint32_t nck = 0;
@@ -69,7 +70,7 @@ struct Adjoint {
         return FLBP_OK;
     }
     ~Adjoint() {
-        if (P) (void)hipSetDevice(P->device);
+        if (device >= 0) (void)hipSetDevice(device);
         for (void *q : allocs) (void)hipFree(q);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -104,11 +105,12 @@ int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_
     if (!(damping >= 0.0 && damping < 1.0))
         return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: damping must be in [0, 1)");
     if (P->max_fac_deg > (uint64_t)kAdjFacDeg)
-        return flbp::fail(FLBP_ERANGE, "flbp_adjoint_create: factor degree above 8");
+        return flbp::fail(FLBP_ERANGE, "flbp_adjoint_create: factor degree above 6");
     HIP_TRY(hipSetDevice(P->device));
     auto *A = new (std::nothrow) Adjoint();
     if (!A) return flbp::fail(FLBP_ENOMEM, "flbp_adjoint_create: out of memory");
     A->P = P;
+    A->device = P->device;
     A->T = n_sweeps;
     A->k = (uint32_t)std::min<uint64_t>(checkpoint_every, n_sweeps);
     A->nck = (uint32_t)((n_sweeps + A->k - 1) / A->k);
@@ -130,7 +132,7 @@ int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_
     if (!rc) rc = A->alloc(A->avf, E);
     if (!rc) rc = A->alloc(A->avf2, E);
     if (!rc) rc = A->alloc(A->afv, E);
-    if (!rc) rc = A->alloc(A->gscratch, P->max_var_deg > (uint64_t)kAdjVarDeg ? E : 1);
+    if (!rc) rc = A->alloc(A->gscratch, P->max_var_deg > (uint64_t)kAdjVarDeg ? 4 * E : 1);
     if (!rc) rc = A->alloc(A->pbar, P->F);
     if (!rc) rc = A->alloc(A->bbar, P->V);
     if (!rc) rc = A->alloc(A->scratch_belief, P->V);
@@ -167,9 +169,12 @@ int flbp_adjoint_forward(flbp_adjoint *adj, double *beliefs) {
         TRY(P->launch_sweep_on(0, -INFINITY, A->damp, P->m_vf, P->m_fv, P->m_fv, P->m_vf, P->belief));
     }
     HIP_TRY(hipEventRecord(A->ev[1], st));
+    std::vector<double> dev(beliefs ? P->V : 0);
     if (beliefs && P->V)
-        HIP_TRY(hipMemcpyAsync(beliefs, P->belief, P->V * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(dev.data(), P->belief, P->V * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (beliefs)   /* device variable order -> variable order */
+        for (uint64_t dv = 0; dv < P->V; dv++) beliefs[P->h_perm[dv]] = dev[dv];
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, A->ev[0], A->ev[1]));
     A->fwd_ms = ms;
@@ -186,7 +191,9 @@ int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp)
     HIP_TRY(hipSetDevice(P->device));
     const uint64_t E = P->E, V = P->V, F = P->F;
     hipStream_t st = P->stream;
-    if (V) HIP_TRY(hipMemcpyAsync(A->bbar, dl_db, V * sizeof(double), hipMemcpyHostToDevice, st));
+    std::vector<double> dev_db(V);
+    for (uint64_t dv = 0; dv < V; dv++) dev_db[dv] = dl_db[P->h_perm[dv]];
+    if (V) HIP_TRY(hipMemcpyAsync(A->bbar, dev_db.data(), V * sizeof(double), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(A->ev[2], st));
     if (E) {
         hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, A->avf, E, 0.0);

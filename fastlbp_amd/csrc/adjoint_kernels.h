@@ -29,154 +29,148 @@ namespace {
 
 using namespace flbp::dev;
 
-constexpr int kAdjFacDeg = 8;
-constexpr int kAdjVarDeg = 8;
+constexpr int kAdjFacDeg = 6;
+constexpr int kAdjVarDeg = 4;
 
 /* Reverse of the variable pass of one sweep, one lane per variable.
  * m_fv: f->v messages of the sweep (factor-major); avf: adjoint of the
  * sweep's v->f outputs (factor-major); bbar: adjoint of its beliefs (NULL
- * when the loss does not read them); afv: in = adjoint of m_fv carried from
- * the next sweep before damping (the next sweep's total), out = total
- * adjoint of this sweep's m_fv. */
+ * when the loss does not read them); afv: in = total adjoint of the next
+ * sweep's m_fv (carried through damping), out = total adjoint of this
+ * sweep's m_fv.
+ *
+ * Per state s, with L_e = prod_{j != e} m_j and g_e the adjoint of L_e, the
+ * adjoint of m_k is sum_{e != k} g_e prod_{j != e,k} m_j + G * L_k.  It is
+ * formed in O(d) from prefix products P, suffix products Q and the two
+ * running sums W_k = sum_{e<k} g_e prod_{j<k, j!=e} m_j (forward) and
+ * U_k = sum_{e>k} g_e prod_{j>k, j!=e} m_j (backward): adjoint = W_k Q_k +
+ * P_k U_k + G P_k Q_k.  Variables above kAdjVarDeg keep P, Q, g, U in a
+ * lane-owned scratch row (4 pairs per edge). */
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_vjp(
     const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
     const double2 *__restrict__ m_fv, const double2 *__restrict__ avf,
-    const double *__restrict__ bbar, double2 *__restrict__ afv, double2 *__restrict__ gscratch,
+    const double *__restrict__ bbar, double2 *__restrict__ afv, double2 *__restrict__ scratch,
     double damp, uint32_t V) {
     const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
     if (v >= V) return;
     const uint32_t s = var_off[v];
     const int d = (int)(var_off[v + 1] - s);
     const double bb = bbar ? bbar[v] : 0.0;
-    if (d <= kAdjVarDeg) {
-        uint32_t slot[kAdjVarDeg];
-        double m0[kAdjVarDeg], m1[kAdjVarDeg];
-#pragma unroll
-        for (int j = 0; j < kAdjVarDeg; j++) {
-            if (j < d) {
-                slot[j] = var_edge[s + j];
-                const double2 m = m_fv[slot[j]];
-                m0[j] = m.x;
-                m1[j] = m.y;
-            } else {
-                slot[j] = 0;
-                m0[j] = 1.0;
-                m1[j] = 1.0;
-            }
+    auto gadj = [&](double L0, double L1, double2 ob, double &g0, double &g1) __attribute__((always_inline)) {
+        const double S = L0 + L1;
+        g0 = g1 = 0.0;
+        if (S != 0.0) {
+            const double dot = ob.x * (L0 / S) + ob.y * (L1 / S);
+            g0 = (ob.x - dot) / S;
+            g1 = (ob.y - dot) / S;
         }
-        /* leave-one-out products by prefix/suffix */
-        double L0[kAdjVarDeg], L1[kAdjVarDeg];
-        double a0 = 1.0, a1 = 1.0;
-#pragma unroll
-        for (int j = 0; j < kAdjVarDeg; j++) {
-            L0[j] = a0;
-            L1[j] = a1;
-            a0 *= m0[j];
-            a1 *= m1[j];
-        }
-        const double P0 = a0, P1 = a1;
-        a0 = 1.0;
-        a1 = 1.0;
-#pragma unroll
-        for (int j = kAdjVarDeg - 1; j >= 0; j--) {
-            L0[j] *= a0;
-            L1[j] *= a1;
-            a0 *= m0[j];
-            a1 *= m1[j];
-        }
-        /* adjoint of each leave-one-out product through the normalization */
-        double g0[kAdjVarDeg], g1[kAdjVarDeg];
-#pragma unroll
-        for (int j = 0; j < kAdjVarDeg; j++) {
-            g0[j] = 0.0;
-            g1[j] = 0.0;
-            const double S = L0[j] + L1[j];
-            if (j < d && S != 0.0) {
-                const double2 ob = avf[slot[j]];
-                const double dot = ob.x * (L0[j] / S) + ob.y * (L1[j] / S);
-                g0[j] = (ob.x - dot) / S;
-                g1[j] = (ob.y - dot) / S;
-            }
-        }
-        double G0 = 0.0, G1 = 0.0;
+    };
+    auto belief_adj = [&](double P0, double P1, double &G0, double &G1) __attribute__((always_inline)) {
+        G0 = G1 = 0.0;
         if (bb != 0.0 && P0 + P1 != 0.0) {
             const double den = (P0 + P1) * (P0 + P1);
             G1 = bb * P0 / den;
             G0 = -bb * P1 / den;
         }
-        /* d/dm_k of sum_e g_e prod_{j != e} m_j, for each k: a running
-         * product/derivative recurrence over j != k */
+    };
+    if (d <= kAdjVarDeg) {
+        constexpr int N = kAdjVarDeg;
+        uint32_t slot[N];
+        double m0[N], m1[N], P0[N], P1[N], Q0[N], Q1[N], g0[N], g1[N];
 #pragma unroll
-        for (int k = 0; k < kAdjVarDeg; k++) {
-            if (k >= d) break;
-            double w0 = 0.0, w1 = 0.0, q0 = 1.0, q1 = 1.0;
+        for (int j = 0; j < N; j++) slot[j] = j < d ? var_edge[s + j] : 0u;
 #pragma unroll
-            for (int j = 0; j < kAdjVarDeg; j++) {
-                if (j == k) continue;
-                w0 = w0 * m0[j] + g0[j] * q0;
-                w1 = w1 * m1[j] + g1[j] * q1;
-                q0 *= m0[j];
-                q1 *= m1[j];
+        for (int j = 0; j < N; j++) {
+            const double2 m = j < d ? m_fv[slot[j]] : make_double2(1.0, 1.0);
+            m0[j] = m.x;
+            m1[j] = m.y;
+        }
+        double a0 = 1.0, a1 = 1.0;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            P0[j] = a0;
+            P1[j] = a1;
+            a0 *= m0[j];
+            a1 *= m1[j];
+        }
+        double G0, G1;
+        belief_adj(a0, a1, G0, G1);
+        a0 = a1 = 1.0;
+#pragma unroll
+        for (int j = N - 1; j >= 0; j--) {
+            Q0[j] = a0;
+            Q1[j] = a1;
+            a0 *= m0[j];
+            a1 *= m1[j];
+        }
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            if (j < d) gadj(P0[j] * Q0[j], P1[j] * Q1[j], avf[slot[j]], g0[j], g1[j]);
+            else g0[j] = g1[j] = 0.0;
+        }
+        /* backward running sums U_k, kept in the g arrays' partners */
+        double U0[N], U1[N];
+        double u0 = 0.0, u1 = 0.0;
+#pragma unroll
+        for (int j = N - 1; j >= 0; j--) {
+            U0[j] = u0;
+            U1[j] = u1;
+            u0 = u0 * m0[j] + g0[j] * Q0[j];
+            u1 = u1 * m1[j] + g1[j] * Q1[j];
+        }
+        double w0 = 0.0, w1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            if (k < d) {
+                const double2 c = afv[slot[k]];
+                const double r0 = w0 * Q0[k] + P0[k] * U0[k] + G0 * (P0[k] * Q0[k]);
+                const double r1 = w1 * Q1[k] + P1[k] * U1[k] + G1 * (P1[k] * Q1[k]);
+                afv[slot[k]] = make_double2(damp * c.x + r0, damp * c.y + r1);
             }
-            const double2 c = afv[slot[k]];
-            afv[slot[k]] = make_double2(damp * c.x + w0 + G0 * L0[k], damp * c.y + w1 + G1 * L1[k]);
+            w0 = w0 * m0[k] + g0[k] * P0[k];
+            w1 = w1 * m1[k] + g1[k] * P1[k];
         }
         return;
     }
-    /* high-degree variables: the same quantities with O(d^2) loops over
-     * global memory (the lane owns these slots and its scratch row) */
-    double P0 = 1.0, P1 = 1.0;
+    /* high degree: three passes over the edges with P, Q, g, U in the
+     * lane's scratch row scratch[4*(s+j) .. 4*(s+j)+3] */
+    double2 *row = scratch + 4ull * s;
+    double a0 = 1.0, a1 = 1.0;
     for (int j = 0; j < d; j++) {
         const double2 m = m_fv[var_edge[s + j]];
-        P0 *= m.x;
-        P1 *= m.y;
+        row[4 * j] = make_double2(a0, a1);          /* P_j */
+        a0 *= m.x;
+        a1 *= m.y;
     }
-    double G0 = 0.0, G1 = 0.0;
-    if (bb != 0.0 && P0 + P1 != 0.0) {
-        const double den = (P0 + P1) * (P0 + P1);
-        G1 = bb * P0 / den;
-        G0 = -bb * P1 / den;
+    double G0, G1;
+    belief_adj(a0, a1, G0, G1);
+    a0 = a1 = 1.0;
+    double u0 = 0.0, u1 = 0.0;
+    for (int j = d - 1; j >= 0; j--) {
+        const uint32_t sl = var_edge[s + j];
+        const double2 m = m_fv[sl];
+        const double2 P = row[4 * j];
+        double g0, g1;
+        gadj(P.x * a0, P.y * a1, avf[sl], g0, g1);
+        row[4 * j + 1] = make_double2(a0, a1);      /* Q_j */
+        row[4 * j + 2] = make_double2(g0, g1);      /* g_j */
+        row[4 * j + 3] = make_double2(u0, u1);      /* U_j */
+        u0 = u0 * m.x + g0 * a0;
+        u1 = u1 * m.y + g1 * a1;
+        a0 *= m.x;
+        a1 *= m.y;
     }
-    auto loo = [&](int e, double &l0, double &l1) __attribute__((always_inline)) {
-        l0 = 1.0;
-        l1 = 1.0;
-        for (int j = 0; j < d; j++) {
-            if (j == e) continue;
-            const double2 m = m_fv[var_edge[s + j]];
-            l0 *= m.x;
-            l1 *= m.y;
-        }
-    };
-    /* g_e of every edge into the var-major scratch row, then the
-     * recurrence per edge: O(d^2) per variable */
-    for (int e = 0; e < d; e++) {
-        double l0, l1;
-        loo(e, l0, l1);
-        const double S = l0 + l1;
-        double2 g = make_double2(0.0, 0.0);
-        if (S != 0.0) {
-            const double2 ob = avf[var_edge[s + e]];
-            const double dot = ob.x * (l0 / S) + ob.y * (l1 / S);
-            g = make_double2((ob.x - dot) / S, (ob.y - dot) / S);
-        }
-        gscratch[s + e] = g;
-    }
+    double w0 = 0.0, w1 = 0.0;
     for (int k = 0; k < d; k++) {
-        double w0 = 0.0, w1 = 0.0, q0 = 1.0, q1 = 1.0;
-        for (int j = 0; j < d; j++) {
-            if (j == k) continue;
-            const double2 m = m_fv[var_edge[s + j]];
-            const double2 g = gscratch[s + j];
-            w0 = w0 * m.x + g.x * q0;
-            w1 = w1 * m.y + g.y * q1;
-            q0 *= m.x;
-            q1 *= m.y;
-        }
-        double l0, l1;
-        loo(k, l0, l1);
         const uint32_t sl = var_edge[s + k];
+        const double2 m = m_fv[sl];
+        const double2 P = row[4 * k], Q = row[4 * k + 1], g = row[4 * k + 2], U = row[4 * k + 3];
         const double2 c = afv[sl];
-        afv[sl] = make_double2(damp * c.x + w0 + G0 * l0, damp * c.y + w1 + G1 * l1);
+        const double r0 = w0 * Q.x + P.x * U.x + G0 * (P.x * Q.x);
+        const double r1 = w1 * Q.y + P.y * U.y + G1 * (P.y * Q.y);
+        afv[sl] = make_double2(damp * c.x + r0, damp * c.y + r1);
+        w0 = w0 * m.x + g.x * P.x;
+        w1 = w1 * m.y + g.y * P.y;
     }
 }
 

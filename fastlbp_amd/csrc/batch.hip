@@ -47,6 +47,7 @@ struct Batch {
     uint32_t *h_budget = nullptr;           /* pinned */
     std::vector<uint64_t> iters;
     std::vector<double> last_diff;
+    std::vector<uint32_t> h_perm;           /* per graph at var_base: device variable -> variable */
     std::vector<void *> allocs;
     uint64_t bytes = 0;
     flbp_batch_opts opts{};
@@ -289,11 +290,12 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
             for (uint64_t I = 0; I < c.n_factors; I++) mk[I] = make_double2(c.mask0[I], c.mask1[I]);
             HIP_TRY(hipMemcpy(B->fac_mask + bg.fac_base, mk.data(), c.n_factors * 16, hipMemcpyHostToDevice));
         }
-        for (uint64_t i = 0; i <= c.n_vars; i++) u32[i] = (uint32_t)c.var_off[i];
-        HIP_TRY(hipMemcpy(B->var_off + bg.var_off_base, u32.data(), (c.n_vars + 1) * 4, hipMemcpyHostToDevice));
-        for (uint64_t k = 0; k < c.n_edges; k++) u32[k] = (uint32_t)c.var_edge[k];
+        std::vector<uint32_t> perm, dev_off, dev_edge;
+        flbp::var_device_order(c, perm, dev_off, dev_edge);
+        B->h_perm.insert(B->h_perm.end(), perm.begin(), perm.end());
+        HIP_TRY(hipMemcpy(B->var_off + bg.var_off_base, dev_off.data(), (c.n_vars + 1) * 4, hipMemcpyHostToDevice));
         if (c.n_edges)
-            HIP_TRY(hipMemcpy(B->var_edge + bg.edge_base, u32.data(), c.n_edges * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(B->var_edge + bg.edge_base, dev_edge.data(), c.n_edges * 4, hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMemset(B->belief, 0, std::max<uint64_t>(V, 1) * 8));
     HIP_TRY(hipMemset(B->m_vf, 0, std::max<uint64_t>(E, 1) * 16));
@@ -396,9 +398,15 @@ int flbp_batch_beliefs(flbp_batch *b, uint64_t g, double *out, uint64_t n) {
     if (!B || (n && !out) || g >= (B ? B->G : 0)) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs: bad argument");
     if (n > B->h_graphs[g].n_var) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs: n > n_vars of the graph");
     HIP_TRY(hipSetDevice(B->device));
-    if (n)
-        HIP_TRY(hipMemcpyAsync(out, B->belief + B->h_graphs[g].var_base, n * 8, hipMemcpyDeviceToHost, B->stream));
+    const BatchGraph &bg = B->h_graphs[g];
+    std::vector<double> dev(bg.n_var);
+    if (bg.n_var)
+        HIP_TRY(hipMemcpyAsync(dev.data(), B->belief + bg.var_base, bg.n_var * 8, hipMemcpyDeviceToHost, B->stream));
     HIP_TRY(hipStreamSynchronize(B->stream));
+    for (uint64_t dv = 0; dv < bg.n_var; dv++) {
+        const uint32_t v = B->h_perm[bg.var_base + dv];
+        if (v < n) out[v] = dev[dv];
+    }
     return FLBP_OK;
 }
 

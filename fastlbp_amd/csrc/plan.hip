@@ -276,8 +276,27 @@ int flbp_plan_beliefs(flbp_plan *plan, double *out, uint64_t n) {
     if (!P || (n && !out)) return flbp::fail(FLBP_EINVAL, "flbp_plan_beliefs: null argument");
     if (n > P->V) return flbp::fail(FLBP_EINVAL, "flbp_plan_beliefs: n > n_vars");
     HIP_TRY(hipSetDevice(P->device));
-    if (n) HIP_TRY(hipMemcpyAsync(out, P->belief, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    if (n) {
+        std::vector<double> dev(P->V);
+        HIP_TRY(hipMemcpyAsync(dev.data(), P->belief, P->V * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        for (uint64_t dv = 0; dv < P->V; dv++)
+            if (P->h_perm[dv] < n) out[P->h_perm[dv]] = dev[dv];
+    }
     HIP_TRY(hipStreamSynchronize(P->stream));
+    return FLBP_OK;
+}
+
+int flbp_plan_reset(flbp_plan *plan) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_reset: null plan");
+    HIP_TRY(hipSetDevice(P->device));
+    if (P->V) hipLaunchKernelGGL(fill_f64, dim3(grid_for(P->V, 256)), dim3(256), 0, P->stream, P->belief, P->V, 0.0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->iters = 0;
+    P->last_maxdiff = INFINITY;
+    P->swept_since_init = false;
     return FLBP_OK;
 }
 

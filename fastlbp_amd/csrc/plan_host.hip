@@ -45,6 +45,26 @@ int validate_csr(const flbp_csr &c) {
     return FLBP_OK;
 }
 
+void var_device_order(const flbp_csr &c, std::vector<uint32_t> &perm, std::vector<uint32_t> &dev_off,
+                      std::vector<uint32_t> &dev_edge) {
+    const uint64_t V = c.n_vars;
+    perm.resize(V);
+    uint64_t maxd = 0;
+    for (uint64_t i = 0; i < V; i++) maxd = std::max<uint64_t>(maxd, c.var_off[i + 1] - c.var_off[i]);
+    /* counting sort by degree, heaviest first, stable in variable index */
+    std::vector<uint64_t> start(maxd + 2, 0);
+    for (uint64_t i = 0; i < V; i++) start[maxd - (c.var_off[i + 1] - c.var_off[i]) + 1]++;
+    for (uint64_t b = 0; b <= maxd; b++) start[b + 1] += start[b];
+    for (uint64_t i = 0; i < V; i++) perm[start[maxd - (c.var_off[i + 1] - c.var_off[i])]++] = (uint32_t)i;
+    dev_off.assign(V + 1, 0);
+    dev_edge.resize(c.n_edges);
+    for (uint64_t dv = 0; dv < V; dv++) {
+        const uint64_t v = perm[dv], a = c.var_off[v], b = c.var_off[v + 1];
+        dev_off[dv + 1] = dev_off[dv] + (uint32_t)(b - a);
+        for (uint64_t k = a; k < b; k++) dev_edge[dev_off[dv] + (k - a)] = (uint32_t)c.var_edge[k];
+    }
+}
+
 template <class T>
 int Plan::alloc(T *&p, uint64_t n) {
     const size_t bytes = std::max<uint64_t>(n, 1) * sizeof(T);
@@ -94,10 +114,12 @@ int Plan::upload(const flbp_csr &c) {
         u32[I] = dev::pack_meta((uint32_t)c.fac_type[I], c.head_clamped[I] != 0,
                                 (uint32_t)(c.head_slot[I] - c.fac_off[I]));
     if (F) HIP_TRY(hipMemcpy(fac_meta, u32.data(), F * 4, hipMemcpyHostToDevice));
-    for (uint64_t i = 0; i <= V; i++) u32[i] = (uint32_t)c.var_off[i];
-    HIP_TRY(hipMemcpy(var_off, u32.data(), (V + 1) * 4, hipMemcpyHostToDevice));
-    for (uint64_t k = 0; k < E; k++) u32[k] = (uint32_t)c.var_edge[k];
-    if (E) HIP_TRY(hipMemcpy(var_edge, u32.data(), E * 4, hipMemcpyHostToDevice));
+    /* variables in device order (heaviest first): each variable's edges keep
+     * their nbV order, so its arithmetic is unchanged */
+    std::vector<uint32_t> dev_off, dev_edge;
+    var_device_order(c, h_perm, dev_off, dev_edge);
+    HIP_TRY(hipMemcpy(var_off, dev_off.data(), (V + 1) * 4, hipMemcpyHostToDevice));
+    if (E) HIP_TRY(hipMemcpy(var_edge, dev_edge.data(), E * 4, hipMemcpyHostToDevice));
     if (F) {
         HIP_TRY(hipMemcpy(fac_p, c.p, F * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(fac_q, c.q, F * 8, hipMemcpyHostToDevice));

@@ -19,6 +19,11 @@ inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + bl
 
 int hip_fail(hipError_t e, const char *what);
 int validate_csr(const flbp_csr &c);
+/* device order of variables: perm[device index] = variable index, sorted by
+ * degree (heaviest first, stable); dev_off/dev_edge = var_off/var_edge in
+ * that order */
+void var_device_order(const flbp_csr &c, std::vector<uint32_t> &perm, std::vector<uint32_t> &dev_off,
+                      std::vector<uint32_t> &dev_edge);
 
 struct Plan {
     int device = -1;
@@ -35,6 +40,7 @@ struct Plan {
     uint64_t max_fac_deg = 0, max_var_deg = 0;
 
     std::vector<uint64_t> h_var_edge;              /* for message export/import */
+    std::vector<uint32_t> h_perm;                  /* device variable -> variable */
 
     uint64_t iters = 0;                            /* CausalBP::_iters */
     double last_maxdiff = INFINITY;

@@ -321,17 +321,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass(
         };
         if (d <= kVarRegDeg) {
 #pragma unroll
-            for (int j = 0; j < kVarRegDeg; j++) {
-                if (j < d) {
-                    slot[j] = var_edge[s + j];
-                    msg[j] = m_fv[slot[j]];
-                }
-            }
+            for (int j = 0; j < kVarRegDeg; j++)
+                if (j < d) slot[j] = var_edge[s + j];
+#pragma unroll
+            for (int j = 0; j < kVarRegDeg; j++)
+                if (j < d) msg[j] = m_fv[slot[j]];
 #pragma unroll
             for (int j = 0; j < kVarRegDeg; j++)
                 if (j < d) accumulate(msg[j]);
         } else {
-            for (int j = 0; j < d; j++) accumulate(m_fv[var_edge[s + j]]);
+            /* high degree: gather in unrolled chunks so kVarRegDeg loads are
+             * in flight at once; the product still runs in edge order */
+            for (int base = 0; base < d; base += kVarRegDeg) {
+#pragma unroll
+                for (int j = 0; j < kVarRegDeg; j++)
+                    if (base + j < d) slot[j] = var_edge[s + base + j];
+#pragma unroll
+                for (int j = 0; j < kVarRegDeg; j++)
+                    if (base + j < d) msg[j] = m_fv[slot[j]];
+#pragma unroll
+                for (int j = 0; j < kVarRegDeg; j++)
+                    if (base + j < d) accumulate(msg[j]);
+            }
         }
         /* getBeliefsVKernel + norm_functor + dist_functor */
         const double b0 = prod_0 * (double)(z0 == 0);
@@ -345,9 +356,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass(
             for (int j = 0; j < kVarRegDeg; j++)
                 if (j < d) m_vf[slot[j]] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
         } else {
-            for (int j = 0; j < d; j++) {
-                const uint32_t sl = var_edge[s + j];
-                m_vf[sl] = vf_msg(prod_0, prod_1, z0, z1, m_fv[sl]);
+            for (int base = 0; base < d; base += kVarRegDeg) {
+#pragma unroll
+                for (int j = 0; j < kVarRegDeg; j++)
+                    if (base + j < d) slot[j] = var_edge[s + base + j];
+#pragma unroll
+                for (int j = 0; j < kVarRegDeg; j++)
+                    if (base + j < d) msg[j] = m_fv[slot[j]];
+#pragma unroll
+                for (int j = 0; j < kVarRegDeg; j++)
+                    if (base + j < d) m_vf[slot[j]] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
             }
         }
     }

@@ -236,6 +236,10 @@ class Plan:
     def init(self, logdomain: bool = False) -> None:
         call("flbp_plan_init", self._h, 1 if logdomain else 0)
 
+    def reset(self) -> None:
+        """Iteration counter and old beliefs back to a fresh plan's."""
+        call("flbp_plan_reset", self._h)
+
     def run(self, tol: float, maxiter: int, damping: float = 0.0, maxtime: float = math.inf,
             chunk: int = 0) -> RunResult:
         o = RunOptsC(tol, maxiter, maxtime, damping, chunk)

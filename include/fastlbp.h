@@ -182,6 +182,11 @@ int flbp_plan_run(flbp_plan *plan, const flbp_run_opts *opts, flbp_run_result *r
 int flbp_plan_run_fixed(flbp_plan *plan, uint64_t n_sweeps, double damping,
                         flbp_run_result *res);
 
+/* Back to the state of a freshly created plan: iteration counter 0 and old
+ * beliefs 0.0 (the reference gets this only by constructing a new
+ * CausalBP).  Messages are left as they are; call flbp_plan_init next. */
+int flbp_plan_reset(flbp_plan *plan);
+
 /* transferBeliefToHost (causal_bp.cu:552-554): P(x_i = 1) for i < n_vars. */
 int flbp_plan_beliefs(flbp_plan *plan, double *out, uint64_t n);
 
@@ -255,8 +260,8 @@ uint64_t flbp_batch_device_bytes(const flbp_batch *b);
  * backward recomputes each segment from its checkpoint and runs the
  * hand-written reverse kernels of the factor and variable passes.  The
  * adjoint works on its plan's device buffers (the forward overwrites the
- * plan's messages and beliefs).  Factors of degree above 8 are rejected
- * (FLBP_ERANGE). */
+ * plan's messages and beliefs).  Factors of degree above 6 are rejected
+ * (FLBP_ERANGE).  Destroy an adjoint before its plan. */
 typedef struct flbp_adjoint flbp_adjoint;
 
 int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_every,

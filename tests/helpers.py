@@ -15,9 +15,17 @@ def oracle_clamp(flist, labels, values):
 
 
 def same_bits(a, b):
+    """Bit-identical doubles; NaNs must sit at the same positions but their
+    payload and sign may differ (x86 produces a negative NaN for 0/0, the
+    GPU a positive one; no later arithmetic can tell them apart)."""
     a = np.ascontiguousarray(a, np.float64)
     b = np.ascontiguousarray(b, np.float64)
-    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    if a.shape != b.shape:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return np.array_equal(a[~na].view(np.uint64), b[~nb].view(np.uint64))
 
 
 def random_alarm_clamps(rng, alarms, max_n):

@@ -15,7 +15,8 @@ import pytest
 
 import fastlbp_amd as fl
 import oracle
-from fastlbp_amd.adjoint import TiedProblem, tied_marginals_function
+from fastlbp_amd.adjoint import TiedProblem, tied_marginals_function, tied_p
+from tests.helpers import same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -29,10 +30,25 @@ def make(n_vars, seed, n_rules, n_alarms, T, k, damping):
     return gen, theta, y, prob
 
 
+def seq_loss(b, y):
+    """sum of squares in alarm order, as the oracle adds them"""
+    L = 0.0
+    for bi, yi in zip(b.tolist(), y.tolist()):
+        L += (bi - yi) * (bi - yi)
+    return L
+
+
 def loss_and_grad(prob, theta, y):
     b = prob.marginals(theta)
     r = b - y
-    return float(np.sum(r * r)), prob.grad(2.0 * r)
+    return seq_loss(b, y), prob.grad(2.0 * r)
+
+
+def oracle_forward(gen, theta, T, damping):
+    o = oracle.OracleGraph(gen.factors.with_p(tied_p(gen.factors.p, gen.rule, theta)))
+    o.init()
+    o.run_fixed(T, damping)
+    return o
 
 
 def close(g, ref, rel=1e-6):
@@ -45,8 +61,10 @@ def test_gradient_matches_oracle_unrolled(seed, damping, T, k):
     n_rules = 20
     gen, theta, y, prob = make(3000, seed, n_rules, 60, T, k, damping)
     L, g = loss_and_grad(prob, theta, y)
+    o = oracle_forward(gen, theta, T, damping)
+    assert same_bits(prob.adj.forward(), o.beliefs())   # the forward is the reference arithmetic
     Lo, go = oracle.grad(gen.factors, gen.rule, n_rules, theta, gen.alarms, y, T, damping)
-    assert L == Lo                      # forward is the reference arithmetic, bit for bit
+    assert L == Lo
     assert close(g, go, rel=1e-9), np.max(np.abs(g - go))
 
 
