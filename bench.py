@@ -192,6 +192,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["plain", "batched", "adjoint"], default="plain")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy a 512 MiB buffer once, so HBM counters can be scaled by a known size")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
@@ -204,6 +206,14 @@ def main():
 
     import torch
     import fastlbp_amd as fl
+
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 536870912 bytes read, 536870912 bytes written")
+        del src, dst
 
     w = WORKLOAD
     t0 = time.perf_counter()

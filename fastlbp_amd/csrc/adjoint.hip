@@ -7,7 +7,9 @@
  * Backward: for segments c = last..0, recompute sweeps c*k+1..c*k+k from
  * checkpoint c into per-sweep state slots (out of place), then walk the
  * segment backwards with var_vjp and fac_vjp (adjoint_kernels.h).  Device
- * memory: ceil(n/k) checkpoints and k state slots of 32 B per edge each,
+ * memory (m_vf and the adjoint of m_fv var-major, m_fv and the adjoint of
+ * m_vf factor-major, so every kernel gathers and writes contiguously):
+ * ceil(n/k) checkpoints and k state slots of 32 B per edge each,
  * three 16 B-per-edge adjoint arrays, and a 64 B-per-edge scratch row when
  * some variable has more than 4 edges.
  */
@@ -219,8 +221,8 @@ int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp)
                                    A->gscratch, A->damp, (uint32_t)V);
             if (F)
                 hipLaunchKernelGGL(fac_vjp, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, st, P->fac_off,
-                                   P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, vf_state(t - 1), A->afv,
-                                   A->avf2, A->pbar, keep, (uint32_t)F);
+                                   P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, P->fac_vpos, vf_state(t - 1),
+                                   A->afv, A->avf2, A->pbar, keep, (uint32_t)F);
             std::swap(A->avf, A->avf2);
         }
         HIP_TRY(hipGetLastError());

@@ -35,9 +35,9 @@ constexpr int kAdjVarDeg = 4;
 /* Reverse of the variable pass of one sweep, one lane per variable.
  * m_fv: f->v messages of the sweep (factor-major); avf: adjoint of the
  * sweep's v->f outputs (factor-major); bbar: adjoint of its beliefs (NULL
- * when the loss does not read them); afv: in = total adjoint of the next
- * sweep's m_fv (carried through damping), out = total adjoint of this
- * sweep's m_fv.
+ * when the loss does not read them); afv (var-major, written at the
+ * variable's own positions): in = total adjoint of the next sweep's m_fv
+ * (carried through damping), out = total adjoint of this sweep's m_fv.
  *
  * Per state s, with L_e = prod_{j != e} m_j and g_e the adjoint of L_e, the
  * adjoint of m_k is sum_{e != k} g_e prod_{j != e,k} m_j + G * L_k.  It is
@@ -122,10 +122,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_vjp(
 #pragma unroll
         for (int k = 0; k < N; k++) {
             if (k < d) {
-                const double2 c = afv[slot[k]];
+                const double2 c = afv[s + k];
                 const double r0 = w0 * Q0[k] + P0[k] * U0[k] + G0 * (P0[k] * Q0[k]);
                 const double r1 = w1 * Q1[k] + P1[k] * U1[k] + G1 * (P1[k] * Q1[k]);
-                afv[slot[k]] = make_double2(damp * c.x + r0, damp * c.y + r1);
+                afv[s + k] = make_double2(damp * c.x + r0, damp * c.y + r1);
             }
             w0 = w0 * m0[k] + g0[k] * P0[k];
             w1 = w1 * m1[k] + g1[k] * P1[k];
@@ -165,10 +165,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_vjp(
         const uint32_t sl = var_edge[s + k];
         const double2 m = m_fv[sl];
         const double2 P = row[4 * k], Q = row[4 * k + 1], g = row[4 * k + 2], U = row[4 * k + 3];
-        const double2 c = afv[sl];
+        const double2 c = afv[s + k];
         const double r0 = w0 * Q.x + P.x * U.x + G0 * (P.x * Q.x);
         const double r1 = w1 * Q.y + P.y * U.y + G1 * (P.y * Q.y);
-        afv[sl] = make_double2(damp * c.x + r0, damp * c.y + r1);
+        afv[s + k] = make_double2(damp * c.x + r0, damp * c.y + r1);
         w0 = w0 * m.x + g.x * P.x;
         w1 = w1 * m.y + g.y * P.y;
     }
@@ -272,14 +272,15 @@ __device__ __forceinline__ void fac_output_vjp(const double (&m0)[N], const doub
 }
 
 /* Reverse of the factor pass of one sweep, one lane per factor.
- * m_vf: the sweep's v->f inputs; afv: total adjoint of the sweep's f->v
- * outputs; avf_out: adjoint of the v->f inputs (written); pbar += dL/dp. */
+ * m_vf: the sweep's v->f inputs (var-major); afv: total adjoint of the
+ * sweep's f->v outputs (var-major); both gathered through fac_vpos.
+ * avf_out: adjoint of the v->f inputs, written factor-major; pbar += dL/dp. */
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void fac_vjp(
     const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
     const double *__restrict__ fac_p, const double *__restrict__ fac_q,
-    const double2 *__restrict__ fac_mask, const double2 *__restrict__ m_vf,
-    const double2 *__restrict__ afv, double2 *__restrict__ avf_out, double *__restrict__ pbar,
-    double keep, uint32_t F) {
+    const double2 *__restrict__ fac_mask, const uint32_t *__restrict__ fac_vpos,
+    const double2 *__restrict__ m_vf, const double2 *__restrict__ afv, double2 *__restrict__ avf_out,
+    double *__restrict__ pbar, double keep, uint32_t F) {
     const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= F) return;
     const uint32_t s = fac_off[f];
@@ -301,14 +302,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void fac_vjp(
         double m0[kAdjFacDeg], m1[kAdjFacDeg], acc0[kAdjFacDeg], acc1[kAdjFacDeg];
 #pragma unroll
         for (int j = 0; j < kAdjFacDeg; j++) {
-            const double2 m = j < d ? m_vf[s + j] : make_double2(1.0, 1.0);
+            const double2 m = j < d ? m_vf[fac_vpos[s + j]] : make_double2(1.0, 1.0);
             m0[j] = m.x;
             m1[j] = m.y;
             acc0[j] = 0.0;
             acc1[j] = 0.0;
         }
         for (int t = 0; t < d; t++) {
-            const double2 a = afv[s + t];
+            const double2 a = afv[fac_vpos[s + t]];
             fac_output_vjp<kAdjFacDeg>(m0, m1, d, t, h, type, clamped, p, q, mask0, mask1,
                                        make_double2(keep * a.x, keep * a.y), acc0, acc1, pb);
         }

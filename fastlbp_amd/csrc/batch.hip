@@ -32,9 +32,11 @@ struct Batch {
     int device = -1;
     hipStream_t stream = nullptr;
     uint64_t G = 0, Ftot = 0, Vtot = 0, Etot = 0;
+    uint64_t max_fac_deg = 0;
     std::vector<BatchGraph> h_graphs;
     BatchGraph *graphs = nullptr;
     uint32_t *fac_off = nullptr, *fac_meta = nullptr, *var_off = nullptr, *var_edge = nullptr;
+    uint32_t *fac_vpos = nullptr;
     double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
     Tile *ftiles = nullptr, *vtiles = nullptr;
@@ -137,14 +139,19 @@ int Batch::record_chunk(int n, double tol, double damping) {
     const uint32_t vg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_vtiles));
     for (int k = 0; k < n; k++) {
         if (n_ftiles) {
-            if (damping != 0.0)
-                hipLaunchKernelGGL(batch_factor_pass<true>, dim3(fg), dim3(kBlock), 0, stream, graphs, ftiles,
-                                   n_ftiles, fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv, damping,
-                                   keep, c, k);
-            else
-                hipLaunchKernelGGL(batch_factor_pass<false>, dim3(fg), dim3(kBlock), 0, stream, graphs, ftiles,
-                                   n_ftiles, fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv, damping,
-                                   keep, c, k);
+            const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
+#define FLBP_BATCH_FACTOR_PASS(DAMP, N)                                                                  \
+    hipLaunchKernelGGL((batch_factor_pass<DAMP, N>), dim3(fg), dim3(kBlock), 0, stream, graphs, ftiles,   \
+                       n_ftiles, fac_off, fac_meta, fac_p, fac_q, fac_mask, fac_vpos, m_vf, m_fv, damping, \
+                       keep, c, k)
+            if (damping != 0.0) {
+                if (narrow) FLBP_BATCH_FACTOR_PASS(true, kFacRegDegSmall);
+                else FLBP_BATCH_FACTOR_PASS(true, kFacRegDeg);
+            } else {
+                if (narrow) FLBP_BATCH_FACTOR_PASS(false, kFacRegDegSmall);
+                else FLBP_BATCH_FACTOR_PASS(false, kFacRegDeg);
+            }
+#undef FLBP_BATCH_FACTOR_PASS
         }
         if (n_vtiles)
             hipLaunchKernelGGL(batch_var_pass, dim3(vg), dim3(kBlock), 0, stream, graphs, vtiles, n_vtiles,
@@ -194,7 +201,8 @@ int default_grid(int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_factor_pass<true>, kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_factor_pass<true, kFacRegDeg>, kBlock, 0) !=
+            hipSuccess ||
         per_cu <= 0)
         per_cu = 4;
     return prop.multiProcessorCount * per_cu;
@@ -244,6 +252,8 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
     for (uint64_t g = 0; g < n_graphs; g++) {
         const flbp_csr &c = *csrs[g];
         B->h_graphs[g] = {F, V, E, F + g, V + g, (uint32_t)c.n_factors, (uint32_t)c.n_vars};
+        for (uint64_t I = 0; I < c.n_factors; I++)
+            B->max_fac_deg = std::max<uint64_t>(B->max_fac_deg, c.fac_off[I + 1] - c.fac_off[I]);
         F += c.n_factors;
         V += c.n_vars;
         E += c.n_edges;
@@ -260,6 +270,7 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
     TRY(B->alloc(B->fac_mask, F));
     TRY(B->alloc(B->var_off, V + n_graphs));
     TRY(B->alloc(B->var_edge, E));
+    TRY(B->alloc(B->fac_vpos, E));
     TRY(B->alloc(B->m_vf, E));
     TRY(B->alloc(B->m_fv, E));
     TRY(B->alloc(B->belief, V));
@@ -290,12 +301,14 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
             for (uint64_t I = 0; I < c.n_factors; I++) mk[I] = make_double2(c.mask0[I], c.mask1[I]);
             HIP_TRY(hipMemcpy(B->fac_mask + bg.fac_base, mk.data(), c.n_factors * 16, hipMemcpyHostToDevice));
         }
-        std::vector<uint32_t> perm, dev_off, dev_edge;
-        flbp::var_device_order(c, perm, dev_off, dev_edge);
+        std::vector<uint32_t> perm, dev_off, dev_edge, vpos;
+        flbp::var_device_order(c, false, perm, dev_off, dev_edge, vpos);
         B->h_perm.insert(B->h_perm.end(), perm.begin(), perm.end());
         HIP_TRY(hipMemcpy(B->var_off + bg.var_off_base, dev_off.data(), (c.n_vars + 1) * 4, hipMemcpyHostToDevice));
-        if (c.n_edges)
+        if (c.n_edges) {
             HIP_TRY(hipMemcpy(B->var_edge + bg.edge_base, dev_edge.data(), c.n_edges * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(B->fac_vpos + bg.edge_base, vpos.data(), c.n_edges * 4, hipMemcpyHostToDevice));
+        }
     }
     HIP_TRY(hipMemset(B->belief, 0, std::max<uint64_t>(V, 1) * 8));
     HIP_TRY(hipMemset(B->m_vf, 0, std::max<uint64_t>(E, 1) * 16));

@@ -54,57 +54,36 @@ __device__ __forceinline__ bool graph_runs(const BatchCtl &c, uint32_t g, int k)
     return prev > c.tol;
 }
 
-template <bool kDamp>
+template <bool kDamp, int N>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_pass(
     const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, uint32_t n_tiles,
     const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
     const double *__restrict__ fac_p, const double *__restrict__ fac_q,
-    const double2 *__restrict__ fac_mask, const double2 *__restrict__ m_vf,
-    double2 *__restrict__ m_fv, double damp, double keep, BatchCtl ctl, int k) {
+    const double2 *__restrict__ fac_mask, const uint32_t *__restrict__ fac_vpos,
+    const double2 *__restrict__ m_vf, double2 *__restrict__ m_fv, double damp, double keep,
+    BatchCtl ctl, int k) {
     for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
         const Tile tile = tiles[ti];
         if (!graph_runs(ctl, tile.graph, k)) continue;
         const BatchGraph gr = graphs[tile.graph];
         const uint32_t *off = fac_off + gr.fac_off_base;
         const double2 *vf = m_vf + gr.edge_base;
+        const uint32_t *vpos = fac_vpos + gr.edge_base;
         double2 *fv = m_fv + gr.edge_base;
         for (uint32_t lf = tile.first + threadIdx.x; lf < tile.first + tile.count; lf += kBlock) {
             const uint64_t f = gr.fac_base + lf;
             const uint32_t s = off[lf];
-            const int d = (int)(off[lf + 1] - s);
             const uint32_t meta = fac_meta[f];
-            const uint32_t type = meta & 3u;
-            const bool clamped = (meta & 4u) != 0;
-            const int h = (int)(meta >> 8);
             const double p = fac_p[f];
-            const double q = type ? fac_q[f] : 0.0;
+            const double q = (meta & 3u) ? fac_q[f] : 0.0;
             double mask0 = 1.0, mask1 = 1.0;
-            if (clamped) {
+            if (meta & 4u) {
                 const double2 mk = fac_mask[f];
                 mask0 = mk.x;
                 mask1 = mk.y;
             }
-            auto store = [&](int t, double2 r) __attribute__((always_inline)) {
-                if (kDamp) {
-                    const double2 old = fv[s + t];
-                    r.x = keep * r.x + damp * old.x;
-                    r.y = keep * r.y + damp * old.y;
-                }
-                fv[s + t] = r;
-            };
-            if (d <= kFacRegDeg) {
-                RegIn<kFacRegDeg> in;
-#pragma unroll
-                for (int j = 0; j < kFacRegDeg; j++) in.m[j] = j < d ? vf[s + j] : make_double2(0.0, 0.0);
-                const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
-                for (int t = 0; t < d; t++)
-                    store(t, fv_msg<kFacRegDeg>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
-            } else {
-                GlobalIn in{vf + s};
-                const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
-                for (int t = 0; t < d; t++)
-                    store(t, fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
-            }
+            factor_update<N, kDamp>(vf, vpos + s, fv + s, fv + s, (int)(off[lf + 1] - s), meta, p, q, mask0,
+                                    mask1, damp, keep);
         }
     }
 }
@@ -127,47 +106,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass
         unsigned long long diff_bits = 0;
         for (uint32_t lv = tile.first + threadIdx.x; lv < tile.first + tile.count; lv += kBlock) {
             const uint32_t s = off[lv];
-            const int d = (int)(off[lv + 1] - s);
-            double prod_0 = 1.0, prod_1 = 1.0;
-            uint32_t z0 = 0, z1 = 0;
-            uint32_t slot[kVarRegDeg];
-            double2 msg[kVarRegDeg];
-            auto accumulate = [&](double2 m) __attribute__((always_inline)) {
-                if (m.x == 0) z0++; else prod_0 *= m.x;
-                if (m.y == 0) z1++; else prod_1 *= m.y;
-                scale_pair(prod_0, prod_1);
-            };
-            if (d <= kVarRegDeg) {
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++) {
-                    if (j < d) {
-                        slot[j] = ve[s + j];
-                        msg[j] = fv[slot[j]];
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++)
-                    if (j < d) accumulate(msg[j]);
-            } else {
-                for (int j = 0; j < d; j++) accumulate(fv[ve[s + j]]);
-            }
-            const double b0 = prod_0 * (double)(z0 == 0);
-            const double b1 = prod_1 * (double)(z1 == 0);
-            const double b = b1 / (b0 + b1);
-            const double diff = fabs(bel[lv] - b);
-            bel[lv] = b;
-            const unsigned long long bits = (unsigned long long)__double_as_longlong(diff);
+            const unsigned long long bits = var_update(ve + s, fv, vf + s, bel + lv, (int)(off[lv + 1] - s));
             diff_bits = bits > diff_bits ? bits : diff_bits;
-            if (d <= kVarRegDeg) {
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++)
-                    if (j < d) vf[slot[j]] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
-            } else {
-                for (int j = 0; j < d; j++) {
-                    const uint32_t sl = ve[s + j];
-                    vf[sl] = vf_msg(prod_0, prod_1, z0, z1, fv[sl]);
-                }
-            }
         }
         diff_bits = wave_max_u64(diff_bits);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -7,9 +7,10 @@
  *   fac_meta u32[F]     type | head-clamped bit | head slot << 8
  *   fac_p    f64[F]     prob(); fac_q f64[F] prob_default()
  *   fac_mask f64x2[F]   head_mask, read only for head-clamped factors
+ *   fac_vpos u32[E]     var-major position of each factor-major slot
  *   var_off  u32[V+1]   var-major edge ranges (h_row_ptr_fv)
  *   var_edge u32[E]     factor-major slot of each var-major edge
- *   m_vf     f64x2[E]   v->f messages, factor-major, (state0, state1) pairs
+ *   m_vf     f64x2[E]   v->f messages, var-major, (state0, state1) pairs
  *   m_fv     f64x2[E]   f->v messages, factor-major
  *   belief   f64[V]     P(x=1) of the last sweep (the reference's _oldBeliefsV)
  *   ctl      u64[chunk] per-sweep max |b - b_old| as IEEE bits
@@ -80,14 +81,19 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
                           const double2 *fv_old, double2 *fv_out, double2 *vf_out, double *bel) {
     const double keep = 1.0 - damping;
     if (F) {
-        if (damping != 0.0)
-            hipLaunchKernelGGL(factor_pass<true>, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, stream,
-                               fac_off, fac_meta, fac_p, fac_q, fac_mask, vf_in, fv_old, fv_out,
-                               (uint32_t)F, damping, keep, ctl, k, tol);
-        else
-            hipLaunchKernelGGL(factor_pass<false>, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, stream,
-                               fac_off, fac_meta, fac_p, fac_q, fac_mask, vf_in, fv_old, fv_out,
-                               (uint32_t)F, damping, keep, ctl, k, tol);
+        const dim3 grid(grid_for(F, kBlock)), block(kBlock);
+        const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
+#define FLBP_FACTOR_PASS(DAMP, N)                                                                     \
+    hipLaunchKernelGGL((factor_pass<DAMP, N>), grid, block, 0, stream, fac_off, fac_meta, fac_p, fac_q, \
+                       fac_mask, fac_vpos, vf_in, fv_old, fv_out, (uint32_t)F, damping, keep, ctl, k, tol)
+        if (damping != 0.0) {
+            if (narrow) FLBP_FACTOR_PASS(true, kFacRegDegSmall);
+            else FLBP_FACTOR_PASS(true, kFacRegDeg);
+        } else {
+            if (narrow) FLBP_FACTOR_PASS(false, kFacRegDegSmall);
+            else FLBP_FACTOR_PASS(false, kFacRegDeg);
+        }
+#undef FLBP_FACTOR_PASS
     }
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
     if (V)
@@ -313,8 +319,14 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
             fv_varmajor[2 * k + 1] = tmp[2 * s + 1];
         }
     }
-    if (vf_facmajor && P->E)
-        HIP_TRY(hipMemcpy(vf_facmajor, P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
+    if (vf_facmajor && P->E) {   /* stored var-major */
+        HIP_TRY(hipMemcpy(tmp.data(), P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
+        for (uint64_t s = 0; s < P->E; s++) {
+            const uint64_t k = P->h_fac_vpos[s];
+            vf_facmajor[2 * s] = tmp[2 * k];
+            vf_facmajor[2 * s + 1] = tmp[2 * k + 1];
+        }
+    }
     return FLBP_OK;
 }
 
@@ -331,8 +343,15 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
         }
         HIP_TRY(hipMemcpy(P->m_fv, tmp.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
     }
-    if (vf_facmajor && P->E)
-        HIP_TRY(hipMemcpy(P->m_vf, vf_facmajor, P->E * sizeof(double2), hipMemcpyHostToDevice));
+    if (vf_facmajor && P->E) {
+        std::vector<double> tmp(2 * P->E);
+        for (uint64_t s = 0; s < P->E; s++) {
+            const uint64_t k = P->h_fac_vpos[s];
+            tmp[2 * k] = vf_facmajor[2 * s];
+            tmp[2 * k + 1] = vf_facmajor[2 * s + 1];
+        }
+        HIP_TRY(hipMemcpy(P->m_vf, tmp.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
+    }
     return FLBP_OK;
 }
 

@@ -45,23 +45,33 @@ int validate_csr(const flbp_csr &c) {
     return FLBP_OK;
 }
 
-void var_device_order(const flbp_csr &c, std::vector<uint32_t> &perm, std::vector<uint32_t> &dev_off,
-                      std::vector<uint32_t> &dev_edge) {
+void var_device_order(const flbp_csr &c, bool by_degree, std::vector<uint32_t> &perm,
+                      std::vector<uint32_t> &dev_off, std::vector<uint32_t> &dev_edge,
+                      std::vector<uint32_t> &vpos) {
     const uint64_t V = c.n_vars;
     perm.resize(V);
-    uint64_t maxd = 0;
-    for (uint64_t i = 0; i < V; i++) maxd = std::max<uint64_t>(maxd, c.var_off[i + 1] - c.var_off[i]);
-    /* counting sort by degree, heaviest first, stable in variable index */
-    std::vector<uint64_t> start(maxd + 2, 0);
-    for (uint64_t i = 0; i < V; i++) start[maxd - (c.var_off[i + 1] - c.var_off[i]) + 1]++;
-    for (uint64_t b = 0; b <= maxd; b++) start[b + 1] += start[b];
-    for (uint64_t i = 0; i < V; i++) perm[start[maxd - (c.var_off[i + 1] - c.var_off[i])]++] = (uint32_t)i;
+    if (by_degree) {
+        uint64_t maxd = 0;
+        for (uint64_t i = 0; i < V; i++) maxd = std::max<uint64_t>(maxd, c.var_off[i + 1] - c.var_off[i]);
+        /* counting sort by degree, heaviest first, stable in variable index */
+        std::vector<uint64_t> start(maxd + 2, 0);
+        for (uint64_t i = 0; i < V; i++) start[maxd - (c.var_off[i + 1] - c.var_off[i]) + 1]++;
+        for (uint64_t b = 0; b <= maxd; b++) start[b + 1] += start[b];
+        for (uint64_t i = 0; i < V; i++) perm[start[maxd - (c.var_off[i + 1] - c.var_off[i])]++] = (uint32_t)i;
+    } else {
+        for (uint64_t i = 0; i < V; i++) perm[i] = (uint32_t)i;
+    }
     dev_off.assign(V + 1, 0);
     dev_edge.resize(c.n_edges);
+    vpos.resize(c.n_edges);
     for (uint64_t dv = 0; dv < V; dv++) {
         const uint64_t v = perm[dv], a = c.var_off[v], b = c.var_off[v + 1];
         dev_off[dv + 1] = dev_off[dv] + (uint32_t)(b - a);
-        for (uint64_t k = a; k < b; k++) dev_edge[dev_off[dv] + (k - a)] = (uint32_t)c.var_edge[k];
+        for (uint64_t k = a; k < b; k++) {
+            const uint32_t pos = dev_off[dv] + (uint32_t)(k - a);
+            dev_edge[pos] = (uint32_t)c.var_edge[k];
+            vpos[c.var_edge[k]] = pos;
+        }
     }
 }
 
@@ -101,6 +111,7 @@ int Plan::upload(const flbp_csr &c) {
     TRY(alloc(fac_mask, F));
     TRY(alloc(var_off, V + 1));
     TRY(alloc(var_edge, E));
+    TRY(alloc(fac_vpos, E));
     TRY(alloc(m_vf, E));
     TRY(alloc(m_fv, E));
     TRY(alloc(belief, V));
@@ -114,12 +125,16 @@ int Plan::upload(const flbp_csr &c) {
         u32[I] = dev::pack_meta((uint32_t)c.fac_type[I], c.head_clamped[I] != 0,
                                 (uint32_t)(c.head_slot[I] - c.fac_off[I]));
     if (F) HIP_TRY(hipMemcpy(fac_meta, u32.data(), F * 4, hipMemcpyHostToDevice));
-    /* variables in device order (heaviest first): each variable's edges keep
-     * their nbV order, so its arithmetic is unchanged */
+    /* variables in device order (label order keeps each variable near its
+     * defining factor); each variable's edges keep their nbV order, so its
+     * arithmetic is unchanged */
     std::vector<uint32_t> dev_off, dev_edge;
-    var_device_order(c, h_perm, dev_off, dev_edge);
+    var_device_order(c, false, h_perm, dev_off, dev_edge, h_fac_vpos);
     HIP_TRY(hipMemcpy(var_off, dev_off.data(), (V + 1) * 4, hipMemcpyHostToDevice));
-    if (E) HIP_TRY(hipMemcpy(var_edge, dev_edge.data(), E * 4, hipMemcpyHostToDevice));
+    if (E) {
+        HIP_TRY(hipMemcpy(var_edge, dev_edge.data(), E * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(fac_vpos, h_fac_vpos.data(), E * 4, hipMemcpyHostToDevice));
+    }
     if (F) {
         HIP_TRY(hipMemcpy(fac_p, c.p, F * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(fac_q, c.q, F * 8, hipMemcpyHostToDevice));

@@ -19,11 +19,13 @@ inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + bl
 
 int hip_fail(hipError_t e, const char *what);
 int validate_csr(const flbp_csr &c);
-/* device order of variables: perm[device index] = variable index, sorted by
- * degree (heaviest first, stable); dev_off/dev_edge = var_off/var_edge in
- * that order */
-void var_device_order(const flbp_csr &c, std::vector<uint32_t> &perm, std::vector<uint32_t> &dev_off,
-                      std::vector<uint32_t> &dev_edge);
+/* device order of variables: perm[device index] = variable index (label
+ * order, or by degree heaviest-first when by_degree); dev_off/dev_edge =
+ * var_off/var_edge in that order; vpos[slot] = the slot's var-major
+ * position in that order */
+void var_device_order(const flbp_csr &c, bool by_degree, std::vector<uint32_t> &perm,
+                      std::vector<uint32_t> &dev_off, std::vector<uint32_t> &dev_edge,
+                      std::vector<uint32_t> &vpos);
 
 struct Plan {
     int device = -1;
@@ -31,6 +33,7 @@ struct Plan {
     uint64_t V = 0, F = 0, E = 0;
 
     uint32_t *fac_off = nullptr, *fac_meta = nullptr, *var_off = nullptr, *var_edge = nullptr;
+    uint32_t *fac_vpos = nullptr;
     double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
     unsigned long long *ctl = nullptr;
@@ -41,6 +44,7 @@ struct Plan {
 
     std::vector<uint64_t> h_var_edge;              /* for message export/import */
     std::vector<uint32_t> h_perm;                  /* device variable -> variable */
+    std::vector<uint32_t> h_fac_vpos;              /* factor-major slot -> var-major position */
 
     uint64_t iters = 0;                            /* CausalBP::_iters */
     double last_maxdiff = INFINITY;

@@ -5,17 +5,19 @@
  * 879-886) is three per-edge/per-variable kernels, two belief kernels, a
  * thrust transform and a thrust reduce.  Here it is two kernels:
  *
- *   factor pass  -- one lane per factor: loads the factor's v->f messages
- *                   (contiguous, factor-major) into registers and writes every
- *                   f->v message of the factor (contiguous, factor-major),
+ *   factor pass  -- one lane per factor: gathers the factor's v->f messages
+ *                   (stored var-major) into registers through the factor's
+ *                   slot -> var-major position map and writes every f->v
+ *                   message of the factor contiguously (stored factor-major),
  *                   damping fused into the store.
  *   var pass     -- one lane per variable: gathers its f->v messages through
  *                   the var-major -> factor-major slot map, forms the product
- *                   with zero counts, the belief and |b - b_old|, then
- *                   scatters the leave-one-out v->f messages through the same
- *                   map; the max |b - b_old| is reduced in the wave (DPP
- *                   shuffles), across the block in LDS, and across the grid
- *                   with one 64-bit atomic max per block.
+ *                   with zero counts, the belief and |b - b_old|, then writes
+ *                   the leave-one-out v->f messages contiguously at its own
+ *                   var-major positions; the max |b - b_old| is reduced in the
+ *                   wave (DPP shuffles), across the block in LDS, and across
+ *                   the grid with one 64-bit atomic max per block.
+ * Both passes only gather and write contiguously: no scattered stores.
  *
  * Every message is computed with the reference's own operation order (the
  * per-output loops of calcMessageFVFusedAllKernel, csrc/kernel/
@@ -36,6 +38,7 @@ namespace flbp {
 namespace dev {
 
 constexpr int kFacRegDeg = 8;     /* factors up to this degree run from registers */
+constexpr int kFacRegDegSmall = 5; /* register slots when no factor is wider (more waves fit) */
 constexpr int kVarRegDeg = 8;     /* variables up to this degree keep messages in registers */
 constexpr int kBlock = 256;
 
@@ -74,9 +77,10 @@ struct RegIn {
     }
 };
 struct GlobalIn {
-    const double2 *__restrict__ m;
-    __device__ __forceinline__ double2 get(int j) const { return m[j]; }
-    __device__ __forceinline__ double2 pick(int j) const { return m[j]; }
+    const double2 *__restrict__ m;     /* message array */
+    const uint32_t *__restrict__ pos;  /* position of slot j in it */
+    __device__ __forceinline__ double2 get(int j) const { return m[pos[j]]; }
+    __device__ __forceinline__ double2 pick(int j) const { return m[pos[j]]; }
 };
 
 /* Runs f(j) for j = 0..d-1 in order; unrolled when the degree bound N is a
@@ -249,56 +253,126 @@ namespace {
 
 using namespace flbp::dev;
 
-/* one lane per factor; damping fused into the store.  m_fv_old may alias
- * m_fv (in place: each lane reads its own slots before writing them). */
-template <bool kDamp>
-__global__ __launch_bounds__(kBlock) void factor_pass(
-    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
-    const double *__restrict__ fac_p, const double *__restrict__ fac_q,
-    const double2 *__restrict__ fac_mask, const double2 *__restrict__ m_vf,
-    const double2 *m_fv_old, double2 *m_fv, uint32_t F, double damp, double keep,
-    const unsigned long long *__restrict__ ctl, int k, double tol) {
-    if (chunk_stopped(ctl, k, tol)) return;
-    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
-    if (f >= F) return;
-    const uint32_t s = fac_off[f];
-    const int d = (int)(fac_off[f + 1] - s);
-    const uint32_t meta = fac_meta[f];
+/* The update of one factor: gathers its v->f messages vf[pos[j]], j < d
+ * (registers up to N slots, global memory above), and writes every f->v
+ * message fv[t] (damped with fv_old[t] when kDamp) for its slots t. */
+template <int N, bool kDamp>
+__device__ __forceinline__ void factor_update(const double2 *__restrict__ vf,
+                                              const uint32_t *__restrict__ pos, const double2 *fv_old,
+                                              double2 *fv, int d, uint32_t meta, double p, double q,
+                                              double mask0, double mask1, double damp, double keep) {
     const uint32_t type = meta & 3u;
     const bool clamped = (meta & 4u) != 0;
     const int h = (int)(meta >> 8);
-    const double p = fac_p[f];
-    const double q = type ? fac_q[f] : 0.0;
-    double mask0 = 1.0, mask1 = 1.0;
-    if (clamped) {
-        const double2 mk = fac_mask[f];
-        mask0 = mk.x;
-        mask1 = mk.y;
-    }
     auto store = [&](int t, double2 r) __attribute__((always_inline)) {
         if (kDamp) {
-            const double2 old = m_fv_old[s + t];
+            const double2 old = fv_old[t];
             r.x = keep * r.x + damp * old.x;
             r.y = keep * r.y + damp * old.y;
         }
-        m_fv[s + t] = r;
+        fv[t] = r;
     };
-    if (d <= kFacRegDeg) {
-        RegIn<kFacRegDeg> in;
+    if (d <= N) {
+        RegIn<N> in;
+        uint32_t ps[N];
 #pragma unroll
-        for (int j = 0; j < kFacRegDeg; j++) in.m[j] = j < d ? m_vf[s + j] : make_double2(0.0, 0.0);
+        for (int j = 0; j < N; j++) ps[j] = j < d ? pos[j] : 0u;
+#pragma unroll
+        for (int j = 0; j < N; j++) in.m[j] = j < d ? vf[ps[j]] : make_double2(0.0, 0.0);
         const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
         for (int t = 0; t < d; t++)
-            store(t, fv_msg<kFacRegDeg>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
+            store(t, fv_msg<N>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
     } else {
-        GlobalIn in{m_vf + s};
+        GlobalIn in{vf, pos};
         const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
         for (int t = 0; t < d; t++)
             store(t, fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1));
     }
 }
 
-/* one lane per variable: product, belief, max-diff, leave-one-out scatter */
+/* one lane per factor; damping fused into the store.  m_fv_old may alias
+ * m_fv (in place: each lane reads its own slots before writing them). */
+template <bool kDamp, int N>
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass(
+    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
+    const double *__restrict__ fac_p, const double *__restrict__ fac_q,
+    const double2 *__restrict__ fac_mask, const uint32_t *__restrict__ fac_vpos,
+    const double2 *__restrict__ m_vf, const double2 *m_fv_old, double2 *m_fv, uint32_t F,
+    double damp, double keep, const unsigned long long *__restrict__ ctl, int k, double tol) {
+    if (chunk_stopped(ctl, k, tol)) return;
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= F) return;
+    const uint32_t s = fac_off[f];
+    const uint32_t meta = fac_meta[f];
+    const double p = fac_p[f];
+    const double q = (meta & 3u) ? fac_q[f] : 0.0;
+    double mask0 = 1.0, mask1 = 1.0;
+    if (meta & 4u) {
+        const double2 mk = fac_mask[f];
+        mask0 = mk.x;
+        mask1 = mk.y;
+    }
+    factor_update<N, kDamp>(m_vf, fac_vpos + s, m_fv_old + s, m_fv + s, (int)(fac_off[f + 1] - s), meta, p, q,
+                            mask0, mask1, damp, keep);
+}
+
+/* The update of one variable: gathers its f->v messages m_fv[ve[j]] for
+ * j < d (ve = its slice of the var-major -> factor-major slot map), forms
+ * the zero-counted product in edge order (calcProdAllKernel), the belief
+ * and |b - b_old| (getBeliefsVKernel, norm_functor, dist_functor), and
+ * writes the leave-one-out v->f messages at its var-major positions vf[j]
+ * (calcMessageVFAllKernel).  Returns |b - b_old| as IEEE bits. */
+__device__ __forceinline__ unsigned long long var_update(const uint32_t *__restrict__ ve,
+                                                         const double2 *__restrict__ m_fv,
+                                                         double2 *__restrict__ vf, double *bel, int d) {
+    double prod_0 = 1.0, prod_1 = 1.0;
+    uint32_t z0 = 0, z1 = 0;
+    uint32_t slot[kVarRegDeg];
+    double2 msg[kVarRegDeg];
+    auto accumulate = [&](double2 m) __attribute__((always_inline)) {
+        if (m.x == 0) z0++; else prod_0 *= m.x;
+        if (m.y == 0) z1++; else prod_1 *= m.y;
+        scale_pair(prod_0, prod_1);
+    };
+    /* gathers in unrolled chunks so up to kVarRegDeg loads are in flight;
+     * a variable of degree <= kVarRegDeg keeps its messages in registers */
+    for (int base = 0; base < d; base += kVarRegDeg) {
+#pragma unroll
+        for (int j = 0; j < kVarRegDeg; j++)
+            if (base + j < d) slot[j] = ve[base + j];
+#pragma unroll
+        for (int j = 0; j < kVarRegDeg; j++)
+            if (base + j < d) msg[j] = m_fv[slot[j]];
+#pragma unroll
+        for (int j = 0; j < kVarRegDeg; j++)
+            if (base + j < d) accumulate(msg[j]);
+    }
+    const double b0 = prod_0 * (double)(z0 == 0);
+    const double b1 = prod_1 * (double)(z1 == 0);
+    const double b = b1 / (b0 + b1);
+    const double diff = fabs(*bel - b);
+    *bel = b;
+    if (d <= kVarRegDeg) {
+#pragma unroll
+        for (int j = 0; j < kVarRegDeg; j++)
+            if (j < d) vf[j] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
+    } else {
+        for (int base = 0; base < d; base += kVarRegDeg) {
+#pragma unroll
+            for (int j = 0; j < kVarRegDeg; j++)
+                if (base + j < d) slot[j] = ve[base + j];
+#pragma unroll
+            for (int j = 0; j < kVarRegDeg; j++)
+                if (base + j < d) msg[j] = m_fv[slot[j]];
+#pragma unroll
+            for (int j = 0; j < kVarRegDeg; j++)
+                if (base + j < d) vf[base + j] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
+        }
+    }
+    return (unsigned long long)__double_as_longlong(diff);
+}
+
+/* one lane per variable */
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass(
     const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
     const double2 *__restrict__ m_fv, double2 *__restrict__ m_vf, double *__restrict__ belief,
@@ -309,65 +383,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass(
     unsigned long long diff_bits = 0;
     if (v < V) {
         const uint32_t s = var_off[v];
-        const int d = (int)(var_off[v + 1] - s);
-        double prod_0 = 1.0, prod_1 = 1.0;
-        uint32_t z0 = 0, z1 = 0;
-        uint32_t slot[kVarRegDeg];
-        double2 msg[kVarRegDeg];
-        auto accumulate = [&](double2 m) __attribute__((always_inline)) {
-            if (m.x == 0) z0++; else prod_0 *= m.x;
-            if (m.y == 0) z1++; else prod_1 *= m.y;
-            scale_pair(prod_0, prod_1);
-        };
-        if (d <= kVarRegDeg) {
-#pragma unroll
-            for (int j = 0; j < kVarRegDeg; j++)
-                if (j < d) slot[j] = var_edge[s + j];
-#pragma unroll
-            for (int j = 0; j < kVarRegDeg; j++)
-                if (j < d) msg[j] = m_fv[slot[j]];
-#pragma unroll
-            for (int j = 0; j < kVarRegDeg; j++)
-                if (j < d) accumulate(msg[j]);
-        } else {
-            /* high degree: gather in unrolled chunks so kVarRegDeg loads are
-             * in flight at once; the product still runs in edge order */
-            for (int base = 0; base < d; base += kVarRegDeg) {
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++)
-                    if (base + j < d) slot[j] = var_edge[s + base + j];
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++)
-                    if (base + j < d) msg[j] = m_fv[slot[j]];
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++)
-                    if (base + j < d) accumulate(msg[j]);
-            }
-        }
-        /* getBeliefsVKernel + norm_functor + dist_functor */
-        const double b0 = prod_0 * (double)(z0 == 0);
-        const double b1 = prod_1 * (double)(z1 == 0);
-        const double b = b1 / (b0 + b1);
-        const double diff = fabs(belief[v] - b);
-        belief[v] = b;
-        diff_bits = (unsigned long long)__double_as_longlong(diff);
-        if (d <= kVarRegDeg) {
-#pragma unroll
-            for (int j = 0; j < kVarRegDeg; j++)
-                if (j < d) m_vf[slot[j]] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
-        } else {
-            for (int base = 0; base < d; base += kVarRegDeg) {
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++)
-                    if (base + j < d) slot[j] = var_edge[s + base + j];
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++)
-                    if (base + j < d) msg[j] = m_fv[slot[j]];
-#pragma unroll
-                for (int j = 0; j < kVarRegDeg; j++)
-                    if (base + j < d) m_vf[slot[j]] = vf_msg(prod_0, prod_1, z0, z1, msg[j]);
-            }
-        }
+        diff_bits = var_update(var_edge + s, m_fv, m_vf + s, belief + v, (int)(var_off[v + 1] - s));
     }
     /* max over non-negative doubles == max over their bit patterns; a NaN
      * (positive after fabs) wins, which stops the loop as NaN > tol is false */
