@@ -41,7 +41,8 @@ struct Batch {
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
     Tile *ftiles = nullptr, *vtiles = nullptr;
     uint32_t n_ftiles = 0, n_vtiles = 0;
-    unsigned long long *diff = nullptr;     /* [kMaxChunk][G] */
+    unsigned long long *diff = nullptr;     /* [kMaxChunk][G][kBatchSlots][kCtlPad] partial maxima */
+    unsigned long long *diff_out = nullptr; /* [kMaxChunk][G] compacted */
     uint8_t *start_active = nullptr;        /* device [G] */
     uint32_t *budget = nullptr;             /* device [G] */
     unsigned long long *h_diff = nullptr;   /* pinned */
@@ -133,8 +134,9 @@ int Batch::build_tiles() {
 int Batch::record_chunk(int n, double tol, double damping) {
     const double keep = 1.0 - damping;
     BatchCtl c{diff, start_active, budget, (uint32_t)G, tol};
-    hipLaunchKernelGGL(batch_clear, dim3(std::max<uint64_t>(1, std::min<uint64_t>(1024, (n * G + 255) / 256))),
-                       dim3(256), 0, stream, diff, (uint64_t)n * G);
+    const uint64_t nslots = (uint64_t)n * G * kBatchSlots * kCtlPad;
+    hipLaunchKernelGGL(batch_clear, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(1024, (nslots + 255) / 256))),
+                       dim3(256), 0, stream, diff, nslots);
     const uint32_t fg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_ftiles));
     const uint32_t vg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_vtiles));
     for (int k = 0; k < n; k++) {
@@ -157,6 +159,9 @@ int Batch::record_chunk(int n, double tol, double damping) {
             hipLaunchKernelGGL(batch_var_pass, dim3(vg), dim3(kBlock), 0, stream, graphs, vtiles, n_vtiles,
                                var_off, var_edge, m_fv, m_vf, belief, c, k);
     }
+    const uint64_t waves = (uint64_t)n * G;
+    hipLaunchKernelGGL(batch_compact, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(1024, (waves * 64 + 255) / 256))),
+                       dim3(256), 0, stream, c, diff_out, n);
     return FLBP_OK;
 }
 
@@ -274,7 +279,8 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
     TRY(B->alloc(B->m_vf, E));
     TRY(B->alloc(B->m_fv, E));
     TRY(B->alloc(B->belief, V));
-    TRY(B->alloc(B->diff, (uint64_t)flbp::kMaxChunk * n_graphs));
+    TRY(B->alloc(B->diff, (uint64_t)flbp::kMaxChunk * n_graphs * kBatchSlots * kCtlPad));
+    TRY(B->alloc(B->diff_out, (uint64_t)flbp::kMaxChunk * n_graphs));
     TRY(B->alloc(B->start_active, n_graphs));
     TRY(B->alloc(B->budget, n_graphs));
     HIP_TRY(hipHostMalloc((void **)&B->h_diff, sizeof(unsigned long long) * flbp::kMaxChunk * n_graphs, hipHostMallocDefault));
@@ -364,7 +370,7 @@ int flbp_batch_run(flbp_batch *b, const flbp_run_opts *o, uint64_t *iters_out, d
         HIP_TRY(hipMemcpyAsync(B->start_active, B->h_active, G, hipMemcpyHostToDevice, B->stream));
         HIP_TRY(hipMemcpyAsync(B->budget, B->h_budget, G * 4, hipMemcpyHostToDevice, B->stream));
         TRY(B->enqueue_chunk(chunk, o->tol, o->damping));
-        HIP_TRY(hipMemcpyAsync(B->h_diff, B->diff, sizeof(unsigned long long) * chunk * G, hipMemcpyDeviceToHost,
+        HIP_TRY(hipMemcpyAsync(B->h_diff, B->diff_out, sizeof(unsigned long long) * chunk * G, hipMemcpyDeviceToHost,
                                B->stream));
         HIP_TRY(hipStreamSynchronize(B->stream));
         n_active = 0;

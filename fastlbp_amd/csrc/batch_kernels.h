@@ -38,20 +38,34 @@ struct Tile {
     uint32_t count;
 };
 
+/* per graph and sweep, kBatchSlots partial maxima one 64-byte line apart
+ * (tile i of a graph folds into slot i mod kBatchSlots) */
+constexpr int kBatchSlots = 8;
+
 struct BatchCtl {
-    unsigned long long *diff;       /* [chunk][G] max |db| bits per sweep */
+    unsigned long long *diff;       /* [chunk][G][kBatchSlots][kCtlPad] partial max |db| bits */
     const uint8_t *start_active;    /* [G] */
     const uint32_t *budget;         /* [G] sweeps left before maxiter */
     uint32_t G;
     double tol;
 };
 
+__device__ __forceinline__ unsigned long long *batch_slot(const BatchCtl &c, int k, uint32_t g, uint32_t j) {
+    return c.diff + (((uint64_t)k * c.G + g) * kBatchSlots + j) * kCtlPad;
+}
+
+/* max of graph g's slots for sweep k; all lanes of the wave get it */
+__device__ __forceinline__ unsigned long long batch_read(const BatchCtl &c, int k, uint32_t g) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long v =
+        lane < kBatchSlots ? __atomic_load_n(batch_slot(c, k, g, lane), __ATOMIC_RELAXED) : 0ull;
+    return wave_max_u64(v);
+}
+
 __device__ __forceinline__ bool graph_runs(const BatchCtl &c, uint32_t g, int k) {
     if (!c.start_active[g] || (uint32_t)k >= c.budget[g]) return false;
     if (k == 0) return true;
-    const double prev = __longlong_as_double(
-        (long long)__atomic_load_n(&c.diff[(uint64_t)(k - 1) * c.G + g], __ATOMIC_RELAXED));
-    return prev > c.tol;
+    return __longlong_as_double((long long)batch_read(c, k - 1, g)) > c.tol;
 }
 
 template <bool kDamp, int N>
@@ -117,9 +131,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass
             unsigned long long m = red[0];
 #pragma unroll
             for (int w = 1; w < kBlock / 64; w++) m = red[w] > m ? red[w] : m;
-            if (m) atomicMax(&ctl.diff[(uint64_t)k * ctl.G + tile.graph], m);
+            if (m) {
+                unsigned long long *slot = batch_slot(ctl, k, tile.graph, ti % kBatchSlots);
+                if (m > __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMax(slot, m);
+            }
         }
         __syncthreads();   /* red[] is reused by the next tile */
+    }
+}
+
+/* compact the chunk's partial maxima into out[k][g] for the host */
+__global__ __attribute__((unused)) void batch_compact(BatchCtl c, unsigned long long *out, int n) {
+    const uint64_t waves = (uint64_t)n * c.G;
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; w < waves;
+         w += ((uint64_t)gridDim.x * blockDim.x) / 64) {
+        const int k = (int)(w / c.G);
+        const uint32_t g = (uint32_t)(w % c.G);
+        const unsigned long long m = batch_read(c, k, g);
+        if ((threadIdx.x & 63) == 0) out[w] = m;
     }
 }
 

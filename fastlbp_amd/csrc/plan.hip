@@ -13,7 +13,8 @@
  *   m_vf     f64x2[E]   v->f messages, var-major, (state0, state1) pairs
  *   m_fv     f64x2[E]   f->v messages, factor-major
  *   belief   f64[V]     P(x=1) of the last sweep (the reference's _oldBeliefsV)
- *   ctl      u64[chunk] per-sweep max |b - b_old| as IEEE bits
+ *   ctl      u64[chunk][64 slots][8]  per-sweep partial max |b - b_old|
+ *            (IEEE bits), one 64-byte line per slot; ctl_out u64[chunk]
  *
  * Replaces the reference's per-iteration sequence in csrc/ext/causal_bp.cu:
  * calcNewMessageFusedAll (409-444), calcBeliefsV (483-515), the dist/max
@@ -41,8 +42,15 @@ namespace {
 /* zero the per-sweep slots of a chunk (a kernel, so the sequence can be
  * captured in a graph) */
 __global__ void clear_u64(unsigned long long *p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+/* one wave: the max over the slots of each of the chunk's n sweeps */
+__global__ void ctl_compact(const unsigned long long *ctl, unsigned long long *out, int n) {
+    for (int k = 0; k < n; k++) {
+        const unsigned long long m = ctl_read(ctl, k);
+        if (threadIdx.x == 0) out[k] = m;
+    }
 }
 
 __global__ void fill_pairs(double2 *p, uint64_t n, double c) {
@@ -107,13 +115,14 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
 /* Enqueue n sweeps (n <= kMaxChunk) and return how many ran before the stop
  * test fired, with the max diff of the last one that ran. */
 int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_diff) {
-    hipLaunchKernelGGL(clear_u64, dim3(1), dim3(kMaxChunk), 0, stream, ctl, kMaxChunk);
+    hipLaunchKernelGGL(clear_u64, dim3(16), dim3(256), 0, stream, ctl, kMaxChunk * kCtlPerSweep);
     if (profiling) HIP_TRY(hipEventRecord(ev[0], stream));
     for (int k = 0; k < n; k++) {
         int rc = launch_sweep(k, tol, damping);
         if (rc) return rc;
     }
-    HIP_TRY(hipMemcpyAsync(host_ctl, ctl, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, stream));
+    hipLaunchKernelGGL(ctl_compact, dim3(1), dim3(64), 0, stream, ctl, ctl_out, n);
+    HIP_TRY(hipMemcpyAsync(host_ctl, ctl_out, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     ran = n;
     for (int k = 0; k < n; k++) {

@@ -115,7 +115,8 @@ int Plan::upload(const flbp_csr &c) {
     TRY(alloc(m_vf, E));
     TRY(alloc(m_fv, E));
     TRY(alloc(belief, V));
-    TRY(alloc(ctl, kMaxChunk));
+    TRY(alloc(ctl, (uint64_t)kMaxChunk * dev::kCtlPerSweep));
+    TRY(alloc(ctl_out, kMaxChunk));
     HIP_TRY(hipHostMalloc((void **)&host_ctl, sizeof(unsigned long long) * kMaxChunk, hipHostMallocDefault));
 
     std::vector<uint32_t> u32(std::max(std::max(F, V), E) + 1);

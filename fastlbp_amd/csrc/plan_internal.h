@@ -36,7 +36,8 @@ struct Plan {
     uint32_t *fac_vpos = nullptr;
     double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
-    unsigned long long *ctl = nullptr;
+    unsigned long long *ctl = nullptr;             /* partial maxima, see sweep_kernels.h */
+    unsigned long long *ctl_out = nullptr;         /* per-sweep maxima of the last chunk */
     unsigned long long *host_ctl = nullptr;        /* pinned */
     std::vector<void *> allocs;
     uint64_t device_bytes = 0;

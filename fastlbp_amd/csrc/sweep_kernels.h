@@ -229,14 +229,6 @@ __device__ __forceinline__ double2 vf_msg(double prod_0, double prod_1, uint32_t
     return r;
 }
 
-/* stop test: sweep k of a chunk runs only if sweep k-1 left max diff > tol
- * (a slot that never ran reads 0 and stops every later sweep) */
-__device__ __forceinline__ bool chunk_stopped(const unsigned long long *ctl, int k, double tol) {
-    if (k <= 0) return false;
-    const double prev = __longlong_as_double((long long)__atomic_load_n(&ctl[k - 1], __ATOMIC_RELAXED));
-    return !(prev > tol);
-}
-
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -245,6 +237,38 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     }
     return v;
 }
+
+/* Per-sweep max |b - b_old| is kept in kCtlSlots partial maxima, one cache
+ * line apart: block b folds its maximum into slot b mod kCtlSlots, so the
+ * grid's atomics spread over many lines instead of queueing on one address
+ * (and are skipped when the slot already holds a larger value).  A reader
+ * takes the max of the slots with one wave-wide load and a DPP reduction. */
+constexpr int kCtlSlots = 64;   /* == wavefront size */
+constexpr int kCtlPad = 8;      /* u64 per slot: one 64-byte line */
+constexpr int kCtlPerSweep = kCtlSlots * kCtlPad;
+
+__device__ __forceinline__ void ctl_fold(unsigned long long *ctl, int k, unsigned long long m) {
+    if (!m) return;
+    unsigned long long *slot = ctl + (uint64_t)k * kCtlPerSweep + (blockIdx.x & (kCtlSlots - 1)) * kCtlPad;
+    if (m > __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMax(slot, m);
+}
+
+/* max over the slots of sweep k; every lane of the wave gets it */
+__device__ __forceinline__ unsigned long long ctl_read(const unsigned long long *ctl, int k) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long v =
+        __atomic_load_n(ctl + (uint64_t)k * kCtlPerSweep + lane * kCtlPad, __ATOMIC_RELAXED);
+    return wave_max_u64(v);
+}
+
+/* stop test: sweep k of a chunk runs only if sweep k-1 left max diff > tol
+ * (a sweep that never ran leaves its slots 0 and stops every later sweep) */
+__device__ __forceinline__ bool chunk_stopped(const unsigned long long *ctl, int k, double tol) {
+    if (k <= 0) return false;
+    const double prev = __longlong_as_double((long long)ctl_read(ctl, k - 1));
+    return !(prev > tol);
+}
+
 
 }  // namespace dev
 }  // namespace flbp
@@ -395,7 +419,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass(
         unsigned long long m = red[0];
 #pragma unroll
         for (int w = 1; w < kBlock / 64; w++) m = red[w] > m ? red[w] : m;
-        if (m) atomicMax(&ctl[k], m);
+        ctl_fold(ctl, k, m);
     }
 }
 
