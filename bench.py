@@ -192,6 +192,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["plain", "batched", "adjoint"], default="plain")
+    ap.add_argument("--variant", type=int, default=0, help="sweep kernels: 0 LDS-staged, 1 one lane per item")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 512 MiB buffer once, so HBM counters can be scaled by a known size")
     args = ap.parse_args()
@@ -222,6 +223,7 @@ def main():
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
     plan = fl.Plan(graph, device=0)
+    plan.set_variant(args.variant)
     t_up = time.perf_counter() - t0
     V, F, E = graph.n_vars, graph.n_factors, graph.n_edges
     log(f"[bench] graph V={V} F={F} E={E} generate+build {t_gen:.2f}s upload {t_up:.2f}s "
@@ -281,6 +283,7 @@ def main():
            "config": {"workload": w["name"], "n_vars": V, "n_factors": F, "n_edges": E,
                       "damping": w["damping"], "tol": w["tol"], "maxiter": w["maxiter"],
                       "seed": w["seed"], "sweeps_per_solve": iters_per_solve,
+                      "kernels": "lds-staged" if args.variant == 0 else "lane-per-item",
                       "step": "init + run() to convergence of the resident graph"},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

@@ -131,6 +131,7 @@ SIGNATURES = {
     "flbp_plan_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_plan_last_timing": (C.c_int, [C.c_void_p, f64p, f64p, u64p, u64p]),
     "flbp_plan_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "flbp_plan_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_batch_default_opts": (None, [C.POINTER(BatchOptsC)]),
     "flbp_batch_create": (C.c_int, [C.c_uint64, C.POINTER(C.POINTER(CsrC)), C.c_int, C.POINTER(BatchOptsC),
                                     C.POINTER(C.c_void_p)]),

@@ -33,6 +33,7 @@
 #include "fastlbp.h"
 #include "flbp_internal.h"
 #include "plan_internal.h"
+#include "staged_kernels.h"
 #include "sweep_kernels.h"
 
 using namespace flbp::dev;
@@ -88,25 +89,46 @@ int Plan::launch_sweep(int k, double tol, double damping) {
 int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_in,
                           const double2 *fv_old, double2 *fv_out, double2 *vf_out, double *bel) {
     const double keep = 1.0 - damping;
+    const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
+    static_assert(kStageItems == (uint32_t)kBlock && kStageSlots == (uint32_t)kStage, "staged block shape");
     if (F) {
-        const dim3 grid(grid_for(F, kBlock)), block(kBlock);
-        const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
+        if (variant == 0) {
+            const dim3 grid((unsigned)n_fac_blocks), block(kBlock);
+#define FLBP_FACTOR_STAGED(DAMP, N)                                                                    \
+    hipLaunchKernelGGL((factor_pass_staged<DAMP, N>), grid, block, 0, stream, fac_blk, fac_off, fac_meta, \
+                       fac_p, fac_q, fac_mask, fac_vpos, vf_in, fv_old, fv_out, damping, keep, ctl, k, tol)
+            if (damping != 0.0) {
+                if (narrow) FLBP_FACTOR_STAGED(true, kFacRegDegSmall);
+                else FLBP_FACTOR_STAGED(true, kFacRegDeg);
+            } else {
+                if (narrow) FLBP_FACTOR_STAGED(false, kFacRegDegSmall);
+                else FLBP_FACTOR_STAGED(false, kFacRegDeg);
+            }
+#undef FLBP_FACTOR_STAGED
+        } else {
+            const dim3 grid(grid_for(F, kBlock)), block(kBlock);
 #define FLBP_FACTOR_PASS(DAMP, N)                                                                     \
     hipLaunchKernelGGL((factor_pass<DAMP, N>), grid, block, 0, stream, fac_off, fac_meta, fac_p, fac_q, \
                        fac_mask, fac_vpos, vf_in, fv_old, fv_out, (uint32_t)F, damping, keep, ctl, k, tol)
-        if (damping != 0.0) {
-            if (narrow) FLBP_FACTOR_PASS(true, kFacRegDegSmall);
-            else FLBP_FACTOR_PASS(true, kFacRegDeg);
-        } else {
-            if (narrow) FLBP_FACTOR_PASS(false, kFacRegDegSmall);
-            else FLBP_FACTOR_PASS(false, kFacRegDeg);
-        }
+            if (damping != 0.0) {
+                if (narrow) FLBP_FACTOR_PASS(true, kFacRegDegSmall);
+                else FLBP_FACTOR_PASS(true, kFacRegDeg);
+            } else {
+                if (narrow) FLBP_FACTOR_PASS(false, kFacRegDegSmall);
+                else FLBP_FACTOR_PASS(false, kFacRegDeg);
+            }
 #undef FLBP_FACTOR_PASS
+        }
     }
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
-    if (V)
-        hipLaunchKernelGGL(var_pass, dim3(grid_for(V, kBlock)), dim3(kBlock), 0, stream, var_off,
-                           var_edge, fv_out, vf_out, bel, (uint32_t)V, ctl, k, tol);
+    if (V) {
+        if (variant == 0)
+            hipLaunchKernelGGL(var_pass_staged, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream, var_blk,
+                               var_off, var_edge, fv_out, vf_out, bel, ctl, k, tol);
+        else
+            hipLaunchKernelGGL(var_pass, dim3(grid_for(V, kBlock)), dim3(kBlock), 0, stream, var_off, var_edge,
+                               fv_out, vf_out, bel, (uint32_t)V, ctl, k, tol);
+    }
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 2], stream));
     HIP_TRY(hipGetLastError());
     return FLBP_OK;
@@ -392,6 +414,13 @@ int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms, uint6
     if (var_ms) *var_ms = P->prof_var_ms;
     if (fac_n) *fac_n = P->prof_fac_n;
     if (var_n) *var_n = P->prof_var_n;
+    return FLBP_OK;
+}
+
+int flbp_plan_set_variant(flbp_plan *plan, int variant) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || variant < 0 || variant > 1) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");
+    P->variant = variant;
     return FLBP_OK;
 }
 

@@ -75,6 +75,25 @@ void var_device_order(const flbp_csr &c, bool by_degree, std::vector<uint32_t> &
     }
 }
 
+void cut_blocks(const uint32_t *off, uint64_t n_items, uint32_t max_items, uint32_t max_slots,
+                std::vector<uint32_t> &blk) {
+    blk.clear();
+    blk.push_back(0);
+    uint64_t i = 0;
+    while (i < n_items) {
+        uint64_t j = i, slots = 0;
+        while (j < n_items && j - i < max_items) {
+            const uint64_t d = off[j + 1] - off[j];
+            if (j > i && slots + d > max_slots) break;
+            slots += d;
+            j++;
+            if (slots > max_slots) break;   /* a wide item stays alone */
+        }
+        blk.push_back((uint32_t)j);
+        i = j;
+    }
+}
+
 template <class T>
 int Plan::alloc(T *&p, uint64_t n) {
     const size_t bytes = std::max<uint64_t>(n, 1) * sizeof(T);
@@ -132,6 +151,18 @@ int Plan::upload(const flbp_csr &c) {
     std::vector<uint32_t> dev_off, dev_edge;
     var_device_order(c, false, h_perm, dev_off, dev_edge, h_fac_vpos);
     HIP_TRY(hipMemcpy(var_off, dev_off.data(), (V + 1) * 4, hipMemcpyHostToDevice));
+    {   /* staged-kernel block ranges */
+        std::vector<uint32_t> fo(F + 1), vb, fb;
+        for (uint64_t I = 0; I <= F; I++) fo[I] = (uint32_t)c.fac_off[I];
+        cut_blocks(fo.data(), F, kStageItems, kStageSlots, fb);
+        cut_blocks(dev_off.data(), V, kStageItems, kStageSlots, vb);
+        n_fac_blocks = fb.size() - 1;
+        n_var_blocks = vb.size() - 1;
+        TRY(alloc(fac_blk, fb.size()));
+        TRY(alloc(var_blk, vb.size()));
+        HIP_TRY(hipMemcpy(fac_blk, fb.data(), fb.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(var_blk, vb.data(), vb.size() * 4, hipMemcpyHostToDevice));
+    }
     if (E) {
         HIP_TRY(hipMemcpy(var_edge, dev_edge.data(), E * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(fac_vpos, h_fac_vpos.data(), E * 4, hipMemcpyHostToDevice));

@@ -14,6 +14,8 @@
 namespace flbp {
 
 constexpr int kMaxChunk = 64;   /* sweeps enqueued between host checks */
+constexpr uint32_t kStageItems = 256;    /* factors / variables per staged block (== kBlock) */
+constexpr uint32_t kStageSlots = 1024;   /* edge slots per staged block (== kStage) */
 
 inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
@@ -23,6 +25,10 @@ int validate_csr(const flbp_csr &c);
  * order, or by degree heaviest-first when by_degree); dev_off/dev_edge =
  * var_off/var_edge in that order; vpos[slot] = the slot's var-major
  * position in that order */
+/* ranges [blk[b], blk[b+1]) of at most max_items items and max_slots
+ * slots each (an item above max_slots alone) */
+void cut_blocks(const uint32_t *off, uint64_t n_items, uint32_t max_items, uint32_t max_slots,
+                std::vector<uint32_t> &blk);
 void var_device_order(const flbp_csr &c, bool by_degree, std::vector<uint32_t> &perm,
                       std::vector<uint32_t> &dev_off, std::vector<uint32_t> &dev_edge,
                       std::vector<uint32_t> &vpos);
@@ -34,6 +40,9 @@ struct Plan {
 
     uint32_t *fac_off = nullptr, *fac_meta = nullptr, *var_off = nullptr, *var_edge = nullptr;
     uint32_t *fac_vpos = nullptr;
+    uint32_t *fac_blk = nullptr, *var_blk = nullptr;   /* staged block ranges */
+    uint64_t n_fac_blocks = 0, n_var_blocks = 0;
+    int variant = 0;                                   /* 0 staged, 1 one lane per item */
     double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
     unsigned long long *ctl = nullptr;             /* partial maxima, see sweep_kernels.h */

@@ -274,6 +274,10 @@ class Plan:
         p = np.ascontiguousarray(p, dtype=np.float64)
         call("flbp_plan_set_p", self._h, _ptr(p, C.c_double), p.shape[0])
 
+    def set_variant(self, variant: int) -> None:
+        """0: LDS-staged sweep kernels (default); 1: one lane per factor/variable."""
+        call("flbp_plan_set_variant", self._h, variant)
+
     def profile(self, enable: bool = True) -> None:
         call("flbp_plan_profile", self._h, 1 if enable else 0)
 

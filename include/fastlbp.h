@@ -208,6 +208,11 @@ int flbp_plan_profile(flbp_plan *plan, int enable);
 int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
                           uint64_t *fac_launches, uint64_t *var_launches);
 
+/* Kernel shape of the sweep: 0 (default) LDS-staged blocks over balanced
+ * edge ranges, 1 one lane per factor / variable.  Results are identical;
+ * this is a tuning and diagnostic switch. */
+int flbp_plan_set_variant(flbp_plan *plan, int variant);
+
 uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
 
 /* ------------------------------------------------------------------ */

@@ -138,3 +138,51 @@ def test_c2_full_size_parity():
     g, plan, r, bg, o, it, md, bo = solve_both(gen.factors, 1e-9, 500, 0.3, threads=16)
     assert_parity(r, bg, it, md, bo)
     assert np.all((bg >= 0) & (bg <= 1))
+
+
+def solve_variant(flist, tol, maxiter, damping, variant):
+    g = fl.Graph(flist)
+    plan = fl.Plan(g)
+    plan.set_variant(variant)
+    plan.init()
+    r = plan.run(tol, maxiter, damping)
+    return r, plan.beliefs(), plan.messages()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_both_kernel_variants_match_oracle(variant):
+    gen = fl.generate(30000, 61, back_frac=0.3)
+    g, plan, r, bg, o, it, md, bo = solve_both(gen.factors, 1e-9, 400, 0.3)
+    rv, bv, (fv, vf) = solve_variant(gen.factors, 1e-9, 400, 0.3, variant)
+    assert rv.iters == it and same_bits(bv, bo)
+    ofv, ovf = o.messages()
+    assert same_bits(fv, ofv) and same_bits(vf, ovf)
+
+
+def wide_graph():
+    """A leaf used by 3000 ANDs (a variable with > 1024 edges), an OR over
+    1500 clauses (a factor with > 1024 slots), and ordinary structure."""
+    facs = [(0, 0, [], 0.7), (0, 1, [], 0.6)]
+    nxt = 10
+    clauses = []
+    for i in range(3000):
+        c = nxt; nxt += 1
+        facs.append((1, c, [0, 1] if i % 3 else [0], 0.9 + 0.0001 * (i % 50), 0.01 * (i % 4)))
+        if i < 1500:
+            clauses.append(c)
+    facs.append((2, nxt, clauses, 1.0, 0.0))
+    return fl.FactorList.build(facs)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_wide_factor_and_variable(variant):
+    flist = wide_graph()
+    g = fl.Graph(flist)
+    assert np.diff(g.fac_off).max() > 1024 and np.diff(g.var_off).max() > 1024
+    o = oracle.OracleGraph(flist)
+    o.init()
+    it, md = o.run(1e-12, 200, 0.2)
+    rv, bv, (fv, vf) = solve_variant(flist, 1e-12, 200, 0.2, variant)
+    assert rv.iters == it and same_bits(bv, o.beliefs())
+    ofv, ovf = o.messages()
+    assert same_bits(fv, ofv) and same_bits(vf, ovf)
