@@ -92,18 +92,21 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
     const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
     static_assert(kStageItems == (uint32_t)kBlock && kStageSlots == (uint32_t)kStage, "staged block shape");
     if (F) {
-        if (variant == 0) {
+        if (variant == 0 || variant == 2) {
             const dim3 grid((unsigned)n_fac_blocks), block(kBlock);
-#define FLBP_FACTOR_STAGED(DAMP, N)                                                                    \
-    hipLaunchKernelGGL((factor_pass_staged<DAMP, N>), grid, block, 0, stream, fac_blk, fac_off, fac_meta, \
+#define FLBP_FACTOR_STAGED(DAMP, N, PULL)                                                                    \
+    hipLaunchKernelGGL((factor_pass_staged<DAMP, N, PULL>), grid, block, 0, stream, fac_blk, fac_off, fac_meta, \
                        fac_p, fac_q, fac_mask, fac_vpos, vf_in, fv_old, fv_out, damping, keep, ctl, k, tol)
+#define FLBP_FACTOR_STAGED_N(DAMP, N) \
+    do { if (variant == 0) FLBP_FACTOR_STAGED(DAMP, N, true); else FLBP_FACTOR_STAGED(DAMP, N, false); } while (0)
             if (damping != 0.0) {
-                if (narrow) FLBP_FACTOR_STAGED(true, kFacRegDegSmall);
-                else FLBP_FACTOR_STAGED(true, kFacRegDeg);
+                if (narrow) FLBP_FACTOR_STAGED_N(true, kFacRegDegSmall);
+                else FLBP_FACTOR_STAGED_N(true, kFacRegDeg);
             } else {
-                if (narrow) FLBP_FACTOR_STAGED(false, kFacRegDegSmall);
-                else FLBP_FACTOR_STAGED(false, kFacRegDeg);
+                if (narrow) FLBP_FACTOR_STAGED_N(false, kFacRegDegSmall);
+                else FLBP_FACTOR_STAGED_N(false, kFacRegDeg);
             }
+#undef FLBP_FACTOR_STAGED_N
 #undef FLBP_FACTOR_STAGED
         } else {
             const dim3 grid(grid_for(F, kBlock)), block(kBlock);
@@ -123,8 +126,11 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
     if (V) {
         if (variant == 0)
-            hipLaunchKernelGGL(var_pass_staged, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream, var_blk,
+            hipLaunchKernelGGL(var_pass_staged<true>, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream, var_blk,
                                var_off, var_edge, fv_out, vf_out, bel, ctl, k, tol);
+        else if (variant == 2)
+            hipLaunchKernelGGL(var_pass_staged<false>, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream,
+                               var_blk, var_off, var_edge, fv_out, vf_out, bel, ctl, k, tol);
         else
             hipLaunchKernelGGL(var_pass, dim3(grid_for(V, kBlock)), dim3(kBlock), 0, stream, var_off, var_edge,
                                fv_out, vf_out, bel, (uint32_t)V, ctl, k, tol);
@@ -132,6 +138,20 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 2], stream));
     HIP_TRY(hipGetLastError());
     return FLBP_OK;
+}
+
+/* Size of the next chunk of a run: sweeps that follow the stop test only
+ * exit at once, but each still costs a launch, so once the max belief change
+ * falls geometrically the chunk is cut to the sweeps that the decay rate
+ * says are left (plus two), else it doubles up to kMaxChunk.  The stop test
+ * is evaluated per sweep on the device, so the chunking changes no result. */
+static int next_chunk(int chunk, double d_prev, double d_last, double tol) {
+    const int grow = std::min(chunk * 2, flbp::kMaxChunk);
+    if (d_last > tol && d_last < d_prev && d_last > 0 && std::isfinite(d_prev) && tol > 0) {
+        const double est = std::log(tol / d_last) / std::log(d_last / d_prev);
+        if (est >= 0 && est < flbp::kMaxChunk) return std::max(4, std::min((int)std::ceil(est) + 2, grow));
+    }
+    return grow;
 }
 
 /* Enqueue n sweeps (n <= kMaxChunk) and return how many ran before the stop
@@ -150,6 +170,7 @@ int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_di
     for (int k = 0; k < n; k++) {
         double dk;
         std::memcpy(&dk, &host_ctl[k], sizeof dk);
+        prev_diff = last_diff;
         last_diff = dk;
         if (!(dk > tol)) {
             ran = k + 1;
@@ -251,13 +272,14 @@ int flbp_plan_run(flbp_plan *plan, const flbp_run_opts *o, flbp_run_result *res)
         const int n = (int)std::min<uint64_t>((uint64_t)chunk, o->maxiter - P->iters);
         int ran = 0;
         double last = maxdiff;
+        P->prev_diff = INFINITY;
         int rc = P->run_chunk(n, o->tol, o->damping, ran, last);
         if (rc) return rc;
         P->iters += ran;
         sweeps += ran;
         maxdiff = last;
         P->swept_since_init = true;
-        if (adaptive) chunk = std::min(chunk * 2, flbp::kMaxChunk);
+        if (adaptive) chunk = flbp::next_chunk(chunk, P->prev_diff, maxdiff, o->tol);
     }
     P->last_maxdiff = maxdiff;
     if (res) {
@@ -350,10 +372,10 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
             fv_varmajor[2 * k + 1] = tmp[2 * s + 1];
         }
     }
-    if (vf_facmajor && P->E) {   /* stored var-major */
+    if (vf_facmajor && P->E) {   /* stored var-major, or factor-major in the push layout */
         HIP_TRY(hipMemcpy(tmp.data(), P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
         for (uint64_t s = 0; s < P->E; s++) {
-            const uint64_t k = P->h_fac_vpos[s];
+            const uint64_t k = P->variant == 2 ? s : P->h_fac_vpos[s];
             vf_facmajor[2 * s] = tmp[2 * k];
             vf_facmajor[2 * s + 1] = tmp[2 * k + 1];
         }
@@ -377,7 +399,7 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
     if (vf_facmajor && P->E) {
         std::vector<double> tmp(2 * P->E);
         for (uint64_t s = 0; s < P->E; s++) {
-            const uint64_t k = P->h_fac_vpos[s];
+            const uint64_t k = P->variant == 2 ? s : P->h_fac_vpos[s];
             tmp[2 * k] = vf_facmajor[2 * s];
             tmp[2 * k + 1] = vf_facmajor[2 * s + 1];
         }
@@ -419,7 +441,21 @@ int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms, uint6
 
 int flbp_plan_set_variant(flbp_plan *plan, int variant) {
     Plan *P = reinterpret_cast<Plan *>(plan);
-    if (!P || variant < 0 || variant > 1) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");
+    if (!P || variant < 0 || variant > 2) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");
+    const bool was_push = P->variant == 2, push = variant == 2;
+    if (was_push != push && P->E) {
+        /* reorder the stored v->f messages to the new layout */
+        std::vector<double> a(2 * P->E), b(2 * P->E);
+        HIP_TRY(hipSetDevice(P->device));
+        HIP_TRY(hipMemcpy(a.data(), P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
+        for (uint64_t s = 0; s < P->E; s++) {
+            const uint64_t k = P->h_fac_vpos[s];
+            const uint64_t from = push ? k : s, to = push ? s : k;
+            b[2 * to] = a[2 * from];
+            b[2 * to + 1] = a[2 * from + 1];
+        }
+        HIP_TRY(hipMemcpy(P->m_vf, b.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
+    }
     P->variant = variant;
     return FLBP_OK;
 }

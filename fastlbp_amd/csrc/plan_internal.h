@@ -75,6 +75,7 @@ struct Plan {
     int launch_sweep_on(int k, double tol, double damping, const double2 *vf_in,
                         const double2 *fv_old, double2 *fv_out, double2 *vf_out, double *bel);
     int run_chunk(int n, double tol, double damping, int &ran, double &last_diff);
+    double prev_diff = INFINITY;   /* max diff of the sweep before the last one run_chunk saw */
     ~Plan();
 
     template <class T>

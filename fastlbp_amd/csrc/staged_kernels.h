@@ -52,7 +52,9 @@ __device__ __forceinline__ void block_fold_diff(unsigned long long *ctl, int k, 
     }
 }
 
-template <bool kDamp, int N>
+/* kPull: v->f messages stored var-major, gathered through fac_vpos; else
+ * stored factor-major (read contiguously) */
+template <bool kDamp, int N, bool kPull>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_staged(
     const uint32_t *__restrict__ blk, const uint32_t *__restrict__ fac_off,
     const uint32_t *__restrict__ fac_meta, const double *__restrict__ fac_p,
@@ -80,6 +82,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
             mask0 = mk.x;
             mask1 = mk.y;
         }
+        if (!kPull) {
+            LdsIn in{m_vf + base};   /* plain pointer read; works on global memory too */
+            const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
+            for (uint32_t t = tid; t < n; t += kBlock) {
+                double2 r = fv_msg<0>(in, type, clamped, (int)n, (int)t, h, mh, mo, p, q, mask0, mask1);
+                if (kDamp) {
+                    const double2 old = m_fv_old[base + t];
+                    r.x = keep * r.x + damp * old.x;
+                    r.y = keep * r.y + damp * old.y;
+                }
+                m_fv[base + t] = r;
+            }
+            return;
+        }
         GlobalIn in{m_vf, fac_vpos + base};
         const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
         for (uint32_t t = tid; t < n; t += kBlock) {
@@ -94,7 +110,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
         return;
     }
     for (uint32_t e = tid; e < n; e += kBlock) {
-        s_in[e] = m_vf[fac_vpos[base + e]];
+        s_in[e] = kPull ? m_vf[fac_vpos[base + e]] : m_vf[base + e];
         if (kDamp) s_out[e] = m_fv_old[base + e];
     }
     __syncthreads();
@@ -140,6 +156,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
     for (uint32_t e = tid; e < n; e += kBlock) m_fv[base + e] = s_out[e];
 }
 
+/* kPull: write v->f messages var-major (contiguous); else scatter them to
+ * their factor-major slots */
+template <bool kPull>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_staged(
     const uint32_t *__restrict__ blk, const uint32_t *__restrict__ var_off,
     const uint32_t *__restrict__ var_edge, const double2 *__restrict__ m_fv, double2 *__restrict__ m_vf,
@@ -191,8 +210,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
         prod_1 = s_prod[1];
         z0 = s_z[0];
         z1 = s_z[1];
-        for (uint32_t e = tid; e < n; e += kBlock)
-            m_vf[base + e] = vf_msg(prod_0, prod_1, z0, z1, m_fv[var_edge[base + e]]);
+        for (uint32_t e = tid; e < n; e += kBlock) {
+            const uint32_t sl = var_edge[base + e];
+            m_vf[kPull ? base + e : sl] = vf_msg(prod_0, prod_1, z0, z1, m_fv[sl]);
+        }
         block_fold_diff(ctl, k, diff_bits, red);
         return;
     }
@@ -219,7 +240,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
         for (int j = 0; j < d; j++) s_m[s + j] = vf_msg(prod_0, prod_1, z0, z1, s_m[s + j]);
     }
     __syncthreads();
-    for (uint32_t e = tid; e < n; e += kBlock) m_vf[base + e] = s_m[e];
+    for (uint32_t e = tid; e < n; e += kBlock) m_vf[kPull ? base + e : var_edge[base + e]] = s_m[e];
     block_fold_diff(ctl, k, diff_bits, red);
 }
 

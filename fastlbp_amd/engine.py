@@ -275,7 +275,10 @@ class Plan:
         call("flbp_plan_set_p", self._h, _ptr(p, C.c_double), p.shape[0])
 
     def set_variant(self, variant: int) -> None:
-        """0: LDS-staged sweep kernels (default); 1: one lane per factor/variable."""
+        """Kernel shape: 0 LDS-staged, v->f messages var-major (default);
+        1 one lane per factor/variable; 2 LDS-staged with v->f messages
+        factor-major (push).  Results are identical; switching to or from 2
+        reorders the stored messages."""
         call("flbp_plan_set_variant", self._h, variant)
 
     def profile(self, enable: bool = True) -> None:

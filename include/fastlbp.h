@@ -209,8 +209,12 @@ int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
                           uint64_t *fac_launches, uint64_t *var_launches);
 
 /* Kernel shape of the sweep: 0 (default) LDS-staged blocks over balanced
- * edge ranges, 1 one lane per factor / variable.  Results are identical;
- * this is a tuning and diagnostic switch. */
+ * edge ranges with v->f messages stored var-major (both passes gather, all
+ * stores contiguous); 1 one lane per factor / variable, same layout; 2
+ * LDS-staged with v->f messages stored factor-major (the factor pass reads
+ * contiguously, the variable pass scatters).  Results are identical; this
+ * is a tuning and diagnostic switch (switching to or from 2 reorders the
+ * stored messages; the adjoint needs 0 or 1). */
 int flbp_plan_set_variant(flbp_plan *plan, int variant);
 
 uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
