@@ -192,7 +192,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["plain", "batched", "adjoint"], default="plain")
-    ap.add_argument("--variant", type=int, default=0, help="sweep kernels: 0 LDS-staged (pull), 1 one lane per item, 2 LDS-staged (push)")
+    ap.add_argument("--variant", type=int, default=3, help="sweep kernels: 0 LDS-staged (pull), 1 one lane per item, 2 LDS-staged (push), "
+                         "3 LDS-staged (scatter)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 512 MiB buffer once, so HBM counters can be scaled by a known size")
     args = ap.parse_args()
@@ -283,7 +284,7 @@ def main():
            "config": {"workload": w["name"], "n_vars": V, "n_factors": F, "n_edges": E,
                       "damping": w["damping"], "tol": w["tol"], "maxiter": w["maxiter"],
                       "seed": w["seed"], "sweeps_per_solve": iters_per_solve,
-                      "kernels": ["lds-staged-pull", "lane-per-item", "lds-staged-push"][args.variant],
+                      "kernels": ["lds-staged-pull", "lane-per-item", "lds-staged-push", "lds-staged-scatter"][args.variant],
                       "step": "init + run() to convergence of the resident graph"},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

@@ -106,8 +106,10 @@ int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_
         return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: bad arguments");
     if (!(damping >= 0.0 && damping < 1.0))
         return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: damping must be in [0, 1)");
-    if (P->variant == 2)
-        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: the adjoint needs the pull layout (variant 0 or 1)");
+    if (P->variant >= 2) {   /* the adjoint walks the pull layout: move the plan's messages there */
+        const int rc = flbp_plan_set_variant(plan, 0);
+        if (rc) return rc;
+    }
     if (P->max_fac_deg > (uint64_t)kAdjFacDeg)
         return flbp::fail(FLBP_ERANGE, "flbp_adjoint_create: factor degree above 6");
     HIP_TRY(hipSetDevice(P->device));
@@ -155,8 +157,8 @@ void flbp_adjoint_destroy(flbp_adjoint *adj) { delete reinterpret_cast<Adjoint *
 int flbp_adjoint_forward(flbp_adjoint *adj, double *beliefs) {
     Adjoint *A = reinterpret_cast<Adjoint *>(adj);
     if (!A) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_forward: null adjoint");
-    if (A->P->variant == 2)
-        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_forward: the plan was switched to the push layout (variant 2)");
+    if (A->P->variant >= 2)
+        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_forward: the plan was switched away from the pull layout");
     Plan *P = A->P;
     HIP_TRY(hipSetDevice(P->device));
     const uint64_t E = P->E;
@@ -193,8 +195,8 @@ int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp)
     Adjoint *A = reinterpret_cast<Adjoint *>(adj);
     if (!A || !dl_db || !dl_dp) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: null argument");
     if (!A->have_forward) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: run the forward first");
-    if (A->P->variant == 2)
-        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: the plan was switched to the push layout (variant 2)");
+    if (A->P->variant >= 2)
+        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: the plan was switched away from the pull layout");
     Plan *P = A->P;
     HIP_TRY(hipSetDevice(P->device));
     const uint64_t E = P->E, V = P->V, F = P->F;

@@ -10,8 +10,11 @@
  *   fac_vpos u32[E]     var-major position of each factor-major slot
  *   var_off  u32[V+1]   var-major edge ranges (h_row_ptr_fv)
  *   var_edge u32[E]     factor-major slot of each var-major edge
- *   m_vf     f64x2[E]   v->f messages, var-major, (state0, state1) pairs
- *   m_fv     f64x2[E]   f->v messages, factor-major
+ *   m_vf     f64x2[E]   v->f messages, (state0, state1) pairs: var-major
+ *                       (variants 0, 1) or factor-major (2, 3)
+ *   m_fv     f64x2[E]   f->v messages, factor-major (in variant 3 a copy kept
+ *                       for damping; fm_stale marks it out of date)
+ *   m_fv_vm  f64x2[E]   f->v messages, var-major (variant 3)
  *   belief   f64[V]     P(x=1) of the last sweep (the reference's _oldBeliefsV)
  *   ctl      u64[chunk][64 slots][8]  per-sweep partial max |b - b_old|
  *            (IEEE bits), one 64-byte line per slot; ctl_out u64[chunk]
@@ -59,6 +62,13 @@ __global__ void fill_pairs(double2 *p, uint64_t n, double c) {
     if (i < n) p[i] = make_double2(c, c);
 }
 
+/* factor-major f->v copy from the var-major array */
+__global__ void gather_pairs(const double2 *__restrict__ src, const uint32_t *__restrict__ pos,
+                             double2 *__restrict__ dst, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[pos[i]];
+}
+
 /* beliefs straight after init with no sweep: prod = (c, c), zero counts 0 */
 __global__ void fill_f64(double *p, uint64_t n, double c) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,6 +90,16 @@ int hip_fail(hipError_t e, const char *what) {
     } while (0)
 
 int Plan::launch_sweep(int k, double tol, double damping) {
+    if (variant == 3) {
+        /* undamped sweeps of the scatter layout skip the factor-major f->v
+         * copy; a damped sweep first rebuilds it from the var-major array */
+        if (damping != 0.0 && fm_stale && E) {
+            hipLaunchKernelGGL(gather_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_fv_vm, fac_vpos,
+                               m_fv, E);
+            fm_stale = false;
+        }
+        if (damping == 0.0) fm_stale = true;
+    }
     return launch_sweep_on(k, tol, damping, m_vf, m_fv, m_fv, m_vf, belief);
 }
 
@@ -92,13 +112,18 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
     const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
     static_assert(kStageItems == (uint32_t)kBlock && kStageSlots == (uint32_t)kStage, "staged block shape");
     if (F) {
-        if (variant == 0 || variant == 2) {
+        if (variant != 1) {
             const dim3 grid((unsigned)n_fac_blocks), block(kBlock);
-#define FLBP_FACTOR_STAGED(DAMP, N, PULL)                                                                    \
-    hipLaunchKernelGGL((factor_pass_staged<DAMP, N, PULL>), grid, block, 0, stream, fac_blk, fac_off, fac_meta, \
-                       fac_p, fac_q, fac_mask, fac_vpos, vf_in, fv_old, fv_out, damping, keep, ctl, k, tol)
-#define FLBP_FACTOR_STAGED_N(DAMP, N) \
-    do { if (variant == 0) FLBP_FACTOR_STAGED(DAMP, N, true); else FLBP_FACTOR_STAGED(DAMP, N, false); } while (0)
+#define FLBP_FACTOR_STAGED(DAMP, N, LAYOUT)                                                                    \
+    hipLaunchKernelGGL((factor_pass_staged<DAMP, N, LAYOUT>), grid, block, 0, stream, fac_blk, fac_off, fac_meta, \
+                       fac_p, fac_q, fac_mask, fac_vpos, fac_work, vf_in, fv_old, fv_out, m_fv_vm, damping, keep, \
+                       ctl, k, tol)
+#define FLBP_FACTOR_STAGED_N(DAMP, N)                                                   \
+    do {                                                                                \
+        if (variant == 0) FLBP_FACTOR_STAGED(DAMP, N, kLayoutPull);                     \
+        else if (variant == 2) FLBP_FACTOR_STAGED(DAMP, N, kLayoutPush);                \
+        else FLBP_FACTOR_STAGED(DAMP, N, kLayoutScatter);                               \
+    } while (0)
             if (damping != 0.0) {
                 if (narrow) FLBP_FACTOR_STAGED_N(true, kFacRegDegSmall);
                 else FLBP_FACTOR_STAGED_N(true, kFacRegDeg);
@@ -125,12 +150,16 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
     }
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
     if (V) {
+        const dim3 vgrid((unsigned)n_var_blocks), vblock(kBlock);
         if (variant == 0)
-            hipLaunchKernelGGL(var_pass_staged<true>, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream, var_blk,
-                               var_off, var_edge, fv_out, vf_out, bel, ctl, k, tol);
+            hipLaunchKernelGGL(var_pass_staged<kLayoutPull>, vgrid, vblock, 0, stream, var_blk, var_off, var_edge,
+                               var_work, fv_out, vf_out, bel, ctl, k, tol);
         else if (variant == 2)
-            hipLaunchKernelGGL(var_pass_staged<false>, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream,
-                               var_blk, var_off, var_edge, fv_out, vf_out, bel, ctl, k, tol);
+            hipLaunchKernelGGL(var_pass_staged<kLayoutPush>, vgrid, vblock, 0, stream, var_blk, var_off, var_edge,
+                               var_work, fv_out, vf_out, bel, ctl, k, tol);
+        else if (variant == 3)
+            hipLaunchKernelGGL(var_pass_staged<kLayoutScatter>, vgrid, vblock, 0, stream, var_blk, var_off,
+                               var_edge, var_work, m_fv_vm, vf_out, bel, ctl, k, tol);
         else
             hipLaunchKernelGGL(var_pass, dim3(grid_for(V, kBlock)), dim3(kBlock), 0, stream, var_off, var_edge,
                                fv_out, vf_out, bel, (uint32_t)V, ctl, k, tol);
@@ -245,9 +274,11 @@ int flbp_plan_init(flbp_plan *plan, int logdomain) {
     if (P->E) {
         hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_vf, P->E, c);
         hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_fv, P->E, c);
+        hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_fv_vm, P->E, c);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(P->stream));
+    P->fm_stale = false;
     P->init_value = c;
     P->swept_since_init = false;
     return FLBP_OK;
@@ -365,9 +396,11 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
     HIP_TRY(hipSetDevice(P->device));
     std::vector<double> tmp(2 * P->E);
     if (fv_varmajor && P->E) {
-        HIP_TRY(hipMemcpy(tmp.data(), P->m_fv, P->E * sizeof(double2), hipMemcpyDeviceToHost));
+        /* factor-major, or var-major in device variable order (variant 3) */
+        const bool vm = P->variant == 3;
+        HIP_TRY(hipMemcpy(tmp.data(), vm ? P->m_fv_vm : P->m_fv, P->E * sizeof(double2), hipMemcpyDeviceToHost));
         for (uint64_t k = 0; k < P->E; k++) {
-            const uint64_t s = P->h_var_edge[k];
+            const uint64_t s = vm ? P->h_fac_vpos[P->h_var_edge[k]] : P->h_var_edge[k];
             fv_varmajor[2 * k] = tmp[2 * s];
             fv_varmajor[2 * k + 1] = tmp[2 * s + 1];
         }
@@ -375,7 +408,7 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
     if (vf_facmajor && P->E) {   /* stored var-major, or factor-major in the push layout */
         HIP_TRY(hipMemcpy(tmp.data(), P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
         for (uint64_t s = 0; s < P->E; s++) {
-            const uint64_t k = P->variant == 2 ? s : P->h_fac_vpos[s];
+            const uint64_t k = P->variant >= 2 ? s : P->h_fac_vpos[s];
             vf_facmajor[2 * s] = tmp[2 * k];
             vf_facmajor[2 * s + 1] = tmp[2 * k + 1];
         }
@@ -395,11 +428,18 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
             tmp[2 * s + 1] = fv_varmajor[2 * k + 1];
         }
         HIP_TRY(hipMemcpy(P->m_fv, tmp.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
+        for (uint64_t k = 0; k < P->E; k++) {   /* the var-major copy, device variable order */
+            const uint64_t d = P->h_fac_vpos[P->h_var_edge[k]];
+            tmp[2 * d] = fv_varmajor[2 * k];
+            tmp[2 * d + 1] = fv_varmajor[2 * k + 1];
+        }
+        HIP_TRY(hipMemcpy(P->m_fv_vm, tmp.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
+        P->fm_stale = false;
     }
     if (vf_facmajor && P->E) {
         std::vector<double> tmp(2 * P->E);
         for (uint64_t s = 0; s < P->E; s++) {
-            const uint64_t k = P->variant == 2 ? s : P->h_fac_vpos[s];
+            const uint64_t k = P->variant >= 2 ? s : P->h_fac_vpos[s];
             tmp[2 * k] = vf_facmajor[2 * s];
             tmp[2 * k + 1] = vf_facmajor[2 * s + 1];
         }
@@ -441,20 +481,16 @@ int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms, uint6
 
 int flbp_plan_set_variant(flbp_plan *plan, int variant) {
     Plan *P = reinterpret_cast<Plan *>(plan);
-    if (!P || variant < 0 || variant > 2) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");
-    const bool was_push = P->variant == 2, push = variant == 2;
-    if (was_push != push && P->E) {
-        /* reorder the stored v->f messages to the new layout */
-        std::vector<double> a(2 * P->E), b(2 * P->E);
-        HIP_TRY(hipSetDevice(P->device));
-        HIP_TRY(hipMemcpy(a.data(), P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
-        for (uint64_t s = 0; s < P->E; s++) {
-            const uint64_t k = P->h_fac_vpos[s];
-            const uint64_t from = push ? k : s, to = push ? s : k;
-            b[2 * to] = a[2 * from];
-            b[2 * to + 1] = a[2 * from + 1];
-        }
-        HIP_TRY(hipMemcpy(P->m_vf, b.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
+    if (!P || variant < 0 || variant > 3) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");
+    auto layout = [](int v) { return v == 1 ? 0 : v; };
+    if (layout(P->variant) != layout(variant) && P->E) {
+        /* move the stored messages to the new layout through their
+         * reference-order form */
+        std::vector<double> fv(2 * P->E), vf(2 * P->E);
+        int rc = flbp_plan_get_messages(plan, fv.data(), vf.data());
+        if (rc) return rc;
+        P->variant = variant;
+        return flbp_plan_set_messages(plan, fv.data(), vf.data());
     }
     P->variant = variant;
     return FLBP_OK;

@@ -94,6 +94,43 @@ void cut_blocks(const uint32_t *off, uint64_t n_items, uint32_t max_items, uint3
     }
 }
 
+void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const std::vector<uint32_t> &fb,
+                       const uint32_t *var_off, const std::vector<uint32_t> &vb, std::vector<uint32_t> &fac_work,
+                       std::vector<uint16_t> &var_work) {
+    const uint64_t E = fac_off[fb.back()];
+    fac_work.assign(E, 0);
+    std::vector<std::pair<uint64_t, uint32_t>> keyed;
+    for (size_t b = 0; b + 1 < fb.size(); b++) {
+        const uint32_t f0 = fb[b], f1 = fb[b + 1], base = fac_off[f0];
+        const uint32_t n = fac_off[f1] - base;
+        if (n > kStageSlots) continue;   /* a wide factor: no work order */
+        keyed.clear();
+        for (uint32_t f = f0; f < f1; f++) {
+            const uint32_t d = fac_off[f + 1] - fac_off[f];
+            const uint32_t type = fac_meta[f] & 3u, h = fac_meta[f] >> 8;
+            for (uint32_t t = 0; t < d; t++) {
+                const uint32_t kind = type == 0 ? 0u : (t == h ? 1u : 2u);
+                const uint32_t sl = fac_off[f] - base + t;
+                keyed.push_back({((uint64_t)kind << 40) | ((uint64_t)d << 20) | sl,
+                                 sl | ((f - f0) << 10)});
+            }
+        }
+        std::sort(keyed.begin(), keyed.end());
+        for (uint32_t i = 0; i < n; i++) fac_work[base + i] = keyed[i].second;
+    }
+    const uint64_t V = vb.back();
+    var_work.assign(V, 0);
+    std::vector<std::pair<uint64_t, uint16_t>> vk;
+    for (size_t b = 0; b + 1 < vb.size(); b++) {
+        const uint32_t v0 = vb[b], v1 = vb[b + 1];
+        vk.clear();
+        for (uint32_t v = v0; v < v1; v++)
+            vk.push_back({((uint64_t)(var_off[v + 1] - var_off[v]) << 20) | (v - v0), (uint16_t)(v - v0)});
+        std::sort(vk.begin(), vk.end());
+        for (uint32_t i = 0; i < v1 - v0; i++) var_work[v0 + i] = vk[i].second;
+    }
+}
+
 template <class T>
 int Plan::alloc(T *&p, uint64_t n) {
     const size_t bytes = std::max<uint64_t>(n, 1) * sizeof(T);
@@ -133,6 +170,7 @@ int Plan::upload(const flbp_csr &c) {
     TRY(alloc(fac_vpos, E));
     TRY(alloc(m_vf, E));
     TRY(alloc(m_fv, E));
+    TRY(alloc(m_fv_vm, E));
     TRY(alloc(belief, V));
     TRY(alloc(ctl, (uint64_t)kMaxChunk * dev::kCtlPerSweep));
     TRY(alloc(ctl_out, kMaxChunk));
@@ -141,10 +179,11 @@ int Plan::upload(const flbp_csr &c) {
     std::vector<uint32_t> u32(std::max(std::max(F, V), E) + 1);
     for (uint64_t I = 0; I <= F; I++) u32[I] = (uint32_t)c.fac_off[I];
     HIP_TRY(hipMemcpy(fac_off, u32.data(), (F + 1) * 4, hipMemcpyHostToDevice));
+    std::vector<uint32_t> meta(F);
     for (uint64_t I = 0; I < F; I++)
-        u32[I] = dev::pack_meta((uint32_t)c.fac_type[I], c.head_clamped[I] != 0,
-                                (uint32_t)(c.head_slot[I] - c.fac_off[I]));
-    if (F) HIP_TRY(hipMemcpy(fac_meta, u32.data(), F * 4, hipMemcpyHostToDevice));
+        meta[I] = dev::pack_meta((uint32_t)c.fac_type[I], c.head_clamped[I] != 0,
+                                 (uint32_t)(c.head_slot[I] - c.fac_off[I]));
+    if (F) HIP_TRY(hipMemcpy(fac_meta, meta.data(), F * 4, hipMemcpyHostToDevice));
     /* variables in device order (label order keeps each variable near its
      * defining factor); each variable's edges keep their nbV order, so its
      * arithmetic is unchanged */
@@ -162,6 +201,13 @@ int Plan::upload(const flbp_csr &c) {
         TRY(alloc(var_blk, vb.size()));
         HIP_TRY(hipMemcpy(fac_blk, fb.data(), fb.size() * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(var_blk, vb.data(), vb.size() * 4, hipMemcpyHostToDevice));
+        std::vector<uint32_t> fw;
+        std::vector<uint16_t> vw;
+        stage_work_orders(fo.data(), meta.data(), fb, dev_off.data(), vb, fw, vw);
+        TRY(alloc(fac_work, E));
+        TRY(alloc(var_work, V));
+        if (E) HIP_TRY(hipMemcpy(fac_work, fw.data(), E * 4, hipMemcpyHostToDevice));
+        if (V) HIP_TRY(hipMemcpy(var_work, vw.data(), V * 2, hipMemcpyHostToDevice));
     }
     if (E) {
         HIP_TRY(hipMemcpy(var_edge, dev_edge.data(), E * 4, hipMemcpyHostToDevice));
@@ -180,6 +226,7 @@ int Plan::upload(const flbp_csr &c) {
      * device_vectors are */
     HIP_TRY(hipMemset(m_vf, 0, std::max<uint64_t>(E, 1) * 16));
     HIP_TRY(hipMemset(m_fv, 0, std::max<uint64_t>(E, 1) * 16));
+    HIP_TRY(hipMemset(m_fv_vm, 0, std::max<uint64_t>(E, 1) * 16));
     h_var_edge.assign(c.var_edge, c.var_edge + E);
     for (uint64_t I = 0; I < F; I++) max_fac_deg = std::max<uint64_t>(max_fac_deg, c.fac_off[I + 1] - c.fac_off[I]);
     for (uint64_t i = 0; i < V; i++) max_var_deg = std::max<uint64_t>(max_var_deg, c.var_off[i + 1] - c.var_off[i]);

@@ -29,6 +29,13 @@ int validate_csr(const flbp_csr &c);
  * slots each (an item above max_slots alone) */
 void cut_blocks(const uint32_t *off, uint64_t n_items, uint32_t max_items, uint32_t max_slots,
                 std::vector<uint32_t> &blk);
+/* work orders of the staged kernels (see staged_kernels.h): for each block
+ * of at most kStageSlots slots, fac_work lists its slots (local slot | local
+ * factor << 10) grouped by output kind (prior, head, body) and factor
+ * degree; var_work lists its variables (local index) by degree */
+void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const std::vector<uint32_t> &fb,
+                       const uint32_t *var_off, const std::vector<uint32_t> &vb, std::vector<uint32_t> &fac_work,
+                       std::vector<uint16_t> &var_work);
 void var_device_order(const flbp_csr &c, bool by_degree, std::vector<uint32_t> &perm,
                       std::vector<uint32_t> &dev_off, std::vector<uint32_t> &dev_edge,
                       std::vector<uint32_t> &vpos);
@@ -41,10 +48,14 @@ struct Plan {
     uint32_t *fac_off = nullptr, *fac_meta = nullptr, *var_off = nullptr, *var_edge = nullptr;
     uint32_t *fac_vpos = nullptr;
     uint32_t *fac_blk = nullptr, *var_blk = nullptr;   /* staged block ranges */
+    uint32_t *fac_work = nullptr;   /* per staged factor block: its output slots in work order */
+    uint16_t *var_work = nullptr;   /* per staged variable block: its variables in work order */
     uint64_t n_fac_blocks = 0, n_var_blocks = 0;
-    int variant = 0;                                   /* 0 staged, 1 one lane per item */
+    int variant = 3;   /* 0 staged pull, 1 one lane per item, 2 staged push, 3 staged scatter */
     double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
+    double2 *m_fv_vm = nullptr;    /* var-major f->v messages (variant 3) */
+    bool fm_stale = false;         /* variant 3: factor-major f->v copy out of date */
     unsigned long long *ctl = nullptr;             /* partial maxima, see sweep_kernels.h */
     unsigned long long *ctl_out = nullptr;         /* per-sweep maxima of the last chunk */
     unsigned long long *host_ctl = nullptr;        /* pinned */

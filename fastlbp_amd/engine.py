@@ -275,10 +275,11 @@ class Plan:
         call("flbp_plan_set_p", self._h, _ptr(p, C.c_double), p.shape[0])
 
     def set_variant(self, variant: int) -> None:
-        """Kernel shape: 0 LDS-staged, v->f messages var-major (default);
-        1 one lane per factor/variable; 2 LDS-staged with v->f messages
-        factor-major (push).  Results are identical; switching to or from 2
-        reorders the stored messages."""
+        """Kernel shape and message layout: 0 LDS-staged, v->f var-major and
+        f->v factor-major (default); 1 one lane per factor/variable, same
+        layout; 2 LDS-staged, both factor-major (push); 3 LDS-staged, v->f
+        factor-major and f->v var-major (both passes scatter).  Results are
+        identical; switching layouts moves the stored messages."""
         call("flbp_plan_set_variant", self._h, variant)
 
     def profile(self, enable: bool = True) -> None:

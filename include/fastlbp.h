@@ -208,13 +208,15 @@ int flbp_plan_profile(flbp_plan *plan, int enable);
 int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
                           uint64_t *fac_launches, uint64_t *var_launches);
 
-/* Kernel shape of the sweep: 0 (default) LDS-staged blocks over balanced
- * edge ranges with v->f messages stored var-major (both passes gather, all
- * stores contiguous); 1 one lane per factor / variable, same layout; 2
- * LDS-staged with v->f messages stored factor-major (the factor pass reads
- * contiguously, the variable pass scatters).  Results are identical; this
- * is a tuning and diagnostic switch (switching to or from 2 reorders the
- * stored messages; the adjoint needs 0 or 1). */
+/* Kernel shape and message layout of the sweep.  3 (default): LDS-staged
+ * blocks over balanced edge ranges, v->f messages factor-major and f->v
+ * var-major, so both passes read contiguously and scatter their outputs
+ * (plus a factor-major f->v copy written by damped sweeps); 0: LDS-staged,
+ * v->f var-major and f->v factor-major (both passes gather, all stores
+ * contiguous); 1: one lane per factor / variable in layout 0; 2:
+ * LDS-staged, both arrays factor-major.  Results are identical; switching
+ * between layouts moves the stored messages, so a solve may continue under
+ * another variant.  flbp_adjoint_create switches its plan to 0. */
 int flbp_plan_set_variant(flbp_plan *plan, int variant);
 
 uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
@@ -269,8 +271,10 @@ uint64_t flbp_batch_device_bytes(const flbp_batch *b);
  * backward recomputes each segment from its checkpoint and runs the
  * hand-written reverse kernels of the factor and variable passes.  The
  * adjoint works on its plan's device buffers (the forward overwrites the
- * plan's messages and beliefs).  Factors of degree above 6 are rejected
- * (FLBP_ERANGE).  Destroy an adjoint before its plan. */
+ * plan's messages and beliefs; creation switches the plan to kernel
+ * variant 0, whose message layout the reverse kernels read, and the plan
+ * must stay there while the adjoint is used).  Factors of degree above 6
+ * are rejected (FLBP_ERANGE).  Destroy an adjoint before its plan. */
 typedef struct flbp_adjoint flbp_adjoint;
 
 int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_every,

@@ -149,7 +149,7 @@ def solve_variant(flist, tol, maxiter, damping, variant):
     return r, plan.beliefs(), plan.messages()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_both_kernel_variants_match_oracle(variant):
     gen = fl.generate(30000, 61, back_frac=0.3)
     g, plan, r, bg, o, it, md, bo = solve_both(gen.factors, 1e-9, 400, 0.3)
@@ -165,12 +165,14 @@ def test_variant_switch_mid_run_keeps_state():
     gen = fl.generate(20000, 62, back_frac=0.2)
     o = oracle.OracleGraph(gen.factors)
     o.init()
-    o.run_fixed(12, 0.3)
     plan = fl.Plan(fl.Graph(gen.factors))
     plan.init()
-    for variant, n in ((0, 3), (2, 4), (1, 2), (2, 1), (0, 2)):
+    segments = ((0, 3, 0.3), (2, 4, 0.3), (3, 2, 0.3), (1, 2, 0.3), (3, 2, 0.0), (3, 2, 0.3), (2, 1, 0.0),
+                (3, 1, 0.0), (0, 2, 0.3))
+    for variant, n, damping in segments:
         plan.set_variant(variant)
-        plan.run_fixed(n, 0.3)
+        plan.run_fixed(n, damping)
+        o.run_fixed(n, damping)
     fv, vf = plan.messages()
     ofv, ovf = o.messages()
     assert same_bits(plan.beliefs(), o.beliefs())
@@ -192,7 +194,7 @@ def wide_graph():
     return fl.FactorList.build(facs)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_wide_factor_and_variable(variant):
     flist = wide_graph()
     g = fl.Graph(flist)
