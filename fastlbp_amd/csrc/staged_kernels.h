@@ -33,6 +33,7 @@ namespace {
 using namespace flbp::dev;
 
 constexpr int kStage = 1024;    /* slots staged per block (16 KiB per message array) */
+constexpr int kPerLane = kStage / kBlock;   /* staged slots per lane */
 
 struct LdsIn {
     const double2 *m;
@@ -126,29 +127,52 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
                                         m_fv_old, m_fv, m_fv_vm, damp, keep);
         return;
     }
+    /* phase 1: the block's work entries into registers, its factors'
+     * offsets and parameters into LDS, and its messages into LDS, all loads
+     * independent of each other */
+    __shared__ uint32_t s_start[kBlock + 1];   /* factor offsets relative to base */
+    __shared__ uint32_t s_meta[kBlock];
+    __shared__ double s_p[kBlock], s_q[kBlock];
+    const uint32_t nf = f1 - f0;
+    uint32_t w[kPerLane];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        w[j] = i < n ? fac_work[base + i] : 0u;
+    }
+    if ((uint32_t)tid < nf) {
+        s_start[tid] = fac_off[f0 + tid] - base;
+        s_meta[tid] = fac_meta[f0 + tid];
+        s_p[tid] = fac_p[f0 + tid];
+        s_q[tid] = fac_q[f0 + tid];
+    }
+    if (tid == 0) s_start[nf] = n;
     for (uint32_t e = tid; e < n; e += kBlock) {
         s_in[e] = kLayout == kLayoutPull ? m_vf[fac_vpos[base + e]] : m_vf[base + e];
         if (kDamp) s_out[e] = m_fv_old[base + e];
     }
     __syncthreads();
-    /* one lane per output slot, in the block's work order:
-     * fac_work[base + i] = local slot | local factor << 10 */
-    for (uint32_t i = tid; i < n; i += kBlock) {
-        const uint32_t w = fac_work[base + i];
-        const uint32_t sl = w & 1023u;
-        const uint32_t f = f0 + (w >> 10);
-        const uint32_t s = fac_off[f] - base;
-        const int d = (int)(fac_off[f + 1] - base - s);
+    /* phase 2: one lane per output slot, in the block's work order
+     * (w = local slot | local factor << 10), from LDS only */
+#pragma unroll 1
+    for (int j = 0; j < kPerLane; j++) {
+        if (tid + j * kBlock >= n) break;
+        uint32_t wj = 0;
+#pragma unroll
+        for (int x = 0; x < kPerLane; x++) wj |= (j == x) ? w[x] : 0u;
+        const uint32_t sl = wj & 1023u, lf = wj >> 10;
+        const uint32_t s = s_start[lf];
+        const int d = (int)(s_start[lf + 1] - s);
         const int t = (int)(sl - s);
-        const uint32_t meta = fac_meta[f];
+        const uint32_t meta = s_meta[lf];
         const uint32_t type = meta & 3u;
         const bool clamped = (meta & 4u) != 0;
         const int h = (int)(meta >> 8);
-        const double p = fac_p[f];
-        const double q = type ? fac_q[f] : 0.0;
+        const double p = s_p[lf];
+        const double q = type ? s_q[lf] : 0.0;
         double mask0 = 1.0, mask1 = 1.0;
         if (clamped) {
-            const double2 mk = fac_mask[f];
+            const double2 mk = fac_mask[f0 + lf];
             mask0 = mk.x;
             mask1 = mk.y;
         }
@@ -156,7 +180,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
         if (d <= N) {
             RegIn<N> in;
 #pragma unroll
-            for (int j = 0; j < N; j++) in.m[j] = j < d ? s_in[s + j] : make_double2(0.0, 0.0);
+            for (int jj = 0; jj < N; jj++) in.m[jj] = jj < d ? s_in[s + jj] : make_double2(0.0, 0.0);
             const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
             r = fv_msg<N>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
         } else {
@@ -241,15 +265,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
         block_fold_diff(ctl, k, diff_bits, red);
         return;
     }
+    /* phase 1: offsets, old beliefs and the work entry of each variable,
+     * and the block's messages, all loads independent of each other */
+    __shared__ uint32_t s_start[kBlock + 1];
+    __shared__ double s_bel[kBlock];
+    const uint32_t nv = v1 - v0;
+    uint32_t lv = 0;
+    if ((uint32_t)tid < nv) {
+        lv = var_work[v0 + tid];
+        s_start[tid] = var_off[v0 + tid] - base;
+        s_bel[tid] = belief[v0 + tid];
+    }
+    if (tid == 0) s_start[nv] = n;
     for (uint32_t e = tid; e < n; e += kBlock)
         s_m[e] = kLayout == kLayoutScatter ? m_fv[base + e] : m_fv[var_edge[base + e]];
     __syncthreads();
-    /* products: one lane per variable, in the block's work order (by degree) */
-    if (tid < v1 - v0) {
-        const uint32_t lv = var_work[v0 + tid];
-        const uint32_t v = v0 + lv;
-        const uint32_t s = var_off[v] - base;
-        const int d = (int)(var_off[v + 1] - base - s);
+    /* phase 2: products, one lane per variable in the block's work order
+     * (by degree), from LDS only */
+    if ((uint32_t)tid < nv) {
+        const uint32_t s = s_start[lv];
+        const int d = (int)(s_start[lv + 1] - s);
         double prod_0 = 1.0, prod_1 = 1.0;
         uint32_t z0 = 0, z1 = 0;
         for (int j = 0; j < d; j++) {
@@ -262,8 +297,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
         const double b0 = prod_0 * (double)(z0 == 0);
         const double b1 = prod_1 * (double)(z1 == 0);
         const double b = b1 / (b0 + b1);
-        const double diff = fabs(belief[v] - b);
-        belief[v] = b;
+        const double diff = fabs(s_bel[lv] - b);
+        s_bel[lv] = b;
         diff_bits = (unsigned long long)__double_as_longlong(diff);
         s_prod[0][lv] = prod_0;
         s_prod[1][lv] = prod_1;
@@ -278,6 +313,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
         const double2 r = vf_msg(s_prod[0][lv], s_prod[1][lv], z & 0xffffu, z >> 16, s_m[e]);
         m_vf[kLayout == kLayoutPull ? base + e : var_edge[base + e]] = r;
     }
+    if ((uint32_t)tid < nv) belief[v0 + tid] = s_bel[tid];
     block_fold_diff(ctl, k, diff_bits, red);
 }
 
