@@ -6,12 +6,14 @@
  * the message state (m_vf, m_fv) is copied to checkpoint c.
  * Backward: for segments c = last..0, recompute sweeps c*k+1..c*k+k from
  * checkpoint c into per-sweep state slots (out of place), then walk the
- * segment backwards with var_vjp and fac_vjp (adjoint_kernels.h).  Device
- * memory (m_vf and the adjoint of m_fv var-major, m_fv and the adjoint of
- * m_vf factor-major, so every kernel gathers and writes contiguously):
- * ceil(n/k) checkpoints and k state slots of 32 B per edge each,
- * three 16 B-per-edge adjoint arrays, and a 64 B-per-edge scratch row when
- * some variable has more than 4 edges.
+ * segment backwards with var_vjp_staged and fac_vjp_staged
+ * (adjoint_kernels.h) over the adjoint's own block ranges (at most
+ * kAdjStage slots each) and work orders.  Device memory (m_vf and the
+ * adjoint of m_fv var-major, m_fv and the adjoint of m_vf factor-major, so
+ * every kernel gathers and writes contiguously): ceil(n/k) checkpoints and
+ * k state slots of 32 B per edge each, three 16 B-per-edge adjoint arrays,
+ * and a 64 B-per-edge scratch row when some variable has more than
+ * kAdjStage edges.
  */
 #include <hip/hip_runtime.h>
 
@@ -50,6 +52,9 @@ struct Adjoint {
     double damp = 0.0;
     std::vector<double2 *> ck_vf, ck_fv, st_vf, st_fv;
     double2 *avf = nullptr, *avf2 = nullptr, *afv = nullptr, *gscratch = nullptr;
+    uint32_t *fac_blk = nullptr, *var_blk = nullptr;    /* block ranges of the reverse kernels */
+    uint16_t *fac_work = nullptr, *var_work = nullptr;  /* per block: factors by type and degree, variables by degree */
+    uint64_t n_fac_blocks = 0, n_var_blocks = 0;
     double *pbar = nullptr, *bbar = nullptr, *scratch_belief = nullptr;
     std::vector<void *> allocs;
     uint64_t bytes = 0;
@@ -69,6 +74,50 @@ struct Adjoint {
         allocs.push_back(q);
         bytes += b;
         p = static_cast<T_ *>(q);
+        return FLBP_OK;
+    }
+    /* block ranges and work orders of the staged reverse kernels, from the
+     * plan's device-order CSR */
+    int build_blocks() {
+        const uint64_t F = P->F, V = P->V;
+        std::vector<uint32_t> fo(F + 1), meta(F), vo(V + 1);
+        if (hipMemcpy(fo.data(), P->fac_off, (F + 1) * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            (F && hipMemcpy(meta.data(), P->fac_meta, F * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
+            hipMemcpy(vo.data(), P->var_off, (V + 1) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(FLBP_EHIP, "adjoint: reading the plan's offsets failed");
+        std::vector<uint32_t> fb, vb;
+        cut_blocks(fo.data(), F, kBlock, kAdjStage, fb);
+        cut_blocks(vo.data(), V, kBlock, kAdjStage, vb);
+        std::vector<uint16_t> fw(F), vw(V);
+        std::vector<std::pair<uint64_t, uint16_t>> key;
+        for (size_t b = 0; b + 1 < fb.size(); b++) {
+            key.clear();
+            for (uint32_t f = fb[b]; f < fb[b + 1]; f++) {
+                const uint64_t d = fo[f + 1] - fo[f], prior = (meta[f] & 3u) == 0;
+                key.push_back({(prior << 40) | (d << 20) | (f - fb[b]), (uint16_t)(f - fb[b])});
+            }
+            std::sort(key.begin(), key.end());
+            for (size_t i = 0; i < key.size(); i++) fw[fb[b] + i] = key[i].second;
+        }
+        for (size_t b = 0; b + 1 < vb.size(); b++) {
+            key.clear();
+            for (uint32_t v = vb[b]; v < vb[b + 1]; v++)
+                key.push_back({((uint64_t)(vo[v + 1] - vo[v]) << 20) | (v - vb[b]), (uint16_t)(v - vb[b])});
+            std::sort(key.begin(), key.end());
+            for (size_t i = 0; i < key.size(); i++) vw[vb[b] + i] = key[i].second;
+        }
+        n_fac_blocks = fb.size() - 1;
+        n_var_blocks = vb.size() - 1;
+        int rc = alloc(fac_blk, fb.size());
+        if (!rc) rc = alloc(var_blk, vb.size());
+        if (!rc) rc = alloc(fac_work, F);
+        if (!rc) rc = alloc(var_work, V);
+        if (rc) return rc;
+        if (hipMemcpy(fac_blk, fb.data(), fb.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(var_blk, vb.data(), vb.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            (F && hipMemcpy(fac_work, fw.data(), F * 2, hipMemcpyHostToDevice) != hipSuccess) ||
+            (V && hipMemcpy(var_work, vw.data(), V * 2, hipMemcpyHostToDevice) != hipSuccess))
+            return fail(FLBP_EHIP, "adjoint: uploading block ranges failed");
         return FLBP_OK;
     }
     ~Adjoint() {
@@ -138,7 +187,8 @@ int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_
     if (!rc) rc = A->alloc(A->avf, E);
     if (!rc) rc = A->alloc(A->avf2, E);
     if (!rc) rc = A->alloc(A->afv, E);
-    if (!rc) rc = A->alloc(A->gscratch, P->max_var_deg > (uint64_t)kAdjVarDeg ? 4 * E : 1);
+    if (!rc) rc = A->alloc(A->gscratch, P->max_var_deg > (uint64_t)kAdjStage ? 4 * E : 1);
+    if (!rc) rc = A->build_blocks();
     if (!rc) rc = A->alloc(A->pbar, P->F);
     if (!rc) rc = A->alloc(A->bbar, P->V);
     if (!rc) rc = A->alloc(A->scratch_belief, P->V);
@@ -224,13 +274,13 @@ int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp)
                                    vf_state(t), A->scratch_belief));
         for (uint64_t t = t1; t > t0; t--) {
             if (V)
-                hipLaunchKernelGGL(var_vjp, dim3(grid_for(V, kBlock)), dim3(kBlock), 0, st, P->var_off,
-                                   P->var_edge, fv_state(t), A->avf, t == A->T ? A->bbar : nullptr, A->afv,
-                                   A->gscratch, A->damp, (uint32_t)V);
+                hipLaunchKernelGGL(var_vjp_staged, dim3((unsigned)A->n_var_blocks), dim3(kBlock), 0, st, A->var_blk,
+                                   P->var_off, P->var_edge, A->var_work, fv_state(t), A->avf,
+                                   t == A->T ? A->bbar : nullptr, A->afv, A->gscratch, A->damp);
             if (F)
-                hipLaunchKernelGGL(fac_vjp, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, st, P->fac_off,
-                                   P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, P->fac_vpos, vf_state(t - 1),
-                                   A->afv, A->avf2, A->pbar, keep, (uint32_t)F);
+                hipLaunchKernelGGL(fac_vjp_staged, dim3((unsigned)A->n_fac_blocks), dim3(kBlock), 0, st, A->fac_blk,
+                                   P->fac_off, P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, P->fac_vpos,
+                                   A->fac_work, vf_state(t - 1), A->afv, A->avf2, A->pbar, keep);
             std::swap(A->avf, A->avf2);
         }
         HIP_TRY(hipGetLastError());

@@ -29,10 +29,10 @@ namespace {
 
 using namespace flbp::dev;
 
-constexpr int kAdjFacDeg = 6;
-constexpr int kAdjVarDeg = 4;
+constexpr int kAdjFacDeg = 6;   /* largest factor degree of the reverse kernels */
+constexpr int kAdjVarDeg = 4;   /* var_vjp_one keeps variables up to this degree in registers */
 
-/* Reverse of the variable pass of one sweep, one lane per variable.
+/* Reverse of the variable pass for one variable v, run by one lane.
  * m_fv: f->v messages of the sweep (factor-major); avf: adjoint of the
  * sweep's v->f outputs (factor-major); bbar: adjoint of its beliefs (NULL
  * when the loss does not read them); afv (var-major, written at the
@@ -46,33 +46,35 @@ constexpr int kAdjVarDeg = 4;
  * U_k = sum_{e>k} g_e prod_{j>k, j!=e} m_j (backward): adjoint = W_k Q_k +
  * P_k U_k + G P_k Q_k.  Variables above kAdjVarDeg keep P, Q, g, U in a
  * lane-owned scratch row (4 pairs per edge). */
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_vjp(
+__device__ __forceinline__ void gadj(double L0, double L1, double2 ob, double &g0, double &g1) {
+    /* adjoint of the unnormalized pair L from the adjoint ob of L / sum(L) */
+    const double S = L0 + L1;
+    g0 = g1 = 0.0;
+    if (S != 0.0) {
+        const double dot = ob.x * (L0 / S) + ob.y * (L1 / S);
+        g0 = (ob.x - dot) / S;
+        g1 = (ob.y - dot) / S;
+    }
+}
+
+__device__ __forceinline__ void belief_adj(double bb, double P0, double P1, double &G0, double &G1) {
+    /* adjoint of the full products from the adjoint bb of b = P1 / (P0 + P1) */
+    G0 = G1 = 0.0;
+    if (bb != 0.0 && P0 + P1 != 0.0) {
+        const double den = (P0 + P1) * (P0 + P1);
+        G1 = bb * P0 / den;
+        G0 = -bb * P1 / den;
+    }
+}
+
+__device__ __attribute__((noinline)) void var_vjp_one(
     const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
     const double2 *__restrict__ m_fv, const double2 *__restrict__ avf,
     const double *__restrict__ bbar, double2 *__restrict__ afv, double2 *__restrict__ scratch,
-    double damp, uint32_t V) {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    if (v >= V) return;
+    double damp, uint32_t v) {
     const uint32_t s = var_off[v];
     const int d = (int)(var_off[v + 1] - s);
     const double bb = bbar ? bbar[v] : 0.0;
-    auto gadj = [&](double L0, double L1, double2 ob, double &g0, double &g1) __attribute__((always_inline)) {
-        const double S = L0 + L1;
-        g0 = g1 = 0.0;
-        if (S != 0.0) {
-            const double dot = ob.x * (L0 / S) + ob.y * (L1 / S);
-            g0 = (ob.x - dot) / S;
-            g1 = (ob.y - dot) / S;
-        }
-    };
-    auto belief_adj = [&](double P0, double P1, double &G0, double &G1) __attribute__((always_inline)) {
-        G0 = G1 = 0.0;
-        if (bb != 0.0 && P0 + P1 != 0.0) {
-            const double den = (P0 + P1) * (P0 + P1);
-            G1 = bb * P0 / den;
-            G0 = -bb * P1 / den;
-        }
-    };
     if (d <= kAdjVarDeg) {
         constexpr int N = kAdjVarDeg;
         uint32_t slot[N];
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_vjp(
             a1 *= m1[j];
         }
         double G0, G1;
-        belief_adj(a0, a1, G0, G1);
+        belief_adj(bb, a0, a1, G0, G1);
         a0 = a1 = 1.0;
 #pragma unroll
         for (int j = N - 1; j >= 0; j--) {
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_vjp(
         a1 *= m.y;
     }
     double G0, G1;
-    belief_adj(a0, a1, G0, G1);
+    belief_adj(bb, a0, a1, G0, G1);
     a0 = a1 = 1.0;
     double u0 = 0.0, u1 = 0.0;
     for (int j = d - 1; j >= 0; j--) {
@@ -271,57 +273,179 @@ __device__ __forceinline__ void fac_output_vjp(const double (&m0)[N], const doub
     }
 }
 
-/* Reverse of the factor pass of one sweep, one lane per factor.
- * m_vf: the sweep's v->f inputs (var-major); afv: total adjoint of the
- * sweep's f->v outputs (var-major); both gathered through fac_vpos.
- * avf_out: adjoint of the v->f inputs, written factor-major; pbar += dL/dp. */
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void fac_vjp(
-    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
-    const double *__restrict__ fac_p, const double *__restrict__ fac_q,
-    const double2 *__restrict__ fac_mask, const uint32_t *__restrict__ fac_vpos,
-    const double2 *__restrict__ m_vf, const double2 *__restrict__ afv, double2 *__restrict__ avf_out,
-    double *__restrict__ pbar, double keep, uint32_t F) {
-    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
-    if (f >= F) return;
-    const uint32_t s = fac_off[f];
-    const int d = (int)(fac_off[f + 1] - s);
-    const uint32_t meta = fac_meta[f];
-    const uint32_t type = meta & 3u;
-    const bool clamped = (meta & 4u) != 0;
-    const int h = (int)(meta >> 8);
-    const double p = fac_p[f];
-    const double q = fac_q[f];
-    double mask0 = 1.0, mask1 = 1.0;
-    if (clamped) {
-        const double2 mk = fac_mask[f];
-        mask0 = mk.x;
-        mask1 = mk.y;
+/* Slots per block of the staged reverse kernels. */
+constexpr int kAdjStage = 512;
+
+/* Reverse of the variable pass, LDS-staged: a block owns a contiguous
+ * range of at most kBlock variables and kAdjStage var-major slots, gathers
+ * the sweep's f->v messages and v->f adjoints and reads the carried f->v
+ * adjoints into LDS, then one lane per variable (work order: by degree)
+ * runs the prefix / suffix passes of var_vjp_one with its scratch in LDS,
+ * and the block writes the f->v adjoints back coalesced.  A variable with
+ * more than kAdjStage slots gets a block of its own and runs var_vjp_one
+ * with the global scratch row. */
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_vjp_staged(
+    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ var_off,
+    const uint32_t *__restrict__ var_edge, const uint16_t *__restrict__ work,
+    const double2 *__restrict__ m_fv, const double2 *__restrict__ avf, const double *__restrict__ bbar,
+    double2 *__restrict__ afv, double2 *__restrict__ scratch, double damp) {
+    __shared__ double2 s_m[kAdjStage];   /* the variable's f->v messages */
+    __shared__ double2 s_a[kAdjStage];   /* v->f adjoint, then g_j * P_j */
+    __shared__ double2 s_c[kAdjStage];   /* carried f->v adjoint, then the result */
+    __shared__ double2 s_x[kAdjStage];   /* prefix P_j, then P_j U_j + G P_j Q_j */
+    __shared__ double2 s_y[kAdjStage];   /* suffix Q_j */
+    __shared__ uint32_t s_start[kBlock + 1];
+    __shared__ double s_bb[kBlock];
+    const uint32_t v0 = blk[blockIdx.x], v1 = blk[blockIdx.x + 1];
+    const uint32_t base = var_off[v0];
+    const uint32_t n = var_off[v1] - base;
+    const int tid = threadIdx.x;
+    if (n > (uint32_t)kAdjStage) {
+        if (tid == 0) var_vjp_one(var_off, var_edge, m_fv, avf, bbar, afv, scratch, damp, v0);
+        return;
+    }
+    const uint32_t nv = v1 - v0;
+    uint32_t lv = 0;
+    if ((uint32_t)tid < nv) {
+        lv = work[v0 + tid];
+        s_start[tid] = var_off[v0 + tid] - base;
+        s_bb[tid] = bbar ? bbar[v0 + tid] : 0.0;
+    }
+    if (tid == 0) s_start[nv] = n;
+    for (uint32_t e = tid; e < n; e += kBlock) {
+        const uint32_t sl = var_edge[base + e];
+        s_m[e] = m_fv[sl];
+        s_a[e] = avf[sl];
+        s_c[e] = afv[base + e];
+    }
+    __syncthreads();
+    if ((uint32_t)tid < nv) {
+        const uint32_t s = s_start[lv];
+        const int d = (int)(s_start[lv + 1] - s);
+        double a0 = 1.0, a1 = 1.0;
+        for (int j = 0; j < d; j++) {
+            const double2 m = s_m[s + j];
+            s_x[s + j] = make_double2(a0, a1);
+            a0 *= m.x;
+            a1 *= m.y;
+        }
+        double G0, G1;
+        belief_adj(s_bb[lv], a0, a1, G0, G1);
+        double q0 = 1.0, q1 = 1.0, u0 = 0.0, u1 = 0.0;
+        for (int j = d - 1; j >= 0; j--) {
+            const double2 m = s_m[s + j], P = s_x[s + j];
+            double g0, g1;
+            gadj(P.x * q0, P.y * q1, s_a[s + j], g0, g1);
+            s_x[s + j] = make_double2(P.x * u0 + G0 * (P.x * q0), P.y * u1 + G1 * (P.y * q1));
+            s_y[s + j] = make_double2(q0, q1);
+            s_a[s + j] = make_double2(g0 * P.x, g1 * P.y);
+            u0 = u0 * m.x + g0 * q0;
+            u1 = u1 * m.y + g1 * q1;
+            q0 *= m.x;
+            q1 *= m.y;
+        }
+        double w0 = 0.0, w1 = 0.0;
+        for (int k = 0; k < d; k++) {
+            const double2 m = s_m[s + k], Q = s_y[s + k], R = s_x[s + k], gP = s_a[s + k], c = s_c[s + k];
+            s_c[s + k] = make_double2(damp * c.x + (w0 * Q.x + R.x), damp * c.y + (w1 * Q.y + R.y));
+            w0 = w0 * m.x + gP.x;
+            w1 = w1 * m.y + gP.y;
+        }
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < n; e += kBlock) afv[base + e] = s_c[e];
+}
+
+/* every output VJP of one factor of degree exactly D (operands in LDS):
+ * accumulates the input adjoints in place of the inputs and returns dL/dp */
+template <int D>
+__device__ __forceinline__ double fac_vjp_exact(double2 *m, const double2 *a, int h, uint32_t type, bool clamped,
+                                                double p, double q, double mask0, double mask1, double keep) {
+    double m0[D], m1[D], acc0[D], acc1[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        const double2 x = m[j];
+        m0[j] = x.x;
+        m1[j] = x.y;
+        acc0[j] = 0.0;
+        acc1[j] = 0.0;
     }
     double pb = 0.0;
-    if (d <= kAdjFacDeg) {
-        double m0[kAdjFacDeg], m1[kAdjFacDeg], acc0[kAdjFacDeg], acc1[kAdjFacDeg];
 #pragma unroll
-        for (int j = 0; j < kAdjFacDeg; j++) {
-            const double2 m = j < d ? m_vf[fac_vpos[s + j]] : make_double2(1.0, 1.0);
-            m0[j] = m.x;
-            m1[j] = m.y;
-            acc0[j] = 0.0;
-            acc1[j] = 0.0;
-        }
-        for (int t = 0; t < d; t++) {
-            const double2 a = afv[fac_vpos[s + t]];
-            fac_output_vjp<kAdjFacDeg>(m0, m1, d, t, h, type, clamped, p, q, mask0, mask1,
-                                       make_double2(keep * a.x, keep * a.y), acc0, acc1, pb);
-        }
-#pragma unroll
-        for (int j = 0; j < kAdjFacDeg; j++)
-            if (j < d) avf_out[s + j] = make_double2(acc0[j], acc1[j]);
-    } else {
-        /* factors above the register degree: not produced by the layered
-         * generator; flagged by the host (adjoint creation rejects them) */
-        for (int j = 0; j < d; j++) avf_out[s + j] = make_double2(0.0, 0.0);
+    for (int t = 0; t < D; t++) {
+        const double2 x = a[t];
+        fac_output_vjp<D>(m0, m1, D, t, h, type, clamped, p, q, mask0, mask1, make_double2(keep * x.x, keep * x.y),
+                          acc0, acc1, pb);
     }
-    pbar[f] += pb;
+#pragma unroll
+    for (int j = 0; j < D; j++) m[j] = make_double2(acc0[j], acc1[j]);
+    return pb;
+}
+
+/* Reverse of the factor pass, LDS-staged: a block owns a contiguous range
+ * of at most kBlock factors and kAdjStage slots, gathers the sweep's v->f
+ * inputs and the f->v adjoints (both var-major) into LDS, one lane per
+ * factor (work order: by type and degree, so a wave runs one degree's
+ * code) forms the adjoints of its inputs in place, and the block writes
+ * them back factor-major, coalesced.  pbar[f] += dL/dp_f by its lane. */
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void fac_vjp_staged(
+    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ fac_off,
+    const uint32_t *__restrict__ fac_meta, const double *__restrict__ fac_p,
+    const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
+    const uint32_t *__restrict__ fac_vpos, const uint16_t *__restrict__ work,
+    const double2 *__restrict__ m_vf, const double2 *__restrict__ afv, double2 *__restrict__ avf_out,
+    double *__restrict__ pbar, double keep) {
+    __shared__ double2 s_m[kAdjStage];
+    __shared__ double2 s_a[kAdjStage];
+    __shared__ uint32_t s_start[kBlock + 1];
+    const uint32_t f0 = blk[blockIdx.x], f1 = blk[blockIdx.x + 1];
+    const uint32_t base = fac_off[f0];
+    const uint32_t n = fac_off[f1] - base;   /* <= kAdjStage: factor degrees are at most kAdjFacDeg */
+    const int tid = threadIdx.x;
+    const uint32_t nf = f1 - f0;
+    uint32_t lf = 0;
+    if ((uint32_t)tid < nf) {
+        lf = work[f0 + tid];
+        s_start[tid] = fac_off[f0 + tid] - base;
+    }
+    if (tid == 0) s_start[nf] = n;
+    for (uint32_t e = tid; e < n; e += kBlock) {
+        const uint32_t k = fac_vpos[base + e];
+        s_m[e] = m_vf[k];
+        s_a[e] = afv[k];
+    }
+    __syncthreads();
+    if ((uint32_t)tid < nf) {
+        const uint32_t f = f0 + lf;
+        const uint32_t s = s_start[lf];
+        const int d = (int)(s_start[lf + 1] - s);
+        const uint32_t meta = fac_meta[f];
+        const uint32_t type = meta & 3u;
+        const bool clamped = (meta & 4u) != 0;
+        const int h = (int)(meta >> 8);
+        const double p = fac_p[f], q = fac_q[f];
+        double mask0 = 1.0, mask1 = 1.0;
+        if (clamped) {
+            const double2 mk = fac_mask[f];
+            mask0 = mk.x;
+            mask1 = mk.y;
+        }
+        double pb = 0.0;
+        double2 *m = s_m + s;
+        const double2 *a = s_a + s;
+        switch (d) {
+        case 1: pb = fac_vjp_exact<1>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
+        case 2: pb = fac_vjp_exact<2>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
+        case 3: pb = fac_vjp_exact<3>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
+        case 4: pb = fac_vjp_exact<4>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
+        case 5: pb = fac_vjp_exact<5>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
+        case 6: pb = fac_vjp_exact<6>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
+        default: break;   /* degree 0 has no slots; above kAdjFacDeg is rejected by the host */
+        }
+        pbar[f] += pb;
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < n; e += kBlock) avf_out[base + e] = s_m[e];
 }
 
 }  // namespace
