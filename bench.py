@@ -192,8 +192,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["plain", "batched", "adjoint"], default="plain")
-    ap.add_argument("--variant", type=int, default=3, help="sweep kernels: 0 LDS-staged (pull), 1 one lane per item, 2 LDS-staged (push), "
-                         "3 LDS-staged (scatter)")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="sweep kernels: -1 the plan's own choice by footprint (default), 0 LDS-staged (pull), "
+                         "1 one lane per item, 2 LDS-staged (push), 3 LDS-staged (scatter)")
+    ap.add_argument("--n-vars", type=int, default=0,
+                    help="plain workload at another graph size (a sizing study, not the headline workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 512 MiB buffer once, so HBM counters can be scaled by a known size")
     args = ap.parse_args()
@@ -217,14 +220,17 @@ def main():
         log("[bench] calibration copy: 536870912 bytes read, 536870912 bytes written")
         del src, dst
 
-    w = WORKLOAD
+    w = dict(WORKLOAD)
+    if args.n_vars:
+        w.update(n_vars=args.n_vars, name=f"plain-fp64-{args.n_vars}")
     t0 = time.perf_counter()
     gen = fl.generate(w["n_vars"], w["seed"])
     graph = fl.Graph(gen.factors)
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
     plan = fl.Plan(graph, device=0)
-    plan.set_variant(args.variant)
+    if args.variant >= 0:
+        plan.set_variant(args.variant)
     t_up = time.perf_counter() - t0
     V, F, E = graph.n_vars, graph.n_factors, graph.n_edges
     log(f"[bench] graph V={V} F={F} E={E} generate+build {t_gen:.2f}s upload {t_up:.2f}s "
@@ -284,7 +290,7 @@ def main():
            "config": {"workload": w["name"], "n_vars": V, "n_factors": F, "n_edges": E,
                       "damping": w["damping"], "tol": w["tol"], "maxiter": w["maxiter"],
                       "seed": w["seed"], "sweeps_per_solve": iters_per_solve,
-                      "kernels": ["lds-staged-pull", "lane-per-item", "lds-staged-push", "lds-staged-scatter"][args.variant],
+                      "kernels": ["lds-staged-pull", "lane-per-item", "lds-staged-push", "lds-staged-scatter"][plan.variant],
                       "step": "init + run() to convergence of the resident graph"},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

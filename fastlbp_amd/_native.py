@@ -90,7 +90,11 @@ class BatchOptsC(C.Structure):
         ("grid_blocks", C.c_uint32),
         ("chunk", C.c_uint32),
         ("use_graph", C.c_uint32),
+        ("layout", C.c_uint32),
     ]
+
+
+LAYOUT_AUTO, LAYOUT_PULL, LAYOUT_SCATTER = 0, 1, 2   # FLBP_LAYOUT_* of fastlbp.h
 
 
 class RunResultC(C.Structure):
@@ -132,6 +136,7 @@ SIGNATURES = {
     "flbp_plan_last_timing": (C.c_int, [C.c_void_p, f64p, f64p, u64p, u64p]),
     "flbp_plan_device_bytes": (C.c_uint64, [C.c_void_p]),
     "flbp_plan_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "flbp_plan_get_variant": (C.c_int, [C.c_void_p]),
     "flbp_batch_default_opts": (None, [C.POINTER(BatchOptsC)]),
     "flbp_batch_create": (C.c_int, [C.c_uint64, C.POINTER(C.POINTER(CsrC)), C.c_int, C.POINTER(BatchOptsC),
                                     C.POINTER(C.c_void_p)]),

@@ -8,8 +8,10 @@
  * persistent factor kernel and one persistent variable kernel over the
  * tile list -- is captured once into a hipGraph and replayed; per-graph
  * stop flags live in device memory, so one replay advances every unfinished
- * graph by up to `chunk` sweeps with no host round trip.  Tile sizes and
- * grid size are chosen by flbp_batch_autotune.
+ * graph by up to `chunk` sweeps with no host round trip.  Tiles are the
+ * plain solve's staged blocks cut per graph (their edge-slot bound is the
+ * tile option), with per-tile work orders; messages use the scatter layout.
+ * The tile bound and the grid size are chosen by flbp_batch_autotune.
  */
 #include <hip/hip_runtime.h>
 
@@ -19,6 +21,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "batch_kernels.h"
@@ -27,6 +30,16 @@
 #include "plan_internal.h"
 
 namespace flbp {
+
+/* one batch_permute launch over the factor tiles; returns on a launch error */
+#define TRY_PERMUTE(TO_FM, SRC, DST)                                                                       \
+    do {                                                                                                    \
+        const uint32_t g_ = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_ftiles));                    \
+        hipLaunchKernelGGL(batch_permute<TO_FM>, dim3(g_), dim3(kBlock), 0, stream, graphs, ftiles, n_ftiles, \
+                           fac_off, fac_vpos, SRC, DST);                                                    \
+        const hipError_t e_ = hipGetLastError();                                                            \
+        if (e_ != hipSuccess) return hip_fail(e_, "batch_permute");                                         \
+    } while (0)
 
 struct Batch {
     int device = -1;
@@ -39,8 +52,16 @@ struct Batch {
     uint32_t *fac_vpos = nullptr;
     double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
+    double2 *m_fv_vm = nullptr;             /* f->v var-major; m_fv is the factor-major copy for damping */
+    bool fm_stale = false;                  /* scatter layout: m_fv out of date after undamped sweeps */
+    int layout = 0;                         /* resolved: kLayoutPull or kLayoutScatter */
+    uint32_t *fac_work = nullptr;           /* per tile: its slots in work order (graph-local tile slots) */
+    uint16_t *var_work = nullptr;           /* per tile: its variables by degree */
+    std::vector<uint32_t> h_fac_off, h_fac_meta, h_var_off;   /* host copies, batch-concatenated */
     Tile *ftiles = nullptr, *vtiles = nullptr;
+    uint64_t cap_ftiles = 0, cap_vtiles = 0;
     uint32_t n_ftiles = 0, n_vtiles = 0;
+    uint32_t cut_f = 0, cut_v = 0;          /* slot bounds of the current tiles */
     unsigned long long *diff = nullptr;     /* [kMaxChunk][G][kBatchSlots][kCtlPad] partial maxima */
     unsigned long long *diff_out = nullptr; /* [kMaxChunk][G] compacted */
     uint8_t *start_active = nullptr;        /* device [G] */
@@ -75,6 +96,11 @@ struct Batch {
         p = static_cast<T *>(q);
         return FLBP_OK;
     }
+    void release(void *q) {
+        auto it = std::find(allocs.begin(), allocs.end(), q);
+        if (it != allocs.end()) allocs.erase(it);
+        (void)hipFree(q);
+    }
     void drop_exec() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -91,6 +117,45 @@ struct Batch {
         if (stream) (void)hipStreamDestroy(stream);
     }
     int build_tiles();
+    int resolve_layout() const {
+        if (opts.layout == FLBP_LAYOUT_PULL) return kLayoutPull;
+        if (opts.layout == FLBP_LAYOUT_SCATTER) return kLayoutScatter;
+        return bytes <= kScatterMaxBytes ? kLayoutScatter : kLayoutPull;
+    }
+    /* undamped sweeps skip the factor-major f->v copy; a damped run first
+     * rebuilds it from the var-major array */
+    int prepare_damping(double damping) {
+        if (layout != kLayoutScatter) return FLBP_OK;
+        if (damping != 0.0 && fm_stale && n_ftiles) TRY_PERMUTE(true, m_fv_vm, m_fv);
+        fm_stale = damping == 0.0 ? true : (fm_stale && !n_ftiles);
+        return FLBP_OK;
+    }
+    /* move the stored messages to layout `to` (kLayoutPull / kLayoutScatter) */
+    int set_layout(int to) {
+        if (to == layout) return FLBP_OK;
+        if (n_ftiles) {
+            if (to == kLayoutScatter) {
+                /* v->f var-major -> factor-major through the spare array,
+                 * f->v factor-major -> var-major; the factor-major f->v
+                 * copy stays valid */
+                TRY_PERMUTE(true, m_vf, m_fv_vm);
+                if (hipMemcpyAsync(m_vf, m_fv_vm, Etot * sizeof(double2), hipMemcpyDeviceToDevice, stream) != hipSuccess)
+                    return fail(FLBP_EHIP, "batch: layout copy failed");
+                TRY_PERMUTE(false, m_fv, m_fv_vm);
+                fm_stale = false;
+            } else {
+                if (fm_stale) TRY_PERMUTE(true, m_fv_vm, m_fv);
+                TRY_PERMUTE(false, m_vf, m_fv_vm);
+                if (hipMemcpyAsync(m_vf, m_fv_vm, Etot * sizeof(double2), hipMemcpyDeviceToDevice, stream) != hipSuccess)
+                    return fail(FLBP_EHIP, "batch: layout copy failed");
+                fm_stale = false;
+            }
+            if (hipStreamSynchronize(stream) != hipSuccess) return fail(FLBP_EHIP, "batch: layout move failed");
+        }
+        layout = to;
+        drop_exec();
+        return FLBP_OK;
+    }
     int record_chunk(int n, double tol, double damping);
     int enqueue_chunk(int n, double tol, double damping);
 };
@@ -107,27 +172,62 @@ struct Batch {
     } while (0)
 
 int Batch::build_tiles() {
-    /* a tile covers at least one block's worth of lanes */
-    const uint32_t tf = std::max<uint32_t>(opts.tile_factors, kBlock), tv = std::max<uint32_t>(opts.tile_vars, kBlock);
-    std::vector<Tile> ft, vt;
-    for (uint32_t g = 0; g < G; g++) {
-        for (uint32_t a = 0; a < h_graphs[g].n_fac; a += tf)
-            ft.push_back({g, a, std::min(tf, h_graphs[g].n_fac - a)});
-        for (uint32_t a = 0; a < h_graphs[g].n_var; a += tv)
-            vt.push_back({g, a, std::min(tv, h_graphs[g].n_var - a)});
-    }
-    if (ft.size() >= (1ull << 31) || vt.size() >= (1ull << 31)) return fail(FLBP_ERANGE, "batch: too many tiles");
-    n_ftiles = (uint32_t)ft.size();
-    n_vtiles = (uint32_t)vt.size();
-    /* tiles are re-cut by the autotuner: the arrays are sized once for the
-     * smallest tile (kBlock items) */
-    if (!ftiles) {
-        TRY(alloc(ftiles, Ftot / kBlock + G + 1));
-        TRY(alloc(vtiles, Vtot / kBlock + G + 1));
-    }
-    if (n_ftiles) HIP_TRY(hipMemcpy(ftiles, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    if (n_vtiles) HIP_TRY(hipMemcpy(vtiles, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    const uint32_t sf = std::min<uint32_t>(std::max<uint32_t>(opts.tile_factors, 64), kStageSlots);
+    const uint32_t sv = std::min<uint32_t>(std::max<uint32_t>(opts.tile_vars, 64), kStageSlots);
     drop_exec();
+    if (sf == cut_f && sv == cut_v && ftiles) return FLBP_OK;
+    /* per graph: staged blocks and their work orders, graphs spread over
+     * host threads */
+    std::vector<std::vector<Tile>> ft(G), vt(G);
+    std::vector<uint32_t> fw(Etot);
+    std::vector<uint16_t> vw(Vtot);
+    auto cut = [&](uint64_t g) {
+        const BatchGraph &bg = h_graphs[g];
+        const uint32_t *fo = h_fac_off.data() + bg.fac_off_base;
+        const uint32_t *vo = h_var_off.data() + bg.var_off_base;
+        std::vector<uint32_t> fb, vb, gfw;
+        std::vector<uint16_t> gvw;
+        cut_blocks(fo, bg.n_fac, kStageItems, sf, fb);
+        cut_blocks(vo, bg.n_var, kStageItems, sv, vb);
+        stage_work_orders(fo, h_fac_meta.data() + bg.fac_base, fb, vo, vb, gfw, gvw);
+        std::copy(gfw.begin(), gfw.end(), fw.begin() + bg.edge_base);
+        std::copy(gvw.begin(), gvw.end(), vw.begin() + bg.var_base);
+        for (size_t b = 0; b + 1 < fb.size(); b++) ft[g].push_back({(uint32_t)g, fb[b], fb[b + 1] - fb[b]});
+        for (size_t b = 0; b + 1 < vb.size(); b++) vt[g].push_back({(uint32_t)g, vb[b], vb[b + 1] - vb[b]});
+    };
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++)
+        pool.emplace_back([&, t] {
+            for (uint64_t g = t; g < G; g += nth) cut(g);
+        });
+    for (auto &th : pool) th.join();
+    std::vector<Tile> fall, vall;
+    for (uint64_t g = 0; g < G; g++) {
+        fall.insert(fall.end(), ft[g].begin(), ft[g].end());
+        vall.insert(vall.end(), vt[g].begin(), vt[g].end());
+    }
+    if (fall.size() >= (1ull << 31) || vall.size() >= (1ull << 31)) return fail(FLBP_ERANGE, "batch: too many tiles");
+    n_ftiles = (uint32_t)fall.size();
+    n_vtiles = (uint32_t)vall.size();
+    if (n_ftiles > cap_ftiles) {
+        if (ftiles) release(ftiles);
+        ftiles = nullptr;
+        TRY(alloc(ftiles, n_ftiles));
+        cap_ftiles = n_ftiles;
+    }
+    if (n_vtiles > cap_vtiles) {
+        if (vtiles) release(vtiles);
+        vtiles = nullptr;
+        TRY(alloc(vtiles, n_vtiles));
+        cap_vtiles = n_vtiles;
+    }
+    if (n_ftiles) HIP_TRY(hipMemcpy(ftiles, fall.data(), fall.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (n_vtiles) HIP_TRY(hipMemcpy(vtiles, vall.data(), vall.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (Etot) HIP_TRY(hipMemcpy(fac_work, fw.data(), Etot * 4, hipMemcpyHostToDevice));
+    if (Vtot) HIP_TRY(hipMemcpy(var_work, vw.data(), Vtot * 2, hipMemcpyHostToDevice));
+    cut_f = sf;
+    cut_v = sv;
     return FLBP_OK;
 }
 
@@ -142,22 +242,33 @@ int Batch::record_chunk(int n, double tol, double damping) {
     for (int k = 0; k < n; k++) {
         if (n_ftiles) {
             const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
-#define FLBP_BATCH_FACTOR_PASS(DAMP, N)                                                                  \
-    hipLaunchKernelGGL((batch_factor_pass<DAMP, N>), dim3(fg), dim3(kBlock), 0, stream, graphs, ftiles,   \
-                       n_ftiles, fac_off, fac_meta, fac_p, fac_q, fac_mask, fac_vpos, m_vf, m_fv, damping, \
-                       keep, c, k)
+#define FLBP_BATCH_FACTOR_PASS(DAMP, N, L)                                                                    \
+    hipLaunchKernelGGL((batch_factor_pass<DAMP, N, L>), dim3(fg), dim3(kBlock), 0, stream, graphs, ftiles,     \
+                       n_ftiles, fac_off, fac_meta, fac_p, fac_q, fac_mask, fac_vpos, fac_work, m_vf, m_fv,     \
+                       m_fv_vm, damping, keep, c, k)
+#define FLBP_BATCH_FACTOR_PASS_L(DAMP, N)                                                   \
+    do {                                                                                    \
+        if (layout == kLayoutScatter) FLBP_BATCH_FACTOR_PASS(DAMP, N, kLayoutScatter);      \
+        else FLBP_BATCH_FACTOR_PASS(DAMP, N, kLayoutPull);                                  \
+    } while (0)
             if (damping != 0.0) {
-                if (narrow) FLBP_BATCH_FACTOR_PASS(true, kFacRegDegSmall);
-                else FLBP_BATCH_FACTOR_PASS(true, kFacRegDeg);
+                if (narrow) FLBP_BATCH_FACTOR_PASS_L(true, kFacRegDegSmall);
+                else FLBP_BATCH_FACTOR_PASS_L(true, kFacRegDeg);
             } else {
-                if (narrow) FLBP_BATCH_FACTOR_PASS(false, kFacRegDegSmall);
-                else FLBP_BATCH_FACTOR_PASS(false, kFacRegDeg);
+                if (narrow) FLBP_BATCH_FACTOR_PASS_L(false, kFacRegDegSmall);
+                else FLBP_BATCH_FACTOR_PASS_L(false, kFacRegDeg);
             }
+#undef FLBP_BATCH_FACTOR_PASS_L
 #undef FLBP_BATCH_FACTOR_PASS
         }
-        if (n_vtiles)
-            hipLaunchKernelGGL(batch_var_pass, dim3(vg), dim3(kBlock), 0, stream, graphs, vtiles, n_vtiles,
-                               var_off, var_edge, m_fv, m_vf, belief, c, k);
+        if (n_vtiles) {
+            if (layout == kLayoutScatter)
+                hipLaunchKernelGGL(batch_var_pass<kLayoutScatter>, dim3(vg), dim3(kBlock), 0, stream, graphs, vtiles,
+                                   n_vtiles, var_off, var_edge, var_work, m_fv_vm, m_vf, belief, c, k);
+            else
+                hipLaunchKernelGGL(batch_var_pass<kLayoutPull>, dim3(vg), dim3(kBlock), 0, stream, graphs, vtiles,
+                                   n_vtiles, var_off, var_edge, var_work, m_fv, m_vf, belief, c, k);
+        }
     }
     const uint64_t waves = (uint64_t)n * G;
     hipLaunchKernelGGL(batch_compact, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(1024, (waves * 64 + 255) / 256))),
@@ -206,7 +317,8 @@ int default_grid(int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_factor_pass<true, kFacRegDeg>, kBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_factor_pass<true, kFacRegDegSmall, kLayoutScatter>,
+                                                     kBlock, 0) !=
             hipSuccess ||
         per_cu <= 0)
         per_cu = 4;
@@ -219,11 +331,12 @@ extern "C" {
 
 void flbp_batch_default_opts(flbp_batch_opts *o) {
     if (!o) return;
-    o->tile_factors = 1024;
-    o->tile_vars = 1024;
+    o->tile_factors = flbp::kStageSlots;
+    o->tile_vars = flbp::kStageSlots;
     o->grid_blocks = 0;
     o->chunk = 32;
     o->use_graph = 1;
+    o->layout = FLBP_LAYOUT_AUTO;
 }
 
 int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device, const flbp_batch_opts *opts,
@@ -250,6 +363,7 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
     B->G = n_graphs;
     if (opts) B->opts = *opts;
     else flbp_batch_default_opts(&B->opts);
+    if (B->opts.layout > FLBP_LAYOUT_SCATTER) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: bad layout");
     if (B->opts.chunk == 0 || B->opts.chunk > (uint32_t)flbp::kMaxChunk) B->opts.chunk = flbp::kMaxChunk;
     if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(device);
     B->h_graphs.resize(n_graphs);
@@ -278,7 +392,13 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
     TRY(B->alloc(B->fac_vpos, E));
     TRY(B->alloc(B->m_vf, E));
     TRY(B->alloc(B->m_fv, E));
+    TRY(B->alloc(B->m_fv_vm, E));
+    TRY(B->alloc(B->fac_work, E));
+    TRY(B->alloc(B->var_work, V));
     TRY(B->alloc(B->belief, V));
+    B->h_fac_off.resize(F + n_graphs);
+    B->h_fac_meta.resize(F);
+    B->h_var_off.resize(V + n_graphs);
     TRY(B->alloc(B->diff, (uint64_t)flbp::kMaxChunk * n_graphs * kBatchSlots * kCtlPad));
     TRY(B->alloc(B->diff_out, (uint64_t)flbp::kMaxChunk * n_graphs));
     TRY(B->alloc(B->start_active, n_graphs));
@@ -295,10 +415,12 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
         const BatchGraph &bg = B->h_graphs[g];
         u32.resize(std::max(std::max(c.n_factors, c.n_vars), c.n_edges) + 1);
         for (uint64_t I = 0; I <= c.n_factors; I++) u32[I] = (uint32_t)c.fac_off[I];
+        std::copy(u32.begin(), u32.begin() + c.n_factors + 1, B->h_fac_off.begin() + bg.fac_off_base);
         HIP_TRY(hipMemcpy(B->fac_off + bg.fac_off_base, u32.data(), (c.n_factors + 1) * 4, hipMemcpyHostToDevice));
         for (uint64_t I = 0; I < c.n_factors; I++)
             u32[I] = flbp::dev::pack_meta((uint32_t)c.fac_type[I], c.head_clamped[I] != 0,
                                           (uint32_t)(c.head_slot[I] - c.fac_off[I]));
+        std::copy(u32.begin(), u32.begin() + c.n_factors, B->h_fac_meta.begin() + bg.fac_base);
         if (c.n_factors) {
             HIP_TRY(hipMemcpy(B->fac_meta + bg.fac_base, u32.data(), c.n_factors * 4, hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(B->fac_p + bg.fac_base, c.p, c.n_factors * 8, hipMemcpyHostToDevice));
@@ -310,6 +432,7 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
         std::vector<uint32_t> perm, dev_off, dev_edge, vpos;
         flbp::var_device_order(c, false, perm, dev_off, dev_edge, vpos);
         B->h_perm.insert(B->h_perm.end(), perm.begin(), perm.end());
+        std::copy(dev_off.begin(), dev_off.end(), B->h_var_off.begin() + bg.var_off_base);
         HIP_TRY(hipMemcpy(B->var_off + bg.var_off_base, dev_off.data(), (c.n_vars + 1) * 4, hipMemcpyHostToDevice));
         if (c.n_edges) {
             HIP_TRY(hipMemcpy(B->var_edge + bg.edge_base, dev_edge.data(), c.n_edges * 4, hipMemcpyHostToDevice));
@@ -319,9 +442,11 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
     HIP_TRY(hipMemset(B->belief, 0, std::max<uint64_t>(V, 1) * 8));
     HIP_TRY(hipMemset(B->m_vf, 0, std::max<uint64_t>(E, 1) * 16));
     HIP_TRY(hipMemset(B->m_fv, 0, std::max<uint64_t>(E, 1) * 16));
+    HIP_TRY(hipMemset(B->m_fv_vm, 0, std::max<uint64_t>(E, 1) * 16));
     B->iters.assign(n_graphs, 0);
     B->last_diff.assign(n_graphs, INFINITY);
     TRY(B->build_tiles());
+    B->layout = B->resolve_layout();
     *out = reinterpret_cast<flbp_batch *>(hold.release());
     return FLBP_OK;
 }
@@ -337,9 +462,11 @@ int flbp_batch_init(flbp_batch *b, int logdomain) {
         const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(B->Etot, 256), 65536);
         hipLaunchKernelGGL(bfill2, dim3(grid), dim3(256), 0, B->stream, B->m_vf, B->Etot, c);
         hipLaunchKernelGGL(bfill2, dim3(grid), dim3(256), 0, B->stream, B->m_fv, B->Etot, c);
+        hipLaunchKernelGGL(bfill2, dim3(grid), dim3(256), 0, B->stream, B->m_fv_vm, B->Etot, c);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(B->stream));
+    B->fm_stale = false;
     return FLBP_OK;
 }
 
@@ -361,6 +488,7 @@ int flbp_batch_run(flbp_batch *b, const flbp_run_opts *o, uint64_t *iters_out, d
         n_active += active[g];
     }
     const int chunk = (int)std::min<uint32_t>(o->chunk ? o->chunk : B->opts.chunk, flbp::kMaxChunk);
+    TRY(B->prepare_damping(o->damping));
     while (n_active && elapsed() < o->maxtime) {
         for (uint64_t g = 0; g < G; g++) {
             B->h_active[g] = active[g];
@@ -433,10 +561,12 @@ int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *o) {
     Batch *B = reinterpret_cast<Batch *>(b);
     if (!B || !o) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: null argument");
     HIP_TRY(hipSetDevice(B->device));
+    if (o->layout > FLBP_LAYOUT_SCATTER) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: bad layout");
     B->opts = *o;
     if (B->opts.chunk == 0 || B->opts.chunk > (uint32_t)flbp::kMaxChunk) B->opts.chunk = flbp::kMaxChunk;
     if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(B->device);
-    return B->build_tiles();
+    TRY(B->build_tiles());
+    return B->set_layout(B->resolve_layout());
 }
 
 int flbp_batch_get_opts(flbp_batch *b, flbp_batch_opts *o) {
@@ -462,19 +592,23 @@ int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps, fl
     HIP_TRY(hipSetDevice(B->device));
     const flbp_batch_opts base = B->opts;
     const uint32_t g0 = (uint32_t)default_grid(B->device);
-    const uint32_t tiles[] = {256, 1024, 4096};
+    const uint32_t tiles[] = {512, flbp::kStageSlots};
     const uint32_t grids[] = {g0, 2 * g0};
+    const uint32_t layouts[] = {FLBP_LAYOUT_PULL, FLBP_LAYOUT_SCATTER};
     const int n = (int)std::min<uint32_t>(trial_sweeps, flbp::kMaxChunk);
     double best_t = INFINITY;
     flbp_batch_opts best_o = base;
+    for (uint32_t lay : layouts)
     for (uint32_t tf : tiles) {
         for (uint32_t gb : grids) {
             flbp_batch_opts o = base;
             o.tile_factors = tf;
             o.tile_vars = tf;
             o.grid_blocks = gb;
+            o.layout = lay;
             TRY(flbp_batch_set_opts(b, &o));
             TRY(flbp_batch_init(b, 0));
+            TRY(B->prepare_damping(damping));
             for (uint64_t g = 0; g < B->G; g++) {
                 B->h_active[g] = 1;
                 B->h_budget[g] = (uint32_t)n;

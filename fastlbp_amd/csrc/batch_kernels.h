@@ -4,9 +4,12 @@
  * Graphs are concatenated: graph g's factors start at fac_base[g], its
  * variables at var_base[g] and its edge slots at edge_base[g] (64-bit
  * bases), and all indices stored inside a graph's arrays are graph-local
- * 32-bit values.  Work is cut into tiles of one graph each; a persistent
- * grid walks the tile list with a stride, so a launch uses exactly the
- * blocks that fit on the chip whatever the batch holds.
+ * 32-bit values.  Work is cut into tiles of one graph each -- the staged
+ * blocks of the plain solve (at most kBlock items and kStage edge slots,
+ * with their work orders), in the pull or the scatter message layout of
+ * staged_kernels.h -- and a persistent grid walks the tile list with a
+ * stride, so a launch uses exactly the blocks that fit on the chip
+ * whatever the batch holds.
  *
  * Each graph keeps its own stop test: sweep k of a chunk runs on graph g
  * only if g was active when the chunk started, k is below its remaining
@@ -20,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "staged_kernels.h"
 #include "sweep_kernels.h"
 
 namespace {
@@ -68,75 +72,78 @@ __device__ __forceinline__ bool graph_runs(const BatchCtl &c, uint32_t g, int k)
     return __longlong_as_double((long long)batch_read(c, k - 1, g)) > c.tol;
 }
 
-template <bool kDamp, int N>
+template <bool kDamp, int N, int kLayout>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_pass(
     const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, uint32_t n_tiles,
     const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
     const double *__restrict__ fac_p, const double *__restrict__ fac_q,
     const double2 *__restrict__ fac_mask, const uint32_t *__restrict__ fac_vpos,
-    const double2 *__restrict__ m_vf, double2 *__restrict__ m_fv, double damp, double keep,
-    BatchCtl ctl, int k) {
-    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
-        const Tile tile = tiles[ti];
-        if (!graph_runs(ctl, tile.graph, k)) continue;
-        const BatchGraph gr = graphs[tile.graph];
-        const uint32_t *off = fac_off + gr.fac_off_base;
-        const double2 *vf = m_vf + gr.edge_base;
-        const uint32_t *vpos = fac_vpos + gr.edge_base;
-        double2 *fv = m_fv + gr.edge_base;
-        for (uint32_t lf = tile.first + threadIdx.x; lf < tile.first + tile.count; lf += kBlock) {
-            const uint64_t f = gr.fac_base + lf;
-            const uint32_t s = off[lf];
-            const uint32_t meta = fac_meta[f];
-            const double p = fac_p[f];
-            const double q = (meta & 3u) ? fac_q[f] : 0.0;
-            double mask0 = 1.0, mask1 = 1.0;
-            if (meta & 4u) {
-                const double2 mk = fac_mask[f];
-                mask0 = mk.x;
-                mask1 = mk.y;
-            }
-            factor_update<N, kDamp>(vf, vpos + s, fv + s, fv + s, (int)(off[lf + 1] - s), meta, p, q, mask0,
-                                    mask1, damp, keep);
-        }
-    }
-}
-
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass(
-    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, uint32_t n_tiles,
-    const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
-    const double2 *__restrict__ m_fv, double2 *__restrict__ m_vf, double *__restrict__ belief,
-    BatchCtl ctl, int k) {
-    __shared__ unsigned long long red[kBlock / 64];
+    const uint32_t *__restrict__ fac_work, const double2 *__restrict__ m_vf, double2 *m_fv,
+    double2 *__restrict__ m_fv_vm, double damp, double keep, BatchCtl ctl, int k) {
+    __shared__ FactorSmem sm;
     for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
         const Tile tile = tiles[ti];
         if (!graph_runs(ctl, tile.graph, k)) continue;   /* uniform per block */
         const BatchGraph gr = graphs[tile.graph];
-        const uint32_t *off = var_off + gr.var_off_base;
-        const uint32_t *ve = var_edge + gr.edge_base;
-        const double2 *fv = m_fv + gr.edge_base;
-        double2 *vf = m_vf + gr.edge_base;
-        double *bel = belief + gr.var_base;
-        unsigned long long diff_bits = 0;
-        for (uint32_t lv = tile.first + threadIdx.x; lv < tile.first + tile.count; lv += kBlock) {
-            const uint32_t s = off[lv];
-            const unsigned long long bits = var_update(ve + s, fv, vf + s, bel + lv, (int)(off[lv + 1] - s));
-            diff_bits = bits > diff_bits ? bits : diff_bits;
-        }
+        const uint64_t eb = gr.edge_base, fb = gr.fac_base;
+        factor_tile<kDamp, N, kLayout>(fac_off + gr.fac_off_base, fac_meta + fb, fac_p + fb, fac_q + fb,
+                                              fac_mask + fb, fac_vpos + eb, fac_work + eb, m_vf + eb, m_fv + eb,
+                                              m_fv + eb, m_fv_vm + eb, damp, keep, tile.first,
+                                              tile.first + tile.count, sm);
+        __syncthreads();   /* LDS is reused by the next tile */
+    }
+}
+
+template <int kLayout>
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass(
+    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, uint32_t n_tiles,
+    const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
+    const uint16_t *__restrict__ var_work, const double2 *__restrict__ m_fv_in, double2 *__restrict__ m_vf,
+    double *__restrict__ belief, BatchCtl ctl, int k) {
+    __shared__ VarSmem sm;
+    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
+        const Tile tile = tiles[ti];
+        if (!graph_runs(ctl, tile.graph, k)) continue;   /* uniform per block */
+        const BatchGraph gr = graphs[tile.graph];
+        const uint64_t eb = gr.edge_base;
+        unsigned long long diff_bits =
+            var_tile<kLayout>(var_off + gr.var_off_base, var_edge + eb, var_work + gr.var_base, m_fv_in + eb,
+                              m_vf + eb, belief + gr.var_base, tile.first, tile.first + tile.count, sm);
         diff_bits = wave_max_u64(diff_bits);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) red[wave] = diff_bits;
+        if (lane == 0) sm.red[wave] = diff_bits;
         __syncthreads();
         if (threadIdx.x == 0) {
-            unsigned long long m = red[0];
+            unsigned long long m = sm.red[0];
 #pragma unroll
-            for (int w = 1; w < kBlock / 64; w++) m = red[w] > m ? red[w] : m;
+            for (int w = 1; w < kBlock / 64; w++) m = sm.red[w] > m ? sm.red[w] : m;
             if (m) {
                 unsigned long long *slot = batch_slot(ctl, k, tile.graph, ti % kBatchSlots);
                 if (m > __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMax(slot, m);
             }
         }
-        __syncthreads();   /* red[] is reused by the next tile */
+        __syncthreads();   /* LDS is reused by the next tile */
+    }
+}
+
+/* message layout moves over every factor tile (graph-local positions):
+ * to_facmajor: dst[s] = src[vpos[s]]; to_varmajor: dst[vpos[s]] = src[s] */
+template <bool kToFacMajor>
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_permute(
+    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, uint32_t n_tiles,
+    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_vpos,
+    const double2 *__restrict__ src, double2 *__restrict__ dst) {
+    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
+        const Tile tile = tiles[ti];
+        const BatchGraph gr = graphs[tile.graph];
+        const uint32_t *off = fac_off + gr.fac_off_base;
+        const uint32_t base = off[tile.first], n = off[tile.first + tile.count] - base;
+        const uint64_t eb = gr.edge_base;
+        for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
+            const uint64_t s = eb + base + e, v = eb + fac_vpos[s];
+            if (kToFacMajor) dst[s] = src[v];
+            else dst[v] = src[s];
+        }
     }
 }
 

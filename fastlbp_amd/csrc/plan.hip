@@ -479,6 +479,11 @@ int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms, uint6
     return FLBP_OK;
 }
 
+int flbp_plan_get_variant(const flbp_plan *plan) {
+    const Plan *P = reinterpret_cast<const Plan *>(plan);
+    return P ? P->variant : flbp::fail(FLBP_EINVAL, "flbp_plan_get_variant: null plan");
+}
+
 int flbp_plan_set_variant(flbp_plan *plan, int variant) {
     Plan *P = reinterpret_cast<Plan *>(plan);
     if (!P || variant < 0 || variant > 3) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");

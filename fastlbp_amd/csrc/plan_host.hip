@@ -230,6 +230,7 @@ int Plan::upload(const flbp_csr &c) {
     h_var_edge.assign(c.var_edge, c.var_edge + E);
     for (uint64_t I = 0; I < F; I++) max_fac_deg = std::max<uint64_t>(max_fac_deg, c.fac_off[I + 1] - c.fac_off[I]);
     for (uint64_t i = 0; i < V; i++) max_var_deg = std::max<uint64_t>(max_var_deg, c.var_off[i + 1] - c.var_off[i]);
+    variant = device_bytes <= kScatterMaxBytes ? 3 : 0;
     return FLBP_OK;
 }
 

@@ -94,18 +94,37 @@ __device__ __forceinline__ void wide_factor(const In &in, uint32_t base, uint32_
     }
 }
 
+/* LDS of one factor tile */
+struct FactorSmem {
+    double2 in[kStage];
+    double2 out[kStage];
+    uint32_t start[kBlock + 1];   /* factor offsets relative to the tile's first slot */
+    uint32_t meta[kBlock];
+    double p[kBlock], q[kBlock];
+};
+
+/* LDS of one variable tile */
+struct VarSmem {
+    double2 m[kStage];
+    double prod[2][kBlock];
+    uint32_t z[kBlock];            /* z0 | z1 << 16 (a staged variable has <= kStage edges) */
+    uint8_t owner[kStage];         /* local variable of each staged slot */
+    uint32_t start[kBlock + 1];
+    double bel[kBlock];
+    unsigned long long red[kBlock / 64];
+};
+
+/* The factor pass over factors [f0, f1) of one graph (every pointer is the
+ * graph's own: offsets and slots are graph-local).  The whole block calls
+ * it; it ends with the block's stores issued and LDS free for reuse after a
+ * barrier. */
 template <bool kDamp, int N, int kLayout>
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_staged(
-    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ fac_off,
-    const uint32_t *__restrict__ fac_meta, const double *__restrict__ fac_p,
-    const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
+__device__ __forceinline__ void factor_tile(
+    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
+    const double *__restrict__ fac_p, const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
     const uint32_t *__restrict__ fac_vpos, const uint32_t *__restrict__ fac_work,
     const double2 *__restrict__ m_vf, const double2 *m_fv_old, double2 *m_fv, double2 *__restrict__ m_fv_vm,
-    double damp, double keep, const unsigned long long *__restrict__ ctl, int k, double tol) {
-    if (chunk_stopped(ctl, k, tol)) return;
-    __shared__ double2 s_in[kStage];
-    __shared__ double2 s_out[kStage];
-    const uint32_t f0 = blk[blockIdx.x], f1 = blk[blockIdx.x + 1];
+    double damp, double keep, uint32_t f0, uint32_t f1, FactorSmem &sm) {
     const uint32_t base = fac_off[f0];
     const uint32_t n = fac_off[f1] - base;
     const int tid = threadIdx.x;
@@ -127,12 +146,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
                                         m_fv_old, m_fv, m_fv_vm, damp, keep);
         return;
     }
-    /* phase 1: the block's work entries into registers, its factors'
+    /* phase 1: the tile's work entries into registers, its factors'
      * offsets and parameters into LDS, and its messages into LDS, all loads
      * independent of each other */
-    __shared__ uint32_t s_start[kBlock + 1];   /* factor offsets relative to base */
-    __shared__ uint32_t s_meta[kBlock];
-    __shared__ double s_p[kBlock], s_q[kBlock];
     const uint32_t nf = f1 - f0;
     uint32_t w[kPerLane];
 #pragma unroll
@@ -141,18 +157,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
         w[j] = i < n ? fac_work[base + i] : 0u;
     }
     if ((uint32_t)tid < nf) {
-        s_start[tid] = fac_off[f0 + tid] - base;
-        s_meta[tid] = fac_meta[f0 + tid];
-        s_p[tid] = fac_p[f0 + tid];
-        s_q[tid] = fac_q[f0 + tid];
+        sm.start[tid] = fac_off[f0 + tid] - base;
+        sm.meta[tid] = fac_meta[f0 + tid];
+        sm.p[tid] = fac_p[f0 + tid];
+        sm.q[tid] = fac_q[f0 + tid];
     }
-    if (tid == 0) s_start[nf] = n;
+    if (tid == 0) sm.start[nf] = n;
     for (uint32_t e = tid; e < n; e += kBlock) {
-        s_in[e] = kLayout == kLayoutPull ? m_vf[fac_vpos[base + e]] : m_vf[base + e];
-        if (kDamp) s_out[e] = m_fv_old[base + e];
+        sm.in[e] = kLayout == kLayoutPull ? m_vf[fac_vpos[base + e]] : m_vf[base + e];
+        if (kDamp) sm.out[e] = m_fv_old[base + e];
     }
     __syncthreads();
-    /* phase 2: one lane per output slot, in the block's work order
+    /* phase 2: one lane per output slot, in the tile's work order
      * (w = local slot | local factor << 10), from LDS only */
 #pragma unroll 1
     for (int j = 0; j < kPerLane; j++) {
@@ -161,15 +177,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
 #pragma unroll
         for (int x = 0; x < kPerLane; x++) wj |= (j == x) ? w[x] : 0u;
         const uint32_t sl = wj & 1023u, lf = wj >> 10;
-        const uint32_t s = s_start[lf];
-        const int d = (int)(s_start[lf + 1] - s);
+        const uint32_t s = sm.start[lf];
+        const int d = (int)(sm.start[lf + 1] - s);
         const int t = (int)(sl - s);
-        const uint32_t meta = s_meta[lf];
+        const uint32_t meta = sm.meta[lf];
         const uint32_t type = meta & 3u;
         const bool clamped = (meta & 4u) != 0;
         const int h = (int)(meta >> 8);
-        const double p = s_p[lf];
-        const double q = type ? s_q[lf] : 0.0;
+        const double p = sm.p[lf];
+        const double q = type ? sm.q[lf] : 0.0;
         double mask0 = 1.0, mask1 = 1.0;
         if (clamped) {
             const double2 mk = fac_mask[f0 + lf];
@@ -180,41 +196,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
         if (d <= N) {
             RegIn<N> in;
 #pragma unroll
-            for (int jj = 0; jj < N; jj++) in.m[jj] = jj < d ? s_in[s + jj] : make_double2(0.0, 0.0);
+            for (int jj = 0; jj < N; jj++) in.m[jj] = jj < d ? sm.in[s + jj] : make_double2(0.0, 0.0);
             const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
             r = fv_msg<N>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
         } else {
-            LdsIn in{s_in + s};
+            LdsIn in{sm.in + s};
             const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
             r = fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
         }
         if (kDamp) {
-            const double2 old = s_out[sl];
+            const double2 old = sm.out[sl];
             r.x = keep * r.x + damp * old.x;
             r.y = keep * r.y + damp * old.y;
         }
-        s_out[sl] = r;
+        sm.out[sl] = r;
     }
     __syncthreads();
-    for (uint32_t e = tid; e < n; e += kBlock) put_fv<kDamp, kLayout>(m_fv, m_fv_vm, fac_vpos, base + e, s_out[e]);
+    /* phase 3: stores */
+    for (uint32_t e = tid; e < n; e += kBlock) put_fv<kDamp, kLayout>(m_fv, m_fv_vm, fac_vpos, base + e, sm.out[e]);
 }
 
-/* f->v messages read var-major (scatter layout) or gathered from their
- * factor slots; v->f messages stored var-major (pull layout) or scattered
- * to their factor slots */
+/* The variable pass over variables [v0, v1) of one graph (graph-local
+ * pointers).  f->v messages are read var-major (scatter layout) or
+ * gathered from their factor slots; v->f messages are stored var-major
+ * (pull layout) or scattered to their factor slots.  Returns the lane's
+ * max |b - b_old| as IEEE bits (0 for lanes without a variable). */
 template <int kLayout>
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_staged(
-    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ var_off,
-    const uint32_t *__restrict__ var_edge, const uint16_t *__restrict__ var_work,
-    const double2 *__restrict__ m_fv, double2 *__restrict__ m_vf, double *__restrict__ belief,
-    unsigned long long *__restrict__ ctl, int k, double tol) {
-    if (chunk_stopped(ctl, k, tol)) return;
-    __shared__ double2 s_m[kStage];
-    __shared__ unsigned long long red[kBlock / 64];
-    __shared__ double s_prod[2][kBlock];
-    __shared__ uint32_t s_z[kBlock];          /* z0 | z1 << 16 (a staged variable has <= kStage edges) */
-    __shared__ uint8_t s_owner[kStage];       /* local variable of each staged slot */
-    const uint32_t v0 = blk[blockIdx.x], v1 = blk[blockIdx.x + 1];
+__device__ __forceinline__ unsigned long long var_tile(
+    const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
+    const uint16_t *__restrict__ var_work, const double2 *__restrict__ m_fv, double2 *__restrict__ m_vf,
+    double *__restrict__ belief, uint32_t v0, uint32_t v1, VarSmem &sm) {
     const uint32_t base = var_off[v0];
     const uint32_t n = var_off[v1] - base;
     const int tid = threadIdx.x;
@@ -228,11 +239,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
         for (uint32_t c = 0; c < n; c += kStage) {
             const uint32_t m = n - c < (uint32_t)kStage ? n - c : (uint32_t)kStage;
             for (uint32_t e = tid; e < m; e += kBlock)
-                s_m[e] = kLayout == kLayoutScatter ? m_fv[base + c + e] : m_fv[var_edge[base + c + e]];
+                sm.m[e] = kLayout == kLayoutScatter ? m_fv[base + c + e] : m_fv[var_edge[base + c + e]];
             __syncthreads();
             if (tid == 0) {
                 for (uint32_t e = 0; e < m; e++) {
-                    const double2 x = s_m[e];
+                    const double2 x = sm.m[e];
                     if (x.x == 0) z0++; else prod_0 *= x.x;
                     if (x.y == 0) z1++; else prod_1 *= x.y;
                     scale_pair(prod_0, prod_1);
@@ -241,10 +252,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
             __syncthreads();
         }
         if (tid == 0) {
-            s_prod[0][0] = prod_0;
-            s_prod[1][0] = prod_1;
-            s_z[0] = z0;
-            s_z[1] = z1;
+            sm.prod[0][0] = prod_0;
+            sm.prod[1][0] = prod_1;
+            sm.z[0] = z0;
+            sm.z[1] = z1;
             const double b0 = prod_0 * (double)(z0 == 0);
             const double b1 = prod_1 * (double)(z1 == 0);
             const double b = b1 / (b0 + b1);
@@ -253,68 +264,94 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
             diff_bits = (unsigned long long)__double_as_longlong(diff);
         }
         __syncthreads();
-        prod_0 = s_prod[0][0];
-        prod_1 = s_prod[1][0];
-        z0 = s_z[0];
-        z1 = s_z[1];
+        prod_0 = sm.prod[0][0];
+        prod_1 = sm.prod[1][0];
+        z0 = sm.z[0];
+        z1 = sm.z[1];
         for (uint32_t e = tid; e < n; e += kBlock) {
             const uint32_t sl = var_edge[base + e];
             const double2 x = kLayout == kLayoutScatter ? m_fv[base + e] : m_fv[sl];
             m_vf[kLayout == kLayoutPull ? base + e : sl] = vf_msg(prod_0, prod_1, z0, z1, x);
         }
-        block_fold_diff(ctl, k, diff_bits, red);
-        return;
+        return diff_bits;
     }
     /* phase 1: offsets, old beliefs and the work entry of each variable,
-     * and the block's messages, all loads independent of each other */
-    __shared__ uint32_t s_start[kBlock + 1];
-    __shared__ double s_bel[kBlock];
+     * and the tile's messages, all loads independent of each other */
     const uint32_t nv = v1 - v0;
     uint32_t lv = 0;
     if ((uint32_t)tid < nv) {
         lv = var_work[v0 + tid];
-        s_start[tid] = var_off[v0 + tid] - base;
-        s_bel[tid] = belief[v0 + tid];
+        sm.start[tid] = var_off[v0 + tid] - base;
+        sm.bel[tid] = belief[v0 + tid];
     }
-    if (tid == 0) s_start[nv] = n;
+    if (tid == 0) sm.start[nv] = n;
     for (uint32_t e = tid; e < n; e += kBlock)
-        s_m[e] = kLayout == kLayoutScatter ? m_fv[base + e] : m_fv[var_edge[base + e]];
+        sm.m[e] = kLayout == kLayoutScatter ? m_fv[base + e] : m_fv[var_edge[base + e]];
     __syncthreads();
-    /* phase 2: products, one lane per variable in the block's work order
+    /* phase 2: products, one lane per variable in the tile's work order
      * (by degree), from LDS only */
     if ((uint32_t)tid < nv) {
-        const uint32_t s = s_start[lv];
-        const int d = (int)(s_start[lv + 1] - s);
+        const uint32_t s = sm.start[lv];
+        const int d = (int)(sm.start[lv + 1] - s);
         double prod_0 = 1.0, prod_1 = 1.0;
         uint32_t z0 = 0, z1 = 0;
         for (int j = 0; j < d; j++) {
-            const double2 x = s_m[s + j];
+            const double2 x = sm.m[s + j];
             if (x.x == 0) z0++; else prod_0 *= x.x;
             if (x.y == 0) z1++; else prod_1 *= x.y;
             scale_pair(prod_0, prod_1);
-            s_owner[s + j] = (uint8_t)lv;
+            sm.owner[s + j] = (uint8_t)lv;
         }
         const double b0 = prod_0 * (double)(z0 == 0);
         const double b1 = prod_1 * (double)(z1 == 0);
         const double b = b1 / (b0 + b1);
-        const double diff = fabs(s_bel[lv] - b);
-        s_bel[lv] = b;
+        const double diff = fabs(sm.bel[lv] - b);
+        sm.bel[lv] = b;
         diff_bits = (unsigned long long)__double_as_longlong(diff);
-        s_prod[0][lv] = prod_0;
-        s_prod[1][lv] = prod_1;
-        s_z[lv] = z0 | (z1 << 16);
+        sm.prod[0][lv] = prod_0;
+        sm.prod[1][lv] = prod_1;
+        sm.z[lv] = z0 | (z1 << 16);
     }
     __syncthreads();
-    /* leave-one-out messages: one lane per slot, stored coalesced (var-major)
-     * or scattered to the factor slots (push layout) */
+    /* phase 3: leave-one-out messages, one lane per slot, stored coalesced
+     * (var-major) or scattered to the factor slots */
     for (uint32_t e = tid; e < n; e += kBlock) {
-        const uint32_t lv = s_owner[e];
-        const uint32_t z = s_z[lv];
-        const double2 r = vf_msg(s_prod[0][lv], s_prod[1][lv], z & 0xffffu, z >> 16, s_m[e]);
+        const uint32_t lv2 = sm.owner[e];
+        const uint32_t z = sm.z[lv2];
+        const double2 r = vf_msg(sm.prod[0][lv2], sm.prod[1][lv2], z & 0xffffu, z >> 16, sm.m[e]);
         m_vf[kLayout == kLayoutPull ? base + e : var_edge[base + e]] = r;
     }
-    if ((uint32_t)tid < nv) belief[v0 + tid] = s_bel[tid];
-    block_fold_diff(ctl, k, diff_bits, red);
+    if ((uint32_t)tid < nv) belief[v0 + tid] = sm.bel[tid];
+    return diff_bits;
+}
+
+/* the factor pass of a plain solve: block b owns factors [blk[b], blk[b+1]) */
+template <bool kDamp, int N, int kLayout>
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_staged(
+    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ fac_off,
+    const uint32_t *__restrict__ fac_meta, const double *__restrict__ fac_p,
+    const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
+    const uint32_t *__restrict__ fac_vpos, const uint32_t *__restrict__ fac_work,
+    const double2 *__restrict__ m_vf, const double2 *m_fv_old, double2 *m_fv, double2 *__restrict__ m_fv_vm,
+    double damp, double keep, const unsigned long long *__restrict__ ctl, int k, double tol) {
+    if (chunk_stopped(ctl, k, tol)) return;
+    __shared__ FactorSmem sm;
+    factor_tile<kDamp, N, kLayout>(fac_off, fac_meta, fac_p, fac_q, fac_mask, fac_vpos, fac_work, m_vf, m_fv_old,
+                                   m_fv, m_fv_vm, damp, keep, blk[blockIdx.x], blk[blockIdx.x + 1], sm);
+}
+
+/* the variable pass of a plain solve: block b owns variables [blk[b], blk[b+1]) */
+template <int kLayout>
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_staged(
+    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ var_off,
+    const uint32_t *__restrict__ var_edge, const uint16_t *__restrict__ var_work,
+    const double2 *__restrict__ m_fv, double2 *__restrict__ m_vf, double *__restrict__ belief,
+    unsigned long long *__restrict__ ctl, int k, double tol) {
+    if (chunk_stopped(ctl, k, tol)) return;
+    __shared__ VarSmem sm;
+    const unsigned long long bits =
+        var_tile<kLayout>(var_off, var_edge, var_work, m_fv, m_vf, belief, blk[blockIdx.x], blk[blockIdx.x + 1], sm);
+    block_fold_diff(ctl, k, bits, sm.red);
 }
 
 }  // namespace

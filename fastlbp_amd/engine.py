@@ -14,7 +14,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._native import (CsrC, FactorListC, GenParamsC, RunOptsC, RunResultC, call, f64p, i32p,
+from ._native import (CsrC, FactorListC, GenParamsC, RunOptsC, RunResultC, call, check, f64p, i32p,
                       lib, u8p, u64p)
 
 
@@ -274,12 +274,22 @@ class Plan:
         p = np.ascontiguousarray(p, dtype=np.float64)
         call("flbp_plan_set_p", self._h, _ptr(p, C.c_double), p.shape[0])
 
+    @property
+    def variant(self) -> int:
+        """The kernel variant in use (a new plan picks 3 or 0 by footprint)."""
+        v = int(lib().flbp_plan_get_variant(self._h))
+        if v < 0:
+            check("flbp_plan_get_variant", v)
+        return v
+
     def set_variant(self, variant: int) -> None:
         """Kernel shape and message layout: 0 LDS-staged, v->f var-major and
-        f->v factor-major (default); 1 one lane per factor/variable, same
-        layout; 2 LDS-staged, both factor-major (push); 3 LDS-staged, v->f
-        factor-major and f->v var-major (both passes scatter).  Results are
-        identical; switching layouts moves the stored messages."""
+        f->v factor-major (the default above the Infinity Cache size); 1 one
+        lane per factor/variable, same layout; 2 LDS-staged, both
+        factor-major (push); 3 LDS-staged, v->f factor-major and f->v
+        var-major (both passes scatter; the default for plans that fit the
+        cache).  Results are identical; switching layouts moves the stored
+        messages."""
         call("flbp_plan_set_variant", self._h, variant)
 
     def profile(self, enable: bool = True) -> None:

@@ -208,16 +208,20 @@ int flbp_plan_profile(flbp_plan *plan, int enable);
 int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
                           uint64_t *fac_launches, uint64_t *var_launches);
 
-/* Kernel shape and message layout of the sweep.  3 (default): LDS-staged
- * blocks over balanced edge ranges, v->f messages factor-major and f->v
- * var-major, so both passes read contiguously and scatter their outputs
- * (plus a factor-major f->v copy written by damped sweeps); 0: LDS-staged,
- * v->f var-major and f->v factor-major (both passes gather, all stores
+/* Kernel shape and message layout of the sweep.  3: LDS-staged blocks
+ * over balanced edge ranges, v->f messages factor-major and f->v var-major,
+ * so both passes read contiguously and scatter their outputs (plus a
+ * factor-major f->v copy written by damped sweeps); 0: LDS-staged, v->f
+ * var-major and f->v factor-major (both passes gather, all stores
  * contiguous); 1: one lane per factor / variable in layout 0; 2:
- * LDS-staged, both arrays factor-major.  Results are identical; switching
- * between layouts moves the stored messages, so a solve may continue under
- * another variant.  flbp_adjoint_create switches its plan to 0. */
+ * LDS-staged, both arrays factor-major.  A new plan starts with 3 when its
+ * device footprint fits the Infinity Cache (at most 320 MB; random scatters
+ * are cheaper than random gathers there) and with 0 otherwise (in HBM
+ * random gathers are cheaper).  Results are identical; switching between
+ * layouts moves the stored messages, so a solve may continue under another
+ * variant.  flbp_adjoint_create switches its plan to 0. */
 int flbp_plan_set_variant(flbp_plan *plan, int variant);
+int flbp_plan_get_variant(const flbp_plan *plan);
 
 uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
 
@@ -233,12 +237,19 @@ uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
  * bit-identical messages and beliefs. */
 typedef struct flbp_batch flbp_batch;
 
+/* message layouts of a batch (see flbp_plan_set_variant: pull is the
+ * layout of variant 0, scatter that of variant 3) */
+#define FLBP_LAYOUT_AUTO 0u      /* scatter while the batch fits the Infinity Cache, else pull */
+#define FLBP_LAYOUT_PULL 1u
+#define FLBP_LAYOUT_SCATTER 2u
+
 typedef struct flbp_batch_opts {
-    uint32_t tile_factors;   /* factors per tile (one block walks a tile) */
-    uint32_t tile_vars;      /* variables per tile */
+    uint32_t tile_factors;   /* edge slots per factor tile, 64..1024 (one block per tile at a time) */
+    uint32_t tile_vars;      /* edge slots per variable tile, 64..1024 */
     uint32_t grid_blocks;    /* persistent grid size; 0 = resident blocks */
     uint32_t chunk;          /* sweeps per hipGraph replay (<= 64) */
     uint32_t use_graph;      /* 0: launch the chunk's kernels directly */
+    uint32_t layout;         /* FLBP_LAYOUT_*; changing it moves the stored messages */
 } flbp_batch_opts;
 
 void flbp_batch_default_opts(flbp_batch_opts *o);
@@ -256,8 +267,8 @@ int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *opts);
 int flbp_batch_get_opts(flbp_batch *b, flbp_batch_opts *opts);
 /* iteration counters and beliefs back to their creation values */
 int flbp_batch_reset_counters(flbp_batch *b);
-/* times trial_sweeps sweeps for each candidate tile size and grid size,
- * keeps the fastest, then resets counters and re-inits */
+/* times trial_sweeps sweeps for each candidate message layout, tile size
+ * and grid size, keeps the fastest, then resets counters and re-inits */
 int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps,
                         flbp_batch_opts *best);
 uint64_t flbp_batch_device_bytes(const flbp_batch *b);

@@ -32,7 +32,9 @@ def plain(graph, tol, maxiter, damping):
 
 
 @pytest.mark.parametrize("opts", [dict(), dict(tile_factors=256, tile_vars=512, grid_blocks=37, chunk=5),
-                                  dict(use_graph=0, chunk=16), dict(tile_factors=4096, tile_vars=4096)])
+                                  dict(use_graph=0, chunk=16), dict(tile_factors=64, tile_vars=96, grid_blocks=3),
+                                  dict(tile_factors=4096, tile_vars=4096), dict(layout=1), dict(layout=2),
+                                  dict(layout=1, use_graph=0, chunk=9)])
 def test_batch_matches_plain_solves(opts):
     graphs = batch_graphs(1, 12, 6)
     tol, maxiter, damping = 1e-9, 300, 0.3
@@ -61,8 +63,50 @@ def test_autotune_keeps_results():
     graphs = batch_graphs(3, 8, 4)
     b = Batch(graphs)
     best = b.autotune(0.3, trial_sweeps=4)
-    assert best["tile_factors"] in (256, 1024, 4096)
+    assert best["tile_factors"] in (512, 1024) and best["layout"] in (1, 2)
     iters, _, _ = b.run(1e-9, 300, 0.3)
     for g, graph in enumerate(graphs):
         it, _, bp = plain(graph, 1e-9, 300, 0.3)
         assert iters[g] == it and same_bits(b.beliefs(g), bp)
+
+
+def test_undamped_then_damped_runs():
+    """Undamped sweeps skip the factor-major f->v copy that damping reads;
+    a damped run after them must rebuild it and continue exactly as the
+    plain solves do."""
+    graphs = batch_graphs(4, 6, 3)
+    b = Batch(graphs, chunk=6)
+    b.init()
+    b.run(1e-14, 5, 0.0)
+    iters, md, _ = b.run(1e-9, 80, 0.4)
+    for g, graph in enumerate(graphs):
+        p = fl.Plan(graph)
+        p.init()
+        p.run(1e-14, 5, 0.0)
+        r = p.run(1e-9, 80, 0.4)
+        assert iters[g] == r.iters and same_bits([md[g]], [r.maxdiff])
+        assert same_bits(b.beliefs(g), p.beliefs()), g
+
+
+def test_layout_switch_mid_run():
+    """set_opts(layout=...) moves the stored messages: a run continued under
+    the other layout matches plain solves that never switched."""
+    graphs = batch_graphs(5, 5, 2)
+    b = Batch(graphs, chunk=4, layout=2)
+    b.init()
+    b.run(1e-14, 6, 0.0)
+    b.set_opts(layout=1)
+    b.run(1e-14, 11, 0.3)
+    b.set_opts(layout=2)
+    b.run(1e-14, 14, 0.0)
+    b.set_opts(layout=1)
+    iters, md, _ = b.run(1e-9, 90, 0.3)
+    for g, graph in enumerate(graphs):
+        p = fl.Plan(graph)
+        p.init()
+        p.run(1e-14, 6, 0.0)
+        p.run(1e-14, 11, 0.3)
+        p.run(1e-14, 14, 0.0)
+        r = p.run(1e-9, 90, 0.3)
+        assert iters[g] == r.iters and same_bits([md[g]], [r.maxdiff])
+        assert same_bits(b.beliefs(g), p.beliefs()), g

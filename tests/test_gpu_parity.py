@@ -179,6 +179,26 @@ def test_variant_switch_mid_run_keeps_state():
     assert same_bits(fv, ofv) and same_bits(vf, ovf)
 
 
+def test_layout_default_follows_footprint():
+    """A plan that fits the Infinity Cache starts in the scatter layout, a
+    larger one in the pull layout; both give the same bits."""
+    small = fl.Plan(fl.Graph(fl.generate(5000, 71).factors))
+    assert small.variant == 3
+    gen = fl.generate(1_600_000, 72)
+    g = fl.Graph(gen.factors)
+    big = fl.Plan(g)
+    assert big.variant == 0 and big.device_bytes > 320 * 2**20
+    other = fl.Plan(g)
+    other.set_variant(3)
+    for p in (big, other):
+        p.init()
+        p.run_fixed(6, 0.3)
+    assert same_bits(big.beliefs(), other.beliefs())
+    fa, va = big.messages()
+    fb, vb = other.messages()
+    assert same_bits(fa, fb) and same_bits(va, vb)
+
+
 def wide_graph():
     """A leaf used by 3000 ANDs (a variable with > 1024 edges), an OR over
     1500 clauses (a factor with > 1024 slots), and ordinary structure."""
