@@ -41,6 +41,105 @@ struct LdsIn {
     __device__ __forceinline__ double2 pick(int j) const { return m[j]; }
 };
 
+/* One f->v output from a compact operand list: ops[0..cnt) are the
+ * messages of the factor's slots other than the target t (and, for a body
+ * target, other than the head h), in slot order -- exactly the slots the
+ * per-output loops of fv_msg visit, in the same order -- so the loops run
+ * cnt steps with no per-slot tests and the arithmetic is fv_msg's.  mh is
+ * the head slot's message; for a degree-2 head output ops[0] is the other
+ * slot. */
+template <int M>
+__device__ __forceinline__ double2 fv_list(const double2 (&ops)[M], int cnt, uint32_t type, bool clamped, int d,
+                                           bool head_target, double2 mh, double p, double q, double mask0,
+                                           double mask1) {
+    double2 r;
+    if (type == 0) {
+        r.x = 1 - p;
+        r.y = p;
+        return r;
+    }
+    const bool is_and = (type == 1);
+    if (!head_target) {
+        /* body_msg */
+        const double hx = is_and ? mh.y : mh.x, hy = is_and ? mh.x : mh.y;
+        const double mx = is_and ? mask1 : mask0, my = is_and ? mask0 : mask1;
+        double P = (p - q) * (hx * mx - hy * my);
+        double Q = q * hx * mx + (1 - q) * hy * my;
+        scale_pair(P, Q);
+        if (Q != 0) {
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                if (k < cnt) {
+                    const double2 m = ops[k];
+                    Q *= m.x + m.y;
+                    P *= is_and ? m.y : m.x;
+                    scale_pair(P, Q);
+                }
+            }
+        }
+        const double QP = Q + P;
+        const double marg = Q + QP;
+        if (marg != 0) {
+            const double a = Q / marg, b = QP / marg;
+            r.x = is_and ? a : b;
+            r.y = is_and ? b : a;
+        } else {
+            r.x = 0.5;
+            r.y = 0.5;
+        }
+        return r;
+    }
+    /* head_uw */
+    double u, w;
+    if (d == 1) {
+        u = p;
+        w = 1 - p;
+    } else if (d == 2) {
+        const double2 m = ops[0];
+        const double x = is_and ? m.y : m.x, y = is_and ? m.x : m.y;
+        u = p * x + q * y;
+        w = (1 - p) * x + (1 - q) * y;
+    } else {
+        double px = 1.0, p10 = 1.0, e1 = 0;
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            if (k < cnt) {
+                const double2 m = ops[k];
+                const double a0 = m.x + m.y;
+                const double x = is_and ? m.y : m.x;
+                const double delta = is_and ? m.x : m.y;
+                px *= x;
+                p10 *= a0;
+                scale_pair(px, p10);
+                if (x != 0 && a0 == x && delta != 0) e1 += delta / x;
+            }
+        }
+        const double R = e1 * p10 + (p10 - px);
+        u = p * px + q * R;
+        w = (1 - p) * px + (1 - q) * R;
+    }
+    double marg0 = is_and ? w : u, marg1 = is_and ? u : w;
+    if (clamped) {
+        marg0 *= mask0;
+        marg1 *= mask1;
+    }
+    const double marg = marg0 + marg1;
+    if (marg != 0) {
+        r.x = marg0 / marg;
+        r.y = marg1 / marg;
+    } else if (clamped && mask0 == 0) {
+        r.x = 0.0;
+        r.y = 1.0;
+    } else if (clamped && mask1 == 0) {
+        r.x = 1.0;
+        r.y = 0.0;
+    } else {
+        r.x = 0.5;
+        r.y = 0.5;
+    }
+    return r;
+}
+
 /* block-wide max of per-lane diff bits, folded into the sweep's slots */
 __device__ __forceinline__ void block_fold_diff(unsigned long long *ctl, int k, unsigned long long bits,
                                                 unsigned long long *red) {
@@ -194,11 +293,21 @@ __device__ __forceinline__ void factor_tile(
         }
         double2 r;
         if (d <= N) {
-            RegIn<N> in;
+            /* the other slots' messages, in slot order, from LDS */
+            const bool head_target = (t == h);
+            const int skip_h = head_target ? -1 : h;
+            const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
+            double2 ops[N - 1];
+            int js = 0;
 #pragma unroll
-            for (int jj = 0; jj < N; jj++) in.m[jj] = jj < d ? sm.in[s + jj] : make_double2(0.0, 0.0);
-            const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
-            r = fv_msg<N>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
+            for (int k = 0; k < N - 1; k++) {
+                js += (js == t || js == skip_h);
+                js += (js == t || js == skip_h);
+                const double2 v = sm.in[s + (k < cnt ? js : 0)];   /* index kept inside the factor */
+                ops[k] = k < cnt ? v : make_double2(0.0, 0.0);
+                js++;
+            }
+            r = fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, sm.in[s + h], p, q, mask0, mask1);
         } else {
             LdsIn in{sm.in + s};
             const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
