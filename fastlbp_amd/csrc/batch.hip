@@ -58,9 +58,14 @@ struct Batch {
     uint32_t *fac_work = nullptr;           /* per tile: its slots in work order (graph-local tile slots) */
     uint16_t *var_work = nullptr;           /* per tile: its variables by degree */
     std::vector<uint32_t> h_fac_off, h_fac_meta, h_var_off;   /* host copies, batch-concatenated */
-    Tile *ftiles = nullptr, *vtiles = nullptr;
+    Tile *ftiles = nullptr, *vtiles = nullptr;           /* every graph's tiles */
     uint64_t cap_ftiles = 0, cap_vtiles = 0;
     uint32_t n_ftiles = 0, n_vtiles = 0;
+    std::vector<Tile> h_ftiles, h_vtiles;                 /* host copies, grouped by graph */
+    std::vector<uint32_t> h_ftile_off, h_vtile_off;       /* per graph: its tiles' range (G+1) */
+    Tile *act_ftiles = nullptr, *act_vtiles = nullptr;    /* tiles of the running graphs */
+    uint32_t *act_count = nullptr;                        /* device: [factor tiles, variable tiles] */
+    std::vector<uint8_t> act_set;                         /* running set the active lists hold */
     uint32_t cut_f = 0, cut_v = 0;          /* slot bounds of the current tiles */
     unsigned long long *diff = nullptr;     /* [kMaxChunk][G][kBatchSlots][kCtlPad] partial maxima */
     unsigned long long *diff_out = nullptr; /* [kMaxChunk][G] compacted */
@@ -117,6 +122,29 @@ struct Batch {
         if (stream) (void)hipStreamDestroy(stream);
     }
     int build_tiles();
+    /* the sweep kernels' tile lists: the tiles of the graphs with run[g]
+     * set, uploaded when that set changes */
+    int set_active(const uint8_t *run) {
+        if (act_set.size() == G && std::equal(act_set.begin(), act_set.end(), run)) return FLBP_OK;
+        act_set.assign(run, run + G);
+        std::vector<Tile> fa, va;
+        for (uint64_t g = 0; g < G; g++) {
+            if (!run[g]) continue;
+            fa.insert(fa.end(), h_ftiles.begin() + h_ftile_off[g], h_ftiles.begin() + h_ftile_off[g + 1]);
+            va.insert(va.end(), h_vtiles.begin() + h_vtile_off[g], h_vtiles.begin() + h_vtile_off[g + 1]);
+        }
+        const uint32_t cnt[2] = {(uint32_t)fa.size(), (uint32_t)va.size()};
+        if (cnt[0] && hipMemcpyAsync(act_ftiles, fa.data(), fa.size() * sizeof(Tile), hipMemcpyHostToDevice, stream) !=
+                          hipSuccess)
+            return fail(FLBP_EHIP, "batch: uploading the active tiles failed");
+        if (cnt[1] && hipMemcpyAsync(act_vtiles, va.data(), va.size() * sizeof(Tile), hipMemcpyHostToDevice, stream) !=
+                          hipSuccess)
+            return fail(FLBP_EHIP, "batch: uploading the active tiles failed");
+        if (hipMemcpyAsync(act_count, cnt, sizeof cnt, hipMemcpyHostToDevice, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess)   /* the host vectors die here */
+            return fail(FLBP_EHIP, "batch: uploading the active tiles failed");
+        return FLBP_OK;
+    }
     int resolve_layout() const {
         if (opts.layout == FLBP_LAYOUT_PULL) return kLayoutPull;
         if (opts.layout == FLBP_LAYOUT_SCATTER) return kLayoutScatter;
@@ -224,6 +252,23 @@ int Batch::build_tiles() {
     }
     if (n_ftiles) HIP_TRY(hipMemcpy(ftiles, fall.data(), fall.size() * sizeof(Tile), hipMemcpyHostToDevice));
     if (n_vtiles) HIP_TRY(hipMemcpy(vtiles, vall.data(), vall.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    h_ftile_off.assign(G + 1, 0);
+    h_vtile_off.assign(G + 1, 0);
+    for (uint64_t g = 0; g < G; g++) {
+        h_ftile_off[g + 1] = h_ftile_off[g] + (uint32_t)ft[g].size();
+        h_vtile_off[g + 1] = h_vtile_off[g] + (uint32_t)vt[g].size();
+    }
+    h_ftiles.swap(fall);
+    h_vtiles.swap(vall);
+    if (act_ftiles) release(act_ftiles);
+    if (act_vtiles) release(act_vtiles);
+    act_ftiles = act_vtiles = nullptr;
+    TRY(alloc(act_ftiles, n_ftiles));
+    TRY(alloc(act_vtiles, n_vtiles));
+    if (!act_count) TRY(alloc(act_count, 2));
+    act_set.clear();
+    std::vector<uint8_t> all(G, 1);
+    TRY(set_active(all.data()));
     if (Etot) HIP_TRY(hipMemcpy(fac_work, fw.data(), Etot * 4, hipMemcpyHostToDevice));
     if (Vtot) HIP_TRY(hipMemcpy(var_work, vw.data(), Vtot * 2, hipMemcpyHostToDevice));
     cut_f = sf;
@@ -243,8 +288,8 @@ int Batch::record_chunk(int n, double tol, double damping) {
         if (n_ftiles) {
             const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
 #define FLBP_BATCH_FACTOR_PASS(DAMP, N, L)                                                                    \
-    hipLaunchKernelGGL((batch_factor_pass<DAMP, N, L>), dim3(fg), dim3(kBlock), 0, stream, graphs, ftiles,     \
-                       n_ftiles, fac_off, fac_meta, fac_p, fac_q, fac_mask, fac_vpos, fac_work, m_vf, m_fv,     \
+    hipLaunchKernelGGL((batch_factor_pass<DAMP, N, L>), dim3(fg), dim3(kBlock), 0, stream, graphs, act_ftiles, \
+                       act_count, fac_off, fac_meta, fac_p, fac_q, fac_mask, fac_vpos, fac_work, m_vf, m_fv,    \
                        m_fv_vm, damping, keep, c, k)
 #define FLBP_BATCH_FACTOR_PASS_L(DAMP, N)                                                   \
     do {                                                                                    \
@@ -263,11 +308,11 @@ int Batch::record_chunk(int n, double tol, double damping) {
         }
         if (n_vtiles) {
             if (layout == kLayoutScatter)
-                hipLaunchKernelGGL(batch_var_pass<kLayoutScatter>, dim3(vg), dim3(kBlock), 0, stream, graphs, vtiles,
-                                   n_vtiles, var_off, var_edge, var_work, m_fv_vm, m_vf, belief, c, k);
+                hipLaunchKernelGGL(batch_var_pass<kLayoutScatter>, dim3(vg), dim3(kBlock), 0, stream, graphs,
+                                   act_vtiles, act_count + 1, var_off, var_edge, var_work, m_fv_vm, m_vf, belief, c, k);
             else
-                hipLaunchKernelGGL(batch_var_pass<kLayoutPull>, dim3(vg), dim3(kBlock), 0, stream, graphs, vtiles,
-                                   n_vtiles, var_off, var_edge, var_work, m_fv, m_vf, belief, c, k);
+                hipLaunchKernelGGL(batch_var_pass<kLayoutPull>, dim3(vg), dim3(kBlock), 0, stream, graphs,
+                                   act_vtiles, act_count + 1, var_off, var_edge, var_work, m_fv, m_vf, belief, c, k);
         }
     }
     const uint64_t waves = (uint64_t)n * G;
@@ -495,6 +540,7 @@ int flbp_batch_run(flbp_batch *b, const flbp_run_opts *o, uint64_t *iters_out, d
             const uint64_t left = o->maxiter > B->iters[g] ? o->maxiter - B->iters[g] : 0;
             B->h_budget[g] = (uint32_t)std::min<uint64_t>(left, 0xffffffffu);
         }
+        TRY(B->set_active(B->h_active));
         HIP_TRY(hipMemcpyAsync(B->start_active, B->h_active, G, hipMemcpyHostToDevice, B->stream));
         HIP_TRY(hipMemcpyAsync(B->budget, B->h_budget, G * 4, hipMemcpyHostToDevice, B->stream));
         TRY(B->enqueue_chunk(chunk, o->tol, o->damping));
@@ -613,6 +659,7 @@ int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps, fl
                 B->h_active[g] = 1;
                 B->h_budget[g] = (uint32_t)n;
             }
+            TRY(B->set_active(B->h_active));
             HIP_TRY(hipMemcpyAsync(B->start_active, B->h_active, B->G, hipMemcpyHostToDevice, B->stream));
             HIP_TRY(hipMemcpyAsync(B->budget, B->h_budget, B->G * 4, hipMemcpyHostToDevice, B->stream));
             TRY(B->enqueue_chunk(n, -INFINITY, damping));   /* warm: captures the graph */

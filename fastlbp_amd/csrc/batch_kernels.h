@@ -7,9 +7,11 @@
  * 32-bit values.  Work is cut into tiles of one graph each -- the staged
  * blocks of the plain solve (at most kBlock items and kStage edge slots,
  * with their work orders), in the pull or the scatter message layout of
- * staged_kernels.h -- and a persistent grid walks the tile list with a
+ * staged_kernels.h -- and a persistent grid walks a tile list with a
  * stride, so a launch uses exactly the blocks that fit on the chip
- * whatever the batch holds.
+ * whatever the batch holds.  The sweep kernels walk the list of the graphs
+ * still running (its length read from device memory, so a captured chunk
+ * stays valid when the host shortens the list between replays).
  *
  * Each graph keeps its own stop test: sweep k of a chunk runs on graph g
  * only if g was active when the chunk started, k is below its remaining
@@ -74,13 +76,14 @@ __device__ __forceinline__ bool graph_runs(const BatchCtl &c, uint32_t g, int k)
 
 template <bool kDamp, int N, int kLayout>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_pass(
-    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, uint32_t n_tiles,
+    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, const uint32_t *__restrict__ n_tiles_p,
     const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
     const double *__restrict__ fac_p, const double *__restrict__ fac_q,
     const double2 *__restrict__ fac_mask, const uint32_t *__restrict__ fac_vpos,
     const uint32_t *__restrict__ fac_work, const double2 *__restrict__ m_vf, double2 *m_fv,
     double2 *__restrict__ m_fv_vm, double damp, double keep, BatchCtl ctl, int k) {
     __shared__ FactorSmem sm;
+    const uint32_t n_tiles = *n_tiles_p;
     for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
         const Tile tile = tiles[ti];
         if (!graph_runs(ctl, tile.graph, k)) continue;   /* uniform per block */
@@ -96,11 +99,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_p
 
 template <int kLayout>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass(
-    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, uint32_t n_tiles,
+    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, const uint32_t *__restrict__ n_tiles_p,
     const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
     const uint16_t *__restrict__ var_work, const double2 *__restrict__ m_fv_in, double2 *__restrict__ m_vf,
     double *__restrict__ belief, BatchCtl ctl, int k) {
     __shared__ VarSmem sm;
+    const uint32_t n_tiles = *n_tiles_p;
     for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
         const Tile tile = tiles[ti];
         if (!graph_runs(ctl, tile.graph, k)) continue;   /* uniform per block */

@@ -457,6 +457,50 @@ int flbp_plan_set_p(flbp_plan *plan, const double *p, uint64_t n) {
     return FLBP_OK;
 }
 
+int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *factors, const uint8_t *values) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || (n && (!factors || !values))) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: null argument");
+    for (uint64_t i = 0; i < n; i++) {
+        if (factors[i] >= P->F) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: factor out of range");
+        if ((P->h_meta[factors[i]] & 3u) == 0)
+            return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: a prior factor is clamped through its p "
+                                           "(rebuild the graph with flbp_clamp)");
+        if (values[i] > 1) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: values are 0 or 1");
+    }
+    HIP_TRY(hipSetDevice(P->device));
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t I = factors[i];
+        const unsigned x = values[i];
+        double m0 = x == 0 ? 1.0 : 0.0, m1 = x == 1 ? 1.0 : 0.0;
+        if (P->h_meta[I] & 4u) {   /* a second clamp of the head multiplies */
+            m0 *= P->h_mask[I].x;
+            m1 *= P->h_mask[I].y;
+        }
+        P->h_meta[I] |= 4u;
+        P->h_mask[I] = make_double2(m0, m1);
+        P->clamp_dirty.push_back(I);
+        HIP_TRY(hipMemcpyAsync(P->fac_meta + I, &P->h_meta[I], 4, hipMemcpyHostToDevice, P->stream));
+        HIP_TRY(hipMemcpyAsync(P->fac_mask + I, &P->h_mask[I], 16, hipMemcpyHostToDevice, P->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return FLBP_OK;
+}
+
+int flbp_plan_reset_head_clamps(flbp_plan *plan) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_reset_head_clamps: null plan");
+    HIP_TRY(hipSetDevice(P->device));
+    for (uint64_t I : P->clamp_dirty) {
+        P->h_meta[I] = P->h_meta0[I];
+        P->h_mask[I] = P->h_mask0[I];
+        HIP_TRY(hipMemcpyAsync(P->fac_meta + I, &P->h_meta[I], 4, hipMemcpyHostToDevice, P->stream));
+        HIP_TRY(hipMemcpyAsync(P->fac_mask + I, &P->h_mask[I], 16, hipMemcpyHostToDevice, P->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->clamp_dirty.clear();
+    return FLBP_OK;
+}
+
 int flbp_plan_profile(flbp_plan *plan, int enable) {
     Plan *P = reinterpret_cast<Plan *>(plan);
     if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_profile: null plan");

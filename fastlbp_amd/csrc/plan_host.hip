@@ -219,6 +219,8 @@ int Plan::upload(const flbp_csr &c) {
         std::vector<double2> mk(F);
         for (uint64_t I = 0; I < F; I++) mk[I] = make_double2(c.mask0[I], c.mask1[I]);
         HIP_TRY(hipMemcpy(fac_mask, mk.data(), F * 16, hipMemcpyHostToDevice));
+        h_mask = mk;
+        h_mask0.swap(mk);
     }
     /* old beliefs are value-initialised to 0.0 (causal_bp.cu:371) */
     HIP_TRY(hipMemset(belief, 0, std::max<uint64_t>(V, 1) * 8));
@@ -230,6 +232,8 @@ int Plan::upload(const flbp_csr &c) {
     h_var_edge.assign(c.var_edge, c.var_edge + E);
     for (uint64_t I = 0; I < F; I++) max_fac_deg = std::max<uint64_t>(max_fac_deg, c.fac_off[I + 1] - c.fac_off[I]);
     for (uint64_t i = 0; i < V; i++) max_var_deg = std::max<uint64_t>(max_var_deg, c.var_off[i + 1] - c.var_off[i]);
+    h_meta0 = meta;
+    h_meta.swap(meta);
     variant = device_bytes <= kScatterMaxBytes ? 3 : 0;
     return FLBP_OK;
 }

@@ -70,6 +70,9 @@ struct Plan {
     std::vector<uint64_t> h_var_edge;              /* for message export/import */
     std::vector<uint32_t> h_perm;                  /* device variable -> variable */
     std::vector<uint32_t> h_fac_vpos;              /* factor-major slot -> var-major position */
+    std::vector<uint32_t> h_meta, h_meta0;         /* factor meta words: current and at creation */
+    std::vector<double2> h_mask, h_mask0;          /* head masks: current and at creation */
+    std::vector<uint64_t> clamp_dirty;             /* factors whose meta / mask differ from creation */
 
     uint64_t iters = 0;                            /* CausalBP::_iters */
     double last_maxdiff = INFINITY;

@@ -197,6 +197,20 @@ class Graph:
                     _ptr(self.p, C.c_double), _ptr(self.q, C.c_double),
                     _ptr(self.mask0, C.c_double), _ptr(self.mask1, C.c_double))
 
+    def head_factors(self, var_indices) -> np.ndarray:
+        """The defining factor (the factor whose head it is) of each variable,
+        -1 for a variable that heads none."""
+        head_var = self.edge_var[self.head_slot]
+        out = np.full(self.n_vars, -1, dtype=np.int64)
+        out[head_var] = np.arange(self.n_factors)
+        return out[np.asarray(var_indices, dtype=np.int64)]
+
+    def body_uses(self, var_indices) -> np.ndarray:
+        """Factors in which each variable appears other than as the head."""
+        v = np.asarray(var_indices, dtype=np.int64)
+        deg = (self.var_off[v + 1] - self.var_off[v]).astype(np.int64)
+        return deg - (self.head_factors(v) >= 0)
+
     def index_of(self, labels) -> np.ndarray:
         labels = np.asarray(labels, dtype=np.uint64)
         idx = np.searchsorted(self.var_label, labels)
@@ -273,6 +287,18 @@ class Plan:
     def set_p(self, p: np.ndarray) -> None:
         p = np.ascontiguousarray(p, dtype=np.float64)
         call("flbp_plan_set_p", self._h, _ptr(p, C.c_double), p.shape[0])
+
+    def set_head_clamps(self, factors, values) -> None:
+        """Head clamps in place (gen_clamped's head rule) on AND/OR factors;
+        equals rebuilding the graph clamped at variables with no body uses."""
+        f = np.ascontiguousarray(factors, dtype=np.uint64)
+        x = np.ascontiguousarray(values, dtype=np.uint8)
+        if f.shape != x.shape:
+            raise ValueError("factors and values differ in length")
+        call("flbp_plan_set_head_clamps", self._h, f.shape[0], _ptr(f, C.c_uint64), _ptr(x, C.c_uint8))
+
+    def reset_head_clamps(self) -> None:
+        call("flbp_plan_reset_head_clamps", self._h)
 
     @property
     def variant(self) -> int:

@@ -201,6 +201,20 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor,
  * training loop; the reference rebuilds the whole CausalBP instead). */
 int flbp_plan_set_p(flbp_plan *plan, const double *p, uint64_t n);
 
+/* Head clamps applied in place: for each listed AND/OR factor, the head
+ * rule of CausalFactor::gen_clamped (csrc/ext/causal_factor.cpp:75-83) --
+ * head_clamped set, mask (1, 0) for value 0 or (0, 1) for value 1,
+ * multiplied into an existing mask.  The graph structure is unchanged, so
+ * this equals rebuilding the graph clamped at variables that appear in no
+ * factor body (CausalFactorGraph::clamp, causal_fg.cpp:17-26); a clamp of a
+ * variable with body uses, or of a prior (Singleton) factor, needs the
+ * rebuild and is rejected here (FLBP_EINVAL).  Not in the reference: it
+ * lets a training loop switch between clamped variants without rebuilding
+ * the plan. */
+int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *factors, const uint8_t *values);
+/* every head clamp and mask back to the plan's creation values */
+int flbp_plan_reset_head_clamps(flbp_plan *plan);
+
 /* Kernel timing of the last run call: sum of device time of the factor and
  * variable passes in milliseconds, and launches of each (0 if not
  * recorded).  Enabled with flbp_plan_profile(plan, 1). */

@@ -199,6 +199,41 @@ def test_layout_default_follows_footprint():
     assert same_bits(fa, fb) and same_bits(va, vb)
 
 
+def test_head_clamps_in_place_equal_rebuilt_graph():
+    """Head clamps applied to a plan equal a graph rebuilt by clamp() at
+    alarms with no body uses; reset restores the unclamped solve."""
+    gen = fl.generate(30000, 81)
+    g = fl.Graph(gen.factors)
+    idx = g.index_of(gen.alarms)
+    free = gen.alarms[g.body_uses(idx) == 0]
+    rng = np.random.default_rng(81)
+    for trial in range(3):
+        labels = rng.choice(free, size=int(rng.integers(1, 40)), replace=False)
+        values = rng.integers(0, 2, labels.shape[0]).astype(np.uint8)
+        rebuilt = fl.Graph(fl.clamp(gen.factors, labels, values))
+        ref = fl.Plan(rebuilt)
+        ref.init()
+        r_ref = ref.run(1e-9, 300, 0.3)
+        plan = fl.Plan(g)
+        plan.set_head_clamps(g.head_factors(g.index_of(labels)), values)
+        plan.init()
+        r = plan.run(1e-9, 300, 0.3)
+        assert r.iters == r_ref.iters
+        b = plan.beliefs()
+        # the rebuilt graph orders variables the same way (labels unchanged)
+        assert same_bits(b, ref.beliefs()[rebuilt.index_of(g.var_label)])
+    plan.reset_head_clamps()
+    plan.reset()
+    plan.init()
+    r = plan.run(1e-9, 300, 0.3)
+    plain = fl.Plan(g)
+    plain.init()
+    assert r.iters == plain.run(1e-9, 300, 0.3).iters and same_bits(plan.beliefs(), plain.beliefs())
+    with pytest.raises(fl.FlbpError):
+        prior = np.flatnonzero(g.fac_type == 0)[:1]
+        plan.set_head_clamps(prior, np.zeros(1, np.uint8))
+
+
 def wide_graph():
     """A leaf used by 3000 ANDs (a variable with > 1024 edges), an OR over
     1500 clauses (a factor with > 1024 slots), and ordinary structure."""
