@@ -15,6 +15,7 @@ bounded number of sweeps of the same graph on this host's cores).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
     python bench.py --workload batched   # 512 graphs + 256 clamped variants vs sequential solves
     python bench.py --workload adjoint   # 100k-variable tied-rule gradient, backward vs forward
+    python bench.py --workload c5 --steps 20   # tied-rule fit over 64 clamped variants of a 20M graph
 """
 from __future__ import annotations
 
@@ -184,6 +185,113 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_c5(args):
+    """Tied-rule fitting over clamped variants of one large graph, on one GPU.
+
+    A generated graph (20M variables by default, 200 tied AND rules, 1,000
+    alarms).  Each variant clamps 1-50 alarms that have no body uses, to 0 or
+    1, in place on the one plan (head clamps; the structure is unchanged).
+    Targets are the variants' alarm marginals at a reference theta; the fit
+    starts from a perturbed theta.  A gradient step runs, for every variant,
+    the adjoint forward (T damped sweeps from init) and backward, sums
+    dL/dp over the variants and reduces it to dL/dtheta once, then takes an
+    Adam step.  Afterwards plain LBP runs to convergence at the fitted theta
+    on every variant, against the T-sweep marginals."""
+    import numpy as np
+    import fastlbp_amd as fl
+    from fastlbp_amd.adjoint import TiedProblem, rule_grad, tied_p
+
+    n_vars = args.n_vars or 20_000_000
+    n_var = args.c5_variants
+    T, k, damp, n_rules = args.c5_sweeps, args.c5_checkpoint, 0.5, 200
+    rng = np.random.default_rng(5)
+    t0 = time.perf_counter()
+    gen = fl.generate(n_vars, 5, n_rules=n_rules, n_alarms=1000)
+    g = fl.Graph(gen.factors)
+    prob = TiedProblem(g, gen.rule, n_rules, gen.alarms, T, k, damp)
+    plan, adj = prob.plan, prob.adj
+    aidx = prob.alarm_idx
+    free = np.flatnonzero(g.body_uses(aidx) == 0)
+    heads = g.head_factors(aidx)
+    variants = []
+    for _ in range(n_var):
+        pick = rng.choice(free, size=int(rng.integers(1, min(51, len(free) + 1))), replace=False)
+        variants.append((heads[pick], rng.integers(0, 2, pick.shape[0]).astype(np.uint8)))
+    log(f"[bench] c5 graph V={g.n_vars} F={g.n_factors} E={g.n_edges}, {n_var} variants, "
+        f"setup {time.perf_counter() - t0:.1f}s, device bytes plan {plan.device_bytes / 1e9:.2f} GB "
+        f"+ adjoint {adj.device_bytes / 1e9:.2f} GB")
+
+    def forward_all(theta):
+        """alarm marginals of every variant after T sweeps at theta"""
+        p = tied_p(prob.base_p, prob.rule, theta)
+        plan.set_p(p)
+        out = []
+        for f, x in variants:
+            plan.set_head_clamps(f, x)
+            out.append(adj.forward()[aidx])
+            plan.reset_head_clamps()
+        return p, out
+
+    theta_ref = rng.normal(2.5, 0.3, n_rules)
+    _, targets = forward_all(theta_ref)
+    theta = theta_ref + rng.normal(0.0, 0.5, n_rules)
+    m, v = np.zeros(n_rules), np.zeros(n_rules)
+    lr, b1, b2, eps = 0.05, 0.9, 0.999, 1e-8
+    losses, fwd_ms, bwd_ms = [], 0.0, 0.0
+    t_fit = time.perf_counter()
+    for step in range(args.steps):
+        p = tied_p(prob.base_p, prob.rule, theta)
+        plan.set_p(p)
+        dl_dp = np.zeros(g.n_factors)
+        loss = 0.0
+        for (f, x), y in zip(variants, targets):
+            plan.set_head_clamps(f, x)
+            b = adj.forward()[aidx]
+            dl_db = np.zeros(g.n_vars)
+            dl_db[aidx] = 2.0 * (b - y)
+            dl_dp += adj.backward(dl_db)
+            a, c = adj.last_timing()
+            fwd_ms += a
+            bwd_ms += c
+            plan.reset_head_clamps()
+            loss += float(np.sum((b - y) ** 2))
+        grad = rule_grad(dl_dp, p, prob.rule, n_rules)
+        m = b1 * m + (1 - b1) * grad
+        v = b2 * v + (1 - b2) * grad * grad
+        mh, vh = m / (1 - b1 ** (step + 1)), v / (1 - b2 ** (step + 1))
+        theta = theta - lr * mh / (np.sqrt(vh) + eps)
+        losses.append(loss)
+        log(f"[bench] c5 step {step}: loss {loss:.6g}")
+    t_fit = time.perf_counter() - t_fit
+    # plain LBP at the fitted theta against the T-sweep marginals
+    p_fit, bT = forward_all(theta)
+    errs, iters = [], []
+    for (f, x), b in zip(variants, bT):
+        plan.set_head_clamps(f, x)
+        plan.reset()
+        plan.init()
+        r = plan.run(1e-9, 500, damp)
+        iters.append(r.iters)
+        errs.append(float(np.max(np.abs(plan.beliefs()[aidx] - b))))
+        plan.reset_head_clamps()
+    sweeps = args.steps * n_var * T
+    out = {"metric": METRIC, "value": 2.0 * g.n_edges * sweeps / t_fit, "unit": "msg-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": 0, "ms_per_step": t_fit * 1e3 / max(args.steps, 1),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"c5-tied-fit-{g.n_vars}-x{n_var}", "n_vars": g.n_vars, "n_factors": g.n_factors,
+                      "n_edges": g.n_edges, "n_variants": n_var, "n_rules": n_rules, "n_alarms": int(len(aidx)),
+                      "T": T, "checkpoint_every": k, "damping": damp, "optimizer": "adam lr 0.05"},
+           "c5": {"steps_per_s": args.steps / t_fit, "loss_first": losses[0] if losses else None,
+                  "loss_last": losses[-1] if losses else None,
+                  "forward_ms_per_variant": fwd_ms / max(1, args.steps * n_var),
+                  "backward_ms_per_variant": bwd_ms / max(1, args.steps * n_var),
+                  "plain_vs_T_sweep_max_abs_err": max(errs) if errs else None,
+                  "plain_vs_T_sweep_mean_max_abs_err": float(np.mean(errs)) if errs else None,
+                  "plain_iters_median": float(np.median(iters)) if iters else None,
+                  "device_bytes": plan.device_bytes + adj.device_bytes}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,7 +299,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["plain", "batched", "adjoint"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "batched", "adjoint", "c5"], default="plain")
+    ap.add_argument("--c5-variants", type=int, default=64, help="c5: clamped variants per gradient step")
+    ap.add_argument("--c5-sweeps", type=int, default=20, help="c5: sweeps T of the differentiated solve")
+    ap.add_argument("--c5-checkpoint", type=int, default=5, help="c5: checkpoint interval of the adjoint")
     ap.add_argument("--variant", type=int, default=-1,
                     help="sweep kernels: -1 the plan's own choice by footprint (default), 0 LDS-staged (pull), "
                          "1 one lane per item, 2 LDS-staged (push), 3 LDS-staged (scatter)")
@@ -208,6 +319,8 @@ def main():
         return bench_batched(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "c5":
+        return bench_c5(args)
 
     import torch
     import fastlbp_amd as fl

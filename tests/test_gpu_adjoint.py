@@ -106,3 +106,24 @@ def test_torch_autograd_function():
     loss.backward()
     _, g = loss_and_grad(prob, theta, y)
     assert np.array_equal(th.grad.numpy(), g)
+
+
+def test_gradient_with_in_place_head_clamps():
+    """The adjoint of a plan with head clamps applied in place equals the
+    adjoint of the graph rebuilt by clamp() (alarms without body uses)."""
+    gen = fl.generate(20000, 9, n_rules=50, n_alarms=300)
+    g = fl.Graph(gen.factors)
+    aidx = g.index_of(gen.alarms)
+    rng = np.random.default_rng(9)
+    free = gen.alarms[g.body_uses(aidx) == 0]
+    labels = rng.choice(free, size=25, replace=False)
+    values = rng.integers(0, 2, 25).astype(np.uint8)
+    theta = rng.normal(2.5, 0.3, 50)
+    y = rng.integers(0, 2, len(gen.alarms)).astype(np.float64)
+    inplace = TiedProblem(g, gen.rule, 50, gen.alarms, 20, 5, 0.5)
+    inplace.plan.set_head_clamps(g.head_factors(g.index_of(labels)), values)
+    rebuilt = TiedProblem(fl.Graph(fl.clamp(gen.factors, labels, values)), gen.rule, 50, gen.alarms, 20, 5, 0.5)
+    la, ga = loss_and_grad(inplace, theta, y)
+    lb, gb = loss_and_grad(rebuilt, theta, y)
+    assert la == lb
+    assert same_bits(ga, gb)
