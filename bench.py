@@ -194,12 +194,13 @@ def bench_c5(args):
     Targets are the variants' alarm marginals at a reference theta; the fit
     starts from a perturbed theta.  A gradient step runs, for every variant,
     the adjoint forward (T damped sweeps from init) and backward, sums
-    dL/dp over the variants and reduces it to dL/dtheta once, then takes an
-    Adam step.  Afterwards plain LBP runs to convergence at the fitted theta
+    dL/dp over the variants on the device (sparse dL/db in, no V- or
+    F-sized transfers) and reduces it to dL/dtheta once, then takes an Adam
+    step.  Afterwards plain LBP runs to convergence at the fitted theta
     on every variant, against the T-sweep marginals."""
     import numpy as np
     import fastlbp_amd as fl
-    from fastlbp_amd.adjoint import TiedProblem, rule_grad, tied_p
+    from fastlbp_amd.adjoint import TiedProblem, tied_p
 
     n_vars = args.n_vars or 20_000_000
     n_var = args.c5_variants
@@ -221,6 +222,8 @@ def bench_c5(args):
         f"setup {time.perf_counter() - t0:.1f}s, device bytes plan {plan.device_bytes / 1e9:.2f} GB "
         f"+ adjoint {adj.device_bytes / 1e9:.2f} GB")
 
+    adj.set_rules(prob.rule, n_rules)
+
     def forward_all(theta):
         """alarm marginals of every variant after T sweeps at theta"""
         p = tied_p(prob.base_p, prob.rule, theta)
@@ -228,7 +231,8 @@ def bench_c5(args):
         out = []
         for f, x in variants:
             plan.set_head_clamps(f, x)
-            out.append(adj.forward()[aidx])
+            adj.forward(download=False)
+            out.append(plan.beliefs_at(aidx))
             plan.reset_head_clamps()
         return p, out
 
@@ -240,22 +244,20 @@ def bench_c5(args):
     losses, fwd_ms, bwd_ms = [], 0.0, 0.0
     t_fit = time.perf_counter()
     for step in range(args.steps):
-        p = tied_p(prob.base_p, prob.rule, theta)
-        plan.set_p(p)
-        dl_dp = np.zeros(g.n_factors)
+        plan.set_p(tied_p(prob.base_p, prob.rule, theta))
+        adj.clear_grad()
         loss = 0.0
         for (f, x), y in zip(variants, targets):
             plan.set_head_clamps(f, x)
-            b = adj.forward()[aidx]
-            dl_db = np.zeros(g.n_vars)
-            dl_db[aidx] = 2.0 * (b - y)
-            dl_dp += adj.backward(dl_db)
+            adj.forward(download=False)
+            b = plan.beliefs_at(aidx)
+            adj.backward_sparse(aidx, 2.0 * (b - y))   # dL/dp summed over the variants on the device
             a, c = adj.last_timing()
             fwd_ms += a
             bwd_ms += c
             plan.reset_head_clamps()
             loss += float(np.sum((b - y) ** 2))
-        grad = rule_grad(dl_dp, p, prob.rule, n_rules)
+        grad = adj.rule_grad()
         m = b1 * m + (1 - b1) * grad
         v = b2 * v + (1 - b2) * grad * grad
         mh, vh = m / (1 - b1 ** (step + 1)), v / (1 - b2 ** (step + 1))
@@ -272,7 +274,7 @@ def bench_c5(args):
         plan.init()
         r = plan.run(1e-9, 500, damp)
         iters.append(r.iters)
-        errs.append(float(np.max(np.abs(plan.beliefs()[aidx] - b))))
+        errs.append(float(np.max(np.abs(plan.beliefs_at(aidx) - b))))
         plan.reset_head_clamps()
     sweeps = args.steps * n_var * T
     out = {"metric": METRIC, "value": 2.0 * g.n_edges * sweeps / t_fit, "unit": "msg-updates/s", "n_gpus": 1,

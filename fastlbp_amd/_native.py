@@ -137,6 +137,7 @@ SIGNATURES = {
     "flbp_plan_device_bytes": (C.c_uint64, [C.c_void_p]),
     "flbp_plan_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_plan_get_variant": (C.c_int, [C.c_void_p]),
+    "flbp_plan_beliefs_at": (C.c_int, [C.c_void_p, C.c_uint64, u64p, f64p]),
     "flbp_plan_set_head_clamps": (C.c_int, [C.c_void_p, C.c_uint64, u64p, u8p]),
     "flbp_plan_reset_head_clamps": (C.c_int, [C.c_void_p]),
     "flbp_batch_default_opts": (None, [C.POINTER(BatchOptsC)]),
@@ -156,6 +157,11 @@ SIGNATURES = {
     "flbp_adjoint_forward": (C.c_int, [C.c_void_p, f64p]),
     "flbp_adjoint_backward": (C.c_int, [C.c_void_p, f64p, f64p]),
     "flbp_adjoint_last_timing": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "flbp_adjoint_backward_sparse": (C.c_int, [C.c_void_p, C.c_uint64, u64p, f64p]),
+    "flbp_adjoint_clear_grad": (C.c_int, [C.c_void_p]),
+    "flbp_adjoint_get_dl_dp": (C.c_int, [C.c_void_p, f64p]),
+    "flbp_adjoint_set_rules": (C.c_int, [C.c_void_p, i32p, C.c_uint64]),
+    "flbp_adjoint_rule_grad": (C.c_int, [C.c_void_p, f64p]),
     "flbp_adjoint_device_bytes": (C.c_uint64, [C.c_void_p]),
     "flbp_tied_p": (C.c_int, [C.c_uint64, f64p, i32p, C.c_uint64, f64p, f64p]),
     "flbp_rule_grad": (C.c_int, [C.c_uint64, f64p, f64p, i32p, C.c_uint64, f64p]),

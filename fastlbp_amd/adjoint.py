@@ -56,7 +56,12 @@ class Adjoint:
         except Exception:
             pass
 
-    def forward(self) -> np.ndarray:
+    def forward(self, download: bool = True):
+        """T sweeps from init; all beliefs, or None (read selected ones with
+        plan.beliefs_at) when download is False."""
+        if not download:
+            call("flbp_adjoint_forward", self._h, None)
+            return None
         out = np.empty(self.plan.graph.n_vars, np.float64)
         call("flbp_adjoint_forward", self._h, _ptr(out, C.c_double))
         return out
@@ -65,6 +70,34 @@ class Adjoint:
         dl_db = np.ascontiguousarray(dl_db, np.float64)
         out = np.empty(self.plan.graph.n_factors, np.float64)
         call("flbp_adjoint_backward", self._h, _ptr(dl_db, C.c_double), _ptr(out, C.c_double))
+        return out
+
+    def backward_sparse(self, var_indices, dl_db) -> None:
+        """Adds dL/dp for dL/db = dl_db at var_indices (0 elsewhere) to the
+        device accumulator."""
+        v = np.ascontiguousarray(var_indices, np.uint64)
+        g = np.ascontiguousarray(dl_db, np.float64)
+        if v.shape != g.shape:
+            raise ValueError("var_indices and dl_db differ in length")
+        call("flbp_adjoint_backward_sparse", self._h, v.shape[0], _ptr(v, C.c_uint64), _ptr(g, C.c_double))
+
+    def clear_grad(self) -> None:
+        call("flbp_adjoint_clear_grad", self._h)
+
+    def dl_dp(self) -> np.ndarray:
+        out = np.empty(self.plan.graph.n_factors, np.float64)
+        call("flbp_adjoint_get_dl_dp", self._h, _ptr(out, C.c_double))
+        return out
+
+    def set_rules(self, rule: np.ndarray, n_rules: int) -> None:
+        r = np.ascontiguousarray(rule, np.int32)
+        self._n_rules = n_rules
+        call("flbp_adjoint_set_rules", self._h, _ptr(r, C.c_int32), n_rules)
+
+    def rule_grad(self) -> np.ndarray:
+        """dL/dtheta from the accumulated dL/dp, reduced on the device."""
+        out = np.empty(self._n_rules, np.float64)
+        call("flbp_adjoint_rule_grad", self._h, _ptr(out, C.c_double))
         return out
 
     def last_timing(self):

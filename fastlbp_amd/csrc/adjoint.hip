@@ -39,6 +39,37 @@ __global__ void fill1(double *p, uint64_t n, double c) {
     if (i < n) p[i] = c;
 }
 
+/* dst[idx[i]] = val[i] */
+__global__ void scatter_f64(double *__restrict__ dst, const uint32_t *__restrict__ idx, const double *__restrict__ val,
+                            uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[idx[i]] = val[i];
+}
+
+/* one block per rule r: out[r] = sum over the rule's factors f (ascending,
+ * rule_fac[rule_off[r] .. rule_off[r+1])) of pbar[f] * p[f] * (1 - p[f]),
+ * summed per lane in a fixed stride and then in a fixed tree: the same
+ * bits on every run */
+__global__ __launch_bounds__(256) void rule_reduce(const uint64_t *__restrict__ rule_off,
+                                                  const uint32_t *__restrict__ rule_fac,
+                                                  const double *__restrict__ pbar, const double *__restrict__ p,
+                                                  double *__restrict__ out) {
+    __shared__ double part[256];
+    const uint32_t r = blockIdx.x;
+    double acc = 0.0;
+    for (uint64_t j = rule_off[r] + threadIdx.x; j < rule_off[r + 1]; j += 256) {
+        const uint32_t f = rule_fac[j];
+        acc += pbar[f] * p[f] * (1.0 - p[f]);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[r] = part[0];
+}
+
 }  // namespace
 
 namespace flbp {
@@ -55,6 +86,11 @@ struct Adjoint {
     uint32_t *fac_blk = nullptr, *var_blk = nullptr;    /* block ranges of the reverse kernels */
     uint16_t *fac_work = nullptr, *var_work = nullptr;  /* per block: factors by type and degree, variables by degree */
     uint64_t n_fac_blocks = 0, n_var_blocks = 0;
+    uint64_t *rule_off = nullptr;   /* per rule: its factors' range in rule_fac (flbp_adjoint_set_rules) */
+    uint32_t *rule_fac = nullptr;
+    double *rule_out = nullptr;
+    uint64_t n_rules = 0;
+    int run_backward();
     double *pbar = nullptr, *bbar = nullptr, *scratch_belief = nullptr;
     std::vector<void *> allocs;
     uint64_t bytes = 0;
@@ -241,56 +277,180 @@ int flbp_adjoint_forward(flbp_adjoint *adj, double *beliefs) {
     return FLBP_OK;
 }
 
-int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp) {
-    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
-    if (!A || !dl_db || !dl_dp) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: null argument");
-    if (!A->have_forward) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: run the forward first");
-    if (A->P->variant >= 2)
-        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: the plan was switched away from the pull layout");
-    Plan *P = A->P;
-    HIP_TRY(hipSetDevice(P->device));
+}  // extern "C"
+
+/* the reverse walk with bbar already on the device; dL/dp is added to pbar */
+int flbp::Adjoint::run_backward() {
     const uint64_t E = P->E, V = P->V, F = P->F;
     hipStream_t st = P->stream;
-    std::vector<double> dev_db(V);
-    for (uint64_t dv = 0; dv < V; dv++) dev_db[dv] = dl_db[P->h_perm[dv]];
-    if (V) HIP_TRY(hipMemcpyAsync(A->bbar, dev_db.data(), V * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipEventRecord(A->ev[2], st));
+    HIP_TRY(hipEventRecord(ev[2], st));
     if (E) {
-        hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, A->avf, E, 0.0);
-        hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, A->afv, E, 0.0);
+        hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, avf, E, 0.0);
+        hipLaunchKernelGGL(fill2, dim3(grid_for(E, 256)), dim3(256), 0, st, afv, E, 0.0);
     }
-    if (F) hipLaunchKernelGGL(fill1, dim3(grid_for(F, 256)), dim3(256), 0, st, A->pbar, F, 0.0);
-    const double keep = 1.0 - A->damp;
-    for (int64_t c = (int64_t)A->nck - 1; c >= 0; c--) {
-        const uint64_t t0 = (uint64_t)c * A->k, t1 = std::min<uint64_t>(t0 + A->k, A->T);
+    const double keep = 1.0 - damp;
+    for (int64_t c = (int64_t)nck - 1; c >= 0; c--) {
+        const uint64_t t0 = (uint64_t)c * k, t1 = std::min<uint64_t>(t0 + k, T);
         auto vf_state = [&](uint64_t t) -> double2 * {   /* m_vf after t sweeps, t0 <= t <= t1 */
-            return t == t0 ? A->ck_vf[c] : A->st_vf[t - t0 - 1];
+            return t == t0 ? ck_vf[c] : st_vf[t - t0 - 1];
         };
         auto fv_state = [&](uint64_t t) -> double2 * {
-            return t == t0 ? A->ck_fv[c] : A->st_fv[t - t0 - 1];
+            return t == t0 ? ck_fv[c] : st_fv[t - t0 - 1];
         };
         for (uint64_t t = t0 + 1; t <= t1; t++)
-            TRY(P->launch_sweep_on(0, -INFINITY, A->damp, vf_state(t - 1), fv_state(t - 1), fv_state(t),
-                                   vf_state(t), A->scratch_belief));
+            TRY(P->launch_sweep_on(0, -INFINITY, damp, vf_state(t - 1), fv_state(t - 1), fv_state(t), vf_state(t),
+                                   scratch_belief));
         for (uint64_t t = t1; t > t0; t--) {
             if (V)
-                hipLaunchKernelGGL(var_vjp_staged, dim3((unsigned)A->n_var_blocks), dim3(kBlock), 0, st, A->var_blk,
-                                   P->var_off, P->var_edge, A->var_work, fv_state(t), A->avf,
-                                   t == A->T ? A->bbar : nullptr, A->afv, A->gscratch, A->damp);
+                hipLaunchKernelGGL(var_vjp_staged, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, st, var_blk,
+                                   P->var_off, P->var_edge, var_work, fv_state(t), avf, t == T ? bbar : nullptr, afv,
+                                   gscratch, damp);
             if (F)
-                hipLaunchKernelGGL(fac_vjp_staged, dim3((unsigned)A->n_fac_blocks), dim3(kBlock), 0, st, A->fac_blk,
-                                   P->fac_off, P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, P->fac_vpos,
-                                   A->fac_work, vf_state(t - 1), A->afv, A->avf2, A->pbar, keep);
-            std::swap(A->avf, A->avf2);
+                hipLaunchKernelGGL(fac_vjp_staged, dim3((unsigned)n_fac_blocks), dim3(kBlock), 0, st, fac_blk,
+                                   P->fac_off, P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, P->fac_vpos, fac_work,
+                                   vf_state(t - 1), afv, avf2, pbar, keep);
+            std::swap(avf, avf2);
         }
         HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipEventRecord(A->ev[3], st));
-    if (F) HIP_TRY(hipMemcpyAsync(dl_dp, A->pbar, F * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipEventRecord(ev[3], st));
+    return FLBP_OK;
+}
+
+extern "C" {
+
+static int adjoint_check(Adjoint *A, const char *what) {
+    if (!A) return flbp::fail(FLBP_EINVAL, std::string(what) + ": null adjoint");
+    if (!A->have_forward) return flbp::fail(FLBP_EINVAL, std::string(what) + ": run the forward first");
+    if (A->P->variant >= 2)
+        return flbp::fail(FLBP_EINVAL, std::string(what) + ": the plan was switched away from the pull layout");
+    return FLBP_OK;
+}
+
+static int adjoint_finish(Adjoint *A) {
+    HIP_TRY(hipStreamSynchronize(A->P->stream));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, A->ev[2], A->ev[3]));
     A->bwd_ms = ms;
+    return FLBP_OK;
+}
+
+int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    if (!dl_db || !dl_dp) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward: null argument");
+    TRY(adjoint_check(A, "flbp_adjoint_backward"));
+    Plan *P = A->P;
+    HIP_TRY(hipSetDevice(P->device));
+    const uint64_t V = P->V, F = P->F;
+    std::vector<double> dev_db(V);
+    for (uint64_t dv = 0; dv < V; dv++) dev_db[dv] = dl_db[P->h_perm[dv]];
+    if (V) HIP_TRY(hipMemcpyAsync(A->bbar, dev_db.data(), V * sizeof(double), hipMemcpyHostToDevice, P->stream));
+    if (F) hipLaunchKernelGGL(fill1, dim3(grid_for(F, 256)), dim3(256), 0, P->stream, A->pbar, F, 0.0);
+    TRY(A->run_backward());
+    if (F) HIP_TRY(hipMemcpyAsync(dl_dp, A->pbar, F * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    return adjoint_finish(A);
+}
+
+int flbp_adjoint_backward_sparse(flbp_adjoint *adj, uint64_t n, const uint64_t *vars, const double *dl_db) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    if (n && (!vars || !dl_db)) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward_sparse: null argument");
+    TRY(adjoint_check(A, "flbp_adjoint_backward_sparse"));
+    Plan *P = A->P;
+    HIP_TRY(hipSetDevice(P->device));
+    /* device positions, repeated variables summed in the given order */
+    std::vector<std::pair<uint32_t, uint64_t>> key(n);
+    for (uint64_t i = 0; i < n; i++) {
+        if (vars[i] >= P->V) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_backward_sparse: variable out of range");
+        key[i] = {P->h_inv_perm[vars[i]], i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<uint32_t> idx;
+    std::vector<double> val;
+    for (uint64_t i = 0; i < n; i++) {
+        if (!idx.empty() && idx.back() == key[i].first) val.back() += dl_db[key[i].second];
+        else {
+            idx.push_back(key[i].first);
+            val.push_back(dl_db[key[i].second]);
+        }
+    }
+    if (P->V) hipLaunchKernelGGL(fill1, dim3(grid_for(P->V, 256)), dim3(256), 0, P->stream, A->bbar, P->V, 0.0);
+    if (!idx.empty()) {
+        TRY(P->stage(idx.size()));
+        HIP_TRY(hipMemcpyAsync(P->stage_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, P->stream));
+        HIP_TRY(hipMemcpyAsync(P->stage_val, val.data(), val.size() * 8, hipMemcpyHostToDevice, P->stream));
+        hipLaunchKernelGGL(scatter_f64, dim3(grid_for(idx.size(), 256)), dim3(256), 0, P->stream, A->bbar,
+                           P->stage_idx, P->stage_val, (uint64_t)idx.size());
+    }
+    TRY(A->run_backward());
+    return adjoint_finish(A);   /* also keeps the host vectors alive until the copies are done */
+}
+
+int flbp_adjoint_clear_grad(flbp_adjoint *adj) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    if (!A) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_clear_grad: null adjoint");
+    HIP_TRY(hipSetDevice(A->device));
+    if (A->P->F)
+        hipLaunchKernelGGL(fill1, dim3(grid_for(A->P->F, 256)), dim3(256), 0, A->P->stream, A->pbar, A->P->F, 0.0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(A->P->stream));
+    return FLBP_OK;
+}
+
+int flbp_adjoint_get_dl_dp(flbp_adjoint *adj, double *dl_dp) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    if (!A || (A->P->F && !dl_dp)) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_get_dl_dp: null argument");
+    HIP_TRY(hipSetDevice(A->device));
+    if (A->P->F)
+        HIP_TRY(hipMemcpyAsync(dl_dp, A->pbar, A->P->F * sizeof(double), hipMemcpyDeviceToHost, A->P->stream));
+    HIP_TRY(hipStreamSynchronize(A->P->stream));
+    return FLBP_OK;
+}
+
+int flbp_adjoint_set_rules(flbp_adjoint *adj, const int32_t *rule, uint64_t n_rules) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    const uint64_t F = A ? A->P->F : 0;
+    if (!A || (F && !rule) || n_rules == 0 || n_rules >= (1ull << 31))
+        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_set_rules: bad argument");
+    std::vector<uint64_t> off(n_rules + 1, 0);
+    for (uint64_t I = 0; I < F; I++) {
+        if (rule[I] < 0) continue;
+        if ((uint64_t)rule[I] >= n_rules) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_set_rules: rule id out of range");
+        off[rule[I] + 1]++;
+    }
+    for (uint64_t r = 0; r < n_rules; r++) off[r + 1] += off[r];
+    std::vector<uint32_t> fac(off[n_rules]);
+    std::vector<uint64_t> pos(off.begin(), off.end() - 1);
+    for (uint64_t I = 0; I < F; I++)
+        if (rule[I] >= 0) fac[pos[rule[I]]++] = (uint32_t)I;
+    HIP_TRY(hipSetDevice(A->device));
+    for (void *q : {(void *)A->rule_off, (void *)A->rule_fac, (void *)A->rule_out}) {
+        if (!q) continue;
+        auto it = std::find(A->allocs.begin(), A->allocs.end(), q);
+        if (it != A->allocs.end()) A->allocs.erase(it);
+        (void)hipFree(q);
+    }
+    A->rule_off = nullptr;
+    A->rule_fac = nullptr;
+    A->rule_out = nullptr;
+    TRY(A->alloc(A->rule_off, n_rules + 1));
+    TRY(A->alloc(A->rule_fac, fac.size()));
+    TRY(A->alloc(A->rule_out, n_rules));
+    HIP_TRY(hipMemcpy(A->rule_off, off.data(), off.size() * 8, hipMemcpyHostToDevice));
+    if (!fac.empty()) HIP_TRY(hipMemcpy(A->rule_fac, fac.data(), fac.size() * 4, hipMemcpyHostToDevice));
+    A->n_rules = n_rules;
+    return FLBP_OK;
+}
+
+int flbp_adjoint_rule_grad(flbp_adjoint *adj, double *dl_dtheta) {
+    Adjoint *A = reinterpret_cast<Adjoint *>(adj);
+    if (!A || !dl_dtheta) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_rule_grad: null argument");
+    if (!A->n_rules) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_rule_grad: call flbp_adjoint_set_rules first");
+    HIP_TRY(hipSetDevice(A->device));
+    hipLaunchKernelGGL(rule_reduce, dim3((unsigned)A->n_rules), dim3(256), 0, A->P->stream, A->rule_off, A->rule_fac,
+                       A->pbar, A->P->fac_p, A->rule_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(dl_dtheta, A->rule_out, A->n_rules * 8, hipMemcpyDeviceToHost, A->P->stream));
+    HIP_TRY(hipStreamSynchronize(A->P->stream));
     return FLBP_OK;
 }
 

@@ -69,6 +69,13 @@ __global__ void gather_pairs(const double2 *__restrict__ src, const uint32_t *__
     if (i < n) dst[i] = src[pos[i]];
 }
 
+/* out[i] = src[idx[i]] */
+__global__ void gather_f64(const double *__restrict__ src, const uint32_t *__restrict__ idx, double *__restrict__ out,
+                           uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
 /* beliefs straight after init with no sweep: prod = (c, c), zero counts 0 */
 __global__ void fill_f64(double *p, uint64_t n, double c) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -223,6 +230,8 @@ int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_di
 Plan::~Plan() {
     if (device >= 0) (void)hipSetDevice(device);
     for (void *p : allocs) (void)hipFree(p);
+    if (stage_idx) (void)hipFree(stage_idx);
+    if (stage_val) (void)hipFree(stage_val);
     if (host_ctl) (void)hipHostFree(host_ctl);
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
@@ -373,6 +382,26 @@ int flbp_plan_beliefs(flbp_plan *plan, double *out, uint64_t n) {
         for (uint64_t dv = 0; dv < P->V; dv++)
             if (P->h_perm[dv] < n) out[P->h_perm[dv]] = dev[dv];
     }
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return FLBP_OK;
+}
+
+int flbp_plan_beliefs_at(flbp_plan *plan, uint64_t n, const uint64_t *vars, double *out) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || (n && (!vars || !out))) return flbp::fail(FLBP_EINVAL, "flbp_plan_beliefs_at: null argument");
+    if (!n) return FLBP_OK;
+    std::vector<uint32_t> idx(n);
+    for (uint64_t i = 0; i < n; i++) {
+        if (vars[i] >= P->V) return flbp::fail(FLBP_EINVAL, "flbp_plan_beliefs_at: variable out of range");
+        idx[i] = P->h_inv_perm[vars[i]];
+    }
+    HIP_TRY(hipSetDevice(P->device));
+    if (int rc = P->stage(n)) return rc;
+    HIP_TRY(hipMemcpyAsync(P->stage_idx, idx.data(), n * 4, hipMemcpyHostToDevice, P->stream));
+    hipLaunchKernelGGL(gather_f64, dim3(grid_for(n, 256)), dim3(256), 0, P->stream, P->belief, P->stage_idx,
+                       P->stage_val, n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, P->stage_val, n * 8, hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
     return FLBP_OK;
 }

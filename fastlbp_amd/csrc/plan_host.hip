@@ -131,6 +131,23 @@ void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const 
     }
 }
 
+int Plan::stage(uint64_t n) {
+    if (n <= stage_cap) return FLBP_OK;
+    if (stage_idx) (void)hipFree(stage_idx);
+    if (stage_val) (void)hipFree(stage_val);
+    stage_idx = nullptr;
+    stage_val = nullptr;
+    stage_cap = 0;
+    const uint64_t cap = std::max<uint64_t>(n, 4096);
+    if (hipMalloc(&stage_idx, cap * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&stage_val, cap * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FLBP_ENOMEM, "plan: staging buffer of " + std::to_string(cap) + " entries failed");
+    }
+    stage_cap = cap;
+    return FLBP_OK;
+}
+
 template <class T>
 int Plan::alloc(T *&p, uint64_t n) {
     const size_t bytes = std::max<uint64_t>(n, 1) * sizeof(T);
@@ -234,6 +251,8 @@ int Plan::upload(const flbp_csr &c) {
     for (uint64_t i = 0; i < V; i++) max_var_deg = std::max<uint64_t>(max_var_deg, c.var_off[i + 1] - c.var_off[i]);
     h_meta0 = meta;
     h_meta.swap(meta);
+    h_inv_perm.assign(V, 0);
+    for (uint64_t dv = 0; dv < V; dv++) h_inv_perm[h_perm[dv]] = (uint32_t)dv;
     variant = device_bytes <= kScatterMaxBytes ? 3 : 0;
     return FLBP_OK;
 }

@@ -69,6 +69,11 @@ struct Plan {
 
     std::vector<uint64_t> h_var_edge;              /* for message export/import */
     std::vector<uint32_t> h_perm;                  /* device variable -> variable */
+    std::vector<uint32_t> h_inv_perm;              /* variable -> device variable */
+    uint32_t *stage_idx = nullptr;                 /* small index / value uploads (grown on demand) */
+    double *stage_val = nullptr;
+    uint64_t stage_cap = 0;
+    int stage(uint64_t n);
     std::vector<uint32_t> h_fac_vpos;              /* factor-major slot -> var-major position */
     std::vector<uint32_t> h_meta, h_meta0;         /* factor meta words: current and at creation */
     std::vector<double2> h_mask, h_mask0;          /* head masks: current and at creation */

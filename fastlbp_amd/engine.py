@@ -288,6 +288,13 @@ class Plan:
         p = np.ascontiguousarray(p, dtype=np.float64)
         call("flbp_plan_set_p", self._h, _ptr(p, C.c_double), p.shape[0])
 
+    def beliefs_at(self, var_indices) -> np.ndarray:
+        """Beliefs of the given variables, gathered on the device."""
+        v = np.ascontiguousarray(var_indices, dtype=np.uint64)
+        out = np.empty(v.shape[0], np.float64)
+        call("flbp_plan_beliefs_at", self._h, v.shape[0], _ptr(v, C.c_uint64), _ptr(out, C.c_double))
+        return out
+
     def set_head_clamps(self, factors, values) -> None:
         """Head clamps in place (gen_clamped's head rule) on AND/OR factors;
         equals rebuilding the graph clamped at variables with no body uses."""

@@ -189,6 +189,10 @@ int flbp_plan_reset(flbp_plan *plan);
 
 /* transferBeliefToHost (causal_bp.cu:552-554): P(x_i = 1) for i < n_vars. */
 int flbp_plan_beliefs(flbp_plan *plan, double *out, uint64_t n);
+/* out[i] = belief of variable vars[i] (variable order): n values gathered on
+ * the device, so a caller that reads a few variables of a large graph does
+ * not copy all V beliefs.  Not in the reference. */
+int flbp_plan_beliefs_at(flbp_plan *plan, uint64_t n, const uint64_t *vars, double *out);
 
 /* transferMessagesToHost (causal_bp.cu:1145-1161) and its inverse: f->v
  * messages in var-major (nbV) order, v->f messages in factor-major order,
@@ -305,12 +309,30 @@ typedef struct flbp_adjoint flbp_adjoint;
 int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_every,
                         double damping, flbp_adjoint **out);
 void flbp_adjoint_destroy(flbp_adjoint *adj);
-/* init + n_sweeps sweeps with the plan's current p; beliefs[V] (NULL skips) */
+/* init + n_sweeps sweeps with the plan's current p; beliefs[V] (NULL skips;
+ * flbp_plan_beliefs_at then reads selected beliefs) */
 int flbp_adjoint_forward(flbp_adjoint *adj, double *beliefs);
 /* dl_db[V] = dL/d(belief) of every variable -> dl_dp[F] = dL/dp of every
  * factor, for the last forward */
 int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp);
 /* device time of the last forward and backward, milliseconds */
+/* The backward with a sparse loss gradient: dL/db is dl_db[i] at variable
+ * vars[i] (variable order; repeated variables add) and 0 elsewhere.  dL/dp
+ * stays on the device and is ADDED to the adjoint's accumulator, so a
+ * training loop can sum over variants without moving V or F values; the
+ * accumulator is cleared by flbp_adjoint_clear_grad and overwritten by
+ * flbp_adjoint_backward. */
+int flbp_adjoint_backward_sparse(flbp_adjoint *adj, uint64_t n, const uint64_t *vars, const double *dl_db);
+int flbp_adjoint_clear_grad(flbp_adjoint *adj);
+/* the accumulated dL/dp[F] */
+int flbp_adjoint_get_dl_dp(flbp_adjoint *adj, double *dl_dp);
+/* rule[F] (-1 for untied factors, else < n_rules): the ties of
+ * flbp_tied_p, kept on the device for flbp_adjoint_rule_grad */
+int flbp_adjoint_set_rules(flbp_adjoint *adj, const int32_t *rule, uint64_t n_rules);
+/* dL/dtheta[n_rules] from the accumulated dL/dp and the plan's current p, as
+ * flbp_rule_grad computes it, reduced on the device (one deterministic
+ * reduction per rule) */
+int flbp_adjoint_rule_grad(flbp_adjoint *adj, double *dl_dtheta);
 int flbp_adjoint_last_timing(flbp_adjoint *adj, double *forward_ms, double *backward_ms);
 uint64_t flbp_adjoint_device_bytes(const flbp_adjoint *adj);
 

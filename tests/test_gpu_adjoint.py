@@ -127,3 +127,44 @@ def test_gradient_with_in_place_head_clamps():
     lb, gb = loss_and_grad(rebuilt, theta, y)
     assert la == lb
     assert same_bits(ga, gb)
+
+
+def test_sparse_backward_and_device_rule_grad():
+    """backward_sparse gives the dense backward's dL/dp bit for bit (repeated
+    variables add), successive calls accumulate, and the device rule
+    reduction matches rule_grad (and repeats its own bits)."""
+    from fastlbp_amd.adjoint import rule_grad
+    gen = fl.generate(30000, 12, n_rules=40, n_alarms=200)
+    g = fl.Graph(gen.factors)
+    prob = TiedProblem(g, gen.rule, 40, gen.alarms, 15, 4, 0.5)
+    rng = np.random.default_rng(12)
+    theta = rng.normal(2.5, 0.3, 40)
+    p = tied_p(prob.base_p, prob.rule, theta)
+    prob.plan.set_p(p)
+    aidx = prob.alarm_idx
+    free = np.flatnonzero(g.body_uses(aidx) == 0)
+    heads = g.head_factors(aidx)
+    adj = prob.adj
+    adj.set_rules(prob.rule, 40)
+    for trial in range(2):
+        pick = rng.choice(free, size=10, replace=False)
+        prob.plan.set_head_clamps(heads[pick], rng.integers(0, 2, 10).astype(np.uint8))
+        full = adj.forward()
+        assert same_bits(prob.plan.beliefs_at(aidx), full[aidx])
+        gb = rng.normal(0, 1, len(aidx))
+        dl_db = np.zeros(g.n_vars)
+        np.add.at(dl_db, aidx, gb)
+        np.add.at(dl_db, aidx[:5], gb[:5])
+        dense = adj.backward(dl_db)
+        adj.clear_grad()
+        adj.backward_sparse(np.concatenate([aidx, aidx[:5]]), np.concatenate([gb, gb[:5]]))
+        assert same_bits(adj.dl_dp(), dense)
+        adj.clear_grad()
+        adj.backward_sparse(aidx, gb)
+        adj.backward_sparse(aidx[:5], gb[:5])
+        assert close(adj.dl_dp(), dense)
+        host = rule_grad(adj.dl_dp(), p, prob.rule, 40)
+        dev = adj.rule_grad()
+        assert close(dev, host)
+        assert same_bits(dev, adj.rule_grad())
+        prob.plan.reset_head_clamps()
