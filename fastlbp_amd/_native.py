@@ -137,6 +137,7 @@ SIGNATURES = {
     "flbp_plan_device_bytes": (C.c_uint64, [C.c_void_p]),
     "flbp_plan_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_plan_get_variant": (C.c_int, [C.c_void_p]),
+    "flbp_plan_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_plan_beliefs_at": (C.c_int, [C.c_void_p, C.c_uint64, u64p, f64p]),
     "flbp_plan_set_head_clamps": (C.c_int, [C.c_void_p, C.c_uint64, u64p, u8p]),
     "flbp_plan_reset_head_clamps": (C.c_int, [C.c_void_p]),

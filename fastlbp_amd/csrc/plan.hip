@@ -192,14 +192,68 @@ static int next_chunk(int chunk, double d_prev, double d_last, double tol) {
 
 /* Enqueue n sweeps (n <= kMaxChunk) and return how many ran before the stop
  * test fired, with the max diff of the last one that ran. */
-int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_diff) {
-    hipLaunchKernelGGL(clear_u64, dim3(16), dim3(256), 0, stream, ctl, kMaxChunk * kCtlPerSweep);
-    if (profiling) HIP_TRY(hipEventRecord(ev[0], stream));
-    for (int k = 0; k < n; k++) {
-        int rc = launch_sweep(k, tol, damping);
-        if (rc) return rc;
+void Plan::drop_graphs() {
+    for (auto &g : graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
     }
+    graphs.clear();
+}
+
+/* the captured chunk of n sweeps: clear, the sweeps, compact (a stale
+ * factor-major copy is rebuilt before the replay, outside the graph) */
+int Plan::chunk_graph(int n, double tol, double damping, hipGraphExec_t &exec) {
+    for (auto &g : graphs)
+        if (g.n == n && g.variant == variant && g.damping == damping && g.tol == tol) {
+            exec = g.exec;
+            return FLBP_OK;
+        }
+    if (graphs.size() >= 2 * (size_t)kMaxChunk) drop_graphs();
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(clear_u64, dim3(16), dim3(256), 0, stream, ctl, kMaxChunk * kCtlPerSweep);
+    int rc = FLBP_OK;
+    for (int k = 0; k < n && !rc; k++) rc = launch_sweep_on(k, tol, damping, m_vf, m_fv, m_fv, m_vf, belief);
     hipLaunchKernelGGL(ctl_compact, dim3(1), dim3(64), 0, stream, ctl, ctl_out, n);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(ei, "hipGraphInstantiate");
+    }
+    graphs.push_back({n, variant, damping, tol, g, x});
+    exec = x;
+    return FLBP_OK;
+}
+
+int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_diff) {
+    if (!profiling && use_graphs) {
+        if (variant == 3) {   /* the scatter layout's damping copy, as launch_sweep keeps it */
+            if (damping != 0.0 && fm_stale && E) {
+                hipLaunchKernelGGL(gather_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_fv_vm, fac_vpos,
+                                   m_fv, E);
+                fm_stale = false;
+            }
+            if (damping == 0.0) fm_stale = true;
+        }
+        hipGraphExec_t exec = nullptr;
+        if (int rc = chunk_graph(n, tol, damping, exec)) return rc;
+        HIP_TRY(hipGraphLaunch(exec, stream));
+    } else {
+        hipLaunchKernelGGL(clear_u64, dim3(16), dim3(256), 0, stream, ctl, kMaxChunk * kCtlPerSweep);
+        if (profiling) HIP_TRY(hipEventRecord(ev[0], stream));
+        for (int k = 0; k < n; k++) {
+            int rc = launch_sweep(k, tol, damping);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(ctl_compact, dim3(1), dim3(64), 0, stream, ctl, ctl_out, n);
+    }
     HIP_TRY(hipMemcpyAsync(host_ctl, ctl_out, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     ran = n;
@@ -229,6 +283,7 @@ int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_di
 
 Plan::~Plan() {
     if (device >= 0) (void)hipSetDevice(device);
+    drop_graphs();
     for (void *p : allocs) (void)hipFree(p);
     if (stage_idx) (void)hipFree(stage_idx);
     if (stage_val) (void)hipFree(stage_val);
@@ -527,6 +582,15 @@ int flbp_plan_reset_head_clamps(flbp_plan *plan) {
     }
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->clamp_dirty.clear();
+    return FLBP_OK;
+}
+
+int flbp_plan_use_graphs(flbp_plan *plan, int enable) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_use_graphs: null plan");
+    HIP_TRY(hipSetDevice(P->device));
+    P->use_graphs = enable != 0;
+    if (!P->use_graphs) P->drop_graphs();
     return FLBP_OK;
 }
 

@@ -85,6 +85,18 @@ struct Plan {
     bool swept_since_init = false;
 
     bool profiling = false;
+    /* captured chunks: one hipGraph per (sweeps, damping, tol, variant),
+     * replayed when profiling is off */
+    struct ChunkGraph {
+        int n, variant;
+        double damping, tol;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<ChunkGraph> graphs;
+    bool use_graphs = false;   /* measured: no gain over direct launches on MI355X */
+    void drop_graphs();
+    int chunk_graph(int n, double tol, double damping, hipGraphExec_t &exec);
     hipEvent_t ev[2 * kMaxChunk + 1] = {};
     double prof_fac_ms = 0, prof_var_ms = 0;
     uint64_t prof_fac_n = 0, prof_var_n = 0;

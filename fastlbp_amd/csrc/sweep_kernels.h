@@ -3,21 +3,26 @@
  *
  * One sweep of the reference (csrc/ext/causal_bp.cu:409-444 + 483-515 +
  * 879-886) is three per-edge/per-variable kernels, two belief kernels, a
- * thrust transform and a thrust reduce.  Here it is two kernels:
+ * thrust transform and a thrust reduce.  Here it is two kernels, a factor
+ * pass and a variable pass.  This file holds what every form of them
+ * shares -- the per-message arithmetic (fv_msg, vf_msg), the max-diff
+ * slots and the device-side stop test -- and the one-lane-per-item kernels
+ * of kernel variant 1:
  *
- *   factor pass  -- one lane per factor: gathers the factor's v->f messages
+ *   factor_pass  -- one lane per factor: gathers the factor's v->f messages
  *                   (stored var-major) into registers through the factor's
  *                   slot -> var-major position map and writes every f->v
- *                   message of the factor contiguously (stored factor-major),
+ *                   message of the factor contiguously (factor-major),
  *                   damping fused into the store.
- *   var pass     -- one lane per variable: gathers its f->v messages through
+ *   var_pass     -- one lane per variable: gathers its f->v messages through
  *                   the var-major -> factor-major slot map, forms the product
  *                   with zero counts, the belief and |b - b_old|, then writes
- *                   the leave-one-out v->f messages contiguously at its own
- *                   var-major positions; the max |b - b_old| is reduced in the
- *                   wave (DPP shuffles), across the block in LDS, and across
- *                   the grid with one 64-bit atomic max per block.
- * Both passes only gather and write contiguously: no scattered stores.
+ *                   the leave-one-out v->f messages at its own var-major
+ *                   positions; the max |b - b_old| is reduced in the wave
+ *                   (DPP shuffles), across the block in LDS and across the
+ *                   grid into line-padded partial slots.
+ *
+ * The LDS-staged kernels of the other variants are in staged_kernels.h.
  *
  * Every message is computed with the reference's own operation order (the
  * per-output loops of calcMessageFVFusedAllKernel, csrc/kernel/

@@ -325,6 +325,11 @@ class Plan:
         messages."""
         call("flbp_plan_set_variant", self._h, variant)
 
+    def use_graphs(self, enable: bool = True) -> None:
+        """Replay chunks of sweeps from captured hipGraphs (off by default:
+        no measured gain over direct launches)."""
+        call("flbp_plan_use_graphs", self._h, 1 if enable else 0)
+
     def profile(self, enable: bool = True) -> None:
         call("flbp_plan_profile", self._h, 1 if enable else 0)
 

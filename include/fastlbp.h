@@ -239,6 +239,13 @@ int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
  * layouts moves the stored messages, so a solve may continue under another
  * variant.  flbp_adjoint_create switches its plan to 0. */
 int flbp_plan_set_variant(flbp_plan *plan, int variant);
+/* Replay each chunk of sweeps from a hipGraph captured on first use (one
+ * per chunk length, damping, tol and variant), or launch its kernels one by
+ * one (0, the default: on MI355X the replay measured within 1% of direct
+ * launches at 100k and 1M variables, the per-sweep cost being on the GPU
+ * side).  Profiling (flbp_plan_profile) always launches one by one, since
+ * its events sit between the kernels.  Results are the same either way. */
+int flbp_plan_use_graphs(flbp_plan *plan, int enable);
 int flbp_plan_get_variant(const flbp_plan *plan);
 
 uint64_t flbp_plan_device_bytes(const flbp_plan *plan);

@@ -261,3 +261,34 @@ def test_wide_factor_and_variable(variant):
     assert rv.iters == it and same_bits(bv, o.beliefs())
     ofv, ovf = o.messages()
     assert same_bits(fv, ofv) and same_bits(vf, ovf)
+
+
+def test_graph_replays_match_direct_launches():
+    """Chunks replayed from captured hipGraphs, solve after solve (eager work
+    between the replays), give the bits of direct launches and the oracle."""
+    gen = fl.generate(20000, 91)
+    g = fl.Graph(gen.factors)
+    o = oracle.OracleGraph(gen.factors)
+    o.init()
+    it, md = o.run(1e-9, 300, 0.3, threads=8)
+    bo = o.beliefs()
+    for variant in (3, 0):
+        plan = fl.Plan(g)
+        plan.set_variant(variant)
+        plan.use_graphs(True)
+        for rep in range(3):
+            plan.reset()
+            plan.init()
+            r = plan.run(1e-9, 300, 0.3)
+            assert r.iters == it and same_bits(plan.beliefs(), bo), (variant, rep)
+        plan.reset()
+        plan.init()
+        plan.run(1e-14, 7, 0.0)           # undamped chunks, then damped ones replayed
+        plan.run(1e-14, 19, 0.3)
+        direct = fl.Plan(g)
+        direct.set_variant(variant)
+        direct.use_graphs(False)
+        direct.init()
+        direct.run(1e-14, 7, 0.0)
+        direct.run(1e-14, 19, 0.3)
+        assert same_bits(plan.beliefs(), direct.beliefs())
