@@ -249,11 +249,12 @@ __device__ __forceinline__ void factor_tile(
      * offsets and parameters into LDS, and its messages into LDS, all loads
      * independent of each other */
     const uint32_t nf = f1 - f0;
-    uint32_t w[kPerLane];
+    uint32_t w[kPerLane], vp[kPerLane];   /* work entries; var-major positions (gather or scatter index) */
 #pragma unroll
     for (int j = 0; j < kPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         w[j] = i < n ? fac_work[base + i] : 0u;
+        vp[j] = (kLayout != kLayoutPush && i < n) ? fac_vpos[base + i] : 0u;
     }
     if ((uint32_t)tid < nf) {
         sm.start[tid] = fac_off[f0 + tid] - base;
@@ -262,9 +263,13 @@ __device__ __forceinline__ void factor_tile(
         sm.q[tid] = fac_q[f0 + tid];
     }
     if (tid == 0) sm.start[nf] = n;
-    for (uint32_t e = tid; e < n; e += kBlock) {
-        sm.in[e] = kLayout == kLayoutPull ? m_vf[fac_vpos[base + e]] : m_vf[base + e];
-        if (kDamp) sm.out[e] = m_fv_old[base + e];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < n) {
+            sm.in[i] = kLayout == kLayoutPull ? m_vf[vp[j]] : m_vf[base + i];
+            if (kDamp) sm.out[i] = m_fv_old[base + i];
+        }
     }
     __syncthreads();
     /* phase 2: one lane per output slot, in the tile's work order
@@ -321,8 +326,16 @@ __device__ __forceinline__ void factor_tile(
         sm.out[sl] = r;
     }
     __syncthreads();
-    /* phase 3: stores */
-    for (uint32_t e = tid; e < n; e += kBlock) put_fv<kDamp, kLayout>(m_fv, m_fv_vm, fac_vpos, base + e, sm.out[e]);
+    /* phase 3: stores (the scatter positions came with phase 1) */
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < n) {
+            const double2 r = sm.out[i];
+            if (kLayout != kLayoutScatter || kDamp) m_fv[base + i] = r;
+            if (kLayout == kLayoutScatter) m_fv_vm[vp[j]] = r;
+        }
+    }
 }
 
 /* The variable pass over variables [v0, v1) of one graph (graph-local
@@ -388,14 +401,23 @@ __device__ __forceinline__ unsigned long long var_tile(
      * and the tile's messages, all loads independent of each other */
     const uint32_t nv = v1 - v0;
     uint32_t lv = 0;
+    uint32_t ve[kPerLane];   /* factor-major slots: gather or scatter index */
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        ve[j] = i < n ? var_edge[base + i] : 0u;
+    }
     if ((uint32_t)tid < nv) {
         lv = var_work[v0 + tid];
         sm.start[tid] = var_off[v0 + tid] - base;
         sm.bel[tid] = belief[v0 + tid];
     }
     if (tid == 0) sm.start[nv] = n;
-    for (uint32_t e = tid; e < n; e += kBlock)
-        sm.m[e] = kLayout == kLayoutScatter ? m_fv[base + e] : m_fv[var_edge[base + e]];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < n) sm.m[i] = kLayout == kLayoutScatter ? m_fv[base + i] : m_fv[ve[j]];
+    }
     __syncthreads();
     /* phase 2: products, one lane per variable in the tile's work order
      * (by degree), from LDS only */
@@ -424,11 +446,15 @@ __device__ __forceinline__ unsigned long long var_tile(
     __syncthreads();
     /* phase 3: leave-one-out messages, one lane per slot, stored coalesced
      * (var-major) or scattered to the factor slots */
-    for (uint32_t e = tid; e < n; e += kBlock) {
-        const uint32_t lv2 = sm.owner[e];
-        const uint32_t z = sm.z[lv2];
-        const double2 r = vf_msg(sm.prod[0][lv2], sm.prod[1][lv2], z & 0xffffu, z >> 16, sm.m[e]);
-        m_vf[kLayout == kLayoutPull ? base + e : var_edge[base + e]] = r;
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < n) {
+            const uint32_t lv2 = sm.owner[i];
+            const uint32_t z = sm.z[lv2];
+            const double2 r = vf_msg(sm.prod[0][lv2], sm.prod[1][lv2], z & 0xffffu, z >> 16, sm.m[i]);
+            m_vf[kLayout == kLayoutPull ? base + i : ve[j]] = r;
+        }
     }
     if ((uint32_t)tid < nv) belief[v0 + tid] = sm.bel[tid];
     return diff_bits;
