@@ -5,11 +5,12 @@ in include/ext and include/lbp).  This package is the Python verification
 harness over it.
 """
 from ._native import FlbpError, LIB_PATH, lib
-from .engine import (FactorList, Generated, Graph, Plan, RunResult, algorithmic_bytes_per_sweep,
-                     clamp, generate, rank_alarms, read_causal_fg, write_causal_fg)
+from .engine import (FactorList, Generated, Graph, Plan, RunResult, Schedule, ScheduleInfo,
+                     algorithmic_bytes_per_sweep, clamp, generate, rank_alarms, read_causal_fg, seed,
+                     write_causal_fg)
 
 __all__ = [
     "FlbpError", "LIB_PATH", "lib", "FactorList", "Generated", "Graph", "Plan", "RunResult",
-    "algorithmic_bytes_per_sweep", "clamp", "generate", "rank_alarms", "read_causal_fg",
-    "write_causal_fg",
+    "Schedule", "ScheduleInfo", "algorithmic_bytes_per_sweep", "clamp", "generate", "rank_alarms",
+    "read_causal_fg", "seed", "write_causal_fg",
 ]

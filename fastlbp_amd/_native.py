@@ -97,6 +97,21 @@ class BatchOptsC(C.Structure):
 LAYOUT_AUTO, LAYOUT_PULL, LAYOUT_SCATTER = 0, 1, 2   # FLBP_LAYOUT_* of fastlbp.h
 
 
+class ScheduleInfoC(C.Structure):
+    _fields_ = [
+        ("n_order", C.c_uint64),
+        ("n_waves", C.c_uint64),
+        ("n_fv_items", C.c_uint64),
+        ("n_vf_items", C.c_uint64),
+        ("n_levels", C.c_uint64),
+        ("updates", C.c_int32),
+        ("respect_preorder", C.c_int32),
+    ]
+
+
+UPDATES_SEQFIX, UPDATES_PARALL, UPDATES_TOPO = 0, 1, 2   # FLBP_UPDATES_* of fastlbp.h
+
+
 class RunResultC(C.Structure):
     _fields_ = [
         ("iters", C.c_uint64),
@@ -166,6 +181,16 @@ SIGNATURES = {
     "flbp_adjoint_device_bytes": (C.c_uint64, [C.c_void_p]),
     "flbp_tied_p": (C.c_int, [C.c_uint64, f64p, i32p, C.c_uint64, f64p, f64p]),
     "flbp_rule_grad": (C.c_int, [C.c_uint64, f64p, f64p, i32p, C.c_uint64, f64p]),
+    "flbp_schedule_create": (C.c_int, [C.POINTER(CsrC), C.c_int, C.POINTER(C.c_void_p)]),
+    "flbp_schedule_from_order": (C.c_int, [C.POINTER(CsrC), u64p, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
+    "flbp_schedule_destroy": (None, [C.c_void_p]),
+    "flbp_schedule_get_info": (C.c_int, [C.c_void_p, C.POINTER(ScheduleInfoC)]),
+    "flbp_schedule_order": (C.c_int, [C.c_void_p, u64p]),
+    "flbp_schedule_waves": (C.c_int, [C.c_void_p, u64p, u64p, u64p, u64p]),
+    "flbp_schedule_levels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "flbp_seed": (None, [C.c_uint32]),
+    "flbp_plan_set_schedule": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "flbp_plan_get_updates": (C.c_int, [C.c_void_p]),
     "flbp_last_error": (C.c_char_p, []),
     "flbp_version": (C.c_char_p, []),
 }

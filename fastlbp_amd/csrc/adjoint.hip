@@ -191,6 +191,9 @@ int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_
         return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: bad arguments");
     if (!(damping >= 0.0 && damping < 1.0))
         return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: damping must be in [0, 1)");
+    if (P->seq)
+        return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: the adjoint differentiates PARALL sweeps, and the plan "
+                                       "has a sequential schedule");
     if (P->variant >= 2) {   /* the adjoint walks the pull layout: move the plan's messages there */
         const int rc = flbp_plan_set_variant(plan, 0);
         if (rc) return rc;

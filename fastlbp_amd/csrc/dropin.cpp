@@ -215,6 +215,7 @@ void CausalBP::setProperties(const PropertySet &opts) {
     props.updates = Properties::UpdateType::SEQFIX;
     if (opts.hasKey("updates")) {
         const std::string u = opts.getStringAs<std::string>("updates");
+        /* anything else is SEQFIX (causal_bp.cu:165-177) */
         if (u == "PARALL") props.updates = Properties::UpdateType::PARALL;
         else if (u == "TOPO") props.updates = Properties::UpdateType::TOPO;
     }
@@ -224,8 +225,6 @@ void CausalBP::setProperties(const PropertySet &opts) {
 }
 
 void CausalBP::construct() {
-    if (props.updates != Properties::UpdateType::PARALL)
-        LOG_FATAL("only updates=PARALL is implemented by this engine (SEQFIX/TOPO are not)");
     const size_t V = nrVars(), F = nrFactors();
     /* factor-major slots in nbF order, var-major edges in nbV order
      * (causal_bp.cu:336-369, 724-817) */
@@ -277,6 +276,16 @@ void CausalBP::construct() {
     c.mask0 = mask0.data();
     c.mask1 = mask1.data();
     check(flbp_plan_create(&c, (int)props.device, &_plan), "flbp_plan_create");
+    if (props.updates != Properties::UpdateType::PARALL) {
+        /* buildFixedUpdateSeq / buildTopoUpdateSeq + putUpdateSeqToKernelFused
+         * (causal_bp.cu:396-406); SEQFIX draws from this thread's rnd_gen */
+        flbp_schedule *s = nullptr;
+        const int u = props.updates == Properties::UpdateType::TOPO ? FLBP_UPDATES_TOPO : FLBP_UPDATES_SEQFIX;
+        check(flbp_schedule_create(&c, u, &s), "flbp_schedule_create");
+        const int rc = flbp_plan_set_schedule(_plan, s);
+        flbp_schedule_destroy(s);
+        check(rc, "flbp_plan_set_schedule");
+    }
     _beliefsV.assign(V, 0.0);
 }
 

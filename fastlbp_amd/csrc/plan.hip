@@ -97,6 +97,10 @@ int hip_fail(hipError_t e, const char *what) {
     } while (0)
 
 int Plan::launch_sweep(int k, double tol, double damping) {
+    if (seq) {   /* a sequential schedule updates the var-major f->v array only */
+        fm_stale = true;
+        return launch_seq_sweep(k, tol);
+    }
     if (variant == 3) {
         /* undamped sweeps of the scatter layout skip the factor-major f->v
          * copy; a damped sweep first rebuilds it from the var-major array */
@@ -212,7 +216,8 @@ int Plan::chunk_graph(int n, double tol, double damping, hipGraphExec_t &exec) {
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(clear_u64, dim3(16), dim3(256), 0, stream, ctl, kMaxChunk * kCtlPerSweep);
     int rc = FLBP_OK;
-    for (int k = 0; k < n && !rc; k++) rc = launch_sweep_on(k, tol, damping, m_vf, m_fv, m_fv, m_vf, belief);
+    for (int k = 0; k < n && !rc; k++)
+        rc = seq ? launch_seq_sweep(k, tol) : launch_sweep_on(k, tol, damping, m_vf, m_fv, m_fv, m_vf, belief);
     hipLaunchKernelGGL(ctl_compact, dim3(1), dim3(64), 0, stream, ctl, ctl_out, n);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(stream, &g);
@@ -234,7 +239,9 @@ int Plan::chunk_graph(int n, double tol, double damping, hipGraphExec_t &exec) {
 
 int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_diff) {
     if (!profiling && use_graphs) {
-        if (variant == 3) {   /* the scatter layout's damping copy, as launch_sweep keeps it */
+        if (seq) {
+            fm_stale = true;
+        } else if (variant == 3) {   /* the scatter layout's damping copy, as launch_sweep keeps it */
             if (damping != 0.0 && fm_stale && E) {
                 hipLaunchKernelGGL(gather_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_fv_vm, fac_vpos,
                                    m_fv, E);
@@ -284,6 +291,7 @@ int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_di
 Plan::~Plan() {
     if (device >= 0) (void)hipSetDevice(device);
     drop_graphs();
+    free_seq_items();
     for (void *p : allocs) (void)hipFree(p);
     if (stage_idx) (void)hipFree(stage_idx);
     if (stage_val) (void)hipFree(stage_val);
@@ -340,6 +348,7 @@ int flbp_plan_init(flbp_plan *plan, int logdomain) {
         hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_fv, P->E, c);
         hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_fv_vm, P->E, c);
     }
+    if (int rc = flbp::seq_reset_state(P, c)) return rc;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->fm_stale = false;
@@ -353,6 +362,8 @@ int flbp_plan_run(flbp_plan *plan, const flbp_run_opts *o, flbp_run_result *res)
     if (!P || !o) return flbp::fail(FLBP_EINVAL, "flbp_plan_run: null argument");
     if (!(o->damping >= 0.0 && o->damping < 1.0))
         return flbp::fail(FLBP_EINVAL, "flbp_plan_run: damping must be in [0, 1)");
+    if (P->seq && o->damping != 0.0)
+        return flbp::fail(FLBP_EINVAL, "flbp_plan_run: damping is defined for PARALL updates only");
     HIP_TRY(hipSetDevice(P->device));
     const auto t0 = std::chrono::steady_clock::now();
     auto elapsed = [&] {
@@ -391,6 +402,8 @@ int flbp_plan_run_fixed(flbp_plan *plan, uint64_t n_sweeps, double damping, flbp
     if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_run_fixed: null plan");
     if (!(damping >= 0.0 && damping < 1.0))
         return flbp::fail(FLBP_EINVAL, "flbp_plan_run_fixed: damping must be in [0, 1)");
+    if (P->seq && damping != 0.0)
+        return flbp::fail(FLBP_EINVAL, "flbp_plan_run_fixed: damping is defined for PARALL updates only");
     HIP_TRY(hipSetDevice(P->device));
     const auto t0 = std::chrono::steady_clock::now();
     P->reset_profile();
@@ -624,6 +637,8 @@ int flbp_plan_get_variant(const flbp_plan *plan) {
 int flbp_plan_set_variant(flbp_plan *plan, int variant) {
     Plan *P = reinterpret_cast<Plan *>(plan);
     if (!P || variant < 0 || variant > 3) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");
+    if (P->seq && variant != 3)
+        return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: a plan with a sequential schedule keeps variant 3");
     auto layout = [](int v) { return v == 1 ? 0 : v; };
     if (layout(P->variant) != layout(variant) && P->E) {
         /* move the stored messages to the new layout through their

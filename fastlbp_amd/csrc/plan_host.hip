@@ -164,6 +164,10 @@ int Plan::alloc(T *&p, uint64_t n) {
     return FLBP_OK;
 }
 
+/* element types allocated from other translation units */
+template int Plan::alloc<double2>(double2 *&, uint64_t);
+template int Plan::alloc<ulonglong2>(ulonglong2 *&, uint64_t);
+
 #define TRY(expr)          \
     do {                   \
         int rc_ = (expr);  \

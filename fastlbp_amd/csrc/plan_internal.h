@@ -43,6 +43,8 @@ void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const 
 void var_device_order(const flbp_csr &c, bool by_degree, std::vector<uint32_t> &perm,
                       std::vector<uint32_t> &dev_off, std::vector<uint32_t> &dev_edge,
                       std::vector<uint32_t> &vpos);
+struct Plan;
+int seq_reset_state(Plan *P, double c);   /* seq.hip */
 
 struct Plan {
     int device = -1;
@@ -78,6 +80,17 @@ struct Plan {
     std::vector<uint32_t> h_meta, h_meta0;         /* factor meta words: current and at creation */
     std::vector<double2> h_mask, h_mask0;          /* head masks: current and at creation */
     std::vector<uint64_t> clamp_dirty;             /* factors whose meta / mask differ from creation */
+
+    /* sequential schedule (updates = SEQFIX / TOPO, seq.hip); none = PARALL */
+    bool seq = false;
+    int seq_updates = -1;                          /* FLBP_UPDATES_* or -1 (explicit order) */
+    uint4 *seq_vf = nullptr, *seq_fv = nullptr;    /* items grouped by level */
+    double2 *seq_prod = nullptr;                   /* per-variable products, maintained in place */
+    ulonglong2 *seq_nz = nullptr;                  /* per-variable zero counts */
+    std::vector<uint32_t> seq_lvl_vf, seq_lvl_fv;  /* [levels+1] item offsets per level */
+    uint64_t seq_waves = 0;
+    int launch_seq_sweep(int k, double tol);
+    void free_seq_items();
 
     uint64_t iters = 0;                            /* CausalBP::_iters */
     double last_maxdiff = INFINITY;

@@ -61,6 +61,27 @@ __device__ __forceinline__ void scale_pair(double &x, double &y) {
     y = x_is_max ? r : one;
 }
 
+/* scale() of the sequential kernels, csrc/kernel/causal_bp_seq.cu:13-23:
+ * the smaller value over the larger, the larger set to 1; equal (or NaN)
+ * values both become 1 */
+__device__ __forceinline__ void scale_seq(double &x, double &y) {
+    if (x < y) {
+        x = x / y;
+        y = 1.0;
+    } else if (y < x) {
+        y = y / x;
+        x = 1.0;
+    } else {
+        x = y = 1.0;
+    }
+}
+
+template <bool kSeq>
+__device__ __forceinline__ void scale_any(double &x, double &y) {
+    if constexpr (kSeq) scale_seq(x, y);
+    else scale_pair(x, y);
+}
+
 /* Messages of one factor, either held in registers or read from global
  * memory.  get(j) is only called with indices that are compile-time
  * constants after unrolling; pick(j) fetches a run-time slot with an AND/OR
@@ -87,6 +108,11 @@ struct GlobalIn {
     __device__ __forceinline__ double2 get(int j) const { return m[pos[j]]; }
     __device__ __forceinline__ double2 pick(int j) const { return m[pos[j]]; }
 };
+struct ContigIn {
+    const double2 *__restrict__ m;     /* the factor's messages, slot j at m[j] */
+    __device__ __forceinline__ double2 get(int j) const { return m[j]; }
+    __device__ __forceinline__ double2 pick(int j) const { return m[j]; }
+};
 
 /* Runs f(j) for j = 0..d-1 in order; unrolled when the degree bound N is a
  * compile-time register count, a plain loop when N == 0. */
@@ -106,7 +132,7 @@ __device__ __forceinline__ void for_slots(int d, F &&f) {
  * state the CPD singles out (state 1 for AND, state 0 for OR); AND gives
  * (marg0, marg1) = (w, u), OR gives (u, w).  causal_bp_parall.cu:113-143,
  * 152-182. */
-template <int N, class In>
+template <int N, bool kSeq = false, class In>
 __device__ __forceinline__ void head_uw(const In &in, int d, int t, double2 m_other, bool is_and,
                                         double p, double q, double &u, double &w) {
     if (d == 1) {
@@ -130,7 +156,7 @@ __device__ __forceinline__ void head_uw(const In &in, int d, int t, double2 m_ot
             const double delta = is_and ? m.x : m.y;
             px *= x;
             p10 *= a0;
-            scale_pair(px, p10);
+            scale_any<kSeq>(px, p10);
             if (x != 0 && a0 == x && delta != 0) e1 += delta / x;
         }
     });
@@ -141,7 +167,7 @@ __device__ __forceinline__ void head_uw(const In &in, int d, int t, double2 m_ot
 
 /* The body-target message (codes 5-8): causal_bp_parall.cu:289-393.
  * Unclamped factors use masks (1, 1), which multiply exactly. */
-template <int N, class In>
+template <int N, bool kSeq = false, class In>
 __device__ __forceinline__ double2 body_msg(const In &in, int d, int t, int h, double2 mh,
                                             bool is_and, double p, double q, double mask0,
                                             double mask1) {
@@ -149,14 +175,14 @@ __device__ __forceinline__ double2 body_msg(const In &in, int d, int t, int h, d
     const double mx = is_and ? mask1 : mask0, my = is_and ? mask0 : mask1;
     double P = (p - q) * (hx * mx - hy * my);
     double Q = q * hx * mx + (1 - q) * hy * my;
-    scale_pair(P, Q);
+    scale_any<kSeq>(P, Q);
     if (Q != 0) {
         for_slots<N>(d, [&](int j) __attribute__((always_inline)) {
             if (j != t && j != h) {
                 const double2 m = in.get(j);
                 Q *= m.x + m.y;
                 P *= is_and ? m.y : m.x;
-                scale_pair(P, Q);
+                scale_any<kSeq>(P, Q);
             }
         });
     }
@@ -177,7 +203,7 @@ __device__ __forceinline__ double2 body_msg(const In &in, int d, int t, int h, d
 /* The f->v message to slot t of a factor (type, degree d, head slot h).
  * mh = message of the head slot, m_other = message of slot (h == 0 ? 1 : 0)
  * (the only other slot of a degree-2 factor). */
-template <int N, class In>
+template <int N, bool kSeq = false, class In>
 __device__ __forceinline__ double2 fv_msg(const In &in, uint32_t type, bool clamped, int d, int t,
                                           int h, double2 mh, double2 m_other, double p, double q,
                                           double mask0, double mask1) {
@@ -188,9 +214,9 @@ __device__ __forceinline__ double2 fv_msg(const In &in, uint32_t type, bool clam
         return r;
     }
     const bool is_and = (type == 1);
-    if (t != h) return body_msg<N>(in, d, t, h, mh, is_and, p, q, mask0, mask1);
+    if (t != h) return body_msg<N, kSeq>(in, d, t, h, mh, is_and, p, q, mask0, mask1);
     double u, w;
-    head_uw<N>(in, d, t, m_other, is_and, p, q, u, w);
+    head_uw<N, kSeq>(in, d, t, m_other, is_and, p, q, u, w);
     double marg0 = is_and ? w : u, marg1 = is_and ? u : w;
     if (clamped) {
         marg0 *= mask0;

@@ -1,9 +1,12 @@
 /*
- * flbp_run -- solves a .causal_fg graph with CausalBP::run() (PARALL) on the
- * GPU through the drop-in classes and prints "iters N", "maxdiff X" and one
+ * flbp_run -- solves a .causal_fg graph with CausalBP::run() on the GPU
+ * through the drop-in classes and prints "iters N", "maxdiff X" and one
  * "label marginal" line per variable (%.17g).
  *
- * usage: flbp_run <file.causal_fg> [tol] [maxiter] [damping]
+ * usage: flbp_run <file.causal_fg> [tol] [maxiter] [damping] [updates] [skip]
+ *   updates: PARALL (default), SEQFIX or TOPO
+ *   skip:    CausalBP instances constructed and dropped first (each SEQFIX
+ *            construction advances the thread's rnd_gen)
  */
 #include <cstdio>
 #include <cstdlib>
@@ -15,7 +18,7 @@ using namespace lbp;
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <file.causal_fg> [tol] [maxiter] [damping]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <file.causal_fg> [tol] [maxiter] [damping] [updates] [skip]\n", argv[0]);
         return 2;
     }
     CausalFactorGraph fg;
@@ -25,8 +28,10 @@ int main(int argc, char **argv) {
     opts.set("maxiter", static_cast<size_t>(argc > 3 ? std::strtoull(argv[3], nullptr, 10) : 1000));
     opts.set("damping", Real(argc > 4 ? std::atof(argv[4]) : 0.0));
     opts.set("maxtime", Real(1e30));
-    opts.set("updates", std::string("PARALL"));
+    opts.set("updates", std::string(argc > 5 ? argv[5] : "PARALL"));
     opts.set("logdomain", false);
+    const size_t skip = argc > 6 ? std::strtoull(argv[6], nullptr, 10) : 0;
+    for (size_t k = 0; k < skip; k++) CausalBP dropped(fg, opts);
     CausalBP bp(fg, opts);
     bp.init();
     bp.run();

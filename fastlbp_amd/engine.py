@@ -14,8 +14,8 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._native import (CsrC, FactorListC, GenParamsC, RunOptsC, RunResultC, call, check, f64p, i32p,
-                      lib, u8p, u64p)
+from ._native import (UPDATES_PARALL, UPDATES_SEQFIX, UPDATES_TOPO, CsrC, FactorListC, GenParamsC, RunOptsC,
+                      RunResultC, ScheduleInfoC, call, check, f64p, i32p, lib, u8p, u64p)
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -220,6 +220,93 @@ class Graph:
 
 
 @dataclass
+class ScheduleInfo:
+    n_order: int
+    n_waves: int
+    n_fv_items: int
+    n_vf_items: int
+    n_levels: int
+    updates: int
+    respect_preorder: bool
+
+
+class Schedule:
+    """A sequential update order split into the reference's waves
+    (flbp_schedule_*); built on the host, no GPU needed.
+
+    ``Schedule(graph, "SEQFIX")`` shuffles the factors with the calling
+    thread's std::mt19937 (seeded 42 per thread, advanced by every SEQFIX
+    schedule; see :func:`seed`), ``Schedule(graph, "TOPO")`` takes the
+    breadth-first preorder, and ``Schedule(graph, order=...)`` splits an
+    explicit order of var-major edge positions."""
+
+    def __init__(self, graph: Graph, updates: str = "SEQFIX", order=None, respect_preorder: bool = False):
+        self.graph = graph
+        self._h = C.c_void_p()
+        c = graph.csr_c()
+        if order is not None:
+            o = np.ascontiguousarray(order, dtype=np.uint64)
+            buf = o if o.size else np.zeros(1, np.uint64)
+            call("flbp_schedule_from_order", C.byref(c), _ptr(buf, C.c_uint64), o.shape[0],
+                 1 if respect_preorder else 0, C.byref(self._h))
+        else:
+            code = {"SEQFIX": UPDATES_SEQFIX, "TOPO": UPDATES_TOPO}.get(updates)
+            if code is None:
+                raise ValueError("updates must be 'SEQFIX' or 'TOPO' (a plan without schedule is PARALL)")
+            call("flbp_schedule_create", C.byref(c), code, C.byref(self._h))
+
+    def close(self):
+        if self._h:
+            lib().flbp_schedule_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> ScheduleInfo:
+        i = ScheduleInfoC()
+        call("flbp_schedule_get_info", self._h, C.byref(i))
+        return ScheduleInfo(int(i.n_order), int(i.n_waves), int(i.n_fv_items), int(i.n_vf_items),
+                            int(i.n_levels), int(i.updates), bool(i.respect_preorder))
+
+    def order(self) -> np.ndarray:
+        n = self.info().n_order
+        out = np.zeros(max(n, 1), np.uint64)
+        call("flbp_schedule_order", self._h, _ptr(out, C.c_uint64))
+        return out[:n]
+
+    def waves(self):
+        """(fv_off, fv_edge, vf_off, vf_slot): wave w updates the var-major
+        edges fv_edge[fv_off[w]:fv_off[w+1]] after recomputing the
+        factor-major v->f slots vf_slot[vf_off[w]:vf_off[w+1]]."""
+        i = self.info()
+        fo = np.zeros(i.n_waves + 1, np.uint64)
+        vo = np.zeros(i.n_waves + 1, np.uint64)
+        fe = np.zeros(max(i.n_fv_items, 1), np.uint64)
+        vs = np.zeros(max(i.n_vf_items, 1), np.uint64)
+        call("flbp_schedule_waves", self._h, _ptr(fo, C.c_uint64), _ptr(fe, C.c_uint64), _ptr(vo, C.c_uint64),
+             _ptr(vs, C.c_uint64))
+        return fo, fe[:i.n_fv_items], vo, vs[:i.n_vf_items]
+
+
+    def levels(self):
+        """(fv_level, vf_level): the dependency level of each item of waves()."""
+        i = self.info()
+        a = np.zeros(max(i.n_fv_items, 1), np.uint32)
+        b = np.zeros(max(i.n_vf_items, 1), np.uint32)
+        call("flbp_schedule_levels", self._h, _ptr(a, C.c_uint32), _ptr(b, C.c_uint32))
+        return a[:i.n_fv_items], b[:i.n_vf_items]
+
+
+def seed(value: int) -> None:
+    """Reseed the calling thread's SEQFIX generator (rnd_gen.seed)."""
+    lib().flbp_seed(int(value) & 0xffffffff)
+
+
+@dataclass
 class RunResult:
     iters: int
     sweeps: int
@@ -324,6 +411,16 @@ class Plan:
         cache).  Results are identical; switching layouts moves the stored
         messages."""
         call("flbp_plan_set_variant", self._h, variant)
+
+    def set_schedule(self, schedule: Optional[Schedule]) -> None:
+        """Run sweeps by a sequential schedule (None: PARALL).  The plan keeps
+        a copy; products and zero counts start at what init() leaves."""
+        call("flbp_plan_set_schedule", self._h, None if schedule is None else schedule._h)
+
+    @property
+    def updates(self) -> int:
+        """FLBP_UPDATES_PARALL without a schedule, else the schedule's."""
+        return int(lib().flbp_plan_get_updates(self._h))
 
     def use_graphs(self, enable: bool = True) -> None:
         """Replay chunks of sweeps from captured hipGraphs (off by default:

@@ -7,8 +7,11 @@
  *   maxtime (Real, 10 s), updates (std::string, "SEQFIX"), verbose (size_t, 0)
  * plus, not in the reference:
  *   damping (Real, 0), device (size_t, 0).
- * Only updates = "PARALL" is implemented; other schedules stop with
- * LOG_FATAL at construction. */
+ * updates: "PARALL" (flooding), "SEQFIX" (the default, and any unknown
+ * string: factors shuffled by the thread's rnd_gen, declared in
+ * utils/utils.h) or "TOPO" (breadth-first preorder).  The sequential
+ * schedules run as dependency levels of the reference's waves with the
+ * reference's per-wave results; damping applies to PARALL only. */
 #pragma once
 
 #include <cstdint>

@@ -251,6 +251,75 @@ int flbp_plan_get_variant(const flbp_plan *plan);
 uint64_t flbp_plan_device_bytes(const flbp_plan *plan);
 
 /* ------------------------------------------------------------------ */
+/* Sequential update schedules (updates = SEQFIX / TOPO)               */
+/* ------------------------------------------------------------------ */
+
+/* CausalBP::Properties::UpdateType (include/ext/causal_bp.h:29-33). */
+#define FLBP_UPDATES_SEQFIX 0
+#define FLBP_UPDATES_PARALL 1
+#define FLBP_UPDATES_TOPO 2
+
+/* An update order split into the reference's waves
+ * (putUpdateSeqToKernelFused, csrc/ext/causal_bp.cu:583-722), built on the
+ * host from a graph's CSR; no GPU is needed.  Orders are var-major edge
+ * positions: the reference's Edge(i, dual) is var_off[i] + dual. */
+typedef struct flbp_schedule flbp_schedule;
+
+typedef struct flbp_schedule_info {
+    uint64_t n_order;        /* length of the update order (E for SEQFIX / TOPO) */
+    uint64_t n_waves;        /* the reference's _updateSeqVF.size() */
+    uint64_t n_fv_items;     /* f->v updates over all waves */
+    uint64_t n_vf_items;     /* v->f recomputations over all waves */
+    uint64_t n_levels;       /* launches per sweep on the GPU */
+    int32_t updates;         /* FLBP_UPDATES_SEQFIX / _TOPO, or -1 for an explicit order */
+    int32_t respect_preorder;
+} flbp_schedule_info;
+
+/* The reference's schedule for `updates`: SEQFIX shuffles the factor
+ * indices with std::shuffle by the calling thread's std::mt19937
+ * (buildFixedUpdateSeq, causal_bp.cu:186-201; the generator starts at seed
+ * 42 in each thread and every SEQFIX schedule advances it, as every
+ * reference CausalBP construction does); TOPO takes the breadth-first
+ * preorder (buildTopoUpdateSeq, 203-304) with respect_preorder set.
+ * FLBP_UPDATES_PARALL is rejected (a plan without schedule is PARALL). */
+int flbp_schedule_create(const flbp_csr *c, int updates, flbp_schedule **out);
+/* An explicit order of n var-major edge positions (repeats allowed). */
+int flbp_schedule_from_order(const flbp_csr *c, const uint64_t *order, uint64_t n, int respect_preorder,
+                             flbp_schedule **out);
+void flbp_schedule_destroy(flbp_schedule *s);
+int flbp_schedule_get_info(const flbp_schedule *s, flbp_schedule_info *info);
+/* the order [n_order] */
+int flbp_schedule_order(const flbp_schedule *s, uint64_t *order);
+/* wave w updates fv_edge[fv_off[w] .. fv_off[w+1]) (var-major positions)
+ * after recomputing vf_slot[vf_off[w] .. vf_off[w+1]) (factor-major slots);
+ * fv_off / vf_off have n_waves + 1 entries.  NULL skips an array. */
+int flbp_schedule_waves(const flbp_schedule *s, uint64_t *fv_off, uint64_t *fv_edge, uint64_t *vf_off,
+                        uint64_t *vf_slot);
+/* the dependency level (1 .. n_levels) of every f->v item [n_fv_items] and
+ * v->f item [n_vf_items], in the order of flbp_schedule_waves.  NULL skips. */
+int flbp_schedule_levels(const flbp_schedule *s, uint32_t *fv_level, uint32_t *vf_level);
+/* rnd_gen.seed(seed) on the calling thread, as examples/bingo_wrapper.cu:115
+ * does before constructing its CausalBP. */
+void flbp_seed(uint32_t seed);
+
+/* Runs the plan's sweeps by a schedule (NULL: back to PARALL).  Its waves
+ * run as dependency levels: every f->v and v->f update of the waves gets a
+ * level one above the updates before it whose results it reads or whose
+ * inputs it overwrites, and one launch runs one level, so the results are
+ * the reference's per-wave results bit for bit with far fewer launches
+ * (a SEQFIX sweep of a 1M-variable graph: thousands of waves, under a
+ * hundred levels).  The plan switches to variant 3 (the reference's message
+ * orders) and keeps per-variable products and zero counts, set to the
+ * values flbp_plan_init leaves (c, no zeros) here and by every init.  The
+ * schedule is copied; it may be destroyed afterwards.  Damping is defined
+ * for PARALL only: run calls with damping != 0 fail under a schedule, and
+ * so do flbp_plan_set_variant to another variant and flbp_adjoint_create. */
+int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *s);
+/* FLBP_UPDATES_PARALL without a schedule, else the schedule's updates
+ * (-1 for an explicit order) */
+int flbp_plan_get_updates(const flbp_plan *plan);
+
+/* ------------------------------------------------------------------ */
 /* Batched solve (not in the reference)                                */
 /* ------------------------------------------------------------------ */
 

@@ -1,5 +1,6 @@
 /* utils/utils.h -- basic types and helpers of the drop-in C++ API.
- * Same names as the reference's include/utils/utils.h:21-22, 109-122. */
+ * Same names as the reference's include/utils/utils.h:21-22, 109-122,
+ * 133-136. */
 #pragma once
 
 #include <sys/time.h>
@@ -7,6 +8,7 @@
 #include <cstddef>
 #include <iostream>
 #include <map>
+#include <random>
 #include <set>
 #include <string>
 #include <vector>
@@ -15,6 +17,11 @@ namespace lbp {
 
 typedef double Real;
 typedef size_t Size;
+
+/* the global generator behind SEQFIX update orders: one per thread, seeded
+ * 42 at thread start (include/utils/utils.h:133-136, causal_bp.cu:116) */
+typedef std::mt19937 rnd_gen_type;
+extern thread_local rnd_gen_type rnd_gen;
 
 /* wall clock in seconds */
 inline Real toc() {

@@ -1,6 +1,7 @@
 """ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
 
-The CPU FP64 restatement of the reference's PARALL path (see oracle.cpp for
+The CPU FP64 restatement of the reference's CausalBP path -- PARALL, SEQFIX
+and TOPO updates (see oracle.cpp for
 the per-function citations).  Only tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg may import this module; the engine never does.
 """
@@ -54,6 +55,21 @@ _SIG = {
                            C.c_uint64, C.c_double, f64p, f64p]),
     "orc_loss": (C.c_int, [C.POINTER(OrcFactorList), i32p, C.c_uint64, f64p, C.c_uint64, u64p, f64p,
                            C.c_uint64, C.c_double, f64p]),
+    "orc_rng_new": (C.c_void_p, [C.c_uint32]),
+    "orc_rng_free": (None, [C.c_void_p]),
+    "orc_rng_next": (C.c_uint32, [C.c_void_p]),
+    "orc_shuffle": (C.c_int, [C.c_void_p, C.c_uint64, u64p]),
+    "orc_std_shuffle": (C.c_int, [C.c_uint32, C.c_uint64, u64p]),
+    "orc_order_seqfix": (C.c_int, [C.c_void_p, C.c_void_p, u64p]),
+    "orc_order_topo": (C.c_int, [C.c_void_p, u64p]),
+    "orc_seq_set": (C.c_int, [C.c_void_p, u64p, C.c_uint64, C.c_int]),
+    "orc_seq_clear": (C.c_int, [C.c_void_p]),
+    "orc_seq_info": (C.c_int, [C.c_void_p, u64p, u64p, u64p]),
+    "orc_seq_waves": (C.c_int, [C.c_void_p, u64p, u64p, u64p, u64p]),
+    "orc_products": (C.c_int, [C.c_void_p, f64p, f64p, u64p, u64p]),
+    "orc_seq_sweep_items": (C.c_int, [C.c_void_p, C.c_uint64, u8p, u64p, f64p]),
+    "orc_fv_message_seq": (None, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, f64p, f64p, f64p, f64p]),
     "orc_last_error": (C.c_char_p, []),
 }
 
@@ -169,12 +185,94 @@ class OracleGraph:
         _chk("orc_set_messages", lib().orc_set_messages(self._h, *[_p(x, C.c_double) for x in a]))
 
 
-def fv_message(code, m0, m1, src, head, p, q=0.0, mask0=1.0, mask1=1.0):
+    # ---- sequential schedules (SEQFIX / TOPO) ----
+    def order_seqfix(self, rng: "Rng") -> np.ndarray:
+        out = np.zeros(max(self.n_edges, 1), np.uint64)
+        _chk("orc_order_seqfix", lib().orc_order_seqfix(self._h, rng._h, _p(out, C.c_uint64)))
+        return out[:self.n_edges]
+
+    def order_topo(self) -> np.ndarray:
+        out = np.zeros(max(self.n_edges, 1), np.uint64)
+        _chk("orc_order_topo", lib().orc_order_topo(self._h, _p(out, C.c_uint64)))
+        return out[:self.n_edges]
+
+    def set_sequence(self, order, respect_preorder=False):
+        o = np.ascontiguousarray(order, np.uint64)
+        if o.size == 0:
+            o = np.zeros(1, np.uint64)
+            n = 0
+        else:
+            n = o.shape[0]
+        _chk("orc_seq_set", lib().orc_seq_set(self._h, _p(o, C.c_uint64), n, 1 if respect_preorder else 0))
+
+    def clear_sequence(self):
+        _chk("orc_seq_clear", lib().orc_seq_clear(self._h))
+
+    def waves(self):
+        """(fv_off, fv_edge, vf_off, vf_slot) of the wave split."""
+        w, a, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _chk("orc_seq_info", lib().orc_seq_info(self._h, C.byref(w), C.byref(a), C.byref(b)))
+        W, nf, nv = int(w.value), int(a.value), int(b.value)
+        fo = np.zeros(W + 1, np.uint64)
+        vo = np.zeros(W + 1, np.uint64)
+        fe = np.zeros(max(nf, 1), np.uint64)
+        vs = np.zeros(max(nv, 1), np.uint64)
+        _chk("orc_seq_waves", lib().orc_seq_waves(self._h, _p(fo, C.c_uint64), _p(fe, C.c_uint64),
+                                                 _p(vo, C.c_uint64), _p(vs, C.c_uint64)))
+        return fo, fe[:nf], vo, vs[:nv]
+
+    def sweep_items(self, kind, item):
+        """One sweep of explicitly ordered items (checker of a level split)."""
+        k = np.ascontiguousarray(kind, np.uint8)
+        it = np.ascontiguousarray(item, np.uint64)
+        md = C.c_double()
+        _chk("orc_seq_sweep_items", lib().orc_seq_sweep_items(self._h, k.shape[0], _p(k, C.c_uint8),
+                                                             _p(it, C.c_uint64), C.byref(md)))
+        return float(md.value)
+
+    def products(self):
+        V = self.n_vars
+        a = [np.zeros(max(V, 1), np.float64) for _ in range(2)] + [np.zeros(max(V, 1), np.uint64) for _ in range(2)]
+        _chk("orc_products", lib().orc_products(self._h, _p(a[0], C.c_double), _p(a[1], C.c_double),
+                                               _p(a[2], C.c_uint64), _p(a[3], C.c_uint64)))
+        return [x[:V] for x in a]
+
+
+class Rng:
+    """A std::mt19937 owned by the oracle (the reference's rnd_gen)."""
+
+    def __init__(self, seed: int = 42):
+        self._h = C.c_void_p(lib().orc_rng_new(seed))
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().orc_rng_free(self._h)
+        except Exception:
+            pass
+
+    def next(self) -> int:
+        return int(lib().orc_rng_next(self._h))
+
+    def shuffle(self, n: int) -> np.ndarray:
+        out = np.zeros(max(n, 1), np.uint64)
+        _chk("orc_shuffle", lib().orc_shuffle(self._h, n, _p(out, C.c_uint64)))
+        return out[:n]
+
+
+def std_shuffle(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(max(n, 1), np.uint64)
+    _chk("orc_std_shuffle", lib().orc_std_shuffle(seed, n, _p(out, C.c_uint64)))
+    return out[:n]
+
+
+def fv_message(code, m0, m1, src, head, p, q=0.0, mask0=1.0, mask1=1.0, seq=False):
     m0 = np.ascontiguousarray(m0, np.float64)
     m1 = np.ascontiguousarray(m1, np.float64)
     o0, o1 = C.c_double(), C.c_double()
-    lib().orc_fv_message(code, m0.shape[0], src, head, p, q, mask0, mask1, _p(m0, C.c_double),
-                         _p(m1, C.c_double), C.byref(o0), C.byref(o1))
+    f = lib().orc_fv_message_seq if seq else lib().orc_fv_message
+    f(code, m0.shape[0], src, head, p, q, mask0, mask1, _p(m0, C.c_double),
+      _p(m1, C.c_double), C.byref(o0), C.byref(o1))
     return o0.value, o1.value
 
 

@@ -1,5 +1,6 @@
 /*
- * oracle.cpp -- CPU FP64 restatement of the reference's PARALL CausalBP path.
+ * oracle.cpp -- CPU FP64 restatement of the reference's CausalBP path
+ * (PARALL, SEQFIX and TOPO updates).
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
@@ -15,6 +16,12 @@
  *   - run loops           csrc/ext/causal_bp.cu:824-1004
  *   - clamp + rebuild     csrc/ext/causal_fg.cpp:17-26, csrc/ext/causal_factor.cpp:63-100,
  *                         examples/fl_wrapper.cu:26-45
+ *   - SEQFIX / TOPO       csrc/ext/causal_bp.cu:186-304 (orders), 583-722 (waves),
+ *                         446-477, 868-872 (per-wave launches); csrc/kernel/
+ *                         causal_bp_seq.cu:13-23, 34-83, 113-147, 149-459
+ *     The SEQFIX order comes from std::shuffle with std::mt19937, restated
+ *     from libstdc++ 11/12 (the reference pins gxx 12.4.0,
+ *     environment.yml:74) and checked against this compiler's std::shuffle.
  * Damping is not in the reference; it is defined here as
  *   m_fv <- (1-d)*m_new + d*m_fv_old   (both states, after normalization)
  * and is skipped entirely when d == 0, so d == 0 is the reference.
@@ -24,7 +31,10 @@
  * BP on trees against exact joint enumeration; graph ordering and clamping
  * against the reference's own graph model compiled from its sources
  * (oracle/_ref).  The adjoint (orc_grad) is pinned by central finite
- * differences of orc_loss.
+ * differences of orc_loss.  The sequential schedules are pinned by tree
+ * exactness, by agreement with the PARALL fixed point on loopy graphs, by
+ * the maintained products matching the messages' products, and by wave
+ * splits worked by hand from the reference's conflict rule.
  */
 #include "oracle.h"
 
@@ -34,6 +44,8 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <queue>
+#include <random>
 #include <set>
 #include <string>
 #include <vector>
@@ -51,10 +63,29 @@ int fail(const std::string &msg) {
 }
 
 /* csrc/kernel/causal_bp_parall.cu:14-18 */
-inline void scale(double &x, double &y) {
+inline void scale_parall(double &x, double &y) {
     double m = std::fmax(x, y);
     x /= m;
     y /= m;
+}
+
+/* csrc/kernel/causal_bp_seq.cu:13-23: the sequential kernels' scale */
+inline void scale_seq(double &x, double &y) {
+    if (x < y) {
+        x = x / y;
+        y = 1.0;
+    } else if (y < x) {
+        y = y / x;
+        x = 1.0;
+    } else {
+        x = y = 1.0;
+    }
+}
+
+template <bool kSeq>
+inline void scale(double &x, double &y) {
+    if (kSeq) scale_seq(x, y);
+    else scale_parall(x, y);
 }
 
 struct VarMajorDesc {       /* Index6Real4 of causal_bp.cu:724-805 */
@@ -65,6 +96,11 @@ struct VarMajorDesc {       /* Index6Real4 of causal_bp.cu:724-805 */
 
 struct FacMajorDesc {       /* Index3 of causal_bp.cu:806-813 */
     uint64_t ind, v, src;
+};
+
+struct SeqFvItem {          /* Index7Real4 of causal_bp.cu:650-700 */
+    uint64_t ind, v;        /* var-major position written, its variable */
+    VarMajorDesc d;
 };
 
 }  // namespace
@@ -84,6 +120,13 @@ struct orc_graph {
     std::vector<uint64_t> nz0, nz1;
     std::vector<double> old_beliefs;     /* value-initialised to 0.0, causal_bp.cu:371 */
     uint64_t iters = 0;
+    /* nbV(i)[k] of var-major edge e = row_ptr_fv[i] + k: its factor and variable */
+    std::vector<uint64_t> vm_fac, vm_var;
+    /* sequential schedule (empty waves = PARALL) */
+    bool seq = false;
+    std::vector<uint64_t> seq_order;                 /* _updateSeq, var-major positions */
+    std::vector<std::vector<FacMajorDesc>> wave_vf;  /* _updateSeqVF */
+    std::vector<std::vector<SeqFvItem>> wave_fv;     /* _updateSeqFV */
 };
 
 struct orc_flist {
@@ -106,8 +149,12 @@ void factor_vars(const orc_factor_list *fl, uint64_t I, std::vector<uint64_t> &o
 }
 
 /* f->v message for one var-major edge: the body of
- * calcMessageFVFusedAllKernel, csrc/kernel/causal_bp_parall.cu:102-397.
- * vf0/vf1 point at the factor-major message arrays. */
+ * calcMessageFVFusedAllKernel, csrc/kernel/causal_bp_parall.cu:102-397
+ * (kSeq = false), or of calcMessageFVFusedKernel,
+ * csrc/kernel/causal_bp_seq.cu:160-452 (kSeq = true), which is the same
+ * arithmetic with the sequential scale().  vf0/vf1 point at the
+ * factor-major message arrays. */
+template <bool kSeq = false>
 void fv_edge(const VarMajorDesc &d, const double *vf0, const double *vf1,
              double &out0, double &out1) {
     const uint64_t src_ind = d.src, h = d.head;
@@ -137,7 +184,7 @@ void fv_edge(const VarMajorDesc &d, const double *vf0, const double *vf1,
                     double a0 = m_0 + m_1, a1 = m_1;
                     prod_1 *= a1;
                     prod_10 *= a0;
-                    scale(prod_1, prod_10);
+                    scale<kSeq>(prod_1, prod_10);
                     double delta = m_0;
                     if (a1 != 0 && a0 == a1 && delta != 0) e1 += delta / a1;
                 }
@@ -179,7 +226,7 @@ void fv_edge(const VarMajorDesc &d, const double *vf0, const double *vf1,
                     double a0 = m_0 + m_1, a1 = m_0;
                     prod_0 *= a1;
                     prod_10 *= a0;
-                    scale(prod_0, prod_10);
+                    scale<kSeq>(prod_0, prod_10);
                     double delta = m_1;
                     if (a1 != 0 && a0 == a1 && delta != 0) e1 += delta / a1;
                 }
@@ -212,14 +259,14 @@ void fv_edge(const VarMajorDesc &d, const double *vf0, const double *vf1,
             prod_1 = (p1 - p0) * (m1h * mask1 - m0h * mask0);
             prod_10 = p0 * m1h * mask1 + (1 - p0) * m0h * mask0;
         }
-        scale(prod_1, prod_10);
+        scale<kSeq>(prod_1, prod_10);
         if (prod_10 != 0) {
             for (uint64_t j = start_ind; j < end_ind; j++) {
                 if ((j != src_ind) && (j != h)) {
                     double m0 = vf0[j], m1 = vf1[j], m = m0 + m1;
                     prod_10 *= m;
                     prod_1 *= m1;
-                    scale(prod_1, prod_10);
+                    scale<kSeq>(prod_1, prod_10);
                 }
             }
         }
@@ -242,14 +289,14 @@ void fv_edge(const VarMajorDesc &d, const double *vf0, const double *vf1,
             prod_0 = (p1 - p0) * (m0h * mask0 - m1h * mask1);
             prod_10 = p0 * m0h * mask0 + (1 - p0) * m1h * mask1;
         }
-        scale(prod_0, prod_10);
+        scale<kSeq>(prod_0, prod_10);
         if (prod_10 != 0) {
             for (uint64_t j = start_ind; j < end_ind; j++) {
                 if ((j != src_ind) && (j != h)) {
                     double m0 = vf0[j], m1 = vf1[j], m = m0 + m1;
                     prod_10 *= m;
                     prod_0 *= m0;
-                    scale(prod_0, prod_10);
+                    scale<kSeq>(prod_0, prod_10);
                 }
             }
         }
@@ -276,7 +323,7 @@ void prod_var(const double *fv0, const double *fv1, uint64_t start_ind, uint64_t
         double m0 = fv0[j], m1 = fv1[j];
         if (m0 == 0) zero_0++; else prod_0 *= m0;
         if (m1 == 0) zero_1++; else prod_1 *= m1;
-        scale(prod_0, prod_1);
+        scale_parall(prod_0, prod_1);
     }
     p0_out = prod_0;
     p1_out = prod_1;
@@ -325,9 +372,14 @@ void set_threads(int threads) {
 #endif
 }
 
+double belief_step(orc_graph *g);
+double seq_sweep(orc_graph *g);
+
 /* One PARALL sweep (calcNewMessageFusedAll, causal_bp.cu:409-444) plus the
- * belief/max-diff step of run() (causal_bp.cu:874-889). */
+ * belief/max-diff step of run() (causal_bp.cu:874-889); with a sequential
+ * schedule set, one pass over its waves instead (causal_bp.cu:868-872). */
 double sweep(orc_graph *g, double damping) {
+    if (g->seq) return seq_sweep(g);
     const int64_t E = (int64_t)g->E, V = (int64_t)g->V;
     /* f -> v over var-major edges */
 #pragma omp parallel for schedule(static)
@@ -357,7 +409,15 @@ double sweep(orc_graph *g, double damping) {
         g->m_vf0[d.ind] = o0;
         g->m_vf1[d.ind] = o1;
     }
-    /* beliefs + max |old - new|, reduce init 0.0 */
+    (void)V;
+    return belief_step(g);
+}
+
+/* calcBeliefsV (causal_bp.cu:483-515) from the products and zero counts,
+ * then max |old - new| (dist_functor + maximum reduce with init 0.0,
+ * causal_bp.cu:878-889); the new beliefs become the old ones. */
+double belief_step(orc_graph *g) {
+    const int64_t V = (int64_t)g->V;
     double maxdiff = 0.0;
 #pragma omp parallel
     {
@@ -373,6 +433,263 @@ double sweep(orc_graph *g, double damping) {
         maxdiff = nanmax(maxdiff, local);
     }
     return maxdiff;
+}
+
+/* ---- sequential schedules (updates = SEQFIX / TOPO) ---------------------- */
+
+/* calcMessageVFKernel body, csrc/kernel/causal_bp_seq.cu:124-145: the
+ * PARALL v->f message without its (0, 0) guard. */
+void vf_edge_seq(double prod_0, double prod_1, uint64_t zero_cnt_0, uint64_t zero_cnt_1,
+                 double m_src_0, double m_src_1, double &out0, double &out1) {
+    if ((zero_cnt_0 >= 2) || (zero_cnt_0 == 1 && m_src_0 != 0)) {
+        prod_0 = 0.0;
+    } else if (m_src_0 != 0) {
+        prod_0 = prod_0 / m_src_0;
+    }
+    if ((zero_cnt_1 >= 2) || (zero_cnt_1 == 1 && m_src_1 != 0)) {
+        prod_1 = 0.0;
+    } else if (m_src_1 != 0) {
+        prod_1 = prod_1 / m_src_1;
+    }
+    double prod = prod_0 + prod_1;
+    out0 = prod_0 / prod;
+    out1 = prod_1 / prod;
+}
+
+/* update_message, csrc/kernel/causal_bp_seq.cu:34-83: divides the old f->v
+ * message out of the variable's product, multiplies the new one in (zeros
+ * counted, not multiplied; unsigned counts wrap as size_t does), rescales
+ * and stores the new message. */
+void update_message(orc_graph *g, uint64_t ind, uint64_t v, double marg0, double marg1) {
+    double old_msg_0 = g->m_fv0[ind], old_msg_1 = g->m_fv1[ind];
+    double prod_0 = g->prod0[v], prod_1 = g->prod1[v];
+    uint64_t zero_cnt_0 = g->nz0[v], zero_cnt_1 = g->nz1[v];
+    if (old_msg_0 == 0) zero_cnt_0--; else prod_0 /= old_msg_0;
+    if (marg0 == 0) zero_cnt_0++; else prod_0 *= marg0;
+    if (old_msg_1 == 0) zero_cnt_1--; else prod_1 /= old_msg_1;
+    if (marg1 == 0) zero_cnt_1++; else prod_1 *= marg1;
+    scale_seq(prod_0, prod_1);
+    g->prod0[v] = prod_0;
+    g->prod1[v] = prod_1;
+    g->nz0[v] = zero_cnt_0;
+    g->nz1[v] = zero_cnt_1;
+    g->m_fv0[ind] = marg0;
+    g->m_fv1[ind] = marg1;
+}
+
+/* One pass over the waves (causal_bp.cu:868-872 with calcNewMessageFused,
+ * 446-477): per wave the v->f kernel, then the fused f->v kernel, then the
+ * belief step.  Items of one kernel touch disjoint outputs and read nothing
+ * another item of the same kernel writes (that is what the wave split
+ * guarantees), so running them one after another here gives the parallel
+ * kernels' results. */
+double seq_sweep(orc_graph *g) {
+    for (size_t w = 0; w < g->wave_vf.size(); w++) {
+        for (const FacMajorDesc &d : g->wave_vf[w]) {
+            double o0, o1;
+            vf_edge_seq(g->prod0[d.v], g->prod1[d.v], g->nz0[d.v], g->nz1[d.v], g->m_fv0[d.src],
+                        g->m_fv1[d.src], o0, o1);
+            g->m_vf0[d.ind] = o0;
+            g->m_vf1[d.ind] = o1;
+        }
+        for (const SeqFvItem &it : g->wave_fv[w]) {
+            double o0, o1;
+            fv_edge<true>(it.d, g->m_vf0.data(), g->m_vf1.data(), o0, o1);
+            update_message(g, it.ind, it.v, o0, o1);
+        }
+    }
+    return belief_step(g);
+}
+
+/* libstdc++'s uniform_int_distribution<unsigned long>::operator() over
+ * [a, b] driven by std::mt19937 (a 32-bit engine): Lemire's nearly
+ * divisionless method ("Fast Random Integer Generation in an Interval",
+ * ACM TOMACS 29(1), 2019) as libstdc++ 11/12 apply it to a 32-bit
+ * generator (bits/uniform_int_dist.h).  Ranges of 2^32 or more are not
+ * needed here (factor counts are below 2^32). */
+uint64_t uniform_between(std::mt19937 &g, uint64_t a, uint64_t b) {
+    const uint64_t urange = b - a;
+    if (urange >= 0xffffffffull) return a + (uint64_t)g();   /* not reached for < 2^32 - 1 items */
+    const uint32_t range = (uint32_t)(urange + 1);
+    uint64_t product = (uint64_t)(uint32_t)g() * (uint64_t)range;
+    uint32_t low = (uint32_t)product;
+    if (low < range) {
+        const uint32_t threshold = (uint32_t)(0u - range) % range;
+        while (low < threshold) {
+            product = (uint64_t)(uint32_t)g() * (uint64_t)range;
+            low = (uint32_t)product;
+        }
+    }
+    return a + (product >> 32);
+}
+
+/* std::shuffle as libstdc++ 11/12 implement it (bits/stl_algo.h): a
+ * Fisher-Yates pass from the front; while n^2 fits the generator's range,
+ * two swap positions come from one draw over (i+1)(i+2) values. */
+void shuffle_like_libstdcxx(std::vector<uint64_t> &v, std::mt19937 &g) {
+    const uint64_t n = v.size();
+    if (n == 0) return;
+    const uint64_t urngrange = 0xffffffffull;
+    if (urngrange / n >= n) {
+        uint64_t i = 1;
+        if (n % 2 == 0) {
+            std::swap(v[i], v[uniform_between(g, 0, 1)]);
+            i++;
+        }
+        while (i != n) {
+            const uint64_t r = i + 1;
+            const uint64_t x = uniform_between(g, 0, r * (r + 1) - 1);
+            std::swap(v[i], v[x / (r + 1)]);
+            i++;
+            std::swap(v[i], v[x % (r + 1)]);
+            i++;
+        }
+        return;
+    }
+    for (uint64_t i = 1; i < n; i++) std::swap(v[i], v[uniform_between(g, 0, i)]);
+}
+
+}  // namespace
+
+struct orc_rng {
+    std::mt19937 g;
+};
+
+namespace {
+
+/* putUpdateSeqToKernelFused, csrc/ext/causal_bp.cu:583-722: walks the order
+ * and closes the current wave when the next f->v update I -> i (a) needs a
+ * v->f message j -> I (j != i) that this wave has not scheduled while this
+ * wave already updates some f->v message J -> j (J != I) -- with
+ * respect_preorder, only one ranked before I -> i -- or (b) targets a
+ * variable this wave already updates.  Each update then adds the v->f
+ * messages j -> I of its factor that the wave does not have yet. */
+int build_waves(orc_graph *g, const uint64_t *order, uint64_t n, bool respect_preorder) {
+    for (uint64_t k = 0; k < n; k++)
+        if (order[k] >= g->E) return fail("update order names an edge out of range");
+    /* _edgePreorderRank (causal_bp.cu:256-302): 0 for edges not in the
+     * order, else the last position holding the edge */
+    std::vector<uint64_t> rank(g->E, 0);
+    for (uint64_t k = 0; k < n; k++) rank[order[k]] = k;
+    g->seq_order.assign(order, order + n);
+    g->wave_vf.clear();
+    g->wave_fv.clear();
+    std::vector<FacMajorDesc> seq_vf;
+    std::vector<SeqFvItem> seq_fv;
+    std::set<std::pair<uint64_t, uint64_t>> fv_set, vf_set;
+    std::set<uint64_t> v_set;
+    for (uint64_t k = 0; k < n; k++) {
+        const uint64_t e = order[k];
+        const uint64_t i = g->vm_var[e], I = g->vm_fac[e];
+        const uint64_t dual = e - g->row_ptr_fv[i];
+        bool conflict = false;
+        for (uint64_t s = g->row_ptr_vf[I]; s < g->row_ptr_vf[I + 1] && !conflict; s++) {
+            const uint64_t j = g->edge_var[s];
+            if (j == i) continue;
+            if (vf_set.count({I, s - g->row_ptr_vf[I]})) continue;
+            for (uint64_t e2 = g->row_ptr_fv[j]; e2 < g->row_ptr_fv[j + 1]; e2++) {
+                if (g->vm_fac[e2] == I) continue;
+                if (fv_set.count({j, e2 - g->row_ptr_fv[j]}) && (!respect_preorder || rank[e2] < rank[e])) {
+                    conflict = true;
+                    break;
+                }
+            }
+        }
+        if (!conflict && v_set.count(i)) conflict = true;
+        if (conflict) {
+            g->wave_vf.push_back(seq_vf);
+            g->wave_fv.push_back(seq_fv);
+            v_set.clear();
+            fv_set.clear();
+            vf_set.clear();
+            seq_vf.clear();
+            seq_fv.clear();
+        }
+        v_set.insert(i);
+        fv_set.insert({i, dual});
+        seq_fv.push_back({e, i, g->fv_desc[e]});
+        for (uint64_t s = g->row_ptr_vf[I]; s < g->row_ptr_vf[I + 1]; s++) {
+            const uint64_t j = g->edge_var[s];
+            if (j == i) continue;
+            if (vf_set.insert({I, s - g->row_ptr_vf[I]}).second) seq_vf.push_back(g->vf_desc[s]);
+        }
+    }
+    g->wave_vf.push_back(seq_vf);
+    g->wave_fv.push_back(seq_fv);
+    g->seq = true;
+    return 0;
+}
+
+/* the var-major position of Edge(i, i.dual) for factor-major slot s */
+inline uint64_t slot_edge(const orc_graph *g, uint64_t s) { return g->vf_desc[s].src; }
+
+/* buildFixedUpdateSeq, csrc/ext/causal_bp.cu:186-201: the factor indices
+ * shuffled by the generator, then each factor's edges in nbF order. */
+void seqfix_order(const orc_graph *g, std::mt19937 &rng, std::vector<uint64_t> &out) {
+    std::vector<uint64_t> numbers(g->F);
+    for (uint64_t I = 0; I < g->F; I++) numbers[I] = I;
+    shuffle_like_libstdcxx(numbers, rng);
+    out.clear();
+    for (uint64_t I : numbers)
+        for (uint64_t s = g->row_ptr_vf[I]; s < g->row_ptr_vf[I + 1]; s++) out.push_back(slot_edge(g, s));
+}
+
+/* buildTopoUpdateSeq, csrc/ext/causal_bp.cu:203-304: breadth-first order of
+ * the undirected factor graph (variables are nodes 0..V-1, factors V..V+F-1,
+ * neighbours visited in ascending node order, roots in ascending order),
+ * then the edges sorted by (later endpoint's rank, earlier endpoint's rank,
+ * factor, variable, dual). */
+void topo_order(const orc_graph *g, std::vector<uint64_t> &out) {
+    const uint64_t V = g->V, F = g->F, nodes = V + F;
+    std::vector<std::vector<uint64_t>> adj(nodes);
+    for (uint64_t I = 0; I < F; I++) {
+        for (uint64_t s = g->row_ptr_vf[I]; s < g->row_ptr_vf[I + 1]; s++) {
+            adj[V + I].push_back(g->edge_var[s]);
+            adj[g->edge_var[s]].push_back(V + I);
+        }
+    }
+    for (auto &a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    std::vector<uint64_t> rank(nodes, nodes);
+    uint64_t next = 0;
+    std::queue<uint64_t> ready;
+    for (uint64_t root = 0; root < nodes; root++) {
+        if (rank[root] != nodes) continue;
+        ready.push(root);
+        rank[root] = next++;
+        while (!ready.empty()) {
+            const uint64_t x = ready.front();
+            ready.pop();
+            for (uint64_t y : adj[x]) {
+                if (rank[y] != nodes) continue;
+                rank[y] = next++;
+                ready.push(y);
+            }
+        }
+    }
+    struct Ranked {
+        uint64_t target, source, factor, var, dual, edge;
+    };
+    std::vector<Ranked> r;
+    r.reserve(g->E);
+    for (uint64_t I = 0; I < F; I++) {
+        for (uint64_t s = g->row_ptr_vf[I]; s < g->row_ptr_vf[I + 1]; s++) {
+            const uint64_t i = g->edge_var[s], e = slot_edge(g, s);
+            const uint64_t a = rank[V + I], b = rank[i];
+            r.push_back({std::max(a, b), std::min(a, b), I, i, e - g->row_ptr_fv[i], e});
+        }
+    }
+    std::stable_sort(r.begin(), r.end(), [](const Ranked &x, const Ranked &y) {
+        if (x.target != y.target) return x.target < y.target;
+        if (x.source != y.source) return x.source < y.source;
+        if (x.factor != y.factor) return x.factor < y.factor;
+        if (x.var != y.var) return x.var < y.var;
+        return x.dual < y.dual;
+    });
+    out.clear();
+    for (const Ranked &x : r) out.push_back(x.edge);
 }
 
 }  // namespace
@@ -430,9 +747,13 @@ int orc_build(const orc_factor_list *fl, orc_graph **out) {
         }
     }
     g->fv_desc.resize(E);
+    g->vm_fac.resize(E);
+    g->vm_var.resize(E);
     for (uint64_t i = 0; i < g->V; i++) {
         for (uint64_t k = 0; k < nbV[i].size(); k++) {
             uint64_t I = nbV[i][k].first, dual = nbV[i][k].second;
+            g->vm_fac[g->row_ptr_fv[i] + k] = I;
+            g->vm_var[g->row_ptr_fv[i] + k] = i;
             VarMajorDesc &d = g->fv_desc[g->row_ptr_fv[i] + k];
             d.src = g->row_ptr_vf[I] + dual;
             d.start = g->row_ptr_vf[I];
@@ -503,6 +824,7 @@ int orc_init(orc_graph *g, int logdomain) {
 /* CausalBP::run(), causal_bp.cu:858-910 (maxtime = infinity). */
 int orc_run(orc_graph *g, double tol, uint64_t maxiter, double damping, int threads,
             uint64_t *iters_out, double *maxdiff_out) {
+    if (g->seq && damping != 0.0) return fail("damping is defined for PARALL updates only");
     set_threads(threads);
     double maxDiff = INFINITY;
     for (; g->iters < maxiter && maxDiff > tol; g->iters++) {
@@ -516,6 +838,7 @@ int orc_run(orc_graph *g, double tol, uint64_t maxiter, double damping, int thre
 /* CausalBP::run(tol, maxIters, histLength), causal_bp.cu:912-1004 for the
  * PARALL schedule: exactly maxIters sweeps, then beliefs once. */
 int orc_run_fixed(orc_graph *g, uint64_t n_iters, double damping, int threads) {
+    if (g->seq && damping != 0.0) return fail("damping is defined for PARALL updates only");
     set_threads(threads);
     for (uint64_t k = 0; k < n_iters; k++, g->iters++) {
         (void)sweep(g, damping);
@@ -524,6 +847,7 @@ int orc_run_fixed(orc_graph *g, uint64_t n_iters, double damping, int threads) {
 }
 
 int orc_sweep(orc_graph *g, double damping, int threads, double *maxdiff_out) {
+    if (g->seq && damping != 0.0) return fail("damping is defined for PARALL updates only");
     set_threads(threads);
     double d = sweep(g, damping);
     if (maxdiff_out) *maxdiff_out = d;
@@ -552,6 +876,128 @@ int orc_set_messages(orc_graph *g, const double *fv0, const double *fv1, const d
     return 0;
 }
 
+
+/* ---- sequential schedules ------------------------------------------------ */
+
+orc_rng *orc_rng_new(uint32_t seed) {
+    orc_rng *r = new orc_rng();
+    r->g.seed(seed);
+    return r;
+}
+void orc_rng_free(orc_rng *r) { delete r; }
+uint32_t orc_rng_next(orc_rng *r) { return (uint32_t)r->g(); }
+
+int orc_shuffle(orc_rng *r, uint64_t n, uint64_t *out) {
+    std::vector<uint64_t> v(n);
+    for (uint64_t k = 0; k < n; k++) v[k] = k;
+    shuffle_like_libstdcxx(v, r->g);
+    std::copy(v.begin(), v.end(), out);
+    return 0;
+}
+
+/* std::shuffle itself (this compiler's libstdc++) on the vector<int> the
+ * reference shuffles, for checking the restatement above */
+int orc_std_shuffle(uint32_t seed, uint64_t n, uint64_t *out) {
+    std::mt19937 g(seed);
+    std::vector<int> v(n);
+    for (uint64_t k = 0; k < n; k++) v[k] = (int)k;
+    std::shuffle(v.begin(), v.end(), g);
+    for (uint64_t k = 0; k < n; k++) out[k] = (uint64_t)v[k];
+    return 0;
+}
+
+int orc_order_seqfix(const orc_graph *g, orc_rng *r, uint64_t *out) {
+    std::vector<uint64_t> o;
+    seqfix_order(g, r->g, o);
+    std::copy(o.begin(), o.end(), out);
+    return 0;
+}
+
+int orc_order_topo(const orc_graph *g, uint64_t *out) {
+    std::vector<uint64_t> o;
+    topo_order(g, o);
+    std::copy(o.begin(), o.end(), out);
+    return 0;
+}
+
+int orc_seq_set(orc_graph *g, const uint64_t *order, uint64_t n, int respect_preorder) {
+    if (n && !order) return fail("null order");
+    return build_waves(g, order, n, respect_preorder != 0);
+}
+
+int orc_seq_clear(orc_graph *g) {
+    g->seq = false;
+    g->seq_order.clear();
+    g->wave_vf.clear();
+    g->wave_fv.clear();
+    return 0;
+}
+
+int orc_seq_info(const orc_graph *g, uint64_t *n_waves, uint64_t *n_fv, uint64_t *n_vf) {
+    uint64_t a = 0, b = 0;
+    for (const auto &w : g->wave_fv) a += w.size();
+    for (const auto &w : g->wave_vf) b += w.size();
+    if (n_waves) *n_waves = g->wave_fv.size();
+    if (n_fv) *n_fv = a;
+    if (n_vf) *n_vf = b;
+    return 0;
+}
+
+int orc_seq_waves(const orc_graph *g, uint64_t *fv_off, uint64_t *fv_edge, uint64_t *vf_off,
+                  uint64_t *vf_slot) {
+    uint64_t a = 0, b = 0;
+    for (size_t w = 0; w < g->wave_fv.size(); w++) {
+        if (fv_off) fv_off[w] = a;
+        if (vf_off) vf_off[w] = b;
+        for (const SeqFvItem &it : g->wave_fv[w]) {
+            if (fv_edge) fv_edge[a] = it.ind;
+            a++;
+        }
+        for (const FacMajorDesc &d : g->wave_vf[w]) {
+            if (vf_slot) vf_slot[b] = d.ind;
+            b++;
+        }
+    }
+    if (fv_off) fv_off[g->wave_fv.size()] = a;
+    if (vf_off) vf_off[g->wave_vf.size()] = b;
+    return 0;
+}
+
+/* Checker of a dependency-level split (not a restatement of the
+ * reference): one sweep that runs the listed items one after another --
+ * kind 0: the v->f message of factor-major slot item[x]; kind 1: the f->v
+ * update of var-major edge item[x] -- then the belief step.  Listed level
+ * by level in any order within a level, the items must give the wave
+ * execution's bits. */
+int orc_seq_sweep_items(orc_graph *g, uint64_t n, const uint8_t *kind, const uint64_t *item, double *maxdiff_out) {
+    for (uint64_t x = 0; x < n; x++) {
+        if (item[x] >= g->E) return fail("item out of range");
+        double o0, o1;
+        if (kind[x] == 0) {
+            const FacMajorDesc &d = g->vf_desc[item[x]];
+            vf_edge_seq(g->prod0[d.v], g->prod1[d.v], g->nz0[d.v], g->nz1[d.v], g->m_fv0[d.src],
+                        g->m_fv1[d.src], o0, o1);
+            g->m_vf0[d.ind] = o0;
+            g->m_vf1[d.ind] = o1;
+        } else {
+            const uint64_t e = item[x];
+            fv_edge<true>(g->fv_desc[e], g->m_vf0.data(), g->m_vf1.data(), o0, o1);
+            update_message(g, e, g->vm_var[e], o0, o1);
+        }
+    }
+    const double d = belief_step(g);
+    if (maxdiff_out) *maxdiff_out = d;
+    return 0;
+}
+
+int orc_products(const orc_graph *g, double *prod0, double *prod1, uint64_t *nz0, uint64_t *nz1) {
+    if (prod0) std::copy(g->prod0.begin(), g->prod0.end(), prod0);
+    if (prod1) std::copy(g->prod1.begin(), g->prod1.end(), prod1);
+    if (nz0) std::copy(g->nz0.begin(), g->nz0.end(), nz0);
+    if (nz1) std::copy(g->nz1.begin(), g->nz1.end(), nz1);
+    return 0;
+}
+
 void orc_fv_message(int code, uint64_t deg, uint64_t src, uint64_t head, double p, double q,
                     double mask0, double mask1, const double *m0, const double *m1,
                     double *out0, double *out1) {
@@ -566,6 +1012,22 @@ void orc_fv_message(int code, uint64_t deg, uint64_t src, uint64_t head, double 
     d.mask0 = mask0;
     d.mask1 = mask1;
     fv_edge(d, m0, m1, *out0, *out1);
+}
+
+void orc_fv_message_seq(int code, uint64_t deg, uint64_t src, uint64_t head, double p, double q,
+                        double mask0, double mask1, const double *m0, const double *m1,
+                        double *out0, double *out1) {
+    VarMajorDesc d;
+    d.code = code;
+    d.src = src;
+    d.start = 0;
+    d.end = deg;
+    d.head = head;
+    d.q = q;
+    d.p = p;
+    d.mask0 = mask0;
+    d.mask1 = mask1;
+    fv_edge<true>(d, m0, m1, *out0, *out1);
 }
 
 void orc_vf_message(uint64_t deg, uint64_t src, const double *m0, const double *m1,

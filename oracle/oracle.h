@@ -1,5 +1,6 @@
 /*
- * oracle.h -- CPU FP64 restatement of the reference's PARALL CausalBP path.
+ * oracle.h -- CPU FP64 restatement of the reference's CausalBP path (PARALL,
+ * SEQFIX and TOPO updates).
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
  * engine: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
@@ -114,6 +115,41 @@ int orc_loss(const orc_factor_list *fl, const int32_t *rule_of_factor,
              uint64_t n_rules, const double *theta, uint64_t n_alarms,
              const uint64_t *alarm_labels, const double *targets, uint64_t T,
              double damping, double *loss_out);
+
+/* ---- sequential schedules (updates = SEQFIX / TOPO) ----
+ * Orders are lists of var-major edge positions (Edge(i, dual) ->
+ * h_row_ptr_fv[i] + dual).  With a schedule set, orc_run / orc_run_fixed /
+ * orc_sweep run its waves instead of the PARALL sweep; damping must be 0. */
+typedef struct orc_rng orc_rng;          /* a std::mt19937 */
+orc_rng *orc_rng_new(uint32_t seed);
+void orc_rng_free(orc_rng *r);
+uint32_t orc_rng_next(orc_rng *r);
+/* libstdc++'s std::shuffle of 0..n-1, restated; and std::shuffle itself */
+int orc_shuffle(orc_rng *r, uint64_t n, uint64_t *out);
+int orc_std_shuffle(uint32_t seed, uint64_t n, uint64_t *out);
+/* buildFixedUpdateSeq (advances r) / buildTopoUpdateSeq: E positions */
+int orc_order_seqfix(const orc_graph *g, orc_rng *r, uint64_t *out);
+int orc_order_topo(const orc_graph *g, uint64_t *out);
+/* putUpdateSeqToKernelFused: split an order into waves */
+int orc_seq_set(orc_graph *g, const uint64_t *order, uint64_t n, int respect_preorder);
+int orc_seq_clear(orc_graph *g);
+int orc_seq_info(const orc_graph *g, uint64_t *n_waves, uint64_t *n_fv, uint64_t *n_vf);
+/* fv_off/vf_off [n_waves+1]; fv_edge [n_fv] var-major positions updated;
+ * vf_slot [n_vf] factor-major slots recomputed */
+int orc_seq_waves(const orc_graph *g, uint64_t *fv_off, uint64_t *fv_edge, uint64_t *vf_off,
+                  uint64_t *vf_slot);
+/* Checker of a dependency-level split (not a restatement): one sweep of the
+ * listed items in the listed order (kind 0: v->f of factor-major slot
+ * item[x]; kind 1: f->v update of var-major edge item[x]), then beliefs. */
+int orc_seq_sweep_items(orc_graph *g, uint64_t n, const uint8_t *kind, const uint64_t *item,
+                        double *maxdiff_out);
+/* per-variable products and zero counts (maintained in place by SEQ sweeps) */
+int orc_products(const orc_graph *g, double *prod0, double *prod1, uint64_t *nz0, uint64_t *nz1);
+/* one f->v message with the sequential kernels' scale() */
+void orc_fv_message_seq(int code, uint64_t deg, uint64_t src, uint64_t head,
+                        double p, double q, double mask0, double mask1,
+                        const double *m0, const double *m1, double *out0,
+                        double *out1);
 
 const char *orc_last_error(void);
 
