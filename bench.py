@@ -16,6 +16,7 @@ bounded number of sweeps of the same graph on this host's cores).
     python bench.py --workload batched   # 512 graphs + 256 clamped variants vs sequential solves
     python bench.py --workload adjoint   # 100k-variable tied-rule gradient, backward vs forward
     python bench.py --workload c5 --steps 20   # tied-rule fit over 64 clamped variants of a 20M graph
+    python bench.py --workload seq [--updates TOPO]   # SEQFIX / TOPO solve of the 1M graph vs PARALL
 """
 from __future__ import annotations
 
@@ -294,6 +295,136 @@ def bench_c5(args):
     print(json.dumps(out), flush=True)
 
 
+def seq_level_bytes(graph, sched):
+    """Algorithmic HBM bytes of the level kernels of one sequential sweep.
+
+    v->f item: item 16, f->v message 16, product 16, zero counts 16, write 16.
+    f->v item: item 16, factor offsets 8, meta 4, p 8, q 8, the factor's d
+               v->f messages 16 d, old message 16, product + counts 32, and
+               writes of product, counts and message 48.
+    beliefs:   product + counts 32, old belief 8, new belief 8 per variable.
+    """
+    import numpy as np
+    fo, fe, vo, vs = sched.waves()
+    deg = np.diff(graph.fac_off.astype(np.int64))
+    slot_fac = np.repeat(np.arange(graph.n_factors), deg)
+    d = deg[slot_fac[graph.var_edge[fe.astype(np.int64)].astype(np.int64)]]
+    level = 80 * len(vs) + int(np.sum(140 + 16 * d))
+    return level, 48 * graph.n_vars
+
+
+def bench_seq(args):
+    """SEQFIX / TOPO: the reference's sequential schedule on the 1M graph,
+    run as dependency levels; beside it the PARALL solve of the same graph
+    at the same tolerance (undamped: the sequential schedules have no
+    damping)."""
+    import torch
+    import fastlbp_amd as fl
+
+    n_vars = args.n_vars or WORKLOAD["n_vars"]
+    tol, maxiter = WORKLOAD["tol"], WORKLOAD["maxiter"]
+    gen = fl.generate(n_vars, WORKLOAD["seed"])
+    graph = fl.Graph(gen.factors)
+    V, F, E = graph.n_vars, graph.n_factors, graph.n_edges
+    t0 = time.perf_counter()
+    fl.seed(42)
+    sched = fl.Schedule(graph, args.updates)
+    t_sched = time.perf_counter() - t0
+    info = sched.info()
+    log(f"[bench] {args.updates}: V={V} E={E} waves {info.n_waves} levels {info.n_levels} "
+        f"f->v items {info.n_fv_items} v->f items {info.n_vf_items} (schedule built in {t_sched:.2f}s)")
+    plan = fl.Plan(graph, device=0)
+    plan.set_schedule(sched)
+    plan.use_graphs(args.graphs)
+
+    def solve():
+        plan.reset()
+        plan.init()
+        return plan.run(tol, maxiter)
+
+    for i in range(args.warmup):
+        r = solve()
+        log(f"[bench] warmup {i}: {r.sweeps} sweeps {r.seconds * 1e3:.2f} ms maxdiff {r.maxdiff:.3g}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sweeps = 0
+    for i in range(args.steps):
+        r = solve()
+        sweeps += r.sweeps
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    per_sweep = info.n_fv_items + info.n_vf_items
+    value = per_sweep * sweeps / dt
+    ms_solve = dt * 1e3 / args.steps
+    log(f"[bench] {args.steps} solves, {sweeps / args.steps:.1f} sweeps each, {ms_solve:.3f} ms/solve, "
+        f"{value:.4g} msg-updates/s, {dt * 1e6 / max(sweeps, 1):.1f} us/sweep")
+    plan.profile(True)
+    lvl_ms = bel_ms = 0.0
+    n_sw = 0
+    for i in range(3):
+        solve()
+        a, b, na, nb = plan.last_timing()
+        lvl_ms += a; bel_ms += b; n_sw += na
+    plan.profile(False)
+    lb, bb = seq_level_bytes(graph, sched)
+    lvl_us = lvl_ms * 1e3 / max(n_sw, 1)
+    bel_us = bel_ms * 1e3 / max(n_sw, 1)
+    achieved = lb / (lvl_us * 1e-6) / 1e9
+    log(f"[bench] levels {lvl_us:.1f} us/sweep ({info.n_levels} launches, {achieved:.0f} GB/s), "
+        f"beliefs {bel_us:.1f} us/sweep")
+    # PARALL on the same graph, same tolerance, undamped and damped
+    par = {}
+    for damping in (0.0, WORKLOAD["damping"]):
+        pp = fl.Plan(graph, device=0)
+        pp.init()
+        r = pp.run(tol, maxiter, damping)
+        t0 = time.perf_counter()
+        for i in range(3):
+            pp.reset()
+            pp.init()
+            r = pp.run(tol, maxiter, damping)
+        torch.cuda.synchronize()
+        par[str(damping)] = {"sweeps": r.sweeps, "maxdiff": r.maxdiff,
+                             "ms_per_solve": (time.perf_counter() - t0) * 1e3 / 3}
+        pp.close()
+        log(f"[bench] PARALL damping {damping}: {r.sweeps} sweeps, maxdiff {r.maxdiff:.3g}, "
+            f"{par[str(damping)]['ms_per_solve']:.2f} ms/solve")
+    out = {"metric": METRIC, "value": value, "unit": "msg-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_solve,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"{args.updates.lower()}-fp64-{n_vars // 1000}k", "n_vars": V, "n_factors": F,
+                      "n_edges": E, "tol": tol, "maxiter": maxiter, "seed": WORKLOAD["seed"],
+                      "updates": args.updates, "waves": info.n_waves, "levels": info.n_levels,
+                      "fv_items_per_sweep": info.n_fv_items, "vf_items_per_sweep": info.n_vf_items,
+                      "sweeps_per_solve": sweeps / args.steps, "last_maxdiff": r.maxdiff,
+                      "graphs": bool(args.graphs),
+                      "step": "init + run() to convergence under the sequential schedule"},
+           "roofline": {"bound": "hbm", "kernel": "seq_level (all levels of a sweep)",
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes_per_sweep": lb, "level_us_per_sweep": round(lvl_us, 2),
+                        "belief_us_per_sweep": round(bel_us, 2),
+                        "timing": "HIP events around the level launches of each sweep in profiled solves"},
+           "parall_same_graph": par, "cpu_baseline": None}
+    if not args.no_cpu:
+        import oracle
+        o = oracle.OracleGraph(gen.factors)
+        o.set_sequence(sched.order(), respect_preorder=info.respect_preorder)
+        o.init()
+        o.sweep()
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            o.sweep()
+            n += 1
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": per_sweep * n / cdt, "unit": "msg-updates/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} {args.updates} sweeps (wave by wave, one thread) of the same "
+                                         f"{V}-variable graph, {cdt:.1f} s"}
+        log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.4g} msg-updates/s (1 thread)")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,7 +432,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["plain", "batched", "adjoint", "c5"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "batched", "adjoint", "c5", "seq"], default="plain")
+    ap.add_argument("--updates", choices=["SEQFIX", "TOPO"], default="SEQFIX", help="seq: the schedule")
+    ap.add_argument("--graphs", action="store_true", help="replay chunks of sweeps from captured hipGraphs")
     ap.add_argument("--c5-variants", type=int, default=64, help="c5: clamped variants per gradient step")
     ap.add_argument("--c5-sweeps", type=int, default=20, help="c5: sweeps T of the differentiated solve")
     ap.add_argument("--c5-checkpoint", type=int, default=5, help="c5: checkpoint interval of the adjoint")
@@ -323,6 +456,8 @@ def main():
         return bench_adjoint(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "seq":
+        return bench_seq(args)
 
     import torch
     import fastlbp_amd as fl

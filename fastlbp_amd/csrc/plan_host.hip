@@ -166,7 +166,7 @@ int Plan::alloc(T *&p, uint64_t n) {
 
 /* element types allocated from other translation units */
 template int Plan::alloc<double2>(double2 *&, uint64_t);
-template int Plan::alloc<ulonglong2>(ulonglong2 *&, uint64_t);
+template int Plan::alloc<dev::VarState>(dev::VarState *&, uint64_t);
 
 #define TRY(expr)          \
     do {                   \
@@ -251,6 +251,7 @@ int Plan::upload(const flbp_csr &c) {
     HIP_TRY(hipMemset(m_fv, 0, std::max<uint64_t>(E, 1) * 16));
     HIP_TRY(hipMemset(m_fv_vm, 0, std::max<uint64_t>(E, 1) * 16));
     h_var_edge.assign(c.var_edge, c.var_edge + E);
+    h_fac_off.assign(c.fac_off, c.fac_off + F + 1);
     for (uint64_t I = 0; I < F; I++) max_fac_deg = std::max<uint64_t>(max_fac_deg, c.fac_off[I + 1] - c.fac_off[I]);
     for (uint64_t i = 0; i < V; i++) max_var_deg = std::max<uint64_t>(max_var_deg, c.var_off[i + 1] - c.var_off[i]);
     h_meta0 = meta;

@@ -13,6 +13,15 @@
 
 namespace flbp {
 
+namespace dev {
+/* prod_fv_0/1 and num_zeros_fv_0/1 of one variable under a sequential
+ * schedule (seq_kernels.h) */
+struct __align__(32) VarState {
+    double p0, p1;
+    unsigned long long z0, z1;
+};
+}  // namespace dev
+
 constexpr int kMaxChunk = 64;   /* sweeps enqueued between host checks */
 constexpr uint32_t kStageItems = 256;    /* factors / variables per staged block (== kBlock) */
 constexpr uint32_t kStageSlots = 1024;   /* edge slots per staged block (== kStage) */
@@ -85,10 +94,13 @@ struct Plan {
     bool seq = false;
     int seq_updates = -1;                          /* FLBP_UPDATES_* or -1 (explicit order) */
     uint4 *seq_vf = nullptr, *seq_fv = nullptr;    /* items grouped by level */
-    double2 *seq_prod = nullptr;                   /* per-variable products, maintained in place */
-    ulonglong2 *seq_nz = nullptr;                  /* per-variable zero counts */
+    uint2 *seq_fv_dt = nullptr;                    /* f->v items: factor degree, target slot */
+    dev::VarState *seq_state = nullptr;            /* per-variable products and zero counts */
     std::vector<uint32_t> seq_lvl_vf, seq_lvl_fv;  /* [levels+1] item offsets per level */
+    uint32_t *seq_lvl_vf_dev = nullptr, *seq_lvl_fv_dev = nullptr;
+    size_t seq_tail_from = 0;                      /* levels from here run in the tail kernel */
     uint64_t seq_waves = 0;
+    std::vector<uint64_t> h_fac_off;               /* host copy of fac_off */
     int launch_seq_sweep(int k, double tol);
     void free_seq_items();
 

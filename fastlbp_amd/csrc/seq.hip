@@ -5,13 +5,17 @@
  *
  * Device state added to the plan (variant-3 message orders, which are the
  * reference's: v->f factor-major m_vf, f->v var-major m_fv_vm):
- *   seq_prod f64x2[V]  products (prod_fv_0, prod_fv_1), maintained in place
- *   seq_nz   u64x2[V]  zero counts (num_zeros_fv_0/1)
- *   seq_vf   u32x4[n_vf]  v->f items (slot, var-major position, variable)
- *   seq_fv   u32x4[n_fv]  f->v items (var-major position, slot, factor, variable)
+ *   seq_state  {f64 p0, p1; u64 z0, z1}[V]  products and zero counts
+ *              (prod_fv_0/1, num_zeros_fv_0/1), maintained in place
+ *   seq_vf     u32x4[n_vf]  v->f items (slot, var-major position, variable, 0)
+ *   seq_fv     u32x4[n_fv]  f->v items (var-major position, variable, factor,
+ *              first slot of the factor)
+ *   seq_fv_dt  u32x2[n_fv]  (factor degree, target slot in the factor)
+ *   seq_lvl_*_dev u32[levels+1]  first item of each level
  * Items are grouped by level; within a level v->f items are ordered by
  * slot and f->v items by factor, so neighbouring lanes touch neighbouring
- * messages.
+ * messages.  The trailing levels of at most kSeqTailItems items each run
+ * in one single-workgroup launch (seq_tail).
  *
  * Replaces the reference's per-wave launches (csrc/ext/causal_bp.cu:446-477,
  * 868-872; the CUDA-graph replay of 82-112, 1028-1035) -- two kernels per
@@ -32,13 +36,14 @@
 
 namespace {
 
-__global__ void fill_state(double2 *prod, ulonglong2 *nz, uint64_t n, double c) {
+__global__ void fill_state(flbp::dev::VarState *st, uint64_t n, double c) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        prod[i] = make_double2(c, c);
-        nz[i] = make_ulonglong2(0ull, 0ull);
-    }
+    if (i < n) st[i] = flbp::dev::VarState{c, c, 0ull, 0ull};
 }
+
+/* trailing levels up to this many items each go to the single-workgroup
+ * tail kernel */
+constexpr uint32_t kSeqTailItems = flbp::dev::kSeqTailPer * flbp::dev::kSeqTailBlock;
 
 }  // namespace
 
@@ -51,29 +56,41 @@ namespace flbp {
     } while (0)
 
 void Plan::free_seq_items() {
-    if (seq_vf) (void)hipFree(seq_vf);
-    if (seq_fv) (void)hipFree(seq_fv);
+    for (void *p : {(void *)seq_vf, (void *)seq_fv, (void *)seq_fv_dt, (void *)seq_lvl_vf_dev,
+                    (void *)seq_lvl_fv_dev})
+        if (p) (void)hipFree(p);
     seq_vf = seq_fv = nullptr;
+    seq_fv_dt = nullptr;
+    seq_lvl_vf_dev = seq_lvl_fv_dev = nullptr;
     seq_lvl_vf.clear();
     seq_lvl_fv.clear();
+    seq_tail_from = 0;
 }
 
 /* One sweep: every level, then beliefs and the max-diff of sweep k. */
 int Plan::launch_seq_sweep(int k, double tol) {
     const size_t L = seq_lvl_fv.empty() ? 0 : seq_lvl_fv.size() - 1;
-    for (size_t l = 0; l < L; l++) {
+    SeqArgs A{seq_vf, seq_fv, seq_fv_dt, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv_vm, seq_state};
+    for (size_t l = 0; l < seq_tail_from; l++) {
         const uint32_t v0 = seq_lvl_vf[l], nv = seq_lvl_vf[l + 1] - v0;
         const uint32_t f0 = seq_lvl_fv[l], nf = seq_lvl_fv[l + 1] - f0;
         if (!nv && !nf) continue;
         const unsigned vb = grid_for(nv, dev::kBlock), fb = grid_for(nf, dev::kBlock);
-        hipLaunchKernelGGL(seq_level, dim3(vb + fb), dim3(dev::kBlock), 0, stream, seq_vf + v0, nv, seq_fv + f0, nf,
-                           vb, fac_off, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv_vm, seq_prod, seq_nz, ctl, k,
-                           tol);
+        SeqArgs a = A;
+        a.vf_items = seq_vf + v0;
+        a.fv_items = seq_fv + f0;
+        a.fv_dt = seq_fv_dt + f0;
+        hipLaunchKernelGGL(seq_level, dim3(vb + fb), dim3(dev::kBlock), 0, stream, a, nv, nf, vb, ctl, k, tol);
+    }
+    for (size_t l0 = seq_tail_from; l0 < L; l0 += dev::kSeqTailLevels) {
+        const size_t l1 = std::min(L, l0 + dev::kSeqTailLevels);
+        hipLaunchKernelGGL(seq_tail, dim3(1), dim3(dev::kSeqTailBlock), 0, stream, A, seq_lvl_vf_dev,
+                           seq_lvl_fv_dev, (uint32_t)l0, (uint32_t)l1, ctl, k, tol);
     }
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
     if (V)
-        hipLaunchKernelGGL(seq_beliefs, dim3(grid_for(V, dev::kBlock)), dim3(dev::kBlock), 0, stream, seq_prod,
-                           seq_nz, belief, (uint32_t)V, ctl, k, tol);
+        hipLaunchKernelGGL(seq_beliefs, dim3(grid_for(V, dev::kBlock)), dim3(dev::kBlock), 0, stream, seq_state,
+                           belief, (uint32_t)V, ctl, k, tol);
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 2], stream));
     HIP_TRY(hipGetLastError());
     return FLBP_OK;
@@ -116,40 +133,61 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
         lv[l + 1] += lv[l];
         lf[l + 1] += lf[l];
     }
-    std::vector<uint4> vf(S.vf_slot.size()), fv(S.fv_edge.size());
+    /* f->v items as (key = factor, edge; payload = position in wave order) */
+    std::vector<uint4> vf(S.vf_slot.size());
+    std::vector<std::pair<uint64_t, uint32_t>> fk(S.fv_edge.size());
     {
         std::vector<uint32_t> at(lv.begin(), lv.end());
         for (size_t x = 0; x < S.vf_slot.size(); x++)
             vf[at[S.vf_level[x]]++] = make_uint4(S.vf_slot[x], S.vf_pos[x], S.vf_var[x], 0u);
         std::vector<uint32_t> bt(lf.begin(), lf.end());
         for (size_t x = 0; x < S.fv_edge.size(); x++)
-            fv[bt[S.fv_level[x]]++] = make_uint4(S.fv_edge[x], S.fv_slot[x], S.fv_fac[x], S.fv_var[x]);
+            fk[bt[S.fv_level[x]]++] = {((uint64_t)S.fv_fac[x] << 32) | S.fv_edge[x], (uint32_t)x};
     }
     for (uint32_t l = 1; l <= nl; l++) {
         std::sort(vf.begin() + lv[l], vf.begin() + lv[l + 1], [](const uint4 &a, const uint4 &b) { return a.x < b.x; });
-        std::sort(fv.begin() + lf[l], fv.begin() + lf[l + 1], [](const uint4 &a, const uint4 &b) {
-            return a.z != b.z ? a.z < b.z : a.x < b.x;
-        });
+        std::sort(fk.begin() + lf[l], fk.begin() + lf[l + 1]);
+    }
+    std::vector<uint4> fv(fk.size());
+    std::vector<uint2> fdt(fk.size());
+    for (size_t y = 0; y < fk.size(); y++) {
+        const uint32_t x = fk[y].second, I = S.fv_fac[x];
+        const uint32_t a = (uint32_t)P->h_fac_off[I], d = (uint32_t)(P->h_fac_off[I + 1] - a);
+        fv[y] = make_uint4(S.fv_edge[x], S.fv_var[x], I, a);
+        fdt[y] = make_uint2(d, S.fv_slot[x] - a);
     }
     P->free_seq_items();
     const size_t nvf = std::max<size_t>(vf.size(), 1), nfv = std::max<size_t>(fv.size(), 1);
     if (hipMalloc(&P->seq_vf, nvf * sizeof(uint4)) != hipSuccess ||
-        hipMalloc(&P->seq_fv, nfv * sizeof(uint4)) != hipSuccess) {
+        hipMalloc(&P->seq_fv, nfv * sizeof(uint4)) != hipSuccess ||
+        hipMalloc(&P->seq_fv_dt, nfv * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&P->seq_lvl_vf_dev, (nl + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&P->seq_lvl_fv_dev, (nl + 1) * sizeof(uint32_t)) != hipSuccess) {
         (void)hipGetLastError();
         P->free_seq_items();
         return flbp::fail(FLBP_ENOMEM, "flbp_plan_set_schedule: item upload allocation failed");
     }
     if (!vf.empty()) HIP_TRY(hipMemcpy(P->seq_vf, vf.data(), vf.size() * sizeof(uint4), hipMemcpyHostToDevice));
-    if (!fv.empty()) HIP_TRY(hipMemcpy(P->seq_fv, fv.data(), fv.size() * sizeof(uint4), hipMemcpyHostToDevice));
+    if (!fv.empty()) {
+        HIP_TRY(hipMemcpy(P->seq_fv, fv.data(), fv.size() * sizeof(uint4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->seq_fv_dt, fdt.data(), fdt.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    }
     /* level l (1-based) -> index l-1 */
     P->seq_lvl_vf.assign(lv.begin() + 1, lv.end());
     P->seq_lvl_fv.assign(lf.begin() + 1, lf.end());
-    if (!P->seq_prod) {
-        if (int rc = P->alloc(P->seq_prod, P->V)) return rc;
-        if (int rc = P->alloc(P->seq_nz, P->V)) return rc;
+    HIP_TRY(hipMemcpy(P->seq_lvl_vf_dev, P->seq_lvl_vf.data(), (nl + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->seq_lvl_fv_dev, P->seq_lvl_fv.data(), (nl + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    /* the tail: the longest run of trailing levels with few items each */
+    size_t tail = nl;
+    while (tail > 0 && (P->seq_lvl_vf[tail] - P->seq_lvl_vf[tail - 1]) + (P->seq_lvl_fv[tail] - P->seq_lvl_fv[tail - 1]) <=
+                           kSeqTailItems)
+        tail--;
+    P->seq_tail_from = tail;
+    if (!P->seq_state) {
+        if (int rc = P->alloc(P->seq_state, P->V)) return rc;
     }
     if (P->V)
-        hipLaunchKernelGGL(fill_state, dim3(grid_for(P->V, 256)), dim3(256), 0, P->stream, P->seq_prod, P->seq_nz, P->V,
+        hipLaunchKernelGGL(fill_state, dim3(grid_for(P->V, 256)), dim3(256), 0, P->stream, P->seq_state, P->V,
                            P->init_value);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(P->stream));
@@ -171,8 +209,8 @@ namespace flbp {
 
 /* products back to the values init() leaves: (c, c), no zeros */
 int seq_reset_state(Plan *P, double c) {
-    if (!P->seq_prod || !P->V) return FLBP_OK;
-    hipLaunchKernelGGL(fill_state, dim3(grid_for(P->V, 256)), dim3(256), 0, P->stream, P->seq_prod, P->seq_nz, P->V, c);
+    if (!P->seq_state || !P->V) return FLBP_OK;
+    hipLaunchKernelGGL(fill_state, dim3(grid_for(P->V, 256)), dim3(256), 0, P->stream, P->seq_state, P->V, c);
     HIP_TRY(hipGetLastError());
     return FLBP_OK;
 }

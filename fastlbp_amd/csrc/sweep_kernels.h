@@ -63,17 +63,14 @@ __device__ __forceinline__ void scale_pair(double &x, double &y) {
 
 /* scale() of the sequential kernels, csrc/kernel/causal_bp_seq.cu:13-23:
  * the smaller value over the larger, the larger set to 1; equal (or NaN)
- * values both become 1 */
+ * values both become 1.  Written with selects (the one quotient taken is
+ * the reference's, smaller / larger); branches that assign x or y made the
+ * compiler pick the target through a stack address. */
 __device__ __forceinline__ void scale_seq(double &x, double &y) {
-    if (x < y) {
-        x = x / y;
-        y = 1.0;
-    } else if (y < x) {
-        y = y / x;
-        x = 1.0;
-    } else {
-        x = y = 1.0;
-    }
+    const bool lt = x < y, gt = y < x;
+    const double r = (lt ? x : y) / (lt ? y : x);
+    x = lt ? r : 1.0;
+    y = gt ? r : 1.0;
 }
 
 template <bool kSeq>
