@@ -1,0 +1,147 @@
+/*
+ * flbp_bingo -- the Bingo inference server protocol (the reference's
+ * examples/bingo_wrapper.cu:25-195) on the MI355X engine, through the
+ * drop-in CausalFactorGraph / CausalBP classes.
+ *
+ * usage: flbp_bingo <file.causal_fg> [seed] [--auto | --manual id] [--digits N]
+ *
+ * Commands, whitespace-separated tokens on stdin:
+ *   O <var index> true|false                 clamp; echoes "O <index> <value>"
+ *   UC <var index>                           drop a clamp; echoes "UC <index>"
+ *   BP <tol> <minIters> <maxIters> <histLen> a new CausalBP on the clamped
+ *                                            graph (updates SEQFIX), init(),
+ *                                            run(tol, minIters, maxIters, histLen);
+ *                                            prints the yet-to-converge fraction
+ *   Q <var index>                            prints newBelief(index) of the last BP
+ *   NL                                       prints an empty line
+ *   BYE                                      ends the session
+ * Variable indices are label ranks (CausalFactorGraph's var order).  Logs
+ * go to stderr.
+ *
+ * As in the reference: seed reseeds the thread's rnd_gen before anything
+ * else (each BP draws the next SEQFIX shuffle from it); clamps are applied
+ * to a copy of the graph in ascending index order; properties maxiter
+ * 10000000, maxtime 57600 s, tol 1e-6, updates SEQFIX; numbers print with
+ * the stream's default 6 significant digits (--digits changes that, not in
+ * the reference).  Not as in the reference: the clamped copy is rebuilt
+ * from its factors before inference.  That is the same graph whenever the
+ * clamped variables appear in no factor body; a body clamp erases the
+ * variable from the factor, and the reference's un-rebuilt copy then no
+ * longer matches its adjacency (its construct writes past its buffers).
+ * --auto takes device 0 (one process per GPU); --manual id takes device id.
+ */
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <string>
+
+#include <ext/causal_bp.h>
+#include <utils/utils.h>
+
+using namespace lbp;
+
+namespace {
+
+CausalFactorGraph g_fg;
+PropertySet g_opts;
+std::unique_ptr<CausalBP> g_bp;
+std::map<int, bool> g_clamps;
+
+void init_bp() {
+    CausalFactorGraph clamped(g_fg);
+    for (const auto &c : g_clamps) clamped.clamp((size_t)c.first, c.second ? 1 : 0);
+    CausalFactorGraph rebuilt(clamped.factors());
+    g_bp.reset();
+    g_bp.reset(new CausalBP(rebuilt, g_opts));
+    g_bp->init();
+}
+
+bool valid_index(int i) { return i >= 0 && (size_t)i < g_fg.nrVars(); }
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc < 2) {
+        std::cerr << "usage: " << argv[0] << " <file.causal_fg> [seed] [--auto | --manual id] [--digits N]\n";
+        return 1;
+    }
+    int digits = 6;
+    size_t device = 0;
+    int a = 2;
+    if (argc > 2 && argv[2][0] != '-') {
+        const int seed = std::stoi(argv[2]);
+        std::clog << "flbp_bingo: setting random seed " << seed << std::endl;
+        rnd_gen.seed(static_cast<unsigned int>(seed));
+        a = 3;
+    }
+    for (; a < argc; a++) {
+        const std::string arg = argv[a];
+        if (arg == "--auto") {
+            device = 0;
+        } else if (arg == "--manual" && a + 1 < argc) {
+            device = std::strtoull(argv[++a], nullptr, 10);
+        } else if (arg == "--digits" && a + 1 < argc) {
+            digits = std::atoi(argv[++a]);
+        } else {
+            std::cerr << "flbp_bingo: wrong arguments\n";
+            return 1;
+        }
+    }
+    const std::string file = argv[1];
+    const size_t dot = file.find_last_of('.');
+    if (dot == std::string::npos || file.substr(dot + 1) != "causal_fg") {
+        std::cerr << "flbp_bingo: only .causal_fg graphs are supported\n";
+        return 1;
+    }
+    g_fg.ReadFromFile(file.c_str());
+    std::clog << "flbp_bingo: read " << g_fg.nrVars() << " variables, " << g_fg.nrFactors() << " factors"
+              << std::endl;
+    g_opts.set("maxiter", static_cast<size_t>(10000000));
+    g_opts.set("maxtime", Real(57600));
+    g_opts.set("tol", Real(1e-6));
+    g_opts.set("updates", std::string("SEQFIX"));
+    g_opts.set("logdomain", false);
+    g_opts.set("device", device);
+    std::cout.precision(digits);
+
+    std::string cmd;
+    while (std::cin >> cmd) {
+        if (cmd == "Q") {
+            int i;
+            std::cin >> i;
+            if (!g_bp) LOG_FATAL("Q before any BP");
+            if (!valid_index(i)) LOG_FATAL("Q: no variable %d", i);
+            std::cout << g_bp->newBelief((size_t)i) << std::endl;
+        } else if (cmd == "BP") {
+            Real tol;
+            size_t min_iters, max_iters, hist;
+            std::cin >> tol >> min_iters >> max_iters >> hist;
+            std::clog << "flbp_bingo: BP " << tol << " " << min_iters << " " << max_iters << " " << hist << std::endl;
+            init_bp();
+            std::cout << g_bp->run(tol, min_iters, max_iters, hist) << std::endl;
+        } else if (cmd == "O") {
+            int i;
+            std::string v;
+            std::cin >> i >> v;
+            if (v != "true" && v != "false") LOG_FATAL("O: value must be true or false, got %s", v.c_str());
+            if (!valid_index(i)) LOG_FATAL("O: no variable %d", i);
+            g_clamps[i] = (v == "true");
+            std::cout << "O " << i << " " << v << std::endl;
+        } else if (cmd == "UC") {
+            int i;
+            std::cin >> i;
+            if (g_clamps.erase(i) == 0) LOG_FATAL("UC: variable %d is not clamped", i);
+            std::cout << "UC " << i << std::endl;
+        } else if (cmd == "BYE") {
+            break;
+        } else if (cmd == "NL") {
+            std::cout << std::endl;
+        } else {
+            LOG_FATAL("unknown command %s", cmd.c_str());
+        }
+    }
+    std::clog << "flbp_bingo: bye" << std::endl;
+    return 0;
+}

@@ -41,6 +41,8 @@ _SIG = {
     "orc_run": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.c_double, C.c_int, u64p, f64p]),
     "orc_run_fixed": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.c_int]),
     "orc_sweep": (C.c_int, [C.c_void_p, C.c_double, C.c_int, f64p]),
+    "orc_run_hist": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, f64p, f64p,
+                               C.POINTER(C.c_int), f64p]),
     "orc_beliefs": (C.c_int, [C.c_void_p, f64p]),
     "orc_iters": (C.c_uint64, [C.c_void_p]),
     "orc_messages": (C.c_int, [C.c_void_p, f64p, f64p, f64p, f64p]),
@@ -156,6 +158,15 @@ class OracleGraph:
 
     def run_fixed(self, n, damping=0.0, threads=1):
         _chk("orc_run_fixed", lib().orc_run_fixed(self._h, n, damping, threads))
+
+    def run_hist(self, tol, min_iters, max_iters, hist_len, damping=0.0):
+        """The 4-argument run: (yet-to-converge fraction, low-pass beliefs,
+        exit reason 0/1/2, last max diff)."""
+        yet, md, reason = C.c_double(), C.c_double(), C.c_int()
+        low = np.zeros(max(self.n_vars, 1), np.float64)
+        _chk("orc_run_hist", lib().orc_run_hist(self._h, tol, min_iters, max_iters, hist_len, damping,
+                                               C.byref(yet), _p(low, C.c_double), C.byref(reason), C.byref(md)))
+        return float(yet.value), low[:self.n_vars], int(reason.value), float(md.value)
 
     def sweep(self, damping=0.0, threads=1):
         md = C.c_double()

@@ -854,6 +854,78 @@ int orc_sweep(orc_graph *g, double damping, int threads, double *maxdiff_out) {
     return 0;
 }
 
+/* CausalBP::run(tol, minIters, maxIters, histLength), causal_bp.cu:1006-1143
+ * with maxtime = infinity, and HostBeliefHistory (28-78): per sweep the
+ * beliefs, the count of variables whose belief moved by more than tol and
+ * a history push of every finite belief; the exit tests in the reference's
+ * order at the top of each pass.  lowpass gets the beliefs on ALL_CONVERGED,
+ * else the mean of each variable's last histLength finite beliefs, oldest
+ * first, divided once.  reason: 0 all converged, 1 big fraction converged,
+ * 2 diverged. */
+int orc_run_hist(orc_graph *g, double tol, uint64_t min_iters, uint64_t max_iters, uint64_t hist_len,
+                 double damping, double *yet_out, double *lowpass, int *reason_out, double *maxdiff_out) {
+    if (hist_len == 0) return fail("belief history length must be positive");
+    if (g->seq && damping != 0.0) return fail("damping is defined for PARALL updates only");
+    const uint64_t V = g->V;
+    std::vector<double> values(V * hist_len, 0.0);
+    std::vector<uint64_t> next(V, 0), count(V, 0);
+    uint64_t num_iters = 0;
+    double max_diff = INFINITY, yet = 1.0, node_frac = 0.0;
+    int reason = 2;
+    for (;; num_iters++, g->iters++) {
+        if (num_iters >= min_iters)
+            node_frac = (double)(uint64_t)(num_iters - min_iters) / (double)(uint64_t)(max_iters - min_iters);
+        if (max_diff <= tol) {
+            reason = 0;
+            break;
+        } else if (num_iters > min_iters && yet < node_frac) {
+            reason = 1;
+            break;
+        } else if (num_iters > max_iters) {
+            reason = 2;
+            break;
+        }
+        const std::vector<double> old = g->old_beliefs;
+        const double diff = sweep(g, damping);
+        /* maxDiff = std::max(diff, -INFINITY) */
+        max_diff = (diff < -INFINITY) ? -INFINITY : diff;
+        uint64_t non_converged = 0;
+        for (uint64_t v = 0; v < V; v++) {
+            const double b = g->old_beliefs[v];
+            non_converged += std::fabs(old[v] - b) > tol;
+            const int c = std::fpclassify(b);
+            if (c != FP_NORMAL && c != FP_SUBNORMAL && c != FP_ZERO) continue;
+            uint64_t slot = next[v];
+            values[slot * V + v] = b;
+            if (++slot == hist_len) slot = 0;
+            next[v] = slot;
+            if (count[v] < hist_len) ++count[v];
+        }
+        yet = (double)non_converged / (double)V;
+    }
+    if (lowpass) {
+        for (uint64_t v = 0; v < V; v++) {
+            if (reason == 0) {
+                lowpass[v] = g->old_beliefs[v];
+                continue;
+            }
+            const uint64_t c = count[v];
+            uint64_t slot = c == hist_len ? next[v] : 0;
+            double sum = 0.0;
+            for (uint64_t k = 0; k < c; k++) {
+                sum += values[slot * V + v];
+                if (++slot == hist_len) slot = 0;
+            }
+            if (c > 0) sum /= (double)c;
+            lowpass[v] = sum;
+        }
+    }
+    if (yet_out) *yet_out = yet;
+    if (reason_out) *reason_out = reason;
+    if (maxdiff_out) *maxdiff_out = max_diff;
+    return 0;
+}
+
 int orc_beliefs(const orc_graph *g, double *out) {
     std::copy(g->old_beliefs.begin(), g->old_beliefs.end(), out);
     return 0;

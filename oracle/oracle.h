@@ -68,6 +68,13 @@ int orc_run(orc_graph *g, double tol, uint64_t maxiter, double damping,
  * once at the end. */
 int orc_run_fixed(orc_graph *g, uint64_t n_iters, double damping, int threads);
 
+/* CausalBP::run(tol, minIters, maxIters, histLength) (maxtime = infinity)
+ * with HostBeliefHistory: returns the yet-to-converge fraction in *yet,
+ * the low-pass beliefs [V] (newBelief), the exit reason (0 all converged,
+ * 1 big fraction converged, 2 diverged) and the last max diff. */
+int orc_run_hist(orc_graph *g, double tol, uint64_t min_iters, uint64_t max_iters, uint64_t hist_len,
+                 double damping, double *yet, double *lowpass, int *reason, double *maxdiff);
+
 /* One sweep + beliefs without touching the iteration counter; writes the
  * max belief change.  Used to time a bounded sample. */
 int orc_sweep(orc_graph *g, double damping, int threads, double *maxdiff_out);
