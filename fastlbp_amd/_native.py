@@ -126,6 +126,8 @@ SIGNATURES = {
     "flbp_graph_build": (C.c_int, [C.POINTER(FactorListC), C.POINTER(C.c_void_p)]),
     "flbp_graph_csr": (C.c_int, [C.c_void_p, C.POINTER(CsrC)]),
     "flbp_graph_free": (None, [C.c_void_p]),
+    "flbp_write_graph": (C.c_int, [C.POINTER(CsrC), C.c_char_p]),
+    "flbp_read_graph": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "flbp_read_causal_fg": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "flbp_write_causal_fg": (C.c_int, [C.POINTER(FactorListC), C.c_char_p]),
     "flbp_clamp": (C.c_int, [C.POINTER(FactorListC), C.c_uint64, u64p, u8p, C.POINTER(C.c_void_p)]),

@@ -9,7 +9,12 @@
 #include <string>
 #include <vector>
 
+#include "fastlbp.h"
+
 namespace flbp {
+
+/* structural checks of a CSR (plan_host.hip) */
+int validate_csr(const flbp_csr &c);
 
 /* Records msg as the calling thread's last error and returns code. */
 int fail(int code, const std::string &msg);

@@ -478,3 +478,187 @@ int flbp_rank_alarms(uint64_t n, const uint64_t *labels, const double *marg, uin
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------ */
+/* Binary graph files (not in the reference)                            */
+/* ------------------------------------------------------------------ */
+
+namespace {
+
+constexpr char kGraphMagic[8] = {'F', 'L', 'B', 'P', 'C', 'S', 'R', '1'};
+
+/* Integrity check over the file's 64-bit words (the tail zero-padded):
+ * h = rotl(h ^ w, 29) * K per word.  Detects truncation and corruption; it
+ * is not a cryptographic hash. */
+struct WordHash {
+    uint64_t h = 0x6a09e667f3bcc909ull;
+    uint64_t pend = 0;
+    unsigned npend = 0;
+    void word(uint64_t w) {
+        h ^= w;
+        h = (h << 29) | (h >> 35);
+        h *= 0x9e3779b97f4a7c15ull;
+    }
+    void bytes(const void *p, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        size_t i = 0;
+        while (npend && i < n) {
+            pend |= (uint64_t)b[i++] << (8 * npend);
+            if (++npend == 8) {
+                word(pend);
+                pend = 0;
+                npend = 0;
+            }
+        }
+        for (; i + 8 <= n; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, b + i, 8);
+            word(w);
+        }
+        for (; i < n; i++) {
+            pend |= (uint64_t)b[i] << (8 * npend);
+            npend++;
+        }
+    }
+    uint64_t digest() {
+        if (npend) word(pend);
+        word(0x8000000000000000ull | npend);
+        return h;
+    }
+};
+
+struct Out {
+    std::FILE *f;
+    WordHash hash;
+    bool ok = true;
+    void put(const void *p, size_t n) {
+        if (!n) return;
+        hash.bytes(p, n);
+        ok = ok && std::fwrite(p, 1, n, f) == n;
+    }
+    template <class T, class S>
+    void narrow(const S *src, uint64_t n) {   /* src[n] as T, in blocks */
+        std::vector<T> buf(std::min<uint64_t>(n, 1 << 20));
+        for (uint64_t i = 0; i < n; i += buf.size()) {
+            const uint64_t m = std::min<uint64_t>(buf.size(), n - i);
+            for (uint64_t k = 0; k < m; k++) buf[k] = (T)src[i + k];
+            put(buf.data(), m * sizeof(T));
+        }
+    }
+};
+
+struct In {
+    std::FILE *f;
+    WordHash hash;
+    bool ok = true;
+    void get(void *p, size_t n) {
+        if (!n || !ok) return;
+        ok = std::fread(p, 1, n, f) == n;
+        if (ok) hash.bytes(p, n);
+    }
+    template <class T, class D>
+    void widen(std::vector<D> &dst, uint64_t n) {   /* n values of T into dst */
+        dst.resize(n);
+        std::vector<T> buf(std::min<uint64_t>(n, 1 << 20));
+        for (uint64_t i = 0; i < n && ok; i += buf.size()) {
+            const uint64_t m = std::min<uint64_t>(buf.size(), n - i);
+            get(buf.data(), m * sizeof(T));
+            for (uint64_t k = 0; k < m; k++) dst[i + k] = (D)buf[k];
+        }
+    }
+};
+
+}  // namespace
+
+int flbp_write_graph(const flbp_csr *c, const char *path) {
+    if (!c || !path) return flbp::fail(FLBP_EINVAL, "flbp_write_graph: null argument");
+    if (int rc = flbp::validate_csr(*c)) return rc;
+    const uint64_t V = c->n_vars, F = c->n_factors, E = c->n_edges;
+    if (V >= (1ull << 32) - 1 || F >= (1ull << 32) - 1 || E >= (1ull << 32) - 1)
+        return flbp::fail(FLBP_ERANGE, "flbp_write_graph: V, F and E must each be below 2^32-1");
+    std::FILE *f = std::fopen(path, "wb");
+    if (!f) return flbp::fail(FLBP_EIO, std::string("flbp_write_graph: cannot open ") + path);
+    Out o{f};
+    const uint64_t hdr[3] = {V, F, E};
+    o.put(kGraphMagic, 8);
+    o.put(hdr, sizeof hdr);
+    o.put(c->var_label, V * 8);
+    o.narrow<uint32_t>(c->fac_off, F + 1);
+    o.narrow<uint32_t>(c->edge_var, E);
+    o.narrow<uint32_t>(c->var_off, V + 1);
+    o.narrow<uint32_t>(c->var_edge, E);
+    o.narrow<uint32_t>(c->head_slot, F);
+    o.narrow<uint8_t>(c->fac_type, F);
+    o.put(c->head_clamped, F);
+    o.put(c->p, F * 8);
+    o.put(c->q, F * 8);
+    o.put(c->mask0, F * 8);
+    o.put(c->mask1, F * 8);
+    const uint64_t digest = o.hash.digest();
+    o.ok = o.ok && std::fwrite(&digest, 1, 8, f) == 8;
+    const bool closed = std::fclose(f) == 0;
+    if (!o.ok || !closed) return flbp::fail(FLBP_EIO, std::string("flbp_write_graph: write to ") + path + " failed");
+    return FLBP_OK;
+}
+
+int flbp_read_graph(const char *path, flbp_graph **out) {
+    if (!path || !out) return flbp::fail(FLBP_EINVAL, "flbp_read_graph: null argument");
+    std::FILE *f = std::fopen(path, "rb");
+    if (!f) return flbp::fail(FLBP_EIO, std::string("flbp_read_graph: cannot open ") + path);
+    In in{f};
+    char magic[8];
+    uint64_t hdr[3];
+    in.get(magic, 8);
+    in.get(hdr, sizeof hdr);
+    if (!in.ok || std::memcmp(magic, kGraphMagic, 8) != 0) {
+        std::fclose(f);
+        return flbp::fail(FLBP_EIO, std::string("flbp_read_graph: ") + path + " is not a graph file");
+    }
+    const uint64_t V = hdr[0], F = hdr[1], E = hdr[2];
+    if (V >= (1ull << 32) - 1 || F >= (1ull << 32) - 1 || E >= (1ull << 32) - 1) {
+        std::fclose(f);
+        return flbp::fail(FLBP_EIO, "flbp_read_graph: header sizes out of range");
+    }
+    auto *g = new (std::nothrow) flbp_graph();
+    if (!g) {
+        std::fclose(f);
+        return flbp::fail(FLBP_ENOMEM, "flbp_read_graph: out of memory");
+    }
+    try {
+        g->var_label.resize(V);
+        in.get(g->var_label.data(), V * 8);
+        in.widen<uint32_t>(g->fac_off, F + 1);
+        in.widen<uint32_t>(g->edge_var, E);
+        in.widen<uint32_t>(g->var_off, V + 1);
+        in.widen<uint32_t>(g->var_edge, E);
+        in.widen<uint32_t>(g->head_slot, F);
+        in.widen<uint8_t>(g->fac_type, F);
+        g->head_clamped.resize(F);
+        in.get(g->head_clamped.data(), F);
+        for (auto *v : {&g->p, &g->q, &g->mask0, &g->mask1}) {
+            v->resize(F);
+            in.get(v->data(), F * 8);
+        }
+    } catch (const std::bad_alloc &) {
+        std::fclose(f);
+        delete g;
+        return flbp::fail(FLBP_ENOMEM, "flbp_read_graph: out of memory");
+    }
+    uint64_t digest = 0;
+    const uint64_t want = in.hash.digest();
+    const bool got = in.ok && std::fread(&digest, 1, 8, f) == 8;
+    const bool extra = got && std::fgetc(f) != EOF;
+    std::fclose(f);
+    if (!got || digest != want || extra) {
+        delete g;
+        return flbp::fail(FLBP_EIO, std::string("flbp_read_graph: ") + path + " is truncated or corrupt");
+    }
+    flbp_csr view;
+    flbp_graph_csr(g, &view);
+    if (int rc = flbp::validate_csr(view)) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return FLBP_OK;
+}

@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fastlbp.h"
@@ -33,15 +35,53 @@ int validate_csr(const flbp_csr &c) {
     }
     for (uint64_t i = 0; i < V; i++)
         if (c.var_off[i + 1] < c.var_off[i]) return bad("var_off not monotone");
-    std::vector<uint8_t> seen(E, 0);
-    for (uint64_t i = 0; i < V; i++) {
-        for (uint64_t k = c.var_off[i]; k < c.var_off[i + 1]; k++) {
-            const uint64_t s = c.var_edge[k];
-            if (s >= E || seen[s]) return bad("var_edge is not a permutation of the slots");
-            if (c.edge_var[s] != i) return bad("var_edge and edge_var disagree");
-            seen[s] = 1;
+    /* var_edge must be a permutation of the slots that agrees with
+     * edge_var: checked by threads over variable ranges of equal edge
+     * counts, marking slots in a shared bitmap (the reads are random, so
+     * threads scale them) */
+    std::vector<std::atomic<uint64_t>> seen((E + 63) / 64);
+    for (auto &w : seen) w.store(0, std::memory_order_relaxed);
+    std::atomic<int> err{0};   /* 1 not a permutation, 2 disagreement */
+    auto check = [&](uint64_t v0, uint64_t v1) {
+        for (uint64_t i = v0; i < v1 && !err.load(std::memory_order_relaxed); i++) {
+            for (uint64_t k = c.var_off[i]; k < c.var_off[i + 1]; k++) {
+                const uint64_t s = c.var_edge[k];
+                if (s >= E) {
+                    err.store(1);
+                    return;
+                }
+                const uint64_t bit = 1ull << (s & 63);
+                if (seen[s >> 6].fetch_or(bit, std::memory_order_relaxed) & bit) {
+                    err.store(1);
+                    return;
+                }
+                if (c.edge_var[s] != i) {
+                    err.store(2);
+                    return;
+                }
+            }
         }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = E < (1u << 20) ? 1u : hw;
+    if (nt == 1) {
+        check(0, V);
+    } else {
+        std::vector<std::thread> pool;
+        uint64_t v0 = 0;
+        for (unsigned t = 0; t < nt; t++) {
+            /* first variable whose edges start at or after (t+1)/nt of E */
+            const uint64_t target = E / nt * (t + 1);
+            const uint64_t v1 = t + 1 == nt ? V
+                                            : (uint64_t)(std::lower_bound(c.var_off, c.var_off + V + 1, target) - c.var_off);
+            const uint64_t a = v0, b = std::max(v0, std::min(v1, V));
+            pool.emplace_back(check, a, b);
+            v0 = b;
+        }
+        for (auto &th : pool) th.join();
     }
+    if (err.load() == 1) return bad("var_edge is not a permutation of the slots");
+    if (err.load() == 2) return bad("var_edge and edge_var disagree");
     return FLBP_OK;
 }
 

@@ -42,7 +42,6 @@ void seqfix_order(const flbp_csr &c, std::mt19937 &g, std::vector<uint64_t> &out
 void topo_order(const flbp_csr &c, std::vector<uint64_t> &out);
 int build_schedule(const flbp_csr &c, const uint64_t *order, uint64_t n, bool respect_preorder, Schedule &S);
 const Schedule *schedule_of(const flbp_schedule *h);
-int validate_csr(const flbp_csr &c);
 
 }  // namespace flbp
 

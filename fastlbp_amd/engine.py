@@ -164,10 +164,30 @@ def generate(n_vars: int, seed: int, **kw) -> Generated:
 class Graph:
     """A factor graph in the reference's CSR orders (flbp_graph_build)."""
 
-    def __init__(self, fl: FactorList):
+    def __init__(self, fl: Optional[FactorList]):
+        if fl is None:
+            return
         h = C.c_void_p()
         c = fl.as_c()
         call("flbp_graph_build", C.byref(c), C.byref(h))
+        self._take(h)
+
+    @classmethod
+    def read(cls, path: str) -> "Graph":
+        """A graph from a binary graph file (flbp_read_graph)."""
+        h = C.c_void_p()
+        call("flbp_read_graph", path.encode(), C.byref(h))
+        g = cls(None)
+        g._take(h)
+        return g
+
+    def write(self, path: str) -> None:
+        """Writes the binary graph file (flbp_write_graph)."""
+        c = self.csr_c()
+        call("flbp_write_graph", C.byref(c), path.encode())
+
+    def _take(self, h) -> None:
+        """Copies the CSR of a native graph handle and frees the handle."""
         try:
             v = CsrC()
             call("flbp_graph_csr", h, C.byref(v))

@@ -96,6 +96,17 @@ int flbp_read_causal_fg(const char *path, flbp_flist **out);
 /* Writes a factor list in the .causal_fg text format the reader accepts. */
 int flbp_write_causal_fg(const flbp_factor_list *fl, const char *path);
 
+/* Binary graph files (not in the reference): the CSR above with 32-bit
+ * index arrays and a trailing 64-bit integrity check, little-endian, so a
+ * graph of tens of millions of variables loads without parsing or
+ * rebuilding.  Layout: "FLBPCSR1", u64 n_vars, n_factors, n_edges,
+ * u64 var_label[V], u32 fac_off[F+1], u32 edge_var[E], u32 var_off[V+1],
+ * u32 var_edge[E], u32 head_slot[F], u8 fac_type[F], u8 head_clamped[F],
+ * f64 p[F], q[F], mask0[F], mask1[F], u64 check.  Reading validates the
+ * check and the CSR's structure (FLBP_EIO / FLBP_EINVAL). */
+int flbp_write_graph(const flbp_csr *csr, const char *path);
+int flbp_read_graph(const char *path, flbp_graph **out);
+
 /* CausalFactorGraph::clamp (causal_fg.cpp:17-26) for each (label, value),
  * applied in ascending label order as examples/fl_wrapper.cu:26-45 does,
  * returning the factor list to rebuild from (fl_wrapper.cu:45). */
