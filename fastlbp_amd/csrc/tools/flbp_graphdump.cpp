@@ -7,7 +7,11 @@
  * Also checks PropertySet's exact-type lookup (properties.h:44-53).
  *
  * usage: flbp_graphdump <file.causal_fg> [observations]
+ *        flbp_graphdump <file.fg>      (generic factor graph: line 1 "V F";
+ *                                       per variable "label states"; per
+ *                                       factor "n labels... | table...")
  *        flbp_graphdump --properties
+ *        flbp_graphdump --index        (IndexFor / Permute on fixed inputs)
  */
 #include <cstdio>
 #include <fstream>
@@ -15,6 +19,8 @@
 #include <string>
 
 #include <ext/causal_fg.h>
+#include <lbp/factorgraph.h>
+#include <lbp/index.h>
 #include <lbp/properties.h>
 
 using namespace lbp;
@@ -34,8 +40,45 @@ static int properties_check() {
     return 0;
 }
 
+/* IndexFor of {x1(2), x3(3)} over {x1(2), x2(2), x3(3)}; Permute of
+ * (x7(2), x3(3), x5(2)) forward and reversed, for linear indices 0..11 */
+static int index_check() {
+    std::vector<Var> for_vars{Var(1, 2), Var(2, 2), Var(3, 3)}, idx_vars{Var(1, 2), Var(3, 3)};
+    IndexFor it(VarSet(idx_vars.begin(), idx_vars.end(), 2), VarSet(for_vars.begin(), for_vars.end(), 3));
+    for (; it.valid(); ++it) std::printf("%zu ", (size_t)it);
+    std::printf("\n");
+    const std::vector<Var> listed{Var(7, 2), Var(3, 3), Var(5, 2)};
+    for (int rev = 0; rev < 2; rev++) {
+        const Permute p(listed, rev != 0);
+        for (size_t li = 0; li < 12; li++) std::printf("%zu ", p.convertLinearIndex(li));
+        std::printf("\n");
+    }
+    return 0;
+}
+
+static int dump_fg(const char *path) {
+    FactorGraph fg;
+    fg.ReadFromFile(path);
+    std::printf("%zu %zu\n", fg.nrVars(), fg.nrFactors());
+    for (size_t i = 0; i < fg.nrVars(); i++) std::printf("%zu %zu\n", fg.var(i).label(), fg.var(i).states());
+    for (size_t I = 0; I < fg.nrFactors(); I++) {
+        const Factor &f = fg.factor(I);
+        std::printf("%zu", f.vars().size());
+        for (const Var &v : f.vars()) std::printf(" %zu", v.label());
+        std::printf(" |");
+        for (size_t k = 0; k < f.nrStates(); k++) std::printf(" %.17g", f[k]);
+        std::printf("\n");
+    }
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc >= 2 && std::string(argv[1]) == "--properties") return properties_check();
+    if (argc >= 2 && std::string(argv[1]) == "--index") return index_check();
+    if (argc >= 2) {
+        const std::string f = argv[1];
+        if (f.size() > 3 && f.compare(f.size() - 3, 3, ".fg") == 0) return dump_fg(argv[1]);
+    }
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s <file.causal_fg> [observations]\n", argv[0]);
         return 2;

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <iostream>
+#include <istream>
 #include <map>
 #include <random>
 #include <set>
@@ -22,6 +23,14 @@ typedef size_t Size;
  * 42 at thread start (include/utils/utils.h:133-136, causal_bp.cu:116) */
 typedef std::mt19937 rnd_gen_type;
 extern thread_local rnd_gen_type rnd_gen;
+
+/* skips the '#' lines that start at the stream's position
+ * (include/utils/utils.h:109-113) */
+inline void read_annotation(std::istream &is) {
+    std::string line;
+    while (is.peek() == '#') std::getline(is, line);
+}
+inline void read_annotation(std::istream &is, std::string line) { read_annotation(is); }
 
 /* wall clock in seconds */
 inline Real toc() {
