@@ -298,10 +298,10 @@ def bench_c5(args):
 def seq_level_bytes(graph, sched):
     """Algorithmic HBM bytes of the level kernels of one sequential sweep.
 
-    v->f item: item 16, f->v message 16, product 16, zero counts 16, write 16.
-    f->v item: item 16, factor offsets 8, meta 4, p 8, q 8, the factor's d
-               v->f messages 16 d, old message 16, product + counts 32, and
-               writes of product, counts and message 48.
+    v->f item: item 16, f->v message 16, product + counts 32, write 16.
+    f->v item: item 32, the factor's d v->f messages 16 d, the factor's
+               p / q / mask record 32, old message 16, product + counts 32,
+               and writes of product, counts and message 48.
     beliefs:   product + counts 32, old belief 8, new belief 8 per variable.
     """
     import numpy as np
@@ -309,7 +309,7 @@ def seq_level_bytes(graph, sched):
     deg = np.diff(graph.fac_off.astype(np.int64))
     slot_fac = np.repeat(np.arange(graph.n_factors), deg)
     d = deg[slot_fac[graph.var_edge[fe.astype(np.int64)].astype(np.int64)]]
-    level = 80 * len(vs) + int(np.sum(140 + 16 * d))
+    level = 80 * len(vs) + int(np.sum(160 + 16 * d))
     return level, 48 * graph.n_vars
 
 
@@ -450,6 +450,14 @@ def main():
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
               "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 536870912 bytes read, 536870912 bytes written")
+        del src, dst
     if args.workload == "batched":
         return bench_batched(args)
     if args.workload == "adjoint":
@@ -461,14 +469,6 @@ def main():
 
     import torch
     import fastlbp_amd as fl
-
-    if args.calibrate_copy:
-        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("[bench] calibration copy: 536870912 bytes read, 536870912 bytes written")
-        del src, dst
 
     w = dict(WORKLOAD)
     if args.n_vars:

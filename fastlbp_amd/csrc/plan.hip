@@ -551,6 +551,8 @@ int flbp_plan_set_p(flbp_plan *plan, const double *p, uint64_t n) {
     if (n != P->F) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_p: n must equal n_factors");
     HIP_TRY(hipSetDevice(P->device));
     if (n) HIP_TRY(hipMemcpy(P->fac_p, p, n * sizeof(double), hipMemcpyHostToDevice));
+    if (int rc = flbp::seq_refresh_factors(P)) return rc;
+    HIP_TRY(hipStreamSynchronize(P->stream));
     return FLBP_OK;
 }
 
@@ -579,6 +581,7 @@ int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *facto
         HIP_TRY(hipMemcpyAsync(P->fac_meta + I, &P->h_meta[I], 4, hipMemcpyHostToDevice, P->stream));
         HIP_TRY(hipMemcpyAsync(P->fac_mask + I, &P->h_mask[I], 16, hipMemcpyHostToDevice, P->stream));
     }
+    if (int rc = flbp::seq_refresh_factors(P)) return rc;
     HIP_TRY(hipStreamSynchronize(P->stream));
     return FLBP_OK;
 }
@@ -593,6 +596,7 @@ int flbp_plan_reset_head_clamps(flbp_plan *plan) {
         HIP_TRY(hipMemcpyAsync(P->fac_meta + I, &P->h_meta[I], 4, hipMemcpyHostToDevice, P->stream));
         HIP_TRY(hipMemcpyAsync(P->fac_mask + I, &P->h_mask[I], 16, hipMemcpyHostToDevice, P->stream));
     }
+    if (int rc = flbp::seq_refresh_factors(P)) return rc;
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->clamp_dirty.clear();
     return FLBP_OK;

@@ -207,6 +207,7 @@ int Plan::alloc(T *&p, uint64_t n) {
 /* element types allocated from other translation units */
 template int Plan::alloc<double2>(double2 *&, uint64_t);
 template int Plan::alloc<dev::VarState>(dev::VarState *&, uint64_t);
+template int Plan::alloc<dev::FacRec>(dev::FacRec *&, uint64_t);
 
 #define TRY(expr)          \
     do {                   \

@@ -20,6 +20,11 @@ struct __align__(32) VarState {
     double p0, p1;
     unsigned long long z0, z1;
 };
+/* a factor's numbers for the sequential kernels in one 32-byte record:
+ * prob, prob_default (0 for priors), head mask ((1, 1) unless clamped) */
+struct __align__(32) FacRec {
+    double p, q, m0, m1;
+};
 }  // namespace dev
 
 constexpr int kMaxChunk = 64;   /* sweeps enqueued between host checks */
@@ -54,6 +59,7 @@ void var_device_order(const flbp_csr &c, bool by_degree, std::vector<uint32_t> &
                       std::vector<uint32_t> &vpos);
 struct Plan;
 int seq_reset_state(Plan *P, double c);   /* seq.hip */
+int seq_refresh_factors(Plan *P);         /* seq.hip: factor records after p / clamp changes */
 
 struct Plan {
     int device = -1;
@@ -94,7 +100,8 @@ struct Plan {
     bool seq = false;
     int seq_updates = -1;                          /* FLBP_UPDATES_* or -1 (explicit order) */
     uint4 *seq_vf = nullptr, *seq_fv = nullptr;    /* items grouped by level */
-    uint2 *seq_fv_dt = nullptr;                    /* f->v items: factor degree, target slot */
+    uint4 *seq_fv_aux = nullptr;                   /* f->v items: degree | type << 24, target slot, head slot */
+    dev::FacRec *seq_fac = nullptr;                /* per-factor records */
     dev::VarState *seq_state = nullptr;            /* per-variable products and zero counts */
     std::vector<uint32_t> seq_lvl_vf, seq_lvl_fv;  /* [levels+1] item offsets per level */
     uint32_t *seq_lvl_vf_dev = nullptr, *seq_lvl_fv_dev = nullptr;

@@ -10,7 +10,9 @@
  *   seq_vf     u32x4[n_vf]  v->f items (slot, var-major position, variable, 0)
  *   seq_fv     u32x4[n_fv]  f->v items (var-major position, variable, factor,
  *              first slot of the factor)
- *   seq_fv_dt  u32x2[n_fv]  (factor degree, target slot in the factor)
+ *   seq_fv_aux u32x4[n_fv]  (degree | factor type << 24, target slot, head slot, 0)
+ *   seq_fac    {f64 p, q, m0, m1}[F]  factor records, rebuilt on the device
+ *              from fac_p / fac_q / fac_meta / fac_mask whenever they change
  *   seq_lvl_*_dev u32[levels+1]  first item of each level
  * Items are grouped by level; within a level v->f items are ordered by
  * slot and f->v items by factor, so neighbouring lanes touch neighbouring
@@ -56,11 +58,10 @@ namespace flbp {
     } while (0)
 
 void Plan::free_seq_items() {
-    for (void *p : {(void *)seq_vf, (void *)seq_fv, (void *)seq_fv_dt, (void *)seq_lvl_vf_dev,
+    for (void *p : {(void *)seq_vf, (void *)seq_fv, (void *)seq_fv_aux, (void *)seq_lvl_vf_dev,
                     (void *)seq_lvl_fv_dev})
         if (p) (void)hipFree(p);
-    seq_vf = seq_fv = nullptr;
-    seq_fv_dt = nullptr;
+    seq_vf = seq_fv = seq_fv_aux = nullptr;
     seq_lvl_vf_dev = seq_lvl_fv_dev = nullptr;
     seq_lvl_vf.clear();
     seq_lvl_fv.clear();
@@ -70,7 +71,7 @@ void Plan::free_seq_items() {
 /* One sweep: every level, then beliefs and the max-diff of sweep k. */
 int Plan::launch_seq_sweep(int k, double tol) {
     const size_t L = seq_lvl_fv.empty() ? 0 : seq_lvl_fv.size() - 1;
-    SeqArgs A{seq_vf, seq_fv, seq_fv_dt, fac_meta, fac_p, fac_q, fac_mask, m_vf, m_fv_vm, seq_state};
+    SeqArgs A{seq_vf, seq_fv, seq_fv_aux, seq_fac, m_vf, m_fv_vm, seq_state};
     for (size_t l = 0; l < seq_tail_from; l++) {
         const uint32_t v0 = seq_lvl_vf[l], nv = seq_lvl_vf[l + 1] - v0;
         const uint32_t f0 = seq_lvl_fv[l], nf = seq_lvl_fv[l + 1] - f0;
@@ -79,7 +80,7 @@ int Plan::launch_seq_sweep(int k, double tol) {
         SeqArgs a = A;
         a.vf_items = seq_vf + v0;
         a.fv_items = seq_fv + f0;
-        a.fv_dt = seq_fv_dt + f0;
+        a.fv_aux = seq_fv_aux + f0;
         hipLaunchKernelGGL(seq_level, dim3(vb + fb), dim3(dev::kBlock), 0, stream, a, nv, nf, vb, ctl, k, tol);
     }
     for (size_t l0 = seq_tail_from; l0 < L; l0 += dev::kSeqTailLevels) {
@@ -148,19 +149,19 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
         std::sort(vf.begin() + lv[l], vf.begin() + lv[l + 1], [](const uint4 &a, const uint4 &b) { return a.x < b.x; });
         std::sort(fk.begin() + lf[l], fk.begin() + lf[l + 1]);
     }
-    std::vector<uint4> fv(fk.size());
-    std::vector<uint2> fdt(fk.size());
+    std::vector<uint4> fv(fk.size()), faux(fk.size());
     for (size_t y = 0; y < fk.size(); y++) {
         const uint32_t x = fk[y].second, I = S.fv_fac[x];
         const uint32_t a = (uint32_t)P->h_fac_off[I], d = (uint32_t)(P->h_fac_off[I + 1] - a);
+        const uint32_t meta = P->h_meta[I];   /* type and head slot never change */
         fv[y] = make_uint4(S.fv_edge[x], S.fv_var[x], I, a);
-        fdt[y] = make_uint2(d, S.fv_slot[x] - a);
+        faux[y] = make_uint4(d | ((meta & 3u) << 24), S.fv_slot[x] - a, meta >> 8, 0u);
     }
     P->free_seq_items();
     const size_t nvf = std::max<size_t>(vf.size(), 1), nfv = std::max<size_t>(fv.size(), 1);
     if (hipMalloc(&P->seq_vf, nvf * sizeof(uint4)) != hipSuccess ||
         hipMalloc(&P->seq_fv, nfv * sizeof(uint4)) != hipSuccess ||
-        hipMalloc(&P->seq_fv_dt, nfv * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&P->seq_fv_aux, nfv * sizeof(uint4)) != hipSuccess ||
         hipMalloc(&P->seq_lvl_vf_dev, (nl + 1) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&P->seq_lvl_fv_dev, (nl + 1) * sizeof(uint32_t)) != hipSuccess) {
         (void)hipGetLastError();
@@ -170,7 +171,7 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     if (!vf.empty()) HIP_TRY(hipMemcpy(P->seq_vf, vf.data(), vf.size() * sizeof(uint4), hipMemcpyHostToDevice));
     if (!fv.empty()) {
         HIP_TRY(hipMemcpy(P->seq_fv, fv.data(), fv.size() * sizeof(uint4), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(P->seq_fv_dt, fdt.data(), fdt.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->seq_fv_aux, faux.data(), faux.size() * sizeof(uint4), hipMemcpyHostToDevice));
     }
     /* level l (1-based) -> index l-1 */
     P->seq_lvl_vf.assign(lv.begin() + 1, lv.end());
@@ -185,13 +186,15 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     P->seq_tail_from = tail;
     if (!P->seq_state) {
         if (int rc = P->alloc(P->seq_state, P->V)) return rc;
+        if (int rc = P->alloc(P->seq_fac, P->F)) return rc;
     }
     if (P->V)
         hipLaunchKernelGGL(fill_state, dim3(grid_for(P->V, 256)), dim3(256), 0, P->stream, P->seq_state, P->V,
                            P->init_value);
+    P->seq = true;
+    if (int rc = flbp::seq_refresh_factors(P)) return rc;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(P->stream));
-    P->seq = true;
     P->seq_updates = S.updates;
     P->seq_waves = S.n_waves;
     return FLBP_OK;
@@ -206,6 +209,14 @@ int flbp_plan_get_updates(const flbp_plan *plan) {
 }  // extern "C"
 
 namespace flbp {
+
+int seq_refresh_factors(Plan *P) {
+    if (!P->seq || !P->F) return FLBP_OK;
+    hipLaunchKernelGGL(seq_fac_build, dim3(grid_for(P->F, dev::kBlock)), dim3(dev::kBlock), 0, P->stream,
+                       P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, P->seq_fac, (uint32_t)P->F);
+    HIP_TRY(hipGetLastError());
+    return FLBP_OK;
+}
 
 /* products back to the values init() leaves: (c, c), no zeros */
 int seq_reset_state(Plan *P, double c) {

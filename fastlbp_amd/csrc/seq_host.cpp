@@ -193,6 +193,8 @@ int build_schedule(const flbp_csr &c, const uint64_t *order, uint64_t n, bool re
         v_stamp[i] = wave;
         v_fac[i] = I;
         v_edge[i] = e;
+        if (S.vf_slot.size() + S.fv_edge.size() + (c.fac_off[I + 1] - c.fac_off[I]) >= (1ull << 32) - 1)
+            return fail(FLBP_ERANGE, "schedule: more than 2^32-2 items");
         S.fv_edge.push_back((uint32_t)e);
         S.fv_slot.push_back((uint32_t)c.var_edge[e]);
         S.fv_fac.push_back(I);

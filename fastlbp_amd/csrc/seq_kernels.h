@@ -9,7 +9,8 @@
  *     leave-one-out message from j's product and zero counts with the
  *     f->v message k divided out, normalized with no (0, 0) guard.
  *   f->v item (var-major position k, variable i, factor I, the factor's
- *     first slot a; degree d and target slot t in a second word):
+ *     first slot a; degree d, type, target slot t and head slot h in a
+ *     second word; p, q and the head mask in the factor's 32-byte record):
  *     calcMessageFVFusedKernel, causal_bp_seq.cu:149-459 -- the PARALL
  *     f->v arithmetic (fv_msg) with the sequential scale(), then
  *     update_message (34-83): the old message k divided out of i's
@@ -20,8 +21,12 @@
  * dependent memory accesses is kept at two: the item, then everything it
  * names (the item carries the factor's slot range, so the v->f messages do
  * not wait for the factor's offsets).  The stop test's read is issued
- * beside the item load.  A variable's product and zero counts share one
- * 32-byte record (one line per access).
+ * beside the item load.  Every kernel boundary empties the L2, so each
+ * random access costs a line from the fabric: a variable's product and
+ * zero counts share one 32-byte record, and so do a factor's p, q and head
+ * mask, whose mask is (1, 1) for an unclamped factor -- multiplying by 1
+ * is exact and the zero-mass fallbacks test the mask for 0, so applying
+ * the mask always gives the unclamped codes' results bit for bit.
  *
  * Items of one level touch disjoint outputs and read nothing another item
  * of the level writes, so lanes need no ordering among themselves.
@@ -85,11 +90,8 @@ __device__ __forceinline__ double2 vf_msg_seq(const VarState &st, double2 m) {
 struct SeqArgs {
     const uint4 *__restrict__ vf_items;   /* slot, var-major position, variable */
     const uint4 *__restrict__ fv_items;   /* var-major position, variable, factor, first slot */
-    const uint2 *__restrict__ fv_dt;      /* degree, target slot in the factor */
-    const uint32_t *__restrict__ fac_meta;
-    const double *__restrict__ fac_p;
-    const double *__restrict__ fac_q;
-    const double2 *__restrict__ fac_mask;
+    const uint4 *__restrict__ fv_aux;     /* degree | type << 24, target slot, head slot */
+    const FacRec *__restrict__ fac;       /* per-factor p, q, head mask */
     double2 *__restrict__ m_vf;           /* factor-major */
     double2 *__restrict__ m_fv_vm;        /* var-major */
     VarState *__restrict__ state;
@@ -101,35 +103,28 @@ __device__ __forceinline__ void seq_vf_item(const SeqArgs &A, const uint4 it) {
     A.m_vf[it.x] = vf_msg_seq(st, m);
 }
 
-__device__ __forceinline__ void seq_fv_item(const SeqArgs &A, const uint4 it, const uint2 dt) {
+__device__ __forceinline__ void seq_fv_item(const SeqArgs &A, const uint4 it, const uint4 aux) {
     const uint32_t I = it.z, v = it.y, a = it.w;
-    const int d = (int)dt.x, t = (int)dt.y;
+    const int d = (int)(aux.x & 0xffffffu), t = (int)aux.y, h = (int)aux.z;
+    const uint32_t type = aux.x >> 24;
     /* everything the item names, all independent of each other */
     RegIn<kSeqRegDeg> in;
 #pragma unroll
     for (int j = 0; j < kSeqRegDeg; j++) in.m[j] = j < d ? A.m_vf[a + j] : make_double2(0.0, 0.0);
-    const uint32_t meta = A.fac_meta[I];
-    const double p = A.fac_p[I];
-    const double qq = A.fac_q[I];
-    const double2 mk = A.fac_mask[I];
+    const FacRec f = A.fac[I];
     const double2 old = A.m_fv_vm[it.x];
     VarState st = A.state[v];
-    const uint32_t type = meta & 3u;
-    const bool clamped = (meta & 4u) != 0;
-    const int h = (int)(meta >> 8);
-    const double q = type ? qq : 0.0;
-    const double mask0 = clamped ? mk.x : 1.0, mask1 = clamped ? mk.y : 1.0;
     double2 r;
     if (type == 0) {
-        r.x = 1 - p;
-        r.y = p;
+        r.x = 1 - f.p;
+        r.y = f.p;
     } else if (d <= kSeqRegDeg) {
-        r = fv_msg<kSeqRegDeg, true>(in, type, clamped, d, t, h, in.pick(h), in.pick(h == 0 ? 1 : 0), p, q,
-                                     mask0, mask1);
+        r = fv_msg<kSeqRegDeg, true>(in, type, true, d, t, h, in.pick(h), in.pick(h == 0 ? 1 : 0), f.p, f.q,
+                                     f.m0, f.m1);
     } else {
         const ContigIn gin{A.m_vf + a};
-        r = fv_msg<0, true>(gin, type, clamped, d, t, h, gin.pick(h), gin.pick(h == 0 ? 1 : 0), p, q, mask0,
-                            mask1);
+        r = fv_msg<0, true>(gin, type, true, d, t, h, gin.pick(h), gin.pick(h == 0 ? 1 : 0), f.p, f.q, f.m0,
+                            f.m1);
     }
     /* update_message */
     if (old.x == 0) st.z0--; else st.p0 /= old.x;
@@ -153,29 +148,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_level(
     }
     const uint32_t x = (blockIdx.x - vf_blocks) * kBlock + threadIdx.x;
     const uint4 it = x < n_fv ? A.fv_items[x] : make_uint4(0u, 0u, 0u, 0u);
-    const uint2 dt = x < n_fv ? A.fv_dt[x] : make_uint2(0u, 0u);
+    const uint4 aux = x < n_fv ? A.fv_aux[x] : make_uint4(0u, 0u, 0u, 0u);
     if (chunk_stopped(ctl, k, tol) || x >= n_fv) return;
-    seq_fv_item(A, it, dt);
+    seq_fv_item(A, it, aux);
 }
 
 /* one item of a tail level: x indexes the level's v->f items, then its
  * f->v items; kind 0 = none, 1 = v->f, 2 = f->v */
 struct TailItem {
     uint4 it;
-    uint2 dt;
+    uint4 aux;
     uint32_t kind;
 };
 
 __device__ __forceinline__ TailItem tail_fetch(const SeqArgs &A, const uint32_t *s_vf, const uint32_t *s_fv,
                                                uint32_t l, uint32_t x) {
-    TailItem r{make_uint4(0u, 0u, 0u, 0u), make_uint2(0u, 0u), 0u};
+    TailItem r{make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u), 0u};
     const uint32_t nv = s_vf[l + 1] - s_vf[l], nf = s_fv[l + 1] - s_fv[l];
     if (x < nv) {
         r.it = A.vf_items[s_vf[l] + x];
         r.kind = 1u;
     } else if (x - nv < nf) {
         r.it = A.fv_items[s_fv[l] + x - nv];
-        r.dt = A.fv_dt[s_fv[l] + x - nv];
+        r.aux = A.fv_aux[s_fv[l] + x - nv];
         r.kind = 2u;
     }
     return r;
@@ -183,7 +178,7 @@ __device__ __forceinline__ TailItem tail_fetch(const SeqArgs &A, const uint32_t 
 
 __device__ __forceinline__ void tail_run(const SeqArgs &A, const TailItem &t) {
     if (t.kind == 1u) seq_vf_item(A, t.it);
-    else if (t.kind == 2u) seq_fv_item(A, t.it, t.dt);
+    else if (t.kind == 2u) seq_fv_item(A, t.it, t.aux);
 }
 
 /* levels [l0, l1) by one workgroup, at most kSeqTailPer * kSeqTailBlock
@@ -209,13 +204,29 @@ __global__ __launch_bounds__(kSeqTailBlock) __attribute__((unused)) void seq_tai
 #pragma unroll
         for (int j = 0; j < kSeqTailPer; j++)
             next[j] = l + 1 < n ? tail_fetch(A, s_vf, s_fv, l + 1, threadIdx.x + j * kSeqTailBlock)
-                                : TailItem{make_uint4(0u, 0u, 0u, 0u), make_uint2(0u, 0u), 0u};
+                                : TailItem{make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u), 0u};
 #pragma unroll
         for (int j = 0; j < kSeqTailPer; j++) tail_run(A, cur[j]);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kSeqTailPer; j++) cur[j] = next[j];
     }
+}
+
+/* the factor records from the plan's arrays */
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_fac_build(
+    const uint32_t *__restrict__ fac_meta, const double *__restrict__ fac_p, const double *__restrict__ fac_q,
+    const double2 *__restrict__ fac_mask, FacRec *__restrict__ out, uint32_t F) {
+    const uint32_t I = blockIdx.x * kBlock + threadIdx.x;
+    if (I >= F) return;
+    const uint32_t meta = fac_meta[I];
+    FacRec r{fac_p[I], (meta & 3u) ? fac_q[I] : 0.0, 1.0, 1.0};
+    if (meta & 4u) {
+        const double2 mk = fac_mask[I];
+        r.m0 = mk.x;
+        r.m1 = mk.y;
+    }
+    out[I] = r;
 }
 
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_beliefs(
