@@ -389,6 +389,11 @@ def bench_seq(args):
         pp.close()
         log(f"[bench] PARALL damping {damping}: {r.sweeps} sweeps, maxdiff {r.maxdiff:.3g}, "
             f"{par[str(damping)]['ms_per_solve']:.2f} ms/solve")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_seq_seqfix_1M.json")
+    if args.updates == "SEQFIX" and n_vars == WORKLOAD["n_vars"] and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("seq_level")
     out = {"metric": METRIC, "value": value, "unit": "msg-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_solve,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -402,7 +407,7 @@ def bench_seq(args):
                       "step": "init + run() to convergence under the sequential schedule"},
            "roofline": {"bound": "hbm", "kernel": "seq_level (all levels of a sweep)",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "algorithmic_bytes_per_sweep": lb, "level_us_per_sweep": round(lvl_us, 2),
                         "belief_us_per_sweep": round(bel_us, 2),
                         "timing": "HIP events around the level launches of each sweep in profiled solves"},
