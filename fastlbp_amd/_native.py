@@ -106,6 +106,7 @@ class ScheduleInfoC(C.Structure):
         ("n_levels", C.c_uint64),
         ("updates", C.c_int32),
         ("respect_preorder", C.c_int32),
+        ("n_exec_items", C.c_uint64),
     ]
 
 
@@ -190,6 +191,8 @@ SIGNATURES = {
     "flbp_schedule_order": (C.c_int, [C.c_void_p, u64p]),
     "flbp_schedule_waves": (C.c_int, [C.c_void_p, u64p, u64p, u64p, u64p]),
     "flbp_schedule_levels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "flbp_schedule_exec": (C.c_int, [C.c_void_p, u8p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "flbp_seed": (None, [C.c_uint32]),
     "flbp_plan_set_schedule": (C.c_int, [C.c_void_p, C.c_void_p]),
     "flbp_plan_get_updates": (C.c_int, [C.c_void_p]),

@@ -13,6 +13,10 @@
 
 namespace flbp {
 
+/* factors up to this degree run a sequential wave's updates as one group
+ * item (seq_host.cpp, seq_kernels.h) */
+constexpr uint32_t kSeqGroupDeg = 6;
+
 /* structural checks of a CSR (plan_host.hip) */
 int validate_csr(const flbp_csr &c);
 

@@ -11,6 +11,11 @@
  *   seq_fv     u32x4[n_fv]  f->v items (var-major position, variable, factor,
  *              first slot of the factor)
  *   seq_fv_aux u32x4[n_fv]  (degree | factor type << 24, target slot, head slot, 0)
+ *   seq_grp    u32x4[2 * lanes]  group items, one lane per slot: (factor,
+ *              first slot, degree | type << 24, head slot | updated mask << 8
+ *              | recomputed mask << 16 | slot << 24), (variable, var-major
+ *              position, 0, 0); a group never straddles a 64-lane wave and
+ *              each level's lanes are padded to whole waves (degree 0)
  *   seq_fac    {f64 p, q, m0, m1}[F]  factor records, rebuilt on the device
  *              from fac_p / fac_q / fac_meta / fac_mask whenever they change
  *   seq_lvl_*_dev u32[levels+1]  first item of each level
@@ -58,35 +63,39 @@ namespace flbp {
     } while (0)
 
 void Plan::free_seq_items() {
-    for (void *p : {(void *)seq_vf, (void *)seq_fv, (void *)seq_fv_aux, (void *)seq_lvl_vf_dev,
-                    (void *)seq_lvl_fv_dev})
+    for (void *p : {(void *)seq_vf, (void *)seq_fv, (void *)seq_fv_aux, (void *)seq_grp, (void *)seq_lvl_vf_dev,
+                    (void *)seq_lvl_fv_dev, (void *)seq_lvl_grp_dev})
         if (p) (void)hipFree(p);
-    seq_vf = seq_fv = seq_fv_aux = nullptr;
-    seq_lvl_vf_dev = seq_lvl_fv_dev = nullptr;
+    seq_vf = seq_fv = seq_fv_aux = seq_grp = nullptr;
+    seq_lvl_vf_dev = seq_lvl_fv_dev = seq_lvl_grp_dev = nullptr;
     seq_lvl_vf.clear();
     seq_lvl_fv.clear();
+    seq_lvl_grp.clear();
     seq_tail_from = 0;
 }
 
 /* One sweep: every level, then beliefs and the max-diff of sweep k. */
 int Plan::launch_seq_sweep(int k, double tol) {
     const size_t L = seq_lvl_fv.empty() ? 0 : seq_lvl_fv.size() - 1;
-    SeqArgs A{seq_vf, seq_fv, seq_fv_aux, seq_fac, m_vf, m_fv_vm, seq_state};
+    SeqArgs A{seq_vf, seq_fv, seq_fv_aux, seq_grp, seq_fac, m_vf, m_fv_vm, seq_state};
     for (size_t l = 0; l < seq_tail_from; l++) {
         const uint32_t v0 = seq_lvl_vf[l], nv = seq_lvl_vf[l + 1] - v0;
         const uint32_t f0 = seq_lvl_fv[l], nf = seq_lvl_fv[l + 1] - f0;
-        if (!nv && !nf) continue;
-        const unsigned vb = grid_for(nv, dev::kBlock), fb = grid_for(nf, dev::kBlock);
+        const uint32_t g0 = seq_lvl_grp[l], ng = seq_lvl_grp[l + 1] - g0;
+        if (!nv && !nf && !ng) continue;
+        const unsigned vb = grid_for(nv, dev::kBlock), fb = grid_for(nf, dev::kBlock), gb = grid_for(ng, dev::kBlock);
         SeqArgs a = A;
         a.vf_items = seq_vf + v0;
         a.fv_items = seq_fv + f0;
         a.fv_aux = seq_fv_aux + f0;
-        hipLaunchKernelGGL(seq_level, dim3(vb + fb), dim3(dev::kBlock), 0, stream, a, nv, nf, vb, ctl, k, tol);
+        a.grp_items = seq_grp + 2 * (uint64_t)g0;
+        hipLaunchKernelGGL(seq_level, dim3(vb + fb + gb), dim3(dev::kBlock), 0, stream, a, nv, nf, ng, vb, fb, ctl,
+                           k, tol);
     }
     for (size_t l0 = seq_tail_from; l0 < L; l0 += dev::kSeqTailLevels) {
         const size_t l1 = std::min(L, l0 + dev::kSeqTailLevels);
         hipLaunchKernelGGL(seq_tail, dim3(1), dim3(dev::kSeqTailBlock), 0, stream, A, seq_lvl_vf_dev,
-                           seq_lvl_fv_dev, (uint32_t)l0, (uint32_t)l1, ctl, k, tol);
+                           seq_lvl_fv_dev, seq_lvl_grp_dev, (uint32_t)l0, (uint32_t)l1, ctl, k, tol);
     }
     if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
     if (V)
@@ -124,46 +133,83 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     if (P->variant != 3) {
         if (int rc = flbp_plan_set_variant(plan, 3)) return rc;
     }
-    /* items grouped by level (counting sort, stable in wave order), then
-     * ordered by slot / factor inside each level */
+    /* the schedule's execution items, grouped by kind and level (counting
+     * sort), then ordered by slot / factor inside each level */
     const uint32_t nl = S.n_levels;
-    std::vector<uint32_t> lv(nl + 2, 0), lf(nl + 2, 0);
-    for (uint32_t l : S.vf_level) lv[l + 1]++;
-    for (uint32_t l : S.fv_level) lf[l + 1]++;
-    for (uint32_t l = 0; l <= nl; l++) {
-        lv[l + 1] += lv[l];
-        lf[l + 1] += lf[l];
-    }
-    /* f->v items as (key = factor, edge; payload = position in wave order) */
-    std::vector<uint4> vf(S.vf_slot.size());
-    std::vector<std::pair<uint64_t, uint32_t>> fk(S.fv_edge.size());
+    const size_t nx = S.ex_kind.size();
+    std::vector<uint32_t> off[3];
+    for (auto &o : off) o.assign(nl + 2, 0);
+    for (size_t x = 0; x < nx; x++) off[S.ex_kind[x]][S.ex_level[x] + 1]++;
+    for (auto &o : off)
+        for (uint32_t l = 0; l <= nl; l++) o[l + 1] += o[l];
+    std::vector<uint32_t> slot_fac(P->E);
+    for (uint64_t I = 0; I < P->F; I++)
+        for (uint64_t sl = P->h_fac_off[I]; sl < P->h_fac_off[I + 1]; sl++) slot_fac[sl] = (uint32_t)I;
+    /* per kind: (sort key, execution item) in level order */
+    std::vector<std::pair<uint64_t, uint32_t>> keyed[3];
+    for (int kd = 0; kd < 3; kd++) keyed[kd].resize(off[kd][nl + 1]);
     {
-        std::vector<uint32_t> at(lv.begin(), lv.end());
-        for (size_t x = 0; x < S.vf_slot.size(); x++)
-            vf[at[S.vf_level[x]]++] = make_uint4(S.vf_slot[x], S.vf_pos[x], S.vf_var[x], 0u);
-        std::vector<uint32_t> bt(lf.begin(), lf.end());
-        for (size_t x = 0; x < S.fv_edge.size(); x++)
-            fk[bt[S.fv_level[x]]++] = {((uint64_t)S.fv_fac[x] << 32) | S.fv_edge[x], (uint32_t)x};
+        std::vector<uint32_t> at[3];
+        for (int kd = 0; kd < 3; kd++) at[kd].assign(off[kd].begin(), off[kd].end());
+        for (size_t x = 0; x < nx; x++) {
+            const int kd = S.ex_kind[x];
+            const uint64_t key = kd == flbp::kExecFV ? ((uint64_t)slot_fac[P->h_var_edge[S.ex_a[x]]] << 32) | S.ex_a[x]
+                                                     : (uint64_t)S.ex_a[x];
+            keyed[kd][at[kd][S.ex_level[x]]++] = {key, (uint32_t)x};
+        }
     }
-    for (uint32_t l = 1; l <= nl; l++) {
-        std::sort(vf.begin() + lv[l], vf.begin() + lv[l + 1], [](const uint4 &a, const uint4 &b) { return a.x < b.x; });
-        std::sort(fk.begin() + lf[l], fk.begin() + lf[l + 1]);
+    for (int kd = 0; kd < 3; kd++)
+        for (uint32_t l = 1; l <= nl; l++) std::sort(keyed[kd].begin() + off[kd][l], keyed[kd].begin() + off[kd][l + 1]);
+    std::vector<uint4> vf(keyed[0].size()), fv(keyed[1].size()), faux(keyed[1].size());
+    for (size_t y = 0; y < keyed[0].size(); y++) {
+        const uint32_t sl = S.ex_a[keyed[0][y].second];
+        vf[y] = make_uint4(sl, S.slot_pos[sl], S.slot_var[sl], 0u);
     }
-    std::vector<uint4> fv(fk.size()), faux(fk.size());
-    for (size_t y = 0; y < fk.size(); y++) {
-        const uint32_t x = fk[y].second, I = S.fv_fac[x];
+    for (size_t y = 0; y < keyed[1].size(); y++) {
+        const uint32_t e = S.ex_a[keyed[1][y].second];
+        const uint32_t own = (uint32_t)P->h_var_edge[e], I = slot_fac[own];
         const uint32_t a = (uint32_t)P->h_fac_off[I], d = (uint32_t)(P->h_fac_off[I + 1] - a);
         const uint32_t meta = P->h_meta[I];   /* type and head slot never change */
-        fv[y] = make_uint4(S.fv_edge[x], S.fv_var[x], I, a);
-        faux[y] = make_uint4(d | ((meta & 3u) << 24), S.fv_slot[x] - a, meta >> 8, 0u);
+        fv[y] = make_uint4(e, S.slot_var[own], I, a);
+        faux[y] = make_uint4(d | ((meta & 3u) << 24), own - a, meta >> 8, 0u);
+    }
+    /* group items: one lane (two words) per slot, a group's lanes inside
+     * one 64-lane wave, each level padded to whole waves; glane[l] = first
+     * lane of level l */
+    std::vector<uint4> grp;
+    std::vector<uint32_t> glane(nl + 2, 0);
+    for (uint32_t l = 1; l <= nl; l++) {
+        const size_t base = grp.size() / 2;
+        auto pad_to_wave = [&]() {
+            while ((grp.size() / 2 - base) % 64) {
+                grp.push_back(make_uint4(0u, 0u, 0u, 0u));
+                grp.push_back(make_uint4(0u, 0u, 0u, 0u));
+            }
+        };
+        for (uint32_t y = off[2][l]; y < off[2][l + 1]; y++) {
+            const size_t x = keyed[2][y].second;
+            const uint32_t I = S.ex_a[x];
+            const uint32_t a = (uint32_t)P->h_fac_off[I], d = (uint32_t)(P->h_fac_off[I + 1] - a);
+            const uint32_t meta = P->h_meta[I];
+            if ((grp.size() / 2 - base) % 64 + d > 64) pad_to_wave();
+            for (uint32_t k = 0; k < d; k++) {
+                grp.push_back(make_uint4(I, a, d | ((meta & 3u) << 24),
+                                         (meta >> 8) | (S.ex_b[x] << 8) | (S.ex_c[x] << 16) | (k << 24)));
+                grp.push_back(make_uint4(S.slot_var[a + k], S.slot_pos[a + k], 0u, 0u));
+            }
+        }
+        pad_to_wave();
+        if (grp.size() / 2 >= 0xffffffffull)
+            return flbp::fail(FLBP_EINVAL, "flbp_plan_set_schedule: too many group lanes for 32-bit offsets");
+        glane[l + 1] = (uint32_t)(grp.size() / 2);
     }
     P->free_seq_items();
-    const size_t nvf = std::max<size_t>(vf.size(), 1), nfv = std::max<size_t>(fv.size(), 1);
-    if (hipMalloc(&P->seq_vf, nvf * sizeof(uint4)) != hipSuccess ||
-        hipMalloc(&P->seq_fv, nfv * sizeof(uint4)) != hipSuccess ||
-        hipMalloc(&P->seq_fv_aux, nfv * sizeof(uint4)) != hipSuccess ||
-        hipMalloc(&P->seq_lvl_vf_dev, (nl + 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&P->seq_lvl_fv_dev, (nl + 1) * sizeof(uint32_t)) != hipSuccess) {
+    auto dev_alloc = [&](auto *&ptr, size_t n) {
+        return hipMalloc((void **)&ptr, std::max<size_t>(n, 1) * sizeof(*ptr)) == hipSuccess;
+    };
+    if (!dev_alloc(P->seq_vf, vf.size()) || !dev_alloc(P->seq_fv, fv.size()) || !dev_alloc(P->seq_fv_aux, faux.size()) ||
+        !dev_alloc(P->seq_grp, grp.size()) || !dev_alloc(P->seq_lvl_vf_dev, nl + 1) ||
+        !dev_alloc(P->seq_lvl_fv_dev, nl + 1) || !dev_alloc(P->seq_lvl_grp_dev, nl + 1)) {
         (void)hipGetLastError();
         P->free_seq_items();
         return flbp::fail(FLBP_ENOMEM, "flbp_plan_set_schedule: item upload allocation failed");
@@ -173,16 +219,21 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
         HIP_TRY(hipMemcpy(P->seq_fv, fv.data(), fv.size() * sizeof(uint4), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(P->seq_fv_aux, faux.data(), faux.size() * sizeof(uint4), hipMemcpyHostToDevice));
     }
+    if (!grp.empty()) HIP_TRY(hipMemcpy(P->seq_grp, grp.data(), grp.size() * sizeof(uint4), hipMemcpyHostToDevice));
     /* level l (1-based) -> index l-1 */
-    P->seq_lvl_vf.assign(lv.begin() + 1, lv.end());
-    P->seq_lvl_fv.assign(lf.begin() + 1, lf.end());
+    P->seq_lvl_vf.assign(off[0].begin() + 1, off[0].end());
+    P->seq_lvl_fv.assign(off[1].begin() + 1, off[1].end());
+    P->seq_lvl_grp.assign(glane.begin() + 1, glane.end());
     HIP_TRY(hipMemcpy(P->seq_lvl_vf_dev, P->seq_lvl_vf.data(), (nl + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(P->seq_lvl_fv_dev, P->seq_lvl_fv.data(), (nl + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->seq_lvl_grp_dev, P->seq_lvl_grp.data(), (nl + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
     /* the tail: the longest run of trailing levels with few items each */
+    auto level_items = [&](size_t l) {   /* items of 0-based level l */
+        return (P->seq_lvl_vf[l + 1] - P->seq_lvl_vf[l]) + (P->seq_lvl_fv[l + 1] - P->seq_lvl_fv[l]) +
+               (P->seq_lvl_grp[l + 1] - P->seq_lvl_grp[l]);
+    };
     size_t tail = nl;
-    while (tail > 0 && (P->seq_lvl_vf[tail] - P->seq_lvl_vf[tail - 1]) + (P->seq_lvl_fv[tail] - P->seq_lvl_fv[tail - 1]) <=
-                           kSeqTailItems)
-        tail--;
+    while (tail > 0 && level_items(tail - 1) <= kSeqTailItems) tail--;
     P->seq_tail_from = tail;
     if (!P->seq_state) {
         if (int rc = P->alloc(P->seq_state, P->V)) return rc;

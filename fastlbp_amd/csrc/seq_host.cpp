@@ -196,42 +196,121 @@ int build_schedule(const flbp_csr &c, const uint64_t *order, uint64_t n, bool re
         if (S.vf_slot.size() + S.fv_edge.size() + (c.fac_off[I + 1] - c.fac_off[I]) >= (1ull << 32) - 1)
             return fail(FLBP_ERANGE, "schedule: more than 2^32-2 items");
         S.fv_edge.push_back((uint32_t)e);
-        S.fv_slot.push_back((uint32_t)c.var_edge[e]);
-        S.fv_fac.push_back(I);
-        S.fv_var.push_back((uint32_t)i);
         for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++) {
             if (c.edge_var[s] == i || vf_stamp[s] == wave) continue;
             vf_stamp[s] = wave;
             S.vf_slot.push_back((uint32_t)s);
-            S.vf_pos.push_back((uint32_t)pos[s]);
-            S.vf_var.push_back((uint32_t)c.edge_var[s]);
         }
     }
     S.fv_wave_off.push_back(S.fv_edge.size());
     S.vf_wave_off.push_back(S.vf_slot.size());
     S.n_waves = S.fv_wave_off.size() - 1;
 
-    /* dependency levels, items visited in wave order (a wave's v->f items
-     * before its f->v items).  Locations: a variable's state (its product,
-     * zero counts and the f->v messages into it: written only by f->v
-     * updates into it, read by v->f messages out of it) and a factor-major
-     * v->f slot (written by its v->f item, read by the factor's f->v
-     * updates to its other slots). */
+    /* Execution items and their dependency levels.  Per wave, the f->v
+     * updates of one factor (of degree <= kSeqGroupDeg) form one group
+     * item, which also recomputes the wave's v->f messages of that factor
+     * -- from the variables' states as they stand before the wave, since it
+     * reads them before its own updates -- unless another factor updates
+     * that variable in the same wave: such a v->f message stays a separate
+     * item.  Wider factors keep one item per v->f and per f->v message.
+     *
+     * Levels are assigned in wave order (a wave's separate v->f items
+     * before the rest).  Locations: a variable's state (product, zero
+     * counts, the f->v messages into it) and a factor-major v->f slot.  An
+     * item's level is one above every earlier item that writes what it
+     * reads or reads or writes what it writes.  Within a wave the groups
+     * and wide f->v items touch disjoint locations: each variable is
+     * updated at most once, a group reads only states no other item of the
+     * wave writes, and each factor's slots belong to one item. */
+    S.slot_var.assign(c.edge_var, c.edge_var + E);
+    S.slot_pos.assign(pos.begin(), pos.end());
     std::vector<uint32_t> w_var(V, 0), r_var(V, 0), w_slot(E, 0), r_slot(E, 0);
-    S.vf_level.resize(S.vf_slot.size());
-    S.fv_level.resize(S.fv_edge.size());
+    S.vf_level.assign(S.vf_slot.size(), 0);
+    S.fv_level.assign(S.fv_edge.size(), 0);
+    std::vector<uint32_t> tgt_stamp(V, 0), tgt_fac(V, 0);   /* factor updating the variable in the wave */
+    std::vector<uint32_t> grp_stamp(c.n_factors, 0), grp_of(c.n_factors, 0);
+    struct Group {
+        uint32_t I, tmask, imask;
+        std::vector<uint64_t> fv_x, vf_x;   /* reference items folded in */
+    };
+    std::vector<Group> groups;
+    std::vector<uint64_t> wide_fv;
     uint32_t top = 0;
+    auto emit = [&](uint8_t kind, uint32_t a, uint32_t b, uint32_t cc, uint32_t L) {
+        S.ex_kind.push_back(kind);
+        S.ex_a.push_back(a);
+        S.ex_b.push_back(b);
+        S.ex_c.push_back(cc);
+        S.ex_level.push_back(L);
+        top = std::max(top, L);
+    };
     for (uint64_t w = 0; w < S.n_waves; w++) {
+        const uint32_t stamp = (uint32_t)w + 1;
+        groups.clear();
+        wide_fv.clear();
+        for (uint64_t x = S.fv_wave_off[w]; x < S.fv_wave_off[w + 1]; x++) {
+            const uint32_t e = S.fv_edge[x];
+            const uint32_t own = (uint32_t)c.var_edge[e], I = sf[own];
+            tgt_stamp[vm_var[e]] = stamp;
+            tgt_fac[vm_var[e]] = I;
+            if (c.fac_off[I + 1] - c.fac_off[I] > kSeqGroupDeg) {
+                wide_fv.push_back(x);
+                continue;
+            }
+            if (grp_stamp[I] != stamp) {
+                grp_stamp[I] = stamp;
+                grp_of[I] = (uint32_t)groups.size();
+                groups.push_back({I, 0u, 0u, {}, {}});
+            }
+            Group &G = groups[grp_of[I]];
+            G.tmask |= 1u << (own - c.fac_off[I]);
+            G.fv_x.push_back(x);
+        }
+        /* the wave's v->f messages: folded into their factor's group when
+         * safe, else separate items (levels first, as they run first) */
         for (uint64_t x = S.vf_wave_off[w]; x < S.vf_wave_off[w + 1]; x++) {
-            const uint32_t s = S.vf_slot[x];
+            const uint32_t s = S.vf_slot[x], I = sf[s];
             const uint64_t j = c.edge_var[s];
+            const bool wide = c.fac_off[I + 1] - c.fac_off[I] > kSeqGroupDeg;
+            if (!wide && !(tgt_stamp[j] == stamp && tgt_fac[j] != I)) {
+                Group &G = groups[grp_of[I]];
+                G.imask |= 1u << (s - c.fac_off[I]);
+                G.vf_x.push_back(x);
+                continue;
+            }
             const uint32_t L = std::max(w_var[j], std::max(w_slot[s], r_slot[s])) + 1;
             r_var[j] = std::max(r_var[j], L);
             w_slot[s] = L;
             S.vf_level[x] = L;
-            top = std::max(top, L);
+            emit(kExecVF, s, 0u, 0u, L);
         }
-        for (uint64_t x = S.fv_wave_off[w]; x < S.fv_wave_off[w + 1]; x++) {
+        for (const Group &G : groups) {
+            const uint64_t a = c.fac_off[G.I], d = c.fac_off[G.I + 1] - a;
+            const bool multi = (G.tmask & (G.tmask - 1)) != 0;
+            uint32_t L = 0;
+            for (uint64_t k = 0; k < d; k++) {
+                const uint64_t s = a + k, j = c.edge_var[s];
+                const bool inl = (G.imask >> k) & 1u, tgt = (G.tmask >> k) & 1u;
+                if (inl) L = std::max(L, std::max(w_var[j], std::max(w_slot[s], r_slot[s])));
+                if (tgt) L = std::max(L, std::max(w_var[j], r_var[j]));
+                if (!inl && (multi || !tgt)) L = std::max(L, w_slot[s]);   /* read from m_vf */
+            }
+            L += 1;
+            for (uint64_t k = 0; k < d; k++) {
+                const uint64_t s = a + k, j = c.edge_var[s];
+                const bool inl = (G.imask >> k) & 1u, tgt = (G.tmask >> k) & 1u;
+                if (inl) {
+                    r_var[j] = std::max(r_var[j], L);
+                    w_slot[s] = L;
+                }
+                if (!inl && (multi || !tgt)) r_slot[s] = std::max(r_slot[s], L);
+                if (tgt) w_var[j] = L;
+            }
+            for (uint64_t x : G.fv_x) S.fv_level[x] = L;
+            for (uint64_t x : G.vf_x) S.vf_level[x] = L;
+            emit(kExecGroup, G.I, G.tmask, G.imask, L);
+        }
+        for (uint64_t x : wide_fv) {
             const uint32_t e = S.fv_edge[x];
             const uint64_t i = vm_var[e], own = c.var_edge[e];
             const uint32_t I = sf[own];
@@ -243,7 +322,7 @@ int build_schedule(const flbp_csr &c, const uint64_t *order, uint64_t n, bool re
                 if (s != own) r_slot[s] = std::max(r_slot[s], L);
             w_var[i] = L;
             S.fv_level[x] = L;
-            top = std::max(top, L);
+            emit(kExecFV, e, 0u, 0u, L);
         }
     }
     S.n_levels = top;
@@ -307,6 +386,19 @@ int flbp_schedule_get_info(const flbp_schedule *h, flbp_schedule_info *info) {
     info->n_levels = h->s.n_levels;
     info->updates = h->s.updates;
     info->respect_preorder = h->s.respect_preorder ? 1 : 0;
+    info->n_exec_items = h->s.ex_kind.size();
+    return FLBP_OK;
+}
+
+int flbp_schedule_exec(const flbp_schedule *h, uint8_t *kind, uint32_t *a, uint32_t *b, uint32_t *c,
+                       uint32_t *level) {
+    if (!h) return flbp::fail(FLBP_EINVAL, "flbp_schedule_exec: null schedule");
+    const flbp::Schedule &S = h->s;
+    if (kind) std::copy(S.ex_kind.begin(), S.ex_kind.end(), kind);
+    if (a) std::copy(S.ex_a.begin(), S.ex_a.end(), a);
+    if (b) std::copy(S.ex_b.begin(), S.ex_b.end(), b);
+    if (c) std::copy(S.ex_c.begin(), S.ex_c.end(), c);
+    if (level) std::copy(S.ex_level.begin(), S.ex_level.end(), level);
     return FLBP_OK;
 }
 

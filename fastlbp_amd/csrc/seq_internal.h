@@ -31,12 +31,23 @@ struct Schedule {
     std::vector<uint64_t> fv_wave_off, vf_wave_off;
     std::vector<uint32_t> fv_edge, vf_slot;
     std::vector<uint32_t> fv_level, vf_level;
-    /* per item, what the kernels need: v->f item -> var-major position of
-     * its slot and its variable; f->v item -> its slot, factor, variable */
-    std::vector<uint32_t> vf_pos, vf_var;
-    std::vector<uint32_t> fv_slot, fv_fac, fv_var;
     uint32_t n_levels = 0;
+
+    /* What runs on the device, each with its level:
+     *   kExecVF    a = factor-major slot: one v->f message
+     *   kExecFV    a = var-major edge: one f->v update (factors wider than
+     *              kSeqGroupDeg)
+     *   kExecGroup a = factor, b = mask of its slots updated (f->v), c = mask
+     *              of its slots whose v->f message it recomputes first, from
+     *              the variables' states before its own updates
+     * fv_level / vf_level give the level at which each reference item is
+     * computed (a v->f message folded into a group takes the group's). */
+    std::vector<uint8_t> ex_kind;
+    std::vector<uint32_t> ex_a, ex_b, ex_c, ex_level;
+    std::vector<uint32_t> slot_var, slot_pos;   /* variable and var-major position of each slot */
 };
+
+enum : uint8_t { kExecVF = 0, kExecFV = 1, kExecGroup = 2 };
 
 void seqfix_order(const flbp_csr &c, std::mt19937 &g, std::vector<uint64_t> &out);
 void topo_order(const flbp_csr &c, std::vector<uint64_t> &out);

@@ -28,6 +28,17 @@
  * is exact and the zero-mass fallbacks test the mask for 0, so applying
  * the mask always gives the unclamped codes' results bit for bit.
  *
+ *   group item (a factor of at most kSeqGroupDeg slots in one wave, with
+ *     the slots it updates and the slots whose v->f message it recomputes
+ *     first): one lane per slot, the factor's lanes side by side in one
+ *     hardware wave.  A recomputing lane derives its slot's v->f message
+ *     from the variable's state as loaded and stores it to m_vf; the other
+ *     lanes load theirs from m_vf; the lanes then exchange the messages
+ *     (ds_bpermute) and each updating lane runs its f->v update.  Every
+ *     state read of the item is issued before the exchange and every state
+ *     write after it, so the item's reads see the states from before its
+ *     own updates, as the wave's v->f kernel comes before its f->v kernel.
+ *
  * Items of one level touch disjoint outputs and read nothing another item
  * of the level writes, so lanes need no ordering among themselves.
  *
@@ -49,13 +60,14 @@
 #include "staged_kernels.h"
 #include "sweep_kernels.h"
 
+#include "flbp_internal.h"
 #include "plan_internal.h"
 
 namespace flbp {
 namespace dev {
 
 constexpr int kSeqRegDeg = 8;       /* factors up to this degree keep messages in registers */
-constexpr int kSeqTailBlock = 1024; /* threads of the tail kernel */
+constexpr int kSeqTailBlock = 512;  /* threads of the tail kernel (two waves per SIMD: 256 VGPRs each) */
 constexpr int kSeqTailLevels = 256; /* level offsets the tail kernel keeps in LDS */
 constexpr int kSeqTailPer = 2;      /* items per thread and level in the tail kernel */
 
@@ -91,6 +103,7 @@ struct SeqArgs {
     const uint4 *__restrict__ vf_items;   /* slot, var-major position, variable */
     const uint4 *__restrict__ fv_items;   /* var-major position, variable, factor, first slot */
     const uint4 *__restrict__ fv_aux;     /* degree | type << 24, target slot, head slot */
+    const uint4 *__restrict__ grp_items;  /* 2 words per group lane (see seq_group_lane) */
     const FacRec *__restrict__ fac;       /* per-factor p, q, head mask */
     double2 *__restrict__ m_vf;           /* factor-major */
     double2 *__restrict__ m_fv_vm;        /* var-major */
@@ -136,9 +149,69 @@ __device__ __forceinline__ void seq_fv_item(const SeqArgs &A, const uint4 it, co
     A.m_fv_vm[it.x] = r;
 }
 
+constexpr int kG = (int)flbp::kSeqGroupDeg;
+
+/* The messages of the group's slots, slot j held by lane base + j of the
+ * wave: every lane of the wave takes part (ds_bpermute reads other lanes'
+ * registers), indices past the group read some other lane's value, which
+ * the f->v code never uses (it reads slots below the degree only). */
+__device__ __forceinline__ RegIn<kG> group_messages(double2 m, int base) {
+    RegIn<kG> in;
+#pragma unroll
+    for (int j = 0; j < kG; j++) {
+        in.m[j].x = __shfl(m.x, base + j, 64);
+        in.m[j].y = __shfl(m.y, base + j, 64);
+    }
+    return in;
+}
+
+/* One lane of a group item, slot k of factor I: w0 = (I, first slot a,
+ * degree | type << 24, head slot | updated mask << 8 | recomputed mask << 16
+ * | k << 24), w1 = (variable, var-major position) of slot k; degree 0 marks
+ * a padding lane.  The lanes of one group sit side by side in one wave and
+ * all lanes of the wave run this together. */
+__device__ __forceinline__ void seq_group_lane(const SeqArgs &A, const uint4 w0, const uint4 w1) {
+    const uint32_t I = w0.x, a = w0.y;
+    const int d = (int)(w0.z & 0xffffffu), h = (int)(w0.w & 0xffu), k = (int)(w0.w >> 24);
+    const uint32_t type = w0.z >> 24;
+    const bool tk = (w0.w >> (8 + k)) & 1u, ik = (w0.w >> (16 + k)) & 1u;   /* masks are 0 on padding */
+    /* everything the lane names, independent of each other */
+    double2 m = make_double2(0.0, 0.0), old = make_double2(0.0, 0.0);
+    VarState st{0.0, 0.0, 0ull, 0ull};
+    FacRec f{0.0, 0.0, 0.0, 0.0};
+    if (d > 0 && !ik) m = A.m_vf[a + k];
+    if (tk || ik) {
+        st = A.state[w1.x];
+        old = A.m_fv_vm[w1.y];
+    }
+    if (tk) f = A.fac[I];
+    /* slot k's v->f message of this wave, from the state before the
+     * factor's own updates */
+    if (ik) {
+        m = vf_msg_seq(st, old);
+        A.m_vf[a + k] = m;
+    }
+    const RegIn<kG> in = group_messages(m, (int)(threadIdx.x & 63u) - k);
+    if (!tk) return;
+    double2 r;
+    if (type == 0) {
+        r.x = 1 - f.p;
+        r.y = f.p;
+    } else {
+        r = fv_msg<kG, true>(in, type, true, d, k, h, in.pick(h), in.pick(h == 0 ? 1 : 0), f.p, f.q, f.m0, f.m1);
+    }
+    if (old.x == 0) st.z0--; else st.p0 /= old.x;
+    if (r.x == 0) st.z0++; else st.p0 *= r.x;
+    if (old.y == 0) st.z1--; else st.p1 /= old.y;
+    if (r.y == 0) st.z1++; else st.p1 *= r.y;
+    scale_seq(st.p0, st.p1);
+    A.state[w1.x] = st;
+    A.m_fv_vm[w1.y] = r;
+}
+
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_level(
-    SeqArgs A, uint32_t n_vf, uint32_t n_fv, uint32_t vf_blocks, const unsigned long long *__restrict__ ctl, int k,
-    double tol) {
+    SeqArgs A, uint32_t n_vf, uint32_t n_fv, uint32_t n_grp, uint32_t vf_blocks, uint32_t fv_blocks,
+    const unsigned long long *__restrict__ ctl, int k, double tol) {
     if (blockIdx.x < vf_blocks) {
         const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
         const uint4 it = x < n_vf ? A.vf_items[x] : make_uint4(0u, 0u, 0u, 0u);
@@ -146,65 +219,91 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_level(
         seq_vf_item(A, it);
         return;
     }
-    const uint32_t x = (blockIdx.x - vf_blocks) * kBlock + threadIdx.x;
-    const uint4 it = x < n_fv ? A.fv_items[x] : make_uint4(0u, 0u, 0u, 0u);
-    const uint4 aux = x < n_fv ? A.fv_aux[x] : make_uint4(0u, 0u, 0u, 0u);
-    if (chunk_stopped(ctl, k, tol) || x >= n_fv) return;
-    seq_fv_item(A, it, aux);
+    if (blockIdx.x < vf_blocks + fv_blocks) {
+        const uint32_t x = (blockIdx.x - vf_blocks) * kBlock + threadIdx.x;
+        const uint4 it = x < n_fv ? A.fv_items[x] : make_uint4(0u, 0u, 0u, 0u);
+        const uint4 aux = x < n_fv ? A.fv_aux[x] : make_uint4(0u, 0u, 0u, 0u);
+        if (chunk_stopped(ctl, k, tol) || x >= n_fv) return;
+        seq_fv_item(A, it, aux);
+        return;
+    }
+    /* group lanes: n_grp is a multiple of 64, so a wave is wholly inside
+     * or wholly past the level and every lane of a running wave reaches
+     * the exchange */
+    const uint32_t x = (blockIdx.x - vf_blocks - fv_blocks) * kBlock + threadIdx.x;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    const uint4 w0 = x < n_grp ? A.grp_items[2 * (uint64_t)x] : z;
+    const uint4 w1 = x < n_grp ? A.grp_items[2 * (uint64_t)x + 1] : z;
+    if (chunk_stopped(ctl, k, tol) || x >= n_grp) return;
+    seq_group_lane(A, w0, w1);
 }
 
-/* one item of a tail level: x indexes the level's v->f items, then its
- * f->v items; kind 0 = none, 1 = v->f, 2 = f->v */
+/* one item of a tail level: x indexes the level's group lanes (a multiple
+ * of 64, so each wave holds group lanes only or none), then its v->f items,
+ * then its f->v items; kind 0 = none, 1 = v->f, 2 = f->v, 3 = group lane */
 struct TailItem {
-    uint4 it;
-    uint4 aux;
+    uint4 w[2];
     uint32_t kind;
 };
 
+__device__ __forceinline__ TailItem tail_none() {
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    return TailItem{{z, z}, 0u};
+}
+
 __device__ __forceinline__ TailItem tail_fetch(const SeqArgs &A, const uint32_t *s_vf, const uint32_t *s_fv,
-                                               uint32_t l, uint32_t x) {
-    TailItem r{make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u), 0u};
-    const uint32_t nv = s_vf[l + 1] - s_vf[l], nf = s_fv[l + 1] - s_fv[l];
-    if (x < nv) {
-        r.it = A.vf_items[s_vf[l] + x];
+                                               const uint32_t *s_grp, uint32_t l, uint32_t x) {
+    TailItem r = tail_none();
+    const uint32_t nv = s_vf[l + 1] - s_vf[l], nf = s_fv[l + 1] - s_fv[l], ng = s_grp[l + 1] - s_grp[l];
+    if (x < ng) {
+        const uint64_t g = 2 * (uint64_t)(s_grp[l] + x);
+        r.w[0] = A.grp_items[g];
+        r.w[1] = A.grp_items[g + 1];
+        r.kind = 3u;
+    } else if (x - ng < nv) {
+        r.w[0] = A.vf_items[s_vf[l] + x - ng];
         r.kind = 1u;
-    } else if (x - nv < nf) {
-        r.it = A.fv_items[s_fv[l] + x - nv];
-        r.aux = A.fv_aux[s_fv[l] + x - nv];
+    } else if (x - ng - nv < nf) {
+        r.w[0] = A.fv_items[s_fv[l] + x - ng - nv];
+        r.w[1] = A.fv_aux[s_fv[l] + x - ng - nv];
         r.kind = 2u;
     }
     return r;
 }
 
 __device__ __forceinline__ void tail_run(const SeqArgs &A, const TailItem &t) {
-    if (t.kind == 1u) seq_vf_item(A, t.it);
-    else if (t.kind == 2u) seq_fv_item(A, t.it, t.aux);
+    if (t.kind == 3u) seq_group_lane(A, t.w[0], t.w[1]);
+    else if (t.kind == 1u) seq_vf_item(A, t.w[0]);
+    else if (t.kind == 2u) seq_fv_item(A, t.w[0], t.w[1]);
 }
 
 /* levels [l0, l1) by one workgroup, at most kSeqTailPer * kSeqTailBlock
- * items each; lvl_vf / lvl_fv hold each level's first item,
- * l1 - l0 <= kSeqTailLevels.  Items are read-only, so a level's items are
- * loaded while the level before it computes. */
+ * items each; lvl_vf / lvl_fv / lvl_grp hold each level's first item of
+ * each kind, l1 - l0 <= kSeqTailLevels.  Items are read-only, so a level's
+ * items are loaded while the level before it computes. */
 __global__ __launch_bounds__(kSeqTailBlock) __attribute__((unused)) void seq_tail(
-    SeqArgs A, const uint32_t *__restrict__ lvl_vf, const uint32_t *__restrict__ lvl_fv, uint32_t l0, uint32_t l1,
-    const unsigned long long *__restrict__ ctl, int k, double tol) {
-    __shared__ uint32_t s_vf[kSeqTailLevels + 1], s_fv[kSeqTailLevels + 1];
+    SeqArgs A, const uint32_t *__restrict__ lvl_vf, const uint32_t *__restrict__ lvl_fv,
+    const uint32_t *__restrict__ lvl_grp, uint32_t l0, uint32_t l1, const unsigned long long *__restrict__ ctl,
+    int k, double tol) {
+    __shared__ uint32_t s_vf[kSeqTailLevels + 1], s_fv[kSeqTailLevels + 1], s_grp[kSeqTailLevels + 1];
     const uint32_t n = l1 - l0;
     for (uint32_t i = threadIdx.x; i <= n; i += kSeqTailBlock) {
         s_vf[i] = lvl_vf[l0 + i];
         s_fv[i] = lvl_fv[l0 + i];
+        s_grp[i] = lvl_grp[l0 + i];
     }
     if (chunk_stopped(ctl, k, tol)) return;   /* the same answer in every wave */
     __syncthreads();
     TailItem cur[kSeqTailPer];
 #pragma unroll
-    for (int j = 0; j < kSeqTailPer; j++) cur[j] = tail_fetch(A, s_vf, s_fv, 0, threadIdx.x + j * kSeqTailBlock);
+    for (int j = 0; j < kSeqTailPer; j++)
+        cur[j] = tail_fetch(A, s_vf, s_fv, s_grp, 0, threadIdx.x + j * kSeqTailBlock);
     for (uint32_t l = 0; l < n; l++) {
         TailItem next[kSeqTailPer];
 #pragma unroll
         for (int j = 0; j < kSeqTailPer; j++)
-            next[j] = l + 1 < n ? tail_fetch(A, s_vf, s_fv, l + 1, threadIdx.x + j * kSeqTailBlock)
-                                : TailItem{make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u), 0u};
+            next[j] = l + 1 < n ? tail_fetch(A, s_vf, s_fv, s_grp, l + 1, threadIdx.x + j * kSeqTailBlock)
+                                : tail_none();
 #pragma unroll
         for (int j = 0; j < kSeqTailPer; j++) tail_run(A, cur[j]);
         __syncthreads();

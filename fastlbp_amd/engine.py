@@ -248,6 +248,7 @@ class ScheduleInfo:
     n_levels: int
     updates: int
     respect_preorder: bool
+    n_exec_items: int
 
 
 class Schedule:
@@ -290,7 +291,7 @@ class Schedule:
         i = ScheduleInfoC()
         call("flbp_schedule_get_info", self._h, C.byref(i))
         return ScheduleInfo(int(i.n_order), int(i.n_waves), int(i.n_fv_items), int(i.n_vf_items),
-                            int(i.n_levels), int(i.updates), bool(i.respect_preorder))
+                            int(i.n_levels), int(i.updates), bool(i.respect_preorder), int(i.n_exec_items))
 
     def order(self) -> np.ndarray:
         n = self.info().n_order
@@ -319,6 +320,15 @@ class Schedule:
         b = np.zeros(max(i.n_vf_items, 1), np.uint32)
         call("flbp_schedule_levels", self._h, _ptr(a, C.c_uint32), _ptr(b, C.c_uint32))
         return a[:i.n_fv_items], b[:i.n_vf_items]
+
+
+    def exec_items(self):
+        """(kind, a, b, c, level) of what runs per sweep on the device."""
+        n = self.info().n_exec_items
+        k = np.zeros(max(n, 1), np.uint8)
+        arrs = [np.zeros(max(n, 1), np.uint32) for _ in range(4)]
+        call("flbp_schedule_exec", self._h, _ptr(k, C.c_uint8), *[_ptr(x, C.c_uint32) for x in arrs])
+        return (k[:n],) + tuple(x[:n] for x in arrs)
 
 
 def seed(value: int) -> None:

@@ -284,6 +284,7 @@ typedef struct flbp_schedule_info {
     uint64_t n_levels;       /* launches per sweep on the GPU */
     int32_t updates;         /* FLBP_UPDATES_SEQFIX / _TOPO, or -1 for an explicit order */
     int32_t respect_preorder;
+    uint64_t n_exec_items;   /* what runs per sweep (flbp_schedule_exec) */
 } flbp_schedule_info;
 
 /* The reference's schedule for `updates`: SEQFIX shuffles the factor
@@ -309,6 +310,15 @@ int flbp_schedule_waves(const flbp_schedule *s, uint64_t *fv_off, uint64_t *fv_e
 /* the dependency level (1 .. n_levels) of every f->v item [n_fv_items] and
  * v->f item [n_vf_items], in the order of flbp_schedule_waves.  NULL skips. */
 int flbp_schedule_levels(const flbp_schedule *s, uint32_t *fv_level, uint32_t *vf_level);
+/* What runs on the device per sweep, n_exec_items entries, each with its
+ * level: kind 0 = the v->f message of factor-major slot a; kind 1 = the
+ * f->v update of var-major edge a (factors wider than 6 variables); kind 2
+ * = factor a's updates of one wave as one item: the v->f messages of its
+ * slots in mask c, recomputed from the variables' states before the item's
+ * own updates, then the f->v updates of its slots in mask b (bit k = the
+ * factor's k-th slot).  NULL skips an array. */
+int flbp_schedule_exec(const flbp_schedule *s, uint8_t *kind, uint32_t *a, uint32_t *b, uint32_t *c,
+                       uint32_t *level);
 /* rnd_gen.seed(seed) on the calling thread, as examples/bingo_wrapper.cu:115
  * does before constructing its CausalBP. */
 void flbp_seed(uint32_t seed);

@@ -70,6 +70,8 @@ _SIG = {
     "orc_seq_waves": (C.c_int, [C.c_void_p, u64p, u64p, u64p, u64p]),
     "orc_products": (C.c_int, [C.c_void_p, f64p, f64p, u64p, u64p]),
     "orc_seq_sweep_items": (C.c_int, [C.c_void_p, C.c_uint64, u8p, u64p, f64p]),
+    "orc_seq_sweep_exec": (C.c_int, [C.c_void_p, C.c_uint64, u8p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_uint32), f64p]),
     "orc_fv_message_seq": (None, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double,
                                   C.c_double, C.c_double, f64p, f64p, f64p, f64p]),
     "orc_last_error": (C.c_char_p, []),
@@ -239,6 +241,15 @@ class OracleGraph:
         md = C.c_double()
         _chk("orc_seq_sweep_items", lib().orc_seq_sweep_items(self._h, k.shape[0], _p(k, C.c_uint8),
                                                              _p(it, C.c_uint64), C.byref(md)))
+        return float(md.value)
+
+    def sweep_exec(self, kind, a, b, c):
+        """One sweep of engine execution items (checker of the level split)."""
+        k = np.ascontiguousarray(kind, np.uint8)
+        arrs = [np.ascontiguousarray(x, np.uint32) for x in (a, b, c)]
+        md = C.c_double()
+        _chk("orc_seq_sweep_exec", lib().orc_seq_sweep_exec(self._h, k.shape[0], _p(k, C.c_uint8),
+                                                           *[_p(x, C.c_uint32) for x in arrs], C.byref(md)))
         return float(md.value)
 
     def products(self):

@@ -1062,6 +1062,56 @@ int orc_seq_sweep_items(orc_graph *g, uint64_t n, const uint8_t *kind, const uin
     return 0;
 }
 
+/* Checker of the engine's execution items (not a restatement of the
+ * reference): one sweep of items in the listed order, then the belief step.
+ *   kind 0: the v->f message of factor-major slot a[x]
+ *   kind 1: the f->v update of var-major edge a[x]
+ *   kind 2: factor a[x]: first the v->f messages of its slots in mask c[x],
+ *           from the current states, then the f->v updates of its slots in
+ *           mask b[x] (bit k = the factor's k-th slot)
+ * Listed level by level in any order within a level, the items must give
+ * the wave execution's bits. */
+int orc_seq_sweep_exec(orc_graph *g, uint64_t n, const uint8_t *kind, const uint32_t *a, const uint32_t *b,
+                       const uint32_t *c, double *maxdiff_out) {
+    for (uint64_t x = 0; x < n; x++) {
+        double o0, o1;
+        if (kind[x] == 0 || kind[x] == 1) {
+            if (a[x] >= g->E) return fail("item out of range");
+            if (kind[x] == 0) {
+                const FacMajorDesc &d = g->vf_desc[a[x]];
+                vf_edge_seq(g->prod0[d.v], g->prod1[d.v], g->nz0[d.v], g->nz1[d.v], g->m_fv0[d.src],
+                            g->m_fv1[d.src], o0, o1);
+                g->m_vf0[d.ind] = o0;
+                g->m_vf1[d.ind] = o1;
+            } else {
+                fv_edge<true>(g->fv_desc[a[x]], g->m_vf0.data(), g->m_vf1.data(), o0, o1);
+                update_message(g, a[x], g->vm_var[a[x]], o0, o1);
+            }
+            continue;
+        }
+        if (kind[x] != 2 || a[x] >= g->F) return fail("bad item");
+        const uint64_t s0 = g->row_ptr_vf[a[x]], deg = g->row_ptr_vf[a[x] + 1] - s0;
+        if (deg > 32) return fail("group item of a factor wider than 32");
+        for (uint64_t k = 0; k < deg; k++) {
+            if (!((c[x] >> k) & 1u)) continue;
+            const FacMajorDesc &d = g->vf_desc[s0 + k];
+            vf_edge_seq(g->prod0[d.v], g->prod1[d.v], g->nz0[d.v], g->nz1[d.v], g->m_fv0[d.src], g->m_fv1[d.src],
+                        o0, o1);
+            g->m_vf0[d.ind] = o0;
+            g->m_vf1[d.ind] = o1;
+        }
+        for (uint64_t k = 0; k < deg; k++) {
+            if (!((b[x] >> k) & 1u)) continue;
+            const uint64_t e = g->vf_desc[s0 + k].src;
+            fv_edge<true>(g->fv_desc[e], g->m_vf0.data(), g->m_vf1.data(), o0, o1);
+            update_message(g, e, g->vm_var[e], o0, o1);
+        }
+    }
+    const double d = belief_step(g);
+    if (maxdiff_out) *maxdiff_out = d;
+    return 0;
+}
+
 int orc_products(const orc_graph *g, double *prod0, double *prod1, uint64_t *nz0, uint64_t *nz1) {
     if (prod0) std::copy(g->prod0.begin(), g->prod0.end(), prod0);
     if (prod1) std::copy(g->prod1.begin(), g->prod1.end(), prod1);

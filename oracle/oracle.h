@@ -150,6 +150,11 @@ int orc_seq_waves(const orc_graph *g, uint64_t *fv_off, uint64_t *fv_edge, uint6
  * item[x]; kind 1: f->v update of var-major edge item[x]), then beliefs. */
 int orc_seq_sweep_items(orc_graph *g, uint64_t n, const uint8_t *kind, const uint64_t *item,
                         double *maxdiff_out);
+/* Checker of the engine's execution items: kind 0 v->f of slot a, kind 1
+ * f->v of var-major edge a, kind 2 factor a: v->f of the slots in mask c
+ * (from current states), then f->v of the slots in mask b. */
+int orc_seq_sweep_exec(orc_graph *g, uint64_t n, const uint8_t *kind, const uint32_t *a, const uint32_t *b,
+                       const uint32_t *c, double *maxdiff_out);
 /* per-variable products and zero counts (maintained in place by SEQ sweeps) */
 int orc_products(const orc_graph *g, double *prod0, double *prod1, uint64_t *nz0, uint64_t *nz1);
 /* one f->v message with the sequential kernels' scale() */

@@ -12,8 +12,10 @@ builder (flbp_schedule_*, no GPU needed):
 - Engine orders and waves equal the oracle's on generated graphs, the
   SEQFIX generator advancing across schedules like the reference's
   thread_local rnd_gen.
-- The dependency-level split: items run level by level, in any order
-  within a level, give the wave execution's bits; merging levels does not.
+- The engine's execution items (per wave one group item per factor, with
+  separate v->f / f->v items where needed) run level by level, in any
+  order within a level, give the wave execution's bits; merging levels
+  does not.
 - Sequential BP arithmetic: exact marginals on trees; the PARALL fixed point
   on loopy graphs; the in-place products agree with the messages' product;
   each f->v code with the sequential scale() against enumeration.
@@ -159,27 +161,40 @@ def test_schedule_argument_errors():
     assert fl.lib().flbp_schedule_create(C.byref(c), 1, C.byref(h)) == -1   # PARALL
 
 
-def _level_items(s):
-    fo, fe, vo, vs = s.waves()
-    flv, vlv = s.levels()
-    kind = np.concatenate([np.zeros(len(vs), np.uint8), np.ones(len(fe), np.uint8)])
-    item = np.concatenate([vs, fe]).astype(np.uint64)
-    lev = np.concatenate([vlv, flv]).astype(np.int64)
-    return kind, item, lev
+def _high_degree_graph():
+    gen = fl.generate(2500, 21, k_min=6, k_max=14, b_min=4, b_max=10, leaf_frac=0.02)
+    assert np.diff(fl.Graph(gen.factors).fac_off).max() > 6   # wide factors keep per-message items
+    return gen
 
 
-@pytest.mark.parametrize("mode", ["SEQFIX", "TOPO"])
-def test_level_split_reproduces_wave_bits(mode):
-    gen = fl.generate(3000, 5)
+@pytest.mark.parametrize("mode,graph", [("SEQFIX", "plain"), ("TOPO", "plain"), ("SEQFIX", "wide"),
+                                        ("TOPO", "wide"), ("order", "plain")])
+def test_execution_items_reproduce_wave_bits(mode, graph):
+    """The engine's execution items -- per wave one group item per factor
+    (its v->f messages recomputed from the states before its own updates,
+    then its f->v updates), separate items where another factor updates a
+    variable in the same wave or the factor is wide -- run level by level
+    in a shuffled order within each level give the wave execution's bits;
+    merging levels pairwise does not."""
+    gen = fl.generate(3000, 5) if graph == "plain" else _high_degree_graph()
     g = fl.Graph(gen.factors)
     fl.seed(42)
-    s = fl.Schedule(g, mode)
-    kind, item, lev = _level_items(s)
+    if mode == "order":
+        order = np.random.default_rng(2).integers(0, g.n_edges, size=g.n_edges + 99).astype(np.uint64)
+        s = fl.Schedule(g, order=order)
+    else:
+        s = fl.Schedule(g, mode)
+    kind, a, b, c, lev = s.exec_items()
+    lev = lev.astype(np.int64)
     assert lev.min() >= 1 and lev.max() == s.info().n_levels
-    # every wave's v->f items sit below its f->v items' levels when they feed them
+    assert (kind == 2).any()
+    if graph == "wide":
+        assert (kind == 0).any() and (kind == 1).any()
+    if mode == "order":
+        assert (kind == 0).any()   # v->f messages of variables another factor updates in the same wave
     waves = oracle.OracleGraph(gen.factors)
     waves.init()
-    waves.set_sequence(s.order(), respect_preorder=(mode == "TOPO"))
+    waves.set_sequence(s.order(), respect_preorder=s.info().respect_preorder)
     levels = oracle.OracleGraph(gen.factors)
     levels.init()
     merged = oracle.OracleGraph(gen.factors)
@@ -187,9 +202,9 @@ def test_level_split_reproduces_wave_bits(mode):
     rng = np.random.default_rng(1)
     for sweep in range(5):
         perm = np.lexsort((rng.random(len(lev)), lev))
-        assert waves.sweep() == levels.sweep_items(kind[perm], item[perm])
+        assert waves.sweep() == levels.sweep_exec(kind[perm], a[perm], b[perm], c[perm])
         perm2 = np.lexsort((rng.random(len(lev)), (lev + 1) // 2))   # a wrong split
-        merged.sweep_items(kind[perm2], item[perm2])
+        merged.sweep_exec(kind[perm2], a[perm2], b[perm2], c[perm2])
     for a, b in zip(waves.messages() + (waves.beliefs(),), levels.messages() + (levels.beliefs(),)):
         assert np.array_equal(a, b)
     for a, b in zip(waves.products(), levels.products()):
