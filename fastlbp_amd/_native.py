@@ -196,6 +196,7 @@ SIGNATURES = {
     "flbp_seed": (None, [C.c_uint32]),
     "flbp_plan_set_schedule": (C.c_int, [C.c_void_p, C.c_void_p]),
     "flbp_plan_get_updates": (C.c_int, [C.c_void_p]),
+    "flbp_plan_set_seq_tail": (C.c_int, [C.c_void_p, C.c_uint32]),
     "flbp_last_error": (C.c_char_p, []),
     "flbp_version": (C.c_char_p, []),
 }

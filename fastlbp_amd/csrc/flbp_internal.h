@@ -16,6 +16,9 @@ namespace flbp {
 /* factors up to this degree run a sequential wave's updates as one group
  * item (seq_host.cpp, seq_kernels.h) */
 constexpr uint32_t kSeqGroupDeg = 6;
+/* the single-workgroup tail kernel of a sequential sweep runs levels of at
+ * most this many items (seq.hip) */
+constexpr uint32_t kSeqTailItemsMax = 1024;
 
 /* structural checks of a CSR (plan_host.hip) */
 int validate_csr(const flbp_csr &c);

@@ -101,12 +101,14 @@ struct Plan {
     int seq_updates = -1;                          /* FLBP_UPDATES_* or -1 (explicit order) */
     uint4 *seq_vf = nullptr, *seq_fv = nullptr;    /* items grouped by level */
     uint4 *seq_fv_aux = nullptr;                   /* f->v items: degree | type << 24, target slot, head slot */
-    uint4 *seq_grp = nullptr;                      /* group items, 4 words each */
+    uint4 *seq_grp = nullptr;                      /* group items, 2 words per lane */
     dev::FacRec *seq_fac = nullptr;                /* per-factor records */
     dev::VarState *seq_state = nullptr;            /* per-variable products and zero counts */
     std::vector<uint32_t> seq_lvl_vf, seq_lvl_fv, seq_lvl_grp;  /* [levels+1] item offsets per level */
     uint32_t *seq_lvl_vf_dev = nullptr, *seq_lvl_fv_dev = nullptr, *seq_lvl_grp_dev = nullptr;
     size_t seq_tail_from = 0;                      /* levels from here run in the tail kernel */
+    uint32_t seq_tail_max = kSeqTailItemsMax;      /* tail levels hold at most this many items */
+    void seq_place_tail();
     uint64_t seq_waves = 0;
     std::vector<uint64_t> h_fac_off;               /* host copy of fac_off */
     int launch_seq_sweep(int k, double tol);

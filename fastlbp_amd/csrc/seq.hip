@@ -21,7 +21,7 @@
  *   seq_lvl_*_dev u32[levels+1]  first item of each level
  * Items are grouped by level; within a level v->f items are ordered by
  * slot and f->v items by factor, so neighbouring lanes touch neighbouring
- * messages.  The trailing levels of at most kSeqTailItems items each run
+ * messages.  The trailing levels of at most seq_tail_max items each run
  * in one single-workgroup launch (seq_tail).
  *
  * Replaces the reference's per-wave launches (csrc/ext/causal_bp.cu:446-477,
@@ -51,6 +51,7 @@ __global__ void fill_state(flbp::dev::VarState *st, uint64_t n, double c) {
 /* trailing levels up to this many items each go to the single-workgroup
  * tail kernel */
 constexpr uint32_t kSeqTailItems = flbp::dev::kSeqTailPer * flbp::dev::kSeqTailBlock;
+static_assert(kSeqTailItems == flbp::kSeqTailItemsMax, "tail kernel capacity");
 
 }  // namespace
 
@@ -72,6 +73,19 @@ void Plan::free_seq_items() {
     seq_lvl_fv.clear();
     seq_lvl_grp.clear();
     seq_tail_from = 0;
+}
+
+/* The tail: the longest run of trailing levels with at most seq_tail_max
+ * items (group lanes count one each) every one. */
+void Plan::seq_place_tail() {
+    const size_t nl = seq_lvl_fv.empty() ? 0 : seq_lvl_fv.size() - 1;
+    auto level_items = [&](size_t l) {   /* items of 0-based level l */
+        return (seq_lvl_vf[l + 1] - seq_lvl_vf[l]) + (seq_lvl_fv[l + 1] - seq_lvl_fv[l]) +
+               (seq_lvl_grp[l + 1] - seq_lvl_grp[l]);
+    };
+    size_t tail = nl;
+    while (tail > 0 && level_items(tail - 1) <= seq_tail_max) tail--;
+    seq_tail_from = tail;
 }
 
 /* One sweep: every level, then beliefs and the max-diff of sweep k. */
@@ -227,14 +241,7 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     HIP_TRY(hipMemcpy(P->seq_lvl_vf_dev, P->seq_lvl_vf.data(), (nl + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(P->seq_lvl_fv_dev, P->seq_lvl_fv.data(), (nl + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(P->seq_lvl_grp_dev, P->seq_lvl_grp.data(), (nl + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
-    /* the tail: the longest run of trailing levels with few items each */
-    auto level_items = [&](size_t l) {   /* items of 0-based level l */
-        return (P->seq_lvl_vf[l + 1] - P->seq_lvl_vf[l]) + (P->seq_lvl_fv[l + 1] - P->seq_lvl_fv[l]) +
-               (P->seq_lvl_grp[l + 1] - P->seq_lvl_grp[l]);
-    };
-    size_t tail = nl;
-    while (tail > 0 && level_items(tail - 1) <= kSeqTailItems) tail--;
-    P->seq_tail_from = tail;
+    P->seq_place_tail();
     if (!P->seq_state) {
         if (int rc = P->alloc(P->seq_state, P->V)) return rc;
         if (int rc = P->alloc(P->seq_fac, P->F)) return rc;
@@ -248,6 +255,20 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->seq_updates = S.updates;
     P->seq_waves = S.n_waves;
+    return FLBP_OK;
+}
+
+int flbp_plan_set_seq_tail(flbp_plan *plan, uint32_t max_items) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_seq_tail: null plan");
+    if (max_items > kSeqTailItems)
+        return flbp::fail(FLBP_EINVAL, "flbp_plan_set_seq_tail: at most " + std::to_string(kSeqTailItems) +
+                                           " items per tail level");
+    HIP_TRY(hipSetDevice(P->device));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->drop_graphs();
+    P->seq_tail_max = max_items;
+    P->seq_place_tail();
     return FLBP_OK;
 }
 

@@ -452,6 +452,12 @@ class Plan:
         """FLBP_UPDATES_PARALL without a schedule, else the schedule's."""
         return int(lib().flbp_plan_get_updates(self._h))
 
+    def set_seq_tail(self, max_items: int) -> None:
+        """Under a schedule, the trailing levels of at most max_items items
+        each (0 .. 1024) run in one single-workgroup launch; 0 launches
+        every level.  Results are the same either way."""
+        call("flbp_plan_set_seq_tail", self._h, C.c_uint32(max_items))
+
     def use_graphs(self, enable: bool = True) -> None:
         """Replay chunks of sweeps from captured hipGraphs (off by default:
         no measured gain over direct launches)."""

@@ -339,6 +339,13 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *s);
 /* FLBP_UPDATES_PARALL without a schedule, else the schedule's updates
  * (-1 for an explicit order) */
 int flbp_plan_get_updates(const flbp_plan *plan);
+/* Execution only, results unchanged: the longest run of trailing levels
+ * with at most max_items items each (0 .. 1024, default 1024) runs in one
+ * single-workgroup launch with barriers between levels; 0 launches every
+ * level.  No counterpart in the reference (its waves are launches); kept
+ * for tests that drive both paths on small graphs.  Applies to the
+ * current schedule and to later ones. */
+int flbp_plan_set_seq_tail(flbp_plan *plan, uint32_t max_items);
 
 /* ------------------------------------------------------------------ */
 /* Batched solve (not in the reference)                                */

@@ -42,11 +42,13 @@ def assert_same_state(plan, o):
     assert same_bits(plan.beliefs(), o.beliefs()), "marginals differ"
 
 
-def run_both(flist, mode, tol, maxiter, chunk=0):
+def run_both(flist, mode, tol, maxiter, chunk=0, tail=None):
     g = fl.Graph(flist)
     s = schedule_for(g, mode)
     plan = fl.Plan(g)
     plan.set_schedule(s)
+    if tail is not None:
+        plan.set_seq_tail(tail)
     assert plan.updates == s.info().updates and plan.variant == 3
     plan.init()
     r = plan.run(tol, maxiter, chunk=chunk)
@@ -59,30 +61,37 @@ def run_both(flist, mode, tol, maxiter, chunk=0):
     return g, plan, o, r
 
 
+# tail None: the default split (small graphs run almost every level in the
+# single-workgroup tail kernel); 0: every level its own launch
+@pytest.mark.parametrize("tail", [None, 0])
 @pytest.mark.parametrize("mode", ["SEQFIX", "TOPO"])
 @pytest.mark.parametrize("seed,n_vars,tol", [(1, 2000, 1e-6), (3, 20000, 1e-9), (4, 777, 1e-12)])
-def test_seeded_graphs(mode, seed, n_vars, tol):
+def test_seeded_graphs(mode, seed, n_vars, tol, tail):
     gen = fl.generate(n_vars, seed)
-    _, _, _, r = run_both(gen.factors, mode, tol, 500)
+    _, _, _, r = run_both(gen.factors, mode, tol, 500, tail=tail)
     assert r.iters < 500
 
 
+@pytest.mark.parametrize("tail", [None, 0, 64])
 @pytest.mark.parametrize("mode", ["SEQFIX", "TOPO"])
-def test_high_degree_factors_and_variables(mode):
-    """Factors above the register path (8 slots) and wide variables."""
+def test_high_degree_factors_and_variables(mode, tail):
+    """Factors above the register path (8 slots) and wide variables, mixed
+    with group items (factors of at most 6 slots)."""
     gen = fl.generate(4000, 21, k_min=6, k_max=14, b_min=4, b_max=10, leaf_frac=0.02)
     g = fl.Graph(gen.factors)
-    assert np.diff(g.fac_off).max() > 8 and np.diff(g.var_off).max() > 8
-    run_both(gen.factors, mode, 1e-9, 300)
+    deg = np.diff(g.fac_off)
+    assert deg.max() > 8 and np.diff(g.var_off).max() > 8 and (deg <= 6).any()
+    run_both(gen.factors, mode, 1e-9, 300, tail=tail)
 
 
+@pytest.mark.parametrize("tail", [None, 0])
 @pytest.mark.parametrize("variant", range(3))
-def test_clamped_variants(variant):
+def test_clamped_variants(variant, tail):
     gen = fl.generate(6000, 31)
     rng = np.random.default_rng(300 + variant)
     labels, values = random_alarm_clamps(rng, gen.alarms, 50)
     ce = fl.clamp(gen.factors, labels, values)
-    run_both(ce, "SEQFIX" if variant % 2 == 0 else "TOPO", 1e-9, 500)
+    run_both(ce, "SEQFIX" if variant % 2 == 0 else "TOPO", 1e-9, 500, tail=tail)
 
 
 def test_edge_case_graph_and_contradictions():
@@ -212,6 +221,17 @@ def test_switching_schedules_mid_run():
     plan.run_fixed(3, 0.3)   # damped PARALL reads the factor-major copy: rebuilt from var-major
     o.run_fixed(3, 0.3)
     assert_same_state(plan, o)
+
+
+def test_tail_setting_rejects_what_the_kernel_cannot_hold():
+    g = fl.Graph(fl.generate(500, 2).factors)
+    plan = fl.Plan(g)
+    with pytest.raises(fl.FlbpError):
+        plan.set_seq_tail(1025)
+    plan.set_seq_tail(0)          # before a schedule: kept for the next one
+    plan.set_schedule(schedule_for(g, "SEQFIX"))
+    plan.init()
+    plan.run(1e-9, 100)
 
 
 def test_rejections_under_a_schedule():
