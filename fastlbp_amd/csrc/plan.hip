@@ -69,6 +69,13 @@ __global__ void gather_pairs(const double2 *__restrict__ src, const uint32_t *__
     if (i < n) dst[i] = src[pos[i]];
 }
 
+/* dst[pos[i]] = src[i] */
+__global__ void scatter_pairs(const double2 *__restrict__ src, const uint32_t *__restrict__ pos,
+                              double2 *__restrict__ dst, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[pos[i]] = src[i];
+}
+
 /* out[i] = src[idx[i]] */
 __global__ void gather_f64(const double *__restrict__ src, const uint32_t *__restrict__ idx, double *__restrict__ out,
                            uint64_t n) {
@@ -96,9 +103,25 @@ int hip_fail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(e_, #expr); \
     } while (0)
 
+int Plan::fm_current() {
+    if (fm_stale && E)
+        hipLaunchKernelGGL(gather_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_fv_vm, fac_vpos, m_fv, E);
+    fm_stale = false;
+    HIP_TRY(hipGetLastError());
+    return FLBP_OK;
+}
+
+int Plan::vm_current() {
+    if (vm_stale && E)
+        hipLaunchKernelGGL(scatter_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_fv, fac_vpos, m_fv_vm, E);
+    vm_stale = false;
+    HIP_TRY(hipGetLastError());
+    return FLBP_OK;
+}
+
 int Plan::launch_sweep(int k, double tol, double damping) {
-    if (seq) {   /* a sequential schedule updates the var-major f->v array only */
-        fm_stale = true;
+    if (seq) {   /* a sequential schedule updates the factor-major f->v array only */
+        vm_stale = true;
         return launch_seq_sweep(k, tol);
     }
     if (variant == 3) {
@@ -240,7 +263,7 @@ int Plan::chunk_graph(int n, double tol, double damping, hipGraphExec_t &exec) {
 int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_diff) {
     if (!profiling && use_graphs) {
         if (seq) {
-            fm_stale = true;
+            vm_stale = true;
         } else if (variant == 3) {   /* the scatter layout's damping copy, as launch_sweep keeps it */
             if (damping != 0.0 && fm_stale && E) {
                 hipLaunchKernelGGL(gather_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_fv_vm, fac_vpos,
@@ -352,6 +375,7 @@ int flbp_plan_init(flbp_plan *plan, int logdomain) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->fm_stale = false;
+    P->vm_stale = false;
     P->init_value = c;
     P->swept_since_init = false;
     return FLBP_OK;
@@ -493,8 +517,9 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
     HIP_TRY(hipSetDevice(P->device));
     std::vector<double> tmp(2 * P->E);
     if (fv_varmajor && P->E) {
-        /* factor-major, or var-major in device variable order (variant 3) */
-        const bool vm = P->variant == 3;
+        /* factor-major, or var-major in device variable order (variant 3,
+         * unless a schedule's sweeps left only the factor-major array current) */
+        const bool vm = P->variant == 3 && !P->vm_stale;
         HIP_TRY(hipMemcpy(tmp.data(), vm ? P->m_fv_vm : P->m_fv, P->E * sizeof(double2), hipMemcpyDeviceToHost));
         for (uint64_t k = 0; k < P->E; k++) {
             const uint64_t s = vm ? P->h_fac_vpos[P->h_var_edge[k]] : P->h_var_edge[k];
@@ -532,6 +557,7 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
         }
         HIP_TRY(hipMemcpy(P->m_fv_vm, tmp.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
         P->fm_stale = false;
+        P->vm_stale = false;
     }
     if (vf_facmajor && P->E) {
         std::vector<double> tmp(2 * P->E);

@@ -77,6 +77,10 @@ struct Plan {
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
     double2 *m_fv_vm = nullptr;    /* var-major f->v messages (variant 3) */
     bool fm_stale = false;         /* variant 3: factor-major f->v copy out of date */
+    bool vm_stale = false;         /* under a schedule: var-major f->v copy out of date (its sweeps
+                                      update the factor-major array only) */
+    int fm_current();              /* bring either copy up to date from the other (plan.hip) */
+    int vm_current();
     unsigned long long *ctl = nullptr;             /* partial maxima, see sweep_kernels.h */
     unsigned long long *ctl_out = nullptr;         /* per-sweep maxima of the last chunk */
     unsigned long long *host_ctl = nullptr;        /* pinned */

@@ -3,19 +3,22 @@
  * TOPO): upload of the schedule's items grouped by dependency level, the
  * per-sweep launches, and the C ABI that attaches a schedule to a plan.
  *
- * Device state added to the plan (variant-3 message orders, which are the
- * reference's: v->f factor-major m_vf, f->v var-major m_fv_vm):
+ * Messages: v->f factor-major in m_vf (the reference's order) and f->v
+ * factor-major in m_fv -- every message a group item touches lies in its
+ * factor's slot range -- with the var-major copy m_fv_vm rebuilt when the
+ * schedule is removed or the messages are read out (vm_stale).
+ * Device state added to the plan:
  *   seq_state  {f64 p0, p1; u64 z0, z1}[V]  products and zero counts
  *              (prod_fv_0/1, num_zeros_fv_0/1), maintained in place
- *   seq_vf     u32x4[n_vf]  v->f items (slot, var-major position, variable, 0)
+ *   seq_vf     u32x4[n_vf]  v->f items (slot, variable, 0, 0)
  *   seq_fv     u32x4[n_fv]  f->v items (var-major position, variable, factor,
  *              first slot of the factor)
  *   seq_fv_aux u32x4[n_fv]  (degree | factor type << 24, target slot, head slot, 0)
- *   seq_grp    u32x4[2 * lanes]  group items, one lane per slot: (factor,
- *              first slot, degree | type << 24, head slot | updated mask << 8
- *              | recomputed mask << 16 | slot << 24), (variable, var-major
- *              position, 0, 0); a group never straddles a 64-lane wave and
- *              each level's lanes are padded to whole waves (degree 0)
+ *   seq_grp    u32x4[lanes]  group items, one lane per slot: (factor, first
+ *              slot, group_word(degree, type, head slot, updated mask,
+ *              recomputed mask, slot), variable); a group never straddles a
+ *              64-lane wave and each level's lanes are padded to whole waves
+ *              (degree 0)
  *   seq_fac    {f64 p, q, m0, m1}[F]  factor records, rebuilt on the device
  *              from fac_p / fac_q / fac_meta / fac_mask whenever they change
  *   seq_lvl_*_dev u32[levels+1]  first item of each level
@@ -91,7 +94,7 @@ void Plan::seq_place_tail() {
 /* One sweep: every level, then beliefs and the max-diff of sweep k. */
 int Plan::launch_seq_sweep(int k, double tol) {
     const size_t L = seq_lvl_fv.empty() ? 0 : seq_lvl_fv.size() - 1;
-    SeqArgs A{seq_vf, seq_fv, seq_fv_aux, seq_grp, seq_fac, m_vf, m_fv_vm, seq_state};
+    SeqArgs A{seq_vf, seq_fv, seq_fv_aux, seq_grp, seq_fac, m_vf, m_fv, seq_state};
     for (size_t l = 0; l < seq_tail_from; l++) {
         const uint32_t v0 = seq_lvl_vf[l], nv = seq_lvl_vf[l + 1] - v0;
         const uint32_t f0 = seq_lvl_fv[l], nf = seq_lvl_fv[l + 1] - f0;
@@ -102,7 +105,7 @@ int Plan::launch_seq_sweep(int k, double tol) {
         a.vf_items = seq_vf + v0;
         a.fv_items = seq_fv + f0;
         a.fv_aux = seq_fv_aux + f0;
-        a.grp_items = seq_grp + 2 * (uint64_t)g0;
+        a.grp_items = seq_grp + g0;
         hipLaunchKernelGGL(seq_level, dim3(vb + fb + gb), dim3(dev::kBlock), 0, stream, a, nv, nf, ng, vb, fb, ctl,
                            k, tol);
     }
@@ -135,6 +138,8 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->drop_graphs();
     if (!sched) {
+        if (int rc = P->vm_current()) return rc;   /* PARALL's var pass reads the var-major array */
+        HIP_TRY(hipStreamSynchronize(P->stream));
         P->free_seq_items();
         P->seq = false;
         P->seq_updates = -1;
@@ -147,6 +152,8 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     if (P->variant != 3) {
         if (int rc = flbp_plan_set_variant(plan, 3)) return rc;
     }
+    /* the schedule's sweeps read and write the factor-major f->v array */
+    if (int rc = P->fm_current()) return rc;
     /* the schedule's execution items, grouped by kind and level (counting
      * sort), then ordered by slot / factor inside each level */
     const uint32_t nl = S.n_levels;
@@ -177,7 +184,7 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     std::vector<uint4> vf(keyed[0].size()), fv(keyed[1].size()), faux(keyed[1].size());
     for (size_t y = 0; y < keyed[0].size(); y++) {
         const uint32_t sl = S.ex_a[keyed[0][y].second];
-        vf[y] = make_uint4(sl, S.slot_pos[sl], S.slot_var[sl], 0u);
+        vf[y] = make_uint4(sl, S.slot_var[sl], 0u, 0u);
     }
     for (size_t y = 0; y < keyed[1].size(); y++) {
         const uint32_t e = S.ex_a[keyed[1][y].second];
@@ -187,35 +194,31 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
         fv[y] = make_uint4(e, S.slot_var[own], I, a);
         faux[y] = make_uint4(d | ((meta & 3u) << 24), own - a, meta >> 8, 0u);
     }
-    /* group items: one lane (two words) per slot, a group's lanes inside
+    /* group items: one lane (one record) per slot, a group's lanes inside
      * one 64-lane wave, each level padded to whole waves; glane[l] = first
      * lane of level l */
     std::vector<uint4> grp;
     std::vector<uint32_t> glane(nl + 2, 0);
+    static_assert(flbp::kSeqGroupDeg <= 6, "group lane record: degree, head and slot in 3 bits, masks in 6");
     for (uint32_t l = 1; l <= nl; l++) {
-        const size_t base = grp.size() / 2;
+        const size_t base = grp.size();
         auto pad_to_wave = [&]() {
-            while ((grp.size() / 2 - base) % 64) {
-                grp.push_back(make_uint4(0u, 0u, 0u, 0u));
-                grp.push_back(make_uint4(0u, 0u, 0u, 0u));
-            }
+            while ((grp.size() - base) % 64) grp.push_back(make_uint4(0u, 0u, 0u, 0u));
         };
         for (uint32_t y = off[2][l]; y < off[2][l + 1]; y++) {
             const size_t x = keyed[2][y].second;
             const uint32_t I = S.ex_a[x];
             const uint32_t a = (uint32_t)P->h_fac_off[I], d = (uint32_t)(P->h_fac_off[I + 1] - a);
             const uint32_t meta = P->h_meta[I];
-            if ((grp.size() / 2 - base) % 64 + d > 64) pad_to_wave();
-            for (uint32_t k = 0; k < d; k++) {
-                grp.push_back(make_uint4(I, a, d | ((meta & 3u) << 24),
-                                         (meta >> 8) | (S.ex_b[x] << 8) | (S.ex_c[x] << 16) | (k << 24)));
-                grp.push_back(make_uint4(S.slot_var[a + k], S.slot_pos[a + k], 0u, 0u));
-            }
+            if ((grp.size() - base) % 64 + d > 64) pad_to_wave();
+            for (uint32_t k = 0; k < d; k++)
+                grp.push_back(make_uint4(I, a, flbp::dev::group_word(d, meta & 3u, meta >> 8, S.ex_b[x], S.ex_c[x], k),
+                                         S.slot_var[a + k]));
         }
         pad_to_wave();
-        if (grp.size() / 2 >= 0xffffffffull)
+        if (grp.size() >= 0xffffffffull)
             return flbp::fail(FLBP_EINVAL, "flbp_plan_set_schedule: too many group lanes for 32-bit offsets");
-        glane[l + 1] = (uint32_t)(grp.size() / 2);
+        glane[l + 1] = (uint32_t)grp.size();
     }
     P->free_seq_items();
     auto dev_alloc = [&](auto *&ptr, size_t n) {

@@ -2,20 +2,24 @@
  * seq_kernels.h -- device code of the sequential schedules (SEQFIX / TOPO).
  *
  * One launch runs every item of one dependency level (seq_host.cpp):
- * blocks [0, vf_blocks) take its v->f items, the rest its f->v items.
+ * blocks [0, vf_blocks) take its v->f items, then its f->v items, then its
+ * group lanes.  Both message arrays are factor-major here: slot s of factor
+ * I holds the v->f message m_vf[s] and the f->v message m_fv[s] of edge
+ * (I, variable of s).
  *
- *   v->f item (slot s, var-major position k of the same edge, variable j):
+ *   v->f item (slot s, variable j):
  *     calcMessageVFKernel, csrc/kernel/causal_bp_seq.cu:113-147 -- the
  *     leave-one-out message from j's product and zero counts with the
- *     f->v message k divided out, normalized with no (0, 0) guard.
- *   f->v item (var-major position k, variable i, factor I, the factor's
+ *     f->v message of the same edge divided out, normalized with no (0, 0)
+ *     guard.
+ *   f->v item (var-major position, variable i, factor I, the factor's
  *     first slot a; degree d, type, target slot t and head slot h in a
  *     second word; p, q and the head mask in the factor's 32-byte record):
  *     calcMessageFVFusedKernel, causal_bp_seq.cu:149-459 -- the PARALL
  *     f->v arithmetic (fv_msg) with the sequential scale(), then
- *     update_message (34-83): the old message k divided out of i's
- *     product, the new one multiplied in (zeros counted, counts as u64 that
- *     wrap like the reference's size_t), rescaled, and stored.
+ *     update_message (34-83): the old message m_fv[a + t] divided out of
+ *     i's product, the new one multiplied in (zeros counted, counts as u64
+ *     that wrap like the reference's size_t), rescaled, and stored.
  *
  * A level is short on work and long on latency, so each lane's chain of
  * dependent memory accesses is kept at two: the item, then everything it
@@ -71,6 +75,14 @@ constexpr int kSeqTailBlock = 512;  /* threads of the tail kernel (two waves per
 constexpr int kSeqTailLevels = 256; /* level offsets the tail kernel keeps in LDS */
 constexpr int kSeqTailPer = 2;      /* items per thread and level in the tail kernel */
 
+/* a group lane's third word: degree (0 = padding lane) bits 0-2, factor
+ * type 3-4, head slot 5-7, updated-slot mask 8-13, recomputed-slot mask
+ * 14-19, the lane's slot 20-22 */
+__host__ __device__ inline uint32_t group_word(uint32_t d, uint32_t type, uint32_t h, uint32_t tmask, uint32_t imask,
+                                               uint32_t k) {
+    return d | (type << 3) | (h << 5) | (tmask << 8) | (imask << 14) | (k << 20);
+}
+
 }  // namespace dev
 }  // namespace flbp
 
@@ -100,19 +112,19 @@ __device__ __forceinline__ double2 vf_msg_seq(const VarState &st, double2 m) {
 }
 
 struct SeqArgs {
-    const uint4 *__restrict__ vf_items;   /* slot, var-major position, variable */
+    const uint4 *__restrict__ vf_items;   /* slot, variable */
     const uint4 *__restrict__ fv_items;   /* var-major position, variable, factor, first slot */
     const uint4 *__restrict__ fv_aux;     /* degree | type << 24, target slot, head slot */
-    const uint4 *__restrict__ grp_items;  /* 2 words per group lane (see seq_group_lane) */
+    const uint4 *__restrict__ grp_items;  /* one record per group lane (see seq_group_lane) */
     const FacRec *__restrict__ fac;       /* per-factor p, q, head mask */
     double2 *__restrict__ m_vf;           /* factor-major */
-    double2 *__restrict__ m_fv_vm;        /* var-major */
+    double2 *__restrict__ m_fv;           /* factor-major */
     VarState *__restrict__ state;
 };
 
 __device__ __forceinline__ void seq_vf_item(const SeqArgs &A, const uint4 it) {
-    const double2 m = A.m_fv_vm[it.y];
-    const VarState st = A.state[it.z];
+    const double2 m = A.m_fv[it.x];
+    const VarState st = A.state[it.y];
     A.m_vf[it.x] = vf_msg_seq(st, m);
 }
 
@@ -125,7 +137,7 @@ __device__ __forceinline__ void seq_fv_item(const SeqArgs &A, const uint4 it, co
 #pragma unroll
     for (int j = 0; j < kSeqRegDeg; j++) in.m[j] = j < d ? A.m_vf[a + j] : make_double2(0.0, 0.0);
     const FacRec f = A.fac[I];
-    const double2 old = A.m_fv_vm[it.x];
+    const double2 old = A.m_fv[a + t];
     VarState st = A.state[v];
     double2 r;
     if (type == 0) {
@@ -146,7 +158,7 @@ __device__ __forceinline__ void seq_fv_item(const SeqArgs &A, const uint4 it, co
     if (r.y == 0) st.z1++; else st.p1 *= r.y;
     scale_seq(st.p0, st.p1);
     A.state[v] = st;
-    A.m_fv_vm[it.x] = r;
+    A.m_fv[a + t] = r;
 }
 
 constexpr int kG = (int)flbp::kSeqGroupDeg;
@@ -165,24 +177,26 @@ __device__ __forceinline__ RegIn<kG> group_messages(double2 m, int base) {
     return in;
 }
 
-/* One lane of a group item, slot k of factor I: w0 = (I, first slot a,
- * degree | type << 24, head slot | updated mask << 8 | recomputed mask << 16
- * | k << 24), w1 = (variable, var-major position) of slot k; degree 0 marks
- * a padding lane.  The lanes of one group sit side by side in one wave and
- * all lanes of the wave run this together. */
-__device__ __forceinline__ void seq_group_lane(const SeqArgs &A, const uint4 w0, const uint4 w1) {
-    const uint32_t I = w0.x, a = w0.y;
-    const int d = (int)(w0.z & 0xffffffu), h = (int)(w0.w & 0xffu), k = (int)(w0.w >> 24);
-    const uint32_t type = w0.z >> 24;
-    const bool tk = (w0.w >> (8 + k)) & 1u, ik = (w0.w >> (16 + k)) & 1u;   /* masks are 0 on padding */
+/* One lane of a group item, slot k of factor I: w = (I, first slot a,
+ * group_word(degree, type, head slot, updated mask, recomputed mask, k),
+ * variable of slot k); degree 0 marks a padding lane.  The lanes of one
+ * group sit side by side in one wave and all lanes of the wave run this
+ * together.  Every message the group touches lies in the factor's slot
+ * range (both message arrays are factor-major), so the variable's state is
+ * the lane's only scattered access. */
+__device__ __forceinline__ void seq_group_lane(const SeqArgs &A, const uint4 w) {
+    const uint32_t I = w.x, a = w.y, g = w.z;
+    const int d = (int)(g & 7u), h = (int)((g >> 5) & 7u), k = (int)((g >> 20) & 7u);
+    const uint32_t type = (g >> 3) & 3u;
+    const bool tk = (g >> (8 + k)) & 1u, ik = (g >> (14 + k)) & 1u;   /* masks are 0 on padding */
     /* everything the lane names, independent of each other */
     double2 m = make_double2(0.0, 0.0), old = make_double2(0.0, 0.0);
     VarState st{0.0, 0.0, 0ull, 0ull};
     FacRec f{0.0, 0.0, 0.0, 0.0};
     if (d > 0 && !ik) m = A.m_vf[a + k];
     if (tk || ik) {
-        st = A.state[w1.x];
-        old = A.m_fv_vm[w1.y];
+        st = A.state[w.w];
+        old = A.m_fv[a + k];
     }
     if (tk) f = A.fac[I];
     /* slot k's v->f message of this wave, from the state before the
@@ -205,8 +219,8 @@ __device__ __forceinline__ void seq_group_lane(const SeqArgs &A, const uint4 w0,
     if (old.y == 0) st.z1--; else st.p1 /= old.y;
     if (r.y == 0) st.z1++; else st.p1 *= r.y;
     scale_seq(st.p0, st.p1);
-    A.state[w1.x] = st;
-    A.m_fv_vm[w1.y] = r;
+    A.state[w.w] = st;
+    A.m_fv[a + k] = r;
 }
 
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_level(
@@ -232,10 +246,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_level(
      * the exchange */
     const uint32_t x = (blockIdx.x - vf_blocks - fv_blocks) * kBlock + threadIdx.x;
     const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-    const uint4 w0 = x < n_grp ? A.grp_items[2 * (uint64_t)x] : z;
-    const uint4 w1 = x < n_grp ? A.grp_items[2 * (uint64_t)x + 1] : z;
+    const uint4 w = x < n_grp ? A.grp_items[x] : z;
     if (chunk_stopped(ctl, k, tol) || x >= n_grp) return;
-    seq_group_lane(A, w0, w1);
+    seq_group_lane(A, w);
 }
 
 /* one item of a tail level: x indexes the level's group lanes (a multiple
@@ -256,9 +269,7 @@ __device__ __forceinline__ TailItem tail_fetch(const SeqArgs &A, const uint32_t 
     TailItem r = tail_none();
     const uint32_t nv = s_vf[l + 1] - s_vf[l], nf = s_fv[l + 1] - s_fv[l], ng = s_grp[l + 1] - s_grp[l];
     if (x < ng) {
-        const uint64_t g = 2 * (uint64_t)(s_grp[l] + x);
-        r.w[0] = A.grp_items[g];
-        r.w[1] = A.grp_items[g + 1];
+        r.w[0] = A.grp_items[s_grp[l] + x];
         r.kind = 3u;
     } else if (x - ng < nv) {
         r.w[0] = A.vf_items[s_vf[l] + x - ng];
@@ -272,7 +283,7 @@ __device__ __forceinline__ TailItem tail_fetch(const SeqArgs &A, const uint32_t 
 }
 
 __device__ __forceinline__ void tail_run(const SeqArgs &A, const TailItem &t) {
-    if (t.kind == 3u) seq_group_lane(A, t.w[0], t.w[1]);
+    if (t.kind == 3u) seq_group_lane(A, t.w[0]);
     else if (t.kind == 1u) seq_vf_item(A, t.w[0]);
     else if (t.kind == 2u) seq_fv_item(A, t.w[0], t.w[1]);
 }
