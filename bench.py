@@ -333,8 +333,13 @@ def bench_seq(args):
     info = sched.info()
     log(f"[bench] {args.updates}: V={V} E={E} waves {info.n_waves} levels {info.n_levels} "
         f"f->v items {info.n_fv_items} v->f items {info.n_vf_items} (schedule built in {t_sched:.2f}s)")
+    t0 = time.perf_counter()
     plan = fl.Plan(graph, device=0)
+    t_plan = time.perf_counter() - t0
+    t0 = time.perf_counter()
     plan.set_schedule(sched)
+    t_attach = time.perf_counter() - t0
+    log(f"[bench] plan upload {t_plan:.2f}s, schedule attached (items laid out and uploaded) in {t_attach:.2f}s")
     plan.use_graphs(args.graphs)
 
     def solve():

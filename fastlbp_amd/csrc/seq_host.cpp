@@ -108,7 +108,9 @@ void topo_order(const flbp_csr &c, std::vector<uint64_t> &out) {
                 const uint64_t I = x - V;
                 for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++) nbr.push_back(c.edge_var[s]);
             }
-            std::sort(nbr.begin(), nbr.end());
+            /* nbV and nbF lists are ascending already in a graph built from
+             * a factor list; other CSRs get sorted */
+            if (!std::is_sorted(nbr.begin(), nbr.end())) std::sort(nbr.begin(), nbr.end());
             nbr.erase(std::unique(nbr.begin(), nbr.end()), nbr.end());
             for (uint64_t y : nbr) {
                 if (rank[y] != N) continue;
@@ -118,8 +120,38 @@ void topo_order(const flbp_csr &c, std::vector<uint64_t> &out) {
         }
     }
     const std::vector<uint64_t> pos = slot_positions(c);
-    /* every (factor, variable) pair is distinct, so the 5-key order is
-     * total and a plain sort equals the reference's stable sort */
+    /* Ranks are distinct, so (later rank, earlier rank) names one pair of
+     * nodes and so one edge: the first two keys already order the edges
+     * totally, and a plain sort on them equals the reference's stable sort
+     * on all five.  With fewer than 2^32 nodes both ranks pack into one
+     * 64-bit key. */
+    if (N < (1ull << 32)) {
+        std::vector<std::pair<uint64_t, uint64_t>> kv;
+        kv.reserve(c.n_edges);
+        for (uint64_t I = 0; I < F; I++) {
+            for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++) {
+                const uint64_t a = rank[V + I], b = rank[c.edge_var[s]];
+                kv.push_back({(std::max(a, b) << 32) | std::min(a, b), pos[s]});
+            }
+        }
+        /* keys are distinct: an LSD radix sort over 16-bit digits, as many
+         * passes as the largest key needs */
+        uint64_t kmax = 0;
+        for (const auto &x : kv) kmax = std::max(kmax, x.first);
+        std::vector<std::pair<uint64_t, uint64_t>> tmp(kv.size());
+        std::vector<uint64_t> cnt(65537);
+        for (int shift = 0; shift < 64 && (kmax >> shift) != 0; shift += 16) {
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (const auto &x : kv) cnt[((x.first >> shift) & 0xffffu) + 1]++;
+            for (size_t d = 0; d < 65536; d++) cnt[d + 1] += cnt[d];
+            for (const auto &x : kv) tmp[cnt[(x.first >> shift) & 0xffffu]++] = x;
+            kv.swap(tmp);
+        }
+        out.clear();
+        out.reserve(kv.size());
+        for (const auto &x : kv) out.push_back(x.second);
+        return;
+    }
     struct Key {
         uint64_t target, source, factor, var, dual, edge;
         bool operator<(const Key &o) const {
@@ -158,53 +190,92 @@ int build_schedule(const flbp_csr &c, const uint64_t *order, uint64_t n, bool re
     S.respect_preorder = respect_preorder;
     S.order.assign(order, order + n);
 
-    const std::vector<uint32_t> sf = slot_factors(c);
-    std::vector<uint64_t> vm_var(E);
+    /* 32-bit copies of what the loops below read per edge, and one record
+     * per variable / slot / factor for what they keep, so that each visit
+     * of a random variable costs one cache line */
+    const uint64_t F = c.n_factors;
+    std::vector<uint32_t> fo(F + 1), sf(E), vm_var(E), vm_own(E), sv(E);
+    for (uint64_t I = 0; I <= F; I++) fo[I] = (uint32_t)c.fac_off[I];
+    for (uint64_t I = 0; I < F; I++)
+        for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++) sf[s] = (uint32_t)I;
+    for (uint64_t s = 0; s < E; s++) sv[s] = (uint32_t)c.edge_var[s];
     for (uint64_t i = 0; i < V; i++)
-        for (uint64_t k = c.var_off[i]; k < c.var_off[i + 1]; k++) vm_var[k] = i;
-    std::vector<uint64_t> rank(E, 0);
-    if (respect_preorder)
-        for (uint64_t k = 0; k < n; k++) rank[order[k]] = k;
+        for (uint64_t k = c.var_off[i]; k < c.var_off[i + 1]; k++) {
+            vm_var[k] = (uint32_t)i;
+            vm_own[k] = (uint32_t)c.var_edge[k];
+        }
+    std::vector<uint32_t> rank;
+    if (respect_preorder) {
+        rank.assign(E, 0);
+        for (uint64_t k = 0; k < n; k++) rank[order[k]] = (uint32_t)k;
+    }
 
-    /* membership stamps: equal to the current wave number = in the wave */
-    std::vector<uint32_t> v_stamp(V, 0), vf_stamp(E, 0);
-    std::vector<uint32_t> v_fac(V, 0);      /* factor updating the variable in the wave */
-    std::vector<uint64_t> v_edge(V, 0);     /* the edge doing it */
-    const std::vector<uint64_t> pos = slot_positions(c);
+    /* the wave split; stamps equal to the current wave number = in the wave */
+    struct VarW {
+        uint32_t stamp, fac, edge;   /* updated in wave stamp by factor fac through edge */
+    };
+    std::vector<VarW> vw(V, VarW{0u, 0u, 0u});
+    std::vector<uint32_t> vf_stamp(E, 0);
     uint32_t wave = 1;
     S.fv_wave_off.push_back(0);
     S.vf_wave_off.push_back(0);
+    S.fv_edge.reserve(n);
+    S.vf_slot.reserve(2 * n);
+    /* Each step's variable and factor slot range, gathered in order first
+     * (independent loads the core overlaps); the loop then prefetches the
+     * records of the variables a few steps ahead will visit. */
+    std::vector<uint32_t> ov(n), oI(n), os0(n), os1(n);
     for (uint64_t k = 0; k < n; k++) {
-        const uint64_t e = order[k];
-        const uint64_t i = vm_var[e];
-        const uint32_t I = sf[c.var_edge[e]];
-        bool conflict = v_stamp[i] == wave;
-        for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1] && !conflict; s++) {
-            const uint64_t j = c.edge_var[s];
+        const uint32_t e = (uint32_t)order[k];
+        const uint32_t I = sf[vm_own[e]];
+        ov[k] = vm_var[e];
+        oI[k] = I;
+        os0[k] = fo[I];
+        os1[k] = fo[I + 1];
+    }
+    constexpr uint64_t kAhead = 8;
+    for (uint64_t k = 0; k < n; k++) {
+        if (k + 2 * kAhead < n) {
+            __builtin_prefetch(&sv[os0[k + 2 * kAhead]]);
+            __builtin_prefetch(&vf_stamp[os0[k + 2 * kAhead]]);
+        }
+        if (k + kAhead < n) {
+            __builtin_prefetch(&vw[ov[k + kAhead]]);
+            for (uint32_t s = os0[k + kAhead]; s < os1[k + kAhead]; s++) __builtin_prefetch(&vw[sv[s]]);
+        }
+        const uint32_t e = (uint32_t)order[k];
+        const uint32_t i = ov[k];
+        const uint32_t I = oI[k];
+        const uint32_t s0 = os0[k], s1 = os1[k];
+        bool conflict = vw[i].stamp == wave;
+        for (uint32_t s = s0; s < s1 && !conflict; s++) {
+            const uint32_t j = sv[s];
             if (j == i || vf_stamp[s] == wave) continue;
-            if (v_stamp[j] == wave && v_fac[j] != I && (!respect_preorder || rank[v_edge[j]] < rank[e]))
-                conflict = true;
+            const VarW &x = vw[j];
+            if (x.stamp == wave && x.fac != I && (!respect_preorder || rank[x.edge] < rank[e])) conflict = true;
         }
         if (conflict) {
             S.fv_wave_off.push_back(S.fv_edge.size());
             S.vf_wave_off.push_back(S.vf_slot.size());
             if (++wave == 0) return fail(FLBP_ERANGE, "schedule: more than 2^32-1 waves");
         }
-        v_stamp[i] = wave;
-        v_fac[i] = I;
-        v_edge[i] = e;
-        if (S.vf_slot.size() + S.fv_edge.size() + (c.fac_off[I + 1] - c.fac_off[I]) >= (1ull << 32) - 1)
+        vw[i] = VarW{wave, I, e};
+        if (S.vf_slot.size() + S.fv_edge.size() + (s1 - s0) >= (1ull << 32) - 1)
             return fail(FLBP_ERANGE, "schedule: more than 2^32-2 items");
-        S.fv_edge.push_back((uint32_t)e);
-        for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++) {
-            if (c.edge_var[s] == i || vf_stamp[s] == wave) continue;
+        S.fv_edge.push_back(e);
+        for (uint32_t s = s0; s < s1; s++) {
+            if (sv[s] == i || vf_stamp[s] == wave) continue;
             vf_stamp[s] = wave;
-            S.vf_slot.push_back((uint32_t)s);
+            S.vf_slot.push_back(s);
         }
     }
     S.fv_wave_off.push_back(S.fv_edge.size());
     S.vf_wave_off.push_back(S.vf_slot.size());
     S.n_waves = S.fv_wave_off.size() - 1;
+    std::vector<uint32_t>().swap(vf_stamp);
+    std::vector<VarW>().swap(vw);
+    std::vector<uint32_t>().swap(os0);
+    std::vector<uint32_t>().swap(os1);
 
     /* Execution items and their dependency levels.  Per wave, the f->v
      * updates of one factor (of degree <= kSeqGroupDeg) form one group
@@ -222,20 +293,36 @@ int build_schedule(const flbp_csr &c, const uint64_t *order, uint64_t n, bool re
      * and wide f->v items touch disjoint locations: each variable is
      * updated at most once, a group reads only states no other item of the
      * wave writes, and each factor's slots belong to one item. */
-    S.slot_var.assign(c.edge_var, c.edge_var + E);
-    S.slot_pos.assign(pos.begin(), pos.end());
-    std::vector<uint32_t> w_var(V, 0), r_var(V, 0), w_slot(E, 0), r_slot(E, 0);
+    S.slot_var.assign(sv.begin(), sv.end());
+    S.slot_pos.resize(E);
+    for (uint64_t k = 0; k < E; k++) S.slot_pos[vm_own[k]] = (uint32_t)k;
+    struct VarL {
+        uint32_t w, r;             /* last level writing / reading the state */
+        uint32_t tgt_stamp, tgt_fac;   /* factor updating the variable in wave tgt_stamp */
+    };
+    struct SlotL {
+        uint32_t w, r;             /* last level writing / reading the v->f slot */
+    };
+    struct FacG {
+        uint32_t stamp, g;         /* group index of the factor in wave stamp */
+    };
+    std::vector<VarL> vl(V, VarL{0u, 0u, 0u, 0u});
+    std::vector<SlotL> sl(E, SlotL{0u, 0u});
+    std::vector<FacG> fg(F, FacG{0u, 0u});
     S.vf_level.assign(S.vf_slot.size(), 0);
     S.fv_level.assign(S.fv_edge.size(), 0);
-    std::vector<uint32_t> tgt_stamp(V, 0), tgt_fac(V, 0);   /* factor updating the variable in the wave */
-    std::vector<uint32_t> grp_stamp(c.n_factors, 0), grp_of(c.n_factors, 0);
     struct Group {
-        uint32_t I, tmask, imask;
-        std::vector<uint64_t> fv_x, vf_x;   /* reference items folded in */
+        uint32_t I, tmask, imask, L;
     };
     std::vector<Group> groups;
+    std::vector<uint32_t> fv_g, vf_g;   /* per item of the wave: its group, or ~0u */
     std::vector<uint64_t> wide_fv;
     uint32_t top = 0;
+    S.ex_kind.reserve(n);
+    S.ex_a.reserve(n);
+    S.ex_b.reserve(n);
+    S.ex_c.reserve(n);
+    S.ex_level.reserve(n);
     auto emit = [&](uint8_t kind, uint32_t a, uint32_t b, uint32_t cc, uint32_t L) {
         S.ex_kind.push_back(kind);
         S.ex_a.push_back(a);
@@ -246,81 +333,109 @@ int build_schedule(const flbp_csr &c, const uint64_t *order, uint64_t n, bool re
     };
     for (uint64_t w = 0; w < S.n_waves; w++) {
         const uint32_t stamp = (uint32_t)w + 1;
+        const uint64_t f0 = S.fv_wave_off[w], f1 = S.fv_wave_off[w + 1];
+        const uint64_t v0 = S.vf_wave_off[w], v1 = S.vf_wave_off[w + 1];
         groups.clear();
         wide_fv.clear();
-        for (uint64_t x = S.fv_wave_off[w]; x < S.fv_wave_off[w + 1]; x++) {
+        fv_g.assign(f1 - f0, ~0u);
+        vf_g.assign(v1 - v0, ~0u);
+        if (w + 1 < S.n_waves) {   /* the next wave's variable and factor records */
+            for (uint64_t x = f1; x < S.fv_wave_off[w + 2] && x < f1 + 16; x++) {
+                __builtin_prefetch(&vl[ov[x]]);
+                __builtin_prefetch(&fg[oI[x]]);
+            }
+        }
+        for (uint64_t x = f0; x < f1; x++) {
+            /* fv_edge[x] is order[x]: each order step is one f->v item */
             const uint32_t e = S.fv_edge[x];
-            const uint32_t own = (uint32_t)c.var_edge[e], I = sf[own];
-            tgt_stamp[vm_var[e]] = stamp;
-            tgt_fac[vm_var[e]] = I;
-            if (c.fac_off[I + 1] - c.fac_off[I] > kSeqGroupDeg) {
+            const uint32_t own = vm_own[e], I = oI[x];
+            VarL &t = vl[ov[x]];
+            t.tgt_stamp = stamp;
+            t.tgt_fac = I;
+            if (fo[I + 1] - fo[I] > kSeqGroupDeg) {
                 wide_fv.push_back(x);
                 continue;
             }
-            if (grp_stamp[I] != stamp) {
-                grp_stamp[I] = stamp;
-                grp_of[I] = (uint32_t)groups.size();
-                groups.push_back({I, 0u, 0u, {}, {}});
+            FacG &q = fg[I];
+            if (q.stamp != stamp) {
+                q.stamp = stamp;
+                q.g = (uint32_t)groups.size();
+                groups.push_back(Group{I, 0u, 0u, 0u});
             }
-            Group &G = groups[grp_of[I]];
-            G.tmask |= 1u << (own - c.fac_off[I]);
-            G.fv_x.push_back(x);
+            groups[q.g].tmask |= 1u << (own - fo[I]);
+            fv_g[x - f0] = q.g;
         }
         /* the wave's v->f messages: folded into their factor's group when
          * safe, else separate items (levels first, as they run first) */
-        for (uint64_t x = S.vf_wave_off[w]; x < S.vf_wave_off[w + 1]; x++) {
+        for (uint64_t x = v0; x < v1; x++) {
+            if (x + 16 < v1) __builtin_prefetch(&sv[S.vf_slot[x + 16]]);
+            if (x + 8 < v1) __builtin_prefetch(&vl[sv[S.vf_slot[x + 8]]]);
             const uint32_t s = S.vf_slot[x], I = sf[s];
-            const uint64_t j = c.edge_var[s];
-            const bool wide = c.fac_off[I + 1] - c.fac_off[I] > kSeqGroupDeg;
-            if (!wide && !(tgt_stamp[j] == stamp && tgt_fac[j] != I)) {
-                Group &G = groups[grp_of[I]];
-                G.imask |= 1u << (s - c.fac_off[I]);
-                G.vf_x.push_back(x);
+            const uint32_t j = sv[s];
+            const bool wide = fo[I + 1] - fo[I] > kSeqGroupDeg;
+            VarL &v = vl[j];
+            if (!wide && !(v.tgt_stamp == stamp && v.tgt_fac != I)) {
+                const uint32_t g = fg[I].g;
+                groups[g].imask |= 1u << (s - fo[I]);
+                vf_g[x - v0] = g;
                 continue;
             }
-            const uint32_t L = std::max(w_var[j], std::max(w_slot[s], r_slot[s])) + 1;
-            r_var[j] = std::max(r_var[j], L);
-            w_slot[s] = L;
+            SlotL &o = sl[s];
+            const uint32_t L = std::max(v.w, std::max(o.w, o.r)) + 1;
+            v.r = std::max(v.r, L);
+            o.w = L;
             S.vf_level[x] = L;
             emit(kExecVF, s, 0u, 0u, L);
         }
-        for (const Group &G : groups) {
-            const uint64_t a = c.fac_off[G.I], d = c.fac_off[G.I + 1] - a;
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+            Group &G = groups[gi];
+            if (gi + 8 < groups.size()) {
+                const uint32_t I8 = groups[gi + 8].I;
+                for (uint32_t sp = fo[I8]; sp < fo[I8 + 1]; sp++) __builtin_prefetch(&vl[sv[sp]]);
+            }
+            const uint32_t a = fo[G.I], d = fo[G.I + 1] - a;
             const bool multi = (G.tmask & (G.tmask - 1)) != 0;
             uint32_t L = 0;
-            for (uint64_t k = 0; k < d; k++) {
-                const uint64_t s = a + k, j = c.edge_var[s];
+            for (uint32_t k = 0; k < d; k++) {
+                const uint32_t s = a + k;
+                const VarL &v = vl[sv[s]];
+                const SlotL &o = sl[s];
                 const bool inl = (G.imask >> k) & 1u, tgt = (G.tmask >> k) & 1u;
-                if (inl) L = std::max(L, std::max(w_var[j], std::max(w_slot[s], r_slot[s])));
-                if (tgt) L = std::max(L, std::max(w_var[j], r_var[j]));
-                if (!inl && (multi || !tgt)) L = std::max(L, w_slot[s]);   /* read from m_vf */
+                if (inl) L = std::max(L, std::max(v.w, std::max(o.w, o.r)));
+                if (tgt) L = std::max(L, std::max(v.w, v.r));
+                if (!inl && (multi || !tgt)) L = std::max(L, o.w);   /* read from m_vf */
             }
             L += 1;
-            for (uint64_t k = 0; k < d; k++) {
-                const uint64_t s = a + k, j = c.edge_var[s];
+            for (uint32_t k = 0; k < d; k++) {
+                const uint32_t s = a + k;
+                VarL &v = vl[sv[s]];
+                SlotL &o = sl[s];
                 const bool inl = (G.imask >> k) & 1u, tgt = (G.tmask >> k) & 1u;
                 if (inl) {
-                    r_var[j] = std::max(r_var[j], L);
-                    w_slot[s] = L;
+                    v.r = std::max(v.r, L);
+                    o.w = L;
                 }
-                if (!inl && (multi || !tgt)) r_slot[s] = std::max(r_slot[s], L);
-                if (tgt) w_var[j] = L;
+                if (!inl && (multi || !tgt)) o.r = std::max(o.r, L);
+                if (tgt) v.w = L;
             }
-            for (uint64_t x : G.fv_x) S.fv_level[x] = L;
-            for (uint64_t x : G.vf_x) S.vf_level[x] = L;
+            G.L = L;
             emit(kExecGroup, G.I, G.tmask, G.imask, L);
         }
+        for (uint64_t x = f0; x < f1; x++)
+            if (fv_g[x - f0] != ~0u) S.fv_level[x] = groups[fv_g[x - f0]].L;
+        for (uint64_t x = v0; x < v1; x++)
+            if (vf_g[x - v0] != ~0u) S.vf_level[x] = groups[vf_g[x - v0]].L;
         for (uint64_t x : wide_fv) {
             const uint32_t e = S.fv_edge[x];
-            const uint64_t i = vm_var[e], own = c.var_edge[e];
-            const uint32_t I = sf[own];
-            uint32_t L = std::max(w_var[i], r_var[i]);
-            for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++)
-                if (s != own) L = std::max(L, w_slot[s]);
+            const uint32_t own = vm_own[e], I = oI[x];
+            VarL &t = vl[ov[x]];
+            uint32_t L = std::max(t.w, t.r);
+            for (uint32_t s = fo[I]; s < fo[I + 1]; s++)
+                if (s != own) L = std::max(L, sl[s].w);
             L += 1;
-            for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++)
-                if (s != own) r_slot[s] = std::max(r_slot[s], L);
-            w_var[i] = L;
+            for (uint32_t s = fo[I]; s < fo[I + 1]; s++)
+                if (s != own) sl[s].r = std::max(sl[s].r, L);
+            t.w = L;
             S.fv_level[x] = L;
             emit(kExecFV, e, 0u, 0u, L);
         }

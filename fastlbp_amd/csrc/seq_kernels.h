@@ -193,7 +193,10 @@ __device__ __forceinline__ void seq_group_lane(const SeqArgs &A, const uint4 w) 
     double2 m = make_double2(0.0, 0.0), old = make_double2(0.0, 0.0);
     VarState st{0.0, 0.0, 0ull, 0ull};
     FacRec f{0.0, 0.0, 0.0, 0.0};
-    if (d > 0 && !ik) m = A.m_vf[a + k];
+    /* a lone target's own v->f message is not an input of its update (the
+     * schedule's levels do not order it against writers of that slot) */
+    const uint32_t tmask = (g >> 8) & 63u;
+    if (d > 0 && !ik && !(tk && (tmask & (tmask - 1u)) == 0u)) m = A.m_vf[a + k];
     if (tk || ik) {
         st = A.state[w.w];
         old = A.m_fv[a + k];
