@@ -71,9 +71,9 @@ namespace flbp {
 namespace dev {
 
 constexpr int kSeqRegDeg = 8;       /* factors up to this degree keep messages in registers */
-constexpr int kSeqTailBlock = 512;  /* threads of the tail kernel (two waves per SIMD: 256 VGPRs each) */
+constexpr int kSeqTailBlock = 1024; /* threads of the tail kernel (four waves per SIMD: 128 VGPRs each) */
 constexpr int kSeqTailLevels = 256; /* level offsets the tail kernel keeps in LDS */
-constexpr int kSeqTailPer = 2;      /* items per thread and level in the tail kernel */
+constexpr int kSeqTailPer = 1;      /* items per thread and level in the tail kernel */
 
 /* a group lane's third word: degree (0 = padding lane) bits 0-2, factor
  * type 3-4, head slot 5-7, updated-slot mask 8-13, recomputed-slot mask
