@@ -17,6 +17,7 @@ bounded number of sweeps of the same graph on this host's cores).
     python bench.py --workload adjoint   # 100k-variable tied-rule gradient, backward vs forward
     python bench.py --workload c5 --steps 20   # tied-rule fit over 64 clamped variants of a 20M graph
     python bench.py --workload seq [--updates TOPO]   # SEQFIX / TOPO solve of the 1M graph vs PARALL
+    python bench.py --workload bingo     # Bingo protocol: BP command latency on the 1M graph
 """
 from __future__ import annotations
 
@@ -435,6 +436,100 @@ def bench_seq(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_bingo(args):
+    """The Bingo protocol (flbp_bingo, the reference's bingo_wrapper.cu):
+    BP command latency on the 1M-variable graph.  The session: NL (answered
+    once the file is read and the graph built), BP with no clamps, clamps on
+    ten alarms, BP again, Q on those alarms, BYE.  Each BP builds a new
+    CausalBP (updates SEQFIX) on the clamped graph and runs init + the
+    4-argument run; the tool logs its sweeps and its construct and run
+    times.  The oracle runs the first BP's solve (same shuffle, wave by
+    wave, one thread) and its yet-to-converge fraction must equal the tool's
+    at 17 digits."""
+    import re
+    import subprocess
+    import tempfile
+    import fastlbp_amd as fl
+
+    n_vars = args.n_vars or WORKLOAD["n_vars"]
+    tol, min_it, max_it, hist = 1e-6, 50, 500, 10
+    gen = fl.generate(n_vars, WORKLOAD["seed"])
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "bingo.causal_fg")
+    fl.write_causal_fg(gen.factors, path)
+    graph = fl.Graph(gen.factors)
+    V, F, E = graph.n_vars, graph.n_factors, graph.n_edges
+    alarms = [int(i) for i in graph.index_of(gen.alarms[:10])]   # the tool takes label ranks
+    errf = open(os.path.join(tmp, "bingo.log"), "w+")
+    t_start = time.perf_counter()
+    proc = subprocess.Popen([os.path.join(ROOT, "fastlbp_amd", "bin", "flbp_bingo"), path, "42", "--auto",
+                             "--digits", "17"], stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=errf,
+                            text=True, bufsize=1)
+
+    def ask(line):
+        t0 = time.perf_counter()
+        proc.stdin.write(line + "\n")
+        proc.stdin.flush()
+        out = proc.stdout.readline()
+        if not out:
+            raise RuntimeError(f"flbp_bingo ended at {line!r}")
+        return out.strip(), time.perf_counter() - t0
+
+    ask("NL")
+    t_load = time.perf_counter() - t_start
+    yet1, t_bp1 = ask(f"BP {tol} {min_it} {max_it} {hist}")
+    q1 = [ask(f"Q {a}")[0] for a in alarms]
+    for a in alarms:
+        ask(f"O {a} true")
+    yet2, t_bp2 = ask(f"BP {tol} {min_it} {max_it} {hist}")
+    qs = [ask(f"Q {a}")[0] for a in alarms]
+    proc.stdin.write("BYE\n")
+    proc.stdin.flush()
+    proc.wait(timeout=60)
+    errf.seek(0)
+    logs = errf.read()
+    done = [tuple(float(x) for x in m) for m in
+            re.findall(r"BP done: (\d+) sweeps over \d+ edges; construct \+ init ([\d.]+) ms, run ([\d.]+) ms", logs)]
+    if proc.returncode != 0 or len(done) != 2:
+        raise RuntimeError(f"flbp_bingo failed (rc {proc.returncode}):\n{logs[-2000:]}")
+    (sw1, c1, r1), (sw2, c2, r2) = done
+    log(f"[bench] bingo: V={V} E={E}; file read + graph build {t_load:.2f}s")
+    log(f"[bench] BP (no clamps): {sw1:.0f} sweeps, {t_bp1 * 1e3:.1f} ms (construct + init {c1:.1f} ms, run {r1:.1f} ms), "
+        f"yet-to-converge {yet1}")
+    log(f"[bench] BP (10 alarms clamped): {sw2:.0f} sweeps, {t_bp2 * 1e3:.1f} ms (construct + init {c2:.1f} ms, "
+        f"run {r2:.1f} ms), yet-to-converge {yet2}")
+    value = 2.0 * E * sw1 / (r1 * 1e-3)
+    out = {"metric": METRIC, "value": value, "unit": "msg-updates/s", "n_gpus": 1, "steps": 2, "warmup": 0,
+           "ms_per_step": t_bp1 * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"bingo-seqfix-fp64-{n_vars // 1000}k", "n_vars": V, "n_factors": F, "n_edges": E,
+                      "bp": [tol, min_it, max_it, hist], "seed": 42, "load_s": t_load,
+                      "bp_ms": [t_bp1 * 1e3, t_bp2 * 1e3], "construct_init_ms": [c1, c2], "run_ms": [r1, r2],
+                      "sweeps": [sw1, sw2], "yet_to_converge": [yet1, yet2], "q_alarms": qs,
+                      "value": "2 E x sweeps / run time of the first BP (the 4-argument run)"},
+           "roofline": None, "cpu_baseline": None}
+    if not args.no_cpu:
+        import oracle
+        flist = fl.read_causal_fg(path)        # the factors exactly as the tool read them
+        fl.seed(42)
+        sched = fl.Schedule(fl.Graph(flist), "SEQFIX")   # the shuffle the tool's first BP drew
+        o = oracle.OracleGraph(flist)
+        o.set_sequence(sched.order())
+        o.init()
+        t0 = time.perf_counter()
+        yet_o, low_o, _, _ = o.run_hist(tol, min_it, max_it, hist)
+        cdt = time.perf_counter() - t0
+        same = format(yet_o, ".17g") == yet1 and [format(low_o[a], ".17g") for a in alarms] == q1
+        out["cpu_baseline"] = {"value": 2.0 * E * sw1 / cdt, "unit": "msg-updates/s", "cores": 1, "kind": "port",
+                               "sample": f"the first BP's 4-argument run ({sw1:.0f} SEQFIX sweeps, wave by wave, "
+                                         f"one thread), {cdt:.1f} s; yet-to-converge fraction and the "
+                                         f"alarms' low-pass beliefs {'equal' if same else 'DIFFERENT'} to the "
+                                         f"tool's at 17 digits"}
+        log(f"[bench] oracle: first BP {cdt:.1f} s, yet-to-converge {yet_o!r}; fraction and queried beliefs "
+            f"{'equal to' if same else 'DIFFERENT from'} the tool's")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,7 +537,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["plain", "batched", "adjoint", "c5", "seq"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "batched", "adjoint", "c5", "seq", "bingo"], default="plain")
     ap.add_argument("--updates", choices=["SEQFIX", "TOPO"], default="SEQFIX", help="seq: the schedule")
     ap.add_argument("--graphs", action="store_true", help="replay chunks of sweeps from captured hipGraphs")
     ap.add_argument("--c5-variants", type=int, default=64, help="c5: clamped variants per gradient step")
@@ -476,6 +571,8 @@ def main():
         return bench_c5(args)
     if args.workload == "seq":
         return bench_seq(args)
+    if args.workload == "bingo":
+        return bench_bingo(args)
 
     import torch
     import fastlbp_amd as fl

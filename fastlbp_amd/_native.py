@@ -122,6 +122,15 @@ class RunResultC(C.Structure):
     ]
 
 
+class HistResultC(C.Structure):
+    _fields_ = [
+        ("yet_to_converge", C.c_double),
+        ("maxdiff", C.c_double),
+        ("iters", C.c_uint64),
+        ("reason", C.c_int),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "flbp_graph_build": (C.c_int, [C.POINTER(FactorListC), C.POINTER(C.c_void_p)]),
@@ -145,6 +154,8 @@ SIGNATURES = {
     "flbp_plan_init": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_plan_run": (C.c_int, [C.c_void_p, C.POINTER(RunOptsC), C.POINTER(RunResultC)]),
     "flbp_plan_run_fixed": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(RunResultC)]),
+    "flbp_plan_run_hist": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double,
+                                     C.c_double, C.POINTER(HistResultC), C.POINTER(C.c_double)]),
     "flbp_plan_beliefs": (C.c_int, [C.c_void_p, f64p, C.c_uint64]),
     "flbp_plan_reset": (C.c_int, [C.c_void_p]),
     "flbp_plan_get_messages": (C.c_int, [C.c_void_p, f64p, f64p]),

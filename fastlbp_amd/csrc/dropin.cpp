@@ -9,6 +9,7 @@
  * errors end the process through LOG_FATAL, as the reference's
  * CUDA_CHECK_ERROR does (include/utils/cuda_utils.h:16-21).
  */
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <iomanip>
@@ -89,24 +90,39 @@ std::ostream &operator<<(std::ostream &os, const CausalFactor &f) {
 /* ------------------------------------------------------------------ */
 
 CausalFactorGraph::CausalFactorGraph(const std::vector<CausalFactor> &P) : _G(), _factors(P) {
-    std::set<Var> all;
+    /* the variables ordered by label, the first one met of each label kept
+     * (the reference collects them in a std::set<Var>): a stable sort
+     * keeps the order of equal labels, unique keeps the first */
     size_t nrEdges = 0;
-    for (const CausalFactor &f : _factors) {
-        all.insert(f.vars().begin(), f.vars().end());
-        nrEdges += f.vars().size();
-    }
-    _vars.assign(all.begin(), all.end());
+    for (const CausalFactor &f : _factors) nrEdges += f.vars().size();
+    _vars.clear();
+    _vars.reserve(nrEdges);
+    for (const CausalFactor &f : _factors) _vars.insert(_vars.end(), f.vars().begin(), f.vars().end());
+    std::stable_sort(_vars.begin(), _vars.end());
+    _vars.erase(std::unique(_vars.begin(), _vars.end()), _vars.end());
+    _vars.shrink_to_fit();
     constructGraph(nrEdges);
 }
 
 void CausalFactorGraph::constructGraph(size_t nrEdges) {
+    /* label -> variable index: a table when labels are dense enough, else
+     * a hash map */
+    const size_t V = _vars.size();
+    const size_t max_label = V ? _vars.back().label() : 0;
+    std::vector<size_t> table;
     std::unordered_map<size_t, size_t> index;
-    index.reserve(_vars.size() * 2 + 1);
-    for (size_t i = 0; i < _vars.size(); i++) index[_vars[i].label()] = i;
+    const bool dense = V && max_label < 8 * V + 1024;
+    if (dense) {
+        table.assign(max_label + 1, 0);
+        for (size_t i = 0; i < V; i++) table[_vars[i].label()] = i;
+    } else {
+        index.reserve(V * 2 + 1);
+        for (size_t i = 0; i < V; i++) index[_vars[i].label()] = i;
+    }
     std::vector<Edge> edges;
     edges.reserve(nrEdges);
     for (size_t I = 0; I < _factors.size(); I++)
-        for (const Var &v : _factors[I].vars()) edges.emplace_back(index[v.label()], I);
+        for (const Var &v : _factors[I].vars()) edges.emplace_back(dense ? table[v.label()] : index[v.label()], I);
     _G.construct(_vars.size(), _factors.size(), edges.begin(), edges.end(), false);
 }
 
@@ -161,40 +177,6 @@ void check(int rc, const char *what) {
 
 /* Per-variable ring of the last `capacity` finite beliefs, averaged oldest
  * to newest (the 4-argument run's low-pass result, causal_bp.cu:28-78). */
-class BeliefHistory {
-    size_t n_, cap_;
-    std::vector<Real> val_;
-    std::vector<size_t> next_, count_;
-
-public:
-    BeliefHistory(size_t n, size_t cap) : n_(n), cap_(cap) {
-        if (cap_ == 0) throw std::invalid_argument("belief history length must be positive");
-        if (n_ > std::numeric_limits<size_t>::max() / sizeof(Real) / cap_)
-            throw std::length_error("belief history allocation size overflow");
-        val_.resize(n_ * cap_);
-        next_.assign(n_, 0);
-        count_.assign(n_, 0);
-    }
-    void push(size_t v, Real x) {
-        const int c = std::fpclassify(x);
-        if (c != FP_NORMAL && c != FP_SUBNORMAL && c != FP_ZERO) return;
-        size_t s = next_[v];
-        val_[s * n_ + v] = x;
-        next_[v] = (s + 1 == cap_) ? 0 : s + 1;
-        if (count_[v] < cap_) count_[v]++;
-    }
-    Real mean(size_t v) const {
-        const size_t c = count_[v];
-        size_t s = (c == cap_) ? next_[v] : 0;
-        Real sum = 0.0;
-        for (size_t k = 0; k < c; k++) {
-            sum += val_[s * n_ + v];
-            s = (s + 1 == cap_) ? 0 : s + 1;
-        }
-        if (c > 0) sum /= c;
-        return sum;
-    }
-};
 
 }  // namespace
 
@@ -332,44 +314,22 @@ Real CausalBP::run(Real tolerance, size_t maxIters, size_t histLength) {
 }
 
 Real CausalBP::run(Real tolerance, size_t minIters, size_t maxIters, size_t histLength) {
-    const Real tic = toc();
+    /* the history's own checks (HostBeliefHistory, causal_bp.cu:28-78);
+     * the loop and the history run in flbp_plan_run_hist */
     const size_t V = nrVars();
-    size_t numIters = 0;
-    Real maxDiff = INFINITY, yetToConverge = 1.0, nodeFracTolerance = 0.0;
-    BeliefHistory hist(V, histLength);
-    enum class Reason { ALL_CONVERGED, BIG_FRAC_CONVERGED, DIVERGED } reason = Reason::DIVERGED;
-    std::vector<Real> prev(V), cur(V);
-    check(flbp_plan_beliefs(_plan, prev.data(), V), "flbp_plan_beliefs");
-    for (;; numIters++, _iters++) {
-        if (numIters >= minIters) nodeFracTolerance = Real(numIters - minIters) / (maxIters - minIters);
-        if (maxDiff <= tolerance) {
-            reason = Reason::ALL_CONVERGED;
-            break;
-        } else if (numIters > minIters && yetToConverge < nodeFracTolerance) {
-            reason = Reason::BIG_FRAC_CONVERGED;
-            break;
-        } else if (numIters > maxIters || (toc() - tic) > props.maxtime) {
-            reason = Reason::DIVERGED;
-            break;
-        }
-        flbp_run_result r;
-        check(flbp_plan_run_fixed(_plan, 1, props.damping, &r), "flbp_plan_run_fixed");
-        check(flbp_plan_beliefs(_plan, cur.data(), V), "flbp_plan_beliefs");
-        maxDiff = r.maxdiff;
-        size_t nonConverged = 0;
-        for (size_t i = 0; i < V; i++) {
-            nonConverged += std::fabs(prev[i] - cur[i]) > tolerance;
-            hist.push(i, cur[i]);
-        }
-        yetToConverge = Real(nonConverged) / V;
-        prev.swap(cur);
-    }
-    if (maxDiff > _maxdiff) _maxdiff = maxDiff;
+    if (histLength == 0) throw std::invalid_argument("belief history length must be positive");
+    if (V > std::numeric_limits<size_t>::max() / sizeof(Real) / histLength)
+        throw std::length_error("belief history allocation size overflow");
+    std::vector<Real> low(V);
+    flbp_hist_result r;
+    check(flbp_plan_run_hist(_plan, tolerance, minIters, maxIters, histLength, props.maxtime, props.damping, &r,
+                             low.data()),
+          "flbp_plan_run_hist");
+    _iters += r.iters;
+    if (r.maxdiff > _maxdiff) _maxdiff = r.maxdiff;
     transferBeliefToHost();
-    _lowPassBeliefs.assign(V, 0.0);
-    for (size_t i = 0; i < V; i++)
-        _lowPassBeliefs[i] = reason == Reason::ALL_CONVERGED ? _beliefsV[i] : hist.mean(i);
-    return yetToConverge;
+    _lowPassBeliefs = std::move(low);
+    return r.yet_to_converge;
 }
 
 void CausalBP::calcBeliefV(size_t i, Prob &p) const { p = Prob(std::vector<Real>{1 - _beliefsV[i], _beliefsV[i]}); }

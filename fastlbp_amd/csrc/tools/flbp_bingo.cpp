@@ -16,7 +16,8 @@
  *   NL                                       prints an empty line
  *   BYE                                      ends the session
  * Variable indices are label ranks (CausalFactorGraph's var order).  Logs
- * go to stderr.
+ * go to stderr, among them each BP's sweeps and its construct + init and
+ * run times.
  *
  * As in the reference: seed reseeds the thread's rnd_gen before anything
  * else (each BP draws the next SEQFIX shuffle from it); clamps are applied
@@ -30,6 +31,7 @@
  * longer matches its adjacency (its construct writes past its buffers).
  * --auto takes device 0 (one process per GPU); --manual id takes device id.
  */
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -119,8 +121,16 @@ int main(int argc, char **argv) {
             size_t min_iters, max_iters, hist;
             std::cin >> tol >> min_iters >> max_iters >> hist;
             std::clog << "flbp_bingo: BP " << tol << " " << min_iters << " " << max_iters << " " << hist << std::endl;
+            const auto t0 = std::chrono::steady_clock::now();
             init_bp();
-            std::cout << g_bp->run(tol, min_iters, max_iters, hist) << std::endl;
+            const auto t1 = std::chrono::steady_clock::now();
+            const Real yet = g_bp->run(tol, min_iters, max_iters, hist);
+            const auto t2 = std::chrono::steady_clock::now();
+            std::cout << yet << std::endl;
+            std::clog << "flbp_bingo: BP done: " << g_bp->Iterations() << " sweeps over " << g_bp->nrEdges()
+                      << " edges; construct + init "
+                      << std::chrono::duration<double, std::milli>(t1 - t0).count() << " ms, run "
+                      << std::chrono::duration<double, std::milli>(t2 - t1).count() << " ms" << std::endl;
         } else if (cmd == "O") {
             int i;
             std::string v;

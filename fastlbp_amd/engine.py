@@ -14,8 +14,8 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._native import (UPDATES_PARALL, UPDATES_SEQFIX, UPDATES_TOPO, CsrC, FactorListC, GenParamsC, RunOptsC,
-                      RunResultC, ScheduleInfoC, call, check, f64p, i32p, lib, u8p, u64p)
+from ._native import (UPDATES_PARALL, UPDATES_SEQFIX, UPDATES_TOPO, CsrC, FactorListC, GenParamsC, HistResultC,
+                      RunOptsC, RunResultC, ScheduleInfoC, call, check, f64p, i32p, lib, u8p, u64p)
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -382,6 +382,16 @@ class Plan:
         r = RunResultC()
         call("flbp_plan_run_fixed", self._h, n, damping, C.byref(r))
         return RunResult(int(r.iters), int(r.sweeps), float(r.maxdiff), float(r.seconds))
+
+    def run_hist(self, tol: float, min_iters: int, max_iters: int, hist_len: int, maxtime: float = math.inf,
+                 damping: float = 0.0):
+        """CausalBP::run(tol, minIters, maxIters, histLength): (yet-to-converge
+        fraction, low-pass beliefs, reason 0/1/2, last max diff, sweeps)."""
+        r = HistResultC()
+        low = np.empty(max(self.graph.n_vars, 1), dtype=np.float64)
+        call("flbp_plan_run_hist", self._h, tol, min_iters, max_iters, hist_len, maxtime, damping, C.byref(r),
+             _ptr(low, C.c_double))
+        return float(r.yet_to_converge), low[:self.graph.n_vars], int(r.reason), float(r.maxdiff), int(r.iters)
 
     def beliefs(self) -> np.ndarray:
         out = np.empty(self.graph.n_vars, dtype=np.float64)

@@ -193,6 +193,24 @@ int flbp_plan_run(flbp_plan *plan, const flbp_run_opts *opts, flbp_run_result *r
 int flbp_plan_run_fixed(flbp_plan *plan, uint64_t n_sweeps, double damping,
                         flbp_run_result *res);
 
+/* CausalBP::run(tol, minIters, maxIters, histLength), the 4-argument run
+ * (causal_bp.cu:1006-1143): one sweep at a time until max-diff <= tol
+ * (reason 0), or, past min_iters, the fraction of variables whose belief
+ * moved by more than tol falls below (iters - min_iters) / (max_iters -
+ * min_iters) (reason 1), or iters > max_iters or maxtime seconds pass
+ * (reason 2).  Each sweep's finite beliefs enter a per-variable ring of
+ * hist_len values, kept on the device (the reference's HostBeliefHistory
+ * keeps it on the host); low_pass[V] (NULL skips) receives the beliefs for
+ * reason 0, else each variable's mean over its ring. */
+typedef struct flbp_hist_result {
+    double yet_to_converge;  /* the moved fraction of the last sweep (1.0 if none ran) */
+    double maxdiff;          /* max belief change of the last sweep (INFINITY if none ran) */
+    uint64_t iters;          /* sweeps run */
+    int reason;              /* 0 converged, 1 the fraction test, 2 max_iters or maxtime */
+} flbp_hist_result;
+int flbp_plan_run_hist(flbp_plan *plan, double tol, uint64_t min_iters, uint64_t max_iters, uint64_t hist_len,
+                       double maxtime, double damping, flbp_hist_result *res, double *low_pass);
+
 /* Back to the state of a freshly created plan: iteration counter 0 and old
  * beliefs 0.0 (the reference gets this only by constructing a new
  * CausalBP).  Messages are left as they are; call flbp_plan_init next. */

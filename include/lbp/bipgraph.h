@@ -3,6 +3,10 @@
  * neighbours are appended in edge order). */
 #pragma once
 
+#include <iterator>
+#include <type_traits>
+#include <vector>
+
 #include <lbp/graph.h>
 
 namespace lbp {
@@ -21,6 +25,18 @@ public:
                    bool check = true) {
         _nb1.assign(nrNodes1, Neighbors());
         _nb2.assign(nrNodes2, Neighbors());
+        using Cat = typename std::iterator_traits<EdgeInputIterator>::iterator_category;
+        if constexpr (std::is_base_of<std::forward_iterator_tag, Cat>::value) {
+            /* room for every node's edges first: one allocation per node
+             * (a second pass over the edges, so forward iterators only) */
+            std::vector<size_t> d1(nrNodes1, 0), d2(nrNodes2, 0);
+            for (EdgeInputIterator e = begin; e != end; ++e) {
+                d1[e->first]++;
+                d2[e->second]++;
+            }
+            for (size_t i = 0; i < nrNodes1; i++) _nb1[i].reserve(d1[i]);
+            for (size_t i = 0; i < nrNodes2; i++) _nb2[i].reserve(d2[i]);
+        }
         for (EdgeInputIterator e = begin; e != end; ++e) addEdge(e->first, e->second, check);
     }
 

@@ -113,3 +113,52 @@ def test_head_clamps_equal_the_reference_in_place_clamp(tmp_path):
     out = [l for l in run_session(path, 3, "\n".join(lines) + "\n", 17) if l and not l.startswith("O ")]
     assert out[0] == fmt(yet, 17)
     assert out[1:] == [fmt(session.low[i], 17) for i in range(0, g.n_vars, 97)]
+
+
+def same_or_both_nan(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+# (tol, minIters, maxIters, histLen): converges; stops on the moved-fraction
+# test; runs out of iterations with a history longer than the sweeps
+@pytest.mark.parametrize("args", [(1e-9, 20, 400, 5), (1e-5, 3, 40, 5), (1e-300, 10, 6, 3), (1e-300, 2, 3, 9)])
+@pytest.mark.parametrize("mode", ["PARALL", "PARALL-damped", "SEQFIX"])
+def test_device_history_run_equals_the_oracle(args, mode):
+    """flbp_plan_run_hist (history on the device) against the oracle's
+    restatement of the 4-argument run: sweeps, exit reason, moved fraction,
+    last max-diff and every low-pass belief bit for bit."""
+    tol, lo, hi, hist = args
+    gen = fl.generate(1200, 8)
+    g = fl.Graph(gen.factors)
+    damping = 0.3 if mode == "PARALL-damped" else 0.0
+    plan = fl.Plan(g)
+    o = oracle.OracleGraph(gen.factors)
+    if mode == "SEQFIX":
+        fl.seed(42)
+        s = fl.Schedule(g, "SEQFIX")
+        plan.set_schedule(s)
+        o.set_sequence(s.order())
+    plan.init()
+    o.init()
+    yet, low, reason, md, iters = plan.run_hist(tol, lo, hi, hist, damping=damping)
+    oyet, olow, oreason, omd = o.run_hist(tol, lo, hi, hist, damping)
+    assert (iters, reason) == (o.iters, oreason)
+    assert same_or_both_nan([yet, md], [oyet, omd])
+    assert same_or_both_nan(low, olow)
+    assert same_or_both_nan(plan.beliefs(), o.beliefs())
+
+
+def test_device_history_keeps_non_finite_beliefs_out():
+    gen = fl.generate(600, 9)
+    plan = fl.Plan(fl.Graph(gen.factors))
+    o = oracle.OracleGraph(gen.factors)
+    plan.init(logdomain=True)
+    o.init(logdomain=True)
+    yet, low, reason, md, iters = plan.run_hist(1e-300, 10, 4, 3)
+    oyet, olow, oreason, omd = o.run_hist(1e-300, 10, 4, 3)
+    assert reason == oreason == 2 and iters == o.iters
+    assert np.any(np.isnan(plan.beliefs()))
+    assert same_or_both_nan(low, olow) and same_or_both_nan([yet, md], [oyet, omd])
+    with pytest.raises(fl.FlbpError):
+        plan.run_hist(1e-6, 1, 5, 0)
