@@ -3,9 +3,9 @@
  * shared by the C ABI (host_graph.cpp) and the C++ drop-in classes.
  */
 #include <algorithm>
+#include <cerrno>
 #include <cstdio>
-#include <set>
-#include <sstream>
+#include <cstdlib>
 #include <string>
 
 #include "fastlbp.h"
@@ -49,6 +49,19 @@ bool to_int(const std::string &s, long long &v) {
     return true;
 }
 
+/* `is >> long` on the rest of a line: blanks, an optional sign, decimal
+ * digits; fails with no digits or out of range (strtol in the C locale
+ * accepts exactly that grammar) */
+bool next_long(const char *&cur, long &v) {
+    char *end = nullptr;
+    errno = 0;
+    const long x = std::strtol(cur, &end, 10);
+    if (end == cur || errno == ERANGE) return false;
+    cur = end;
+    v = x;
+    return true;
+}
+
 bool to_real(const std::string &s, double &v) {
     try {
         v = static_cast<double>(std::stold(s));
@@ -70,7 +83,11 @@ bool parse_causal_fg(std::istream &is, std::vector<ParsedFactor> &out, std::stri
         err = "Cannot read number of causal factors";
         return false;
     }
-    std::set<uint64_t> heads, all_vars;
+    /* every head and every label met, sorted and deduplicated at the end
+     * (the reference keeps two std::sets) */
+    std::vector<uint64_t> heads, all_vars;
+    heads.reserve((size_t)std::min<long long>(n, 1ll << 26));
+    all_vars.reserve((size_t)std::min<long long>(3 * n, 1ll << 27));
     for (long long I = 0; I < n; I++) {
         if (!next_line(is, line) || !line.empty())
             std::fprintf(stderr, "[ERROR] (causal_fg reader) Expecting empty line\n");
@@ -83,8 +100,8 @@ bool parse_causal_fg(std::istream &is, std::vector<ParsedFactor> &out, std::stri
         }
         ParsedFactor f;
         f.head = (uint64_t)head_id;
-        heads.insert(f.head);
-        all_vars.insert(f.head);
+        heads.push_back(f.head);
+        all_vars.push_back(f.head);
         next_line(is, line);
         skip_annotation(is);
         if (line.empty()) {
@@ -134,23 +151,30 @@ bool parse_causal_fg(std::istream &is, std::vector<ParsedFactor> &out, std::stri
             }
             next_line(is, line);
             skip_annotation(is);
-            std::istringstream body_ss(line);
+            const char *cur = line.c_str();
             for (long long b = 0; b < body_len; b++) {
                 long body_id;
-                if (!(body_ss >> body_id)) {
+                if (!next_long(cur, body_id)) {
                     err = "short body line in factor " + std::to_string(I);
                     return false;
                 }
                 f.body.push_back((uint64_t)body_id);
-                all_vars.insert((uint64_t)body_id);
+                all_vars.push_back((uint64_t)body_id);
             }
             std::sort(f.body.begin(), f.body.end());
             f.body.erase(std::unique(f.body.begin(), f.body.end()), f.body.end());
         }
         out.push_back(std::move(f));
     }
+    std::sort(heads.begin(), heads.end());
+    heads.erase(std::unique(heads.begin(), heads.end()), heads.end());
+    std::sort(all_vars.begin(), all_vars.end());
+    all_vars.erase(std::unique(all_vars.begin(), all_vars.end()), all_vars.end());
+    /* labels that head no factor get Singleton(0.5), in ascending order */
+    size_t h = 0;
     for (uint64_t lab : all_vars) {
-        if (!heads.count(lab)) out.push_back(ParsedFactor{0, lab, {}, 0.5, 0.0});
+        while (h < heads.size() && heads[h] < lab) h++;
+        if (h == heads.size() || heads[h] != lab) out.push_back(ParsedFactor{0, lab, {}, 0.5, 0.0});
     }
     return true;
 }

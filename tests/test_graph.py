@@ -147,6 +147,30 @@ def test_reader_errors_are_reported(tmp_path):
         fl.read_causal_fg(str(tmp_path / "missing.causal_fg"))
 
 
+@pytest.mark.skipif(not os.path.exists(oracle.REF_DUMP), reason="oracle/_ref not built")
+def test_body_lines_parse_like_the_reference(tmp_path):
+    """Body lines as `is >> long` reads them (causal_fg.cpp's body loop):
+    tabs and runs of blanks, a leading '+', duplicates, tokens past the
+    declared length and trailing junk after it -- against the reference's
+    own reader."""
+    path = tmp_path / "odd.causal_fg"
+    path.write_text("5\n\n"
+                    "10\n*0.9;0.05\n2\n\t+3 \t  1\n\n"
+                    "11\n+0.7;0.1\n3\n 4 4 2 99 junk\n\n"
+                    "12\n*;0.2\n1\n7 and more\n\n"
+                    "13\nI\n0.25\n\n"
+                    "14\n+\n2\n013 +10\n")
+    assert_same_dump(engine_as_dump(fl.read_causal_fg(str(path))), oracle.ref_dump(str(path)))
+
+
+def test_body_line_shorter_than_declared_is_an_error(tmp_path):
+    for body in ("3 x", "3 -", "3 99999999999999999999999"):
+        path = tmp_path / "short.causal_fg"
+        path.write_text(f"1\n\n10\n*0.9;0.05\n2\n{body}\n")
+        with pytest.raises(fl.FlbpError):
+            fl.read_causal_fg(str(path))
+
+
 def test_implicit_singletons_and_label_ranks():
     """Labels that head no factor get Singleton(0.5) appended in ascending
     order (causal_fg.cpp:264-269); var index = rank of label."""
