@@ -254,6 +254,17 @@ def test_rejections_under_a_schedule():
         plan.set_schedule(fl.Schedule(other, "TOPO"))
 
 
+@pytest.mark.parametrize("mode", ["SEQFIX", "TOPO"])
+def test_full_size_graph(mode):
+    """The benchmarked configuration: the 1M-variable graph (seed 2),
+    tol 1e-9, at most 500 sweeps.  Iteration count, last max-diff, every
+    marginal and every message equal the oracle's wave-by-wave solve (about
+    20 s of oracle time each)."""
+    gen = fl.generate(1_000_000, 2)
+    _, plan, o, r = run_both(gen.factors, mode, 1e-9, 500)
+    assert r.maxdiff <= 1e-9 and r.iters < 500
+
+
 def test_200k_seqfix_converges_like_the_oracle():
     gen = fl.generate(200000, 9)
     _, plan, o, r = run_both(gen.factors, "SEQFIX", 1e-9, 200)
