@@ -185,6 +185,11 @@ CausalBP::CausalBP(const CausalFactorGraph &fg, const PropertySet &opts) : DAIAl
     construct();
 }
 
+CausalBP::CausalBP(CausalFactorGraph &&fg, const PropertySet &opts) : DAIAlg<CausalFactorGraph>(std::move(fg)) {
+    setProperties(opts);
+    construct();
+}
+
 CausalBP::~CausalBP() {
     if (_plan) flbp_plan_destroy(_plan);
 }

@@ -24,11 +24,12 @@
  * to a copy of the graph in ascending index order; properties maxiter
  * 10000000, maxtime 57600 s, tol 1e-6, updates SEQFIX; numbers print with
  * the stream's default 6 significant digits (--digits changes that, not in
- * the reference).  Not as in the reference: the clamped copy is rebuilt
- * from its factors before inference.  That is the same graph whenever the
- * clamped variables appear in no factor body; a body clamp erases the
- * variable from the factor, and the reference's un-rebuilt copy then no
- * longer matches its adjacency (its construct writes past its buffers).
+ * the reference).  Not as in the reference: when a clamped variable
+ * appears in a factor body, the clamped copy is rebuilt from its factors
+ * before inference -- a body clamp erases the variable from the factor,
+ * and the reference's un-rebuilt copy then no longer matches its adjacency
+ * (its construct writes past its buffers).  With head clamps only the copy
+ * is used as it is, as in the reference.
  * --auto takes device 0 (one process per GPU); --manual id takes device id.
  */
 #include <chrono>
@@ -51,12 +52,26 @@ PropertySet g_opts;
 std::unique_ptr<CausalBP> g_bp;
 std::map<int, bool> g_clamps;
 
+/* does clamping variable i change a factor's variable set (i in a body)? */
+bool used_in_a_body(size_t i) {
+    for (const Neighbor &I : g_fg.nbV(i))
+        if (!(g_fg.factor(I.node).head() == g_fg.var(i))) return true;
+    return false;
+}
+
 void init_bp() {
     CausalFactorGraph clamped(g_fg);
-    for (const auto &c : g_clamps) clamped.clamp((size_t)c.first, c.second ? 1 : 0);
-    CausalFactorGraph rebuilt(clamped.factors());
+    bool body = false;
+    for (const auto &c : g_clamps) {
+        clamped.clamp((size_t)c.first, c.second ? 1 : 0);
+        body = body || used_in_a_body((size_t)c.first);
+    }
     g_bp.reset();
-    g_bp.reset(new CausalBP(rebuilt, g_opts));
+    if (body) {   /* the adjacency no longer matches the factors: rebuild */
+        g_bp.reset(new CausalBP(CausalFactorGraph(clamped.factors()), g_opts));
+    } else {      /* head clamps only: the clamped copy is consistent as it is */
+        g_bp.reset(new CausalBP(std::move(clamped), g_opts));
+    }
     g_bp->init();
 }
 

@@ -56,6 +56,9 @@ class CausalBP : public DAIAlg<CausalFactorGraph> {
 
 public:
     CausalBP(const CausalFactorGraph &fg, const PropertySet &opts);
+    /* Not in the reference: builds on a graph the caller gives up, without
+     * copying it (a server building one CausalBP per request). */
+    CausalBP(CausalFactorGraph &&fg, const PropertySet &opts);
     CausalBP(const CausalBP &) = delete;
     CausalBP &operator=(const CausalBP &) = delete;
     ~CausalBP();

@@ -2,6 +2,8 @@
  * include/lbp/daialg.h:14-33): DAIAlg<GRM> is-a GRM holding a copy. */
 #pragma once
 
+#include <utility>
+
 namespace lbp {
 
 class InfAlg {
@@ -14,6 +16,7 @@ class DAIAlg : public InfAlg, public GRM {
 public:
     DAIAlg() = default;
     DAIAlg(const GRM &grm) : InfAlg(), GRM(grm) {}
+    DAIAlg(GRM &&grm) : InfAlg(), GRM(std::move(grm)) {}   /* not in the reference: takes the graph over */
 };
 
 }  // namespace lbp
