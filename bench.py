@@ -140,6 +140,35 @@ def bench_batched(args):
     same = list(iters) == seq_iters and all(
         np.array_equal(b.beliefs(i), bel) for i, bel in zip((0, len(graphs) // 2, len(graphs) - 1), seq_bel))
     msgs = sum(2.0 * g.n_edges * int(it) for g, it in zip(graphs, iters))
+    # algorithmic bytes of the whole run: each graph's B_iter per sweep it ran
+    alg = sum((84.0 * g.n_edges + 24.0 * g.n_factors + 20.0 * g.n_vars) * int(it) for g, it in zip(graphs, iters))
+    achieved = alg / t_bat / 1e9
+    roofline = {"bound": "hbm", "kernel": "batched factor + var passes (whole run)", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": alg,
+                "timing": "wall time of the batched run (init excluded) over the algorithmic bytes of every "
+                          "sweep of every graph, 84 E + 24 F + 20 V per damped sweep"}
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = len(os.sched_getaffinity(0))
+        env_t = os.environ.get("OMP_NUM_THREADS")
+        if env_t and env_t.isdigit():
+            threads = min(threads, int(env_t))
+        done_msgs, n_done, t0 = 0.0, 0, time.perf_counter()
+        for g_i, n in enumerate(sizes):   # the batch's first generated graphs, solved to convergence
+            if time.perf_counter() - t0 >= args.cpu_seconds:
+                break
+            o = oracle.OracleGraph(fl.generate(int(n), 10_000 + g_i).factors)
+            o.init()
+            it_o, _ = o.run(tol, maxiter, damp, threads)
+            done_msgs += 2.0 * o.n_edges * it_o
+            n_done += 1
+        cdt = time.perf_counter() - t0
+        cpu = {"value": done_msgs / cdt, "unit": "msg-updates/s", "cores": threads, "kind": "port",
+               "sample": f"the batch's first {n_done} generated graphs solved to convergence, {cdt:.1f} s "
+                         f"(generation included), OpenMP over {threads} threads"}
+        log(f"[bench] cpu baseline {cpu['value']:.4g} msg-updates/s ({n_done} graphs, {threads} threads)")
     out = {"metric": METRIC, "value": msgs / t_bat, "unit": "msg-updates/s", "n_gpus": 1, "steps": 1,
            "warmup": 0, "ms_per_step": t_bat * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -149,7 +178,8 @@ def bench_batched(args):
                       "maxiter": maxiter, "batch_opts": best},
            "batched": {"seconds": t_bat, "sequential_seconds": t_seq, "speedup_vs_sequential": t_seq / t_bat,
                        "sequential_msg_updates_per_s": msgs / t_seq, "iterations_identical": bool(same),
-                       "max_sweeps": int(iters.max()), "device_bytes": b.device_bytes}}
+                       "max_sweeps": int(iters.max()), "device_bytes": b.device_bytes},
+           "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
