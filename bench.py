@@ -206,6 +206,28 @@ def bench_adjoint(args):
             bw.append(b_ms)
     f_ms, b_ms = float(np.median(fw)), float(np.median(bw))
     msgs = 2.0 * g.n_edges * 50
+    T, k = 50, 10
+    b_iter = 84.0 * g.n_edges + 24.0 * g.n_factors + 20.0 * g.n_vars   # one damped sweep
+    alg_fwd = T * b_iter + -(-T // k) * 32.0 * g.n_edges              # sweeps + checkpoints
+    alg_bwd = 3.0 * T * b_iter                                           # recompute + reverse
+    achieved = alg_bwd / (b_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "adjoint backward (segment recompute + reverse sweeps)",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": alg_bwd,
+                "forward_achieved": round(alg_fwd / (f_ms * 1e-3) / 1e9, 1),
+                "timing": "HIP events around the adjoint's forward and backward (median of the timed steps); "
+                          "bytes: T sweeps of 84 E + 24 F + 20 V forward plus 32 E per checkpoint, three times "
+                          "the sweeps backward"}
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oracle.grad(gen.factors, gen.rule, 200, theta, gen.alarms, y, T, 0.5)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": 1.0 / cdt, "unit": "gradients/s", "cores": 1, "kind": "port",
+               "sample": f"one orc_grad (unrolled forward + reverse mode, T = {T}) of the same problem, "
+                         f"{cdt:.2f} s, one thread"}
+        log(f"[bench] cpu baseline: one gradient in {cdt:.2f} s (1 thread); GPU {f_ms + b_ms:.2f} ms")
     out = {"metric": METRIC, "value": msgs / (f_ms * 1e-3), "unit": "msg-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": f_ms + b_ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -213,7 +235,8 @@ def bench_adjoint(args):
                       "n_rules": 200, "n_alarms": int(len(gen.alarms)), "T": 50, "checkpoint_every": 10,
                       "damping": 0.5},
            "adjoint": {"forward_ms": f_ms, "backward_ms": b_ms, "backward_over_forward": b_ms / f_ms,
-                       "device_bytes": prob.adj.device_bytes}}
+                       "gradients_per_s": 1e3 / (f_ms + b_ms), "device_bytes": prob.adj.device_bytes},
+           "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
