@@ -606,7 +606,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(it runs one GPU's share of a sharded job); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         import torch
