@@ -29,7 +29,7 @@ struct __align__(32) FacRec {
 
 constexpr int kMaxChunk = 64;   /* sweeps enqueued between host checks */
 constexpr uint32_t kStageItems = 256;    /* factors / variables per staged block (== kBlock) */
-constexpr uint32_t kStageSlots = 1024;   /* edge slots per staged block (== kStage) */
+constexpr uint32_t kStageSlots = 512;    /* edge slots per staged block (== kStage) */
 /* device footprint up to which the scatter layout is the default: random
  * 16-byte scatters beat random gathers while the messages stay in the
  * 256 MB Infinity Cache, and lose to them once they live in HBM */

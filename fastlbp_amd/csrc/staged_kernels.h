@@ -32,7 +32,7 @@ namespace {
 
 using namespace flbp::dev;
 
-constexpr int kStage = 1024;    /* slots staged per block (16 KiB per message array) */
+constexpr int kStage = 512;     /* slots staged per block (8 KiB per message array) */
 constexpr int kPerLane = kStage / kBlock;   /* staged slots per lane */
 
 struct LdsIn {
