@@ -460,6 +460,16 @@ __device__ __forceinline__ unsigned long long var_tile(
     return diff_bits;
 }
 
+/* Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8).
+ * Tile xcd_block(b, n) gives each XCD one contiguous run of tiles, so
+ * neighbouring tiles, whose scattered stores land in neighbouring lines,
+ * share one L2 and their partial-line writes merge there.  A bijection of
+ * [0, n): XCD x runs n / 8 tiles, plus one if x < n mod 8. */
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+    const uint32_t x = b & 7u, i = b >> 3, q = n >> 3, r = n & 7u;
+    return x * q + (x < r ? x : r) + i;
+}
+
 /* the factor pass of a plain solve: block b owns factors [blk[b], blk[b+1]) */
 template <bool kDamp, int N, int kLayout>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_staged(
@@ -484,8 +494,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_stage
     unsigned long long *__restrict__ ctl, int k, double tol) {
     if (chunk_stopped(ctl, k, tol)) return;
     __shared__ VarSmem sm;
+    /* the scatter layout's var pass writes its v->f messages into the
+     * factor-major lines of nearby factors: XCD-local runs of tiles measured
+     * ~1 us faster per launch at 1M variables, and slower for the other
+     * layouts and for the factor pass, which keep the plain order */
+    const uint32_t b = kLayout == kLayoutScatter ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const unsigned long long bits =
-        var_tile<kLayout>(var_off, var_edge, var_work, m_fv, m_vf, belief, blk[blockIdx.x], blk[blockIdx.x + 1], sm);
+        var_tile<kLayout>(var_off, var_edge, var_work, m_fv, m_vf, belief, blk[b], blk[b + 1], sm);
     block_fold_diff(ctl, k, bits, sm.red);
 }
 
