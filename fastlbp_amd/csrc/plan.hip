@@ -541,6 +541,9 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
 int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const double *vf_facmajor) {
     Plan *P = reinterpret_cast<Plan *>(plan);
     if (!P) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_messages: null plan");
+    if (P->seq && fv_varmajor)
+        return flbp::fail(FLBP_EINVAL, "flbp_plan_set_messages: a sequential schedule keeps its products in step "
+                                       "with the f->v messages; remove the schedule before setting them");
     HIP_TRY(hipSetDevice(P->device));
     if (fv_varmajor && P->E) {
         std::vector<double> tmp(2 * P->E);

@@ -227,6 +227,9 @@ int flbp_plan_beliefs_at(flbp_plan *plan, uint64_t n, const uint64_t *vars, doub
  * messages in var-major (nbV) order, v->f messages in factor-major order,
  * each as interleaved (state0, state1) pairs [2*E].  NULL skips one. */
 int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facmajor);
+/* (Not in the reference.)  Under a sequential schedule only the v->f
+ * messages may be set: the schedule keeps each variable's product of its
+ * f->v messages up to date incrementally, so setting f->v messages fails. */
 int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor,
                            const double *vf_facmajor);
 

@@ -234,6 +234,27 @@ def test_tail_setting_rejects_what_the_kernel_cannot_hold():
     plan.run(1e-9, 100)
 
 
+def test_messages_under_a_schedule():
+    """Export works under a schedule (the f->v messages are kept
+    factor-major there); setting f->v messages is refused, v->f accepted;
+    without the schedule the plan takes both again."""
+    gen = fl.generate(1500, 5)
+    g = fl.Graph(gen.factors)
+    plan = fl.Plan(g)
+    plan.set_schedule(schedule_for(g, "SEQFIX"))
+    plan.init()
+    plan.run(1e-9, 50)
+    fv, vf = plan.messages()
+    with pytest.raises(fl.FlbpError):
+        plan.set_messages(fv, None)
+    plan.set_messages(None, vf)
+    assert same_bits(plan.messages()[1], vf) and same_bits(plan.messages()[0], fv)
+    plan.set_schedule(None)                     # var-major copy rebuilt for PARALL
+    assert same_bits(plan.messages()[0], fv)
+    plan.set_messages(fv, vf)
+    assert same_bits(plan.messages()[0], fv)
+
+
 def test_rejections_under_a_schedule():
     gen = fl.generate(500, 2)
     g = fl.Graph(gen.factors)
