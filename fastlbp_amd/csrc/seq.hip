@@ -100,13 +100,14 @@ int Plan::launch_seq_sweep(int k, double tol) {
         const uint32_t f0 = seq_lvl_fv[l], nf = seq_lvl_fv[l + 1] - f0;
         const uint32_t g0 = seq_lvl_grp[l], ng = seq_lvl_grp[l + 1] - g0;
         if (!nv && !nf && !ng) continue;
-        const unsigned vb = grid_for(nv, dev::kBlock), fb = grid_for(nf, dev::kBlock), gb = grid_for(ng, dev::kBlock);
+        const unsigned bs = dev::kSeqLevelBlock;
+        const unsigned vb = grid_for(nv, bs), fb = grid_for(nf, bs), gb = grid_for(ng, bs);
         SeqArgs a = A;
         a.vf_items = seq_vf + v0;
         a.fv_items = seq_fv + f0;
         a.fv_aux = seq_fv_aux + f0;
         a.grp_items = seq_grp + g0;
-        hipLaunchKernelGGL(seq_level, dim3(vb + fb + gb), dim3(dev::kBlock), 0, stream, a, nv, nf, ng, vb, fb, ctl,
+        hipLaunchKernelGGL(seq_level, dim3(vb + fb + gb), dim3(bs), 0, stream, a, nv, nf, ng, vb, fb, ctl,
                            k, tol);
     }
     for (size_t l0 = seq_tail_from; l0 < L; l0 += dev::kSeqTailLevels) {

@@ -73,6 +73,7 @@ namespace dev {
 constexpr int kSeqRegDeg = 8;       /* factors up to this degree keep messages in registers */
 constexpr int kSeqTailBlock = 1024; /* threads of the tail kernel (four waves per SIMD: 128 VGPRs each) */
 constexpr int kSeqTailLevels = 256; /* level offsets the tail kernel keeps in LDS */
+constexpr int kSeqLevelBlock = 64;  /* threads per block of a level launch (one wave) */
 constexpr int kSeqTailPer = 1;      /* items per thread and level in the tail kernel */
 
 /* a group lane's third word: degree (0 = padding lane) bits 0-2, factor
@@ -226,18 +227,18 @@ __device__ __forceinline__ void seq_group_lane(const SeqArgs &A, const uint4 w) 
     A.m_fv[a + k] = r;
 }
 
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_level(
+__global__ __launch_bounds__(kSeqLevelBlock) __attribute__((unused)) void seq_level(
     SeqArgs A, uint32_t n_vf, uint32_t n_fv, uint32_t n_grp, uint32_t vf_blocks, uint32_t fv_blocks,
     const unsigned long long *__restrict__ ctl, int k, double tol) {
     if (blockIdx.x < vf_blocks) {
-        const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+        const uint32_t x = blockIdx.x * kSeqLevelBlock + threadIdx.x;
         const uint4 it = x < n_vf ? A.vf_items[x] : make_uint4(0u, 0u, 0u, 0u);
         if (chunk_stopped(ctl, k, tol) || x >= n_vf) return;
         seq_vf_item(A, it);
         return;
     }
     if (blockIdx.x < vf_blocks + fv_blocks) {
-        const uint32_t x = (blockIdx.x - vf_blocks) * kBlock + threadIdx.x;
+        const uint32_t x = (blockIdx.x - vf_blocks) * kSeqLevelBlock + threadIdx.x;
         const uint4 it = x < n_fv ? A.fv_items[x] : make_uint4(0u, 0u, 0u, 0u);
         const uint4 aux = x < n_fv ? A.fv_aux[x] : make_uint4(0u, 0u, 0u, 0u);
         if (chunk_stopped(ctl, k, tol) || x >= n_fv) return;
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void seq_level(
     /* group lanes: n_grp is a multiple of 64, so a wave is wholly inside
      * or wholly past the level and every lane of a running wave reaches
      * the exchange */
-    const uint32_t x = (blockIdx.x - vf_blocks - fv_blocks) * kBlock + threadIdx.x;
+    const uint32_t x = (blockIdx.x - vf_blocks - fv_blocks) * kSeqLevelBlock + threadIdx.x;
     const uint4 z = make_uint4(0u, 0u, 0u, 0u);
     const uint4 w = x < n_grp ? A.grp_items[x] : z;
     if (chunk_stopped(ctl, k, tol) || x >= n_grp) return;
