@@ -175,8 +175,11 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
         for (int kd = 0; kd < 3; kd++) at[kd].assign(off[kd].begin(), off[kd].end());
         for (size_t x = 0; x < nx; x++) {
             const int kd = S.ex_kind[x];
-            const uint64_t key = kd == flbp::kExecFV ? ((uint64_t)slot_fac[P->h_var_edge[S.ex_a[x]]] << 32) | S.ex_a[x]
-                                                     : (uint64_t)S.ex_a[x];
+            /* f->v items by factor; groups by degree, then factor, so the
+             * lanes of a wave mostly run loops of one length */
+            uint64_t key = S.ex_a[x];
+            if (kd == flbp::kExecFV) key |= (uint64_t)slot_fac[P->h_var_edge[S.ex_a[x]]] << 32;
+            if (kd == flbp::kExecGroup) key |= (P->h_fac_off[S.ex_a[x] + 1] - P->h_fac_off[S.ex_a[x]]) << 32;
             keyed[kd][at[kd][S.ex_level[x]]++] = {key, (uint32_t)x};
         }
     }
