@@ -165,6 +165,16 @@ def test_explicit_orders_with_repeats():
         o.init()
         o.run_fixed(5)
         assert_same_state(plan, o)
+        # run to a tolerance: the max belief change of every sweep, from the
+        # variables' last updates and from those no update reaches
+        plan.reset()
+        plan.init()
+        r = plan.run(1e-9, 60)
+        o = oracle_for(gen.factors, s)
+        o.init()
+        it, md = o.run(1e-9, 60)
+        assert r.iters == it and same_bits([r.maxdiff], [md])
+        assert_same_state(plan, o)
 
 
 def test_in_place_head_clamps_under_a_schedule():
