@@ -580,6 +580,9 @@ def bench_bingo(args):
                                          f"tool's at 17 digits"}
         log(f"[bench] oracle: first BP {cdt:.1f} s, yet-to-converge {yet_o!r}; fraction and queried beliefs "
             f"{'equal to' if same else 'DIFFERENT from'} the tool's")
+    errf.close()
+    import shutil
+    shutil.rmtree(tmp, ignore_errors=True)
     print(json.dumps(out), flush=True)
 
 
