@@ -387,8 +387,8 @@ typedef struct flbp_batch flbp_batch;
 #define FLBP_LAYOUT_SCATTER 2u
 
 typedef struct flbp_batch_opts {
-    uint32_t tile_factors;   /* edge slots per factor tile, 64..1024 (one block per tile at a time) */
-    uint32_t tile_vars;      /* edge slots per variable tile, 64..1024 */
+    uint32_t tile_factors;   /* edge slots per factor tile, 64..512 (one block per tile at a time) */
+    uint32_t tile_vars;      /* edge slots per variable tile, 64..512 */
     uint32_t grid_blocks;    /* persistent grid size; 0 = resident blocks */
     uint32_t chunk;          /* sweeps per hipGraph replay (<= 64) */
     uint32_t use_graph;      /* 0: launch the chunk's kernels directly */
