@@ -462,9 +462,10 @@ __device__ __forceinline__ unsigned long long var_tile(
 
 /* Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8).
  * Tile xcd_block(b, n) gives each XCD one contiguous run of tiles, so
- * neighbouring tiles, whose scattered stores land in neighbouring lines,
- * share one L2 and their partial-line writes merge there.  A bijection of
- * [0, n): XCD x runs n / 8 tiles, plus one if x < n mod 8. */
+ * neighbouring tiles, whose scattered stores or gathered loads touch
+ * neighbouring lines, share one L2: partial-line writes merge there and a
+ * gathered line is fetched once.  A bijection of [0, n): XCD x runs n / 8
+ * tiles, plus one if x < n mod 8. */
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
     const uint32_t x = b & 7u, i = b >> 3, q = n >> 3, r = n & 7u;
     return x * q + (x < r ? x : r) + i;
