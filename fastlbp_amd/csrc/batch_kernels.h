@@ -105,7 +105,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass
     double *__restrict__ belief, BatchCtl ctl, int k) {
     __shared__ VarSmem sm;
     const uint32_t n_tiles = *n_tiles_p;
-    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
+    /* each round of the loop covers tiles [base, base + span); in the pull
+     * layout the round's tiles go to the XCDs in runs of kPullVarRun, as in
+     * the plain var pass (465.7 vs 470.9 ms for the 768-graph batch) */
+    for (uint32_t base = 0; base + blockIdx.x < n_tiles; base += gridDim.x) {
+        const uint32_t left = n_tiles - base, span = left < gridDim.x ? left : gridDim.x;
+        const uint32_t ti = base + (kLayout == kLayoutPull ? xcd_runs<kPullVarRun>(blockIdx.x, span) : blockIdx.x);
         const Tile tile = tiles[ti];
         if (!graph_runs(ctl, tile.graph, k)) continue;   /* uniform per block */
         const BatchGraph gr = graphs[tile.graph];
