@@ -261,3 +261,112 @@ def test_damping_fixed_point_and_first_sweep():
     o2.init()
     o2.run(1e-15, 2000, damping=0.5)
     np.testing.assert_allclose(o2.beliefs(), exact_marginals(flist), atol=1e-10)
+
+
+# ------------------------------------------------------------------ damping
+#
+# Damping is build-defined (the reference has none): after the factor pass
+# has normalized an f->v message, m_fv <- (1 - d) * m_new + d * m_fv_old, on
+# both states, with the old state starting at init's unnormalized (1, 1)
+# (csrc/ext/causal_bp.cu:824-856).  The pins below use d = 0.3 and 0.8,
+# where d != 1 - d, so swapped weights, damping applied before the
+# normalization, or an old state other than the previous sweep's damped
+# message each fail one of them.
+
+def _fv_by_enumeration(flist, dump, vf):
+    """Every var-major f->v message recomputed from the factor-major v->f
+    messages vf[E, 2] by exact enumeration (undamped, normalized)."""
+    fac_off = dump["fac_off"].astype(np.int64)
+    var_src = dump["var_src"].astype(np.int64)
+    head_slot = dump["head_slot"].astype(np.int64)
+    out = []
+    for s in var_src:
+        I = int(np.searchsorted(fac_off, s, side="right") - 1)
+        lo, hi = int(fac_off[I]), int(fac_off[I + 1])
+        m = [(float(vf[j, 0]), float(vf[j, 1])) for j in range(lo, hi)]
+        kind = int(flist.type[I])
+        clamped = bool(flist.head_clamped[I])
+        mask = (float(flist.mask0[I]), float(flist.mask1[I]))
+        if kind == 0:
+            out.append((1.0 - float(flist.p[I]), float(flist.p[I])))
+            continue
+        out.append(exact_fv(kind, m, int(s) - lo, int(head_slot[I]) - lo, float(flist.p[I]),
+                            float(flist.q[I]), clamped, mask))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("d", [0.3, 0.8])
+def test_damping_weights_first_and_second_sweep(d):
+    """Sweep 1 from init: each f->v pair is (1-d) * normalize(new) + d * (1, 1),
+    with normalize(new) from exact enumeration over the factor (incoming v->f
+    messages all (1, 1)).  Sweep 2: (1-d) * normalize(new) + d * (sweep 1's
+    damped message), new again by enumeration from the exported v->f state."""
+    gen = fl.generate(40, 11, k_max=3, b_max=3)
+    flist = gen.factors
+    o = oracle.OracleGraph(flist)
+    dump = o.dump()
+    E = o.n_edges
+    o.init()
+    ones = np.ones((E, 2))
+    new1 = _fv_by_enumeration(flist, dump, ones)
+    o.sweep(d)
+    fv1, vf1 = o.messages()
+    want1 = (1 - d) * new1 + d * 1.0
+    np.testing.assert_allclose(fv1, want1, rtol=1e-13, atol=1e-15)
+    swapped = d * new1 + (1 - d) * 1.0
+    assert np.max(np.abs(swapped - want1)) > 1e-3          # the pin discriminates
+    new2 = _fv_by_enumeration(flist, dump, vf1)
+    o.sweep(d)
+    fv2, _ = o.messages()
+    want2 = (1 - d) * new2 + d * fv1
+    np.testing.assert_allclose(fv2, want2, rtol=1e-13, atol=1e-15)
+    assert np.max(np.abs((1 - d) * new2 + d * want1 - want2)) < 1e-12
+    assert np.max(np.abs((1 - d) * new2 + d * new1 - want2)) > 1e-4   # old = damped, not undamped
+
+
+@pytest.mark.parametrize("d", [Fraction(3, 10), Fraction(4, 5)])
+def test_damping_two_sweep_trajectory_by_hand(d):
+    """x has a prior a; h = AND(x) with p, q.  Worked by hand from the damping
+    rule, in rational arithmetic, from init's (1, 1) messages:
+
+    sweep 1:  F0->x = (1-d)(1-a, a) + d            (sum 1 + d)
+              F1->x = (1-d)(1/2, 1/2) + d          (head message (1, 1): flat)
+              F1->h = (1-d)((2-p-q)/2, (p+q)/2) + d
+              v->f:  x->F1 = F0->x / (1 + d),  h->F1 = (1/2, 1/2)
+    sweep 2:  F0->x = (1-d)(1-a, a) + d * F0->x(1)
+              F1->x = (1-d)(1/2, 1/2) + d * F1->x(1)   (head message flat)
+              F1->h = (1-d)((1-p)u1 + (1-q)u0, p u1 + q u0) + d * F1->h(1),
+                      (u0, u1) = x->F1, u0 + u1 = 1
+    beliefs:  b_x = F0->x[1] / sum (F1->x is flat), b_h = F1->h[1] / sum."""
+    a, p, q = Fraction(7, 10), Fraction(19, 20), Fraction(1, 10)
+    flist = fl.FactorList.build([(0, 1, [], float(a)), (1, 2, [1], float(p), float(q))])
+    o = oracle.OracleGraph(flist)
+    o.init()
+    f0x = ((1 - d) * (1 - a) + d, (1 - d) * a + d)
+    f1x = ((1 - d) / 2 + d, (1 - d) / 2 + d)
+    f1h = ((1 - d) * (2 - p - q) / 2 + d, (1 - d) * (p + q) / 2 + d)
+    o.sweep(float(d))
+    b1 = o.beliefs()
+    assert abs(b1[0] - float(f0x[1] / (f0x[0] + f0x[1]))) < 1e-15
+    assert abs(b1[1] - float(f1h[1] / (f1h[0] + f1h[1]))) < 1e-15
+    s = f0x[0] + f0x[1]
+    u0, u1 = f0x[0] / s, f0x[1] / s
+    f0x2 = ((1 - d) * (1 - a) + d * f0x[0], (1 - d) * a + d * f0x[1])
+    f1x2 = ((1 - d) / 2 + d * f1x[0], (1 - d) / 2 + d * f1x[1])
+    new_h = ((1 - p) * u1 + (1 - q) * u0, p * u1 + q * u0)
+    f1h2 = ((1 - d) * new_h[0] + d * f1h[0], (1 - d) * new_h[1] + d * f1h[1])
+    o.sweep(float(d))
+    b2 = o.beliefs()
+    fv2, _ = o.messages()
+    assert abs(b2[0] - float(f0x2[1] / (f0x2[0] + f0x2[1]))) < 1e-15
+    assert abs(b2[1] - float(f1h2[1] / (f1h2[0] + f1h2[1]))) < 1e-15
+    # var-major order: x's edges (F0, F1), then h's (F1)
+    want = np.array([[float(f0x2[0]), float(f0x2[1])], [float(f1x2[0]), float(f1x2[1])],
+                     [float(f1h2[0]), float(f1h2[1])]])
+    np.testing.assert_allclose(fv2, want, rtol=1e-15, atol=1e-16)
+    # the same trajectory with the weights swapped, or damping applied to the
+    # unnormalized F1->h of sweep 1, lands elsewhere
+    sw = ((d) * (2 - p - q) / 2 + (1 - d), d * (p + q) / 2 + (1 - d))
+    assert abs(float(sw[1] / (sw[0] + sw[1])) - b1[1]) > 1e-3
+    un = ((1 - d) * (2 - p - q) + d, (1 - d) * (p + q) + d)
+    assert abs(float(un[1] / (un[0] + un[1])) - b1[1]) > 1e-3
