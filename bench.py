@@ -268,11 +268,10 @@ def bench_c5(args):
     plan, adj = prob.plan, prob.adj
     aidx = prob.alarm_idx
     free = np.flatnonzero(g.body_uses(aidx) == 0)
-    heads = g.head_factors(aidx)
     variants = []
     for _ in range(n_var):
         pick = rng.choice(free, size=int(rng.integers(1, min(51, len(free) + 1))), replace=False)
-        variants.append((heads[pick], rng.integers(0, 2, pick.shape[0]).astype(np.uint8)))
+        variants.append((aidx[pick], rng.integers(0, 2, pick.shape[0]).astype(np.uint8)))
     log(f"[bench] c5 graph V={g.n_vars} F={g.n_factors} E={g.n_edges}, {n_var} variants, "
         f"setup {time.perf_counter() - t0:.1f}s, device bytes plan {plan.device_bytes / 1e9:.2f} GB "
         f"+ adjoint {adj.device_bytes / 1e9:.2f} GB")
@@ -285,7 +284,7 @@ def bench_c5(args):
         plan.set_p(p)
         out = []
         for f, x in variants:
-            plan.set_head_clamps(f, x)
+            plan.clamp_vars(f, x)
             adj.forward(download=False)
             out.append(plan.beliefs_at(aidx))
             plan.reset_head_clamps()
@@ -303,7 +302,7 @@ def bench_c5(args):
         adj.clear_grad()
         loss = 0.0
         for (f, x), y in zip(variants, targets):
-            plan.set_head_clamps(f, x)
+            plan.clamp_vars(f, x)
             adj.forward(download=False)
             b = plan.beliefs_at(aidx)
             adj.backward_sparse(aidx, 2.0 * (b - y))   # dL/dp summed over the variants on the device
@@ -324,7 +323,7 @@ def bench_c5(args):
     p_fit, bT = forward_all(theta)
     errs, iters = [], []
     for (f, x), b in zip(variants, bT):
-        plan.set_head_clamps(f, x)
+        plan.clamp_vars(f, x)
         plan.reset()
         plan.init()
         r = plan.run(1e-9, 500, damp)

@@ -170,6 +170,7 @@ SIGNATURES = {
     "flbp_plan_beliefs_at": (C.c_int, [C.c_void_p, C.c_uint64, u64p, f64p]),
     "flbp_plan_set_head_clamps": (C.c_int, [C.c_void_p, C.c_uint64, u64p, u8p]),
     "flbp_plan_reset_head_clamps": (C.c_int, [C.c_void_p]),
+    "flbp_plan_clamp_vars": (C.c_int, [C.c_void_p, C.c_uint64, u64p, u8p]),
     "flbp_batch_default_opts": (None, [C.POINTER(BatchOptsC)]),
     "flbp_batch_create": (C.c_int, [C.c_uint64, C.POINTER(C.POINTER(CsrC)), C.c_int, C.POINTER(BatchOptsC),
                                     C.POINTER(C.c_void_p)]),

@@ -124,13 +124,26 @@ class TiedProblem:
         self.plan = Plan(graph, device)
         self.adj = Adjoint(self.plan, n_sweeps, checkpoint_every, damping)
         self.p = self.base_p
+        self.last_theta = None   # theta of the forward the adjoint holds
 
     def marginals(self, theta: np.ndarray) -> np.ndarray:
+        theta = np.array(theta, np.float64, copy=True)
         self.p = tied_p(self.base_p, self.rule, theta)
         self.plan.set_p(self.p)
-        return self.adj.forward()[self.alarm_idx]
+        out = self.adj.forward()[self.alarm_idx]
+        self.last_theta = theta
+        return out
+
+    def grad_at(self, theta: np.ndarray, dl_dalarm: np.ndarray) -> np.ndarray:
+        """dL/dtheta at theta: the adjoint differentiates its last forward, so
+        the forward is run again first when that was at another theta."""
+        theta = np.asarray(theta, np.float64)
+        if self.last_theta is None or not np.array_equal(self.last_theta.view(np.uint64), theta.view(np.uint64)):
+            self.marginals(theta)
+        return self.grad(dl_dalarm)
 
     def grad(self, dl_dalarm: np.ndarray) -> np.ndarray:
+        """dL/dtheta at the theta of the last forward."""
         dl_db = np.zeros(self.graph.n_vars, np.float64)
         np.add.at(dl_db, self.alarm_idx, dl_dalarm)
         dl_dp = self.adj.backward(dl_db)
@@ -145,12 +158,15 @@ def tied_marginals_function():
         @staticmethod
         def forward(ctx, theta, problem: TiedProblem):
             ctx.problem = problem
-            b = problem.marginals(theta.detach().cpu().numpy().astype(np.float64))
+            # the theta of this node: several applications may run before one
+            # backward, and each must be differentiated at its own theta
+            ctx.theta = theta.detach().cpu().numpy().astype(np.float64).copy()
+            b = problem.marginals(ctx.theta)
             return torch.from_numpy(b)
 
         @staticmethod
         def backward(ctx, grad_out):
-            g = ctx.problem.grad(grad_out.detach().cpu().numpy().astype(np.float64))
+            g = ctx.problem.grad_at(ctx.theta, grad_out.detach().cpu().numpy().astype(np.float64))
             return torch.from_numpy(g), None
 
     return TiedMarginals

@@ -194,12 +194,12 @@ int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_
     if (P->seq)
         return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: the adjoint differentiates PARALL sweeps, and the plan "
                                        "has a sequential schedule");
+    if (P->max_fac_deg > (uint64_t)kAdjFacDeg)
+        return flbp::fail(FLBP_ERANGE, "flbp_adjoint_create: factor degree above 6");
     if (P->variant >= 2) {   /* the adjoint walks the pull layout: move the plan's messages there */
         const int rc = flbp_plan_set_variant(plan, 0);
         if (rc) return rc;
     }
-    if (P->max_fac_deg > (uint64_t)kAdjFacDeg)
-        return flbp::fail(FLBP_ERANGE, "flbp_adjoint_create: factor degree above 6");
     HIP_TRY(hipSetDevice(P->device));
     auto *A = new (std::nothrow) Adjoint();
     if (!A) return flbp::fail(FLBP_ENOMEM, "flbp_adjoint_create: out of memory");

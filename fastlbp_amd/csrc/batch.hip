@@ -409,6 +409,10 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
     if (opts) B->opts = *opts;
     else flbp_batch_default_opts(&B->opts);
     if (B->opts.layout > FLBP_LAYOUT_SCATTER) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: bad layout");
+    if (B->opts.tile_factors < 64 || B->opts.tile_factors > flbp::kStageSlots || B->opts.tile_vars < 64 ||
+        B->opts.tile_vars > flbp::kStageSlots)
+        return flbp::fail(FLBP_EINVAL, "flbp_batch_create: tile bounds must be 64.." +
+                                           std::to_string(flbp::kStageSlots) + " slots");
     if (B->opts.chunk == 0 || B->opts.chunk > (uint32_t)flbp::kMaxChunk) B->opts.chunk = flbp::kMaxChunk;
     if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(device);
     B->h_graphs.resize(n_graphs);
@@ -608,6 +612,10 @@ int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *o) {
     if (!B || !o) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: null argument");
     HIP_TRY(hipSetDevice(B->device));
     if (o->layout > FLBP_LAYOUT_SCATTER) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: bad layout");
+    if (o->tile_factors < 64 || o->tile_factors > flbp::kStageSlots || o->tile_vars < 64 ||
+        o->tile_vars > flbp::kStageSlots)
+        return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: tile bounds must be 64.." +
+                                           std::to_string(flbp::kStageSlots) + " slots");
     B->opts = *o;
     if (B->opts.chunk == 0 || B->opts.chunk > (uint32_t)flbp::kMaxChunk) B->opts.chunk = flbp::kMaxChunk;
     if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(B->device);
@@ -638,7 +646,7 @@ int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps, fl
     HIP_TRY(hipSetDevice(B->device));
     const flbp_batch_opts base = B->opts;
     const uint32_t g0 = (uint32_t)default_grid(B->device);
-    const uint32_t tiles[] = {512, flbp::kStageSlots};
+    const uint32_t tiles[] = {flbp::kStageSlots / 2, flbp::kStageSlots};
     const uint32_t grids[] = {g0, 2 * g0};
     const uint32_t layouts[] = {FLBP_LAYOUT_PULL, FLBP_LAYOUT_SCATTER};
     const int n = (int)std::min<uint32_t>(trial_sweeps, flbp::kMaxChunk);

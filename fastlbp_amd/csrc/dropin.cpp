@@ -293,7 +293,7 @@ void CausalBP::run() {
     flbp_run_result r;
     check(flbp_plan_run(_plan, &o, &r), "flbp_plan_run");
     _iters = r.iters;
-    _maxdiff = r.maxdiff;
+    _lastdiff = r.maxdiff;   /* the reference's run() keeps its maxDiff local (causal_bp.cu:864-890) */
     if (props.verbose >= 1) {
         if (r.maxdiff > props.tol) {
             if (props.verbose == 1) std::cerr << std::endl;

@@ -593,6 +593,11 @@ int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *facto
         if ((P->h_meta[factors[i]] & 3u) == 0)
             return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: a prior factor is clamped through its p "
                                            "(rebuild the graph with flbp_clamp)");
+        if (P->h_var_body[P->h_fac_head[factors[i]]])
+            return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: the head of factor " +
+                                               std::to_string(factors[i]) +
+                                               " appears in a factor body; its clamp changes the structure "
+                                               "(rebuild the graph with flbp_clamp)");
         if (values[i] > 1) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: values are 0 or 1");
     }
     HIP_TRY(hipSetDevice(P->device));
@@ -613,6 +618,29 @@ int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *facto
     if (int rc = flbp::seq_refresh_factors(P)) return rc;
     HIP_TRY(hipStreamSynchronize(P->stream));
     return FLBP_OK;
+}
+
+int flbp_plan_clamp_vars(flbp_plan *plan, uint64_t n, const uint64_t *vars, const uint8_t *values) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || (n && (!vars || !values))) return flbp::fail(FLBP_EINVAL, "flbp_plan_clamp_vars: null argument");
+    std::vector<uint64_t> fac;
+    std::vector<uint8_t> val;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t v = vars[i];
+        if (v >= P->V) return flbp::fail(FLBP_EINVAL, "flbp_plan_clamp_vars: variable out of range");
+        if (values[i] > 1) return flbp::fail(FLBP_EINVAL, "flbp_plan_clamp_vars: values are 0 or 1");
+        if (P->h_var_body[v])
+            return flbp::fail(FLBP_EINVAL, "flbp_plan_clamp_vars: variable " + std::to_string(v) +
+                                               " appears in a factor body; its clamp changes the structure "
+                                               "(rebuild the graph with flbp_clamp)");
+        if (P->h_head_off[v] == P->h_head_off[v + 1])
+            return flbp::fail(FLBP_EINVAL, "flbp_plan_clamp_vars: variable " + std::to_string(v) + " heads no factor");
+        for (uint32_t k = P->h_head_off[v]; k < P->h_head_off[v + 1]; k++) {
+            fac.push_back(P->h_head_fac[k]);
+            val.push_back(values[i]);
+        }
+    }
+    return flbp_plan_set_head_clamps(plan, fac.size(), fac.data(), val.data());
 }
 
 int flbp_plan_reset_head_clamps(flbp_plan *plan) {

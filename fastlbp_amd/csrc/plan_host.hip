@@ -297,6 +297,22 @@ int Plan::upload(const flbp_csr &c) {
     for (uint64_t i = 0; i < V; i++) max_var_deg = std::max<uint64_t>(max_var_deg, c.var_off[i + 1] - c.var_off[i]);
     h_meta0 = meta;
     h_meta.swap(meta);
+    /* head of each factor, body uses of each variable, factors per head */
+    h_fac_head.assign(F, 0);
+    h_var_body.assign(V, 0);
+    h_head_off.assign(V + 1, 0);
+    for (uint64_t I = 0; I < F; I++) {
+        h_fac_head[I] = (uint32_t)c.edge_var[c.head_slot[I]];
+        h_head_off[h_fac_head[I] + 1]++;
+        for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++)
+            if (s != c.head_slot[I]) h_var_body[c.edge_var[s]] = 1;
+    }
+    for (uint64_t i = 0; i < V; i++) h_head_off[i + 1] += h_head_off[i];
+    h_head_fac.assign(F, 0);
+    {
+        std::vector<uint32_t> fill(h_head_off.begin(), h_head_off.end() - 1);
+        for (uint64_t I = 0; I < F; I++) h_head_fac[fill[h_fac_head[I]]++] = (uint32_t)I;
+    }
     h_inv_perm.assign(V, 0);
     for (uint64_t dv = 0; dv < V; dv++) h_inv_perm[h_perm[dv]] = (uint32_t)dv;
     variant = device_bytes <= kScatterMaxBytes ? 3 : 0;

@@ -99,6 +99,9 @@ struct Plan {
     std::vector<uint32_t> h_meta, h_meta0;         /* factor meta words: current and at creation */
     std::vector<double2> h_mask, h_mask0;          /* head masks: current and at creation */
     std::vector<uint64_t> clamp_dirty;             /* factors whose meta / mask differ from creation */
+    std::vector<uint32_t> h_fac_head;              /* head variable of each factor */
+    std::vector<uint8_t> h_var_body;               /* 1: the variable appears in some factor body */
+    std::vector<uint32_t> h_head_off, h_head_fac;  /* factors headed by each variable (CSR, factor order) */
 
     /* sequential schedule (updates = SEQFIX / TOPO, seq.hip); none = PARALL */
     bool seq = false;

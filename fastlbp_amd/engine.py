@@ -218,18 +218,25 @@ class Graph:
                     _ptr(self.mask0, C.c_double), _ptr(self.mask1, C.c_double))
 
     def head_factors(self, var_indices) -> np.ndarray:
-        """The defining factor (the factor whose head it is) of each variable,
-        -1 for a variable that heads none."""
-        head_var = self.edge_var[self.head_slot]
+        """The first factor (lowest index) whose head each variable is, -1 for
+        a variable that heads none.  A variable may head several factors;
+        Plan.clamp_vars clamps all of them."""
+        head_var = self.edge_var[self.head_slot].astype(np.int64)
         out = np.full(self.n_vars, -1, dtype=np.int64)
-        out[head_var] = np.arange(self.n_factors)
+        out[head_var[::-1]] = np.arange(self.n_factors)[::-1]
         return out[np.asarray(var_indices, dtype=np.int64)]
+
+    def n_head_factors(self, var_indices) -> np.ndarray:
+        """Number of factors whose head each variable is."""
+        head_var = self.edge_var[self.head_slot].astype(np.int64)
+        cnt = np.bincount(head_var, minlength=self.n_vars)
+        return cnt[np.asarray(var_indices, dtype=np.int64)]
 
     def body_uses(self, var_indices) -> np.ndarray:
         """Factors in which each variable appears other than as the head."""
         v = np.asarray(var_indices, dtype=np.int64)
         deg = (self.var_off[v + 1] - self.var_off[v]).astype(np.int64)
-        return deg - (self.head_factors(v) >= 0)
+        return deg - self.n_head_factors(v)
 
     def index_of(self, labels) -> np.ndarray:
         labels = np.asarray(labels, dtype=np.uint64)
@@ -430,6 +437,16 @@ class Plan:
         if f.shape != x.shape:
             raise ValueError("factors and values differ in length")
         call("flbp_plan_set_head_clamps", self._h, f.shape[0], _ptr(f, C.c_uint64), _ptr(x, C.c_uint8))
+
+    def clamp_vars(self, var_indices, values) -> None:
+        """CausalFactorGraph::clamp in place for variables without body uses:
+        every AND/OR factor they head gets the head clamp (equals the rebuilt
+        clamped graph)."""
+        v = np.ascontiguousarray(var_indices, dtype=np.uint64)
+        x = np.ascontiguousarray(values, dtype=np.uint8)
+        if v.shape != x.shape:
+            raise ValueError("variables and values differ in length")
+        call("flbp_plan_clamp_vars", self._h, v.shape[0], _ptr(v, C.c_uint64), _ptr(x, C.c_uint8))
 
     def reset_head_clamps(self) -> None:
         call("flbp_plan_reset_head_clamps", self._h)

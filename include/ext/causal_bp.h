@@ -39,7 +39,8 @@ class CausalBP : public DAIAlg<CausalFactorGraph> {
         Real damping;
         size_t device;
     } props;
-    Real _maxdiff = 0;
+    Real _maxdiff = 0;        /* the reference leaves it uninitialised; only the 4-argument run raises it */
+    Real _lastdiff = INFINITY; /* last max belief change of run() (not in the reference) */
     size_t _iters = 0;
     struct EdgeProp {
         Prob message;
@@ -75,7 +76,11 @@ public:
 
     /* not in the reference */
     size_t Iterations() const { return _iters; }
+    /* the running maximum the 4-argument run keeps (causal_bp.cu:1101-1102);
+     * run() leaves it alone, as in the reference */
     Real maxDiff() const { return _maxdiff; }
+    /* the max belief change of the last sweep of the last run() */
+    Real lastMaxDiff() const { return _lastdiff; }
     const std::vector<Real> &marginals() const { return _beliefsV; }
 };
 

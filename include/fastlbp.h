@@ -240,14 +240,22 @@ int flbp_plan_set_p(flbp_plan *plan, const double *p, uint64_t n);
 /* Head clamps applied in place: for each listed AND/OR factor, the head
  * rule of CausalFactor::gen_clamped (csrc/ext/causal_factor.cpp:75-83) --
  * head_clamped set, mask (1, 0) for value 0 or (0, 1) for value 1,
- * multiplied into an existing mask.  The graph structure is unchanged, so
- * this equals rebuilding the graph clamped at variables that appear in no
- * factor body (CausalFactorGraph::clamp, causal_fg.cpp:17-26); a clamp of a
- * variable with body uses, or of a prior (Singleton) factor, needs the
- * rebuild and is rejected here (FLBP_EINVAL).  Not in the reference: it
- * lets a training loop switch between clamped variants without rebuilding
- * the plan. */
+ * multiplied into an existing mask.  The graph structure is unchanged.  A
+ * factor whose head appears in some factor body (its clamp would also
+ * erase or collapse those body uses, causal_factor.cpp:84-95), and a prior
+ * (Singleton) factor, need the rebuild (flbp_clamp) and are rejected here
+ * (FLBP_EINVAL).  CausalFactorGraph::clamp (causal_fg.cpp:17-26) clamps
+ * every factor of the variable; flbp_plan_clamp_vars below does that.  Not
+ * in the reference: it lets a training loop switch between clamped
+ * variants without rebuilding the plan. */
 int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *factors, const uint8_t *values);
+/* CausalFactorGraph::clamp(i, x) (causal_fg.cpp:17-26) in place, for
+ * variables (indices) that appear in no factor body: every AND/OR factor
+ * headed by vars[k] gets the head clamp of values[k], as
+ * flbp_plan_set_head_clamps does.  This equals rebuilding the clamped graph
+ * (fl_wrapper.cu:26-45).  A variable with body uses, one that heads no
+ * factor or only a prior, is rejected (FLBP_EINVAL) and nothing changes. */
+int flbp_plan_clamp_vars(flbp_plan *plan, uint64_t n, const uint64_t *vars, const uint8_t *values);
 /* every head clamp and mask back to the plan's creation values */
 int flbp_plan_reset_head_clamps(flbp_plan *plan);
 

@@ -121,7 +121,7 @@ def test_gradient_with_in_place_head_clamps():
     theta = rng.normal(2.5, 0.3, 50)
     y = rng.integers(0, 2, len(gen.alarms)).astype(np.float64)
     inplace = TiedProblem(g, gen.rule, 50, gen.alarms, 20, 5, 0.5)
-    inplace.plan.set_head_clamps(g.head_factors(g.index_of(labels)), values)
+    inplace.plan.clamp_vars(g.index_of(labels), values)
     rebuilt = TiedProblem(fl.Graph(fl.clamp(gen.factors, labels, values)), gen.rule, 50, gen.alarms, 20, 5, 0.5)
     la, ga = loss_and_grad(inplace, theta, y)
     lb, gb = loss_and_grad(rebuilt, theta, y)
@@ -143,12 +143,11 @@ def test_sparse_backward_and_device_rule_grad():
     prob.plan.set_p(p)
     aidx = prob.alarm_idx
     free = np.flatnonzero(g.body_uses(aidx) == 0)
-    heads = g.head_factors(aidx)
     adj = prob.adj
     adj.set_rules(prob.rule, 40)
     for trial in range(2):
         pick = rng.choice(free, size=10, replace=False)
-        prob.plan.set_head_clamps(heads[pick], rng.integers(0, 2, 10).astype(np.uint8))
+        prob.plan.clamp_vars(aidx[pick], rng.integers(0, 2, 10).astype(np.uint8))
         full = adj.forward()
         assert same_bits(prob.plan.beliefs_at(aidx), full[aidx])
         gb = rng.normal(0, 1, len(aidx))

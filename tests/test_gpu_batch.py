@@ -33,7 +33,7 @@ def plain(graph, tol, maxiter, damping):
 
 @pytest.mark.parametrize("opts", [dict(), dict(tile_factors=256, tile_vars=512, grid_blocks=37, chunk=5),
                                   dict(use_graph=0, chunk=16), dict(tile_factors=64, tile_vars=96, grid_blocks=3),
-                                  dict(tile_factors=4096, tile_vars=4096), dict(layout=1), dict(layout=2),
+                                  dict(layout=1), dict(layout=2),
                                   dict(layout=1, use_graph=0, chunk=9)])
 def test_batch_matches_plain_solves(opts):
     graphs = batch_graphs(1, 12, 6)
@@ -59,11 +59,21 @@ def test_batch_maxiter_cap_and_undamped():
         assert iters[g] == it and same_bits(b.beliefs(g), bp)
 
 
+def test_tile_bounds_outside_the_documented_range_are_rejected():
+    graphs = batch_graphs(2, 2, 3)
+    for bad in (dict(tile_factors=4096, tile_vars=4096), dict(tile_factors=32), dict(tile_vars=1024)):
+        with pytest.raises(fl.FlbpError):
+            Batch(graphs, **bad)
+    b = Batch(graphs)
+    with pytest.raises(fl.FlbpError):
+        b.set_opts(tile_factors=1024)
+
+
 def test_autotune_keeps_results():
     graphs = batch_graphs(3, 8, 4)
     b = Batch(graphs)
     best = b.autotune(0.3, trial_sweeps=4)
-    assert best["tile_factors"] in (512, 1024) and best["layout"] in (1, 2)
+    assert best["tile_factors"] in (256, 512) and best["layout"] in (1, 2)
     iters, _, _ = b.run(1e-9, 300, 0.3)
     for g, graph in enumerate(graphs):
         it, _, bp = plain(graph, 1e-9, 300, 0.3)

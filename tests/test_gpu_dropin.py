@@ -70,3 +70,28 @@ def test_fl_wrapper_cli_matches_oracle(tmp_path):
         want = b if 0 <= b <= 1 else (float(clamps[l]) if l in clamps else 0.5)
         assert marg[l] == want, (l, marg[l], want)
     assert len(marg) == int(labs.max()) + 1
+
+
+@pytest.mark.gpu
+def test_maxdiff_member_follows_the_reference(tmp_path):
+    """maxDiff() is the member the 4-argument run raises to its last max
+    belief change (causal_bp.cu:1101-1102); run() keeps its own maxDiff
+    local and leaves the member alone (causal_bp.cu:864-890).  A fresh
+    CausalBP holds 0."""
+    gen = fl.generate(2000, 3)
+    path = str(tmp_path / "g.causal_fg")
+    fl.write_causal_fg(gen.factors, path)
+    tool = os.path.join(BIN, "flbp_run")
+    r = subprocess.run([tool, path, "1e-6", "1000", "0"], check=True, capture_output=True, text=True, timeout=120)
+    md = [float(ln.split()[1]) for ln in r.stderr.split("\n") if ln.startswith("maxDiff ")]
+    assert md == [0.0, 0.0]
+    r = subprocess.run([tool, path, "1e-6", "1000", "0", "PARALL", "0", "5,50,3"], check=True,
+                       capture_output=True, text=True, timeout=120)
+    md = [float(ln.split()[1]) for ln in r.stderr.split("\n") if ln.startswith("maxDiff ")]
+    o = oracle.OracleGraph(fl.read_causal_fg(path))
+    o.init()
+    _, _, _, hist_md = o.run_hist(1e-6, 5, 50, 3)
+    assert md[0] == hist_md and md[1] == hist_md      # run() leaves it alone
+    it, last = o.run(1e-6, 1000)
+    lines = r.stdout.strip().split("\n")
+    assert int(lines[0].split()[1]) == it and float(lines[1].split()[1]) == last

@@ -215,7 +215,7 @@ def test_head_clamps_in_place_equal_rebuilt_graph():
         ref.init()
         r_ref = ref.run(1e-9, 300, 0.3)
         plan = fl.Plan(g)
-        plan.set_head_clamps(g.head_factors(g.index_of(labels)), values)
+        plan.clamp_vars(g.index_of(labels), values)
         plan.init()
         r = plan.run(1e-9, 300, 0.3)
         assert r.iters == r_ref.iters
@@ -232,6 +232,14 @@ def test_head_clamps_in_place_equal_rebuilt_graph():
     with pytest.raises(fl.FlbpError):
         prior = np.flatnonzero(g.fac_type == 0)[:1]
         plan.set_head_clamps(prior, np.zeros(1, np.uint8))
+    # a head that appears in some factor body needs the rebuild
+    used = np.flatnonzero((g.body_uses(np.arange(g.n_vars)) > 0) & (g.n_head_factors(np.arange(g.n_vars)) > 0))
+    fac = g.head_factors(used)
+    fac = fac[g.fac_type[fac] != 0][:1]
+    with pytest.raises(fl.FlbpError):
+        plan.set_head_clamps(fac, np.zeros(1, np.uint8))
+    with pytest.raises(fl.FlbpError):
+        plan.clamp_vars(used[:1], np.zeros(1, np.uint8))
 
 
 def wide_graph():

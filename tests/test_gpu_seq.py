@@ -185,15 +185,11 @@ def test_in_place_head_clamps_under_a_schedule():
     g = fl.Graph(gen.factors)
     idx = g.index_of(gen.alarms)
     ok = idx[g.body_uses(idx) == 0][:20]
-    hf = g.head_factors(ok)
-    ok, hf = ok[hf >= 0], hf[hf >= 0]
-    ok = ok[g.fac_type[hf] != 0]
-    hf = g.head_factors(ok)
     vals = (np.arange(len(ok)) % 2).astype(np.uint8)
     s = schedule_for(g, "SEQFIX", seed=3)
     plan = fl.Plan(g)
     plan.set_schedule(s)
-    plan.set_head_clamps(hf, vals)
+    plan.clamp_vars(ok, vals)
     plan.init()
     r = plan.run(1e-9, 300)
     c = oracle_clamp(gen.factors, g.var_label[ok], vals)
