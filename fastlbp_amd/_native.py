@@ -70,6 +70,9 @@ class GenParamsC(C.Structure):
         ("p_leaf_hi", C.c_double),
         ("back_frac", C.c_double),
         ("n_alarms", C.c_uint64),
+        ("extra_frac", C.c_double),
+        ("extra_q", C.c_double),
+        ("alarm_free", C.c_uint32),
     ]
 
 

@@ -341,12 +341,16 @@ void flbp_gen_defaults(flbp_gen_params *p) {
     p->p_leaf_hi = 0.99;
     p->back_frac = 0.10;
     p->n_alarms = 100;
+    p->extra_frac = 0.5;
+    p->extra_q = 0.5;
+    p->alarm_free = 1;
 }
 
 int flbp_generate(const flbp_gen_params *prm, flbp_generated **out) {
     if (!prm || !out || prm->n_vars < 1 || prm->n_layers < 2 || prm->k_min < 1 ||
         prm->k_max < prm->k_min || prm->b_min < 1 || prm->b_max < prm->b_min ||
-        !(prm->leaf_frac > 0 && prm->leaf_frac <= 1))
+        !(prm->leaf_frac > 0 && prm->leaf_frac <= 1) || !(prm->extra_frac >= 0 && prm->extra_frac <= 64) ||
+        !(prm->extra_q >= 0 && prm->extra_q <= 1))
         return flbp::fail(FLBP_EINVAL, "flbp_generate: bad parameters");
     if (prm->n_vars >= (1ull << 40)) return flbp::fail(FLBP_ERANGE, "flbp_generate: too many variables");
     Rng rng(prm->seed);
@@ -401,37 +405,58 @@ int flbp_generate(const flbp_gen_params *prm, flbp_generated **out) {
         g->rule.push_back(-1);
     }
     /* heads: OR over clauses, each clause an AND over earlier tuples */
+    std::vector<uint8_t> in_body(prm->alarm_free ? V : 0, 0);
     std::vector<uint64_t> body, clauses;
+    /* a rule body for a head of layer `layer`: b tuples of earlier layers,
+     * plus with probability back_frac one of a later layer (a loop); with
+     * alarm_free the last layer (the alarms' layer) feeds no rule */
+    const uint64_t later_end = prm->alarm_free ? layer_start[NL - 1] : tuples.size();
+    auto draw_body = [&](uint32_t layer) {
+        const uint64_t avail = layer_start[layer];               /* tuples of earlier layers */
+        const uint64_t later_lo = layer_start[layer + 1], later_hi = later_end;
+        uint64_t b = rng.range(prm->b_min, prm->b_max);
+        b = std::min<uint64_t>(b, avail);
+        body.clear();
+        while (body.size() < b) {
+            uint64_t cand = tuples[rng.below(avail)];
+            if (std::find(body.begin(), body.end(), cand) == body.end()) body.push_back(cand);
+        }
+        if (rng.uniform() < prm->back_frac && later_hi > later_lo) {
+            uint64_t cand = tuples[later_lo + rng.below(later_hi - later_lo)];
+            if (std::find(body.begin(), body.end(), cand) == body.end()) body.push_back(cand);
+        }
+        std::sort(body.begin(), body.end());
+        if (prm->alarm_free)
+            for (uint64_t x : body) in_body[x] = 1;
+    };
     for (const Head &h : heads) {
         clauses.clear();
         for (uint32_t c = 0; c < h.k; c++) clauses.push_back(h.first_clause + c);
         fl.push(2, h.label, clauses.data(), clauses.size(), 1.0, 0.0, 0, 1.0, 1.0);
         g->rule.push_back(-1);
-        const uint64_t avail = layer_start[h.layer];               /* tuples of earlier layers */
-        const uint64_t later_lo = layer_start[h.layer + 1], later_hi = tuples.size();
         for (uint32_t c = 0; c < h.k; c++) {
-            uint64_t b = rng.range(prm->b_min, prm->b_max);
-            b = std::min<uint64_t>(b, avail);
-            body.clear();
-            while (body.size() < b) {
-                uint64_t cand = tuples[rng.below(avail)];
-                if (std::find(body.begin(), body.end(), cand) == body.end()) body.push_back(cand);
-            }
-            if (rng.uniform() < prm->back_frac && later_hi > later_lo) {
-                uint64_t cand = tuples[later_lo + rng.below(later_hi - later_lo)];
-                if (std::find(body.begin(), body.end(), cand) == body.end()) body.push_back(cand);
-            }
-            std::sort(body.begin(), body.end());
+            draw_body(h.layer);
             double p = rng.uniform(prm->p_and_lo, prm->p_and_hi);
             int32_t r = prm->n_rules ? (int32_t)rng.below(prm->n_rules) : -1;
             fl.push(1, h.first_clause + c, body.data(), body.size(), p, 0.0, 0, 1.0, 1.0);
             g->rule.push_back(r);
         }
     }
-    /* alarms: distinct heads of the last layer */
+    /* further rule instances: AND factors on existing heads */
+    const uint64_t n_extra = heads.empty() ? 0 : (uint64_t)std::llround(prm->extra_frac * (double)V);
+    for (uint64_t e = 0; e < n_extra; e++) {
+        const Head &h = heads[rng.below(heads.size())];
+        draw_body(h.layer);
+        double p = rng.uniform(prm->p_and_lo, prm->p_and_hi);
+        int32_t r = prm->n_rules ? (int32_t)rng.below(prm->n_rules) : -1;
+        fl.push(1, h.label, body.data(), body.size(), p, prm->extra_q, 0, 1.0, 1.0);
+        g->rule.push_back(r);
+    }
+    /* alarms: distinct heads of the last layer (without body uses when
+     * alarm_free is set) */
     std::vector<uint64_t> last;
     for (const Head &h : heads)
-        if (h.layer == NL - 1) last.push_back(h.label);
+        if (h.layer == NL - 1 && !(prm->alarm_free && in_body[h.label])) last.push_back(h.label);
     const uint64_t na = std::min<uint64_t>(prm->n_alarms, last.size());
     for (uint64_t a = 0; a < na; a++) {      /* partial Fisher-Yates */
         uint64_t j = a + rng.below(last.size() - a);

@@ -120,7 +120,15 @@ void flbp_flist_free(flbp_flist *l);
  * head of layers 1..n_layers-1 is a deterministic OR over k clause
  * variables, each clause an AND factor over b body variables from earlier
  * layers, plus with probability back_frac one body variable from a later
- * layer (a loop). */
+ * layer (a loop).  On top of that one defining factor per variable,
+ * extra_frac * n_vars further AND factors (further rule instances) each
+ * take a uniformly chosen head of layers 1.. as their head, with a body
+ * drawn as a clause's and q = extra_q: when its body holds the factor
+ * favours its head by p : 1-p, otherwise (extra_q = 0.5) it is flat.  With
+ * the defaults this gives F = 1.5 V and E = 4.5 V.  alarm_free = 1 keeps the
+ * last layer out of every rule body (loops close through layers
+ * 1..n_layers-2 only) and draws the alarms from the last layer's heads, so
+ * every alarm clamp is a head clamp that keeps the graph's structure. */
 typedef struct flbp_gen_params {
     uint64_t n_vars;
     uint64_t seed;
@@ -133,6 +141,9 @@ typedef struct flbp_gen_params {
     double p_leaf_lo, p_leaf_hi;
     double back_frac;
     uint64_t n_alarms;
+    double extra_frac;           /* further AND factors per variable (default 0.5) */
+    double extra_q;              /* their q (default 0.5) */
+    uint32_t alarm_free;         /* 1: alarms only among heads without body uses (default 1) */
 } flbp_gen_params;
 
 typedef struct flbp_generated flbp_generated;

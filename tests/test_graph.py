@@ -194,6 +194,31 @@ def test_generator_is_deterministic_and_sized():
     assert t.rule.max() == 199 and np.all(t.rule[t.factors.type != 1] == -1)
 
 
+@pytest.mark.parametrize("n,seed", [(2000, 1), (100000, 4)])
+def test_generator_realizes_the_stated_sizes(n, seed):
+    """F = 1.5 V (one defining factor per variable plus V/2 further AND
+    rule instances on existing heads) and E ~ 4.5 V; the further factors
+    have q = 0.5 and a head of layers 1..; alarms are heads without body
+    uses, so every alarm clamp keeps the structure.  With extra_frac = 0 and
+    alarm_free = 0 the one-factor-per-variable graph of the same seed comes
+    back (F = V)."""
+    gen = fl.generate(n, seed)
+    g = fl.Graph(gen.factors)
+    assert g.n_vars == n and g.n_factors == n + round(0.5 * n)
+    assert 4.3 * n < g.n_edges < 4.7 * n
+    f = gen.factors
+    extra = np.arange(n, f.n_factors)
+    assert np.all(f.type[extra] == 1) and np.all(f.q[extra] == 0.5)
+    assert np.all((f.p[extra] >= 0.90) & (f.p[extra] <= 0.99))
+    heads_of_or = set(int(h) for h in f.head[:n][f.type[:n] == 2])
+    assert all(int(h) in heads_of_or for h in f.head[extra])
+    a = g.index_of(gen.alarms)
+    assert len(gen.alarms) == 100 and np.all(g.body_uses(a) == 0)
+    assert np.all(g.n_head_factors(a) >= 1)
+    base = fl.generate(n, seed, extra_frac=0.0, alarm_free=0)
+    assert base.factors.n_factors == n
+
+
 def test_rank_alarms_ties_by_label():
     labels = np.array([40, 10, 30, 20], np.uint64)
     marg = np.array([0.5, 0.9, 0.5, float("nan")])
