@@ -52,14 +52,18 @@ def kernel_bytes(n_vars, n_factors, n_edges, damping):
     return fac, var
 
 
-def load_traffic():
+def load_traffic(n_edges, variant):
     """Measured HBM bytes per launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if
+    it was taken on this graph (same edge count) with the same kernels."""
     path = os.path.join(ROOT, "profiles", "traffic_plain_fp64_1M.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)
+        t = json.load(f)
+    if t.get("n_edges") != n_edges or t.get("variant") != variant:
+        return None
+    return t
 
 
 def cpu_baseline(flist, damping, budget_s=15.0):
@@ -600,7 +604,7 @@ def main():
     ap.add_argument("--c5-checkpoint", type=int, default=5, help="c5: checkpoint interval of the adjoint")
     ap.add_argument("--variant", type=int, default=-1,
                     help="sweep kernels: -1 the plan's own choice by footprint (default), 0 LDS-staged (pull), "
-                         "1 one lane per item, 2 LDS-staged (push), 3 LDS-staged (scatter)")
+                         "1 one lane per item, 2 LDS-staged (push), 3 LDS-staged (scatter), 4 LDS-staged (state)")
     ap.add_argument("--n-vars", type=int, default=0,
                     help="plain workload at another graph size (a sizing study, not the headline workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -689,7 +693,7 @@ def main():
     else:
         dom, dbytes, dus = "factor_pass", fb, fac_us
     achieved = dbytes / (dus * 1e-6) / 1e9
-    traffic = load_traffic()
+    traffic = load_traffic(E, plan.variant)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get(dom) if traffic else None,
@@ -710,7 +714,8 @@ def main():
            "config": {"workload": w["name"], "n_vars": V, "n_factors": F, "n_edges": E,
                       "damping": w["damping"], "tol": w["tol"], "maxiter": w["maxiter"],
                       "seed": w["seed"], "sweeps_per_solve": iters_per_solve,
-                      "kernels": ["lds-staged-pull", "lane-per-item", "lds-staged-push", "lds-staged-scatter"][plan.variant],
+                      "kernels": ["lds-staged-pull", "lane-per-item", "lds-staged-push", "lds-staged-scatter",
+                                  "lds-staged-state"][plan.variant],
                       "step": "init + run() to convergence of the resident graph"},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

@@ -479,7 +479,7 @@ int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device
             HIP_TRY(hipMemcpy(B->fac_mask + bg.fac_base, mk.data(), c.n_factors * 16, hipMemcpyHostToDevice));
         }
         std::vector<uint32_t> perm, dev_off, dev_edge, vpos;
-        flbp::var_device_order(c, false, perm, dev_off, dev_edge, vpos);
+        flbp::var_device_order(c, flbp::kOrderLabel, perm, dev_off, dev_edge, vpos);
         B->h_perm.insert(B->h_perm.end(), perm.begin(), perm.end());
         std::copy(dev_off.begin(), dev_off.end(), B->h_var_off.begin() + bg.var_off_base);
         HIP_TRY(hipMemcpy(B->var_off + bg.var_off_base, dev_off.data(), (c.n_vars + 1) * 4, hipMemcpyHostToDevice));

@@ -38,6 +38,7 @@
 #include "plan_internal.h"
 #include "staged_kernels.h"
 #include "sweep_kernels.h"
+#include "state_kernels.h"
 
 using namespace flbp::dev;
 
@@ -103,6 +104,23 @@ int hip_fail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(e_, #expr); \
     } while (0)
 
+/* variant 4: m_vf <- the current v->f messages in var-major order (the
+ * records and f->v messages they derive from, or the init value) */
+int Plan::materialize_vf() {
+    if (!E) return FLBP_OK;
+    if (vf_src == kVfState)
+        hipLaunchKernelGGL(vf_from_state, dim3(grid_for(E, 256)), dim3(256), 0, stream, vprod, vzc, fac_var, fac_vpos,
+                           m_fv, m_vf, E);
+    else if (vf_src == kVfConst)
+        hipLaunchKernelGGL(fill_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_vf, E, init_value);
+    vf_src = kVfArray;
+    HIP_TRY(hipGetLastError());
+    /* callers copy m_vf or overwrite m_fv with blocking copies next, which
+     * do not order against the plan's non-blocking stream */
+    HIP_TRY(hipStreamSynchronize(stream));
+    return FLBP_OK;
+}
+
 int Plan::fm_current() {
     if (fm_stale && E)
         hipLaunchKernelGGL(gather_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_fv_vm, fac_vpos, m_fv, E);
@@ -145,6 +163,45 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
     const double keep = 1.0 - damping;
     const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
     static_assert(kStageItems == (uint32_t)kBlock && kStageSlots == (uint32_t)kStage, "staged block shape");
+    if (variant == 4) {
+        /* the state layout works in place on the plan's own arrays: the
+         * factor-major f->v messages and the per-variable products */
+        if (vf_in != m_vf || fv_old != m_fv || fv_out != m_fv || vf_out != m_vf)
+            return fail(FLBP_EINVAL, "plan: the state layout sweeps the plan's own message arrays only");
+        if (F) {
+            const dim3 grid((unsigned)n_fac_blocks), block(kBlock);
+            const uint32_t *idx = vf_src == kVfArray ? fac_vpos : fac_var;
+#define FLBP_FACTOR_STATE(DAMP, N, MODE)                                                                        \
+    hipLaunchKernelGGL((factor_pass_state<DAMP, N, MODE>), grid, block, 0, stream, fac_blk, fac_off, fac_meta,    \
+                       fac_p, fac_q, fac_mask, idx, fac_work, vprod, vzc, m_vf, m_fv, m_vf, init_value, damping, \
+                       keep, ctl, k, tol)
+#define FLBP_FACTOR_STATE_M(DAMP, N)                                       \
+    do {                                                                   \
+        if (vf_src == kVfState) FLBP_FACTOR_STATE(DAMP, N, kVfState);      \
+        else if (vf_src == kVfConst) FLBP_FACTOR_STATE(DAMP, N, kVfConst); \
+        else FLBP_FACTOR_STATE(DAMP, N, kVfArray);                         \
+    } while (0)
+            if (damping != 0.0) {
+                if (narrow) FLBP_FACTOR_STATE_M(true, kFacRegDegSmall);
+                else FLBP_FACTOR_STATE_M(true, kFacRegDeg);
+            } else {
+                if (narrow) FLBP_FACTOR_STATE_M(false, kFacRegDegSmall);
+                else FLBP_FACTOR_STATE_M(false, kFacRegDeg);
+            }
+#undef FLBP_FACTOR_STATE_M
+#undef FLBP_FACTOR_STATE
+        }
+        if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
+        if (V)
+            hipLaunchKernelGGL(var_pass_state, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream, var_blk, var_off,
+                               var_edge, var_work, m_fv, vprod, vzc, bel, ctl, k, tol);
+        if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 2], stream));
+        HIP_TRY(hipGetLastError());
+        /* from the second sweep on, the v->f inputs come from the products
+         * the var pass just wrote (a sweep the stop test skips changes nothing) */
+        vf_src = kVfState;
+        return FLBP_OK;
+    }
     if (F) {
         if (variant != 1) {
             const dim3 grid((unsigned)n_fac_blocks), block(kBlock);
@@ -231,11 +288,13 @@ void Plan::drop_graphs() {
  * factor-major copy is rebuilt before the replay, outside the graph) */
 int Plan::chunk_graph(int n, double tol, double damping, hipGraphExec_t &exec) {
     for (auto &g : graphs)
-        if (g.n == n && g.variant == variant && g.damping == damping && g.tol == tol) {
+        if (g.n == n && g.variant == variant && g.vf_src == vf_src && g.damping == damping && g.tol == tol) {
             exec = g.exec;
+            if (variant == 4) vf_src = kVfState;   /* as the captured launches leave it */
             return FLBP_OK;
         }
     if (graphs.size() >= 2 * (size_t)kMaxChunk) drop_graphs();
+    const int vf_src0 = vf_src;
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(clear_u64, dim3(16), dim3(256), 0, stream, ctl, kMaxChunk * kCtlPerSweep);
     int rc = FLBP_OK;
@@ -255,7 +314,7 @@ int Plan::chunk_graph(int n, double tol, double damping, hipGraphExec_t &exec) {
         (void)hipGraphDestroy(g);
         return hip_fail(ei, "hipGraphInstantiate");
     }
-    graphs.push_back({n, variant, damping, tol, g, x});
+    graphs.push_back({n, variant, vf_src0, damping, tol, g, x});
     exec = x;
     return FLBP_OK;
 }
@@ -292,7 +351,7 @@ int Plan::run_chunk(int n, double tol, double damping, int &ran, double &last_di
         std::memcpy(&dk, &host_ctl[k], sizeof dk);
         prev_diff = last_diff;
         last_diff = dk;
-        if (!(dk > tol)) {
+        if (!(dk > tol) && tol != -INFINITY) {   /* -inf: fixed sweeps, none stops */
             ran = k + 1;
             break;
         }
@@ -377,6 +436,7 @@ int flbp_plan_init(flbp_plan *plan, int logdomain) {
     P->fm_stale = false;
     P->vm_stale = false;
     P->init_value = c;
+    P->vf_src = kVfConst;
     P->swept_since_init = false;
     return FLBP_OK;
 }
@@ -527,10 +587,12 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
             fv_varmajor[2 * k + 1] = tmp[2 * s + 1];
         }
     }
-    if (vf_facmajor && P->E) {   /* stored var-major, or factor-major in the push layout */
+    if (vf_facmajor && P->E) {   /* stored var-major, or factor-major in the push and scatter layouts */
+        if (P->variant == 4)      /* derived from the products: materialized var-major first */
+            if (int rc = P->materialize_vf()) return rc;
         HIP_TRY(hipMemcpy(tmp.data(), P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
         for (uint64_t s = 0; s < P->E; s++) {
-            const uint64_t k = P->variant >= 2 ? s : P->h_fac_vpos[s];
+            const uint64_t k = (P->variant == 2 || P->variant == 3) ? s : P->h_fac_vpos[s];
             vf_facmajor[2 * s] = tmp[2 * k];
             vf_facmajor[2 * s + 1] = tmp[2 * k + 1];
         }
@@ -545,6 +607,10 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
         return flbp::fail(FLBP_EINVAL, "flbp_plan_set_messages: a sequential schedule keeps its products in step "
                                        "with the f->v messages; remove the schedule before setting them");
     HIP_TRY(hipSetDevice(P->device));
+    /* the state layout derives v->f from the f->v messages: keep the current
+     * v->f messages explicitly before the f->v messages change under them */
+    if (P->variant == 4 && fv_varmajor && !vf_facmajor)
+        if (int rc = P->materialize_vf()) return rc;
     if (fv_varmajor && P->E) {
         std::vector<double> tmp(2 * P->E);
         for (uint64_t k = 0; k < P->E; k++) {
@@ -565,11 +631,12 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
     if (vf_facmajor && P->E) {
         std::vector<double> tmp(2 * P->E);
         for (uint64_t s = 0; s < P->E; s++) {
-            const uint64_t k = P->variant >= 2 ? s : P->h_fac_vpos[s];
+            const uint64_t k = (P->variant == 2 || P->variant == 3) ? s : P->h_fac_vpos[s];
             tmp[2 * k] = vf_facmajor[2 * s];
             tmp[2 * k + 1] = vf_facmajor[2 * s + 1];
         }
         HIP_TRY(hipMemcpy(P->m_vf, tmp.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
+        if (P->variant == 4) P->vf_src = kVfArray;
     }
     return FLBP_OK;
 }
@@ -697,7 +764,7 @@ int flbp_plan_get_variant(const flbp_plan *plan) {
 
 int flbp_plan_set_variant(flbp_plan *plan, int variant) {
     Plan *P = reinterpret_cast<Plan *>(plan);
-    if (!P || variant < 0 || variant > 3) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");
+    if (!P || variant < 0 || variant > 4) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: bad argument");
     if (P->seq && variant != 3)
         return flbp::fail(FLBP_EINVAL, "flbp_plan_set_variant: a plan with a sequential schedule keeps variant 3");
     auto layout = [](int v) { return v == 1 ? 0 : v; };

@@ -156,7 +156,8 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     /* the schedule's sweeps read and write the factor-major f->v array */
     if (int rc = P->fm_current()) return rc;
     /* the schedule's execution items, grouped by kind and level (counting
-     * sort), then ordered by slot / factor inside each level */
+     * sort), then ordered by slot / factor inside each level; an item names
+     * its variable by device index, the index of the products and beliefs */
     const uint32_t nl = S.n_levels;
     const size_t nx = S.ex_kind.size();
     std::vector<uint32_t> off[3];
@@ -188,14 +189,14 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
     std::vector<uint4> vf(keyed[0].size()), fv(keyed[1].size()), faux(keyed[1].size());
     for (size_t y = 0; y < keyed[0].size(); y++) {
         const uint32_t sl = S.ex_a[keyed[0][y].second];
-        vf[y] = make_uint4(sl, S.slot_var[sl], 0u, 0u);
+        vf[y] = make_uint4(sl, P->h_inv_perm[S.slot_var[sl]], 0u, 0u);
     }
     for (size_t y = 0; y < keyed[1].size(); y++) {
         const uint32_t e = S.ex_a[keyed[1][y].second];
         const uint32_t own = (uint32_t)P->h_var_edge[e], I = slot_fac[own];
         const uint32_t a = (uint32_t)P->h_fac_off[I], d = (uint32_t)(P->h_fac_off[I + 1] - a);
         const uint32_t meta = P->h_meta[I];   /* type and head slot never change */
-        fv[y] = make_uint4(e, S.slot_var[own], I, a);
+        fv[y] = make_uint4(e, P->h_inv_perm[S.slot_var[own]], I, a);
         faux[y] = make_uint4(d | ((meta & 3u) << 24), own - a, meta >> 8, 0u);
     }
     /* group items: one lane (one record) per slot, a group's lanes inside
@@ -217,7 +218,7 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
             if ((grp.size() - base) % 64 + d > 64) pad_to_wave();
             for (uint32_t k = 0; k < d; k++)
                 grp.push_back(make_uint4(I, a, flbp::dev::group_word(d, meta & 3u, meta >> 8, S.ex_b[x], S.ex_c[x], k),
-                                         S.slot_var[a + k]));
+                                         P->h_inv_perm[S.slot_var[a + k]]));
         }
         pad_to_wave();
         if (grp.size() >= 0xffffffffull)

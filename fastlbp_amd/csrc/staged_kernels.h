@@ -213,6 +213,68 @@ struct VarSmem {
     unsigned long long red[kBlock / 64];
 };
 
+/* Phase 2 of a factor tile: one lane per output slot, in the tile's work
+ * order (w = local slot | local factor << 10), from LDS only: sm.in holds
+ * the tile's v->f inputs, sm.out receives the f->v outputs (and holds the
+ * old f->v messages when damped). */
+template <bool kDamp, int N>
+__device__ __forceinline__ void factor_tile_compute(FactorSmem &sm, const uint32_t (&w)[kPerLane], uint32_t n,
+                                                    uint32_t f0, const double2 *__restrict__ fac_mask, double damp,
+                                                    double keep) {
+    const int tid = threadIdx.x;
+#pragma unroll 1
+    for (int j = 0; j < kPerLane; j++) {
+        if (tid + j * kBlock >= n) break;
+        uint32_t wj = 0;
+#pragma unroll
+        for (int x = 0; x < kPerLane; x++) wj |= (j == x) ? w[x] : 0u;
+        const uint32_t sl = wj & 1023u, lf = wj >> 10;
+        const uint32_t s = sm.start[lf];
+        const int d = (int)(sm.start[lf + 1] - s);
+        const int t = (int)(sl - s);
+        const uint32_t meta = sm.meta[lf];
+        const uint32_t type = meta & 3u;
+        const bool clamped = (meta & 4u) != 0;
+        const int h = (int)(meta >> 8);
+        const double p = sm.p[lf];
+        const double q = type ? sm.q[lf] : 0.0;
+        double mask0 = 1.0, mask1 = 1.0;
+        if (clamped) {
+            const double2 mk = fac_mask[f0 + lf];
+            mask0 = mk.x;
+            mask1 = mk.y;
+        }
+        double2 r;
+        if (d <= N) {
+            /* the other slots' messages, in slot order, from LDS */
+            const bool head_target = (t == h);
+            const int skip_h = head_target ? -1 : h;
+            const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
+            double2 ops[N - 1];
+            int js = 0;
+#pragma unroll
+            for (int k = 0; k < N - 1; k++) {
+                js += (js == t || js == skip_h);
+                js += (js == t || js == skip_h);
+                const double2 v = sm.in[s + (k < cnt ? js : 0)];   /* index kept inside the factor */
+                ops[k] = k < cnt ? v : make_double2(0.0, 0.0);
+                js++;
+            }
+            r = fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, sm.in[s + h], p, q, mask0, mask1);
+        } else {
+            LdsIn in{sm.in + s};
+            const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
+            r = fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
+        }
+        if (kDamp) {
+            const double2 old = sm.out[sl];
+            r.x = keep * r.x + damp * old.x;
+            r.y = keep * r.y + damp * old.y;
+        }
+        sm.out[sl] = r;
+    }
+}
+
 /* The factor pass over factors [f0, f1) of one graph (every pointer is the
  * graph's own: offsets and slots are graph-local).  The whole block calls
  * it; it ends with the block's stores issued and LDS free for reuse after a
@@ -272,59 +334,7 @@ __device__ __forceinline__ void factor_tile(
         }
     }
     __syncthreads();
-    /* phase 2: one lane per output slot, in the tile's work order
-     * (w = local slot | local factor << 10), from LDS only */
-#pragma unroll 1
-    for (int j = 0; j < kPerLane; j++) {
-        if (tid + j * kBlock >= n) break;
-        uint32_t wj = 0;
-#pragma unroll
-        for (int x = 0; x < kPerLane; x++) wj |= (j == x) ? w[x] : 0u;
-        const uint32_t sl = wj & 1023u, lf = wj >> 10;
-        const uint32_t s = sm.start[lf];
-        const int d = (int)(sm.start[lf + 1] - s);
-        const int t = (int)(sl - s);
-        const uint32_t meta = sm.meta[lf];
-        const uint32_t type = meta & 3u;
-        const bool clamped = (meta & 4u) != 0;
-        const int h = (int)(meta >> 8);
-        const double p = sm.p[lf];
-        const double q = type ? sm.q[lf] : 0.0;
-        double mask0 = 1.0, mask1 = 1.0;
-        if (clamped) {
-            const double2 mk = fac_mask[f0 + lf];
-            mask0 = mk.x;
-            mask1 = mk.y;
-        }
-        double2 r;
-        if (d <= N) {
-            /* the other slots' messages, in slot order, from LDS */
-            const bool head_target = (t == h);
-            const int skip_h = head_target ? -1 : h;
-            const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
-            double2 ops[N - 1];
-            int js = 0;
-#pragma unroll
-            for (int k = 0; k < N - 1; k++) {
-                js += (js == t || js == skip_h);
-                js += (js == t || js == skip_h);
-                const double2 v = sm.in[s + (k < cnt ? js : 0)];   /* index kept inside the factor */
-                ops[k] = k < cnt ? v : make_double2(0.0, 0.0);
-                js++;
-            }
-            r = fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, sm.in[s + h], p, q, mask0, mask1);
-        } else {
-            LdsIn in{sm.in + s};
-            const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
-            r = fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
-        }
-        if (kDamp) {
-            const double2 old = sm.out[sl];
-            r.x = keep * r.x + damp * old.x;
-            r.y = keep * r.y + damp * old.y;
-        }
-        sm.out[sl] = r;
-    }
+    factor_tile_compute<kDamp, N>(sm, w, n, f0, fac_mask, damp, keep);
     __syncthreads();
     /* phase 3: stores (the scatter positions came with phase 1) */
 #pragma unroll

@@ -1,0 +1,299 @@
+/*
+ * state_kernels.h -- the sweep kernels of the state layout (kernel
+ * variant 4).
+ *
+ * The reference stores both message directions per edge and recomputes all
+ * of them every iteration: calcMessageFVFusedAll writes f->v, calcProdAll
+ * forms each variable's zero-counted product of its f->v messages, and
+ * calcMessageVFAll divides one f->v message back out of that product to get
+ * each v->f message (csrc/kernel/causal_bp_parall.cu:20-92, 94-398).  The
+ * v->f message of edge (I, v) is therefore a function of two values only:
+ * the variable's product record and the f->v message of the same edge.
+ *
+ * This layout keeps exactly those two: f->v messages factor-major (the
+ * factor pass reads and rewrites them in place, contiguously) and, per
+ * variable, its product pair (16 bytes) and a one-byte code of its zero
+ * counts (pack_zc).  The variable pass forms each variable's product,
+ * belief and |b - b_old| and stores the pair and code -- it writes no
+ * per-edge data.  The factor pass derives every v->f input of its tile
+ * from the pair and code of the slot's variable and the slot's own
+ * previous f->v message (vf_msg, the reference's division-based
+ * leave-one-out), then computes and stores the new f->v messages.  The
+ * factor pass's random reads go to V-sized arrays (the codes of a million
+ * variables fit one XCD's L2), which caches hold far better than a
+ * per-edge array.  The one per-edge random access of a sweep is the
+ * variable pass's gather of its f->v messages from their factor slots.
+ * (Scattering a var-major copy from the factor pass instead, so that the
+ * variable pass reads in order, measured 2.2x slower per factor pass on a
+ * 4.5M-edge graph in HBM: 177 against 80 us.)
+ *
+ * The arithmetic is the reference's, so results are bit-identical to the
+ * other layouts: each v->f value is vf_msg of the same product and message
+ * the variable pass of the previous sweep would have used.  The first sweep
+ * after init() takes the init value (c, c) as its v->f input, as the
+ * reference's first calcMessageFVFusedAll does; after a caller sets v->f
+ * messages explicitly, the next factor pass gathers them from a var-major
+ * array, as the pull layout does.
+ */
+#ifndef FLBP_STATE_KERNELS_H
+#define FLBP_STATE_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "plan_internal.h"
+#include "staged_kernels.h"
+
+namespace {
+
+using namespace flbp::dev;
+
+struct ConstIn {
+    double c;
+    __device__ __forceinline__ double2 get(int) const { return make_double2(c, c); }
+    __device__ __forceinline__ double2 pick(int) const { return make_double2(c, c); }
+};
+
+__device__ __forceinline__ double2 vf_of(double2 prod, uint32_t zc, double2 m) {
+    return vf_msg(prod.x, prod.y, zc & 3u, zc >> 2, m);
+}
+
+/* The factor pass over factors [f0, f1) in the state layout.  idx is the
+ * slot -> device variable map (kVfState) or the slot -> var-major position
+ * map (kVfArray).  m_fv is read and rewritten in place: each slot belongs to
+ * one factor, and a block reads all its slots before it writes any. */
+template <bool kDamp, int N, int kMode>
+__device__ __forceinline__ void factor_tile_state(
+    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
+    const double *__restrict__ fac_p, const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
+    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ fac_work, const double2 *__restrict__ vprod,
+    const uint8_t *__restrict__ vzc, const double2 *m_vf, double2 *m_fv, double2 *scratch, double c, double damp,
+    double keep, uint32_t f0, uint32_t f1, FactorSmem &sm) {
+    const uint32_t base = fac_off[f0];
+    const uint32_t n = fac_off[f1] - base;
+    const int tid = threadIdx.x;
+    if (n > (uint32_t)kStage) {
+        /* one wide factor, from global memory.  In the state mode its v->f
+         * inputs go to scratch first (the factor's own slot range of it):
+         * its outputs overwrite the f->v messages they are derived from. */
+        const uint32_t meta = fac_meta[f0];
+        const double p = fac_p[f0], q = (meta & 3u) ? fac_q[f0] : 0.0;
+        double mask0 = 1.0, mask1 = 1.0;
+        if (meta & 4u) {
+            const double2 mk = fac_mask[f0];
+            mask0 = mk.x;
+            mask1 = mk.y;
+        }
+        if constexpr (kMode == kVfArray) {
+            wide_factor<kDamp, kLayoutPull>(GlobalIn{m_vf, idx + base}, base, n, meta, p, q, mask0, mask1, nullptr,
+                                            m_fv, m_fv, nullptr, damp, keep);
+        } else if constexpr (kMode == kVfConst) {
+            wide_factor<kDamp, kLayoutPull>(ConstIn{c}, base, n, meta, p, q, mask0, mask1, nullptr, m_fv, m_fv,
+                                            nullptr, damp, keep);
+        } else {
+            for (uint32_t t = tid; t < n; t += kBlock) {
+                const uint32_t v = idx[base + t];
+                scratch[base + t] = vf_of(vprod[v], vzc[v], m_fv[base + t]);
+            }
+            __syncthreads();
+            wide_factor<kDamp, kLayoutPull>(LdsIn{scratch + base}, base, n, meta, p, q, mask0, mask1, nullptr, m_fv,
+                                            m_fv, nullptr, damp, keep);
+        }
+        return;
+    }
+    /* phase 1: work entries and indices into registers, the factors'
+     * offsets and parameters into LDS, and the v->f inputs (derived from the
+     * products and the old f->v messages) into LDS */
+    const uint32_t nf = f1 - f0;
+    uint32_t w[kPerLane], ix[kPerLane];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        w[j] = i < n ? fac_work[base + i] : 0u;
+        ix[j] = (kMode != kVfConst && i < n) ? idx[base + i] : 0u;
+    }
+    if ((uint32_t)tid < nf) {
+        sm.start[tid] = fac_off[f0 + tid] - base;
+        sm.meta[tid] = fac_meta[f0 + tid];
+        sm.p[tid] = fac_p[f0 + tid];
+        sm.q[tid] = fac_q[f0 + tid];
+    }
+    if (tid == 0) sm.start[nf] = n;
+    double2 old[kPerLane], pr[kPerLane];
+    uint32_t zc[kPerLane];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < n) {
+            if (kDamp || kMode == kVfState) old[j] = m_fv[base + i];
+            if (kMode == kVfState) {
+                pr[j] = vprod[ix[j]];
+                zc[j] = vzc[ix[j]];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < n) {
+            double2 in;
+            if constexpr (kMode == kVfState) in = vf_of(pr[j], zc[j], old[j]);
+            else if constexpr (kMode == kVfConst) in = make_double2(c, c);
+            else in = m_vf[ix[j]];
+            sm.in[i] = in;
+            if (kDamp) sm.out[i] = old[j];
+        }
+    }
+    __syncthreads();
+    factor_tile_compute<kDamp, N>(sm, w, n, f0, fac_mask, damp, keep);
+    __syncthreads();
+    /* phase 3: the new f->v messages, in place and coalesced */
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < n) m_fv[base + i] = sm.out[i];
+    }
+}
+
+/* The variable pass over variables [v0, v1) in the state layout: the f->v
+ * messages gathered from their factor slots, then each variable's product
+ * pair, zero-count code and belief stored.  Returns the lane's max
+ * |b - b_old| as IEEE bits. */
+__device__ __forceinline__ unsigned long long var_tile_state(
+    const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
+    const uint16_t *__restrict__ var_work, const double2 *__restrict__ m_fv, double2 *__restrict__ vprod,
+    uint8_t *__restrict__ vzc, double *__restrict__ belief, uint32_t v0, uint32_t v1, VarSmem &sm) {
+    const uint32_t base = var_off[v0];
+    const uint32_t n = var_off[v1] - base;
+    const int tid = threadIdx.x;
+    unsigned long long diff_bits = 0;
+    if (n > (uint32_t)kStage) {
+        /* one wide variable: the product in edge order on lane 0 over
+         * kStage-slot chunks gathered by the whole block */
+        double prod_0 = 1.0, prod_1 = 1.0;
+        uint32_t z0 = 0, z1 = 0;
+        for (uint32_t c = 0; c < n; c += kStage) {
+            const uint32_t m = n - c < (uint32_t)kStage ? n - c : (uint32_t)kStage;
+            for (uint32_t e = tid; e < m; e += kBlock) sm.m[e] = m_fv[var_edge[base + c + e]];
+            __syncthreads();
+            if (tid == 0) {
+                for (uint32_t e = 0; e < m; e++) {
+                    const double2 x = sm.m[e];
+                    if (x.x == 0) z0++; else prod_0 *= x.x;
+                    if (x.y == 0) z1++; else prod_1 *= x.y;
+                    scale_pair(prod_0, prod_1);
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const double b0 = prod_0 * (double)(z0 == 0);
+            const double b1 = prod_1 * (double)(z1 == 0);
+            const double b = b1 / (b0 + b1);
+            const double diff = fabs(belief[v0] - b);
+            belief[v0] = b;
+            vprod[v0] = make_double2(prod_0, prod_1);
+            vzc[v0] = pack_zc(z0, z1);
+            diff_bits = (unsigned long long)__double_as_longlong(diff);
+        }
+        return diff_bits;
+    }
+    /* phase 1: offsets, old beliefs, work entries and the tile's messages,
+     * all loads independent of each other */
+    const uint32_t nv = v1 - v0;
+    uint32_t lv = 0;
+    uint32_t ve[kPerLane];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        ve[j] = i < n ? var_edge[base + i] : 0u;
+    }
+    if ((uint32_t)tid < nv) {
+        lv = var_work[v0 + tid];
+        sm.start[tid] = var_off[v0 + tid] - base;
+        sm.bel[tid] = belief[v0 + tid];
+    }
+    if (tid == 0) sm.start[nv] = n;
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < n) sm.m[i] = m_fv[ve[j]];
+    }
+    __syncthreads();
+    /* phase 2: products, one lane per variable in the tile's work order */
+    if ((uint32_t)tid < nv) {
+        const uint32_t s = sm.start[lv];
+        const int d = (int)(sm.start[lv + 1] - s);
+        double prod_0 = 1.0, prod_1 = 1.0;
+        uint32_t z0 = 0, z1 = 0;
+        for (int j = 0; j < d; j++) {
+            const double2 x = sm.m[s + j];
+            if (x.x == 0) z0++; else prod_0 *= x.x;
+            if (x.y == 0) z1++; else prod_1 *= x.y;
+            scale_pair(prod_0, prod_1);
+        }
+        const double b0 = prod_0 * (double)(z0 == 0);
+        const double b1 = prod_1 * (double)(z1 == 0);
+        const double b = b1 / (b0 + b1);
+        const double diff = fabs(sm.bel[lv] - b);
+        sm.bel[lv] = b;
+        diff_bits = (unsigned long long)__double_as_longlong(diff);
+        sm.prod[0][lv] = prod_0;
+        sm.prod[1][lv] = prod_1;
+        sm.z[lv] = pack_zc(z0, z1);
+    }
+    __syncthreads();
+    /* phase 3: products, codes and beliefs, coalesced */
+    if ((uint32_t)tid < nv) {
+        vprod[v0 + tid] = make_double2(sm.prod[0][tid], sm.prod[1][tid]);
+        vzc[v0 + tid] = (uint8_t)sm.z[tid];
+        belief[v0 + tid] = sm.bel[tid];
+    }
+    return diff_bits;
+}
+
+template <bool kDamp, int N, int kMode>
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_state(
+    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ fac_off,
+    const uint32_t *__restrict__ fac_meta, const double *__restrict__ fac_p,
+    const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
+    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ fac_work, const double2 *__restrict__ vprod,
+    const uint8_t *__restrict__ vzc, const double2 *m_vf, double2 *m_fv, double2 *scratch, double c, double damp,
+    double keep, const unsigned long long *__restrict__ ctl, int k, double tol) {
+    if (chunk_stopped(ctl, k, tol)) return;
+    __shared__ FactorSmem sm;
+    const uint32_t b = blockIdx.x;
+    factor_tile_state<kDamp, N, kMode>(fac_off, fac_meta, fac_p, fac_q, fac_mask, idx, fac_work, vprod, vzc, m_vf,
+                                       m_fv, scratch, c, damp, keep, blk[b], blk[b + 1], sm);
+}
+
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_state(
+    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ var_off,
+    const uint32_t *__restrict__ var_edge, const uint16_t *__restrict__ var_work,
+    const double2 *__restrict__ m_fv, double2 *__restrict__ vprod, uint8_t *__restrict__ vzc,
+    double *__restrict__ belief, unsigned long long *__restrict__ ctl, int k, double tol) {
+    if (chunk_stopped(ctl, k, tol)) return;
+    __shared__ VarSmem sm;
+    /* gathers from factor-major lines, as the pull layout's var pass */
+    const uint32_t b = xcd_runs<kPullVarRun>(blockIdx.x, gridDim.x);
+    const unsigned long long bits =
+        var_tile_state(var_off, var_edge, var_work, m_fv, vprod, vzc, belief, blk[b], blk[b + 1], sm);
+    block_fold_diff(ctl, k, bits, sm.red);
+}
+
+/* the v->f messages of the state layout, materialized var-major (the pull
+ * layout's order): vf[vpos[s]] = vf_msg(products of slot s's variable,
+ * f->v[s]) */
+__global__ void vf_from_state(const double2 *__restrict__ vprod, const uint8_t *__restrict__ vzc,
+                              const uint32_t *__restrict__ fac_var, const uint32_t *__restrict__ fac_vpos,
+                              const double2 *__restrict__ m_fv, double2 *__restrict__ vf, uint64_t n) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) {
+        const uint32_t v = fac_var[s];
+        vf[fac_vpos[s]] = vf_of(vprod[v], vzc[v], m_fv[s]);
+    }
+}
+
+}  // namespace
+
+#endif

@@ -290,9 +290,12 @@ __device__ __forceinline__ unsigned long long ctl_read(const unsigned long long 
 }
 
 /* stop test: sweep k of a chunk runs only if sweep k-1 left max diff > tol
- * (a sweep that never ran leaves its slots 0 and stops every later sweep) */
+ * (a sweep that never ran leaves its slots 0 and stops every later sweep).
+ * tol = -inf means a fixed number of sweeps (the 3-argument run): no sweep
+ * stops, whatever max diff the one before left -- NaN included, which no
+ * comparison with tol could let through. */
 __device__ __forceinline__ bool chunk_stopped(const unsigned long long *ctl, int k, double tol) {
-    if (k <= 0) return false;
+    if (k <= 0 || tol == -__builtin_inf()) return false;
     const double prev = __longlong_as_double((long long)ctl_read(ctl, k - 1));
     return !(prev > tol);
 }

@@ -453,20 +453,20 @@ class Plan:
 
     @property
     def variant(self) -> int:
-        """The kernel variant in use (a new plan picks 3 or 0 by footprint)."""
+        """The kernel variant in use (a new plan starts with 4)."""
         v = int(lib().flbp_plan_get_variant(self._h))
         if v < 0:
             check("flbp_plan_get_variant", v)
         return v
 
     def set_variant(self, variant: int) -> None:
-        """Kernel shape and message layout: 0 LDS-staged, v->f var-major and
-        f->v factor-major (the default above the Infinity Cache size); 1 one
-        lane per factor/variable, same layout; 2 LDS-staged, both
+        """Kernel shape and message layout: 4 the state layout (the default:
+        f->v factor-major plus per-variable products, v->f derived in the
+        factor pass); 0 LDS-staged, v->f var-major and f->v factor-major; 1
+        one lane per factor/variable, same layout; 2 LDS-staged, both
         factor-major (push); 3 LDS-staged, v->f factor-major and f->v
-        var-major (both passes scatter; the default for plans that fit the
-        cache).  Results are identical; switching layouts moves the stored
-        messages."""
+        var-major (both passes scatter).  Results are identical; switching
+        layouts moves the stored messages."""
         call("flbp_plan_set_variant", self._h, variant)
 
     def set_schedule(self, schedule: Optional[Schedule]) -> None:

@@ -277,18 +277,20 @@ int flbp_plan_profile(flbp_plan *plan, int enable);
 int flbp_plan_last_timing(flbp_plan *plan, double *fac_ms, double *var_ms,
                           uint64_t *fac_launches, uint64_t *var_launches);
 
-/* Kernel shape and message layout of the sweep.  3: LDS-staged blocks
- * over balanced edge ranges, v->f messages factor-major and f->v var-major,
- * so both passes read contiguously and scatter their outputs (plus a
- * factor-major f->v copy written by damped sweeps); 0: LDS-staged, v->f
- * var-major and f->v factor-major (both passes gather, all stores
- * contiguous); 1: one lane per factor / variable in layout 0; 2:
- * LDS-staged, both arrays factor-major.  A new plan starts with 3 when its
- * device footprint fits the Infinity Cache (at most 320 MB; random scatters
- * are cheaper than random gathers there) and with 0 otherwise (in HBM
- * random gathers are cheaper).  Results are identical; switching between
- * layouts moves the stored messages, so a solve may continue under another
- * variant.  flbp_adjoint_create switches its plan to 0. */
+/* Kernel shape and message layout of the sweep.  4 (the default above a
+ * 64 MB device footprint, 3 below it): the state layout -- f->v messages factor-major, rewritten in place by the
+ * factor pass, and per variable its product pair and zero-count code,
+ * from which the factor pass derives each v->f input (the reference's
+ * leave-one-out division); the variable pass gathers f->v messages and
+ * writes no per-edge data.  3: LDS-staged blocks over balanced edge
+ * ranges, v->f messages factor-major and f->v var-major, so both passes
+ * read contiguously and scatter their outputs (plus a factor-major f->v
+ * copy written by damped sweeps); 0: LDS-staged, v->f var-major and f->v
+ * factor-major (both passes gather, all stores contiguous); 1: one lane per
+ * factor / variable in layout 0; 2: LDS-staged, both arrays factor-major.
+ * Results are identical; switching between layouts moves the stored
+ * messages, so a solve may continue under another variant.
+ * flbp_adjoint_create switches its plan to 0. */
 int flbp_plan_set_variant(flbp_plan *plan, int variant);
 /* Replay each chunk of sweeps from a hipGraph captured on first use (one
  * per chunk length, damping, tol and variant), or launch its kernels one by

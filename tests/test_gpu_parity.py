@@ -149,7 +149,7 @@ def solve_variant(flist, tol, maxiter, damping, variant):
     return r, plan.beliefs(), plan.messages()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_both_kernel_variants_match_oracle(variant):
     gen = fl.generate(30000, 61, back_frac=0.3)
     g, plan, r, bg, o, it, md, bo = solve_both(gen.factors, 1e-9, 400, 0.3)
@@ -168,7 +168,8 @@ def test_variant_switch_mid_run_keeps_state():
     plan = fl.Plan(fl.Graph(gen.factors))
     plan.init()
     segments = ((0, 3, 0.3), (2, 4, 0.3), (3, 2, 0.3), (1, 2, 0.3), (3, 2, 0.0), (3, 2, 0.3), (2, 1, 0.0),
-                (3, 1, 0.0), (0, 2, 0.3))
+                (3, 1, 0.0), (0, 2, 0.3), (4, 3, 0.3), (3, 1, 0.3), (4, 2, 0.0), (4, 1, 0.3), (0, 1, 0.3),
+                (4, 2, 0.3), (2, 1, 0.3), (4, 1, 0.0), (0, 1, 0.0), (4, 2, 0.3), (3, 1, 0.3))
     for variant, n, damping in segments:
         plan.set_variant(variant)
         plan.run_fixed(n, damping)
@@ -180,23 +181,27 @@ def test_variant_switch_mid_run_keeps_state():
 
 
 def test_layout_default_follows_footprint():
-    """A plan that fits the Infinity Cache starts in the scatter layout, a
-    larger one in the pull layout; both give the same bits."""
+    """A new plan starts in the scatter layout (variant 3) up to a 64 MB
+    footprint, in the state layout (variant 4) above; the scatter and pull
+    layouts give the same bits."""
     small = fl.Plan(fl.Graph(fl.generate(5000, 71).factors))
     assert small.variant == 3
     gen = fl.generate(1_600_000, 72)
     g = fl.Graph(gen.factors)
     big = fl.Plan(g)
-    assert big.variant == 0 and big.device_bytes > 320 * 2**20
+    assert big.variant == 4 and big.device_bytes > 320 * 2**20
     other = fl.Plan(g)
     other.set_variant(3)
-    for p in (big, other):
+    pull = fl.Plan(g)
+    pull.set_variant(0)
+    for p in (big, other, pull):
         p.init()
         p.run_fixed(6, 0.3)
-    assert same_bits(big.beliefs(), other.beliefs())
     fa, va = big.messages()
-    fb, vb = other.messages()
-    assert same_bits(fa, fb) and same_bits(va, vb)
+    for p in (other, pull):
+        assert same_bits(big.beliefs(), p.beliefs())
+        fb, vb = p.messages()
+        assert same_bits(fa, fb) and same_bits(va, vb)
 
 
 def test_head_clamps_in_place_equal_rebuilt_graph():
@@ -257,7 +262,7 @@ def wide_graph():
     return fl.FactorList.build(facs)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_wide_factor_and_variable(variant):
     flist = wide_graph()
     g = fl.Graph(flist)
@@ -280,7 +285,7 @@ def test_graph_replays_match_direct_launches():
     o.init()
     it, md = o.run(1e-9, 300, 0.3, threads=8)
     bo = o.beliefs()
-    for variant in (3, 0):
+    for variant in (3, 0, 4):
         plan = fl.Plan(g)
         plan.set_variant(variant)
         plan.use_graphs(True)
@@ -300,3 +305,74 @@ def test_graph_replays_match_direct_launches():
         direct.run(1e-14, 7, 0.0)
         direct.run(1e-14, 19, 0.3)
         assert same_bits(plan.beliefs(), direct.beliefs())
+
+
+@pytest.mark.parametrize("logdomain", [False, True])
+def test_state_layout_messages_in_and_out(logdomain):
+    """The state layout keeps products instead of v->f messages: messages
+    read back after any sweep (also straight after init), messages set
+    explicitly (both directions, or f->v only, which keeps the current v->f
+    messages), and the solve continued afterwards all equal the oracle."""
+    gen = fl.generate(15000, 63, back_frac=0.2)
+    g = fl.Graph(gen.factors)
+    o = oracle.OracleGraph(gen.factors)
+    o.init(logdomain)
+    plan = fl.Plan(g)
+    plan.set_variant(4)
+    plan.init(logdomain)
+    fv, vf = plan.messages()
+    ofv, ovf = o.messages()
+    assert same_bits(fv, ofv) and same_bits(vf, ovf)
+    plan.run_fixed(3, 0.3)
+    o.run_fixed(3, 0.3)
+    fv, vf = plan.messages()
+    ofv, ovf = o.messages()
+    assert same_bits(fv, ofv) and same_bits(vf, ovf) and same_bits(plan.beliefs(), o.beliefs())
+    # a second plan takes the state over explicitly and both continue
+    other = fl.Plan(g)
+    other.set_variant(4)
+    other.init(logdomain)
+    other.set_messages(fv, vf)
+    other.run_fixed(4, 0.3)
+    plan.run_fixed(4, 0.3)
+    o.run_fixed(4, 0.3)
+    ofv, ovf = o.messages()
+    for p in (plan, other):
+        f2, v2 = p.messages()
+        assert same_bits(f2, ofv) and same_bits(v2, ovf)
+    assert same_bits(plan.beliefs(), o.beliefs())
+    # f->v set alone: the v->f messages stay what the last sweep left
+    rng = np.random.default_rng(3)
+    pert = ofv.copy()
+    pert[rng.integers(0, len(pert), 50)] = [0.25, 0.75]
+    plan.set_messages(pert, None)
+    o.set_messages(pert, ovf)
+    plan.run_fixed(3, 0.3)
+    o.run_fixed(3, 0.3)
+    f3, v3 = plan.messages()
+    ofv, ovf = o.messages()
+    assert same_bits(f3, ofv) and same_bits(v3, ovf) and same_bits(plan.beliefs(), o.beliefs())
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_fixed_sweeps_run_through_nan_beliefs(variant):
+    """run(tol, maxIters, hist) does exactly maxIters sweeps (causal_bp.cu:
+    935-938), whatever the beliefs: after a logdomain init every belief is
+    NaN from the first sweep on, and no later sweep may be skipped or
+    repeated (the prior messages' damped sums count the sweeps: 1 - d^k)."""
+    gen = fl.generate(15000, 63, back_frac=0.2)
+    g = fl.Graph(gen.factors)
+    o = oracle.OracleGraph(gen.factors)
+    o.init(True)
+    plan = fl.Plan(g)
+    plan.set_variant(variant)
+    plan.init(True)
+    for n in (3, 1, 5):
+        plan.run_fixed(n, 0.3)
+        o.run_fixed(n, 0.3)
+        fv, vf = plan.messages()
+        ofv, ovf = o.messages()
+        assert same_bits(fv, ofv) and same_bits(vf, ovf), n
+        assert same_bits(plan.beliefs(), o.beliefs())
+    assert np.isnan(plan.beliefs()).any()
+    assert abs(fv[0].sum() - (1 - 0.3 ** 9)) < 1e-12
