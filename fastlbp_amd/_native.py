@@ -97,6 +97,16 @@ class BatchOptsC(C.Structure):
     ]
 
 
+class BatchGroupC(C.Structure):
+    _fields_ = [
+        ("graph", C.c_void_p),
+        ("n_variants", C.c_uint32),
+        ("clamp_off", C.c_void_p),
+        ("clamp_var", C.c_void_p),
+        ("clamp_val", C.c_void_p),
+    ]
+
+
 LAYOUT_AUTO, LAYOUT_PULL, LAYOUT_SCATTER = 0, 1, 2   # FLBP_LAYOUT_* of fastlbp.h
 
 
@@ -177,7 +187,12 @@ SIGNATURES = {
     "flbp_batch_default_opts": (None, [C.POINTER(BatchOptsC)]),
     "flbp_batch_create": (C.c_int, [C.c_uint64, C.POINTER(C.POINTER(CsrC)), C.c_int, C.POINTER(BatchOptsC),
                                     C.POINTER(C.c_void_p)]),
+    "flbp_batch_create_groups": (C.c_int, [C.c_uint64, C.POINTER(BatchGroupC), C.c_int, C.POINTER(BatchOptsC),
+                                             C.POINTER(C.c_void_p)]),
     "flbp_batch_destroy": (None, [C.c_void_p]),
+    "flbp_batch_n_variants": (C.c_uint64, [C.c_void_p]),
+    "flbp_batch_beliefs_at": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, f64p]),
+    "flbp_batch_set_p": (C.c_int, [C.c_void_p, C.c_uint64, f64p, C.c_uint64]),
     "flbp_batch_init": (C.c_int, [C.c_void_p, C.c_int]),
     "flbp_batch_run": (C.c_int, [C.c_void_p, C.POINTER(RunOptsC), u64p, f64p, C.POINTER(RunResultC)]),
     "flbp_batch_beliefs": (C.c_int, [C.c_void_p, C.c_uint64, f64p, C.c_uint64]),

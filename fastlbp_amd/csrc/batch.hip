@@ -1,17 +1,18 @@
 /*
- * batch.hip -- batched PARALL solve of many graphs (independent graphs or
- * clamped variants of one graph) in one device state.
+ * batch.hip -- batched PARALL solve of many graphs and clamped variants in
+ * one device state (no counterpart in the reference, whose CausalBP solves
+ * one graph per object; its only capture precedent is MessageSweepGraph,
+ * csrc/ext/causal_bp.cu:82-112, and a variant there is clamp -> rebuild ->
+ * run, examples/fl_wrapper.cu:26-89).
  *
- * Layout: every per-factor / per-variable / per-edge array of all graphs
- * concatenated (graph bases in BatchGraph, 64-bit; indices inside a graph
- * 32-bit).  A chunk of sweeps -- one clear kernel and, per sweep, one
- * persistent factor kernel and one persistent variable kernel over the
- * tile list -- is captured once into a hipGraph and replayed; per-graph
- * stop flags live in device memory, so one replay advances every unfinished
- * graph by up to `chunk` sweeps with no host round trip.  Tiles are the
- * plain solve's staged blocks cut per graph (their edge-slot bound is the
- * tile option), with per-tile work orders; messages use the scatter layout.
- * The tile bound and the grid size are chosen by flbp_batch_autotune.
+ * The batch holds groups (see batch_kernels.h): a group is one structure
+ * and K variants of it, K-interleaved.  Independent graphs are groups of one
+ * variant.  A whole solve -- every sweep up to the iteration cap, each a
+ * factor pass, a variable pass and the per-variant stop kernel -- is
+ * captured once into a hipGraph and replayed with one launch; the stop test
+ * and the list of running groups live on the device, so nothing returns to
+ * the host between sweeps.  Tile slot bounds and the persistent grid size
+ * are chosen by flbp_batch_autotune.
  */
 #include <hip/hip_runtime.h>
 
@@ -31,61 +32,59 @@
 
 namespace flbp {
 
-/* one batch_permute launch over the factor tiles; returns on a launch error */
-#define TRY_PERMUTE(TO_FM, SRC, DST)                                                                       \
-    do {                                                                                                    \
-        const uint32_t g_ = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_ftiles));                    \
-        hipLaunchKernelGGL(batch_permute<TO_FM>, dim3(g_), dim3(kBlock), 0, stream, graphs, ftiles, n_ftiles, \
-                           fac_off, fac_vpos, SRC, DST);                                                    \
-        const hipError_t e_ = hipGetLastError();                                                            \
-        if (e_ != hipSuccess) return hip_fail(e_, "batch_permute");                                         \
-    } while (0)
+struct GroupHost {
+    GroupDev d;
+    uint64_t n_edges = 0;
+    uint64_t perm_base = 0;   /* into h_perm: device variable -> variable */
+    uint64_t mask_rows = 0;
+};
 
 struct Batch {
     int device = -1;
     hipStream_t stream = nullptr;
-    uint64_t G = 0, Ftot = 0, Vtot = 0, Etot = 0;
+    uint64_t G = 0, NV = 0;                  /* groups, variants */
+    uint64_t Ftot = 0, Vtot = 0, Etot = 0;   /* structure totals */
+    uint64_t Mtot = 0, Stot = 0;             /* K-interleaved message / variable-state totals */
     uint64_t max_fac_deg = 0;
-    std::vector<BatchGraph> h_graphs;
-    BatchGraph *graphs = nullptr;
-    uint32_t *fac_off = nullptr, *fac_meta = nullptr, *var_off = nullptr, *var_edge = nullptr;
-    uint32_t *fac_vpos = nullptr;
-    double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
-    double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
-    double2 *m_fv_vm = nullptr;             /* f->v var-major; m_fv is the factor-major copy for damping */
-    bool fm_stale = false;                  /* scatter layout: m_fv out of date after undamped sweeps */
-    int layout = 0;                         /* resolved: kLayoutPull or kLayoutScatter */
-    uint32_t *fac_work = nullptr;           /* per tile: its slots in work order (graph-local tile slots) */
-    uint16_t *var_work = nullptr;           /* per tile: its variables by degree */
-    std::vector<uint32_t> h_fac_off, h_fac_meta, h_var_off;   /* host copies, batch-concatenated */
-    Tile *ftiles = nullptr, *vtiles = nullptr;           /* every graph's tiles */
-    uint64_t cap_ftiles = 0, cap_vtiles = 0;
+    std::vector<GroupHost> hg;
+    std::vector<uint32_t> h_fac_off, h_fac_meta, h_var_off;   /* host copies for tiling */
+    std::vector<uint32_t> h_perm;
+    std::vector<uint32_t> var_group;         /* variant -> group */
+    GroupDev *groups = nullptr;
+    uint32_t *fac_off = nullptr, *fac_meta = nullptr, *fac_var = nullptr, *fac_work = nullptr, *fac_crow = nullptr;
+    double *fac_p = nullptr, *fac_q = nullptr;
+    double2 *masks = nullptr;
+    uint32_t *var_off = nullptr, *var_edge = nullptr;
+    uint16_t *var_work = nullptr;
+    double2 *m_fv = nullptr, *vprod = nullptr, *scratch = nullptr;
+    uint8_t *vzc = nullptr;
+    double *belief = nullptr;
+    uint8_t *vrun = nullptr, *vfresh = nullptr;
+    uint32_t *budget = nullptr;
+    unsigned long long *iters_dev = nullptr, *diff = nullptr;
+    double *last_dev = nullptr;
+    uint32_t *act_n = nullptr, *act_group = nullptr, *act_fpre = nullptr, *act_vpre = nullptr;
+    Tile *ftiles = nullptr, *vtiles = nullptr;
+    uint64_t cap_ftiles = 0, cap_vtiles = 0, cap_scratch = 0;
     uint32_t n_ftiles = 0, n_vtiles = 0;
-    std::vector<Tile> h_ftiles, h_vtiles;                 /* host copies, grouped by graph */
-    std::vector<uint32_t> h_ftile_off, h_vtile_off;       /* per graph: its tiles' range (G+1) */
-    Tile *act_ftiles = nullptr, *act_vtiles = nullptr;    /* tiles of the running graphs */
-    uint32_t *act_count = nullptr;                        /* device: [factor tiles, variable tiles] */
-    std::vector<uint8_t> act_set;                         /* running set the active lists hold */
-    uint32_t cut_f = 0, cut_v = 0;          /* slot bounds of the current tiles */
-    unsigned long long *diff = nullptr;     /* [kMaxChunk][G][kBatchSlots][kCtlPad] partial maxima */
-    unsigned long long *diff_out = nullptr; /* [kMaxChunk][G] compacted */
-    uint8_t *start_active = nullptr;        /* device [G] */
-    uint32_t *budget = nullptr;             /* device [G] */
-    unsigned long long *h_diff = nullptr;   /* pinned */
-    uint8_t *h_active = nullptr;            /* pinned */
-    uint32_t *h_budget = nullptr;           /* pinned */
+    uint32_t cut_f = 0, cut_v = 0;
+    double init_c = 1.0;
+    /* pinned host staging */
+    uint8_t *h_run = nullptr;
+    uint32_t *h_budget = nullptr;
+    unsigned long long *h_iters = nullptr;
+    double *h_last = nullptr;
+    uint32_t *h_actn = nullptr;   /* act_n after each chunk launch */
     std::vector<uint64_t> iters;
     std::vector<double> last_diff;
-    std::vector<uint32_t> h_perm;           /* per graph at var_base: device variable -> variable */
     std::vector<void *> allocs;
     uint64_t bytes = 0;
     flbp_batch_opts opts{};
-    /* captured chunk */
+    /* the captured solve */
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    int exec_chunk = 0;
+    uint32_t exec_n = 0, exec_fgrid = 0, exec_vgrid = 0;
     double exec_damp = -1, exec_tol = 0;
-    uint32_t exec_fgrid = 0, exec_vgrid = 0;
 
     template <class T>
     int alloc(T *&p, uint64_t n) {
@@ -116,76 +115,49 @@ struct Batch {
         if (device >= 0) (void)hipSetDevice(device);
         drop_exec();
         for (void *q : allocs) (void)hipFree(q);
-        if (h_diff) (void)hipHostFree(h_diff);
-        if (h_active) (void)hipHostFree(h_active);
-        if (h_budget) (void)hipHostFree(h_budget);
+        for (void *q : {(void *)h_run, (void *)h_budget, (void *)h_iters, (void *)h_last, (void *)h_actn})
+            if (q) (void)hipHostFree(q);
         if (stream) (void)hipStreamDestroy(stream);
     }
+    BatchArgs args() const {
+        BatchArgs a;
+        a.groups = groups;
+        a.ftiles = ftiles;
+        a.vtiles = vtiles;
+        a.act_n = act_n;
+        a.act_group = act_group;
+        a.act_fpre = act_fpre;
+        a.act_vpre = act_vpre;
+        a.fac_off = fac_off;
+        a.fac_meta = fac_meta;
+        a.fac_var = fac_var;
+        a.fac_work = fac_work;
+        a.fac_crow = fac_crow;
+        a.fac_p = fac_p;
+        a.fac_q = fac_q;
+        a.masks = masks;
+        a.var_off = var_off;
+        a.var_edge = var_edge;
+        a.var_work = var_work;
+        a.m_fv = m_fv;
+        a.vprod = vprod;
+        a.vzc = vzc;
+        a.belief = belief;
+        a.scratch = scratch;
+        a.vrun = vrun;
+        a.vfresh = vfresh;
+        a.diff = diff;
+        a.c = init_c;
+        return a;
+    }
+    StopArgs stop_args(double tol) const {
+        return StopArgs{groups, (uint32_t)G, (uint32_t)NV, vrun, vfresh, budget, iters_dev, last_dev, diff,
+                        act_n, act_group, act_fpre, act_vpre, tol};
+    }
     int build_tiles();
-    /* the sweep kernels' tile lists: the tiles of the graphs with run[g]
-     * set, uploaded when that set changes */
-    int set_active(const uint8_t *run) {
-        if (act_set.size() == G && std::equal(act_set.begin(), act_set.end(), run)) return FLBP_OK;
-        act_set.assign(run, run + G);
-        std::vector<Tile> fa, va;
-        for (uint64_t g = 0; g < G; g++) {
-            if (!run[g]) continue;
-            fa.insert(fa.end(), h_ftiles.begin() + h_ftile_off[g], h_ftiles.begin() + h_ftile_off[g + 1]);
-            va.insert(va.end(), h_vtiles.begin() + h_vtile_off[g], h_vtiles.begin() + h_vtile_off[g + 1]);
-        }
-        const uint32_t cnt[2] = {(uint32_t)fa.size(), (uint32_t)va.size()};
-        if (cnt[0] && hipMemcpyAsync(act_ftiles, fa.data(), fa.size() * sizeof(Tile), hipMemcpyHostToDevice, stream) !=
-                          hipSuccess)
-            return fail(FLBP_EHIP, "batch: uploading the active tiles failed");
-        if (cnt[1] && hipMemcpyAsync(act_vtiles, va.data(), va.size() * sizeof(Tile), hipMemcpyHostToDevice, stream) !=
-                          hipSuccess)
-            return fail(FLBP_EHIP, "batch: uploading the active tiles failed");
-        if (hipMemcpyAsync(act_count, cnt, sizeof cnt, hipMemcpyHostToDevice, stream) != hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess)   /* the host vectors die here */
-            return fail(FLBP_EHIP, "batch: uploading the active tiles failed");
-        return FLBP_OK;
-    }
-    int resolve_layout() const {
-        if (opts.layout == FLBP_LAYOUT_PULL) return kLayoutPull;
-        if (opts.layout == FLBP_LAYOUT_SCATTER) return kLayoutScatter;
-        return bytes <= kScatterMaxBytes ? kLayoutScatter : kLayoutPull;
-    }
-    /* undamped sweeps skip the factor-major f->v copy; a damped run first
-     * rebuilds it from the var-major array */
-    int prepare_damping(double damping) {
-        if (layout != kLayoutScatter) return FLBP_OK;
-        if (damping != 0.0 && fm_stale && n_ftiles) TRY_PERMUTE(true, m_fv_vm, m_fv);
-        fm_stale = damping == 0.0 ? true : (fm_stale && !n_ftiles);
-        return FLBP_OK;
-    }
-    /* move the stored messages to layout `to` (kLayoutPull / kLayoutScatter) */
-    int set_layout(int to) {
-        if (to == layout) return FLBP_OK;
-        if (n_ftiles) {
-            if (to == kLayoutScatter) {
-                /* v->f var-major -> factor-major through the spare array,
-                 * f->v factor-major -> var-major; the factor-major f->v
-                 * copy stays valid */
-                TRY_PERMUTE(true, m_vf, m_fv_vm);
-                if (hipMemcpyAsync(m_vf, m_fv_vm, Etot * sizeof(double2), hipMemcpyDeviceToDevice, stream) != hipSuccess)
-                    return fail(FLBP_EHIP, "batch: layout copy failed");
-                TRY_PERMUTE(false, m_fv, m_fv_vm);
-                fm_stale = false;
-            } else {
-                if (fm_stale) TRY_PERMUTE(true, m_fv_vm, m_fv);
-                TRY_PERMUTE(false, m_vf, m_fv_vm);
-                if (hipMemcpyAsync(m_vf, m_fv_vm, Etot * sizeof(double2), hipMemcpyDeviceToDevice, stream) != hipSuccess)
-                    return fail(FLBP_EHIP, "batch: layout copy failed");
-                fm_stale = false;
-            }
-            if (hipStreamSynchronize(stream) != hipSuccess) return fail(FLBP_EHIP, "batch: layout move failed");
-        }
-        layout = to;
-        drop_exec();
-        return FLBP_OK;
-    }
-    int record_chunk(int n, double tol, double damping);
-    int enqueue_chunk(int n, double tol, double damping);
+    int record_sweeps(uint32_t n, double tol, double damping);
+    int launch_solve(uint32_t n, double tol, double damping);
+    int start_run(uint64_t maxiter, double tol);
 };
 
 #define TRY(expr)            \
@@ -199,29 +171,42 @@ struct Batch {
         if (e_ != hipSuccess) return hip_fail(e_, #expr); \
     } while (0)
 
+/* Tiles of every group for the current slot bounds: a K-variant group's
+ * tiles hold at most bound / K slots (and 256 / K variables per variable
+ * tile), so each tile fills a block's kStage LDS slots with all K variants;
+ * per tile the plain solve's work orders.  A factor wider than its bound
+ * gets a tile of its own and a scratch range for its v->f inputs. */
 int Batch::build_tiles() {
-    const uint32_t sf = std::min<uint32_t>(std::max<uint32_t>(opts.tile_factors, 64), kStageSlots);
-    const uint32_t sv = std::min<uint32_t>(std::max<uint32_t>(opts.tile_vars, 64), kStageSlots);
-    drop_exec();
+    const uint32_t sf = opts.tile_factors, sv = opts.tile_vars;
     if (sf == cut_f && sv == cut_v && ftiles) return FLBP_OK;
-    /* per graph: staged blocks and their work orders, graphs spread over
-     * host threads */
+    drop_exec();
     std::vector<std::vector<Tile>> ft(G), vt(G);
+    std::vector<uint64_t> scr(G, 0);
     std::vector<uint32_t> fw(Etot);
     std::vector<uint16_t> vw(Vtot);
     auto cut = [&](uint64_t g) {
-        const BatchGraph &bg = h_graphs[g];
-        const uint32_t *fo = h_fac_off.data() + bg.fac_off_base;
-        const uint32_t *vo = h_var_off.data() + bg.var_off_base;
+        const GroupDev &d = hg[g].d;
+        const uint32_t *fo = h_fac_off.data() + d.fac_off_base;
+        const uint32_t *vo = h_var_off.data() + d.var_off_base;
+        const uint32_t fs = std::max<uint32_t>(1, sf >> d.lk), vs = std::max<uint32_t>(1, sv >> d.lk);
+        const uint32_t vi = std::max<uint32_t>(1, kStageItems >> d.lk);
         std::vector<uint32_t> fb, vb, gfw;
         std::vector<uint16_t> gvw;
-        cut_blocks(fo, bg.n_fac, kStageItems, sf, fb);
-        cut_blocks(vo, bg.n_var, kStageItems, sv, vb);
-        stage_work_orders(fo, h_fac_meta.data() + bg.fac_base, fb, vo, vb, gfw, gvw);
-        std::copy(gfw.begin(), gfw.end(), fw.begin() + bg.edge_base);
-        std::copy(gvw.begin(), gvw.end(), vw.begin() + bg.var_base);
-        for (size_t b = 0; b + 1 < fb.size(); b++) ft[g].push_back({(uint32_t)g, fb[b], fb[b + 1] - fb[b]});
-        for (size_t b = 0; b + 1 < vb.size(); b++) vt[g].push_back({(uint32_t)g, vb[b], vb[b + 1] - vb[b]});
+        cut_blocks(fo, d.n_fac, kStageItems, fs, fb);
+        cut_blocks(vo, d.n_var, vi, vs, vb);
+        stage_work_orders(fo, h_fac_meta.data() + d.fac_base, fb, vo, vb, gfw, gvw);
+        std::copy(gfw.begin(), gfw.end(), fw.begin() + d.edge_base);
+        std::copy(gvw.begin(), gvw.end(), vw.begin() + d.var_base);
+        for (size_t b = 0; b + 1 < fb.size(); b++) {
+            Tile t{(uint32_t)g, fb[b], fb[b + 1] - fb[b], 0u};
+            const uint32_t n = fo[fb[b + 1]] - fo[fb[b]];
+            if (((uint64_t)n << d.lk) > (uint64_t)kStageSlots) {   /* a wide factor: scratch range */
+                t.aux = (uint32_t)scr[g];
+                scr[g] += n;
+            }
+            ft[g].push_back(t);
+        }
+        for (size_t b = 0; b + 1 < vb.size(); b++) vt[g].push_back({(uint32_t)g, vb[b], vb[b + 1] - vb[b], 0u});
     };
     const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> pool;
@@ -231,11 +216,23 @@ int Batch::build_tiles() {
         });
     for (auto &th : pool) th.join();
     std::vector<Tile> fall, vall;
+    uint64_t scr_tot = 0;
     for (uint64_t g = 0; g < G; g++) {
+        GroupDev &d = hg[g].d;
+        d.ftile_first = (uint32_t)fall.size();
+        d.ftile_count = (uint32_t)ft[g].size();
+        d.vtile_first = (uint32_t)vall.size();
+        d.vtile_count = (uint32_t)vt[g].size();
+        for (Tile &t : ft[g])
+            if (((uint64_t)(h_fac_off[d.fac_off_base + t.first + t.count] - h_fac_off[d.fac_off_base + t.first])
+                 << d.lk) > (uint64_t)kStageSlots)
+                t.aux += (uint32_t)scr_tot;
+        scr_tot += scr[g];
         fall.insert(fall.end(), ft[g].begin(), ft[g].end());
         vall.insert(vall.end(), vt[g].begin(), vt[g].end());
+        if (fall.size() >= (1ull << 31) || vall.size() >= (1ull << 31) || scr_tot >= (1ull << 32))
+            return fail(FLBP_ERANGE, "batch: too many tiles");
     }
-    if (fall.size() >= (1ull << 31) || vall.size() >= (1ull << 31)) return fail(FLBP_ERANGE, "batch: too many tiles");
     n_ftiles = (uint32_t)fall.size();
     n_vtiles = (uint32_t)vall.size();
     if (n_ftiles > cap_ftiles) {
@@ -250,25 +247,17 @@ int Batch::build_tiles() {
         TRY(alloc(vtiles, n_vtiles));
         cap_vtiles = n_vtiles;
     }
+    if (scr_tot > cap_scratch || !scratch) {
+        if (scratch) release(scratch);
+        scratch = nullptr;
+        TRY(alloc(scratch, scr_tot));
+        cap_scratch = std::max<uint64_t>(scr_tot, 1);
+    }
     if (n_ftiles) HIP_TRY(hipMemcpy(ftiles, fall.data(), fall.size() * sizeof(Tile), hipMemcpyHostToDevice));
     if (n_vtiles) HIP_TRY(hipMemcpy(vtiles, vall.data(), vall.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    h_ftile_off.assign(G + 1, 0);
-    h_vtile_off.assign(G + 1, 0);
-    for (uint64_t g = 0; g < G; g++) {
-        h_ftile_off[g + 1] = h_ftile_off[g] + (uint32_t)ft[g].size();
-        h_vtile_off[g + 1] = h_vtile_off[g] + (uint32_t)vt[g].size();
-    }
-    h_ftiles.swap(fall);
-    h_vtiles.swap(vall);
-    if (act_ftiles) release(act_ftiles);
-    if (act_vtiles) release(act_vtiles);
-    act_ftiles = act_vtiles = nullptr;
-    TRY(alloc(act_ftiles, n_ftiles));
-    TRY(alloc(act_vtiles, n_vtiles));
-    if (!act_count) TRY(alloc(act_count, 2));
-    act_set.clear();
-    std::vector<uint8_t> all(G, 1);
-    TRY(set_active(all.data()));
+    std::vector<GroupDev> gd(G);
+    for (uint64_t g = 0; g < G; g++) gd[g] = hg[g].d;
+    HIP_TRY(hipMemcpy(groups, gd.data(), G * sizeof(GroupDev), hipMemcpyHostToDevice));
     if (Etot) HIP_TRY(hipMemcpy(fac_work, fw.data(), Etot * 4, hipMemcpyHostToDevice));
     if (Vtot) HIP_TRY(hipMemcpy(var_work, vw.data(), Vtot * 2, hipMemcpyHostToDevice));
     cut_f = sf;
@@ -276,72 +265,84 @@ int Batch::build_tiles() {
     return FLBP_OK;
 }
 
-int Batch::record_chunk(int n, double tol, double damping) {
+/* n sweeps: factor pass, variable pass and stop kernel each */
+int Batch::record_sweeps(uint32_t n, double tol, double damping) {
     const double keep = 1.0 - damping;
-    BatchCtl c{diff, start_active, budget, (uint32_t)G, tol};
-    const uint64_t nslots = (uint64_t)n * G * kBatchSlots * kCtlPad;
-    hipLaunchKernelGGL(batch_clear, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(1024, (nslots + 255) / 256))),
-                       dim3(256), 0, stream, diff, nslots);
+    const BatchArgs a = args();
+    const StopArgs st = stop_args(tol);
     const uint32_t fg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_ftiles));
     const uint32_t vg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_vtiles));
-    for (int k = 0; k < n; k++) {
-        if (n_ftiles) {
-            const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
-#define FLBP_BATCH_FACTOR_PASS(DAMP, N, L)                                                                    \
-    hipLaunchKernelGGL((batch_factor_pass<DAMP, N, L>), dim3(fg), dim3(kBlock), 0, stream, graphs, act_ftiles, \
-                       act_count, fac_off, fac_meta, fac_p, fac_q, fac_mask, fac_vpos, fac_work, m_vf, m_fv,    \
-                       m_fv_vm, damping, keep, c, k)
-#define FLBP_BATCH_FACTOR_PASS_L(DAMP, N)                                                   \
-    do {                                                                                    \
-        if (layout == kLayoutScatter) FLBP_BATCH_FACTOR_PASS(DAMP, N, kLayoutScatter);      \
-        else FLBP_BATCH_FACTOR_PASS(DAMP, N, kLayoutPull);                                  \
-    } while (0)
-            if (damping != 0.0) {
-                if (narrow) FLBP_BATCH_FACTOR_PASS_L(true, kFacRegDegSmall);
-                else FLBP_BATCH_FACTOR_PASS_L(true, kFacRegDeg);
-            } else {
-                if (narrow) FLBP_BATCH_FACTOR_PASS_L(false, kFacRegDegSmall);
-                else FLBP_BATCH_FACTOR_PASS_L(false, kFacRegDeg);
-            }
-#undef FLBP_BATCH_FACTOR_PASS_L
-#undef FLBP_BATCH_FACTOR_PASS
-        }
-        if (n_vtiles) {
-            if (layout == kLayoutScatter)
-                hipLaunchKernelGGL(batch_var_pass<kLayoutScatter>, dim3(vg), dim3(kBlock), 0, stream, graphs,
-                                   act_vtiles, act_count + 1, var_off, var_edge, var_work, m_fv_vm, m_vf, belief, c, k);
+    const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
+    for (uint32_t k = 0; k < n; k++) {
+        if (damping != 0.0) {
+            if (narrow)
+                hipLaunchKernelGGL((batch_factor_pass<true, kFacRegDegSmall>), dim3(fg), dim3(kBlock), 0, stream, a,
+                                   damping, keep);
             else
-                hipLaunchKernelGGL(batch_var_pass<kLayoutPull>, dim3(vg), dim3(kBlock), 0, stream, graphs,
-                                   act_vtiles, act_count + 1, var_off, var_edge, var_work, m_fv, m_vf, belief, c, k);
+                hipLaunchKernelGGL((batch_factor_pass<true, kFacRegDeg>), dim3(fg), dim3(kBlock), 0, stream, a,
+                                   damping, keep);
+        } else {
+            if (narrow)
+                hipLaunchKernelGGL((batch_factor_pass<false, kFacRegDegSmall>), dim3(fg), dim3(kBlock), 0, stream, a,
+                                   damping, keep);
+            else
+                hipLaunchKernelGGL((batch_factor_pass<false, kFacRegDeg>), dim3(fg), dim3(kBlock), 0, stream, a,
+                                   damping, keep);
         }
+        hipLaunchKernelGGL(batch_var_pass, dim3(vg), dim3(kBlock), 0, stream, a);
+        hipLaunchKernelGGL(batch_stop, dim3(1), dim3(kStopBlock), 0, stream, st, 1);
     }
-    const uint64_t waves = (uint64_t)n * G;
-    hipLaunchKernelGGL(batch_compact, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(1024, (waves * 64 + 255) / 256))),
-                       dim3(256), 0, stream, c, diff_out, n);
+    HIP_TRY(hipGetLastError());
     return FLBP_OK;
 }
 
-int Batch::enqueue_chunk(int n, double tol, double damping) {
-    if (!opts.use_graph) {
-        TRY(record_chunk(n, tol, damping));
-        HIP_TRY(hipGetLastError());
-        return FLBP_OK;
-    }
-    if (!exec || exec_chunk != n || exec_damp != damping || exec_tol != tol) {
+/* enqueue n sweeps as one replay of the captured solve (captured on first
+ * use for this length, damping, tol and grid) or as direct launches */
+int Batch::launch_solve(uint32_t n, double tol, double damping) {
+    if (!opts.use_graph) return record_sweeps(n, tol, damping);
+    const uint32_t fg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_ftiles));
+    const uint32_t vg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_vtiles));
+    if (!exec || exec_n != n || exec_damp != damping || exec_tol != tol || exec_fgrid != fg || exec_vgrid != vg) {
         drop_exec();
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = record_chunk(n, tol, damping);
+        const int rc = record_sweeps(n, tol, damping);
         hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(stream, &g);
-        if (rc) return rc;
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
         if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
         graph = g;
-        HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        exec_chunk = n;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            drop_exec();
+            return hip_fail(ei, "hipGraphInstantiate");
+        }
+        exec_n = n;
         exec_damp = damping;
         exec_tol = tol;
+        exec_fgrid = fg;
+        exec_vgrid = vg;
     }
     HIP_TRY(hipGraphLaunch(exec, stream));
+    return FLBP_OK;
+}
+
+/* per-variant budgets and run flags for a run up to maxiter, then the
+ * first list of running groups */
+int Batch::start_run(uint64_t maxiter, double tol) {
+    for (uint64_t v = 0; v < NV; v++) {
+        const uint64_t left = maxiter > iters[v] ? maxiter - iters[v] : 0;
+        h_budget[v] = (uint32_t)std::min<uint64_t>(left, 0xffffffffu);
+        h_run[v] = left > 0;
+        h_last[v] = INFINITY;
+    }
+    HIP_TRY(hipMemcpyAsync(budget, h_budget, NV * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(vrun, h_run, NV, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(last_dev, h_last, NV * 8, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(batch_stop, dim3(1), dim3(kStopBlock), 0, stream, stop_args(tol), 0);
+    HIP_TRY(hipGetLastError());
     return FLBP_OK;
 }
 
@@ -353,22 +354,30 @@ using flbp::hip_fail;
 
 namespace {
 
-__global__ void bfill2(double2 *p, uint64_t n, double c) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = make_double2(c, c);
-}
-
 int default_grid(int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_factor_pass<true, kFacRegDegSmall, kLayoutScatter>,
-                                                     kBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_factor_pass<true, kFacRegDegSmall>, kBlock, 0) !=
             hipSuccess ||
         per_cu <= 0)
         per_cu = 4;
     return prop.multiProcessorCount * per_cu;
 }
+
+int check_opts(const flbp_batch_opts &o, const char *who) {
+    if (o.layout != 0) return flbp::fail(FLBP_EINVAL, std::string(who) + ": layout must be 0 (the state layout)");
+    if (o.tile_factors < 64 || o.tile_factors > flbp::kStageSlots || o.tile_vars < 64 ||
+        o.tile_vars > flbp::kStageSlots)
+        return flbp::fail(FLBP_EINVAL, std::string(who) + ": tile bounds must be 64.." +
+                                           std::to_string(flbp::kStageSlots) + " slots");
+    return FLBP_OK;
+}
+
+/* host view of one group's variant clamps, checked against its structure */
+struct ClampSet {
+    std::vector<std::vector<std::pair<uint64_t, uint8_t>>> per_variant;   /* (variable, value), ascending variable */
+};
 
 }  // namespace
 
@@ -379,125 +388,253 @@ void flbp_batch_default_opts(flbp_batch_opts *o) {
     o->tile_factors = flbp::kStageSlots;
     o->tile_vars = flbp::kStageSlots;
     o->grid_blocks = 0;
-    o->chunk = 32;
+    o->chunk = 0;
     o->use_graph = 1;
-    o->layout = FLBP_LAYOUT_AUTO;
+    o->layout = 0;
+}
+
+int flbp_batch_create_groups(uint64_t n_groups, const flbp_batch_group *gs, int device, const flbp_batch_opts *opts,
+                             flbp_batch **out) {
+    if (!gs || !out || n_groups == 0) return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: bad arguments");
+    if (n_groups >= (1ull << 31)) return flbp::fail(FLBP_ERANGE, "flbp_batch_create_groups: too many groups");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return flbp::fail(FLBP_ENODEV, "flbp_batch_create_groups: no HIP device");
+    if (device < 0 || device >= ndev) return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: bad device");
+    flbp_batch_opts o;
+    if (opts) o = *opts;
+    else flbp_batch_default_opts(&o);
+    if (int rc = check_opts(o, "flbp_batch_create_groups")) return rc;
+    /* validation: structures, variant counts, clamps */
+    uint64_t NV = 0, F = 0, V = 0, E = 0, M = 0, S = 0;
+    for (uint64_t g = 0; g < n_groups; g++) {
+        const flbp_batch_group &bg = gs[g];
+        if (!bg.graph) return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: null graph");
+        const flbp_csr &c = *bg.graph;
+        if (c.n_vars >= (1ull << 32) - 1 || c.n_factors >= (1ull << 32) - 1 || c.n_edges >= (1ull << 32) - 1)
+            return flbp::fail(FLBP_ERANGE, "flbp_batch_create_groups: a graph has V, F or E >= 2^32-1");
+        const uint32_t K = bg.n_variants;
+        if (K == 0 || K > 64 || (K & (K - 1)))
+            return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: variants per group must be 1, 2, 4, ..., 64");
+        if (bg.clamp_off && (!bg.clamp_var || !bg.clamp_val) && bg.clamp_off[K] > bg.clamp_off[0])
+            return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: clamp arrays missing");
+        if (int rc = flbp::validate_csr(c)) return rc;
+        NV += K;
+        F += c.n_factors;
+        V += c.n_vars;
+        E += c.n_edges;
+        M += c.n_edges * K;
+        S += c.n_vars * K;
+    }
+    if (NV >= (1ull << 31)) return flbp::fail(FLBP_ERANGE, "flbp_batch_create_groups: too many variants");
+    HIP_TRY(hipSetDevice(device));
+    auto *B = new (std::nothrow) Batch();
+    if (!B) return flbp::fail(FLBP_ENOMEM, "flbp_batch_create_groups: out of memory");
+    std::unique_ptr<Batch> hold(B);
+    B->device = device;
+    B->G = n_groups;
+    B->NV = NV;
+    B->Ftot = F;
+    B->Vtot = V;
+    B->Etot = E;
+    B->Mtot = M;
+    B->Stot = S;
+    B->opts = o;
+    if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(device);
+    B->hg.resize(n_groups);
+    {
+        uint64_t fb = 0, vb = 0, eb = 0, mb = 0, sb = 0, nb = 0;
+        for (uint64_t g = 0; g < n_groups; g++) {
+            const flbp_csr &c = *gs[g].graph;
+            GroupDev &d = B->hg[g].d;
+            d = GroupDev{};
+            d.fac_base = fb;
+            d.var_base = vb;
+            d.edge_base = eb;
+            d.fac_off_base = fb + g;
+            d.var_off_base = vb + g;
+            d.msg_base = mb;
+            d.vst_base = sb;
+            d.n_fac = (uint32_t)c.n_factors;
+            d.n_var = (uint32_t)c.n_vars;
+            d.K = gs[g].n_variants;
+            d.lk = (uint32_t)__builtin_ctz(d.K);
+            d.vbase = (uint32_t)nb;
+            B->hg[g].n_edges = c.n_edges;
+            B->hg[g].perm_base = vb;
+            for (uint64_t k = 0; k < d.K; k++) B->var_group.push_back((uint32_t)g);
+            for (uint64_t I = 0; I < c.n_factors; I++)
+                B->max_fac_deg = std::max<uint64_t>(B->max_fac_deg, c.fac_off[I + 1] - c.fac_off[I]);
+            fb += c.n_factors;
+            vb += c.n_vars;
+            eb += c.n_edges;
+            mb += c.n_edges * d.K;
+            sb += c.n_vars * d.K;
+            nb += d.K;
+        }
+    }
+    HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    TRY(B->alloc(B->groups, n_groups));
+    TRY(B->alloc(B->fac_off, F + n_groups));
+    TRY(B->alloc(B->fac_meta, F));
+    TRY(B->alloc(B->fac_var, E));
+    TRY(B->alloc(B->fac_work, E));
+    TRY(B->alloc(B->fac_crow, F));
+    TRY(B->alloc(B->fac_p, F));
+    TRY(B->alloc(B->fac_q, F));
+    TRY(B->alloc(B->var_off, V + n_groups));
+    TRY(B->alloc(B->var_edge, E));
+    TRY(B->alloc(B->var_work, V));
+    TRY(B->alloc(B->m_fv, M));
+    TRY(B->alloc(B->vprod, S));
+    TRY(B->alloc(B->vzc, S));
+    TRY(B->alloc(B->belief, S));
+    TRY(B->alloc(B->vrun, NV));
+    TRY(B->alloc(B->vfresh, NV));
+    TRY(B->alloc(B->budget, NV));
+    TRY(B->alloc(B->iters_dev, NV));
+    TRY(B->alloc(B->last_dev, NV));
+    TRY(B->alloc(B->diff, NV * kBatchSlots * kCtlPad));
+    TRY(B->alloc(B->act_n, 1));
+    TRY(B->alloc(B->act_group, n_groups));
+    TRY(B->alloc(B->act_fpre, n_groups + 1));
+    TRY(B->alloc(B->act_vpre, n_groups + 1));
+    HIP_TRY(hipHostMalloc((void **)&B->h_run, NV, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&B->h_budget, NV * 4, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&B->h_iters, NV * 8, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&B->h_last, NV * 8, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&B->h_actn, 4 * 64, hipHostMallocDefault));
+    B->h_fac_off.resize(F + n_groups);
+    B->h_fac_meta.resize(F);
+    B->h_var_off.resize(V + n_groups);
+    /* structure upload, group by group; head masks per clamped factor */
+    std::vector<double2> mask_all;
+    std::vector<uint32_t> crow(F, 0);
+    for (uint64_t g = 0; g < n_groups; g++) {
+        const flbp_batch_group &bg = gs[g];
+        const flbp_csr &c = *bg.graph;
+        GroupDev &d = B->hg[g].d;
+        const uint64_t Fg = c.n_factors, Vg = c.n_vars, Eg = c.n_edges;
+        const uint32_t K = d.K;
+        /* variables' head factors and body uses, for the clamps */
+        std::vector<uint32_t> head_var(Fg);
+        std::vector<uint8_t> body_use(Vg, 0);
+        for (uint64_t I = 0; I < Fg; I++) {
+            head_var[I] = (uint32_t)c.edge_var[c.head_slot[I]];
+            for (uint64_t s = c.fac_off[I]; s < c.fac_off[I + 1]; s++)
+                if (s != c.head_slot[I]) body_use[c.edge_var[s]] = 1;
+        }
+        std::vector<std::vector<uint32_t>> heads_of;   /* built on demand */
+        std::vector<std::vector<std::pair<uint32_t, uint8_t>>> clamps(K);   /* (factor, value) per variant */
+        if (bg.clamp_off) {
+            heads_of.assign(Vg, {});
+            for (uint64_t I = 0; I < Fg; I++)
+                if (c.fac_type[I] != 0) heads_of[head_var[I]].push_back((uint32_t)I);
+            for (uint32_t k = 0; k < K; k++) {
+                std::vector<std::pair<uint64_t, uint8_t>> cv;
+                for (uint64_t x = bg.clamp_off[k]; x < bg.clamp_off[k + 1]; x++) {
+                    const uint64_t v = bg.clamp_var[x];
+                    if (v >= Vg) return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: clamped variable out of range");
+                    if (bg.clamp_val[x] > 1) return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: clamp values are 0 or 1");
+                    if (body_use[v])
+                        return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: variable " + std::to_string(v) +
+                                                           " appears in a factor body; its clamp changes the structure "
+                                                           "(rebuild the graph with flbp_clamp and give it its own group)");
+                    if (heads_of[v].empty())
+                        return flbp::fail(FLBP_EINVAL, "flbp_batch_create_groups: variable " + std::to_string(v) +
+                                                           " heads no AND/OR factor");
+                    cv.push_back({v, bg.clamp_val[x]});
+                }
+                /* ascending variable (= label) order, as fl_wrapper applies clamps */
+                std::stable_sort(cv.begin(), cv.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+                for (const auto &pv : cv)
+                    for (uint32_t I : heads_of[pv.first]) clamps[k].push_back({I, pv.second});
+            }
+        }
+        /* per factor: base clamp, and each variant's mask (gen_clamped's
+         * head rule: one-hot, a repeated clamp multiplies) */
+        std::vector<uint8_t> any(Fg, 0);
+        for (uint64_t I = 0; I < Fg; I++) any[I] = c.head_clamped[I] != 0;
+        for (uint32_t k = 0; k < K; k++)
+            for (const auto &pr : clamps[k]) any[pr.first] = 1;
+        std::vector<int64_t> row(Fg, -1);
+        for (uint64_t I = 0; I < Fg; I++) {
+            if (!any[I]) continue;
+            row[I] = (int64_t)mask_all.size();
+            crow[d.fac_base + I] = (uint32_t)mask_all.size();
+            for (uint32_t k = 0; k < K; k++)
+                mask_all.push_back(c.head_clamped[I] ? make_double2(c.mask0[I], c.mask1[I]) : make_double2(1.0, 1.0));
+        }
+        std::vector<uint8_t> clamped(Fg, 0);   /* per factor: clamped before the clamp at hand */
+        for (uint32_t k = 0; k < K; k++) {
+            for (const auto &pr : clamps[k]) clamped[pr.first] = c.head_clamped[pr.first] != 0;
+            for (const auto &pr : clamps[k]) {
+                const uint32_t I = pr.first;
+                const uint8_t x = pr.second;
+                double2 &m = mask_all[(uint64_t)row[I] + k];
+                double m0 = x == 0 ? 1.0 : 0.0, m1 = x == 1 ? 1.0 : 0.0;
+                if (clamped[I]) {   /* a repeated clamp multiplies (causal_factor.cpp:78-81) */
+                    m0 *= m.x;
+                    m1 *= m.y;
+                }
+                m = make_double2(m0, m1);
+                clamped[I] = 1;
+            }
+        }
+        if (mask_all.size() >= (1ull << 32)) return flbp::fail(FLBP_ERANGE, "flbp_batch_create_groups: too many clamps");
+        std::vector<uint32_t> u32(std::max(std::max(Fg, Vg), Eg) + 1);
+        for (uint64_t I = 0; I <= Fg; I++) u32[I] = (uint32_t)c.fac_off[I];
+        std::copy(u32.begin(), u32.begin() + Fg + 1, B->h_fac_off.begin() + d.fac_off_base);
+        HIP_TRY(hipMemcpy(B->fac_off + d.fac_off_base, u32.data(), (Fg + 1) * 4, hipMemcpyHostToDevice));
+        for (uint64_t I = 0; I < Fg; I++)
+            u32[I] = flbp::dev::pack_meta((uint32_t)c.fac_type[I], any[I] != 0, (uint32_t)(c.head_slot[I] - c.fac_off[I]));
+        std::copy(u32.begin(), u32.begin() + Fg, B->h_fac_meta.begin() + d.fac_base);
+        if (Fg) {
+            HIP_TRY(hipMemcpy(B->fac_meta + d.fac_base, u32.data(), Fg * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(B->fac_p + d.fac_base, c.p, Fg * 8, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(B->fac_q + d.fac_base, c.q, Fg * 8, hipMemcpyHostToDevice));
+        }
+        std::vector<uint32_t> perm, dev_off, dev_edge, vpos;
+        flbp::var_device_order(c, flbp::kOrderHotFirst, perm, dev_off, dev_edge, vpos);
+        B->h_perm.insert(B->h_perm.end(), perm.begin(), perm.end());
+        std::copy(dev_off.begin(), dev_off.end(), B->h_var_off.begin() + d.var_off_base);
+        HIP_TRY(hipMemcpy(B->var_off + d.var_off_base, dev_off.data(), (Vg + 1) * 4, hipMemcpyHostToDevice));
+        if (Eg) {
+            HIP_TRY(hipMemcpy(B->var_edge + d.edge_base, dev_edge.data(), Eg * 4, hipMemcpyHostToDevice));
+            std::vector<uint32_t> inv(Vg), fv(Eg);
+            for (uint64_t dv = 0; dv < Vg; dv++) inv[perm[dv]] = (uint32_t)dv;
+            for (uint64_t s = 0; s < Eg; s++) fv[s] = inv[c.edge_var[s]];
+            HIP_TRY(hipMemcpy(B->fac_var + d.edge_base, fv.data(), Eg * 4, hipMemcpyHostToDevice));
+        }
+    }
+    TRY(B->alloc(B->masks, mask_all.size()));
+    if (!mask_all.empty())
+        HIP_TRY(hipMemcpy(B->masks, mask_all.data(), mask_all.size() * sizeof(double2), hipMemcpyHostToDevice));
+    if (F) HIP_TRY(hipMemcpy(B->fac_crow, crow.data(), F * 4, hipMemcpyHostToDevice));
+    /* messages zero until init, old beliefs 0.0, every variant fresh */
+    HIP_TRY(hipMemset(B->m_fv, 0, std::max<uint64_t>(M, 1) * 16));
+    HIP_TRY(hipMemset(B->belief, 0, std::max<uint64_t>(S, 1) * 8));
+    HIP_TRY(hipMemset(B->vprod, 0, std::max<uint64_t>(S, 1) * 16));
+    HIP_TRY(hipMemset(B->vzc, 0, std::max<uint64_t>(S, 1)));
+    HIP_TRY(hipMemset(B->vfresh, 1, std::max<uint64_t>(NV, 1)));
+    HIP_TRY(hipMemset(B->iters_dev, 0, std::max<uint64_t>(NV, 1) * 8));
+    HIP_TRY(hipMemset(B->diff, 0, std::max<uint64_t>(NV, 1) * kBatchSlots * kCtlPad * 8));
+    B->iters.assign(NV, 0);
+    B->last_diff.assign(NV, INFINITY);
+    TRY(B->build_tiles());
+    *out = reinterpret_cast<flbp_batch *>(hold.release());
+    return FLBP_OK;
 }
 
 int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *csrs, int device, const flbp_batch_opts *opts,
                       flbp_batch **out) {
     if (!csrs || !out || n_graphs == 0) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: bad arguments");
-    if (n_graphs >= (1ull << 31)) return flbp::fail(FLBP_ERANGE, "flbp_batch_create: too many graphs");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-        return flbp::fail(FLBP_ENODEV, "flbp_batch_create: no HIP device");
-    if (device < 0 || device >= ndev) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: bad device");
+    std::vector<flbp_batch_group> gs(n_graphs);
     for (uint64_t g = 0; g < n_graphs; g++) {
         if (!csrs[g]) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: null graph");
-        const flbp_csr &c = *csrs[g];
-        if (c.n_vars >= (1ull << 32) - 1 || c.n_factors >= (1ull << 32) - 1 || c.n_edges >= (1ull << 32) - 1)
-            return flbp::fail(FLBP_ERANGE, "flbp_batch_create: a graph has V, F or E >= 2^32-1");
-        int rc = flbp::validate_csr(c);
-        if (rc) return rc;
+        gs[g] = flbp_batch_group{csrs[g], 1u, nullptr, nullptr, nullptr};
     }
-    HIP_TRY(hipSetDevice(device));
-    auto *B = new (std::nothrow) Batch();
-    if (!B) return flbp::fail(FLBP_ENOMEM, "flbp_batch_create: out of memory");
-    std::unique_ptr<Batch> hold(B);
-    B->device = device;
-    B->G = n_graphs;
-    if (opts) B->opts = *opts;
-    else flbp_batch_default_opts(&B->opts);
-    if (B->opts.layout > FLBP_LAYOUT_SCATTER) return flbp::fail(FLBP_EINVAL, "flbp_batch_create: bad layout");
-    if (B->opts.tile_factors < 64 || B->opts.tile_factors > flbp::kStageSlots || B->opts.tile_vars < 64 ||
-        B->opts.tile_vars > flbp::kStageSlots)
-        return flbp::fail(FLBP_EINVAL, "flbp_batch_create: tile bounds must be 64.." +
-                                           std::to_string(flbp::kStageSlots) + " slots");
-    if (B->opts.chunk == 0 || B->opts.chunk > (uint32_t)flbp::kMaxChunk) B->opts.chunk = flbp::kMaxChunk;
-    if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(device);
-    B->h_graphs.resize(n_graphs);
-    uint64_t F = 0, V = 0, E = 0;
-    for (uint64_t g = 0; g < n_graphs; g++) {
-        const flbp_csr &c = *csrs[g];
-        B->h_graphs[g] = {F, V, E, F + g, V + g, (uint32_t)c.n_factors, (uint32_t)c.n_vars};
-        for (uint64_t I = 0; I < c.n_factors; I++)
-            B->max_fac_deg = std::max<uint64_t>(B->max_fac_deg, c.fac_off[I + 1] - c.fac_off[I]);
-        F += c.n_factors;
-        V += c.n_vars;
-        E += c.n_edges;
-    }
-    B->Ftot = F;
-    B->Vtot = V;
-    B->Etot = E;
-    HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    TRY(B->alloc(B->graphs, n_graphs));
-    TRY(B->alloc(B->fac_off, F + n_graphs));
-    TRY(B->alloc(B->fac_meta, F));
-    TRY(B->alloc(B->fac_p, F));
-    TRY(B->alloc(B->fac_q, F));
-    TRY(B->alloc(B->fac_mask, F));
-    TRY(B->alloc(B->var_off, V + n_graphs));
-    TRY(B->alloc(B->var_edge, E));
-    TRY(B->alloc(B->fac_vpos, E));
-    TRY(B->alloc(B->m_vf, E));
-    TRY(B->alloc(B->m_fv, E));
-    TRY(B->alloc(B->m_fv_vm, E));
-    TRY(B->alloc(B->fac_work, E));
-    TRY(B->alloc(B->var_work, V));
-    TRY(B->alloc(B->belief, V));
-    B->h_fac_off.resize(F + n_graphs);
-    B->h_fac_meta.resize(F);
-    B->h_var_off.resize(V + n_graphs);
-    TRY(B->alloc(B->diff, (uint64_t)flbp::kMaxChunk * n_graphs * kBatchSlots * kCtlPad));
-    TRY(B->alloc(B->diff_out, (uint64_t)flbp::kMaxChunk * n_graphs));
-    TRY(B->alloc(B->start_active, n_graphs));
-    TRY(B->alloc(B->budget, n_graphs));
-    HIP_TRY(hipHostMalloc((void **)&B->h_diff, sizeof(unsigned long long) * flbp::kMaxChunk * n_graphs, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&B->h_active, n_graphs, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&B->h_budget, sizeof(uint32_t) * n_graphs, hipHostMallocDefault));
-    HIP_TRY(hipMemcpy(B->graphs, B->h_graphs.data(), n_graphs * sizeof(BatchGraph), hipMemcpyHostToDevice));
-    /* upload graph by graph through one staging vector */
-    std::vector<uint32_t> u32;
-    std::vector<double2> mk;
-    for (uint64_t g = 0; g < n_graphs; g++) {
-        const flbp_csr &c = *csrs[g];
-        const BatchGraph &bg = B->h_graphs[g];
-        u32.resize(std::max(std::max(c.n_factors, c.n_vars), c.n_edges) + 1);
-        for (uint64_t I = 0; I <= c.n_factors; I++) u32[I] = (uint32_t)c.fac_off[I];
-        std::copy(u32.begin(), u32.begin() + c.n_factors + 1, B->h_fac_off.begin() + bg.fac_off_base);
-        HIP_TRY(hipMemcpy(B->fac_off + bg.fac_off_base, u32.data(), (c.n_factors + 1) * 4, hipMemcpyHostToDevice));
-        for (uint64_t I = 0; I < c.n_factors; I++)
-            u32[I] = flbp::dev::pack_meta((uint32_t)c.fac_type[I], c.head_clamped[I] != 0,
-                                          (uint32_t)(c.head_slot[I] - c.fac_off[I]));
-        std::copy(u32.begin(), u32.begin() + c.n_factors, B->h_fac_meta.begin() + bg.fac_base);
-        if (c.n_factors) {
-            HIP_TRY(hipMemcpy(B->fac_meta + bg.fac_base, u32.data(), c.n_factors * 4, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(B->fac_p + bg.fac_base, c.p, c.n_factors * 8, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(B->fac_q + bg.fac_base, c.q, c.n_factors * 8, hipMemcpyHostToDevice));
-            mk.resize(c.n_factors);
-            for (uint64_t I = 0; I < c.n_factors; I++) mk[I] = make_double2(c.mask0[I], c.mask1[I]);
-            HIP_TRY(hipMemcpy(B->fac_mask + bg.fac_base, mk.data(), c.n_factors * 16, hipMemcpyHostToDevice));
-        }
-        std::vector<uint32_t> perm, dev_off, dev_edge, vpos;
-        flbp::var_device_order(c, flbp::kOrderLabel, perm, dev_off, dev_edge, vpos);
-        B->h_perm.insert(B->h_perm.end(), perm.begin(), perm.end());
-        std::copy(dev_off.begin(), dev_off.end(), B->h_var_off.begin() + bg.var_off_base);
-        HIP_TRY(hipMemcpy(B->var_off + bg.var_off_base, dev_off.data(), (c.n_vars + 1) * 4, hipMemcpyHostToDevice));
-        if (c.n_edges) {
-            HIP_TRY(hipMemcpy(B->var_edge + bg.edge_base, dev_edge.data(), c.n_edges * 4, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(B->fac_vpos + bg.edge_base, vpos.data(), c.n_edges * 4, hipMemcpyHostToDevice));
-        }
-    }
-    HIP_TRY(hipMemset(B->belief, 0, std::max<uint64_t>(V, 1) * 8));
-    HIP_TRY(hipMemset(B->m_vf, 0, std::max<uint64_t>(E, 1) * 16));
-    HIP_TRY(hipMemset(B->m_fv, 0, std::max<uint64_t>(E, 1) * 16));
-    HIP_TRY(hipMemset(B->m_fv_vm, 0, std::max<uint64_t>(E, 1) * 16));
-    B->iters.assign(n_graphs, 0);
-    B->last_diff.assign(n_graphs, INFINITY);
-    TRY(B->build_tiles());
-    B->layout = B->resolve_layout();
-    *out = reinterpret_cast<flbp_batch *>(hold.release());
-    return FLBP_OK;
+    return flbp_batch_create_groups(n_graphs, gs.data(), device, opts, out);
 }
 
 void flbp_batch_destroy(flbp_batch *b) { delete reinterpret_cast<Batch *>(b); }
@@ -507,15 +644,16 @@ int flbp_batch_init(flbp_batch *b, int logdomain) {
     if (!B) return flbp::fail(FLBP_EINVAL, "flbp_batch_init: null batch");
     HIP_TRY(hipSetDevice(B->device));
     const double c = logdomain ? 0.0 : 1.0;
-    if (B->Etot) {
-        const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(B->Etot, 256), 65536);
-        hipLaunchKernelGGL(bfill2, dim3(grid), dim3(256), 0, B->stream, B->m_vf, B->Etot, c);
-        hipLaunchKernelGGL(bfill2, dim3(grid), dim3(256), 0, B->stream, B->m_fv, B->Etot, c);
-        hipLaunchKernelGGL(bfill2, dim3(grid), dim3(256), 0, B->stream, B->m_fv_vm, B->Etot, c);
+    if (B->Mtot) {
+        const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(B->Mtot, 256), 65536);
+        hipLaunchKernelGGL(bfill_pairs, dim3(grid), dim3(256), 0, B->stream, B->m_fv, B->Mtot, c);
     }
+    hipLaunchKernelGGL(bfill_u8, dim3((unsigned)std::min<uint64_t>(grid_for(B->NV, 256), 65536)), dim3(256), 0,
+                       B->stream, B->vfresh, B->NV, (uint8_t)1);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(B->stream));
-    B->fm_stale = false;
+    B->init_c = c;
+    B->drop_exec();   /* the captured kernels carry the init value */
     return FLBP_OK;
 }
 
@@ -528,99 +666,164 @@ int flbp_batch_run(flbp_batch *b, const flbp_run_opts *o, uint64_t *iters_out, d
     HIP_TRY(hipSetDevice(B->device));
     const auto t0 = std::chrono::steady_clock::now();
     auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
-    const uint64_t G = B->G;
-    std::vector<uint8_t> active(G);
-    std::vector<double> md(G, INFINITY);
-    uint64_t n_active = 0, sweeps = 0;
-    for (uint64_t g = 0; g < G; g++) {
-        active[g] = B->iters[g] < o->maxiter;
-        n_active += active[g];
-    }
-    const int chunk = (int)std::min<uint32_t>(o->chunk ? o->chunk : B->opts.chunk, flbp::kMaxChunk);
-    TRY(B->prepare_damping(o->damping));
-    while (n_active && elapsed() < o->maxtime) {
-        for (uint64_t g = 0; g < G; g++) {
-            B->h_active[g] = active[g];
-            const uint64_t left = o->maxiter > B->iters[g] ? o->maxiter - B->iters[g] : 0;
-            B->h_budget[g] = (uint32_t)std::min<uint64_t>(left, 0xffffffffu);
-        }
-        TRY(B->set_active(B->h_active));
-        HIP_TRY(hipMemcpyAsync(B->start_active, B->h_active, G, hipMemcpyHostToDevice, B->stream));
-        HIP_TRY(hipMemcpyAsync(B->budget, B->h_budget, G * 4, hipMemcpyHostToDevice, B->stream));
-        TRY(B->enqueue_chunk(chunk, o->tol, o->damping));
-        HIP_TRY(hipMemcpyAsync(B->h_diff, B->diff_out, sizeof(unsigned long long) * chunk * G, hipMemcpyDeviceToHost,
-                               B->stream));
-        HIP_TRY(hipStreamSynchronize(B->stream));
-        n_active = 0;
-        uint64_t most = 0;
-        for (uint64_t g = 0; g < G; g++) {
-            if (!active[g]) continue;
-            const uint64_t cap = std::min<uint64_t>((uint64_t)chunk, B->h_budget[g]);
-            uint64_t ran = cap;
-            bool stop = false;
-            for (uint64_t k = 0; k < cap; k++) {
-                double dk;
-                std::memcpy(&dk, &B->h_diff[k * G + g], sizeof dk);
-                md[g] = dk;
-                if (!(dk > o->tol)) {
-                    ran = k + 1;
-                    stop = true;
+    const uint64_t NV = B->NV;
+    uint64_t most = 0;
+    for (uint64_t v = 0; v < NV; v++) most = std::max<uint64_t>(most, o->maxiter > B->iters[v] ? o->maxiter - B->iters[v] : 0);
+    most = std::min<uint64_t>(most, 0xffffffffu);
+    if (most) {
+        TRY(B->start_run(o->maxiter, o->tol));
+        /* chunk 0: the whole solve as one replay; else replays of `chunk`
+         * sweeps, enqueued back to back (at most two ahead of the host,
+         * which stops enqueuing once no group runs or maxtime passed) */
+        const uint32_t chunk = o->chunk ? o->chunk : B->opts.chunk;
+        if (!chunk || chunk >= most) {
+            TRY(B->launch_solve((uint32_t)most, o->tol, o->damping));
+        } else {
+            const uint64_t n_launch = (most + chunk - 1) / chunk;
+            std::vector<hipEvent_t> ev;
+            int rc = FLBP_OK;
+            for (uint64_t i = 0; i < n_launch && !rc; i++) {
+                if (i >= 2) {   /* launch i-2 has finished: stop once nothing runs */
+                    if (hipEventSynchronize(ev[i - 2]) != hipSuccess) {
+                        rc = hip_fail(hipGetLastError(), "hipEventSynchronize");
+                        break;
+                    }
+                    if (B->h_actn[(i - 2) % 64] == 0 || elapsed() >= o->maxtime) break;
+                }
+                const uint32_t n = (uint32_t)std::min<uint64_t>(chunk, most - i * chunk);
+                rc = B->launch_solve(n, o->tol, o->damping);
+                if (rc) break;
+                hipEvent_t e = nullptr;
+                if (hipMemcpyAsync(&B->h_actn[i % 64], B->act_n, 4, hipMemcpyDeviceToHost, B->stream) != hipSuccess ||
+                    hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
+                    hipEventRecord(e, B->stream) != hipSuccess) {
+                    rc = hip_fail(hipGetLastError(), "batch: chunk bookkeeping");
+                    if (e) (void)hipEventDestroy(e);
                     break;
                 }
+                ev.push_back(e);
             }
-            B->iters[g] += ran;
-            most = std::max(most, ran);
-            if (stop || B->iters[g] >= o->maxiter) active[g] = 0;
-            n_active += active[g];
+            (void)hipStreamSynchronize(B->stream);
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+            if (rc) return rc;
         }
-        sweeps += most;
+        HIP_TRY(hipMemcpyAsync(B->h_iters, B->iters_dev, NV * 8, hipMemcpyDeviceToHost, B->stream));
+        HIP_TRY(hipMemcpyAsync(B->h_last, B->last_dev, NV * 8, hipMemcpyDeviceToHost, B->stream));
+        HIP_TRY(hipStreamSynchronize(B->stream));
     }
-    for (uint64_t g = 0; g < G; g++) {
-        B->last_diff[g] = md[g];
-        if (iters_out) iters_out[g] = B->iters[g];
-        if (maxdiff_out) maxdiff_out[g] = md[g];
+    uint64_t sweeps = 0;
+    double md_max = 0;
+    for (uint64_t v = 0; v < NV; v++) {
+        const uint64_t before = B->iters[v];
+        if (most) {
+            B->iters[v] = B->h_iters[v];
+            B->last_diff[v] = B->h_last[v];
+        } else {
+            B->last_diff[v] = INFINITY;
+        }
+        sweeps = std::max<uint64_t>(sweeps, B->iters[v] - before);
+        if (iters_out) iters_out[v] = B->iters[v];
+        if (maxdiff_out) maxdiff_out[v] = B->last_diff[v];
+        md_max = std::max(md_max, B->last_diff[v]);
     }
     if (total) {
         total->iters = sweeps;
         total->sweeps = sweeps;
-        total->maxdiff = 0;
-        for (uint64_t g = 0; g < G; g++) total->maxdiff = std::max(total->maxdiff, md[g]);
+        total->maxdiff = md_max;
         total->seconds = elapsed();
     }
     return FLBP_OK;
 }
 
-int flbp_batch_beliefs(flbp_batch *b, uint64_t g, double *out, uint64_t n) {
+static int variant_slot(const Batch *B, uint64_t v, const flbp::GroupHost *&gh, uint32_t &k) {
+    if (v >= B->NV) return flbp::fail(FLBP_EINVAL, "batch: variant index out of range");
+    gh = &B->hg[B->var_group[v]];
+    k = (uint32_t)(v - gh->d.vbase);
+    return FLBP_OK;
+}
+
+int flbp_batch_beliefs(flbp_batch *b, uint64_t v, double *out, uint64_t n) {
     Batch *B = reinterpret_cast<Batch *>(b);
-    if (!B || (n && !out) || g >= (B ? B->G : 0)) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs: bad argument");
-    if (n > B->h_graphs[g].n_var) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs: n > n_vars of the graph");
+    if (!B || (n && !out)) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs: bad argument");
+    const flbp::GroupHost *gh = nullptr;
+    uint32_t k = 0;
+    TRY(variant_slot(B, v, gh, k));
+    const GroupDev &d = gh->d;
+    if (n > d.n_var) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs: n > n_vars of the graph");
     HIP_TRY(hipSetDevice(B->device));
-    const BatchGraph &bg = B->h_graphs[g];
-    std::vector<double> dev(bg.n_var);
-    if (bg.n_var)
-        HIP_TRY(hipMemcpyAsync(dev.data(), B->belief + bg.var_base, bg.n_var * 8, hipMemcpyDeviceToHost, B->stream));
+    std::vector<double> dev((uint64_t)d.n_var << d.lk);
+    if (d.n_var)
+        HIP_TRY(hipMemcpyAsync(dev.data(), B->belief + d.vst_base, dev.size() * 8, hipMemcpyDeviceToHost, B->stream));
     HIP_TRY(hipStreamSynchronize(B->stream));
-    for (uint64_t dv = 0; dv < bg.n_var; dv++) {
-        const uint32_t v = B->h_perm[bg.var_base + dv];
-        if (v < n) out[v] = dev[dv];
+    for (uint64_t dv = 0; dv < d.n_var; dv++) {
+        const uint32_t x = B->h_perm[gh->perm_base + dv];
+        if (x < n) out[x] = dev[(dv << d.lk) + k];
     }
+    return FLBP_OK;
+}
+
+int flbp_batch_beliefs_at(flbp_batch *b, uint64_t v, uint64_t n, const uint64_t *vars, double *out) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B || (n && (!vars || !out))) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs_at: bad argument");
+    const flbp::GroupHost *gh = nullptr;
+    uint32_t k = 0;
+    TRY(variant_slot(B, v, gh, k));
+    if (!n) return FLBP_OK;
+    const GroupDev &d = gh->d;
+    std::vector<uint32_t> inv;   /* variable -> device variable, for the asked ones */
+    std::vector<uint64_t> idx(n);
+    {
+        std::vector<std::pair<uint64_t, uint64_t>> want(n);
+        for (uint64_t i = 0; i < n; i++) {
+            if (vars[i] >= d.n_var) return flbp::fail(FLBP_EINVAL, "flbp_batch_beliefs_at: variable out of range");
+            want[i] = {vars[i], i};
+        }
+        std::sort(want.begin(), want.end());
+        std::vector<uint32_t> pos(d.n_var);
+        for (uint64_t dv = 0; dv < d.n_var; dv++) pos[B->h_perm[gh->perm_base + dv]] = (uint32_t)dv;
+        for (uint64_t i = 0; i < n; i++) idx[want[i].second] = d.vst_base + ((uint64_t)pos[want[i].first] << d.lk) + k;
+    }
+    HIP_TRY(hipSetDevice(B->device));
+    uint64_t *didx = nullptr;
+    double *dout = nullptr;
+    if (hipMalloc(&didx, n * 8) != hipSuccess || hipMalloc(&dout, n * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        if (didx) (void)hipFree(didx);
+        return flbp::fail(FLBP_ENOMEM, "flbp_batch_beliefs_at: staging allocation failed");
+    }
+    int rc = FLBP_OK;
+    if (hipMemcpyAsync(didx, idx.data(), n * 8, hipMemcpyHostToDevice, B->stream) != hipSuccess) rc = FLBP_EHIP;
+    if (!rc) {
+        hipLaunchKernelGGL(bgather_f64, dim3(grid_for(n, 256)), dim3(256), 0, B->stream, B->belief, didx, dout, n);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, B->stream) != hipSuccess ||
+            hipStreamSynchronize(B->stream) != hipSuccess)
+            rc = FLBP_EHIP;
+    }
+    (void)hipFree(didx);
+    (void)hipFree(dout);
+    return rc ? flbp::fail(FLBP_EHIP, "flbp_batch_beliefs_at: device gather failed") : FLBP_OK;
+}
+
+int flbp_batch_set_p(flbp_batch *b, uint64_t group, const double *p, uint64_t n) {
+    Batch *B = reinterpret_cast<Batch *>(b);
+    if (!B || (n && !p) || group >= (B ? B->G : 0)) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_p: bad argument");
+    const GroupDev &d = B->hg[group].d;
+    if (n != d.n_fac) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_p: n must equal the group's n_factors");
+    HIP_TRY(hipSetDevice(B->device));
+    if (n) HIP_TRY(hipMemcpy(B->fac_p + d.fac_base, p, n * 8, hipMemcpyHostToDevice));
     return FLBP_OK;
 }
 
 int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *o) {
     Batch *B = reinterpret_cast<Batch *>(b);
     if (!B || !o) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: null argument");
+    if (int rc = check_opts(*o, "flbp_batch_set_opts")) return rc;
     HIP_TRY(hipSetDevice(B->device));
-    if (o->layout > FLBP_LAYOUT_SCATTER) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: bad layout");
-    if (o->tile_factors < 64 || o->tile_factors > flbp::kStageSlots || o->tile_vars < 64 ||
-        o->tile_vars > flbp::kStageSlots)
-        return flbp::fail(FLBP_EINVAL, "flbp_batch_set_opts: tile bounds must be 64.." +
-                                           std::to_string(flbp::kStageSlots) + " slots");
     B->opts = *o;
-    if (B->opts.chunk == 0 || B->opts.chunk > (uint32_t)flbp::kMaxChunk) B->opts.chunk = flbp::kMaxChunk;
     if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(B->device);
-    TRY(B->build_tiles());
-    return B->set_layout(B->resolve_layout());
+    B->drop_exec();
+    return B->build_tiles();
 }
 
 int flbp_batch_get_opts(flbp_batch *b, flbp_batch_opts *o) {
@@ -634,8 +837,10 @@ int flbp_batch_reset_counters(flbp_batch *b) {
     Batch *B = reinterpret_cast<Batch *>(b);
     if (!B) return flbp::fail(FLBP_EINVAL, "flbp_batch_reset_counters: null batch");
     std::fill(B->iters.begin(), B->iters.end(), 0);
+    std::fill(B->last_diff.begin(), B->last_diff.end(), INFINITY);
     HIP_TRY(hipSetDevice(B->device));
-    HIP_TRY(hipMemsetAsync(B->belief, 0, std::max<uint64_t>(B->Vtot, 1) * 8, B->stream));
+    HIP_TRY(hipMemsetAsync(B->belief, 0, std::max<uint64_t>(B->Stot, 1) * 8, B->stream));
+    HIP_TRY(hipMemsetAsync(B->iters_dev, 0, std::max<uint64_t>(B->NV, 1) * 8, B->stream));
     HIP_TRY(hipStreamSynchronize(B->stream));
     return FLBP_OK;
 }
@@ -643,45 +848,36 @@ int flbp_batch_reset_counters(flbp_batch *b) {
 int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps, flbp_batch_opts *best) {
     Batch *B = reinterpret_cast<Batch *>(b);
     if (!B || trial_sweeps == 0) return flbp::fail(FLBP_EINVAL, "flbp_batch_autotune: bad argument");
+    if (!(damping >= 0.0 && damping < 1.0)) return flbp::fail(FLBP_EINVAL, "flbp_batch_autotune: bad damping");
     HIP_TRY(hipSetDevice(B->device));
     const flbp_batch_opts base = B->opts;
     const uint32_t g0 = (uint32_t)default_grid(B->device);
     const uint32_t tiles[] = {flbp::kStageSlots / 2, flbp::kStageSlots};
     const uint32_t grids[] = {g0, 2 * g0};
-    const uint32_t layouts[] = {FLBP_LAYOUT_PULL, FLBP_LAYOUT_SCATTER};
-    const int n = (int)std::min<uint32_t>(trial_sweeps, flbp::kMaxChunk);
     double best_t = INFINITY;
     flbp_batch_opts best_o = base;
-    for (uint32_t lay : layouts)
-    for (uint32_t tf : tiles) {
+    for (uint32_t tf : tiles)
         for (uint32_t gb : grids) {
             flbp_batch_opts o = base;
             o.tile_factors = tf;
             o.tile_vars = tf;
             o.grid_blocks = gb;
-            o.layout = lay;
             TRY(flbp_batch_set_opts(b, &o));
+            TRY(flbp_batch_reset_counters(b));
             TRY(flbp_batch_init(b, 0));
-            TRY(B->prepare_damping(damping));
-            for (uint64_t g = 0; g < B->G; g++) {
-                B->h_active[g] = 1;
-                B->h_budget[g] = (uint32_t)n;
-            }
-            TRY(B->set_active(B->h_active));
-            HIP_TRY(hipMemcpyAsync(B->start_active, B->h_active, B->G, hipMemcpyHostToDevice, B->stream));
-            HIP_TRY(hipMemcpyAsync(B->budget, B->h_budget, B->G * 4, hipMemcpyHostToDevice, B->stream));
-            TRY(B->enqueue_chunk(n, -INFINITY, damping));   /* warm: captures the graph */
-            HIP_TRY(hipStreamSynchronize(B->stream));
+            flbp_run_opts ro{-INFINITY, trial_sweeps, INFINITY, damping, 0};
+            TRY(flbp_batch_run(b, &ro, nullptr, nullptr, nullptr));   /* warm: captures the solve */
+            TRY(flbp_batch_reset_counters(b));
+            TRY(flbp_batch_init(b, 0));
             const auto t0 = std::chrono::steady_clock::now();
-            TRY(B->enqueue_chunk(n, -INFINITY, damping));
-            HIP_TRY(hipStreamSynchronize(B->stream));
+            flbp_run_result r;
+            TRY(flbp_batch_run(b, &ro, nullptr, nullptr, &r));
             const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (t < best_t) {
                 best_t = t;
                 best_o = o;
             }
         }
-    }
     TRY(flbp_batch_set_opts(b, &best_o));
     TRY(flbp_batch_reset_counters(b));
     TRY(flbp_batch_init(b, 0));
@@ -692,6 +888,11 @@ int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps, fl
 uint64_t flbp_batch_device_bytes(const flbp_batch *b) {
     const Batch *B = reinterpret_cast<const Batch *>(b);
     return B ? B->bytes : 0;
+}
+
+uint64_t flbp_batch_n_variants(const flbp_batch *b) {
+    const Batch *B = reinterpret_cast<const Batch *>(b);
+    return B ? B->NV : 0;
 }
 
 }  // extern "C"

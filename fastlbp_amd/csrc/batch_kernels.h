@@ -1,23 +1,37 @@
 /*
- * batch_kernels.h -- one PARALL sweep over many graphs at once.
+ * batch_kernels.h -- PARALL sweeps over many graphs at once.
  *
- * Graphs are concatenated: graph g's factors start at fac_base[g], its
- * variables at var_base[g] and its edge slots at edge_base[g] (64-bit
- * bases), and all indices stored inside a graph's arrays are graph-local
- * 32-bit values.  Work is cut into tiles of one graph each -- the staged
- * blocks of the plain solve (at most kBlock items and kStage edge slots,
- * with their work orders), in the pull or the scatter message layout of
- * staged_kernels.h -- and a persistent grid walks a tile list with a
- * stride, so a launch uses exactly the blocks that fit on the chip
- * whatever the batch holds.  The sweep kernels walk the list of the graphs
- * still running (its length read from device memory, so a captured chunk
- * stays valid when the host shortens the list between replays).
+ * A batch holds groups.  A group is one graph structure (a CSR in the
+ * reference's orders) solved for K >= 1 variants: independent graphs are
+ * groups of one variant; clamped feedback variants of one graph whose
+ * clamps keep its structure (head clamps of variables without body uses,
+ * causal_factor.cpp:75-83) share one group.  Structure arrays (offsets,
+ * factor meta, p, q, slot maps, work orders) are stored once per group;
+ * per-variant state is interleaved: f->v message of slot s in variant k at
+ * msg_base + s*K + k, product pair / zero-count code / belief of device
+ * variable v in variant k at vst_base + v*K + k.  A tile of a K-variant
+ * group covers at most kStage/K slots for all K variants, so one index load
+ * serves K lanes and one gathered line carries K messages (K = 8: a whole
+ * 128-byte line of f->v messages per gathered slot).
  *
- * Each graph keeps its own stop test: sweep k of a chunk runs on graph g
- * only if g was active when the chunk started, k is below its remaining
- * iteration budget, and its max belief change of sweep k-1 was > tol.  A
- * graph therefore stops after exactly the sweeps a plain solve of it would
- * run, and its messages are bit-identical to that solve.
+ * Messages use the state layout of state_kernels.h: the factor pass derives
+ * each v->f input from the variable's product pair and zero-count code and
+ * the slot's old f->v message, the variable pass gathers f->v messages and
+ * writes per-variable products.  Arithmetic and operation order are the
+ * plain solve's, so every variant's results are bit-identical to a plain
+ * solve of its graph.
+ *
+ * Stop test, per variant and entirely on the device: vrun[v] says whether
+ * variant v runs the next sweep.  After each sweep, batch_stop reads each
+ * running variant's max belief change, advances its counter, clears the
+ * change slots and sets vrun for the next sweep (max change > tol and
+ * iteration budget left), then compacts the list of groups with a running
+ * variant and the prefix sums of their tile counts.  The sweep kernels walk
+ * only those groups' tiles, and lanes of a variant that stopped store
+ * nothing, so a variant stops after exactly the sweeps its plain solve
+ * runs.  A whole solve -- every sweep up to the iteration cap -- is one
+ * captured hipGraph; the sweeps after the last variant stopped find no
+ * active group and exit at once.
  */
 #ifndef FLBP_BATCH_KERNELS_H
 #define FLBP_BATCH_KERNELS_H
@@ -25,152 +39,515 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "plan_internal.h"
 #include "staged_kernels.h"
+#include "state_kernels.h"
 #include "sweep_kernels.h"
 
 namespace {
 
 using namespace flbp::dev;
 
-struct BatchGraph {
-    uint64_t fac_base, var_base, edge_base;   /* element offsets of graph g */
-    uint64_t fac_off_base, var_off_base;      /* offsets into fac_off / var_off (F_g+1, V_g+1 entries) */
+struct GroupDev {
+    uint64_t fac_base, var_base, edge_base;   /* structure element offsets */
+    uint64_t fac_off_base, var_off_base;      /* into fac_off (F+1 entries per group) / var_off (V+1) */
+    uint64_t msg_base;                        /* K-interleaved f->v messages */
+    uint64_t vst_base;                        /* K-interleaved products, codes, beliefs */
     uint32_t n_fac, n_var;
+    uint32_t K, lk;                           /* variants, log2 K */
+    uint32_t vbase;                           /* batch index of variant 0 */
+    uint32_t ftile_first, ftile_count, vtile_first, vtile_count;
+    uint32_t pad;
 };
 
 struct Tile {
-    uint32_t graph;
+    uint32_t group;
     uint32_t first;    /* first local factor (factor tiles) or variable (var tiles) */
     uint32_t count;
+    uint32_t aux;      /* wide factor tiles: offset of the factor's scratch range */
 };
 
-/* per graph and sweep, kBatchSlots partial maxima one 64-byte line apart
- * (tile i of a graph folds into slot i mod kBatchSlots) */
+/* per variant, kBatchSlots partial maxima of the sweep's belief change one
+ * 64-byte line apart (tile i of a group folds into slot i mod kBatchSlots) */
 constexpr int kBatchSlots = 8;
 
-struct BatchCtl {
-    unsigned long long *diff;       /* [chunk][G][kBatchSlots][kCtlPad] partial max |db| bits */
-    const uint8_t *start_active;    /* [G] */
-    const uint32_t *budget;         /* [G] sweeps left before maxiter */
-    uint32_t G;
+struct BatchArgs {
+    const GroupDev *__restrict__ groups;
+    const Tile *__restrict__ ftiles, *__restrict__ vtiles;
+    const uint32_t *__restrict__ act_n;       /* groups with a running variant */
+    const uint32_t *__restrict__ act_group;   /* [G] their indices */
+    const uint32_t *__restrict__ act_fpre;    /* [G+1] prefix sums of their factor tile counts */
+    const uint32_t *__restrict__ act_vpre;    /* [G+1] ... of their variable tile counts */
+    const uint32_t *__restrict__ fac_off, *__restrict__ fac_meta, *__restrict__ fac_var, *__restrict__ fac_work;
+    const uint32_t *__restrict__ fac_crow;    /* clamped factor: first of its K masks */
+    const double *__restrict__ fac_p, *__restrict__ fac_q;
+    const double2 *__restrict__ masks;        /* per clamped factor of a group, K head masks */
+    const uint32_t *__restrict__ var_off, *__restrict__ var_edge;
+    const uint16_t *__restrict__ var_work;
+    double2 *m_fv;
+    double2 *__restrict__ vprod;
+    uint8_t *__restrict__ vzc;
+    double *__restrict__ belief;
+    double2 *__restrict__ scratch;            /* wide factors' v->f inputs */
+    const uint8_t *__restrict__ vrun;         /* [n variants] */
+    const uint8_t *__restrict__ vfresh;       /* [n variants] 1: no sweep since init */
+    unsigned long long *__restrict__ diff;    /* [n variants][kBatchSlots][kCtlPad] */
+    double c;                                 /* init value: the v->f input of a fresh variant */
+};
+
+/* the active tile ti of the sweep: group g and its tile */
+__device__ __forceinline__ void find_tile(const uint32_t *__restrict__ pre, uint32_t n_act, uint32_t ti,
+                                          uint32_t &j) {
+    uint32_t lo = 0, hi = n_act;   /* pre[lo] <= ti < pre[hi] */
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= ti) lo = mid;
+        else hi = mid;
+    }
+    j = lo;
+}
+
+/* Messages of one factor from a global array with stride K. */
+struct StridedIn {
+    const double2 *__restrict__ m;
+    uint32_t lk;
+    __device__ __forceinline__ double2 get(int j) const { return m[(uint64_t)j << lk]; }
+    __device__ __forceinline__ double2 pick(int j) const { return m[(uint64_t)j << lk]; }
+};
+
+/* One factor tile of group gr: factors [f0, f1), all K variants. */
+template <bool kDamp, int N>
+__device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const GroupDev &gr, const Tile &tile,
+                                                  double damp, double keep, FactorSmem &sm) {
+    const uint32_t *fac_off = A.fac_off + gr.fac_off_base;
+    const uint64_t fb = gr.fac_base, eb = gr.edge_base;
+    const uint32_t f0 = tile.first, f1 = tile.first + tile.count;
+    const uint32_t base = fac_off[f0];
+    const uint32_t n = fac_off[f1] - base;
+    const uint32_t lk = gr.lk, K = gr.K, km = K - 1;
+    const uint32_t nk = n << lk;
+    const int tid = threadIdx.x;
+    double2 *const fv = A.m_fv + gr.msg_base + ((uint64_t)base << lk);
+    if (nk > (uint32_t)kStage) {
+        /* one wide factor, each variant in turn, from global memory */
+        const uint32_t meta = A.fac_meta[fb + f0];
+        const uint32_t type = meta & 3u;
+        const bool clamped = (meta & 4u) != 0;
+        const int h = (int)(meta >> 8);
+        const double p = A.fac_p[fb + f0], q = type ? A.fac_q[fb + f0] : 0.0;
+        double2 *const scr = A.scratch + tile.aux;
+        for (uint32_t k = 0; k < K; k++) {
+            if (!A.vrun[gr.vbase + k]) continue;   /* uniform over the block */
+            double mask0 = 1.0, mask1 = 1.0;
+            if (clamped) {
+                const double2 mk = A.masks[(uint64_t)A.fac_crow[fb + f0] + k];
+                mask0 = mk.x;
+                mask1 = mk.y;
+            }
+            const bool fresh = A.vfresh[gr.vbase + k] != 0;
+            if (!fresh) {
+                for (uint32_t t = tid; t < n; t += kBlock) {
+                    const uint64_t v = gr.vst_base + ((uint64_t)A.fac_var[eb + base + t] << lk) + k;
+                    scr[t] = vf_of(A.vprod[v], A.vzc[v], fv[((uint64_t)t << lk) + k]);
+                }
+            }
+            __syncthreads();
+            const LdsIn in{scr};
+            const ConstIn cin{A.c};
+            const double2 mh = fresh ? cin.pick(h) : in.pick(h), mo = fresh ? cin.pick(h == 0 ? 1 : 0) : in.pick(h == 0 ? 1 : 0);
+            for (uint32_t t = tid; t < n; t += kBlock) {
+                double2 r = fresh ? fv_msg<0>(cin, type, clamped, (int)n, (int)t, h, mh, mo, p, q, mask0, mask1)
+                                  : fv_msg<0>(in, type, clamped, (int)n, (int)t, h, mh, mo, p, q, mask0, mask1);
+                double2 *dst = fv + ((uint64_t)t << lk) + k;
+                if (kDamp) {
+                    const double2 old = *dst;
+                    r.x = keep * r.x + damp * old.x;
+                    r.y = keep * r.y + damp * old.y;
+                }
+                *dst = r;
+            }
+            __syncthreads();   /* the scratch range is reused by the next variant */
+        }
+        return;
+    }
+    /* phase 1 */
+    const uint32_t nf = f1 - f0;
+    uint32_t w[kPerLane], ix[kPerLane];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        w[j] = i < nk ? A.fac_work[eb + base + (i >> lk)] : 0u;
+        ix[j] = i < nk ? A.fac_var[eb + base + (i >> lk)] : 0u;
+    }
+    if ((uint32_t)tid < nf) {
+        sm.start[tid] = fac_off[f0 + tid] - base;
+        sm.meta[tid] = A.fac_meta[fb + f0 + tid];
+        sm.p[tid] = A.fac_p[fb + f0 + tid];
+        sm.q[tid] = A.fac_q[fb + f0 + tid];
+    }
+    if (tid == 0) sm.start[nf] = n;
+    double2 old[kPerLane], pr[kPerLane];
+    uint32_t zc[kPerLane];
+    bool fresh[kPerLane];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < nk) {
+            const uint32_t k = i & km;
+            const uint64_t v = gr.vst_base + ((uint64_t)ix[j] << lk) + k;
+            old[j] = fv[i];
+            pr[j] = A.vprod[v];
+            zc[j] = A.vzc[v];
+            fresh[j] = A.vfresh[gr.vbase + k] != 0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < nk) {
+            sm.in[i] = fresh[j] ? make_double2(A.c, A.c) : vf_of(pr[j], zc[j], old[j]);
+            if (kDamp) sm.out[i] = old[j];
+        }
+    }
+    __syncthreads();
+    /* phase 2: position i of the tile's outputs is work entry i >> lk of
+     * variant i & (K - 1); messages of slot s, variant k at LDS s*K + k */
+#pragma unroll 1
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i >= nk) break;
+        uint32_t wj = 0;
+#pragma unroll
+        for (int x = 0; x < kPerLane; x++) wj |= (j == x) ? w[x] : 0u;
+        const uint32_t k = i & km;
+        const uint32_t sl = wj & 1023u, lf = wj >> 10;
+        const uint32_t s = sm.start[lf];
+        const int d = (int)(sm.start[lf + 1] - s);
+        const int t = (int)(sl - s);
+        const uint32_t meta = sm.meta[lf];
+        const uint32_t type = meta & 3u;
+        const bool clamped = (meta & 4u) != 0;
+        const int h = (int)(meta >> 8);
+        const double p = sm.p[lf];
+        const double q = type ? sm.q[lf] : 0.0;
+        double mask0 = 1.0, mask1 = 1.0;
+        if (clamped) {
+            const double2 mk = A.masks[(uint64_t)A.fac_crow[fb + f0 + lf] + k];
+            mask0 = mk.x;
+            mask1 = mk.y;
+        }
+        const double2 *fin = sm.in + (s << lk) + k;
+        double2 r;
+        if (d <= N) {
+            const bool head_target = (t == h);
+            const int skip_h = head_target ? -1 : h;
+            const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
+            double2 ops[N - 1];
+            int js = 0;
+#pragma unroll
+            for (int kk = 0; kk < N - 1; kk++) {
+                js += (js == t || js == skip_h);
+                js += (js == t || js == skip_h);
+                const double2 v = fin[(uint32_t)(kk < cnt ? js : 0) << lk];   /* index kept inside the factor */
+                ops[kk] = kk < cnt ? v : make_double2(0.0, 0.0);
+                js++;
+            }
+            r = fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, fin[(uint32_t)h << lk], p, q, mask0, mask1);
+        } else {
+            const StridedIn in{fin, lk};
+            const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
+            r = fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
+        }
+        const uint32_t o = (sl << lk) + k;
+        if (kDamp) {
+            const double2 old2 = sm.out[o];
+            r.x = keep * r.x + damp * old2.x;
+            r.y = keep * r.y + damp * old2.y;
+        }
+        sm.out[o] = r;
+    }
+    __syncthreads();
+    /* phase 3: the variants that run store their new messages */
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < nk && A.vrun[gr.vbase + (i & km)]) fv[i] = sm.out[i];
+    }
+}
+
+/* max over lanes with the same residue mod K (K a power of two <= 64):
+ * afterwards lane l holds the max over lanes l' of the wave with
+ * l' = l mod K */
+__device__ __forceinline__ unsigned long long wave_max_mod(unsigned long long v, uint32_t K) {
+#pragma unroll
+    for (uint32_t off = 32; off > 0; off >>= 1) {
+        if (off < K) break;
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void fold_diff(const BatchArgs &A, uint32_t variant, uint32_t ti, unsigned long long m) {
+    if (!m) return;
+    unsigned long long *slot = A.diff + ((uint64_t)variant * kBatchSlots + (ti % kBatchSlots)) * kCtlPad;
+    if (m > __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMax(slot, m);
+}
+
+/* One variable tile of group gr: variables [v0, v1), all K variants. */
+__device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDev &gr, const Tile &tile,
+                                               uint32_t ti, VarSmem &sm) {
+    const uint32_t *var_off = A.var_off + gr.var_off_base;
+    const uint64_t eb = gr.edge_base;
+    const uint32_t v0 = tile.first, v1 = tile.first + tile.count;
+    const uint32_t base = var_off[v0];
+    const uint32_t n = var_off[v1] - base;
+    const uint32_t lk = gr.lk, K = gr.K, km = K - 1;
+    const uint32_t nk = n << lk;
+    const int tid = threadIdx.x;
+    const double2 *fv = A.m_fv + gr.msg_base;
+    const uint64_t vst = gr.vst_base + ((uint64_t)v0 << lk);
+    if (nk > (uint32_t)kStage) {
+        /* one wide variable, each variant in turn: the product in edge
+         * order on lane 0 over kStage-slot chunks gathered by the block */
+        for (uint32_t k = 0; k < K; k++) {
+            if (!A.vrun[gr.vbase + k]) continue;   /* uniform over the block */
+            double prod_0 = 1.0, prod_1 = 1.0;
+            uint32_t z0 = 0, z1 = 0;
+            for (uint32_t c = 0; c < n; c += kStage) {
+                const uint32_t m = n - c < (uint32_t)kStage ? n - c : (uint32_t)kStage;
+                for (uint32_t e = tid; e < m; e += kBlock)
+                    sm.m[e] = fv[((uint64_t)A.var_edge[eb + base + c + e] << lk) + k];
+                __syncthreads();
+                if (tid == 0) {
+                    for (uint32_t e = 0; e < m; e++) {
+                        const double2 x = sm.m[e];
+                        if (x.x == 0) z0++; else prod_0 *= x.x;
+                        if (x.y == 0) z1++; else prod_1 *= x.y;
+                        scale_pair(prod_0, prod_1);
+                    }
+                }
+                __syncthreads();
+            }
+            if (tid == 0) {
+                const double b0 = prod_0 * (double)(z0 == 0);
+                const double b1 = prod_1 * (double)(z1 == 0);
+                const double b = b1 / (b0 + b1);
+                const double diff = fabs(A.belief[vst + k] - b);
+                A.belief[vst + k] = b;
+                A.vprod[vst + k] = make_double2(prod_0, prod_1);
+                A.vzc[vst + k] = pack_zc(z0, z1);
+                fold_diff(A, gr.vbase + k, ti, (unsigned long long)__double_as_longlong(diff));
+            }
+        }
+        return;
+    }
+    /* phase 1 */
+    const uint32_t nv = v1 - v0, nvk = nv << lk;
+    uint32_t ve[kPerLane];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        ve[j] = i < nk ? A.var_edge[eb + base + (i >> lk)] : 0u;
+    }
+    uint32_t lv = 0;
+    if ((uint32_t)tid < nvk) {
+        lv = A.var_work[gr.var_base + v0 + (tid >> lk)];
+        sm.bel[tid] = A.belief[vst + tid];
+    }
+    if ((uint32_t)tid < nv) sm.start[tid] = var_off[v0 + tid] - base;
+    if (tid == 0) sm.start[nv] = n;
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < nk) sm.m[i] = fv[((uint64_t)ve[j] << lk) + (i & km)];
+    }
+    __syncthreads();
+    /* phase 2: lane t is work entry t >> lk (a variable) of variant t & (K-1) */
+    unsigned long long diff_bits = 0;
+    if ((uint32_t)tid < nvk) {
+        const uint32_t k = tid & km;
+        const uint32_t s = sm.start[lv];
+        const int d = (int)(sm.start[lv + 1] - s);
+        double prod_0 = 1.0, prod_1 = 1.0;
+        uint32_t z0 = 0, z1 = 0;
+        for (int j = 0; j < d; j++) {
+            const double2 x = sm.m[((s + j) << lk) + k];
+            if (x.x == 0) z0++; else prod_0 *= x.x;
+            if (x.y == 0) z1++; else prod_1 *= x.y;
+            scale_pair(prod_0, prod_1);
+        }
+        const uint32_t o = (lv << lk) + k;
+        const double b0 = prod_0 * (double)(z0 == 0);
+        const double b1 = prod_1 * (double)(z1 == 0);
+        const double b = b1 / (b0 + b1);
+        const double diff = fabs(sm.bel[o] - b);
+        sm.bel[o] = b;
+        diff_bits = (unsigned long long)__double_as_longlong(diff);
+        sm.prod[0][o] = prod_0;
+        sm.prod[1][o] = prod_1;
+        sm.z[o] = pack_zc(z0, z1);
+    }
+    /* the wave's max change per variant (K <= 64 divides the wave) */
+    diff_bits = wave_max_mod(diff_bits, K);
+    const uint32_t lane = tid & 63;
+    __syncthreads();
+    /* phase 3: the variants that run store products, codes and beliefs */
+    if ((uint32_t)tid < nvk && A.vrun[gr.vbase + (tid & km)]) {
+        A.vprod[vst + tid] = make_double2(sm.prod[0][tid], sm.prod[1][tid]);
+        A.vzc[vst + tid] = (uint8_t)sm.z[tid];
+        A.belief[vst + tid] = sm.bel[tid];
+    }
+    /* per-variant fold: lanes 0..K-1 of each wave hold their variant's max */
+    if (lane < K && A.vrun[gr.vbase + lane]) fold_diff(A, gr.vbase + lane, ti, diff_bits);
+}
+
+template <bool kDamp, int N>
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_pass(BatchArgs A, double damp,
+                                                                                    double keep) {
+    __shared__ FactorSmem sm;
+    const uint32_t n_act = *A.act_n;
+    if (!n_act) return;
+    const uint32_t n_tiles = A.act_fpre[n_act];
+    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
+        uint32_t j;
+        find_tile(A.act_fpre, n_act, ti, j);
+        const GroupDev gr = A.groups[A.act_group[j]];
+        const Tile tile = A.ftiles[gr.ftile_first + (ti - A.act_fpre[j])];
+        batch_factor_tile<kDamp, N>(A, gr, tile, damp, keep, sm);
+        __syncthreads();   /* LDS is reused by the next tile */
+    }
+}
+
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass(BatchArgs A) {
+    __shared__ VarSmem sm;
+    const uint32_t n_act = *A.act_n;
+    if (!n_act) return;
+    const uint32_t n_tiles = A.act_vpre[n_act];
+    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
+        uint32_t j;
+        find_tile(A.act_vpre, n_act, ti, j);
+        const GroupDev gr = A.groups[A.act_group[j]];
+        const Tile tile = A.vtiles[gr.vtile_first + (ti - A.act_vpre[j])];
+        batch_var_tile(A, gr, tile, ti, sm);
+        __syncthreads();   /* LDS is reused by the next tile */
+    }
+}
+
+constexpr int kStopBlock = 1024;
+
+/* After a sweep (count = 1) or before the first one (count = 0): per
+ * running variant, its max change, counter and next vrun; then the list of
+ * groups with a running variant and their tile prefix sums. */
+struct StopArgs {
+    const GroupDev *__restrict__ groups;
+    uint32_t G, nvar;
+    uint8_t *__restrict__ vrun;
+    uint8_t *__restrict__ vfresh;
+    uint32_t *__restrict__ budget;
+    unsigned long long *__restrict__ iters;
+    double *__restrict__ last;
+    unsigned long long *__restrict__ diff;
+    uint32_t *__restrict__ act_n, *__restrict__ act_group, *__restrict__ act_fpre, *__restrict__ act_vpre;
     double tol;
 };
 
-__device__ __forceinline__ unsigned long long *batch_slot(const BatchCtl &c, int k, uint32_t g, uint32_t j) {
-    return c.diff + (((uint64_t)k * c.G + g) * kBatchSlots + j) * kCtlPad;
-}
-
-/* max of graph g's slots for sweep k; all lanes of the wave get it */
-__device__ __forceinline__ unsigned long long batch_read(const BatchCtl &c, int k, uint32_t g) {
-    const int lane = threadIdx.x & 63;
-    const unsigned long long v =
-        lane < kBatchSlots ? __atomic_load_n(batch_slot(c, k, g, lane), __ATOMIC_RELAXED) : 0ull;
-    return wave_max_u64(v);
-}
-
-__device__ __forceinline__ bool graph_runs(const BatchCtl &c, uint32_t g, int k) {
-    if (!c.start_active[g] || (uint32_t)k >= c.budget[g]) return false;
-    if (k == 0) return true;
-    return __longlong_as_double((long long)batch_read(c, k - 1, g)) > c.tol;
-}
-
-template <bool kDamp, int N, int kLayout>
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_pass(
-    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, const uint32_t *__restrict__ n_tiles_p,
-    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
-    const double *__restrict__ fac_p, const double *__restrict__ fac_q,
-    const double2 *__restrict__ fac_mask, const uint32_t *__restrict__ fac_vpos,
-    const uint32_t *__restrict__ fac_work, const double2 *__restrict__ m_vf, double2 *m_fv,
-    double2 *__restrict__ m_fv_vm, double damp, double keep, BatchCtl ctl, int k) {
-    __shared__ FactorSmem sm;
-    const uint32_t n_tiles = *n_tiles_p;
-    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
-        const Tile tile = tiles[ti];
-        if (!graph_runs(ctl, tile.graph, k)) continue;   /* uniform per block */
-        const BatchGraph gr = graphs[tile.graph];
-        const uint64_t eb = gr.edge_base, fb = gr.fac_base;
-        factor_tile<kDamp, N, kLayout>(fac_off + gr.fac_off_base, fac_meta + fb, fac_p + fb, fac_q + fb,
-                                              fac_mask + fb, fac_vpos + eb, fac_work + eb, m_vf + eb, m_fv + eb,
-                                              m_fv + eb, m_fv_vm + eb, damp, keep, tile.first,
-                                              tile.first + tile.count, sm);
-        __syncthreads();   /* LDS is reused by the next tile */
-    }
-}
-
-template <int kLayout>
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass(
-    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, const uint32_t *__restrict__ n_tiles_p,
-    const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
-    const uint16_t *__restrict__ var_work, const double2 *__restrict__ m_fv_in, double2 *__restrict__ m_vf,
-    double *__restrict__ belief, BatchCtl ctl, int k) {
-    __shared__ VarSmem sm;
-    const uint32_t n_tiles = *n_tiles_p;
-    /* each round of the loop covers tiles [base, base + span); in the pull
-     * layout the round's tiles go to the XCDs in runs of kPullVarRun, as in
-     * the plain var pass (465.7 vs 470.9 ms for the 768-graph batch) */
-    for (uint32_t base = 0; base + blockIdx.x < n_tiles; base += gridDim.x) {
-        const uint32_t left = n_tiles - base, span = left < gridDim.x ? left : gridDim.x;
-        const uint32_t ti = base + (kLayout == kLayoutPull ? xcd_runs<kPullVarRun>(blockIdx.x, span) : blockIdx.x);
-        const Tile tile = tiles[ti];
-        if (!graph_runs(ctl, tile.graph, k)) continue;   /* uniform per block */
-        const BatchGraph gr = graphs[tile.graph];
-        const uint64_t eb = gr.edge_base;
-        unsigned long long diff_bits =
-            var_tile<kLayout>(var_off + gr.var_off_base, var_edge + eb, var_work + gr.var_base, m_fv_in + eb,
-                              m_vf + eb, belief + gr.var_base, tile.first, tile.first + tile.count, sm);
-        diff_bits = wave_max_u64(diff_bits);
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) sm.red[wave] = diff_bits;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long m = sm.red[0];
+__global__ __launch_bounds__(kStopBlock) __attribute__((unused)) void batch_stop(StopArgs S, int count) {
+    __shared__ uint32_t s_cnt[kStopBlock / 64][3];
+    __shared__ uint32_t s_base[3];
+    const uint32_t tid = threadIdx.x;
+    if (count) {
+        for (uint32_t v = tid; v < S.nvar; v += kStopBlock) {
+            if (!S.vrun[v]) continue;
+            unsigned long long m = 0;
+            unsigned long long *sl = S.diff + (uint64_t)v * kBatchSlots * kCtlPad;
 #pragma unroll
-            for (int w = 1; w < kBlock / 64; w++) m = sm.red[w] > m ? sm.red[w] : m;
-            if (m) {
-                unsigned long long *slot = batch_slot(ctl, k, tile.graph, ti % kBatchSlots);
-                if (m > __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMax(slot, m);
+            for (int j = 0; j < kBatchSlots; j++) {
+                const unsigned long long x = sl[j * kCtlPad];
+                m = x > m ? x : m;
+                sl[j * kCtlPad] = 0ull;
+            }
+            const double d = __longlong_as_double((long long)m);
+            S.iters[v] += 1;
+            S.last[v] = d;
+            S.vfresh[v] = 0;
+            const uint32_t left = S.budget[v] - 1;
+            S.budget[v] = left;
+            /* tol = -inf: a fixed number of sweeps, NaN changes included */
+            S.vrun[v] = (left > 0 && (d > S.tol || S.tol == -__builtin_inf())) ? 1 : 0;
+        }
+        __syncthreads();
+    }
+    /* compaction of the groups with a running variant, in group order */
+    if (tid == 0) {
+        s_base[0] = 0;
+        s_base[1] = 0;
+        s_base[2] = 0;
+    }
+    __syncthreads();
+    for (uint32_t g0 = 0; g0 < S.G; g0 += kStopBlock) {
+        const uint32_t g = g0 + tid;
+        uint32_t on = 0, nf = 0, nv = 0;
+        if (g < S.G) {
+            const GroupDev gr = S.groups[g];
+            for (uint32_t k = 0; k < gr.K && !on; k++) on = S.vrun[gr.vbase + k];
+            if (on) {
+                nf = gr.ftile_count;
+                nv = gr.vtile_count;
             }
         }
-        __syncthreads();   /* LDS is reused by the next tile */
-    }
-}
-
-/* message layout moves over every factor tile (graph-local positions):
- * to_facmajor: dst[s] = src[vpos[s]]; to_varmajor: dst[vpos[s]] = src[s] */
-template <bool kToFacMajor>
-__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_permute(
-    const BatchGraph *__restrict__ graphs, const Tile *__restrict__ tiles, uint32_t n_tiles,
-    const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_vpos,
-    const double2 *__restrict__ src, double2 *__restrict__ dst) {
-    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
-        const Tile tile = tiles[ti];
-        const BatchGraph gr = graphs[tile.graph];
-        const uint32_t *off = fac_off + gr.fac_off_base;
-        const uint32_t base = off[tile.first], n = off[tile.first + tile.count] - base;
-        const uint64_t eb = gr.edge_base;
-        for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
-            const uint64_t s = eb + base + e, v = eb + fac_vpos[s];
-            if (kToFacMajor) dst[s] = src[v];
-            else dst[v] = src[s];
+        /* block scan of (on, nf, nv): wave scans, then across waves */
+        uint32_t x[3] = {on, nf, nv};
+        const uint32_t lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+#pragma unroll
+            for (uint32_t off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(x[c], off, 64);
+                if (lane >= off) x[c] += y;
+            }
+            if (lane == 63) s_cnt[wave][c] = x[c];
         }
+        __syncthreads();
+        uint32_t pre[3] = {0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            for (uint32_t w = 0; w < wave; w++) pre[c] += s_cnt[w][c];
+        if (on) {
+            const uint32_t slot = s_base[0] + pre[0] + x[0] - 1;
+            S.act_group[slot] = g;
+            S.act_fpre[slot] = s_base[1] + pre[1] + x[1] - nf;
+            S.act_vpre[slot] = s_base[2] + pre[2] + x[2] - nv;
+        }
+        __syncthreads();
+        if (tid == kStopBlock - 1) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) s_base[c] += pre[c] + x[c];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        *S.act_n = s_base[0];
+        S.act_fpre[s_base[0]] = s_base[1];
+        S.act_vpre[s_base[0]] = s_base[2];
     }
 }
 
-/* compact the chunk's partial maxima into out[k][g] for the host */
-__global__ __attribute__((unused)) void batch_compact(BatchCtl c, unsigned long long *out, int n) {
-    const uint64_t waves = (uint64_t)n * c.G;
-    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; w < waves;
-         w += ((uint64_t)gridDim.x * blockDim.x) / 64) {
-        const int k = (int)(w / c.G);
-        const uint32_t g = (uint32_t)(w % c.G);
-        const unsigned long long m = batch_read(c, k, g);
-        if ((threadIdx.x & 63) == 0) out[w] = m;
-    }
-}
-
-__global__ __attribute__((unused)) void batch_clear(unsigned long long *p, uint64_t n) {
+__global__ __attribute__((unused)) void bfill_pairs(double2 *p, uint64_t n, double c) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = 0ull;
+        p[i] = make_double2(c, c);
+}
+
+__global__ __attribute__((unused)) void bfill_u8(uint8_t *p, uint64_t n, uint8_t c) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = c;
+}
+
+/* out[i] = src[idx[i]] */
+__global__ __attribute__((unused)) void bgather_f64(const double *__restrict__ src, const uint64_t *__restrict__ idx,
+                                                    double *__restrict__ out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
 }
 
 }  // namespace

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "fastlbp.h"
+#include "flbp_internal.h"
 
 namespace flbp {
 

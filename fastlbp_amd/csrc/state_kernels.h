@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_state
 /* the v->f messages of the state layout, materialized var-major (the pull
  * layout's order): vf[vpos[s]] = vf_msg(products of slot s's variable,
  * f->v[s]) */
-__global__ void vf_from_state(const double2 *__restrict__ vprod, const uint8_t *__restrict__ vzc,
+__global__ __attribute__((unused)) void vf_from_state(const double2 *__restrict__ vprod, const uint8_t *__restrict__ vzc,
                               const uint32_t *__restrict__ fac_var, const uint32_t *__restrict__ fac_vpos,
                               const double2 *__restrict__ m_fv, double2 *__restrict__ vf, uint64_t n) {
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

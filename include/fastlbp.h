@@ -393,46 +393,68 @@ int flbp_plan_set_seq_tail(flbp_plan *plan, uint32_t max_items);
 /* Batched solve (not in the reference)                                */
 /* ------------------------------------------------------------------ */
 
-/* Many graphs -- independent graphs or clamped variants of one graph --
- * solved together: one device state, a persistent factor kernel and a
- * persistent variable kernel per sweep over a tile list of all unfinished
- * graphs, a chunk of sweeps captured once in a hipGraph and replayed.  Each
- * graph stops after exactly the sweeps its plain solve would run, with
- * bit-identical messages and beliefs. */
+/* Many graphs -- independent graphs and clamped variants of them -- solved
+ * together in one device state.  The batch holds groups: a group is one
+ * graph structure solved for K = 1, 2, 4, ..., 64 variants, whose messages,
+ * products and beliefs are stored K-interleaved (an index load and a
+ * gathered cache line serve all K variants).  A variant of a group differs
+ * from the group's graph only by head clamps of variables that appear in no
+ * factor body -- CausalFactorGraph::clamp (csrc/ext/causal_fg.cpp:17-26) of
+ * such a variable keeps the structure -- so it is bit for bit the solve of
+ * the graph rebuilt with those clamps (examples/fl_wrapper.cu:26-45).
+ * Variants are numbered across groups in order (group 0's variants first);
+ * an independent graph is a group of one variant.  Sweeps use the state
+ * layout of the plain solve (variant 4); a whole solve, every sweep up to
+ * maxiter, is captured into one hipGraph and replayed with one launch, the
+ * per-variant stop test running on the device.  Each variant stops after
+ * exactly the sweeps its plain solve runs, with bit-identical beliefs. */
 typedef struct flbp_batch flbp_batch;
 
-/* message layouts of a batch (see flbp_plan_set_variant: pull is the
- * layout of variant 0, scatter that of variant 3) */
-#define FLBP_LAYOUT_AUTO 0u      /* scatter while the batch fits the Infinity Cache, else pull */
-#define FLBP_LAYOUT_PULL 1u
-#define FLBP_LAYOUT_SCATTER 2u
-
 typedef struct flbp_batch_opts {
-    uint32_t tile_factors;   /* edge slots per factor tile, 64..512 (one block per tile at a time) */
-    uint32_t tile_vars;      /* edge slots per variable tile, 64..512 */
+    uint32_t tile_factors;   /* edge slots per factor tile of a 1-variant group, 64..512 (K variants: /K) */
+    uint32_t tile_vars;      /* edge slots per variable tile, 64..512 (/K) */
     uint32_t grid_blocks;    /* persistent grid size; 0 = resident blocks */
-    uint32_t chunk;          /* sweeps per hipGraph replay (<= 64) */
-    uint32_t use_graph;      /* 0: launch the chunk's kernels directly */
-    uint32_t layout;         /* FLBP_LAYOUT_*; changing it moves the stored messages */
+    uint32_t chunk;          /* 0: the whole solve in one hipGraph launch; else sweeps per launch, launches
+                                enqueued back to back and stopped once no variant runs */
+    uint32_t use_graph;      /* 0: launch the sweeps' kernels directly */
+    uint32_t layout;         /* reserved, 0 */
 } flbp_batch_opts;
 
+typedef struct flbp_batch_group {
+    const flbp_csr *graph;       /* the structure (copied at creation) */
+    uint32_t n_variants;         /* K */
+    const uint64_t *clamp_off;   /* [K+1], NULL for no clamps: variant k clamps clamp_var/clamp_val
+                                    [clamp_off[k] .. clamp_off[k+1]) */
+    const uint64_t *clamp_var;   /* variable indices of the graph, without body uses */
+    const uint8_t *clamp_val;    /* 0 or 1 */
+} flbp_batch_group;
+
 void flbp_batch_default_opts(flbp_batch_opts *o);
+int flbp_batch_create_groups(uint64_t n_groups, const flbp_batch_group *groups, int device,
+                             const flbp_batch_opts *opts, flbp_batch **out);
+/* independent graphs: one group of one variant per graph */
 int flbp_batch_create(uint64_t n_graphs, const flbp_csr *const *graphs, int device,
                       const flbp_batch_opts *opts, flbp_batch **out);
 void flbp_batch_destroy(flbp_batch *b);
-/* CausalBP::init for every graph */
+uint64_t flbp_batch_n_variants(const flbp_batch *b);
+/* CausalBP::init for every variant */
 int flbp_batch_init(flbp_batch *b, int logdomain);
-/* CausalBP::run() for every graph: iters_out[G] / maxdiff_out[G] per graph
- * (NULL skips); total->sweeps = sweeps of the longest-running graph */
+/* CausalBP::run() for every variant: iters_out[NV] / maxdiff_out[NV] per
+ * variant (NULL skips); total->sweeps = sweeps of the longest-running one */
 int flbp_batch_run(flbp_batch *b, const flbp_run_opts *opts, uint64_t *iters_out,
                    double *maxdiff_out, flbp_run_result *total);
-int flbp_batch_beliefs(flbp_batch *b, uint64_t graph, double *out, uint64_t n);
+/* beliefs of variant v, in its graph's variable order */
+int flbp_batch_beliefs(flbp_batch *b, uint64_t v, double *out, uint64_t n);
+/* out[i] = belief of variable vars[i] of variant v, gathered on the device */
+int flbp_batch_beliefs_at(flbp_batch *b, uint64_t v, uint64_t n, const uint64_t *vars, double *out);
+/* p of every factor of group g's structure (shared by its variants) */
+int flbp_batch_set_p(flbp_batch *b, uint64_t g, const double *p, uint64_t n);
 int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *opts);
 int flbp_batch_get_opts(flbp_batch *b, flbp_batch_opts *opts);
 /* iteration counters and beliefs back to their creation values */
 int flbp_batch_reset_counters(flbp_batch *b);
-/* times trial_sweeps sweeps for each candidate message layout, tile size
- * and grid size, keeps the fastest, then resets counters and re-inits */
+/* times trial_sweeps sweeps for each candidate tile bound and grid size,
+ * keeps the fastest, then resets counters and re-inits */
 int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps,
                         flbp_batch_opts *best);
 uint64_t flbp_batch_device_bytes(const flbp_batch *b);
