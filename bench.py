@@ -94,96 +94,138 @@ def build_graphs_parallel(specs, workers=16):
         return list(ex.map(lambda f: f(), specs))
 
 
-def bench_batched(args):
-    """512 graphs with V log-uniform in [1k, 200k] plus 256 clamped variants
-    (1-50 alarms clamped to 0/1 each) of one 200k-variable graph; damping 0.3,
-    tol 1e-9, at most 500 sweeps.  Sequential plain solves (one plan per
-    graph, each init + run) against one batched solve (init + run)."""
+def c3_workload(variants_per_group=32):
+    """C3: 512 graphs with V log-uniform in [1k, 200k] (generator seeds
+    10000 + g) plus 256 clamped variants of one 200k-variable graph (seed
+    99000), each clamping 1-50 of its alarms to 0/1 (alarms head no factor
+    body, so a clamp keeps the structure).  Returns (generated lists,
+    graphs, base graph, clamps, batch groups)."""
+    import numpy as np
+    import fastlbp_amd as fl
+    from fastlbp_amd.batch import Group, split_variants
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(7)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(200000), 512)).astype(np.int64)
+    with ThreadPoolExecutor(16) as ex:
+        gens = list(ex.map(lambda a: fl.generate(int(a[1]), 10_000 + a[0]), enumerate(sizes)))
+        graphs = list(ex.map(lambda gn: fl.Graph(gn.factors), gens))
+    base = fl.generate(200_000, 99_000)
+    bg = fl.Graph(base.factors)
+    aidx = bg.index_of(base.alarms)
+    clamps = []
+    for _ in range(256):
+        n = int(rng.integers(1, 51))
+        pick = rng.choice(len(aidx), size=n, replace=False)
+        clamps.append((aidx[pick], rng.integers(0, 2, n).astype(np.uint8)))
+    groups = [Group(g) for g in graphs] + split_variants(bg, clamps, variants_per_group)
+    return gens, graphs, bg, clamps, groups
+
+
+def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=15.0):
+    """Sequential plain solves (one plan per graph, one plan for the 256
+    variants with their clamps applied in place; each solve init + run,
+    timed alone) against one batched solve (init excluded)."""
     import numpy as np
     import torch
     import fastlbp_amd as fl
     from fastlbp_amd.batch import Batch
 
-    rng = np.random.default_rng(7)
-    sizes = np.exp(rng.uniform(np.log(1000), np.log(200000), 512)).astype(np.int64)
-    base = fl.generate(200000, 99_000)
-    clamps = []
-    for v in range(256):
-        n = int(rng.integers(1, 51))
-        idx = rng.choice(len(base.alarms), size=n, replace=False)
-        clamps.append((base.alarms[idx], rng.integers(0, 2, n).astype(np.uint8)))
-    specs = [(lambda n=int(n), g=g: fl.Graph(fl.generate(n, 10_000 + g).factors)) for g, n in enumerate(sizes)]
-    specs += [(lambda c=c: fl.Graph(fl.clamp(base.factors, c[0], c[1]))) for c in clamps]
     t0 = time.perf_counter()
-    graphs = build_graphs_parallel(specs)
-    log(f"[bench] built {len(graphs)} graphs in {time.perf_counter() - t0:.1f}s; "
-        f"sum V={sum(g.n_vars for g in graphs)} sum E={sum(g.n_edges for g in graphs)}")
+    gens, graphs, bg, clamps, groups = c3_workload(variants_per_group)
+    log(f"[bench] c3: built {len(graphs)} graphs + 256 variants of V={bg.n_vars} in {time.perf_counter() - t0:.1f}s; "
+        f"sum V={sum(g.n_vars for g in graphs) + 256 * bg.n_vars} "
+        f"sum E={sum(g.n_edges for g in graphs) + 256 * bg.n_edges}")
     tol, maxiter, damp = 1e-9, 500, 0.3
-    # sequential plain launches
     plans = [fl.Plan(g) for g in graphs]
+    vplan = fl.Plan(bg)
+    t_seq, seq_iters = 0.0, []
+    seq_bel = {}
+    sample = (0, 255, 511, 512, 512 + 100, 512 + 255)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    seq_iters = []
-    for p in plans:
+    for i, p in enumerate(plans):
         p.reset()
+        t1 = time.perf_counter()
         p.init()
         seq_iters.append(p.run(tol, maxiter, damp).iters)
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
-    seq_bel = [plans[i].beliefs() for i in (0, len(plans) // 2, len(plans) - 1)]
-    del plans
-    b = Batch(graphs)
-    best = b.autotune(damp, trial_sweeps=8)
-    log(f"[bench] autotuned batch opts {best}")
+        t_seq += time.perf_counter() - t1
+        if i in sample:
+            seq_bel[i] = p.beliefs()
+    for k, (v, x) in enumerate(clamps):
+        vplan.reset_head_clamps()
+        vplan.clamp_vars(v, x)
+        vplan.reset()
+        t1 = time.perf_counter()
+        vplan.init()
+        seq_iters.append(vplan.run(tol, maxiter, damp).iters)
+        t_seq += time.perf_counter() - t1
+        if 512 + k in sample:
+            seq_bel[512 + k] = vplan.beliefs()
+    del plans, vplan
+    b = Batch(groups)
+    best = b.autotune(damp, trial_sweeps=8, cache=tune_cache)
+    log(f"[bench] c3: batch opts {best}, device bytes {b.device_bytes / 1e9:.2f} GB")
     b.init()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     iters, md, tot = b.run(tol, maxiter, damp)
     torch.cuda.synchronize()
     t_bat = time.perf_counter() - t0
-    same = list(iters) == seq_iters and all(
-        np.array_equal(b.beliefs(i), bel) for i, bel in zip((0, len(graphs) // 2, len(graphs) - 1), seq_bel))
-    msgs = sum(2.0 * g.n_edges * int(it) for g, it in zip(graphs, iters))
-    # algorithmic bytes of the whole run: each graph's B_iter per sweep it ran
-    alg = sum((84.0 * g.n_edges + 24.0 * g.n_factors + 20.0 * g.n_vars) * int(it) for g, it in zip(graphs, iters))
-    achieved = alg / t_bat / 1e9
-    roofline = {"bound": "hbm", "kernel": "batched factor + var passes (whole run)", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": alg,
-                "timing": "wall time of the batched run (init excluded) over the algorithmic bytes of every "
-                          "sweep of every graph, 84 E + 24 F + 20 V per damped sweep"}
-    cpu = None
-    if not args.no_cpu:
+    vgraphs = graphs + [bg] * 256
+    same = list(iters) == seq_iters and all(np.array_equal(b.beliefs(i), bel) for i, bel in seq_bel.items())
+    msgs = sum(2.0 * g.n_edges * int(it) for g, it in zip(vgraphs, iters))
+    alg = sum((84.0 * g.n_edges + 24.0 * g.n_factors + 20.0 * g.n_vars) * int(it) for g, it in zip(vgraphs, iters))
+    log(f"[bench] c3: sequential {t_seq * 1e3:.1f} ms, batched {t_bat * 1e3:.1f} ms, "
+        f"speedup {t_seq / t_bat:.2f}x, {msgs / t_bat:.4g} msg-updates/s, identical {same}")
+    out = {"seconds": t_bat, "sequential_seconds": t_seq, "speedup_vs_sequential": t_seq / t_bat,
+           "msg_updates_per_s": msgs / t_bat, "sequential_msg_updates_per_s": msgs / t_seq,
+           "alg_bytes": alg, "frac_hbm": alg / t_bat / 1e9 / HBM_PEAK_GBS,
+           "iterations_identical": bool(same), "max_sweeps": int(iters.max()), "n_variants": len(vgraphs),
+           "variants_per_group": variants_per_group, "batch_opts": best, "device_bytes": b.device_bytes,
+           "sum_edges": int(sum(g.n_edges for g in vgraphs))}
+    if with_cpu:
         import oracle
-        threads = len(os.sched_getaffinity(0))
-        env_t = os.environ.get("OMP_NUM_THREADS")
-        if env_t and env_t.isdigit():
-            threads = min(threads, int(env_t))
+        threads = cpu_threads()
         done_msgs, n_done, t0 = 0.0, 0, time.perf_counter()
-        for g_i, n in enumerate(sizes):   # the batch's first generated graphs, solved to convergence
-            if time.perf_counter() - t0 >= args.cpu_seconds:
+        for gen in gens:   # the batch's graphs in order, solved to convergence
+            if time.perf_counter() - t0 >= cpu_seconds:
                 break
-            o = oracle.OracleGraph(fl.generate(int(n), 10_000 + g_i).factors)
+            o = oracle.OracleGraph(gen.factors)
             o.init()
             it_o, _ = o.run(tol, maxiter, damp, threads)
             done_msgs += 2.0 * o.n_edges * it_o
             n_done += 1
         cdt = time.perf_counter() - t0
-        cpu = {"value": done_msgs / cdt, "unit": "msg-updates/s", "cores": threads, "kind": "port",
-               "sample": f"the batch's first {n_done} generated graphs solved to convergence, {cdt:.1f} s "
-                         f"(generation included), OpenMP over {threads} threads"}
-        log(f"[bench] cpu baseline {cpu['value']:.4g} msg-updates/s ({n_done} graphs, {threads} threads)")
-    out = {"metric": METRIC, "value": msgs / t_bat, "unit": "msg-updates/s", "n_gpus": 1, "steps": 1,
-           "warmup": 0, "ms_per_step": t_bat * 1e3, "higher_is_better": True, "scaling": "weak",
+        out["cpu_baseline"] = {"value": done_msgs / cdt, "unit": "msg-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"the batch's first {n_done} graphs solved to convergence, {cdt:.1f} s, "
+                                         f"OpenMP over {threads} threads"}
+    return out
+
+
+def cpu_threads():
+    threads = len(os.sched_getaffinity(0))
+    env_t = os.environ.get("OMP_NUM_THREADS")
+    if env_t and env_t.isdigit():
+        threads = min(threads, int(env_t))
+    return threads
+
+
+def bench_batched(args):
+    """The C3 workload alone: one JSON line for the batched solve."""
+    r = run_c3(args.variants_per_group, args.tune_cache, not args.no_cpu, args.cpu_seconds)
+    cpu = r.pop("cpu_baseline", None)
+    roofline = {"bound": "hbm", "kernel": "batched factor + var passes (whole run)",
+                "achieved": round(r["alg_bytes"] / r["seconds"] / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(r["frac_hbm"], 4), "traffic": None, "algorithmic_bytes": r["alg_bytes"],
+                "timing": "wall time of the batched run (init excluded) over the algorithmic bytes of every "
+                          "sweep of every variant, 84 E + 24 F + 20 V per damped sweep"}
+    out = {"metric": METRIC, "value": r["msg_updates_per_s"], "unit": "msg-updates/s", "n_gpus": 1, "steps": 1,
+           "warmup": 0, "ms_per_step": r["seconds"] * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "batched-512-graphs+256-variants", "n_graphs": len(graphs),
-                      "sum_vars": int(sum(g.n_vars for g in graphs)),
-                      "sum_edges": int(sum(g.n_edges for g in graphs)), "damping": damp, "tol": tol,
-                      "maxiter": maxiter, "batch_opts": best},
-           "batched": {"seconds": t_bat, "sequential_seconds": t_seq, "speedup_vs_sequential": t_seq / t_bat,
-                       "sequential_msg_updates_per_s": msgs / t_seq, "iterations_identical": bool(same),
-                       "max_sweeps": int(iters.max()), "device_bytes": b.device_bytes},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "config": {"workload": "batched-512-graphs+256-variants", "n_variants": r["n_variants"],
+                      "sum_edges": r["sum_edges"], "damping": 0.3, "tol": 1e-9, "maxiter": 500,
+                      "variants_per_group": r["variants_per_group"], "batch_opts": r["batch_opts"]},
+           "batched": r, "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
@@ -600,6 +642,9 @@ def main():
     ap.add_argument("--updates", choices=["SEQFIX", "TOPO"], default="SEQFIX", help="seq: the schedule")
     ap.add_argument("--graphs", action="store_true", help="replay chunks of sweeps from captured hipGraphs")
     ap.add_argument("--c5-variants", type=int, default=64, help="c5: clamped variants per gradient step")
+    ap.add_argument("--variants-per-group", type=int, default=32,
+                    help="batched: variants of one graph per K-interleaved group (1, 2, 4, ..., 64)")
+    ap.add_argument("--tune-cache", default=None, help="batched: autotune cache file (JSON, by size class)")
     ap.add_argument("--c5-sweeps", type=int, default=20, help="c5: sweeps T of the differentiated solve")
     ap.add_argument("--c5-checkpoint", type=int, default=5, help="c5: checkpoint interval of the adjoint")
     ap.add_argument("--variant", type=int, default=-1,

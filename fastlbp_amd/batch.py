@@ -143,8 +143,7 @@ class Batch:
                 f"-K{kmax}")
 
     def autotune(self, damping: float, trial_sweeps: int = 8, cache: Optional[str] = None) -> dict:
-        """Times the candidate tile bounds and grid sizes and keeps the
-        fastest.  With a cache file, a batch of a size class tuned before
+        """Times the candidate tile bounds and keeps the fastest.  With a cache file, a batch of a size class tuned before
         takes the stored choice without timing, and a new choice is stored."""
         key = self.size_class()
         if cache and os.path.exists(cache):
@@ -163,7 +162,7 @@ class Batch:
             if os.path.exists(cache):
                 with open(cache) as f:
                     known = json.load(f)
-            known[key] = {k: best[k] for k in ("tile_factors", "tile_vars", "grid_blocks")}
+            known[key] = {k: best[k] for k in ("tile_factors", "tile_vars")}
             tmp = cache + ".tmp"
             with open(tmp, "w") as f:
                 json.dump(known, f, indent=1, sort_keys=True)

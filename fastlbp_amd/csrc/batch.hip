@@ -11,8 +11,8 @@
  * factor pass, a variable pass and the per-variant stop kernel -- is
  * captured once into a hipGraph and replayed with one launch; the stop test
  * and the list of running groups live on the device, so nothing returns to
- * the host between sweeps.  Tile slot bounds and the persistent grid size
- * are chosen by flbp_batch_autotune.
+ * the host between sweeps.  The tile slot bounds are chosen by
+ * flbp_batch_autotune.
  */
 #include <hip/hip_runtime.h>
 
@@ -65,6 +65,8 @@ struct Batch {
     double *last_dev = nullptr;
     uint32_t *act_n = nullptr, *act_group = nullptr, *act_fpre = nullptr, *act_vpre = nullptr;
     Tile *ftiles = nullptr, *vtiles = nullptr;
+    Tile *act_ftiles = nullptr, *act_vtiles = nullptr;   /* the running groups' tiles */
+    uint32_t *act_count = nullptr;                        /* [2] */
     uint64_t cap_ftiles = 0, cap_vtiles = 0, cap_scratch = 0;
     uint32_t n_ftiles = 0, n_vtiles = 0;
     uint32_t cut_f = 0, cut_v = 0;
@@ -122,12 +124,9 @@ struct Batch {
     BatchArgs args() const {
         BatchArgs a;
         a.groups = groups;
-        a.ftiles = ftiles;
-        a.vtiles = vtiles;
-        a.act_n = act_n;
-        a.act_group = act_group;
-        a.act_fpre = act_fpre;
-        a.act_vpre = act_vpre;
+        a.act_ftiles = act_ftiles;
+        a.act_vtiles = act_vtiles;
+        a.act_count = act_count;
         a.fac_off = fac_off;
         a.fac_meta = fac_meta;
         a.fac_var = fac_var;
@@ -155,6 +154,12 @@ struct Batch {
                         act_n, act_group, act_fpre, act_vpre, tol};
     }
     int build_tiles();
+    /* after batch_stop: the flat lists of the running groups' tiles */
+    void expand() {
+        const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n_ftiles + kBlock - 1) / kBlock, 1024));
+        hipLaunchKernelGGL(batch_expand, dim3(grid), dim3(kBlock), 0, stream, groups, ftiles, vtiles, act_n, act_group,
+                           act_fpre, act_vpre, act_ftiles, act_vtiles, act_count);
+    }
     int record_sweeps(uint32_t n, double tol, double damping);
     int launch_solve(uint32_t n, double tol, double damping);
     int start_run(uint64_t maxiter, double tol);
@@ -237,14 +242,18 @@ int Batch::build_tiles() {
     n_vtiles = (uint32_t)vall.size();
     if (n_ftiles > cap_ftiles) {
         if (ftiles) release(ftiles);
-        ftiles = nullptr;
+        if (act_ftiles) release(act_ftiles);
+        ftiles = act_ftiles = nullptr;
         TRY(alloc(ftiles, n_ftiles));
+        TRY(alloc(act_ftiles, n_ftiles));
         cap_ftiles = n_ftiles;
     }
     if (n_vtiles > cap_vtiles) {
         if (vtiles) release(vtiles);
-        vtiles = nullptr;
+        if (act_vtiles) release(act_vtiles);
+        vtiles = act_vtiles = nullptr;
         TRY(alloc(vtiles, n_vtiles));
+        TRY(alloc(act_vtiles, n_vtiles));
         cap_vtiles = n_vtiles;
     }
     if (scr_tot > cap_scratch || !scratch) {
@@ -270,8 +279,7 @@ int Batch::record_sweeps(uint32_t n, double tol, double damping) {
     const double keep = 1.0 - damping;
     const BatchArgs a = args();
     const StopArgs st = stop_args(tol);
-    const uint32_t fg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_ftiles));
-    const uint32_t vg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_vtiles));
+    const uint32_t fg = std::max<uint32_t>(1, n_ftiles), vg = std::max<uint32_t>(1, n_vtiles);
     const bool narrow = max_fac_deg <= (uint64_t)kFacRegDegSmall;
     for (uint32_t k = 0; k < n; k++) {
         if (damping != 0.0) {
@@ -291,6 +299,7 @@ int Batch::record_sweeps(uint32_t n, double tol, double damping) {
         }
         hipLaunchKernelGGL(batch_var_pass, dim3(vg), dim3(kBlock), 0, stream, a);
         hipLaunchKernelGGL(batch_stop, dim3(1), dim3(kStopBlock), 0, stream, st, 1);
+        expand();
     }
     HIP_TRY(hipGetLastError());
     return FLBP_OK;
@@ -300,8 +309,7 @@ int Batch::record_sweeps(uint32_t n, double tol, double damping) {
  * use for this length, damping, tol and grid) or as direct launches */
 int Batch::launch_solve(uint32_t n, double tol, double damping) {
     if (!opts.use_graph) return record_sweeps(n, tol, damping);
-    const uint32_t fg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_ftiles));
-    const uint32_t vg = std::max<uint32_t>(1, std::min(opts.grid_blocks, n_vtiles));
+    const uint32_t fg = std::max<uint32_t>(1, n_ftiles), vg = std::max<uint32_t>(1, n_vtiles);
     if (!exec || exec_n != n || exec_damp != damping || exec_tol != tol || exec_fgrid != fg || exec_vgrid != vg) {
         drop_exec();
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -342,6 +350,7 @@ int Batch::start_run(uint64_t maxiter, double tol) {
     HIP_TRY(hipMemcpyAsync(vrun, h_run, NV, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipMemcpyAsync(last_dev, h_last, NV * 8, hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(batch_stop, dim3(1), dim3(kStopBlock), 0, stream, stop_args(tol), 0);
+    expand();
     HIP_TRY(hipGetLastError());
     return FLBP_OK;
 }
@@ -388,7 +397,7 @@ void flbp_batch_default_opts(flbp_batch_opts *o) {
     o->tile_factors = flbp::kStageSlots;
     o->tile_vars = flbp::kStageSlots;
     o->grid_blocks = 0;
-    o->chunk = 0;
+    o->chunk = 8;
     o->use_graph = 1;
     o->layout = 0;
 }
@@ -499,6 +508,7 @@ int flbp_batch_create_groups(uint64_t n_groups, const flbp_batch_group *gs, int 
     TRY(B->alloc(B->act_group, n_groups));
     TRY(B->alloc(B->act_fpre, n_groups + 1));
     TRY(B->alloc(B->act_vpre, n_groups + 1));
+    TRY(B->alloc(B->act_count, 2));
     HIP_TRY(hipHostMalloc((void **)&B->h_run, NV, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&B->h_budget, NV * 4, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&B->h_iters, NV * 8, hipHostMallocDefault));
@@ -851,17 +861,14 @@ int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps, fl
     if (!(damping >= 0.0 && damping < 1.0)) return flbp::fail(FLBP_EINVAL, "flbp_batch_autotune: bad damping");
     HIP_TRY(hipSetDevice(B->device));
     const flbp_batch_opts base = B->opts;
-    const uint32_t g0 = (uint32_t)default_grid(B->device);
-    const uint32_t tiles[] = {flbp::kStageSlots / 2, flbp::kStageSlots};
-    const uint32_t grids[] = {g0, 2 * g0};
+    const uint32_t tiles[] = {flbp::kStageSlots / 4, flbp::kStageSlots / 2, flbp::kStageSlots};
     double best_t = INFINITY;
     flbp_batch_opts best_o = base;
     for (uint32_t tf : tiles)
-        for (uint32_t gb : grids) {
+        for (uint32_t tv : tiles) {
             flbp_batch_opts o = base;
             o.tile_factors = tf;
-            o.tile_vars = tf;
-            o.grid_blocks = gb;
+            o.tile_vars = tv;
             TRY(flbp_batch_set_opts(b, &o));
             TRY(flbp_batch_reset_counters(b));
             TRY(flbp_batch_init(b, 0));

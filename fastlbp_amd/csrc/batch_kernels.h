@@ -73,11 +73,9 @@ constexpr int kBatchSlots = 8;
 
 struct BatchArgs {
     const GroupDev *__restrict__ groups;
-    const Tile *__restrict__ ftiles, *__restrict__ vtiles;
-    const uint32_t *__restrict__ act_n;       /* groups with a running variant */
-    const uint32_t *__restrict__ act_group;   /* [G] their indices */
-    const uint32_t *__restrict__ act_fpre;    /* [G+1] prefix sums of their factor tile counts */
-    const uint32_t *__restrict__ act_vpre;    /* [G+1] ... of their variable tile counts */
+    const Tile *__restrict__ act_ftiles;      /* the running groups' factor tiles, in group order */
+    const Tile *__restrict__ act_vtiles;      /* ... and variable tiles */
+    const uint32_t *__restrict__ act_count;   /* [2]: how many of each */
     const uint32_t *__restrict__ fac_off, *__restrict__ fac_meta, *__restrict__ fac_var, *__restrict__ fac_work;
     const uint32_t *__restrict__ fac_crow;    /* clamped factor: first of its K masks */
     const double *__restrict__ fac_p, *__restrict__ fac_q;
@@ -95,18 +93,6 @@ struct BatchArgs {
     double c;                                 /* init value: the v->f input of a fresh variant */
 };
 
-/* the active tile ti of the sweep: group g and its tile */
-__device__ __forceinline__ void find_tile(const uint32_t *__restrict__ pre, uint32_t n_act, uint32_t ti,
-                                          uint32_t &j) {
-    uint32_t lo = 0, hi = n_act;   /* pre[lo] <= ti < pre[hi] */
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= ti) lo = mid;
-        else hi = mid;
-    }
-    j = lo;
-}
-
 /* Messages of one factor from a global array with stride K. */
 struct StridedIn {
     const double2 *__restrict__ m;
@@ -114,6 +100,60 @@ struct StridedIn {
     __device__ __forceinline__ double2 get(int j) const { return m[(uint64_t)j << lk]; }
     __device__ __forceinline__ double2 pick(int j) const { return m[(uint64_t)j << lk]; }
 };
+
+/* One wide factor tile (n slots, n * K > kStage): each variant in turn,
+ * from global memory. */
+template <bool kDamp>
+__device__ __forceinline__ void batch_wide_factor(const BatchArgs &A, const GroupDev &gr, const Tile &tile,
+                                                           double damp, double keep) {
+    const uint32_t *fac_off = A.fac_off + gr.fac_off_base;
+    const uint64_t fb = gr.fac_base, eb = gr.edge_base;
+    const uint32_t f0 = tile.first;
+    const uint32_t base = fac_off[f0];
+    const uint32_t n = fac_off[f0 + 1] - base;
+    const uint32_t lk = gr.lk, K = gr.K;
+    const int tid = threadIdx.x;
+    double2 *const fv = A.m_fv + gr.msg_base + ((uint64_t)base << lk);
+    /* one wide factor, each variant in turn, from global memory */
+    const uint32_t meta = A.fac_meta[fb + f0];
+    const uint32_t type = meta & 3u;
+    const bool clamped = (meta & 4u) != 0;
+    const int h = (int)(meta >> 8);
+    const double p = A.fac_p[fb + f0], q = type ? A.fac_q[fb + f0] : 0.0;
+    double2 *const scr = A.scratch + tile.aux;
+    for (uint32_t k = 0; k < K; k++) {
+        if (!A.vrun[gr.vbase + k]) continue;   /* uniform over the block */
+        double mask0 = 1.0, mask1 = 1.0;
+        if (clamped) {
+            const double2 mk = A.masks[(uint64_t)A.fac_crow[fb + f0] + k];
+            mask0 = mk.x;
+            mask1 = mk.y;
+        }
+        const bool fresh = A.vfresh[gr.vbase + k] != 0;
+        if (!fresh) {
+            for (uint32_t t = tid; t < n; t += kBlock) {
+                const uint64_t v = gr.vst_base + ((uint64_t)A.fac_var[eb + base + t] << lk) + k;
+                scr[t] = vf_of(A.vprod[v], A.vzc[v], fv[((uint64_t)t << lk) + k]);
+            }
+        }
+        __syncthreads();
+        const LdsIn in{scr};
+        const ConstIn cin{A.c};
+        const double2 mh = fresh ? cin.pick(h) : in.pick(h), mo = fresh ? cin.pick(h == 0 ? 1 : 0) : in.pick(h == 0 ? 1 : 0);
+        for (uint32_t t = tid; t < n; t += kBlock) {
+            double2 r = fresh ? fv_msg<0>(cin, type, clamped, (int)n, (int)t, h, mh, mo, p, q, mask0, mask1)
+                              : fv_msg<0>(in, type, clamped, (int)n, (int)t, h, mh, mo, p, q, mask0, mask1);
+            double2 *dst = fv + ((uint64_t)t << lk) + k;
+            if (kDamp) {
+                const double2 old = *dst;
+                r.x = keep * r.x + damp * old.x;
+                r.y = keep * r.y + damp * old.y;
+            }
+            *dst = r;
+        }
+        __syncthreads();   /* the scratch range is reused by the next variant */
+    }
+}
 
 /* One factor tile of group gr: factors [f0, f1), all K variants. */
 template <bool kDamp, int N>
@@ -129,45 +169,7 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
     const int tid = threadIdx.x;
     double2 *const fv = A.m_fv + gr.msg_base + ((uint64_t)base << lk);
     if (nk > (uint32_t)kStage) {
-        /* one wide factor, each variant in turn, from global memory */
-        const uint32_t meta = A.fac_meta[fb + f0];
-        const uint32_t type = meta & 3u;
-        const bool clamped = (meta & 4u) != 0;
-        const int h = (int)(meta >> 8);
-        const double p = A.fac_p[fb + f0], q = type ? A.fac_q[fb + f0] : 0.0;
-        double2 *const scr = A.scratch + tile.aux;
-        for (uint32_t k = 0; k < K; k++) {
-            if (!A.vrun[gr.vbase + k]) continue;   /* uniform over the block */
-            double mask0 = 1.0, mask1 = 1.0;
-            if (clamped) {
-                const double2 mk = A.masks[(uint64_t)A.fac_crow[fb + f0] + k];
-                mask0 = mk.x;
-                mask1 = mk.y;
-            }
-            const bool fresh = A.vfresh[gr.vbase + k] != 0;
-            if (!fresh) {
-                for (uint32_t t = tid; t < n; t += kBlock) {
-                    const uint64_t v = gr.vst_base + ((uint64_t)A.fac_var[eb + base + t] << lk) + k;
-                    scr[t] = vf_of(A.vprod[v], A.vzc[v], fv[((uint64_t)t << lk) + k]);
-                }
-            }
-            __syncthreads();
-            const LdsIn in{scr};
-            const ConstIn cin{A.c};
-            const double2 mh = fresh ? cin.pick(h) : in.pick(h), mo = fresh ? cin.pick(h == 0 ? 1 : 0) : in.pick(h == 0 ? 1 : 0);
-            for (uint32_t t = tid; t < n; t += kBlock) {
-                double2 r = fresh ? fv_msg<0>(cin, type, clamped, (int)n, (int)t, h, mh, mo, p, q, mask0, mask1)
-                                  : fv_msg<0>(in, type, clamped, (int)n, (int)t, h, mh, mo, p, q, mask0, mask1);
-                double2 *dst = fv + ((uint64_t)t << lk) + k;
-                if (kDamp) {
-                    const double2 old = *dst;
-                    r.x = keep * r.x + damp * old.x;
-                    r.y = keep * r.y + damp * old.y;
-                }
-                *dst = r;
-            }
-            __syncthreads();   /* the scratch range is reused by the next variant */
-        }
+        batch_wide_factor<kDamp>(A, gr, tile, damp, keep);
         return;
     }
     /* phase 1 */
@@ -308,37 +310,36 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
     const double2 *fv = A.m_fv + gr.msg_base;
     const uint64_t vst = gr.vst_base + ((uint64_t)v0 << lk);
     if (nk > (uint32_t)kStage) {
-        /* one wide variable, each variant in turn: the product in edge
-         * order on lane 0 over kStage-slot chunks gathered by the block */
-        for (uint32_t k = 0; k < K; k++) {
-            if (!A.vrun[gr.vbase + k]) continue;   /* uniform over the block */
-            double prod_0 = 1.0, prod_1 = 1.0;
-            uint32_t z0 = 0, z1 = 0;
-            for (uint32_t c = 0; c < n; c += kStage) {
-                const uint32_t m = n - c < (uint32_t)kStage ? n - c : (uint32_t)kStage;
-                for (uint32_t e = tid; e < m; e += kBlock)
-                    sm.m[e] = fv[((uint64_t)A.var_edge[eb + base + c + e] << lk) + k];
-                __syncthreads();
-                if (tid == 0) {
-                    for (uint32_t e = 0; e < m; e++) {
-                        const double2 x = sm.m[e];
-                        if (x.x == 0) z0++; else prod_0 *= x.x;
-                        if (x.y == 0) z1++; else prod_1 *= x.y;
-                        scale_pair(prod_0, prod_1);
-                    }
+        /* one wide variable: chunks of kStage / K edges of all K variants
+         * staged by the block, lane k multiplying variant k's messages in
+         * edge order */
+        const uint32_t C = (uint32_t)kStage >> lk;
+        double prod_0 = 1.0, prod_1 = 1.0;
+        uint32_t z0 = 0, z1 = 0;
+        for (uint32_t c = 0; c < n; c += C) {
+            const uint32_t m = n - c < C ? n - c : C;
+            for (uint32_t i = tid; i < (m << lk); i += kBlock)
+                sm.m[i] = fv[((uint64_t)A.var_edge[eb + base + c + (i >> lk)] << lk) + (i & km)];
+            __syncthreads();
+            if ((uint32_t)tid < K) {
+                for (uint32_t e = 0; e < m; e++) {
+                    const double2 x = sm.m[(e << lk) + tid];
+                    if (x.x == 0) z0++; else prod_0 *= x.x;
+                    if (x.y == 0) z1++; else prod_1 *= x.y;
+                    scale_pair(prod_0, prod_1);
                 }
-                __syncthreads();
             }
-            if (tid == 0) {
-                const double b0 = prod_0 * (double)(z0 == 0);
-                const double b1 = prod_1 * (double)(z1 == 0);
-                const double b = b1 / (b0 + b1);
-                const double diff = fabs(A.belief[vst + k] - b);
-                A.belief[vst + k] = b;
-                A.vprod[vst + k] = make_double2(prod_0, prod_1);
-                A.vzc[vst + k] = pack_zc(z0, z1);
-                fold_diff(A, gr.vbase + k, ti, (unsigned long long)__double_as_longlong(diff));
-            }
+            __syncthreads();
+        }
+        if ((uint32_t)tid < K && A.vrun[gr.vbase + tid]) {
+            const double b0 = prod_0 * (double)(z0 == 0);
+            const double b1 = prod_1 * (double)(z1 == 0);
+            const double b = b1 / (b0 + b1);
+            const double diff = fabs(A.belief[vst + tid] - b);
+            A.belief[vst + tid] = b;
+            A.vprod[vst + tid] = make_double2(prod_0, prod_1);
+            A.vzc[vst + tid] = pack_zc(z0, z1);
+            fold_diff(A, gr.vbase + tid, ti, (unsigned long long)__double_as_longlong(diff));
         }
         return;
     }
@@ -402,35 +403,64 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
     if (lane < K && A.vrun[gr.vbase + lane]) fold_diff(A, gr.vbase + lane, ti, diff_bits);
 }
 
+/* The sweep kernels launch one block per tile of the batch; block b takes
+ * active tile b, and blocks past the active count exit at once.  (A
+ * persistent grid walking the tiles in a loop was latency-bound: each tile
+ * starts with a chain of dependent loads -- tile, group, offsets, slot map,
+ * messages -- that a few blocks per CU cannot overlap; one block per tile
+ * lets the dispatcher keep every CU full.) */
 template <bool kDamp, int N>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_pass(BatchArgs A, double damp,
                                                                                     double keep) {
     __shared__ FactorSmem sm;
-    const uint32_t n_act = *A.act_n;
-    if (!n_act) return;
-    const uint32_t n_tiles = A.act_fpre[n_act];
-    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
-        uint32_t j;
-        find_tile(A.act_fpre, n_act, ti, j);
-        const GroupDev gr = A.groups[A.act_group[j]];
-        const Tile tile = A.ftiles[gr.ftile_first + (ti - A.act_fpre[j])];
-        batch_factor_tile<kDamp, N>(A, gr, tile, damp, keep, sm);
-        __syncthreads();   /* LDS is reused by the next tile */
-    }
+    const uint32_t ti = blockIdx.x;
+    if (ti >= A.act_count[0]) return;
+    const Tile tile = A.act_ftiles[ti];
+    const GroupDev gr = A.groups[tile.group];
+    batch_factor_tile<kDamp, N>(A, gr, tile, damp, keep, sm);
 }
 
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass(BatchArgs A) {
     __shared__ VarSmem sm;
-    const uint32_t n_act = *A.act_n;
-    if (!n_act) return;
-    const uint32_t n_tiles = A.act_vpre[n_act];
-    for (uint32_t ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
-        uint32_t j;
-        find_tile(A.act_vpre, n_act, ti, j);
-        const GroupDev gr = A.groups[A.act_group[j]];
-        const Tile tile = A.vtiles[gr.vtile_first + (ti - A.act_vpre[j])];
-        batch_var_tile(A, gr, tile, ti, sm);
-        __syncthreads();   /* LDS is reused by the next tile */
+    const uint32_t ti = blockIdx.x;
+    if (ti >= A.act_count[1]) return;
+    const Tile tile = A.act_vtiles[ti];
+    const GroupDev gr = A.groups[tile.group];
+    batch_var_tile(A, gr, tile, ti, sm);
+}
+
+/* the active group of active tile ti: pre[j] <= ti < pre[j + 1] */
+__device__ __forceinline__ uint32_t tile_group(const uint32_t *__restrict__ pre, uint32_t n_act, uint32_t ti) {
+    uint32_t lo = 0, hi = n_act;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= ti) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* The running groups' tiles copied into the flat lists the sweep kernels
+ * walk, one thread per active tile (grid-stride). */
+__global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_expand(
+    const GroupDev *__restrict__ groups, const Tile *__restrict__ ftiles, const Tile *__restrict__ vtiles,
+    const uint32_t *__restrict__ act_n, const uint32_t *__restrict__ act_group, const uint32_t *__restrict__ act_fpre,
+    const uint32_t *__restrict__ act_vpre, Tile *__restrict__ act_ftiles, Tile *__restrict__ act_vtiles,
+    uint32_t *__restrict__ act_count) {
+    const uint32_t n_act = *act_n;
+    const uint32_t nf = act_fpre[n_act], nv = act_vpre[n_act];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        act_count[0] = nf;
+        act_count[1] = nv;
+    }
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < nf; t += stride) {
+        const uint32_t j = tile_group(act_fpre, n_act, t);
+        act_ftiles[t] = ftiles[groups[act_group[j]].ftile_first + (t - act_fpre[j])];
+    }
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < nv; t += stride) {
+        const uint32_t j = tile_group(act_vpre, n_act, t);
+        act_vtiles[t] = vtiles[groups[act_group[j]].vtile_first + (t - act_vpre[j])];
     }
 }
 

@@ -404,18 +404,20 @@ int flbp_plan_set_seq_tail(flbp_plan *plan, uint32_t max_items);
  * the graph rebuilt with those clamps (examples/fl_wrapper.cu:26-45).
  * Variants are numbered across groups in order (group 0's variants first);
  * an independent graph is a group of one variant.  Sweeps use the state
- * layout of the plain solve (variant 4); a whole solve, every sweep up to
- * maxiter, is captured into one hipGraph and replayed with one launch, the
- * per-variant stop test running on the device.  Each variant stops after
+ * layout of the plain solve (variant 4); sweeps are captured into a
+ * hipGraph and replayed, the per-variant stop test and the list of running
+ * groups kept on the device (opts.chunk).  Each variant stops after
  * exactly the sweeps its plain solve runs, with bit-identical beliefs. */
 typedef struct flbp_batch flbp_batch;
 
 typedef struct flbp_batch_opts {
     uint32_t tile_factors;   /* edge slots per factor tile of a 1-variant group, 64..512 (K variants: /K) */
     uint32_t tile_vars;      /* edge slots per variable tile, 64..512 (/K) */
-    uint32_t grid_blocks;    /* persistent grid size; 0 = resident blocks */
-    uint32_t chunk;          /* 0: the whole solve in one hipGraph launch; else sweeps per launch, launches
-                                enqueued back to back and stopped once no variant runs */
+    uint32_t grid_blocks;    /* reserved: the sweep kernels launch one block per tile */
+    uint32_t chunk;          /* sweeps per hipGraph launch (default 8): launches are enqueued back to back,
+                                at most two ahead of a finished one, and stop once no variant runs; 0: the
+                                whole solve up to maxiter in one launch (the sweeps after the last variant
+                                stopped still dispatch their blocks) */
     uint32_t use_graph;      /* 0: launch the sweeps' kernels directly */
     uint32_t layout;         /* reserved, 0 */
 } flbp_batch_opts;
@@ -453,8 +455,9 @@ int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *opts);
 int flbp_batch_get_opts(flbp_batch *b, flbp_batch_opts *opts);
 /* iteration counters and beliefs back to their creation values */
 int flbp_batch_reset_counters(flbp_batch *b);
-/* times trial_sweeps sweeps for each candidate tile bound and grid size,
- * keeps the fastest, then resets counters and re-inits */
+/* times trial_sweeps sweeps for each pair of candidate factor and variable
+ * tile bounds (128, 256, 512 slots), keeps the fastest, then resets
+ * counters and re-inits */
 int flbp_batch_autotune(flbp_batch *b, double damping, uint32_t trial_sweeps,
                         flbp_batch_opts *best);
 uint64_t flbp_batch_device_bytes(const flbp_batch *b);
