@@ -194,8 +194,6 @@ int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_
     if (P->seq)
         return flbp::fail(FLBP_EINVAL, "flbp_adjoint_create: the adjoint differentiates PARALL sweeps, and the plan "
                                        "has a sequential schedule");
-    if (P->max_fac_deg > (uint64_t)kAdjFacDeg)
-        return flbp::fail(FLBP_ERANGE, "flbp_adjoint_create: factor degree above 6");
     if (P->variant >= 2) {   /* the adjoint walks the pull layout: move the plan's messages there */
         const int rc = flbp_plan_set_variant(plan, 0);
         if (rc) return rc;
@@ -226,7 +224,11 @@ int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_
     if (!rc) rc = A->alloc(A->avf, E);
     if (!rc) rc = A->alloc(A->avf2, E);
     if (!rc) rc = A->alloc(A->afv, E);
-    if (!rc) rc = A->alloc(A->gscratch, P->max_var_deg > (uint64_t)kAdjStage ? 4 * E : 1);
+    /* scratch rows (4 pairs per slot) for variables wider than a staged
+     * block and for factors above the exact-degree code */
+    if (!rc)
+        rc = A->alloc(A->gscratch,
+                      P->max_var_deg > (uint64_t)kAdjStage || P->max_fac_deg > (uint64_t)kAdjFacDeg ? 4 * E : 1);
     if (!rc) rc = A->build_blocks();
     if (!rc) rc = A->alloc(A->pbar, P->F);
     if (!rc) rc = A->alloc(A->bbar, P->V);
@@ -311,7 +313,7 @@ int flbp::Adjoint::run_backward() {
             if (F)
                 hipLaunchKernelGGL(fac_vjp_staged, dim3((unsigned)n_fac_blocks), dim3(kBlock), 0, st, fac_blk,
                                    P->fac_off, P->fac_meta, P->fac_p, P->fac_q, P->fac_mask, P->fac_vpos, fac_work,
-                                   vf_state(t - 1), afv, avf2, pbar, keep);
+                                   vf_state(t - 1), afv, avf2, pbar, gscratch, keep);
             std::swap(avf, avf2);
         }
         HIP_TRY(hipGetLastError());

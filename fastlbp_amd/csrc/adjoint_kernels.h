@@ -29,7 +29,7 @@ namespace {
 
 using namespace flbp::dev;
 
-constexpr int kAdjFacDeg = 6;   /* largest factor degree of the reverse kernels */
+constexpr int kAdjFacDeg = 6;   /* factors up to this degree use the exact-degree reverse code */
 constexpr int kAdjVarDeg = 4;   /* var_vjp_one keeps variables up to this degree in registers */
 
 /* Reverse of the variable pass for one variable v, run by one lane.
@@ -273,6 +273,120 @@ __device__ __forceinline__ void fac_output_vjp(const double (&m0)[N], const doub
     }
 }
 
+/* The VJP of every output of one factor of any degree d in O(d): m(j) is
+ * the v->f input of slot j, a(j) the adjoint of output j (before the
+ * damping weight keep), out(j, r) receives the adjoint r of input j; row
+ * holds 4 pairs of scratch per slot.  With x the state the CPD singles out
+ * and A = x + y, over the body slots B (every slot but the head h):
+ *   head output:   pi = prod_B x, Z = prod_B A
+ *   body output t: pi_t = prod_{B\t} x, Z_t = prod_{B\t} A
+ * Input j of B gets Xb_h dpi/dx_j + Zb_h dZ/dA_j from the head output and
+ * sum_{t != j} (Xb_t prod_{B\{t,j}} x + Zb_t prod_{B\{t,j}} A) from the
+ * body outputs; the double leave-outs come from prefix products P, suffix
+ * products Q and the running sums W (forward) and U (backward), as in
+ * var_vjp_one.  The head slot gets the sum of the body outputs' head
+ * adjoints.  Reads of m(j) precede out(j), so out may overwrite m. */
+template <class MIn, class AIn, class Out>
+__device__ __forceinline__ double fac_vjp_general(const MIn &m, const AIn &a, const Out &out, double2 *row, int d,
+                                                  int h, uint32_t type, bool clamped, double p, double q,
+                                                  double mask0, double mask1, double keep) {
+    double pb = 0.0;
+    if (type == 0) {   /* (1-p, p) */
+        const double2 ub = make_double2(keep * a(0).x, keep * a(0).y);
+        const double u0 = 1 - p, u1 = p, S = u0 + u1;
+        const double dot = ub.x * (u0 / S) + ub.y * (u1 / S);
+        pb = ((ub.y - dot) - (ub.x - dot)) / S;
+        for (int j = 0; j < d; j++) out(j, make_double2(0.0, 0.0));
+        return pb;
+    }
+    const bool is_and = (type == 1);
+    const double mk0 = clamped ? mask0 : 1.0, mk1 = clamped ? mask1 : 1.0;
+    auto xa = [&](double2 v, double &x, double &A) {
+        x = is_and ? v.y : v.x;
+        A = v.x + v.y;
+    };
+    const double2 mh = m(h);
+    /* pass 1: prefix products over B */
+    double px = 1.0, pa = 1.0;
+    for (int j = 0; j < d; j++) {
+        if (j == h) continue;
+        double x, A;
+        xa(m(j), x, A);
+        row[4 * j] = make_double2(px, pa);
+        px *= x;
+        pa *= A;
+    }
+    /* the head output */
+    double Xbh = 0.0, Zbh = 0.0;
+    {
+        const double2 ub = make_double2(keep * a(h).x, keep * a(h).y);
+        const double Mu = (p - q) * px + q * pa, Mw = (q - p) * px + (1 - q) * pa;
+        const double M0 = (is_and ? Mw : Mu) * mk0, M1 = (is_and ? Mu : Mw) * mk1;
+        const double S = M0 + M1;
+        if (S != 0.0) {
+            const double dot = ub.x * (M0 / S) + ub.y * (M1 / S);
+            const double Mb0 = (ub.x - dot) / S * mk0, Mb1 = (ub.y - dot) / S * mk1;
+            const double Ub = is_and ? Mb1 : Mb0, Wb = is_and ? Mb0 : Mb1;
+            Xbh = (p - q) * (Ub - Wb);
+            Zbh = q * Ub + (1 - q) * Wb;
+            pb += (Ub - Wb) * px;
+        }
+    }
+    /* pass 2: suffix products, the body outputs' adjoints, running sums U */
+    const double hx = is_and ? mh.y * mk1 : mh.x * mk0;
+    const double hy = is_and ? mh.x * mk0 : mh.y * mk1;
+    const double B = (p - q) * (hx - hy), Aq = q * hx + (1 - q) * hy;
+    double hxb = 0.0, hyb = 0.0;
+    double qx = 1.0, qa = 1.0, ux = 0.0, ua = 0.0;
+    for (int j = d - 1; j >= 0; j--) {
+        if (j == h) continue;
+        double x, A;
+        xa(m(j), x, A);
+        const double2 P = row[4 * j];
+        const double pit = P.x * qx, Zt = P.y * qa;
+        const double2 ub = make_double2(keep * a(j).x, keep * a(j).y);
+        const double Ma = Aq * Zt, Mb = Ma + B * pit;
+        const double M0 = is_and ? Ma : Mb, M1 = is_and ? Mb : Ma;
+        const double S = M0 + M1;
+        double Xb = 0.0, Zb = 0.0;
+        if (S != 0.0) {
+            const double dot = ub.x * (M0 / S) + ub.y * (M1 / S);
+            const double Mb0 = (ub.x - dot) / S, Mb1 = (ub.y - dot) / S;
+            const double Mab = is_and ? Mb0 : Mb1, Mbb = is_and ? Mb1 : Mb0;
+            Zb = (Mab + Mbb) * Aq;
+            const double Ab = (Mab + Mbb) * Zt;
+            Xb = Mbb * B;
+            const double Bbar = Mbb * pit;
+            pb += Bbar * (hx - hy);
+            hxb += Bbar * (p - q) + Ab * q;
+            hyb += -Bbar * (p - q) + Ab * (1 - q);
+        }
+        row[4 * j + 1] = make_double2(qx, qa);
+        row[4 * j + 2] = make_double2(Xb, Zb);
+        row[4 * j + 3] = make_double2(ux, ua);
+        ux = ux * x + Xb * qx;
+        ua = ua * A + Zb * qa;
+        qx *= x;
+        qa *= A;
+    }
+    /* pass 3: running sums W and the input adjoints */
+    double wx = 0.0, wa = 0.0;
+    for (int k = 0; k < d; k++) {
+        if (k == h) continue;
+        double x, A;
+        xa(m(k), x, A);
+        const double2 P = row[4 * k], Q = row[4 * k + 1], g = row[4 * k + 2], U = row[4 * k + 3];
+        const double Sx = wx * Q.x + P.x * U.x, SA = wa * Q.y + P.y * U.y;
+        const double xb = Xbh * (P.x * Q.x) + Zbh * (P.y * Q.y) + Sx + SA;
+        const double yb = Zbh * (P.y * Q.y) + SA;
+        out(k, is_and ? make_double2(yb, xb) : make_double2(xb, yb));
+        wx = wx * x + g.x * P.x;
+        wa = wa * A + g.y * P.y;
+    }
+    out(h, is_and ? make_double2(hyb * mk0, hxb * mk1) : make_double2(hxb * mk0, hyb * mk1));
+    return pb;
+}
+
 /* Slots per block of the staged reverse kernels. */
 constexpr int kAdjStage = 512;
 
@@ -394,14 +508,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void fac_vjp_staged
     const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
     const uint32_t *__restrict__ fac_vpos, const uint16_t *__restrict__ work,
     const double2 *__restrict__ m_vf, const double2 *__restrict__ afv, double2 *__restrict__ avf_out,
-    double *__restrict__ pbar, double keep) {
+    double *__restrict__ pbar, double2 *__restrict__ scratch, double keep) {
     __shared__ double2 s_m[kAdjStage];
     __shared__ double2 s_a[kAdjStage];
     __shared__ uint32_t s_start[kBlock + 1];
     const uint32_t f0 = blk[blockIdx.x], f1 = blk[blockIdx.x + 1];
     const uint32_t base = fac_off[f0];
-    const uint32_t n = fac_off[f1] - base;   /* <= kAdjStage: factor degrees are at most kAdjFacDeg */
+    const uint32_t n = fac_off[f1] - base;
     const int tid = threadIdx.x;
+    if (n > (uint32_t)kAdjStage) {
+        /* one factor wider than a staged block: lane 0, from global memory */
+        if (tid == 0) {
+            const uint32_t meta = fac_meta[f0];
+            double mask0 = 1.0, mask1 = 1.0;
+            if (meta & 4u) {
+                const double2 mk = fac_mask[f0];
+                mask0 = mk.x;
+                mask1 = mk.y;
+            }
+            const uint32_t *vp = fac_vpos + base;
+            auto m = [&](int j) { return m_vf[vp[j]]; };
+            auto a = [&](int j) { return afv[vp[j]]; };
+            auto out = [&](int j, double2 r) { avf_out[base + j] = r; };
+            pbar[f0] += fac_vjp_general(m, a, out, scratch + 4ull * base, (int)n, (int)(meta >> 8), meta & 3u,
+                                        (meta & 4u) != 0, fac_p[f0], fac_q[f0], mask0, mask1, keep);
+        }
+        return;
+    }
     const uint32_t nf = f1 - f0;
     uint32_t lf = 0;
     if ((uint32_t)tid < nf) {
@@ -440,7 +573,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void fac_vjp_staged
         case 4: pb = fac_vjp_exact<4>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
         case 5: pb = fac_vjp_exact<5>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
         case 6: pb = fac_vjp_exact<6>(m, a, h, type, clamped, p, q, mask0, mask1, keep); break;
-        default: break;   /* degree 0 has no slots; above kAdjFacDeg is rejected by the host */
+        default:
+            if (d > kAdjFacDeg) {   /* wider factors: the O(d) form, scratch rows at the factor's slots */
+                auto mi = [&](int j) { return m[j]; };
+                auto ai = [&](int j) { return a[j]; };
+                auto out = [&](int j, double2 r) { m[j] = r; };
+                pb = fac_vjp_general(mi, ai, out, scratch + 4ull * (base + s), d, h, type, clamped, p, q, mask0,
+                                     mask1, keep);
+            }
+            break;   /* degree 0 has no slots */
         }
         pbar[f] += pb;
     }

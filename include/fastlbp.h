@@ -473,8 +473,10 @@ uint64_t flbp_batch_device_bytes(const flbp_batch *b);
  * adjoint works on its plan's device buffers (the forward overwrites the
  * plan's messages and beliefs; creation switches the plan to kernel
  * variant 0, whose message layout the reverse kernels read, and the plan
- * must stay there while the adjoint is used).  Factors of degree above 6
- * are rejected (FLBP_ERANGE).  Destroy an adjoint before its plan. */
+ * must stay there while the adjoint is used).  Factors of any degree are
+ * differentiated (degrees up to 6 by exact-degree code, wider ones in
+ * O(degree) with prefix / suffix products).  Destroy an adjoint before its
+ * plan. */
 typedef struct flbp_adjoint flbp_adjoint;
 
 int flbp_adjoint_create(flbp_plan *plan, uint64_t n_sweeps, uint32_t checkpoint_every,

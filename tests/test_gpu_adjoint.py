@@ -167,3 +167,62 @@ def test_sparse_backward_and_device_rule_grad():
         assert close(dev, host)
         assert same_bits(dev, adj.rule_grad())
         prob.plan.reset_head_clamps()
+
+
+@pytest.mark.parametrize("seed,damping,T,k", [(21, 0.4, 15, 5), (22, 0.0, 12, 12)])
+def test_gradient_of_high_degree_factors_matches_oracle(seed, damping, T, k):
+    """OR heads over up to 14 clauses and AND bodies of up to 10 variables:
+    factors above the exact-degree code (degree > 6) take the O(degree)
+    reverse path; the gradient equals the oracle's unrolled reverse mode
+    (itself pinned by central differences on such factors)."""
+    n_rules = 12
+    gen = fl.generate(2500, seed, n_rules=n_rules, n_alarms=60, k_max=14, b_max=10)
+    g_ = fl.Graph(gen.factors)
+    assert np.diff(g_.fac_off).max() > 6
+    rng = np.random.default_rng(seed)
+    theta = rng.normal(-1.0, 0.3, n_rules)
+    y = rng.integers(0, 2, len(gen.alarms)).astype(np.float64)
+    prob = TiedProblem(g_, gen.rule, n_rules, gen.alarms, T, k, damping)
+    L, g = loss_and_grad(prob, theta, y)
+    o = oracle_forward(gen, theta, T, damping)
+    assert same_bits(prob.adj.forward(), o.beliefs())
+    Lo, go = oracle.grad(gen.factors, gen.rule, n_rules, theta, gen.alarms, y, T, damping)
+    assert L == Lo
+    assert np.max(np.abs(go)) > 1e-3
+    assert close(g, go, rel=1e-9), np.max(np.abs(g - go))
+
+
+def test_gradient_of_a_wide_factor_and_variable_vs_central_differences():
+    """An OR over 700 clauses (wider than a staged block of the reverse
+    kernels) and a leaf used by 1500 ANDs; the 1500 ANDs tied to 6 rules.
+    The oracle's unrolled reverse mode is cubic in the degree, so the pin is
+    central differences of the GPU forward (h = 1e-6)."""
+    facs = [(0, 0, [], 0.7), (0, 1, [], 0.6)]
+    nxt, clauses, rule = 2, [], [-1, -1]
+    for i in range(1500):
+        facs.append((1, nxt, [0, 1] if i % 3 else [0], 0.5, 0.01))
+        rule.append(i % 6)
+        if i < 700:
+            clauses.append(nxt)
+        nxt += 1
+    facs.append((2, nxt, clauses, 1.0, 0.0))
+    rule.append(-1)
+    flist = fl.FactorList.build(facs)
+    rule = np.array(rule, np.int32)
+    g_ = fl.Graph(flist)
+    assert np.diff(g_.fac_off).max() == 701 and np.diff(g_.var_off).max() == 1501
+    labels = np.array([nxt, 3, 800, 1499], np.uint64)
+    rng = np.random.default_rng(5)
+    theta = rng.normal(-3.0, 0.3, 6)
+    y = np.array([1.0, 0.0, 1.0, 0.0])
+    prob = TiedProblem(g_, rule, 6, labels, 12, 4, 0.3)
+    L, g = loss_and_grad(prob, theta, y)
+    h = 1e-6
+    fd = np.zeros(6)
+    for r in range(6):
+        tp, tm = theta.copy(), theta.copy()
+        tp[r] += h
+        tm[r] -= h
+        fd[r] = (seq_loss(prob.marginals(tp), y) - seq_loss(prob.marginals(tm), y)) / (2 * h)
+    assert np.max(np.abs(fd)) > 1e-4
+    assert close(g, fd, rel=1e-6), (g, fd)

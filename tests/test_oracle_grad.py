@@ -41,3 +41,30 @@ def test_gradient_is_zero_for_unused_rules():
     L, g = oracle.grad(gen.factors, gen.rule, 500, theta, gen.alarms, targets, 10, 0.5)
     unused = np.setdiff1d(np.arange(500), gen.rule[gen.rule >= 0])
     assert unused.size > 0 and np.all(g[unused] == 0.0)
+
+
+def test_reverse_mode_on_high_degree_factors():
+    """OR heads over up to 14 clauses and AND bodies of up to 10 variables
+    (factor degrees up to 15): the reverse mode still matches central
+    differences -- the reference the GPU adjoint's general-degree factor
+    code is checked against.  Rule probabilities near sigmoid(-1) keep the
+    wide noisy-ORs away from saturation, so the gradient is not tiny."""
+    n_rules = 8
+    gen = fl.generate(300, 21, n_rules=n_rules, n_alarms=30, k_max=14, b_max=10)
+    assert np.max(np.diff(gen.factors.body_off)) >= 10
+    rng = np.random.default_rng(21)
+    theta = rng.normal(-1.0, 0.3, n_rules)
+    targets = rng.integers(0, 2, len(gen.alarms)).astype(np.float64)
+    args = (gen.factors, gen.rule, n_rules)
+    L, g = oracle.grad(*args, theta, gen.alarms, targets, 15, 0.4)
+    h = 1e-6
+    fd = np.zeros(n_rules)
+    for r in range(n_rules):
+        tp, tm = theta.copy(), theta.copy()
+        tp[r] += h
+        tm[r] -= h
+        fd[r] = (oracle.loss(*args, tp, gen.alarms, targets, 15, 0.4) -
+                 oracle.loss(*args, tm, gen.alarms, targets, 15, 0.4)) / (2 * h)
+    scale = np.max(np.abs(fd))
+    assert scale > 1e-6
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-6 * scale)
