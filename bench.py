@@ -229,9 +229,10 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_adjoint(args):
-    """100k variables, 200 tied rules, 1000 alarms, T = 50, damping 0.5,
-    checkpoint every 10: forward and backward device time."""
+def run_adjoint(steps=10, warmup=2):
+    """C4: 100k variables, 200 tied rules, 1000 alarms, T = 50, damping 0.5,
+    checkpoint every 10: forward and backward device time (HIP events,
+    median over the timed steps)."""
     import numpy as np
     import fastlbp_amd as fl
     from fastlbp_amd.adjoint import TiedProblem
@@ -243,24 +244,38 @@ def bench_adjoint(args):
     g = fl.Graph(gen.factors)
     prob = TiedProblem(g, gen.rule, 200, gen.alarms, 50, 10, 0.5)
     fw, bw = [], []
-    for i in range(args.warmup + args.steps):
+    for i in range(warmup + steps):
         b = prob.marginals(theta)
         prob.grad(2.0 * (b - y))
         f_ms, b_ms = prob.adj.last_timing()
-        if i >= args.warmup:
+        if i >= warmup:
             fw.append(f_ms)
             bw.append(b_ms)
     f_ms, b_ms = float(np.median(fw)), float(np.median(bw))
-    msgs = 2.0 * g.n_edges * 50
     T, k = 50, 10
     b_iter = 84.0 * g.n_edges + 24.0 * g.n_factors + 20.0 * g.n_vars   # one damped sweep
     alg_fwd = T * b_iter + -(-T // k) * 32.0 * g.n_edges              # sweeps + checkpoints
     alg_bwd = 3.0 * T * b_iter                                           # recompute + reverse
-    achieved = alg_bwd / (b_ms * 1e-3) / 1e9
+    log(f"[bench] adjoint: forward {f_ms:.3f} ms, backward {b_ms:.3f} ms, ratio {b_ms / f_ms:.2f}")
+    return {"forward_ms": f_ms, "backward_ms": b_ms, "backward_over_forward": b_ms / f_ms,
+            "gradients_per_s": 1e3 / (f_ms + b_ms), "device_bytes": prob.adj.device_bytes,
+            "n_vars": g.n_vars, "n_factors": g.n_factors, "n_edges": g.n_edges, "T": T, "checkpoint_every": k,
+            "alg_bytes_forward": alg_fwd, "alg_bytes_backward": alg_bwd,
+            "msg_updates_per_s_forward": 2.0 * g.n_edges * T / (f_ms * 1e-3),
+            "_problem": (gen, theta, y)}
+
+
+def bench_adjoint(args):
+    """The C4 workload alone: one JSON line for the adjoint."""
+    r = run_adjoint(args.steps, args.warmup)
+    gen, theta, y = r.pop("_problem")
+    f_ms, b_ms = r["forward_ms"], r["backward_ms"]
+    achieved = r["alg_bytes_backward"] / (b_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": "adjoint backward (segment recompute + reverse sweeps)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": alg_bwd,
-                "forward_achieved": round(alg_fwd / (f_ms * 1e-3) / 1e9, 1),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": r["alg_bytes_backward"],
+                "forward_achieved": round(r["alg_bytes_forward"] / (f_ms * 1e-3) / 1e9, 1),
                 "timing": "HIP events around the adjoint's forward and backward (median of the timed steps); "
                           "bytes: T sweeps of 84 E + 24 F + 20 V forward plus 32 E per checkpoint, three times "
                           "the sweeps backward"}
@@ -268,21 +283,19 @@ def bench_adjoint(args):
     if not args.no_cpu:
         import oracle
         t0 = time.perf_counter()
-        oracle.grad(gen.factors, gen.rule, 200, theta, gen.alarms, y, T, 0.5)
+        oracle.grad(gen.factors, gen.rule, 200, theta, gen.alarms, y, r["T"], 0.5)
         cdt = time.perf_counter() - t0
         cpu = {"value": 1.0 / cdt, "unit": "gradients/s", "cores": 1, "kind": "port",
-               "sample": f"one orc_grad (unrolled forward + reverse mode, T = {T}) of the same problem, "
+               "sample": f"one orc_grad (unrolled forward + reverse mode, T = {r['T']}) of the same problem, "
                          f"{cdt:.2f} s, one thread"}
         log(f"[bench] cpu baseline: one gradient in {cdt:.2f} s (1 thread); GPU {f_ms + b_ms:.2f} ms")
-    out = {"metric": METRIC, "value": msgs / (f_ms * 1e-3), "unit": "msg-updates/s", "n_gpus": 1,
+    out = {"metric": METRIC, "value": r["msg_updates_per_s_forward"], "unit": "msg-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": f_ms + b_ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "adjoint-100k-T50", "n_vars": g.n_vars, "n_edges": g.n_edges,
-                      "n_rules": 200, "n_alarms": int(len(gen.alarms)), "T": 50, "checkpoint_every": 10,
-                      "damping": 0.5},
-           "adjoint": {"forward_ms": f_ms, "backward_ms": b_ms, "backward_over_forward": b_ms / f_ms,
-                       "gradients_per_s": 1e3 / (f_ms + b_ms), "device_bytes": prob.adj.device_bytes},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "config": {"workload": "adjoint-100k-T50", "n_vars": r["n_vars"], "n_factors": r["n_factors"],
+                      "n_edges": r["n_edges"], "n_rules": 200, "n_alarms": int(len(gen.alarms)), "T": r["T"],
+                      "checkpoint_every": r["checkpoint_every"], "damping": 0.5},
+           "adjoint": r, "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
@@ -637,6 +650,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="plain workload only: skip the batched and adjoint summaries of the default line")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["plain", "batched", "adjoint", "c5", "seq", "bingo"], default="plain")
     ap.add_argument("--updates", choices=["SEQFIX", "TOPO"], default="SEQFIX", help="seq: the schedule")
@@ -752,6 +767,23 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(gen.factors, w["damping"], args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.4g} msg-updates/s on {cpu['cores']} threads")
+    variant = plan.variant
+    extra = {}
+    if not args.plain_only and not args.n_vars:
+        # the other two headline forms, measured in the same run: the C3
+        # batched solve against sequential plain solves, and the C4 adjoint
+        del plan
+        c3 = run_c3(args.variants_per_group, args.tune_cache)
+        extra["batched"] = {k: c3[k] for k in ("speedup_vs_sequential", "seconds", "sequential_seconds",
+                                                "msg_updates_per_s", "iterations_identical", "n_variants",
+                                                "variants_per_group", "sum_edges", "frac_hbm", "max_sweeps")}
+        extra["batched"]["workload"] = "512 graphs (V log-uniform in [1k, 200k]) + 256 clamped variants of a " \
+                                       "200k-variable graph, damping 0.3, tol 1e-9, maxiter 500"
+        ad = run_adjoint()
+        ad.pop("_problem")
+        extra["adjoint"] = {k: ad[k] for k in ("backward_over_forward", "forward_ms", "backward_ms",
+                                               "gradients_per_s", "n_vars", "n_edges", "T", "checkpoint_every")}
+        extra["adjoint"]["workload"] = "100k variables, 200 tied rules, 1000 alarms, damping 0.5"
     out = {"metric": METRIC, "value": value, "unit": "msg-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -760,9 +792,9 @@ def main():
                       "damping": w["damping"], "tol": w["tol"], "maxiter": w["maxiter"],
                       "seed": w["seed"], "sweeps_per_solve": iters_per_solve,
                       "kernels": ["lds-staged-pull", "lane-per-item", "lds-staged-push", "lds-staged-scatter",
-                                  "lds-staged-state"][plan.variant],
+                                  "lds-staged-state"][variant],
                       "step": "init + run() to convergence of the resident graph"},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, **extra}
     print(json.dumps(out), flush=True)
 
 

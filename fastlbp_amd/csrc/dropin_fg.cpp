@@ -130,6 +130,8 @@ void FactorGraph::ReadFromFile(const char *filename) {
  * read position (read_annotation), so a comment after a blank line is a
  * parse error.  Unlisted entries are 0.  Where the reference reads on from
  * a failed extraction, this reader stops with LOG_FATAL. */
+constexpr size_t kMaxTable = size_t(1) << 28;
+
 std::istream &operator>>(std::istream &is, FactorGraph &fg) {
     std::string line;
     size_t n_factors = 0;
@@ -140,22 +142,42 @@ std::istream &operator>>(std::istream &is, FactorGraph &fg) {
     if (is.fail() || !line.empty()) LOG_FATAL(".fg: expected only the number of factors on the first line");
     std::map<long, size_t> dims_of;
     std::vector<Factor> facs;
-    facs.reserve(n_factors);
+    /* counts come from the file: storage grows as entries are read, so a
+     * damaged count ends in a failed read, not in a huge allocation */
     for (size_t I = 0; I < n_factors; I++) {
         size_t n_members = 0;
         read_annotation(is);
         is >> n_members;
-        std::vector<long> labels(n_members);
-        for (auto &l : labels) {
+        if (is.fail()) LOG_FATAL(".fg: factor %zu: bad member count", I);
+        std::vector<long> labels;
+        for (size_t m = 0; m < n_members; m++) {
+            long l = 0;
             read_annotation(is);
             is >> l;
+            if (is.fail()) LOG_FATAL(".fg: factor %zu: bad member list", I);
+            labels.push_back(l);
         }
-        std::vector<size_t> dims(n_members);
-        for (auto &d : dims) {
+        std::vector<size_t> dims;
+        size_t table = 1;
+        for (size_t m = 0; m < n_members; m++) {
+            size_t d = 0;
             read_annotation(is);
             is >> d;
+            if (is.fail()) LOG_FATAL(".fg: factor %zu: bad member list", I);
+            /* a variable has at least one state; a table beyond 2^28
+             * entries (2 GiB of doubles) is refused */
+            if (d == 0 || d > kMaxTable / table)
+                LOG_FATAL(".fg: factor %zu: %zu states for member %zu (table limit %zu entries)", I, d, m,
+                          kMaxTable);
+            table *= d;
+            dims.push_back(d);
         }
-        if (is.fail()) LOG_FATAL(".fg: factor %zu: bad member list", I);
+        {   /* a label listed twice would index past the (merged) table */
+            std::vector<long> sorted_labels(labels);
+            std::sort(sorted_labels.begin(), sorted_labels.end());
+            if (std::adjacent_find(sorted_labels.begin(), sorted_labels.end()) != sorted_labels.end())
+                LOG_FATAL(".fg: factor %zu lists a variable twice", I);
+        }
         std::vector<Var> members;
         members.reserve(n_members);
         for (size_t m = 0; m < n_members; m++) {

@@ -644,6 +644,17 @@ int flbp_read_graph(const char *path, flbp_graph **out) {
         std::fclose(f);
         return flbp::fail(FLBP_EIO, "flbp_read_graph: header sizes out of range");
     }
+    /* the file's length follows from the header: a damaged size field is
+     * refused here, before anything of that size is allocated */
+    const uint64_t want_len = 32 + 8 * V + 4 * (F + 1) + 4 * E + 4 * (V + 1) + 4 * E + 4 * F + F + F + 32 * F + 8;
+    const long here = std::ftell(f);
+    const bool sized = here >= 0 && std::fseek(f, 0, SEEK_END) == 0 && std::ftell(f) >= 0 &&
+                       (uint64_t)std::ftell(f) == want_len && std::fseek(f, here, SEEK_SET) == 0;
+    if (!sized) {
+        std::fclose(f);
+        return flbp::fail(FLBP_EIO, std::string("flbp_read_graph: ") + path +
+                                        " is truncated or corrupt (length does not match its header)");
+    }
     auto *g = new (std::nothrow) flbp_graph();
     if (!g) {
         std::fclose(f);

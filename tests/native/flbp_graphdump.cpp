@@ -12,12 +12,23 @@
  *                                       factor "n labels... | table...")
  *        flbp_graphdump --properties
  *        flbp_graphdump --index        (IndexFor / Permute on fixed inputs)
+ *        flbp_graphdump --graph <file> (binary graph file: flbp_read_graph;
+ *                                       prints "V F E" or the error)
+ *        flbp_graphdump --schedules <file.causal_fg>
+ *                                      (C-ABI reader, graph build and the
+ *                                       SEQFIX and TOPO schedules; prints
+ *                                       sizes or the error)
+ *
+ * A test tool, not part of the library: the CPU tests compare its dumps
+ * with the reference's graph model, and the sanitizer target of the
+ * Makefile beside it replays damaged input files through it.
  */
 #include <cstdio>
 #include <fstream>
 #include <map>
 #include <string>
 
+#include <fastlbp.h>
 #include <ext/causal_fg.h>
 #include <lbp/factorgraph.h>
 #include <lbp/index.h>
@@ -72,9 +83,65 @@ static int dump_fg(const char *path) {
     return 0;
 }
 
+/* the binary graph reader: prints the sizes, or the status and message */
+static int read_graph(const char *path) {
+    flbp_graph *g = nullptr;
+    const int st = flbp_read_graph(path, &g);
+    if (st != FLBP_OK) {
+        std::printf("error %d %s\n", st, flbp_last_error());
+        return 1;
+    }
+    flbp_csr c;
+    flbp_graph_csr(g, &c);
+    std::printf("%llu %llu %llu\n", (unsigned long long)c.n_vars, (unsigned long long)c.n_factors,
+                (unsigned long long)c.n_edges);
+    flbp_graph_free(g);
+    return 0;
+}
+
+/* the C-ABI .causal_fg reader, the graph build and both sequential schedules */
+static int schedules(const char *path) {
+    flbp_flist *l = nullptr;
+    int st = flbp_read_causal_fg(path, &l);
+    if (st != FLBP_OK) {
+        std::printf("error %d %s\n", st, flbp_last_error());
+        return 1;
+    }
+    flbp_factor_list view;
+    flbp_flist_view(l, &view);
+    flbp_graph *g = nullptr;
+    st = flbp_graph_build(&view, &g);
+    flbp_flist_free(l);
+    if (st != FLBP_OK) {
+        std::printf("error %d %s\n", st, flbp_last_error());
+        return 1;
+    }
+    flbp_csr c;
+    flbp_graph_csr(g, &c);
+    std::printf("%llu %llu %llu", (unsigned long long)c.n_vars, (unsigned long long)c.n_factors,
+                (unsigned long long)c.n_edges);
+    for (int upd : {FLBP_UPDATES_SEQFIX, FLBP_UPDATES_TOPO}) {
+        flbp_schedule *s = nullptr;
+        st = flbp_schedule_create(&c, upd, &s);
+        if (st != FLBP_OK) {
+            std::printf(" error %d", st);
+            continue;
+        }
+        flbp_schedule_info info;
+        flbp_schedule_get_info(s, &info);
+        std::printf(" %llu/%llu", (unsigned long long)info.n_order, (unsigned long long)info.n_levels);
+        flbp_schedule_destroy(s);
+    }
+    std::printf("\n");
+    flbp_graph_free(g);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc >= 2 && std::string(argv[1]) == "--properties") return properties_check();
     if (argc >= 2 && std::string(argv[1]) == "--index") return index_check();
+    if (argc >= 3 && std::string(argv[1]) == "--graph") return read_graph(argv[2]);
+    if (argc >= 3 && std::string(argv[1]) == "--schedules") return schedules(argv[2]);
     if (argc >= 2) {
         const std::string f = argv[1];
         if (f.size() > 3 && f.compare(f.size() - 3, 3, ".fg") == 0) return dump_fg(argv[1]);

@@ -11,7 +11,7 @@ import oracle
 from tests.test_graph import golden_cases
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "fastlbp_amd", "bin")
+BIN = os.path.join(ROOT, "tests", "native", "_bin")
 
 
 def run(*args):

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DUMP = os.path.join(ROOT, "fastlbp_amd", "bin", "flbp_graphdump")
+DUMP = os.path.join(ROOT, "tests", "native", "_bin", "flbp_graphdump")
 
 
 def dump(path):
