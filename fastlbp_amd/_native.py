@@ -169,6 +169,8 @@ SIGNATURES = {
     "flbp_plan_run_fixed": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(RunResultC)]),
     "flbp_plan_run_hist": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double,
                                      C.c_double, C.POINTER(HistResultC), C.POINTER(C.c_double)]),
+    "flbp_plan_run_hist_noalloc": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double,
+                                     C.c_double, C.POINTER(HistResultC), C.POINTER(C.c_double)]),
     "flbp_plan_beliefs": (C.c_int, [C.c_void_p, f64p, C.c_uint64]),
     "flbp_plan_reset": (C.c_int, [C.c_void_p]),
     "flbp_plan_get_messages": (C.c_int, [C.c_void_p, f64p, f64p]),

@@ -13,13 +13,17 @@
  * their order are the reference's; the mean adds the stored values oldest
  * first and divides by their number, as HostBeliefHistory::mean does, so
  * the low-pass beliefs are its bits.
+ *
+ * Two entry points share this: flbp_plan_run_hist allocates the history
+ * per call; flbp_plan_run_hist_noalloc takes it from a fixed array in
+ * device memory (`scratch` below) and allocates nothing.
  */
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <cmath>
 #include <limits>
-#include <vector>
+#include <mutex>
 
 #include "fastlbp.h"
 #include "flbp_internal.h"
@@ -69,16 +73,108 @@ __global__ __launch_bounds__(kHistBlock) void hist_mean(const double *__restrict
     out[v] = sum;
 }
 
-/* device buffers of one call, freed on every return */
-struct HistBufs {
+/* u32 array <- 0 (the history's counters; a kernel rather than a memset) */
+__global__ __launch_bounds__(kHistBlock) void hist_zero_u32(uint32_t *__restrict__ p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * kHistBlock + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void hist_zero_moved(unsigned long long *moved) { *moved = 0ull; }
+
+/* out[perm[dv]] = src[dv]: device order -> variable order */
+__global__ __launch_bounds__(kHistBlock) void hist_unpermute(const double *__restrict__ src,
+                                                             const uint32_t *__restrict__ perm,
+                                                             double *__restrict__ out, uint64_t V) {
+    const uint64_t dv = (uint64_t)blockIdx.x * kHistBlock + threadIdx.x;
+    if (dv < V) out[perm[dv]] = src[dv];
+}
+
+/* The history's device memory: ring [hist_len][V] doubles, prev [V], low
+ * [V], next [V] and count [V] (u32), the moved counter; each range 256-byte
+ * aligned. */
+struct HistDev {
     double *ring = nullptr, *prev = nullptr, *low = nullptr;
     uint32_t *next = nullptr, *count = nullptr;
     unsigned long long *moved = nullptr;
-    ~HistBufs() {
-        for (void *p : {(void *)ring, (void *)prev, (void *)low, (void *)next, (void *)count, (void *)moved})
-            if (p) (void)hipFree(p);
+};
+
+struct HistLayout {
+    uint64_t moved, prev, low, next, count, ring, total;
+};
+
+inline uint64_t up256(uint64_t x) { return (x + 255u) & ~uint64_t(255); }
+
+/* byte offsets of the ranges for V variables and hist_len L; the caller has
+ * checked that 8 * max(V, 1) * L does not overflow */
+HistLayout hist_layout(uint64_t V, uint64_t L) {
+    const uint64_t nb = V ? V : 1;
+    HistLayout h;
+    uint64_t off = 0;
+    h.moved = off;
+    off += 256;
+    h.prev = off;
+    off += up256(8 * nb);
+    h.low = off;
+    off += up256(8 * nb);
+    h.next = off;
+    off += up256(4 * nb);
+    h.count = off;
+    off += up256(4 * nb);
+    h.ring = off;
+    off += up256(8 * nb * L);
+    h.total = off;
+    return h;
+}
+
+HistDev hist_at(unsigned char *base, const HistLayout &h) {
+    HistDev B;
+    B.moved = reinterpret_cast<unsigned long long *>(base + h.moved);
+    B.prev = reinterpret_cast<double *>(base + h.prev);
+    B.low = reinterpret_cast<double *>(base + h.low);
+    B.next = reinterpret_cast<uint32_t *>(base + h.next);
+    B.count = reinterpret_cast<uint32_t *>(base + h.count);
+    B.ring = reinterpret_cast<double *>(base + h.ring);
+    return B;
+}
+
+/* device memory of one flbp_plan_run_hist call, freed on every return */
+struct HistAlloc {
+    unsigned char *base = nullptr;
+    ~HistAlloc() {
+        if (base) (void)hipFree(base);
     }
 };
+
+/* The no-allocation entry point's history: one array per device, resolved
+ * to its device address once per device (hipGetSymbolAddress, with that
+ * device current) and kept for the life of the process.  The array's name
+ * in host code is not a device address: only the resolved pointer is
+ * passed to kernels and copies. */
+constexpr uint64_t kHistScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kHistScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+/* the current device's scratch address (hipSetDevice done by the caller) */
+int scratch_on(int device, unsigned char **out) {
+    if (device < 0 || device >= kMaxDevices)
+        return flbp::fail(FLBP_EINVAL, "flbp_plan_run_hist_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return flbp::fail(FLBP_EHIP, std::string("flbp_plan_run_hist_noalloc: hipGetSymbolAddress: ") +
+                                             hipGetErrorString(e));
+        }
+        g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return FLBP_OK;
+}
 
 }  // namespace
 
@@ -91,31 +187,29 @@ using flbp::grid_for;
         if (e_ != hipSuccess) return flbp::fail(FLBP_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-extern "C" int flbp_plan_run_hist(flbp_plan *plan, double tol, uint64_t min_iters, uint64_t max_iters,
-                                  uint64_t hist_len, double maxtime, double damping, flbp_hist_result *res,
-                                  double *low_pass) {
-    Plan *P = reinterpret_cast<Plan *>(plan);
-    if (!P || !res) return flbp::fail(FLBP_EINVAL, "flbp_plan_run_hist: null argument");
-    if (hist_len == 0) return flbp::fail(FLBP_EINVAL, "flbp_plan_run_hist: belief history length must be positive");
+namespace {
+
+/* argument checks shared by both entry points */
+int hist_args(Plan *P, const flbp_hist_result *res, uint64_t hist_len, const char *who) {
+    if (!P || !res) return flbp::fail(FLBP_EINVAL, std::string(who) + ": null argument");
+    if (hist_len == 0)
+        return flbp::fail(FLBP_EINVAL, std::string(who) + ": belief history length must be positive");
     const uint64_t V = P->V;
-    if (hist_len > 0xffffffffull || (V && hist_len > std::numeric_limits<uint64_t>::max() / 8 / V))
-        return flbp::fail(FLBP_ERANGE, "flbp_plan_run_hist: belief history too large");
-    HIP_TRY(hipSetDevice(P->device));
-    const auto tic = std::chrono::steady_clock::now();
-    const uint32_t L = (uint32_t)hist_len;
-    HistBufs B;
+    if (hist_len > 0xffffffffull || (V && hist_len > std::numeric_limits<uint64_t>::max() / 16 / V))
+        return flbp::fail(FLBP_ERANGE, std::string(who) + ": belief history too large");
+    return FLBP_OK;
+}
+
+/* the loop of causal_bp.cu:1006-1143 on the history B */
+int run_hist_on(Plan *P, double tol, uint64_t min_iters, uint64_t max_iters, uint32_t L, double maxtime,
+                double damping, const HistDev &B, flbp_hist_result *res, double *low_pass) {
+    flbp_plan *plan = reinterpret_cast<flbp_plan *>(P);
+    const uint64_t V = P->V;
     const uint64_t nb = V ? V : 1;
-    if (hipMalloc((void **)&B.ring, nb * hist_len * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&B.prev, nb * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&B.low, nb * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&B.next, nb * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void **)&B.count, nb * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void **)&B.moved, sizeof(unsigned long long)) != hipSuccess) {
-        (void)hipGetLastError();
-        return flbp::fail(FLBP_ENOMEM, "flbp_plan_run_hist: belief history allocation failed");
-    }
-    HIP_TRY(hipMemsetAsync(B.next, 0, nb * sizeof(uint32_t), P->stream));
-    HIP_TRY(hipMemsetAsync(B.count, 0, nb * sizeof(uint32_t), P->stream));
+    const auto tic = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(hist_zero_u32, dim3(grid_for(nb, kHistBlock)), dim3(kHistBlock), 0, P->stream, B.next, nb);
+    hipLaunchKernelGGL(hist_zero_u32, dim3(grid_for(nb, kHistBlock)), dim3(kHistBlock), 0, P->stream, B.count, nb);
+    HIP_TRY(hipGetLastError());
     /* the beliefs before the first sweep, as the reference reads them */
     if (V) HIP_TRY(hipMemcpyAsync(B.prev, P->belief, V * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
     uint64_t num_iters = 0;
@@ -138,7 +232,7 @@ extern "C" int flbp_plan_run_hist(flbp_plan *plan, double tol, uint64_t min_iter
         if (int rc = flbp_plan_run_fixed(plan, 1, damping, &r)) return rc;
         max_diff = r.maxdiff;
         unsigned long long moved = 0;
-        HIP_TRY(hipMemsetAsync(B.moved, 0, sizeof(unsigned long long), P->stream));
+        hipLaunchKernelGGL(hist_zero_moved, dim3(1), dim3(1), 0, P->stream, B.moved);
         if (V)
             hipLaunchKernelGGL(hist_step, dim3(grid_for(V, kHistBlock)), dim3(kHistBlock), 0, P->stream, P->belief,
                                B.prev, B.ring, B.next, B.count, V, L, tol, B.moved);
@@ -159,11 +253,53 @@ extern "C" int flbp_plan_run_hist(flbp_plan *plan, double tol, uint64_t min_iter
             HIP_TRY(hipGetLastError());
             src = B.low;
         }
-        std::vector<double> dev(V);
-        HIP_TRY(hipMemcpyAsync(dev.data(), src, V * sizeof(double), hipMemcpyDeviceToHost, P->stream));
-        HIP_TRY(hipStreamSynchronize(P->stream));
-        for (uint64_t dv = 0; dv < V; dv++) low_pass[P->h_perm[dv]] = dev[dv];
+        /* into variable order on the device (prev is free now), then one copy */
+        hipLaunchKernelGGL(hist_unpermute, dim3(grid_for(V, kHistBlock)), dim3(kHistBlock), 0, P->stream, src,
+                           P->var_perm, B.prev, V);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(low_pass, B.prev, V * sizeof(double), hipMemcpyDeviceToHost, P->stream));
     }
     HIP_TRY(hipStreamSynchronize(P->stream));
     return FLBP_OK;
+}
+
+}  // namespace
+
+extern "C" int flbp_plan_run_hist(flbp_plan *plan, double tol, uint64_t min_iters, uint64_t max_iters,
+                                  uint64_t hist_len, double maxtime, double damping, flbp_hist_result *res,
+                                  double *low_pass) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (int rc = hist_args(P, res, hist_len, "flbp_plan_run_hist")) return rc;
+    HIP_TRY(hipSetDevice(P->device));
+    const HistLayout h = hist_layout(P->V, hist_len);
+    HistAlloc A;
+    if (hipMalloc((void **)&A.base, h.total) != hipSuccess) {
+        A.base = nullptr;
+        (void)hipGetLastError();
+        return flbp::fail(FLBP_ENOMEM, "flbp_plan_run_hist: belief history allocation failed");
+    }
+    return run_hist_on(P, tol, min_iters, max_iters, (uint32_t)hist_len, maxtime, damping, hist_at(A.base, h), res,
+                       low_pass);
+}
+
+extern "C" int flbp_plan_run_hist_noalloc(flbp_plan *plan, double tol, uint64_t min_iters, uint64_t max_iters,
+                                          uint64_t hist_len, double maxtime, double damping, flbp_hist_result *res,
+                                          double *low_pass) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (int rc = hist_args(P, res, hist_len, "flbp_plan_run_hist_noalloc")) return rc;
+    const HistLayout h = hist_layout(P->V, hist_len);
+    if (h.total > kHistScratchBytes)
+        return flbp::fail(FLBP_ERANGE, "flbp_plan_run_hist_noalloc: the belief history needs " +
+                                           std::to_string(h.total) + " bytes, above the fixed " +
+                                           std::to_string(kHistScratchBytes) + " (use flbp_plan_run_hist)");
+    HIP_TRY(hipSetDevice(P->device));
+    unsigned char *base = nullptr;
+    if (int rc = scratch_on(P->device, &base)) return rc;
+    /* sweeps by direct launches: no graph is instantiated here */
+    const bool graphs = P->use_graphs;
+    P->use_graphs = false;
+    const int rc = run_hist_on(P, tol, min_iters, max_iters, (uint32_t)hist_len, maxtime, damping,
+                               hist_at(base, h), res, low_pass);
+    P->use_graphs = graphs;
+    return rc;
 }

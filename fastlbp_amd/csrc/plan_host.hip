@@ -329,6 +329,8 @@ int Plan::upload(const flbp_csr &c) {
     }
     h_inv_perm.assign(V, 0);
     for (uint64_t dv = 0; dv < V; dv++) h_inv_perm[h_perm[dv]] = (uint32_t)dv;
+    TRY(alloc(var_perm, V));
+    if (V) HIP_TRY(hipMemcpy(var_perm, h_perm.data(), V * 4, hipMemcpyHostToDevice));
     /* per solve on one MI355X (generated graphs, E = 4.5 V, damping 0.3):
      * 100k variables (40 MB) scatter 1.20 ms, state 1.31 ms; 300k (120 MB)
      * state 2.03, scatter 2.23; 1M state 5.82 ms, pull 7.40; 3M state 20.7,

@@ -118,6 +118,7 @@ struct Plan {
 
     std::vector<uint64_t> h_var_edge;              /* for message export/import */
     std::vector<uint32_t> h_perm;                  /* device variable -> variable */
+    uint32_t *var_perm = nullptr;                  /* the same on the device */
     std::vector<uint32_t> h_inv_perm;              /* variable -> device variable */
     uint32_t *stage_idx = nullptr;                 /* small index / value uploads (grown on demand) */
     double *stage_val = nullptr;

@@ -221,6 +221,22 @@ typedef struct flbp_hist_result {
 } flbp_hist_result;
 int flbp_plan_run_hist(flbp_plan *plan, double tol, uint64_t min_iters, uint64_t max_iters, uint64_t hist_len,
                        double maxtime, double damping, flbp_hist_result *res, double *low_pass);
+/* flbp_plan_run_hist with no device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory during the
+ * call (serving loops with a fixed memory budget, harnesses that forbid
+ * hipMalloc).  Same arguments, results and low-pass bits as
+ * flbp_plan_run_hist.  The belief history lives in a fixed 64 MiB array of
+ * device memory per GPU, held from the library's first use of that GPU
+ * until the process ends: ring hist_len x V doubles plus 24 bytes per
+ * variable, each range rounded up to 256 bytes; a history beyond 64 MiB
+ * (hist_len <= 5 at V = 1M, 416 at V = 20k) returns FLBP_ERANGE.
+ * Every call on one GPU shares that array and clears what it needs at its
+ * start: calls that overlap on one GPU (from several threads or plans of
+ * the same device) race and must be serialized by the caller.  The sweeps
+ * run as direct launches even where flbp_plan_use_graphs enabled graphs. */
+int flbp_plan_run_hist_noalloc(flbp_plan *plan, double tol, uint64_t min_iters, uint64_t max_iters,
+                               uint64_t hist_len, double maxtime, double damping, flbp_hist_result *res,
+                               double *low_pass);
 
 /* Back to the state of a freshly created plan: iteration counter 0 and old
  * beliefs 0.0 (the reference gets this only by constructing a new
