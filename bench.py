@@ -303,107 +303,119 @@ def bench_c5(args):
     """Tied-rule fitting over clamped variants of one large graph, on one GPU.
 
     A generated graph (20M variables by default, 200 tied AND rules, 1,000
-    alarms).  Each variant clamps 1-50 alarms that have no body uses, to 0 or
-    1, in place on the one plan (head clamps; the structure is unchanged).
-    Targets are the variants' alarm marginals at a reference theta; the fit
-    starts from a perturbed theta.  A gradient step runs, for every variant,
-    the adjoint forward (T damped sweeps from init) and backward, sums
-    dL/dp over the variants on the device (sparse dL/db in, no V- or
-    F-sized transfers) and reduces it to dL/dtheta once, then takes an Adam
-    step.  Afterwards plain LBP runs to convergence at the fitted theta
-    on every variant, against the T-sweep marginals."""
+    alarms) and 64 variants, each clamping 1-50 alarms without body uses to
+    0 or 1.  The variants run in micro-batches of K: one plan holds K
+    interleaved copies of the graph, each clamped as one variant, and one
+    adjoint differentiates the T-sweep damped solve of all K copies
+    (fastlbp_amd.variants.VariantBatch).  Targets are the variants' T-sweep
+    alarm marginals at a reference theta; the fit starts from a perturbed
+    theta.  A gradient step runs, per micro-batch, the forward and the
+    backward, sums dL/dp over all 64 variants on the device (sparse dL/db
+    in) and reduces it to dL/dtheta once, then takes an Adam step.
+    Afterwards the batched solver (flbp_batch, K-interleaved groups, each
+    variant stopping on its own) runs plain LBP to convergence at the fitted
+    theta for all 64 variants, against the T-sweep marginals."""
+    import gc
     import numpy as np
     import fastlbp_amd as fl
-    from fastlbp_amd.adjoint import TiedProblem, tied_p
+    from fastlbp_amd.adjoint import tied_p
+    from fastlbp_amd.batch import Batch, split_variants
+    from fastlbp_amd.variants import VariantBatch
 
     n_vars = args.n_vars or 20_000_000
-    n_var = args.c5_variants
-    T, k, damp, n_rules = args.c5_sweeps, args.c5_checkpoint, 0.5, 200
+    n_var, K = args.c5_variants, args.c5_micro
+    if n_var % K:
+        raise SystemExit("c5: the variant count must be a multiple of the micro-batch size")
+    T, k, damp, n_rules = args.c5_sweeps, args.c5_checkpoint, args.c5_damping, 200
     rng = np.random.default_rng(5)
     t0 = time.perf_counter()
     gen = fl.generate(n_vars, 5, n_rules=n_rules, n_alarms=1000)
     g = fl.Graph(gen.factors)
-    prob = TiedProblem(g, gen.rule, n_rules, gen.alarms, T, k, damp)
-    plan, adj = prob.plan, prob.adj
-    aidx = prob.alarm_idx
-    free = np.flatnonzero(g.body_uses(aidx) == 0)
+    aidx = g.index_of(gen.alarms)
+    free = gen.alarms[g.body_uses(aidx) == 0]
     variants = []
     for _ in range(n_var):
-        pick = rng.choice(free, size=int(rng.integers(1, min(51, len(free) + 1))), replace=False)
-        variants.append((aidx[pick], rng.integers(0, 2, pick.shape[0]).astype(np.uint8)))
-    log(f"[bench] c5 graph V={g.n_vars} F={g.n_factors} E={g.n_edges}, {n_var} variants, "
-        f"setup {time.perf_counter() - t0:.1f}s, device bytes plan {plan.device_bytes / 1e9:.2f} GB "
-        f"+ adjoint {adj.device_bytes / 1e9:.2f} GB")
-
-    adj.set_rules(prob.rule, n_rules)
+        m = int(rng.integers(1, min(51, len(free) + 1)))
+        lab = np.sort(rng.choice(free, size=m, replace=False)).astype(np.uint64)
+        variants.append((lab, rng.integers(0, 2, m).astype(np.uint8)))
+    micro = [variants[i:i + K] for i in range(0, n_var, K)]
+    log(f"[bench] c5 graph V={g.n_vars} F={g.n_factors} E={g.n_edges}, {n_var} variants in micro-batches of {K}, "
+        f"generated in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    vb = VariantBatch(gen.factors, gen.rule, n_rules, gen.alarms, K, T, k, damp)
+    log(f"[bench] c5 {K} interleaved copies: E={vb.graph.n_edges}, built and uploaded in "
+        f"{time.perf_counter() - t0:.1f}s, device bytes {vb.device_bytes / 1e9:.1f} GB "
+        f"(plan {vb.plan.device_bytes / 1e9:.1f} + adjoint {vb.adj.device_bytes / 1e9:.1f})")
 
     def forward_all(theta):
-        """alarm marginals of every variant after T sweeps at theta"""
-        p = tied_p(prob.base_p, prob.rule, theta)
-        plan.set_p(p)
+        vb.set_theta(theta)
         out = []
-        for f, x in variants:
-            plan.clamp_vars(f, x)
-            adj.forward(download=False)
-            out.append(plan.beliefs_at(aidx))
-            plan.reset_head_clamps()
-        return p, out
+        for mb in micro:
+            vb.set_clamps(mb)
+            out.append(vb.forward())
+        return np.concatenate(out)
 
     theta_ref = rng.normal(2.5, 0.3, n_rules)
-    _, targets = forward_all(theta_ref)
+    targets = forward_all(theta_ref)
     theta = theta_ref + rng.normal(0.0, 0.5, n_rules)
-    m, v = np.zeros(n_rules), np.zeros(n_rules)
+    m1, m2 = np.zeros(n_rules), np.zeros(n_rules)
     lr, b1, b2, eps = 0.05, 0.9, 0.999, 1e-8
     losses, fwd_ms, bwd_ms = [], 0.0, 0.0
     t_fit = time.perf_counter()
     for step in range(args.steps):
-        plan.set_p(tied_p(prob.base_p, prob.rule, theta))
-        adj.clear_grad()
+        vb.set_theta(theta)
+        vb.clear_grad()
         loss = 0.0
-        for (f, x), y in zip(variants, targets):
-            plan.clamp_vars(f, x)
-            adj.forward(download=False)
-            b = plan.beliefs_at(aidx)
-            adj.backward_sparse(aidx, 2.0 * (b - y))   # dL/dp summed over the variants on the device
-            a, c = adj.last_timing()
+        for i, mb in enumerate(micro):
+            vb.set_clamps(mb)
+            b = vb.forward()
+            y = targets[i * K:(i + 1) * K]
+            vb.backward(2.0 * (b - y))   # dL/dp of the K variants added on the device
+            a, c = vb.last_timing()
             fwd_ms += a
             bwd_ms += c
-            plan.reset_head_clamps()
             loss += float(np.sum((b - y) ** 2))
-        grad = adj.rule_grad()
-        m = b1 * m + (1 - b1) * grad
-        v = b2 * v + (1 - b2) * grad * grad
-        mh, vh = m / (1 - b1 ** (step + 1)), v / (1 - b2 ** (step + 1))
+        grad = vb.rule_grad()
+        m1 = b1 * m1 + (1 - b1) * grad
+        m2 = b2 * m2 + (1 - b2) * grad * grad
+        mh, vh = m1 / (1 - b1 ** (step + 1)), m2 / (1 - b2 ** (step + 1))
         theta = theta - lr * mh / (np.sqrt(vh) + eps)
         losses.append(loss)
-        log(f"[bench] c5 step {step}: loss {loss:.6g}")
+        log(f"[bench] c5 step {step}: loss {loss:.6g} ({time.perf_counter() - t_fit:.1f}s)")
     t_fit = time.perf_counter() - t_fit
-    # plain LBP at the fitted theta against the T-sweep marginals
-    p_fit, bT = forward_all(theta)
-    errs, iters = [], []
-    for (f, x), b in zip(variants, bT):
-        plan.clamp_vars(f, x)
-        plan.reset()
-        plan.init()
-        r = plan.run(1e-9, 500, damp)
-        iters.append(r.iters)
-        errs.append(float(np.max(np.abs(plan.beliefs_at(aidx) - b))))
-        plan.reset_head_clamps()
+    bT = forward_all(theta)   # T-sweep marginals at the fitted theta
+    dev_bytes = vb.device_bytes
+    vb.close()
+    del vb
+    gc.collect()
+    # plain LBP to convergence at the fitted theta, all variants batched
+    t0 = time.perf_counter()
+    g_fit = fl.Graph(gen.factors.with_p(tied_p(gen.factors.p, gen.rule, theta)))
+    clamps = [(g_fit.index_of(lab), val) for lab, val in variants]
+    batch = Batch(split_variants(g_fit, clamps, args.c5_eval_group))
+    batch.init()
+    iters, _, tot = batch.run(1e-9, 500, damp)
+    errs = np.array([np.max(np.abs(batch.beliefs_at(v, aidx) - bT[v])) for v in range(n_var)])
+    t_eval = time.perf_counter() - t0
+    log(f"[bench] c5 plain LBP at the fitted theta: {tot.seconds:.2f}s batched ({n_var} variants, "
+        f"median {np.median(iters):.0f} sweeps, max {iters.max()}), max |T-sweep - converged| {errs.max():.3g}")
     sweeps = args.steps * n_var * T
     out = {"metric": METRIC, "value": 2.0 * g.n_edges * sweeps / t_fit, "unit": "msg-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": 0, "ms_per_step": t_fit * 1e3 / max(args.steps, 1),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"c5-tied-fit-{g.n_vars}-x{n_var}", "n_vars": g.n_vars, "n_factors": g.n_factors,
-                      "n_edges": g.n_edges, "n_variants": n_var, "n_rules": n_rules, "n_alarms": int(len(aidx)),
-                      "T": T, "checkpoint_every": k, "damping": damp, "optimizer": "adam lr 0.05"},
+                      "n_edges": g.n_edges, "n_variants": n_var, "micro_batch": K, "n_rules": n_rules,
+                      "n_alarms": int(len(aidx)), "T": T, "checkpoint_every": k, "damping": damp,
+                      "optimizer": "adam lr 0.05"},
            "c5": {"steps_per_s": args.steps / t_fit, "loss_first": losses[0] if losses else None,
-                  "loss_last": losses[-1] if losses else None,
-                  "forward_ms_per_variant": fwd_ms / max(1, args.steps * n_var),
-                  "backward_ms_per_variant": bwd_ms / max(1, args.steps * n_var),
-                  "plain_vs_T_sweep_max_abs_err": max(errs) if errs else None,
-                  "plain_vs_T_sweep_mean_max_abs_err": float(np.mean(errs)) if errs else None,
-                  "plain_iters_median": float(np.median(iters)) if iters else None,
-                  "device_bytes": plan.device_bytes + adj.device_bytes}}
+                  "loss_last": losses[-1] if losses else None, "losses": losses,
+                  "forward_ms_per_micro_batch": fwd_ms / max(1, args.steps * len(micro)),
+                  "backward_ms_per_micro_batch": bwd_ms / max(1, args.steps * len(micro)),
+                  "plain_vs_T_sweep_max_abs_err": float(errs.max()),
+                  "plain_vs_T_sweep_mean_max_abs_err": float(errs.mean()),
+                  "plain_iters_median": float(np.median(iters)), "plain_iters_max": int(iters.max()),
+                  "plain_batched_seconds": tot.seconds, "plain_eval_seconds": t_eval,
+                  "device_bytes_fit": dev_bytes, "device_bytes_eval": batch.device_bytes}}
     print(json.dumps(out), flush=True)
 
 
@@ -660,8 +672,12 @@ def main():
     ap.add_argument("--variants-per-group", type=int, default=32,
                     help="batched: variants of one graph per K-interleaved group (1, 2, 4, ..., 64)")
     ap.add_argument("--tune-cache", default=None, help="batched: autotune cache file (JSON, by size class)")
-    ap.add_argument("--c5-sweeps", type=int, default=20, help="c5: sweeps T of the differentiated solve")
-    ap.add_argument("--c5-checkpoint", type=int, default=5, help="c5: checkpoint interval of the adjoint")
+    ap.add_argument("--c5-sweeps", type=int, default=40, help="c5: sweeps T of the differentiated solve")
+    ap.add_argument("--c5-checkpoint", type=int, default=8, help="c5: checkpoint interval of the adjoint")
+    ap.add_argument("--c5-damping", type=float, default=0.3, help="c5: damping of the differentiated solve")
+    ap.add_argument("--c5-micro", type=int, default=4, help="c5: variants per micro-batch (interleaved copies)")
+    ap.add_argument("--c5-eval-group", type=int, default=16,
+                    help="c5: variants per K-interleaved group of the converged evaluation")
     ap.add_argument("--variant", type=int, default=-1,
                     help="sweep kernels: -1 the plan's own choice by footprint (default), 0 LDS-staged (pull), "
                          "1 one lane per item, 2 LDS-staged (push), 3 LDS-staged (scatter), 4 LDS-staged (state)")

@@ -7,12 +7,16 @@
  *
  * The batch holds groups (see batch_kernels.h): a group is one structure
  * and K variants of it, K-interleaved.  Independent graphs are groups of one
- * variant.  A whole solve -- every sweep up to the iteration cap, each a
- * factor pass, a variable pass and the per-variant stop kernel -- is
- * captured once into a hipGraph and replayed with one launch; the stop test
- * and the list of running groups live on the device, so nothing returns to
- * the host between sweeps.  The tile slot bounds are chosen by
- * flbp_batch_autotune.
+ * variant.  Each sweep is the per-variant stop kernel, the active-tile
+ * expansion, a factor pass and a variable pass; the stop test and the list
+ * of running groups live on the device, so no sweep waits for the host.
+ * Sweeps are captured into hipGraphs of `chunk` sweeps (default 8) and
+ * replayed back to back, at most two replays ahead of the host, which
+ * reads one word per replay (how many groups still run) and stops
+ * enqueuing once none does; chunk 0 replays the whole solve, up to the
+ * iteration cap, as one launch (its sweeps after the last variant stopped
+ * find no active tile and exit at once, but each still dispatches its
+ * grid).  The tile slot bounds are chosen by flbp_batch_autotune.
  */
 #include <hip/hip_runtime.h>
 

@@ -41,10 +41,6 @@ enum : int { kVfState = 0, kVfConst = 1, kVfArray = 2 };
 constexpr int kMaxChunk = 64;   /* sweeps enqueued between host checks */
 constexpr uint32_t kStageItems = 256;    /* factors / variables per staged block (== kBlock) */
 constexpr uint32_t kStageSlots = 512;    /* edge slots per staged block (== kStage) */
-/* device footprint up to which the scatter layout is the default: random
- * 16-byte scatters beat random gathers while the messages stay in the
- * 256 MB Infinity Cache, and lose to them once they live in HBM */
-constexpr uint64_t kScatterMaxBytes = 320ull << 20;
 /* a plan's default layout: the scatter layout up to this footprint, the
  * state layout above it (plan_host.hip has the measurements) */
 constexpr uint64_t kStateMinBytes = 64ull << 20;
