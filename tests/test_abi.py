@@ -39,3 +39,22 @@ def test_errors_without_gpu_are_status_codes():
     h = ctypes.c_void_p()
     rc = lib.flbp_graph_build(None, ctypes.byref(h))
     assert rc < 0 and b"bad arguments" in lib.flbp_last_error()
+
+
+def test_size_limits_and_null_arguments_without_gpu():
+    """Checks that come before any device call: sizes beyond the 32-bit
+    index arrays, null handles and empty group lists are status codes."""
+    from fastlbp_amd._native import CsrC, HistResultC
+    lib = fl.lib()
+    FLBP_EINVAL, FLBP_ERANGE = -1, -5
+    big = CsrC()
+    big.n_vars = (1 << 32) - 1
+    h = ctypes.c_void_p()
+    assert lib.flbp_plan_create(ctypes.byref(big), 0, ctypes.byref(h)) == FLBP_ERANGE
+    assert b"2^32-1" in lib.flbp_last_error()
+    big.n_vars, big.n_edges = 10, 1 << 32
+    assert lib.flbp_plan_create(ctypes.byref(big), 0, ctypes.byref(h)) == FLBP_ERANGE
+    assert lib.flbp_batch_create_groups(0, None, 0, None, ctypes.byref(h)) == FLBP_EINVAL
+    res = HistResultC()
+    assert lib.flbp_plan_run_hist_noalloc(None, 1e-6, 0, 10, 4, 1.0, 0.0, ctypes.byref(res), None) == FLBP_EINVAL
+    assert lib.flbp_plan_run_hist(None, 1e-6, 0, 10, 4, 1.0, 0.0, ctypes.byref(res), None) == FLBP_EINVAL
