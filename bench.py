@@ -122,10 +122,12 @@ def c3_workload(variants_per_group=32):
     return gens, graphs, bg, clamps, groups
 
 
-def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=15.0):
+def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=15.0, batch_only=False):
     """Sequential plain solves (one plan per graph, one plan for the 256
     variants with their clamps applied in place; each solve init + run,
-    timed alone) against one batched solve (init excluded)."""
+    timed alone) against one batched solve (init excluded).  batch_only:
+    the batched solve alone, with the default tile bounds (for counter
+    runs: no sequential solves, no autotune trials)."""
     import numpy as np
     import torch
     import fastlbp_amd as fl
@@ -137,8 +139,8 @@ def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=1
         f"sum V={sum(g.n_vars for g in graphs) + 256 * bg.n_vars} "
         f"sum E={sum(g.n_edges for g in graphs) + 256 * bg.n_edges}")
     tol, maxiter, damp = 1e-9, 500, 0.3
-    plans = [fl.Plan(g) for g in graphs]
-    vplan = fl.Plan(bg)
+    plans = [] if batch_only else [fl.Plan(g) for g in graphs]
+    vplan = None if batch_only else fl.Plan(bg)
     t_seq, seq_iters = 0.0, []
     seq_bel = {}
     sample = (0, 255, 511, 512, 512 + 100, 512 + 255)
@@ -151,7 +153,7 @@ def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=1
         t_seq += time.perf_counter() - t1
         if i in sample:
             seq_bel[i] = p.beliefs()
-    for k, (v, x) in enumerate(clamps):
+    for k, (v, x) in enumerate([] if batch_only else clamps):
         vplan.reset_head_clamps()
         vplan.clamp_vars(v, x)
         vplan.reset()
@@ -163,7 +165,7 @@ def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=1
             seq_bel[512 + k] = vplan.beliefs()
     del plans, vplan
     b = Batch(groups)
-    best = b.autotune(damp, trial_sweeps=8, cache=tune_cache)
+    best = b.opts() if batch_only else b.autotune(damp, trial_sweeps=8, cache=tune_cache)
     log(f"[bench] c3: batch opts {best}, device bytes {b.device_bytes / 1e9:.2f} GB")
     b.init()
     torch.cuda.synchronize()
@@ -172,9 +174,15 @@ def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=1
     torch.cuda.synchronize()
     t_bat = time.perf_counter() - t0
     vgraphs = graphs + [bg] * 256
-    same = list(iters) == seq_iters and all(np.array_equal(b.beliefs(i), bel) for i, bel in seq_bel.items())
     msgs = sum(2.0 * g.n_edges * int(it) for g, it in zip(vgraphs, iters))
     alg = sum((84.0 * g.n_edges + 24.0 * g.n_factors + 20.0 * g.n_vars) * int(it) for g, it in zip(vgraphs, iters))
+    if batch_only:
+        log(f"[bench] c3: batched {t_bat * 1e3:.1f} ms, {msgs / t_bat:.4g} msg-updates/s (no sequential solves)")
+        return {"seconds": t_bat, "msg_updates_per_s": msgs / t_bat, "alg_bytes": alg,
+                "frac_hbm": alg / t_bat / 1e9 / HBM_PEAK_GBS, "max_sweeps": int(iters.max()),
+                "n_variants": len(vgraphs), "variants_per_group": variants_per_group, "batch_opts": best,
+                "device_bytes": b.device_bytes, "sum_edges": int(sum(g.n_edges for g in vgraphs))}
+    same = list(iters) == seq_iters and all(np.array_equal(b.beliefs(i), bel) for i, bel in seq_bel.items())
     log(f"[bench] c3: sequential {t_seq * 1e3:.1f} ms, batched {t_bat * 1e3:.1f} ms, "
         f"speedup {t_seq / t_bat:.2f}x, {msgs / t_bat:.4g} msg-updates/s, identical {same}")
     out = {"seconds": t_bat, "sequential_seconds": t_seq, "speedup_vs_sequential": t_seq / t_bat,
@@ -212,7 +220,11 @@ def cpu_threads():
 
 def bench_batched(args):
     """The C3 workload alone: one JSON line for the batched solve."""
-    r = run_c3(args.variants_per_group, args.tune_cache, not args.no_cpu, args.cpu_seconds)
+    r = run_c3(args.variants_per_group, args.tune_cache, not args.no_cpu and not args.batch_only, args.cpu_seconds,
+               args.batch_only)
+    if args.batch_only:
+        print(json.dumps({"batched": r}), flush=True)
+        return
     cpu = r.pop("cpu_baseline", None)
     roofline = {"bound": "hbm", "kernel": "batched factor + var passes (whole run)",
                 "achieved": round(r["alg_bytes"] / r["seconds"] / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -672,6 +684,9 @@ def main():
     ap.add_argument("--variants-per-group", type=int, default=32,
                     help="batched: variants of one graph per K-interleaved group (1, 2, 4, ..., 64)")
     ap.add_argument("--tune-cache", default=None, help="batched: autotune cache file (JSON, by size class)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batched: the batched solve alone, default tile bounds (for counter runs; prints a "
+                         "summary line, not the bench line)")
     ap.add_argument("--c5-sweeps", type=int, default=40, help="c5: sweeps T of the differentiated solve")
     ap.add_argument("--c5-checkpoint", type=int, default=8, help="c5: checkpoint interval of the adjoint")
     ap.add_argument("--c5-damping", type=float, default=0.3, help="c5: damping of the differentiated solve")

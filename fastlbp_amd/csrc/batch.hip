@@ -182,7 +182,8 @@ struct Batch {
 
 /* Tiles of every group for the current slot bounds: a K-variant group's
  * tiles hold at most bound / K slots (and 256 / K variables per variable
- * tile), so each tile fills a block's kStage LDS slots with all K variants;
+ * tile), twice that for K >= kDirectK, so each tile fills a block's LDS
+ * slots (batch_tile_cap) with all K variants;
  * per tile the plain solve's work orders.  A factor wider than its bound
  * gets a tile of its own and a scratch range for its v->f inputs. */
 int Batch::build_tiles() {
@@ -197,8 +198,11 @@ int Batch::build_tiles() {
         const GroupDev &d = hg[g].d;
         const uint32_t *fo = h_fac_off.data() + d.fac_off_base;
         const uint32_t *vo = h_var_off.data() + d.var_off_base;
-        const uint32_t fs = std::max<uint32_t>(1, sf >> d.lk), vs = std::max<uint32_t>(1, sv >> d.lk);
-        const uint32_t vi = std::max<uint32_t>(1, kStageItems >> d.lk);
+        /* a tile of a group of K >= kDirectK variants holds twice the slots
+         * (batch_tile_cap) and up to two variables' records per lane */
+        const uint32_t wide = d.K >= kDirectK ? 2u : 1u;
+        const uint32_t fs = std::max<uint32_t>(1, (wide * sf) >> d.lk), vs = std::max<uint32_t>(1, (wide * sv) >> d.lk);
+        const uint32_t vi = std::max<uint32_t>(1, (wide * kStageItems) >> d.lk);
         std::vector<uint32_t> fb, vb, gfw;
         std::vector<uint16_t> gvw;
         cut_blocks(fo, d.n_fac, kStageItems, fs, fb);
@@ -209,7 +213,7 @@ int Batch::build_tiles() {
         for (size_t b = 0; b + 1 < fb.size(); b++) {
             Tile t{(uint32_t)g, fb[b], fb[b + 1] - fb[b], 0u};
             const uint32_t n = fo[fb[b + 1]] - fo[fb[b]];
-            if (((uint64_t)n << d.lk) > (uint64_t)kStageSlots) {   /* a wide factor: scratch range */
+            if (((uint64_t)n << d.lk) > (uint64_t)batch_tile_cap(d.K)) {   /* a wide factor: scratch range */
                 t.aux = (uint32_t)scr[g];
                 scr[g] += n;
             }
@@ -234,7 +238,7 @@ int Batch::build_tiles() {
         d.vtile_count = (uint32_t)vt[g].size();
         for (Tile &t : ft[g])
             if (((uint64_t)(h_fac_off[d.fac_off_base + t.first + t.count] - h_fac_off[d.fac_off_base + t.first])
-                 << d.lk) > (uint64_t)kStageSlots)
+                 << d.lk) > (uint64_t)batch_tile_cap(d.K))
                 t.aux += (uint32_t)scr_tot;
         scr_tot += scr[g];
         fall.insert(fall.end(), ft[g].begin(), ft[g].end());

@@ -71,6 +71,32 @@ struct Tile {
  * 64-byte line apart (tile i of a group folds into slot i mod kBatchSlots) */
 constexpr int kBatchSlots = 8;
 
+/* Tiles of groups with K >= kDirectK variants hold up to kBatchStage
+ * message slots (all variants) and store their outputs straight from the
+ * compute phase: lanes next to each other compute the same slot of
+ * neighbouring variants, so each run of K outputs is one contiguous
+ * K * 16-byte store, and no LDS staging of outputs is needed.  Tiles of
+ * smaller groups hold kStage slots and stage their outputs in the upper
+ * half of the same LDS buffer, as the plain kernels do (their outputs in
+ * work order would be scattered stores). */
+constexpr uint32_t kDirectK = 4;
+constexpr int kBatchStage = 2 * kStage;
+constexpr int kBatchPerLane = kBatchStage / kBlock;
+
+__host__ __device__ inline uint32_t batch_tile_cap(uint32_t K) { return K >= kDirectK ? kBatchStage : kStage; }
+
+struct BatchFactorSmem {
+    double2 buf[kBatchStage];
+    uint32_t start[kBlock + 1];   /* factor offsets relative to the tile's first slot */
+    uint32_t meta[kBlock];
+    double p[kBlock], q[kBlock];
+};
+
+struct BatchVarSmem {
+    double2 buf[kBatchStage];
+    uint32_t start[kBlock + 1];
+};
+
 struct BatchArgs {
     const GroupDev *__restrict__ groups;
     const Tile *__restrict__ act_ftiles;      /* the running groups' factor tiles, in group order */
@@ -158,7 +184,7 @@ __device__ __forceinline__ void batch_wide_factor(const BatchArgs &A, const Grou
 /* One factor tile of group gr: factors [f0, f1), all K variants. */
 template <bool kDamp, int N>
 __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const GroupDev &gr, const Tile &tile,
-                                                  double damp, double keep, FactorSmem &sm) {
+                                                  double damp, double keep, BatchFactorSmem &sm) {
     const uint32_t *fac_off = A.fac_off + gr.fac_off_base;
     const uint64_t fb = gr.fac_base, eb = gr.edge_base;
     const uint32_t f0 = tile.first, f1 = tile.first + tile.count;
@@ -168,15 +194,18 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
     const uint32_t nk = n << lk;
     const int tid = threadIdx.x;
     double2 *const fv = A.m_fv + gr.msg_base + ((uint64_t)base << lk);
-    if (nk > (uint32_t)kStage) {
+    const bool direct = K >= kDirectK;
+    if (nk > batch_tile_cap(K)) {
         batch_wide_factor<kDamp>(A, gr, tile, damp, keep);
         return;
     }
+    double2 *const s_in = sm.buf;
+    double2 *const s_out = sm.buf + kStage;   /* staged outputs of a tile of K < kDirectK (nk <= kStage) */
     /* phase 1 */
     const uint32_t nf = f1 - f0;
-    uint32_t w[kPerLane], ix[kPerLane];
+    uint32_t w[kBatchPerLane], ix[kBatchPerLane];
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
+    for (int j = 0; j < kBatchPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         w[j] = i < nk ? A.fac_work[eb + base + (i >> lk)] : 0u;
         ix[j] = i < nk ? A.fac_var[eb + base + (i >> lk)] : 0u;
@@ -188,39 +217,30 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
         sm.q[tid] = A.fac_q[fb + f0 + tid];
     }
     if (tid == 0) sm.start[nf] = n;
-    double2 old[kPerLane], pr[kPerLane];
-    uint32_t zc[kPerLane];
-    bool fresh[kPerLane];
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
+    for (int j = 0; j < kBatchPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         if (i < nk) {
             const uint32_t k = i & km;
             const uint64_t v = gr.vst_base + ((uint64_t)ix[j] << lk) + k;
-            old[j] = fv[i];
-            pr[j] = A.vprod[v];
-            zc[j] = A.vzc[v];
-            fresh[j] = A.vfresh[gr.vbase + k] != 0;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
-        const uint32_t i = tid + j * kBlock;
-        if (i < nk) {
-            sm.in[i] = fresh[j] ? make_double2(A.c, A.c) : vf_of(pr[j], zc[j], old[j]);
-            if (kDamp) sm.out[i] = old[j];
+            const double2 old = fv[i];
+            const double2 pr = A.vprod[v];
+            const uint32_t zc = A.vzc[v];
+            const bool fresh = A.vfresh[gr.vbase + k] != 0;
+            s_in[i] = fresh ? make_double2(A.c, A.c) : vf_of(pr, zc, old);
+            if (kDamp && !direct) s_out[i] = old;
         }
     }
     __syncthreads();
     /* phase 2: position i of the tile's outputs is work entry i >> lk of
      * variant i & (K - 1); messages of slot s, variant k at LDS s*K + k */
 #pragma unroll 1
-    for (int j = 0; j < kPerLane; j++) {
+    for (int j = 0; j < kBatchPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         if (i >= nk) break;
         uint32_t wj = 0;
 #pragma unroll
-        for (int x = 0; x < kPerLane; x++) wj |= (j == x) ? w[x] : 0u;
+        for (int x = 0; x < kBatchPerLane; x++) wj |= (j == x) ? w[x] : 0u;
         const uint32_t k = i & km;
         const uint32_t sl = wj & 1023u, lf = wj >> 10;
         const uint32_t s = sm.start[lf];
@@ -238,7 +258,7 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
             mask0 = mk.x;
             mask1 = mk.y;
         }
-        const double2 *fin = sm.in + (s << lk) + k;
+        const double2 *fin = s_in + (s << lk) + k;
         double2 r;
         if (d <= N) {
             const bool head_target = (t == h);
@@ -261,19 +281,33 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
             r = fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
         }
         const uint32_t o = (sl << lk) + k;
-        if (kDamp) {
-            const double2 old2 = sm.out[o];
-            r.x = keep * r.x + damp * old2.x;
-            r.y = keep * r.y + damp * old2.y;
+        if (direct) {
+            /* the slot's old message is read back here, once every lane has
+             * passed phase 1; each output slot belongs to one lane */
+            if (A.vrun[gr.vbase + k]) {
+                if (kDamp) {
+                    const double2 old2 = fv[o];
+                    r.x = keep * r.x + damp * old2.x;
+                    r.y = keep * r.y + damp * old2.y;
+                }
+                fv[o] = r;
+            }
+        } else {
+            if (kDamp) {
+                const double2 old2 = s_out[o];
+                r.x = keep * r.x + damp * old2.x;
+                r.y = keep * r.y + damp * old2.y;
+            }
+            s_out[o] = r;
         }
-        sm.out[o] = r;
     }
+    if (direct) return;
     __syncthreads();
     /* phase 3: the variants that run store their new messages */
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
+    for (int j = 0; j < kStage / kBlock; j++) {
         const uint32_t i = tid + j * kBlock;
-        if (i < nk && A.vrun[gr.vbase + (i & km)]) fv[i] = sm.out[i];
+        if (i < nk && A.vrun[gr.vbase + (i & km)]) fv[i] = s_out[i];
     }
 }
 
@@ -298,7 +332,7 @@ __device__ __forceinline__ void fold_diff(const BatchArgs &A, uint32_t variant, 
 
 /* One variable tile of group gr: variables [v0, v1), all K variants. */
 __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDev &gr, const Tile &tile,
-                                               uint32_t ti, VarSmem &sm) {
+                                               uint32_t ti, BatchVarSmem &sm) {
     const uint32_t *var_off = A.var_off + gr.var_off_base;
     const uint64_t eb = gr.edge_base;
     const uint32_t v0 = tile.first, v1 = tile.first + tile.count;
@@ -309,21 +343,24 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
     const int tid = threadIdx.x;
     const double2 *fv = A.m_fv + gr.msg_base;
     const uint64_t vst = gr.vst_base + ((uint64_t)v0 << lk);
-    if (nk > (uint32_t)kStage) {
-        /* one wide variable: chunks of kStage / K edges of all K variants
+    const bool direct = K >= kDirectK;
+    const uint32_t cap = batch_tile_cap(K);
+    double2 *const s_m = sm.buf;
+    if (nk > cap) {
+        /* one wide variable: chunks of cap / K edges of all K variants
          * staged by the block, lane k multiplying variant k's messages in
          * edge order */
-        const uint32_t C = (uint32_t)kStage >> lk;
+        const uint32_t C = cap >> lk;
         double prod_0 = 1.0, prod_1 = 1.0;
         uint32_t z0 = 0, z1 = 0;
         for (uint32_t c = 0; c < n; c += C) {
             const uint32_t m = n - c < C ? n - c : C;
             for (uint32_t i = tid; i < (m << lk); i += kBlock)
-                sm.m[i] = fv[((uint64_t)A.var_edge[eb + base + c + (i >> lk)] << lk) + (i & km)];
+                s_m[i] = fv[((uint64_t)A.var_edge[eb + base + c + (i >> lk)] << lk) + (i & km)];
             __syncthreads();
             if ((uint32_t)tid < K) {
                 for (uint32_t e = 0; e < m; e++) {
-                    const double2 x = sm.m[(e << lk) + tid];
+                    const double2 x = s_m[(e << lk) + tid];
                     if (x.x == 0) z0++; else prod_0 *= x.x;
                     if (x.y == 0) z1++; else prod_1 *= x.y;
                     scale_pair(prod_0, prod_1);
@@ -343,61 +380,89 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
         }
         return;
     }
+    /* staged results of a tile of K < kDirectK (nk <= kStage), in the upper
+     * half of the buffer: 3 * kBlock doubles and kBlock codes fit its 8 KiB */
+    double *const s_prod0 = reinterpret_cast<double *>(sm.buf + kStage);
+    double *const s_prod1 = s_prod0 + kBlock;
+    double *const s_bel = s_prod1 + kBlock;
+    uint32_t *const s_z = reinterpret_cast<uint32_t *>(s_bel + kBlock);
     /* phase 1 */
-    const uint32_t nv = v1 - v0, nvk = nv << lk;
-    uint32_t ve[kPerLane];
+    const uint32_t nv = v1 - v0, nvk = nv << lk;   /* nvk <= 2 * kBlock (direct), kBlock (staged) */
+    uint32_t ve[kBatchPerLane];
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
+    for (int j = 0; j < kBatchPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         ve[j] = i < nk ? A.var_edge[eb + base + (i >> lk)] : 0u;
     }
-    uint32_t lv = 0;
-    if ((uint32_t)tid < nvk) {
-        lv = A.var_work[gr.var_base + v0 + (tid >> lk)];
-        sm.bel[tid] = A.belief[vst + tid];
+    uint32_t lv[2] = {0u, 0u};
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const uint32_t t = tid + r * kBlock;
+        if (t < nvk) lv[r] = A.var_work[gr.var_base + v0 + (t >> lk)];
     }
+    if (!direct && (uint32_t)tid < nvk) s_bel[tid] = A.belief[vst + tid];
     if ((uint32_t)tid < nv) sm.start[tid] = var_off[v0 + tid] - base;
     if (tid == 0) sm.start[nv] = n;
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
+    for (int j = 0; j < kBatchPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
-        if (i < nk) sm.m[i] = fv[((uint64_t)ve[j] << lk) + (i & km)];
+        if (i < nk) s_m[i] = fv[((uint64_t)ve[j] << lk) + (i & km)];
     }
     __syncthreads();
-    /* phase 2: lane t is work entry t >> lk (a variable) of variant t & (K-1) */
+    /* phase 2: lane t (and t + kBlock in a direct tile) is work entry
+     * t >> lk (a variable) of variant t & (K - 1); both entries of a lane
+     * belong to the same variant, since K divides kBlock */
     unsigned long long diff_bits = 0;
-    if ((uint32_t)tid < nvk) {
-        const uint32_t k = tid & km;
-        const uint32_t s = sm.start[lv];
-        const int d = (int)(sm.start[lv + 1] - s);
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const uint32_t t = tid + r * kBlock;
+        if (t >= nvk) break;
+        const uint32_t k = t & km;
+        const uint32_t s = sm.start[lv[r]];
+        const int d = (int)(sm.start[lv[r] + 1] - s);
         double prod_0 = 1.0, prod_1 = 1.0;
         uint32_t z0 = 0, z1 = 0;
         for (int j = 0; j < d; j++) {
-            const double2 x = sm.m[((s + j) << lk) + k];
+            const double2 x = s_m[((s + j) << lk) + k];
             if (x.x == 0) z0++; else prod_0 *= x.x;
             if (x.y == 0) z1++; else prod_1 *= x.y;
             scale_pair(prod_0, prod_1);
         }
-        const uint32_t o = (lv << lk) + k;
+        const uint32_t o = (lv[r] << lk) + k;
         const double b0 = prod_0 * (double)(z0 == 0);
         const double b1 = prod_1 * (double)(z1 == 0);
         const double b = b1 / (b0 + b1);
-        const double diff = fabs(sm.bel[o] - b);
-        sm.bel[o] = b;
-        diff_bits = (unsigned long long)__double_as_longlong(diff);
-        sm.prod[0][o] = prod_0;
-        sm.prod[1][o] = prod_1;
-        sm.z[o] = pack_zc(z0, z1);
+        if (direct) {
+            /* runs of K neighbouring lanes store K contiguous records */
+            if (A.vrun[gr.vbase + k]) {
+                const double diff = fabs(A.belief[vst + o] - b);
+                A.belief[vst + o] = b;
+                A.vprod[vst + o] = make_double2(prod_0, prod_1);
+                A.vzc[vst + o] = pack_zc(z0, z1);
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(diff);
+                diff_bits = bits > diff_bits ? bits : diff_bits;
+            }
+        } else {
+            const double diff = fabs(s_bel[o] - b);
+            s_bel[o] = b;
+            diff_bits = (unsigned long long)__double_as_longlong(diff);
+            s_prod0[o] = prod_0;
+            s_prod1[o] = prod_1;
+            s_z[o] = pack_zc(z0, z1);
+        }
+        if (!direct) break;
     }
     /* the wave's max change per variant (K <= 64 divides the wave) */
     diff_bits = wave_max_mod(diff_bits, K);
     const uint32_t lane = tid & 63;
-    __syncthreads();
-    /* phase 3: the variants that run store products, codes and beliefs */
-    if ((uint32_t)tid < nvk && A.vrun[gr.vbase + (tid & km)]) {
-        A.vprod[vst + tid] = make_double2(sm.prod[0][tid], sm.prod[1][tid]);
-        A.vzc[vst + tid] = (uint8_t)sm.z[tid];
-        A.belief[vst + tid] = sm.bel[tid];
+    if (!direct) {
+        __syncthreads();
+        /* phase 3: the variants that run store products, codes and beliefs */
+        if ((uint32_t)tid < nvk && A.vrun[gr.vbase + (tid & km)]) {
+            A.vprod[vst + tid] = make_double2(s_prod0[tid], s_prod1[tid]);
+            A.vzc[vst + tid] = (uint8_t)s_z[tid];
+            A.belief[vst + tid] = s_bel[tid];
+        }
     }
     /* per-variant fold: lanes 0..K-1 of each wave hold their variant's max */
     if (lane < K && A.vrun[gr.vbase + lane]) fold_diff(A, gr.vbase + lane, ti, diff_bits);
@@ -412,7 +477,7 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
 template <bool kDamp, int N>
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_pass(BatchArgs A, double damp,
                                                                                     double keep) {
-    __shared__ FactorSmem sm;
+    __shared__ BatchFactorSmem sm;
     const uint32_t ti = blockIdx.x;
     if (ti >= A.act_count[0]) return;
     const Tile tile = A.act_ftiles[ti];
@@ -421,7 +486,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_p
 }
 
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass(BatchArgs A) {
-    __shared__ VarSmem sm;
+    __shared__ BatchVarSmem sm;
     const uint32_t ti = blockIdx.x;
     if (ti >= A.act_count[1]) return;
     const Tile tile = A.act_vtiles[ti];
