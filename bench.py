@@ -532,9 +532,11 @@ def bench_seq(args):
             f"{par[str(damping)]['ms_per_solve']:.2f} ms/solve")
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_seq_seqfix_1M.json")
-    if args.updates == "SEQFIX" and n_vars == WORKLOAD["n_vars"] and os.path.exists(tpath):
+    if args.updates == "SEQFIX" and os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("seq_level")
+            t = json.load(f)
+        if t.get("n_edges") == E:   # measured on this graph
+            traffic = t.get("seq_level")
     out = {"metric": METRIC, "value": value, "unit": "msg-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_solve,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
