@@ -77,7 +77,7 @@ def check_against_oracle(b, gens, tol, maxiter, damping, iters, md, logdomain=Fa
 
 @pytest.mark.parametrize("opts,K", [(dict(), 4), (dict(tile_factors=256, tile_vars=512, grid_blocks=37), 8),
                                     (dict(use_graph=0), 2), (dict(tile_factors=64, tile_vars=96, grid_blocks=3), 4),
-                                    (dict(chunk=5), 16), (dict(chunk=7, use_graph=0), 1)])
+                                    (dict(chunk=5), 16), (dict(chunk=7, use_graph=0), 1), (dict(chunk=0), 4)])
 def test_batch_matches_oracle(opts, K):
     gens, groups = mixed_batch(1, 6, K, 2)
     tol, maxiter, damping = 1e-9, 300, 0.3
@@ -142,7 +142,7 @@ def wide_graph():
     return fl.FactorList.build(facs), nxt
 
 
-@pytest.mark.parametrize("K", [1, 4])
+@pytest.mark.parametrize("K", [1, 2, 4, 64])
 def test_wide_factor_and_variable_in_groups(K):
     flist, head = wide_graph()
     g = fl.Graph(flist)
