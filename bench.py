@@ -15,7 +15,7 @@ bounded number of sweeps of the same graph on this host's cores).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
     python bench.py --workload batched   # 512 graphs + 256 clamped variants vs sequential solves
     python bench.py --workload adjoint   # 100k-variable tied-rule gradient, backward vs forward
-    python bench.py --workload c5 --steps 20   # tied-rule fit over 64 clamped variants of a 20M graph
+    python bench.py --workload fit --steps 20  # tied-rule fit over 64 clamped variants of a 20M graph
     python bench.py --workload seq [--updates TOPO]   # SEQFIX / TOPO solve of the 1M graph vs PARALL
     python bench.py --workload bingo     # Bingo protocol: BP command latency on the 1M graph
 """
@@ -94,8 +94,8 @@ def build_graphs_parallel(specs, workers=16):
         return list(ex.map(lambda f: f(), specs))
 
 
-def c3_workload(variants_per_group=32):
-    """C3: 512 graphs with V log-uniform in [1k, 200k] (generator seeds
+def batch768_workload(variants_per_group=32):
+    """The 768-graph batch: 512 graphs with V log-uniform in [1k, 200k] (generator seeds
     10000 + g) plus 256 clamped variants of one 200k-variable graph (seed
     99000), each clamping 1-50 of its alarms to 0/1 (alarms head no factor
     body, so a clamp keeps the structure).  Returns (generated lists,
@@ -122,7 +122,7 @@ def c3_workload(variants_per_group=32):
     return gens, graphs, bg, clamps, groups
 
 
-def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=15.0, batch_only=False):
+def run_batch768(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=15.0, batch_only=False):
     """Sequential plain solves (one plan per graph, one plan for the 256
     variants with their clamps applied in place; each solve init + run,
     timed alone) against one batched solve (init excluded).  batch_only:
@@ -134,8 +134,8 @@ def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=1
     from fastlbp_amd.batch import Batch
 
     t0 = time.perf_counter()
-    gens, graphs, bg, clamps, groups = c3_workload(variants_per_group)
-    log(f"[bench] c3: built {len(graphs)} graphs + 256 variants of V={bg.n_vars} in {time.perf_counter() - t0:.1f}s; "
+    gens, graphs, bg, clamps, groups = batch768_workload(variants_per_group)
+    log(f"[bench] batch: built {len(graphs)} graphs + 256 variants of V={bg.n_vars} in {time.perf_counter() - t0:.1f}s; "
         f"sum V={sum(g.n_vars for g in graphs) + 256 * bg.n_vars} "
         f"sum E={sum(g.n_edges for g in graphs) + 256 * bg.n_edges}")
     tol, maxiter, damp = 1e-9, 500, 0.3
@@ -166,7 +166,7 @@ def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=1
     del plans, vplan
     b = Batch(groups)
     best = b.opts() if batch_only else b.autotune(damp, trial_sweeps=8, cache=tune_cache)
-    log(f"[bench] c3: batch opts {best}, device bytes {b.device_bytes / 1e9:.2f} GB")
+    log(f"[bench] batch: batch opts {best}, device bytes {b.device_bytes / 1e9:.2f} GB")
     b.init()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -177,13 +177,13 @@ def run_c3(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=1
     msgs = sum(2.0 * g.n_edges * int(it) for g, it in zip(vgraphs, iters))
     alg = sum((84.0 * g.n_edges + 24.0 * g.n_factors + 20.0 * g.n_vars) * int(it) for g, it in zip(vgraphs, iters))
     if batch_only:
-        log(f"[bench] c3: batched {t_bat * 1e3:.1f} ms, {msgs / t_bat:.4g} msg-updates/s (no sequential solves)")
+        log(f"[bench] batch: batched {t_bat * 1e3:.1f} ms, {msgs / t_bat:.4g} msg-updates/s (no sequential solves)")
         return {"seconds": t_bat, "msg_updates_per_s": msgs / t_bat, "alg_bytes": alg,
                 "frac_hbm": alg / t_bat / 1e9 / HBM_PEAK_GBS, "max_sweeps": int(iters.max()),
                 "n_variants": len(vgraphs), "variants_per_group": variants_per_group, "batch_opts": best,
                 "device_bytes": b.device_bytes, "sum_edges": int(sum(g.n_edges for g in vgraphs))}
     same = list(iters) == seq_iters and all(np.array_equal(b.beliefs(i), bel) for i, bel in seq_bel.items())
-    log(f"[bench] c3: sequential {t_seq * 1e3:.1f} ms, batched {t_bat * 1e3:.1f} ms, "
+    log(f"[bench] batch: sequential {t_seq * 1e3:.1f} ms, batched {t_bat * 1e3:.1f} ms, "
         f"speedup {t_seq / t_bat:.2f}x, {msgs / t_bat:.4g} msg-updates/s, identical {same}")
     out = {"seconds": t_bat, "sequential_seconds": t_seq, "speedup_vs_sequential": t_seq / t_bat,
            "msg_updates_per_s": msgs / t_bat, "sequential_msg_updates_per_s": msgs / t_seq,
@@ -219,8 +219,8 @@ def cpu_threads():
 
 
 def bench_batched(args):
-    """The C3 workload alone: one JSON line for the batched solve."""
-    r = run_c3(args.variants_per_group, args.tune_cache, not args.no_cpu and not args.batch_only, args.cpu_seconds,
+    """The 768-graph batch alone: one JSON line for the batched solve."""
+    r = run_batch768(args.variants_per_group, args.tune_cache, not args.no_cpu and not args.batch_only, args.cpu_seconds,
                args.batch_only)
     if args.batch_only:
         print(json.dumps({"batched": r}), flush=True)
@@ -242,7 +242,7 @@ def bench_batched(args):
 
 
 def run_adjoint(steps=10, warmup=2):
-    """C4: 100k variables, 200 tied rules, 1000 alarms, T = 50, damping 0.5,
+    """The tied-rule adjoint: 100k variables, 200 tied rules, 1000 alarms, T = 50, damping 0.5,
     checkpoint every 10: forward and backward device time (HIP events,
     median over the timed steps)."""
     import numpy as np
@@ -278,7 +278,7 @@ def run_adjoint(steps=10, warmup=2):
 
 
 def bench_adjoint(args):
-    """The C4 workload alone: one JSON line for the adjoint."""
+    """The tied-rule adjoint alone: one JSON line for the adjoint."""
     r = run_adjoint(args.steps, args.warmup)
     gen, theta, y = r.pop("_problem")
     f_ms, b_ms = r["forward_ms"], r["backward_ms"]
@@ -311,7 +311,7 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_c5(args):
+def bench_fit(args):
     """Tied-rule fitting over clamped variants of one large graph, on one GPU.
 
     A generated graph (20M variables by default, 200 tied AND rules, 1,000
@@ -335,10 +335,10 @@ def bench_c5(args):
     from fastlbp_amd.variants import VariantBatch
 
     n_vars = args.n_vars or 20_000_000
-    n_var, K = args.c5_variants, args.c5_micro
+    n_var, K = args.fit_variants, args.fit_micro
     if n_var % K:
-        raise SystemExit("c5: the variant count must be a multiple of the micro-batch size")
-    T, k, damp, n_rules = args.c5_sweeps, args.c5_checkpoint, args.c5_damping, 200
+        raise SystemExit("fit: the variant count must be a multiple of the micro-batch size")
+    T, k, damp, n_rules = args.fit_sweeps, args.fit_checkpoint, args.fit_damping, 200
     rng = np.random.default_rng(5)
     t0 = time.perf_counter()
     gen = fl.generate(n_vars, 5, n_rules=n_rules, n_alarms=1000)
@@ -351,11 +351,11 @@ def bench_c5(args):
         lab = np.sort(rng.choice(free, size=m, replace=False)).astype(np.uint64)
         variants.append((lab, rng.integers(0, 2, m).astype(np.uint8)))
     micro = [variants[i:i + K] for i in range(0, n_var, K)]
-    log(f"[bench] c5 graph V={g.n_vars} F={g.n_factors} E={g.n_edges}, {n_var} variants in micro-batches of {K}, "
+    log(f"[bench] fit graph V={g.n_vars} F={g.n_factors} E={g.n_edges}, {n_var} variants in micro-batches of {K}, "
         f"generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
     vb = VariantBatch(gen.factors, gen.rule, n_rules, gen.alarms, K, T, k, damp)
-    log(f"[bench] c5 {K} interleaved copies: E={vb.graph.n_edges}, built and uploaded in "
+    log(f"[bench] fit {K} interleaved copies: E={vb.graph.n_edges}, built and uploaded in "
         f"{time.perf_counter() - t0:.1f}s, device bytes {vb.device_bytes / 1e9:.1f} GB "
         f"(plan {vb.plan.device_bytes / 1e9:.1f} + adjoint {vb.adj.device_bytes / 1e9:.1f})")
 
@@ -393,7 +393,7 @@ def bench_c5(args):
         mh, vh = m1 / (1 - b1 ** (step + 1)), m2 / (1 - b2 ** (step + 1))
         theta = theta - lr * mh / (np.sqrt(vh) + eps)
         losses.append(loss)
-        log(f"[bench] c5 step {step}: loss {loss:.6g} ({time.perf_counter() - t_fit:.1f}s)")
+        log(f"[bench] fit step {step}: loss {loss:.6g} ({time.perf_counter() - t_fit:.1f}s)")
     t_fit = time.perf_counter() - t_fit
     bT = forward_all(theta)   # T-sweep marginals at the fitted theta
     dev_bytes = vb.device_bytes
@@ -404,22 +404,22 @@ def bench_c5(args):
     t0 = time.perf_counter()
     g_fit = fl.Graph(gen.factors.with_p(tied_p(gen.factors.p, gen.rule, theta)))
     clamps = [(g_fit.index_of(lab), val) for lab, val in variants]
-    batch = Batch(split_variants(g_fit, clamps, args.c5_eval_group))
+    batch = Batch(split_variants(g_fit, clamps, args.fit_eval_group))
     batch.init()
     iters, _, tot = batch.run(1e-9, 500, damp)
     errs = np.array([np.max(np.abs(batch.beliefs_at(v, aidx) - bT[v])) for v in range(n_var)])
     t_eval = time.perf_counter() - t0
-    log(f"[bench] c5 plain LBP at the fitted theta: {tot.seconds:.2f}s batched ({n_var} variants, "
+    log(f"[bench] fit plain LBP at the fitted theta: {tot.seconds:.2f}s batched ({n_var} variants, "
         f"median {np.median(iters):.0f} sweeps, max {iters.max()}), max |T-sweep - converged| {errs.max():.3g}")
     sweeps = args.steps * n_var * T
     out = {"metric": METRIC, "value": 2.0 * g.n_edges * sweeps / t_fit, "unit": "msg-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": 0, "ms_per_step": t_fit * 1e3 / max(args.steps, 1),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"c5-tied-fit-{g.n_vars}-x{n_var}", "n_vars": g.n_vars, "n_factors": g.n_factors,
+           "config": {"workload": f"tied-fit-{g.n_vars}-x{n_var}", "n_vars": g.n_vars, "n_factors": g.n_factors,
                       "n_edges": g.n_edges, "n_variants": n_var, "micro_batch": K, "n_rules": n_rules,
                       "n_alarms": int(len(aidx)), "T": T, "checkpoint_every": k, "damping": damp,
                       "optimizer": "adam lr 0.05"},
-           "c5": {"steps_per_s": args.steps / t_fit, "loss_first": losses[0] if losses else None,
+           "fit": {"steps_per_s": args.steps / t_fit, "loss_first": losses[0] if losses else None,
                   "loss_last": losses[-1] if losses else None, "losses": losses,
                   "forward_ms_per_micro_batch": fwd_ms / max(1, args.steps * len(micro)),
                   "backward_ms_per_micro_batch": bwd_ms / max(1, args.steps * len(micro)),
@@ -679,22 +679,22 @@ def main():
     ap.add_argument("--plain-only", action="store_true",
                     help="plain workload only: skip the batched and adjoint summaries of the default line")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["plain", "batched", "adjoint", "c5", "seq", "bingo"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "batched", "adjoint", "fit", "seq", "bingo"], default="plain")
     ap.add_argument("--updates", choices=["SEQFIX", "TOPO"], default="SEQFIX", help="seq: the schedule")
     ap.add_argument("--graphs", action="store_true", help="replay chunks of sweeps from captured hipGraphs")
-    ap.add_argument("--c5-variants", type=int, default=64, help="c5: clamped variants per gradient step")
+    ap.add_argument("--fit-variants", type=int, default=64, help="fit: clamped variants per gradient step")
     ap.add_argument("--variants-per-group", type=int, default=32,
                     help="batched: variants of one graph per K-interleaved group (1, 2, 4, ..., 64)")
     ap.add_argument("--tune-cache", default=None, help="batched: autotune cache file (JSON, by size class)")
     ap.add_argument("--batch-only", action="store_true",
                     help="batched: the batched solve alone, default tile bounds (for counter runs; prints a "
                          "summary line, not the bench line)")
-    ap.add_argument("--c5-sweeps", type=int, default=40, help="c5: sweeps T of the differentiated solve")
-    ap.add_argument("--c5-checkpoint", type=int, default=8, help="c5: checkpoint interval of the adjoint")
-    ap.add_argument("--c5-damping", type=float, default=0.3, help="c5: damping of the differentiated solve")
-    ap.add_argument("--c5-micro", type=int, default=4, help="c5: variants per micro-batch (interleaved copies)")
-    ap.add_argument("--c5-eval-group", type=int, default=16,
-                    help="c5: variants per K-interleaved group of the converged evaluation")
+    ap.add_argument("--fit-sweeps", type=int, default=40, help="fit: sweeps T of the differentiated solve")
+    ap.add_argument("--fit-checkpoint", type=int, default=8, help="fit: checkpoint interval of the adjoint")
+    ap.add_argument("--fit-damping", type=float, default=0.3, help="fit: damping of the differentiated solve")
+    ap.add_argument("--fit-micro", type=int, default=4, help="fit: variants per micro-batch (interleaved copies)")
+    ap.add_argument("--fit-eval-group", type=int, default=16,
+                    help="fit: variants per K-interleaved group of the converged evaluation")
     ap.add_argument("--variant", type=int, default=-1,
                     help="sweep kernels: -1 the plan's own choice by footprint (default), 0 LDS-staged (pull), "
                          "1 one lane per item, 2 LDS-staged (push), 3 LDS-staged (scatter), 4 LDS-staged (state)")
@@ -719,8 +719,8 @@ def main():
         return bench_batched(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
-    if args.workload == "c5":
-        return bench_c5(args)
+    if args.workload == "fit":
+        return bench_fit(args)
     if args.workload == "seq":
         return bench_seq(args)
     if args.workload == "bingo":
@@ -803,11 +803,12 @@ def main():
     variant = plan.variant
     extra = {}
     if not args.plain_only and not args.n_vars:
-        # the other two headline forms, measured in the same run: the C3
-        # batched solve against sequential plain solves, and the C4 adjoint
+        # the other two headline forms, measured in the same run: the
+        # 768-graph batched solve against sequential plain solves, and the
+        # 100k-variable tied-rule adjoint
         del plan
-        c3 = run_c3(args.variants_per_group, args.tune_cache)
-        extra["batched"] = {k: c3[k] for k in ("speedup_vs_sequential", "seconds", "sequential_seconds",
+        b768 = run_batch768(args.variants_per_group, args.tune_cache)
+        extra["batched"] = {k: b768[k] for k in ("speedup_vs_sequential", "seconds", "sequential_seconds",
                                                 "msg_updates_per_s", "iterations_identical", "n_variants",
                                                 "variants_per_group", "sum_edges", "frac_hbm", "max_sweeps")}
         extra["batched"]["workload"] = "512 graphs (V log-uniform in [1k, 200k]) + 256 clamped variants of a " \

@@ -2,7 +2,7 @@
 the oracle's iteration count for its graph (the graph rebuilt with the
 variant's clamps, examples/fl_wrapper.cu:26-45) with bit-identical beliefs
 and last max change, whatever the group sizes, tile bounds, grid size,
-launch mode or hipGraph use; and the C3 workload (512 graphs with V
+launch mode or hipGraph use; and the 768-graph batch (512 graphs with V
 log-uniform in [1k, 200k] plus 256 clamped variants of a 200k-variable
 graph) as one batch."""
 import numpy as np
@@ -203,8 +203,8 @@ def test_set_p_and_beliefs_at():
         assert same_bits(b.beliefs_at(v, aidx), b.beliefs(v)[aidx])
 
 
-def test_c3_workload():
-    """C3: 512 graphs with V log-uniform in [1k, 200k] (seeds 10000 + g) plus
+def test_768_graph_batch():
+    """512 graphs with V log-uniform in [1k, 200k] (seeds 10000 + g) plus
     256 clamped variants (1-50 alarms each to 0/1) of one 200k-variable graph,
     one batch, damping 0.3, tol 1e-9, at most 500 sweeps.  Every variant
     against the oracle where it solves within the test's budget (the graphs

@@ -26,7 +26,7 @@ def parse_marginals(lines):
 
 def test_causal_bp_run_matches_oracle(tmp_path):
     gen = fl.generate(2000, 1)
-    path = str(tmp_path / "c1.causal_fg")
+    path = str(tmp_path / "g2k.causal_fg")
     fl.write_causal_fg(gen.factors, path)
     r = subprocess.run([os.path.join(BIN, "flbp_run"), path, "1e-6", "1000", "0"], check=True,
                        capture_output=True, text=True, timeout=120)

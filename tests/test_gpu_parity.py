@@ -44,7 +44,7 @@ def assert_parity(r, bg, it, md, bo):
     assert (math.isinf(md) and math.isinf(r.maxdiff)) or same_bits([r.maxdiff], [md])
 
 
-def test_c1_plain_fp64():
+def test_2k_variable_graph_plain_fp64():
     """2,000 variables, no damping, tol 1e-6, at most 1000 iterations."""
     gen = fl.generate(2000, 1)
     g, plan, r, bg, o, it, md, bo = solve_both(gen.factors, 1e-6, 1000, 0.0)
@@ -131,7 +131,7 @@ def test_run_semantics_counter_and_init():
     assert same_bits(plan.beliefs(), o.beliefs())
 
 
-def test_c2_full_size_parity():
+def test_1m_variable_graph_full_size_parity():
     """1M variables, damping 0.3, tol 1e-9, at most 500 iterations: the full
     configuration against the oracle (OpenMP)."""
     gen = fl.generate(1_000_000, 2)

@@ -112,8 +112,8 @@ def test_variant_batch_gradient_by_finite_differences():
     vb.close()
 
 
-def test_c5_step_reduced_64_variants_against_oracle():
-    """One C5 gradient step at reduced size: 64 clamped variants in 4
+def test_fit_step_reduced_64_variants_against_oracle():
+    """One gradient step of the tied-rule fit at reduced size: 64 clamped variants in 4
     micro-batches of 16 interleaved copies, dL/dp accumulated on the device
     over all of them; dL/dtheta against the sum of the oracle's gradients
     of the 64 variants, and every variant's T-sweep alarm marginals bit for
@@ -129,7 +129,7 @@ def test_c5_step_reduced_64_variants_against_oracle():
     vb.clear_grad()
     bs = []
     for m in range(64 // K):
-        print(f"[variants] C5 reduced: micro-batch {m}", flush=True)
+        print(f"[variants] fit, reduced: micro-batch {m}", flush=True)
         vb.set_clamps(vs[m * K:(m + 1) * K])
         b = vb.forward()
         vb.backward(2.0 * (b - ys[m * K:(m + 1) * K]))
@@ -153,20 +153,20 @@ def test_c5_step_reduced_64_variants_against_oracle():
     assert close(g, g_ref, 1e-9), np.max(np.abs(g - g_ref))
 
 
-def test_c5_step_full_size_properties():
-    """One gradient step on the C5 graph itself (20M variables, 30M factors,
+def test_fit_step_full_size_properties():
+    """One gradient step of the tied-rule fit on its full graph (20M variables, 30M factors,
     89.7M edges; 2 interleaved variants, T = 40, damping 0.3): marginals
     finite and in [0, 1], every clamped alarm at its clamp value, a small
     step against the gradient lowers the loss, and the gradient matches
     central differences on the two rules with the largest components."""
     n_rules, K, T, k, damping = 200, 2, 40, 8, 0.3
-    print("[variants] C5 full size: generating 20M variables", flush=True)
+    print("[variants] fit, full size: generating 20M variables", flush=True)
     gen = fl.generate(20_000_000, 5, n_rules=n_rules, n_alarms=1000)
     rng = np.random.default_rng(5)
     vs = make_variants(gen, K, rng, max_clamps=50)
     ys = rng.integers(0, 2, (K, len(gen.alarms))).astype(np.float64)
     theta = rng.normal(2.5, 0.3, n_rules)
-    print("[variants] C5 full size: building 2 interleaved copies", flush=True)
+    print("[variants] fit, full size: building 2 interleaved copies", flush=True)
     vb = VariantBatch(gen.factors, gen.rule, n_rules, gen.alarms, K, T, k, damping)
     assert vb.graph.n_edges == 2 * 89_709_144
     vb.set_clamps(vs)
@@ -176,7 +176,7 @@ def test_c5_step_full_size_properties():
         b = vb.forward()
         return float(np.sum((b - ys) ** 2)), b
 
-    print("[variants] C5 full size: forward + backward", flush=True)
+    print("[variants] fit, full size: forward + backward", flush=True)
     L0, b = loss(theta)
     assert np.all(np.isfinite(b)) and np.all((b >= 0) & (b <= 1))
     pos = {int(a): i for i, a in enumerate(gen.alarms)}
@@ -188,13 +188,13 @@ def test_c5_step_full_size_properties():
     g = vb.rule_grad()
     assert np.all(np.isfinite(g)) and np.max(np.abs(g)) > 0
     L1, _ = loss(theta - 1e-3 * g / np.max(np.abs(g)))
-    print(f"[variants] C5 full size: loss {L0:.9g} -> {L1:.9g} after a step against the gradient", flush=True)
+    print(f"[variants] fit, full size: loss {L0:.9g} -> {L1:.9g} after a step against the gradient", flush=True)
     assert L1 < L0
     h = 1e-6
     for r in np.argsort(-np.abs(g))[:2]:
         e = np.zeros(n_rules)
         e[r] = h
         fd = (loss(theta + e)[0] - loss(theta - e)[0]) / (2 * h)
-        print(f"[variants] C5 full size: rule {r}: adjoint {g[r]:.9e} central difference {fd:.9e}", flush=True)
+        print(f"[variants] fit, full size: rule {r}: adjoint {g[r]:.9e} central difference {fd:.9e}", flush=True)
         assert abs(g[r] - fd) <= 1e-5 * abs(fd) + 1e-7 * np.max(np.abs(g)), (r, g[r], fd)
     vb.close()
