@@ -122,7 +122,8 @@ def batch768_workload(variants_per_group=32):
     return gens, graphs, bg, clamps, groups
 
 
-def run_batch768(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=15.0, batch_only=False):
+def run_batch768(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_seconds=15.0, batch_only=False,
+                 batch_opts=None):
     """Sequential plain solves (one plan per graph, one plan for the 256
     variants with their clamps applied in place; each solve init + run,
     timed alone) against one batched solve (init excluded).  batch_only:
@@ -166,6 +167,9 @@ def run_batch768(variants_per_group=32, tune_cache=None, with_cpu=False, cpu_sec
     del plans, vplan
     b = Batch(groups)
     best = b.opts() if batch_only else b.autotune(damp, trial_sweeps=8, cache=tune_cache)
+    if batch_opts:
+        b.set_opts(**batch_opts)
+        best = b.opts()
     log(f"[bench] batch: batch opts {best}, device bytes {b.device_bytes / 1e9:.2f} GB")
     b.init()
     torch.cuda.synchronize()
@@ -221,7 +225,7 @@ def cpu_threads():
 def bench_batched(args):
     """The 768-graph batch alone: one JSON line for the batched solve."""
     r = run_batch768(args.variants_per_group, args.tune_cache, not args.no_cpu and not args.batch_only, args.cpu_seconds,
-               args.batch_only)
+                     args.batch_only, json.loads(args.batch_opts) if args.batch_opts else None)
     if args.batch_only:
         print(json.dumps({"batched": r}), flush=True)
         return
@@ -686,6 +690,8 @@ def main():
     ap.add_argument("--variants-per-group", type=int, default=32,
                     help="batched: variants of one graph per K-interleaved group (1, 2, 4, ..., 64)")
     ap.add_argument("--tune-cache", default=None, help="batched: autotune cache file (JSON, by size class)")
+    ap.add_argument("--batch-opts", default=None,
+                    help='batched: batch options as JSON, set after the autotune (e.g. \'{"chunk": 0}\')')
     ap.add_argument("--batch-only", action="store_true",
                     help="batched: the batched solve alone, default tile bounds (for counter runs; prints a "
                          "summary line, not the bench line)")
