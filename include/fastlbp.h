@@ -427,8 +427,9 @@ int flbp_plan_set_seq_tail(flbp_plan *plan, uint32_t max_items);
 typedef struct flbp_batch flbp_batch;
 
 typedef struct flbp_batch_opts {
-    uint32_t tile_factors;   /* edge slots per factor tile of a 1-variant group, 64..512 (K variants: /K) */
-    uint32_t tile_vars;      /* edge slots per variable tile, 64..512 (/K) */
+    uint32_t tile_factors;   /* edge slots per factor tile of a 1-variant group, 64..512 (K variants: 1/K
+                                of it, 2/K for K >= 4) */
+    uint32_t tile_vars;      /* edge slots per variable tile, 64..512 (same scaling) */
     uint32_t grid_blocks;    /* reserved: the sweep kernels launch one block per tile */
     uint32_t chunk;          /* sweeps per hipGraph launch (default 8): launches are enqueued back to back,
                                 at most two ahead of a finished one, and stop once no variant runs; 0: the
