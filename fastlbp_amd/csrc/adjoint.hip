@@ -352,7 +352,12 @@ int flbp_adjoint_backward(flbp_adjoint *adj, const double *dl_db, double *dl_dp)
     if (V) HIP_TRY(hipMemcpyAsync(A->bbar, dev_db.data(), V * sizeof(double), hipMemcpyHostToDevice, P->stream));
     if (F) hipLaunchKernelGGL(fill1, dim3(grid_for(F, 256)), dim3(256), 0, P->stream, A->pbar, F, 0.0);
     TRY(A->run_backward());
-    if (F) HIP_TRY(hipMemcpyAsync(dl_dp, A->pbar, F * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    if (F) {   /* device factor order -> factor order */
+        std::vector<double> dev_dp(F);
+        HIP_TRY(hipMemcpyAsync(dev_dp.data(), A->pbar, F * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        for (uint64_t I = 0; I < F; I++) dl_dp[I] = dev_dp[P->h_finv[I]];
+    }
     return adjoint_finish(A);
 }
 
@@ -405,9 +410,13 @@ int flbp_adjoint_get_dl_dp(flbp_adjoint *adj, double *dl_dp) {
     Adjoint *A = reinterpret_cast<Adjoint *>(adj);
     if (!A || (A->P->F && !dl_dp)) return flbp::fail(FLBP_EINVAL, "flbp_adjoint_get_dl_dp: null argument");
     HIP_TRY(hipSetDevice(A->device));
-    if (A->P->F)
-        HIP_TRY(hipMemcpyAsync(dl_dp, A->pbar, A->P->F * sizeof(double), hipMemcpyDeviceToHost, A->P->stream));
-    HIP_TRY(hipStreamSynchronize(A->P->stream));
+    const uint64_t F = A->P->F;
+    if (F) {   /* device factor order -> factor order */
+        std::vector<double> dev_dp(F);
+        HIP_TRY(hipMemcpyAsync(dev_dp.data(), A->pbar, F * sizeof(double), hipMemcpyDeviceToHost, A->P->stream));
+        HIP_TRY(hipStreamSynchronize(A->P->stream));
+        for (uint64_t I = 0; I < F; I++) dl_dp[I] = dev_dp[A->P->h_finv[I]];
+    }
     return FLBP_OK;
 }
 
@@ -423,10 +432,12 @@ int flbp_adjoint_set_rules(flbp_adjoint *adj, const int32_t *rule, uint64_t n_ru
         off[rule[I] + 1]++;
     }
     for (uint64_t r = 0; r < n_rules; r++) off[r + 1] += off[r];
+    /* each rule's factors as device indices, listed in factor order (the
+     * order the reduction adds them in) */
     std::vector<uint32_t> fac(off[n_rules]);
     std::vector<uint64_t> pos(off.begin(), off.end() - 1);
     for (uint64_t I = 0; I < F; I++)
-        if (rule[I] >= 0) fac[pos[rule[I]]++] = (uint32_t)I;
+        if (rule[I] >= 0) fac[pos[rule[I]]++] = A->P->h_finv[I];
     HIP_TRY(hipSetDevice(A->device));
     for (void *q : {(void *)A->rule_off, (void *)A->rule_fac, (void *)A->rule_out}) {
         if (!q) continue;

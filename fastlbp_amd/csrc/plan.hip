@@ -591,10 +591,11 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
         if (P->variant == 4)      /* derived from the products: materialized var-major first */
             if (int rc = P->materialize_vf()) return rc;
         HIP_TRY(hipMemcpy(tmp.data(), P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
-        for (uint64_t s = 0; s < P->E; s++) {
+        for (uint64_t r = 0; r < P->E; r++) {   /* reference slot r, device slot s */
+            const uint64_t s = P->h_slot_dev[r];
             const uint64_t k = (P->variant == 2 || P->variant == 3) ? s : P->h_fac_vpos[s];
-            vf_facmajor[2 * s] = tmp[2 * k];
-            vf_facmajor[2 * s + 1] = tmp[2 * k + 1];
+            vf_facmajor[2 * r] = tmp[2 * k];
+            vf_facmajor[2 * r + 1] = tmp[2 * k + 1];
         }
     }
     return FLBP_OK;
@@ -630,10 +631,11 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
     }
     if (vf_facmajor && P->E) {
         std::vector<double> tmp(2 * P->E);
-        for (uint64_t s = 0; s < P->E; s++) {
+        for (uint64_t r = 0; r < P->E; r++) {   /* reference slot r, device slot s */
+            const uint64_t s = P->h_slot_dev[r];
             const uint64_t k = (P->variant == 2 || P->variant == 3) ? s : P->h_fac_vpos[s];
-            tmp[2 * k] = vf_facmajor[2 * s];
-            tmp[2 * k + 1] = vf_facmajor[2 * s + 1];
+            tmp[2 * k] = vf_facmajor[2 * r];
+            tmp[2 * k + 1] = vf_facmajor[2 * r + 1];
         }
         HIP_TRY(hipMemcpy(P->m_vf, tmp.data(), P->E * sizeof(double2), hipMemcpyHostToDevice));
         if (P->variant == 4) P->vf_src = kVfArray;
@@ -646,30 +648,37 @@ int flbp_plan_set_p(flbp_plan *plan, const double *p, uint64_t n) {
     if (!P || (n && !p)) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_p: null argument");
     if (n != P->F) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_p: n must equal n_factors");
     HIP_TRY(hipSetDevice(P->device));
-    if (n) HIP_TRY(hipMemcpy(P->fac_p, p, n * sizeof(double), hipMemcpyHostToDevice));
+    if (n) {
+        std::vector<double> pd(n);   /* device factor order */
+        for (uint64_t d = 0; d < n; d++) pd[d] = p[P->h_fperm[d]];
+        HIP_TRY(hipMemcpy(P->fac_p, pd.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
     if (int rc = flbp::seq_refresh_factors(P)) return rc;
     HIP_TRY(hipStreamSynchronize(P->stream));
     return FLBP_OK;
 }
 
-int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *factors, const uint8_t *values) {
-    Plan *P = reinterpret_cast<Plan *>(plan);
-    if (!P || (n && (!factors || !values))) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: null argument");
+}  // extern "C"
+
+/* head clamps of device factors dev[0..n) (flbp_plan_set_head_clamps and
+ * flbp_plan_clamp_vars); ref names each factor by its reference index in
+ * error messages */
+static int set_head_clamps_dev(Plan *P, uint64_t n, const uint64_t *dev, const uint64_t *ref, const uint8_t *values,
+                               const char *who) {
     for (uint64_t i = 0; i < n; i++) {
-        if (factors[i] >= P->F) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: factor out of range");
-        if ((P->h_meta[factors[i]] & 3u) == 0)
-            return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: a prior factor is clamped through its p "
-                                           "(rebuild the graph with flbp_clamp)");
-        if (P->h_var_body[P->h_fac_head[factors[i]]])
-            return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: the head of factor " +
-                                               std::to_string(factors[i]) +
+        const uint64_t I = dev[i];
+        if ((P->h_meta[I] & 3u) == 0)
+            return flbp::fail(FLBP_EINVAL, std::string(who) + ": a prior factor is clamped through its p "
+                                                              "(rebuild the graph with flbp_clamp)");
+        if (P->h_var_body[P->h_fac_head[I]])
+            return flbp::fail(FLBP_EINVAL, std::string(who) + ": the head of factor " + std::to_string(ref[i]) +
                                                " appears in a factor body; its clamp changes the structure "
                                                "(rebuild the graph with flbp_clamp)");
-        if (values[i] > 1) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: values are 0 or 1");
+        if (values[i] > 1) return flbp::fail(FLBP_EINVAL, std::string(who) + ": values are 0 or 1");
     }
     HIP_TRY(hipSetDevice(P->device));
     for (uint64_t i = 0; i < n; i++) {
-        const uint64_t I = factors[i];
+        const uint64_t I = dev[i];
         const unsigned x = values[i];
         double m0 = x == 0 ? 1.0 : 0.0, m1 = x == 1 ? 1.0 : 0.0;
         if (P->h_meta[I] & 4u) {   /* a second clamp of the head multiplies */
@@ -685,6 +694,19 @@ int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *facto
     if (int rc = flbp::seq_refresh_factors(P)) return rc;
     HIP_TRY(hipStreamSynchronize(P->stream));
     return FLBP_OK;
+}
+
+extern "C" {
+
+int flbp_plan_set_head_clamps(flbp_plan *plan, uint64_t n, const uint64_t *factors, const uint8_t *values) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    if (!P || (n && (!factors || !values))) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: null argument");
+    std::vector<uint64_t> dev(n);
+    for (uint64_t i = 0; i < n; i++) {
+        if (factors[i] >= P->F) return flbp::fail(FLBP_EINVAL, "flbp_plan_set_head_clamps: factor out of range");
+        dev[i] = P->h_finv[factors[i]];
+    }
+    return set_head_clamps_dev(P, n, dev.data(), factors, values, "flbp_plan_set_head_clamps");
 }
 
 int flbp_plan_clamp_vars(flbp_plan *plan, uint64_t n, const uint64_t *vars, const uint8_t *values) {
@@ -703,11 +725,13 @@ int flbp_plan_clamp_vars(flbp_plan *plan, uint64_t n, const uint64_t *vars, cons
         if (P->h_head_off[v] == P->h_head_off[v + 1])
             return flbp::fail(FLBP_EINVAL, "flbp_plan_clamp_vars: variable " + std::to_string(v) + " heads no factor");
         for (uint32_t k = P->h_head_off[v]; k < P->h_head_off[v + 1]; k++) {
-            fac.push_back(P->h_head_fac[k]);
+            fac.push_back(P->h_head_fac[k]);   /* device factor */
             val.push_back(values[i]);
         }
     }
-    return flbp_plan_set_head_clamps(plan, fac.size(), fac.data(), val.data());
+    std::vector<uint64_t> ref(fac.size());
+    for (size_t i = 0; i < fac.size(); i++) ref[i] = P->h_fperm[fac[i]];
+    return set_head_clamps_dev(P, fac.size(), fac.data(), ref.data(), val.data(), "flbp_plan_clamp_vars");
 }
 
 int flbp_plan_reset_head_clamps(flbp_plan *plan) {

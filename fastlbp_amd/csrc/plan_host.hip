@@ -123,6 +123,64 @@ void var_device_order(const flbp_csr &c, int order, std::vector<uint32_t> &perm,
     }
 }
 
+void factor_device_order(const flbp_csr &c, const std::vector<uint32_t> &var_inv, DeviceCsr &out,
+                         std::vector<uint32_t> &fperm, std::vector<uint32_t> &finv, std::vector<uint32_t> &slot_dev,
+                         std::vector<uint32_t> &slot_ref) {
+    const uint64_t V = c.n_vars, F = c.n_factors, E = c.n_edges;
+    /* counting sort of the factors by their head's device index (stable) */
+    std::vector<uint64_t> start(V + 1, 0);
+    for (uint64_t I = 0; I < F; I++) start[var_inv[c.edge_var[c.head_slot[I]]] + 1]++;
+    for (uint64_t v = 0; v < V; v++) start[v + 1] += start[v];
+    fperm.resize(F);
+    finv.resize(F);
+    for (uint64_t I = 0; I < F; I++) {
+        const uint64_t d = start[var_inv[c.edge_var[c.head_slot[I]]]]++;
+        fperm[d] = (uint32_t)I;
+        finv[I] = (uint32_t)d;
+    }
+    out.fac_off.assign(F + 1, 0);
+    out.edge_var.resize(E);
+    out.head_slot.resize(F);
+    out.fac_type.resize(F);
+    out.head_clamped.resize(F);
+    out.p.resize(F);
+    out.q.resize(F);
+    out.mask0.resize(F);
+    out.mask1.resize(F);
+    slot_dev.resize(E);
+    slot_ref.resize(E);
+    for (uint64_t d = 0; d < F; d++) {
+        const uint64_t I = fperm[d], a = c.fac_off[I], n = c.fac_off[I + 1] - a, b = out.fac_off[d];
+        out.fac_off[d + 1] = b + n;
+        for (uint64_t j = 0; j < n; j++) {
+            slot_dev[a + j] = (uint32_t)(b + j);
+            slot_ref[b + j] = (uint32_t)(a + j);
+            out.edge_var[b + j] = c.edge_var[a + j];
+        }
+        out.head_slot[d] = b + (c.head_slot[I] - a);
+        out.fac_type[d] = c.fac_type[I];
+        out.head_clamped[d] = c.head_clamped[I];
+        out.p[d] = c.p[I];
+        out.q[d] = c.q[I];
+        out.mask0[d] = c.mask0[I];
+        out.mask1[d] = c.mask1[I];
+    }
+    out.var_edge.resize(E);
+    for (uint64_t k = 0; k < E; k++) out.var_edge[k] = slot_dev[c.var_edge[k]];
+    flbp_csr &w = out.view;
+    w = c;   /* sizes, labels and var_off are the reference's */
+    w.fac_off = out.fac_off.data();
+    w.edge_var = out.edge_var.data();
+    w.var_edge = out.var_edge.data();
+    w.fac_type = out.fac_type.data();
+    w.head_slot = out.head_slot.data();
+    w.head_clamped = out.head_clamped.data();
+    w.p = out.p.data();
+    w.q = out.q.data();
+    w.mask0 = out.mask0.data();
+    w.mask1 = out.mask1.data();
+}
+
 void cut_blocks(const uint32_t *off, uint64_t n_items, uint32_t max_items, uint32_t max_slots,
                 std::vector<uint32_t> &blk) {
     blk.clear();
@@ -228,7 +286,18 @@ template int Plan::alloc<dev::FacRec>(dev::FacRec *&, uint64_t);
         if (e_ != hipSuccess) return hip_fail(e_, #expr); \
     } while (0)
 
-int Plan::upload(const flbp_csr &c) {
+int Plan::upload(const flbp_csr &c_ref) {
+    /* factors in device order (factor_device_order); everything below works
+     * on that CSR */
+    DeviceCsr dcsr;
+    {
+        std::vector<uint32_t> perm0, off0, edge0, vpos0;
+        var_device_order(c_ref, kOrderHotFirst, perm0, off0, edge0, vpos0);
+        std::vector<uint32_t> inv0(V);
+        for (uint64_t dv = 0; dv < V; dv++) inv0[perm0[dv]] = (uint32_t)dv;
+        factor_device_order(c_ref, inv0, dcsr, h_fperm, h_finv, h_slot_dev, h_slot_ref);
+    }
+    const flbp_csr &c = dcsr.view;
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     TRY(alloc(fac_off, F + 1));
     TRY(alloc(fac_meta, F));

@@ -76,6 +76,26 @@ void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const 
 void var_device_order(const flbp_csr &c, int order, std::vector<uint32_t> &perm,
                       std::vector<uint32_t> &dev_off, std::vector<uint32_t> &dev_edge,
                       std::vector<uint32_t> &vpos);
+/* A CSR with its factors in device order: factors sorted (stably) by the
+ * device index of their head variable, so that a head's factors sit
+ * together and the factor pass walks head variables in device order.  Each
+ * factor keeps its slots in their order and each variable its edges in
+ * their nbV order, so every message and product is computed exactly as in
+ * the reference's order.  The plan works in this order throughout; the
+ * entry points translate factor indices and factor-major slot positions. */
+struct DeviceCsr {
+    std::vector<uint64_t> fac_off, edge_var, var_edge, head_slot;
+    std::vector<int32_t> fac_type;
+    std::vector<uint8_t> head_clamped;
+    std::vector<double> p, q, mask0, mask1;
+    flbp_csr view{};
+};
+/* fperm[d] = the reference factor at device position d, finv its inverse;
+ * slot_dev[s] = device slot of reference slot s, slot_ref its inverse */
+void factor_device_order(const flbp_csr &c, const std::vector<uint32_t> &var_inv, DeviceCsr &out,
+                         std::vector<uint32_t> &fperm, std::vector<uint32_t> &finv, std::vector<uint32_t> &slot_dev,
+                         std::vector<uint32_t> &slot_ref);
+
 struct Plan;
 int seq_reset_state(Plan *P, double c);   /* seq.hip */
 int seq_refresh_factors(Plan *P);         /* seq.hip: factor records after p / clamp changes */
@@ -116,6 +136,8 @@ struct Plan {
     std::vector<uint32_t> h_perm;                  /* device variable -> variable */
     uint32_t *var_perm = nullptr;                  /* the same on the device */
     std::vector<uint32_t> h_inv_perm;              /* variable -> device variable */
+    std::vector<uint32_t> h_fperm, h_finv;         /* device factor <-> factor (factor_device_order) */
+    std::vector<uint32_t> h_slot_dev, h_slot_ref;  /* factor-major slot <-> device slot */
     uint32_t *stage_idx = nullptr;                 /* small index / value uploads (grown on demand) */
     double *stage_val = nullptr;
     uint64_t stage_cap = 0;

@@ -179,8 +179,12 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
             /* f->v items by factor; groups by degree, then factor, so the
              * lanes of a wave mostly run loops of one length */
             uint64_t key = S.ex_a[x];
+            if (kd == flbp::kExecVF) key = P->h_slot_dev[S.ex_a[x]];
             if (kd == flbp::kExecFV) key |= (uint64_t)slot_fac[P->h_var_edge[S.ex_a[x]]] << 32;
-            if (kd == flbp::kExecGroup) key |= (P->h_fac_off[S.ex_a[x] + 1] - P->h_fac_off[S.ex_a[x]]) << 32;
+            if (kd == flbp::kExecGroup) {
+                const uint32_t I = P->h_finv[S.ex_a[x]];
+                key = I | (P->h_fac_off[I + 1] - P->h_fac_off[I]) << 32;
+            }
             keyed[kd][at[kd][S.ex_level[x]]++] = {key, (uint32_t)x};
         }
     }
@@ -188,15 +192,15 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
         for (uint32_t l = 1; l <= nl; l++) std::sort(keyed[kd].begin() + off[kd][l], keyed[kd].begin() + off[kd][l + 1]);
     std::vector<uint4> vf(keyed[0].size()), fv(keyed[1].size()), faux(keyed[1].size());
     for (size_t y = 0; y < keyed[0].size(); y++) {
-        const uint32_t sl = S.ex_a[keyed[0][y].second];
-        vf[y] = make_uint4(sl, P->h_inv_perm[S.slot_var[sl]], 0u, 0u);
+        const uint32_t sl = S.ex_a[keyed[0][y].second];   /* reference slot */
+        vf[y] = make_uint4(P->h_slot_dev[sl], P->h_inv_perm[S.slot_var[sl]], 0u, 0u);
     }
     for (size_t y = 0; y < keyed[1].size(); y++) {
         const uint32_t e = S.ex_a[keyed[1][y].second];
         const uint32_t own = (uint32_t)P->h_var_edge[e], I = slot_fac[own];
         const uint32_t a = (uint32_t)P->h_fac_off[I], d = (uint32_t)(P->h_fac_off[I + 1] - a);
         const uint32_t meta = P->h_meta[I];   /* type and head slot never change */
-        fv[y] = make_uint4(e, P->h_inv_perm[S.slot_var[own]], I, a);
+        fv[y] = make_uint4(e, P->h_inv_perm[S.slot_var[P->h_slot_ref[own]]], I, a);
         faux[y] = make_uint4(d | ((meta & 3u) << 24), own - a, meta >> 8, 0u);
     }
     /* group items: one lane (one record) per slot, a group's lanes inside
@@ -212,13 +216,13 @@ int flbp_plan_set_schedule(flbp_plan *plan, const flbp_schedule *sched) {
         };
         for (uint32_t y = off[2][l]; y < off[2][l + 1]; y++) {
             const size_t x = keyed[2][y].second;
-            const uint32_t I = S.ex_a[x];
+            const uint32_t I = P->h_finv[S.ex_a[x]];   /* device factor */
             const uint32_t a = (uint32_t)P->h_fac_off[I], d = (uint32_t)(P->h_fac_off[I + 1] - a);
             const uint32_t meta = P->h_meta[I];
             if ((grp.size() - base) % 64 + d > 64) pad_to_wave();
             for (uint32_t k = 0; k < d; k++)
                 grp.push_back(make_uint4(I, a, flbp::dev::group_word(d, meta & 3u, meta >> 8, S.ex_b[x], S.ex_c[x], k),
-                                         P->h_inv_perm[S.slot_var[a + k]]));
+                                         P->h_inv_perm[S.slot_var[P->h_slot_ref[a + k]]]));
         }
         pad_to_wave();
         if (grp.size() >= 0xffffffffull)
