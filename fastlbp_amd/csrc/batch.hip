@@ -38,6 +38,7 @@ namespace flbp {
 
 struct GroupHost {
     GroupDev d;
+    std::vector<uint32_t> fperm;   /* device factor -> factor (factor_device_order) */
     uint64_t n_edges = 0;
     uint64_t perm_base = 0;   /* into h_perm: device variable -> variable */
     uint64_t mask_rows = 0;
@@ -530,7 +531,18 @@ int flbp_batch_create_groups(uint64_t n_groups, const flbp_batch_group *gs, int 
     std::vector<uint32_t> crow(F, 0);
     for (uint64_t g = 0; g < n_groups; g++) {
         const flbp_batch_group &bg = gs[g];
-        const flbp_csr &c = *bg.graph;
+        /* the group's factors in device order, as a plan keeps them
+         * (factor_device_order); clamps name variables, which keep theirs */
+        flbp::DeviceCsr dcsr;
+        {
+            const flbp_csr &c0 = *bg.graph;
+            std::vector<uint32_t> perm0, off0, edge0, vpos0, finv, slot_dev, slot_ref;
+            flbp::var_device_order(c0, flbp::kOrderHotFirst, perm0, off0, edge0, vpos0);
+            std::vector<uint32_t> inv0(c0.n_vars);
+            for (uint64_t dv = 0; dv < c0.n_vars; dv++) inv0[perm0[dv]] = (uint32_t)dv;
+            flbp::factor_device_order(c0, inv0, dcsr, B->hg[g].fperm, finv, slot_dev, slot_ref);
+        }
+        const flbp_csr &c = dcsr.view;
         GroupDev &d = B->hg[g].d;
         const uint64_t Fg = c.n_factors, Vg = c.n_vars, Eg = c.n_edges;
         const uint32_t K = d.K;
@@ -829,7 +841,12 @@ int flbp_batch_set_p(flbp_batch *b, uint64_t group, const double *p, uint64_t n)
     const GroupDev &d = B->hg[group].d;
     if (n != d.n_fac) return flbp::fail(FLBP_EINVAL, "flbp_batch_set_p: n must equal the group's n_factors");
     HIP_TRY(hipSetDevice(B->device));
-    if (n) HIP_TRY(hipMemcpy(B->fac_p + d.fac_base, p, n * 8, hipMemcpyHostToDevice));
+    if (n) {
+        const std::vector<uint32_t> &fperm = B->hg[group].fperm;
+        std::vector<double> pd(n);   /* device factor order */
+        for (uint64_t f = 0; f < n; f++) pd[f] = p[fperm[f]];
+        HIP_TRY(hipMemcpy(B->fac_p + d.fac_base, pd.data(), n * 8, hipMemcpyHostToDevice));
+    }
     return FLBP_OK;
 }
 
