@@ -372,17 +372,6 @@ using flbp::hip_fail;
 
 namespace {
 
-int default_grid(int device) {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_factor_pass<true, kFacRegDegSmall>, kBlock, 0) !=
-            hipSuccess ||
-        per_cu <= 0)
-        per_cu = 4;
-    return prop.multiProcessorCount * per_cu;
-}
-
 int check_opts(const flbp_batch_opts &o, const char *who) {
     if (o.layout != 0) return flbp::fail(FLBP_EINVAL, std::string(who) + ": layout must be 0 (the state layout)");
     if (o.tile_factors < 64 || o.tile_factors > flbp::kStageSlots || o.tile_vars < 64 ||
@@ -458,7 +447,6 @@ int flbp_batch_create_groups(uint64_t n_groups, const flbp_batch_group *gs, int 
     B->Mtot = M;
     B->Stot = S;
     B->opts = o;
-    if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(device);
     B->hg.resize(n_groups);
     {
         uint64_t fb = 0, vb = 0, eb = 0, mb = 0, sb = 0, nb = 0;
@@ -856,7 +844,6 @@ int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *o) {
     if (int rc = check_opts(*o, "flbp_batch_set_opts")) return rc;
     HIP_TRY(hipSetDevice(B->device));
     B->opts = *o;
-    if (B->opts.grid_blocks == 0) B->opts.grid_blocks = (uint32_t)default_grid(B->device);
     B->drop_exec();
     return B->build_tiles();
 }
