@@ -11,9 +11,14 @@
  *   var_off  u32[V+1]   var-major edge ranges (h_row_ptr_fv)
  *   var_edge u32[E]     factor-major slot of each var-major edge
  *   m_vf     f64x2[E]   v->f messages, (state0, state1) pairs: var-major
- *                       (variants 0, 1) or factor-major (2, 3)
+ *                       (variants 0, 1), factor-major (2, 3) or in variant
+ *                       4's storage order
  *   m_fv     f64x2[E]   f->v messages, factor-major (in variant 3 a copy kept
- *                       for damping; fm_stale marks it out of date)
+ *                       for damping; fm_stale marks it out of date); in
+ *                       variant 4 each staged factor tile's slots in its
+ *                       work order (h_store: slot -> storage position)
+ *   fac_var, var_src u32[E]  variant 4: device variable at each storage
+ *                       position; storage position of each var-major edge
  *   m_fv_vm  f64x2[E]   f->v messages, var-major (variant 3)
  *   belief   f64[V]     P(x=1) of the last sweep (the reference's _oldBeliefsV)
  *   ctl      u64[chunk][64 slots][8]  per-sweep partial max |b - b_old|
@@ -104,13 +109,14 @@ int hip_fail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(e_, #expr); \
     } while (0)
 
-/* variant 4: m_vf <- the current v->f messages in var-major order (the
- * records and f->v messages they derive from, or the init value) */
+/* variant 4: m_vf <- the current v->f messages in the storage order of the
+ * f->v messages (derived from the records and the f->v messages, or the
+ * init value) */
 int Plan::materialize_vf() {
     if (!E) return FLBP_OK;
     if (vf_src == kVfState)
-        hipLaunchKernelGGL(vf_from_state, dim3(grid_for(E, 256)), dim3(256), 0, stream, vprod, vzc, fac_var, fac_vpos,
-                           m_fv, m_vf, E);
+        hipLaunchKernelGGL(vf_from_state, dim3(grid_for(E, 256)), dim3(256), 0, stream, vprod, vzc, fac_var, m_fv,
+                           m_vf, E);
     else if (vf_src == kVfConst)
         hipLaunchKernelGGL(fill_pairs, dim3(grid_for(E, 256)), dim3(256), 0, stream, m_vf, E, init_value);
     vf_src = kVfArray;
@@ -170,10 +176,9 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
             return fail(FLBP_EINVAL, "plan: the state layout sweeps the plan's own message arrays only");
         if (F) {
             const dim3 grid((unsigned)n_fac_blocks), block(kBlock);
-            const uint32_t *idx = vf_src == kVfArray ? fac_vpos : fac_var;
-#define FLBP_FACTOR_STATE(DAMP, N, MODE)                                                                        \
-    hipLaunchKernelGGL((factor_pass_state<DAMP, N, MODE>), grid, block, 0, stream, fac_blk, fac_off, fac_meta,    \
-                       fac_p, fac_q, fac_mask, idx, fac_work, vprod, vzc, m_vf, m_fv, m_vf, init_value, damping, \
+#define FLBP_FACTOR_STATE(DAMP, N, MODE)                                                                         \
+    hipLaunchKernelGGL((factor_pass_state<DAMP, N, MODE>), grid, block, 0, stream, fac_blk, fac_off, fac_meta,     \
+                       fac_p, fac_q, fac_mask, fac_var, fac_work, vprod, vzc, m_vf, m_fv, m_vf, init_value, damping, \
                        keep, ctl, k, tol)
 #define FLBP_FACTOR_STATE_M(DAMP, N)                                       \
     do {                                                                   \
@@ -194,7 +199,7 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
         if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
         if (V)
             hipLaunchKernelGGL(var_pass_state, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream, var_blk, var_off,
-                               var_edge, var_work, m_fv, vprod, vzc, bel, ctl, k, tol);
+                               var_src, var_work, m_fv, vprod, vzc, bel, ctl, k, tol);
         if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 2], stream));
         HIP_TRY(hipGetLastError());
         /* from the second sweep on, the v->f inputs come from the products
@@ -577,23 +582,25 @@ int flbp_plan_get_messages(flbp_plan *plan, double *fv_varmajor, double *vf_facm
     HIP_TRY(hipSetDevice(P->device));
     std::vector<double> tmp(2 * P->E);
     if (fv_varmajor && P->E) {
-        /* factor-major, or var-major in device variable order (variant 3,
-         * unless a schedule's sweeps left only the factor-major array current) */
-        const bool vm = P->variant == 3 && !P->vm_stale;
+        /* factor-major, in the state layout's storage order (variant 4), or
+         * var-major in device variable order (variant 3, unless a schedule's
+         * sweeps left only the factor-major array current) */
+        const bool vm = P->variant == 3 && !P->vm_stale, st = P->variant == 4;
         HIP_TRY(hipMemcpy(tmp.data(), vm ? P->m_fv_vm : P->m_fv, P->E * sizeof(double2), hipMemcpyDeviceToHost));
         for (uint64_t k = 0; k < P->E; k++) {
-            const uint64_t s = vm ? P->h_fac_vpos[P->h_var_edge[k]] : P->h_var_edge[k];
+            const uint64_t e = P->h_var_edge[k];
+            const uint64_t s = vm ? P->h_fac_vpos[e] : st ? P->h_store[e] : e;
             fv_varmajor[2 * k] = tmp[2 * s];
             fv_varmajor[2 * k + 1] = tmp[2 * s + 1];
         }
     }
-    if (vf_facmajor && P->E) {   /* stored var-major, or factor-major in the push and scatter layouts */
-        if (P->variant == 4)      /* derived from the products: materialized var-major first */
+    if (vf_facmajor && P->E) {   /* stored var-major, factor-major in the push and scatter layouts, or in the
+                                    state layout's storage order */
+        if (P->variant == 4)      /* derived from the products: materialized first */
             if (int rc = P->materialize_vf()) return rc;
         HIP_TRY(hipMemcpy(tmp.data(), P->m_vf, P->E * sizeof(double2), hipMemcpyDeviceToHost));
         for (uint64_t r = 0; r < P->E; r++) {   /* reference slot r, device slot s */
-            const uint64_t s = P->h_slot_dev[r];
-            const uint64_t k = (P->variant == 2 || P->variant == 3) ? s : P->h_fac_vpos[s];
+            const uint64_t k = P->vf_pos(P->h_slot_dev[r]);
             vf_facmajor[2 * r] = tmp[2 * k];
             vf_facmajor[2 * r + 1] = tmp[2 * k + 1];
         }
@@ -615,7 +622,7 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
     if (fv_varmajor && P->E) {
         std::vector<double> tmp(2 * P->E);
         for (uint64_t k = 0; k < P->E; k++) {
-            const uint64_t s = P->h_var_edge[k];
+            const uint64_t s = P->variant == 4 ? P->h_store[P->h_var_edge[k]] : P->h_var_edge[k];
             tmp[2 * s] = fv_varmajor[2 * k];
             tmp[2 * s + 1] = fv_varmajor[2 * k + 1];
         }
@@ -632,8 +639,7 @@ int flbp_plan_set_messages(flbp_plan *plan, const double *fv_varmajor, const dou
     if (vf_facmajor && P->E) {
         std::vector<double> tmp(2 * P->E);
         for (uint64_t r = 0; r < P->E; r++) {   /* reference slot r, device slot s */
-            const uint64_t s = P->h_slot_dev[r];
-            const uint64_t k = (P->variant == 2 || P->variant == 3) ? s : P->h_fac_vpos[s];
+            const uint64_t k = P->vf_pos(P->h_slot_dev[r]);
             tmp[2 * k] = vf_facmajor[2 * r];
             tmp[2 * k + 1] = vf_facmajor[2 * r + 1];
         }

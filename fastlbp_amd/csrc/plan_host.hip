@@ -314,6 +314,7 @@ int Plan::upload(const flbp_csr &c_ref) {
     TRY(alloc(vprod, V));
     TRY(alloc(vzc, V));
     TRY(alloc(fac_var, E));
+    TRY(alloc(var_src, E));
     TRY(alloc(ctl, (uint64_t)kMaxChunk * dev::kCtlPerSweep));
     TRY(alloc(ctl_out, kMaxChunk));
     HIP_TRY(hipHostMalloc((void **)&host_ctl, sizeof(unsigned long long) * kMaxChunk, hipHostMallocDefault));
@@ -349,14 +350,24 @@ int Plan::upload(const flbp_csr &c_ref) {
         TRY(alloc(var_work, V));
         if (E) HIP_TRY(hipMemcpy(fac_work, fw.data(), E * 4, hipMemcpyHostToDevice));
         if (V) HIP_TRY(hipMemcpy(var_work, vw.data(), V * 2, hipMemcpyHostToDevice));
+        /* the state layout's storage order: a staged tile's slots in its
+         * work order, a wide factor's in slot order */
+        h_store.resize(E);
+        for (size_t b = 0; b + 1 < fb.size(); b++) {
+            const uint32_t base = fo[fb[b]], n = fo[fb[b + 1]] - base;
+            for (uint32_t i = 0; i < n; i++)
+                h_store[n > kStageSlots ? base + i : base + (fw[base + i] & 1023u)] = base + i;
+        }
     }
     if (E) {
         HIP_TRY(hipMemcpy(var_edge, dev_edge.data(), E * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(fac_vpos, h_fac_vpos.data(), E * 4, hipMemcpyHostToDevice));
         std::vector<uint32_t> inv(V), fv(E);
         for (uint64_t dv = 0; dv < V; dv++) inv[h_perm[dv]] = (uint32_t)dv;
-        for (uint64_t s = 0; s < E; s++) fv[s] = inv[c.edge_var[s]];
+        for (uint64_t s = 0; s < E; s++) fv[h_store[s]] = inv[c.edge_var[s]];
         HIP_TRY(hipMemcpy(fac_var, fv.data(), E * 4, hipMemcpyHostToDevice));
+        for (uint64_t k = 0; k < E; k++) fv[k] = h_store[dev_edge[k]];
+        HIP_TRY(hipMemcpy(var_src, fv.data(), E * 4, hipMemcpyHostToDevice));
     }
     if (F) {
         HIP_TRY(hipMemcpy(fac_p, c.p, F * 8, hipMemcpyHostToDevice));

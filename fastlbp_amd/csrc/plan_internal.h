@@ -114,9 +114,15 @@ struct Plan {
     int variant = 3;   /* 0 staged pull, 1 one lane per item, 2 staged push, 3 staged scatter, 4 staged state */
     double2 *vprod = nullptr;       /* variant 4: per-variable products (device order) */
     uint8_t *vzc = nullptr;         /* variant 4: per-variable zero-count codes (pack_zc) */
-    uint32_t *fac_var = nullptr;    /* variant 4: device variable of each factor-major slot */
+    uint32_t *fac_var = nullptr;    /* variant 4: device variable at each storage position */
+    uint32_t *var_src = nullptr;    /* variant 4: storage position of each var-major edge */
     int vf_src = dev::kVfConst;     /* variant 4: v->f inputs of the next factor pass */
-    int materialize_vf();           /* variant 4: the v->f messages into m_vf, var-major (plan.hip) */
+    int materialize_vf();           /* variant 4: the v->f messages into m_vf, storage order (plan.hip) */
+    /* position in m_vf of the v->f message of device slot s: var-major
+     * (variants 0, 1), factor-major (2, 3) or in storage order (4) */
+    uint64_t vf_pos(uint64_t s) const {
+        return variant == 4 ? h_store[s] : (variant == 2 || variant == 3) ? s : h_fac_vpos[s];
+    }
     double *fac_p = nullptr, *fac_q = nullptr, *belief = nullptr;
     double2 *fac_mask = nullptr, *m_vf = nullptr, *m_fv = nullptr;
     double2 *m_fv_vm = nullptr;    /* var-major f->v messages (variant 3) */
@@ -143,6 +149,7 @@ struct Plan {
     uint64_t stage_cap = 0;
     int stage(uint64_t n);
     std::vector<uint32_t> h_fac_vpos;              /* factor-major slot -> var-major position */
+    std::vector<uint32_t> h_store;                 /* factor-major slot -> variant 4 storage position */
     std::vector<uint32_t> h_meta, h_meta0;         /* factor meta words: current and at creation */
     std::vector<double2> h_mask, h_mask0;          /* head masks: current and at creation */
     std::vector<uint64_t> clamp_dirty;             /* factors whose meta / mask differ from creation */

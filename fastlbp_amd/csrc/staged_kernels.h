@@ -213,6 +213,49 @@ struct VarSmem {
     unsigned long long red[kBlock / 64];
 };
 
+/* One output of a staged factor tile: the f->v message of local slot sl of
+ * local factor lf, from the v->f inputs of the tile's slots in sm.in (slot
+ * order) and its factors' offsets and parameters in sm.start / meta / p / q. */
+template <int N, class Smem>
+__device__ __forceinline__ double2 factor_output(const Smem &sm, uint32_t sl, uint32_t lf, uint32_t f0,
+                                                 const double2 *__restrict__ fac_mask) {
+    const uint32_t s = sm.start[lf];
+    const int d = (int)(sm.start[lf + 1] - s);
+    const int t = (int)(sl - s);
+    const uint32_t meta = sm.meta[lf];
+    const uint32_t type = meta & 3u;
+    const bool clamped = (meta & 4u) != 0;
+    const int h = (int)(meta >> 8);
+    const double p = sm.p[lf];
+    const double q = type ? sm.q[lf] : 0.0;
+    double mask0 = 1.0, mask1 = 1.0;
+    if (clamped) {
+        const double2 mk = fac_mask[f0 + lf];
+        mask0 = mk.x;
+        mask1 = mk.y;
+    }
+    if (d <= N) {
+        /* the other slots' messages, in slot order, from LDS */
+        const bool head_target = (t == h);
+        const int skip_h = head_target ? -1 : h;
+        const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
+        double2 ops[N - 1];
+        int js = 0;
+#pragma unroll
+        for (int k = 0; k < N - 1; k++) {
+            js += (js == t || js == skip_h);
+            js += (js == t || js == skip_h);
+            const double2 v = sm.in[s + (k < cnt ? js : 0)];   /* index kept inside the factor */
+            ops[k] = k < cnt ? v : make_double2(0.0, 0.0);
+            js++;
+        }
+        return fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, sm.in[s + h], p, q, mask0, mask1);
+    }
+    LdsIn in{sm.in + s};
+    const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
+    return fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
+}
+
 /* Phase 2 of a factor tile: one lane per output slot, in the tile's work
  * order (w = local slot | local factor << 10), from LDS only: sm.in holds
  * the tile's v->f inputs, sm.out receives the f->v outputs (and holds the
@@ -229,43 +272,7 @@ __device__ __forceinline__ void factor_tile_compute(FactorSmem &sm, const uint32
 #pragma unroll
         for (int x = 0; x < kPerLane; x++) wj |= (j == x) ? w[x] : 0u;
         const uint32_t sl = wj & 1023u, lf = wj >> 10;
-        const uint32_t s = sm.start[lf];
-        const int d = (int)(sm.start[lf + 1] - s);
-        const int t = (int)(sl - s);
-        const uint32_t meta = sm.meta[lf];
-        const uint32_t type = meta & 3u;
-        const bool clamped = (meta & 4u) != 0;
-        const int h = (int)(meta >> 8);
-        const double p = sm.p[lf];
-        const double q = type ? sm.q[lf] : 0.0;
-        double mask0 = 1.0, mask1 = 1.0;
-        if (clamped) {
-            const double2 mk = fac_mask[f0 + lf];
-            mask0 = mk.x;
-            mask1 = mk.y;
-        }
-        double2 r;
-        if (d <= N) {
-            /* the other slots' messages, in slot order, from LDS */
-            const bool head_target = (t == h);
-            const int skip_h = head_target ? -1 : h;
-            const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
-            double2 ops[N - 1];
-            int js = 0;
-#pragma unroll
-            for (int k = 0; k < N - 1; k++) {
-                js += (js == t || js == skip_h);
-                js += (js == t || js == skip_h);
-                const double2 v = sm.in[s + (k < cnt ? js : 0)];   /* index kept inside the factor */
-                ops[k] = k < cnt ? v : make_double2(0.0, 0.0);
-                js++;
-            }
-            r = fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, sm.in[s + h], p, q, mask0, mask1);
-        } else {
-            LdsIn in{sm.in + s};
-            const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
-            r = fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
-        }
+        double2 r = factor_output<N>(sm, sl, lf, f0, fac_mask);
         if (kDamp) {
             const double2 old = sm.out[sl];
             r.x = keep * r.x + damp * old.x;

@@ -10,7 +10,7 @@
  * v->f message of edge (I, v) is therefore a function of two values only:
  * the variable's product record and the f->v message of the same edge.
  *
- * This layout keeps exactly those two: f->v messages factor-major (the
+ * This layout keeps exactly those two: f->v messages by factor tile (the
  * factor pass reads and rewrites them in place, contiguously) and, per
  * variable, its product pair (16 bytes) and a one-byte code of its zero
  * counts (pack_zc).  The variable pass forms each variable's product,
@@ -22,7 +22,12 @@
  * factor pass's random reads go to V-sized arrays (the codes of a million
  * variables fit one XCD's L2), which caches hold far better than a
  * per-edge array.  The one per-edge random access of a sweep is the
- * variable pass's gather of its f->v messages from their factor slots.
+ * variable pass's gather of its f->v messages from their storage positions
+ * (var_src).  Within a factor tile the messages are stored in the tile's
+ * work order, heads before bodies: a variable's head messages then sit in
+ * the short head run of its factors' tile, so the gathers of a variable
+ * tile touch fewer lines than with each factor's slots side by side
+ * (353 against 402 MB per pass on the 1M-variable graph by a line count).
  * (Scattering a var-major copy from the factor pass instead, so that the
  * variable pass reads in order, measured 2.2x slower per factor pass on a
  * 4.5M-edge graph in HBM: 177 against 80 us.)
@@ -32,8 +37,8 @@
  * the variable pass of the previous sweep would have used.  The first sweep
  * after init() takes the init value (c, c) as its v->f input, as the
  * reference's first calcMessageFVFusedAll does; after a caller sets v->f
- * messages explicitly, the next factor pass gathers them from a var-major
- * array, as the pull layout does.
+ * messages explicitly, the next factor pass reads them from m_vf, stored in
+ * the same order as the f->v messages.
  */
 #ifndef FLBP_STATE_KERNELS_H
 #define FLBP_STATE_KERNELS_H
@@ -58,17 +63,34 @@ __device__ __forceinline__ double2 vf_of(double2 prod, uint32_t zc, double2 m) {
     return vf_msg(prod.x, prod.y, zc & 3u, zc >> 2, m);
 }
 
-/* The factor pass over factors [f0, f1) in the state layout.  idx is the
- * slot -> device variable map (kVfState) or the slot -> var-major position
- * map (kVfArray).  m_fv is read and rewritten in place: each slot belongs to
- * one factor, and a block reads all its slots before it writes any. */
+/* LDS of one factor tile of the state layout: the v->f inputs of its slots
+ * and its factors' offsets and parameters (the outputs go from registers
+ * to HBM). */
+struct StateFactorSmem {
+    double2 in[kStage];
+    uint32_t start[kBlock + 1];   /* factor offsets relative to the tile's first slot */
+    uint32_t meta[kBlock];
+    double p[kBlock], q[kBlock];
+};
+
+/* The factor pass over factors [f0, f1) in the state layout.
+ *
+ * Storage order: the f->v message of a staged tile's slot sits at the
+ * tile's base plus the slot's position in the tile's work order (so a
+ * tile's head outputs, which the work order puts before its body outputs,
+ * form one run, next to the runs of the neighbouring tiles); a wide
+ * factor's slots keep their slot order.  idx is the storage position ->
+ * device variable map, m_vf the explicit v->f inputs in storage order
+ * (kVfArray).  m_fv is read and rewritten in place: every lane reads the
+ * old messages of the storage positions it will write, and each slot
+ * belongs to one factor of one tile. */
 template <bool kDamp, int N, int kMode>
 __device__ __forceinline__ void factor_tile_state(
     const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
     const double *__restrict__ fac_p, const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
     const uint32_t *__restrict__ idx, const uint32_t *__restrict__ fac_work, const double2 *__restrict__ vprod,
     const uint8_t *__restrict__ vzc, const double2 *m_vf, double2 *m_fv, double2 *scratch, double c, double damp,
-    double keep, uint32_t f0, uint32_t f1, FactorSmem &sm) {
+    double keep, uint32_t f0, uint32_t f1, StateFactorSmem &sm) {
     const uint32_t base = fac_off[f0];
     const uint32_t n = fac_off[f1] - base;
     const int tid = threadIdx.x;
@@ -85,8 +107,8 @@ __device__ __forceinline__ void factor_tile_state(
             mask1 = mk.y;
         }
         if constexpr (kMode == kVfArray) {
-            wide_factor<kDamp, kLayoutPull>(GlobalIn{m_vf, idx + base}, base, n, meta, p, q, mask0, mask1, nullptr,
-                                            m_fv, m_fv, nullptr, damp, keep);
+            wide_factor<kDamp, kLayoutPull>(LdsIn{m_vf + base}, base, n, meta, p, q, mask0, mask1, nullptr, m_fv,
+                                            m_fv, nullptr, damp, keep);
         } else if constexpr (kMode == kVfConst) {
             wide_factor<kDamp, kLayoutPull>(ConstIn{c}, base, n, meta, p, q, mask0, mask1, nullptr, m_fv, m_fv,
                                             nullptr, damp, keep);
@@ -101,16 +123,17 @@ __device__ __forceinline__ void factor_tile_state(
         }
         return;
     }
-    /* phase 1: work entries and indices into registers, the factors'
-     * offsets and parameters into LDS, and the v->f inputs (derived from the
-     * products and the old f->v messages) into LDS */
+    /* phase 1: work entries, variables and old messages of the lane's
+     * storage positions into registers, the factors' offsets and parameters
+     * into LDS, and each slot's v->f input (derived from the products and
+     * the old f->v message) into LDS at its slot */
     const uint32_t nf = f1 - f0;
     uint32_t w[kPerLane], ix[kPerLane];
 #pragma unroll
     for (int j = 0; j < kPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         w[j] = i < n ? fac_work[base + i] : 0u;
-        ix[j] = (kMode != kVfConst && i < n) ? idx[base + i] : 0u;
+        ix[j] = (kMode == kVfState && i < n) ? idx[base + i] : 0u;
     }
     if ((uint32_t)tid < nf) {
         sm.start[tid] = fac_off[f0 + tid] - base;
@@ -124,11 +147,14 @@ __device__ __forceinline__ void factor_tile_state(
 #pragma unroll
     for (int j = 0; j < kPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
+        old[j] = make_double2(0.0, 0.0);
         if (i < n) {
             if (kDamp || kMode == kVfState) old[j] = m_fv[base + i];
-            if (kMode == kVfState) {
+            if constexpr (kMode == kVfState) {
                 pr[j] = vprod[ix[j]];
                 zc[j] = vzc[ix[j]];
+            } else if constexpr (kMode == kVfArray) {
+                pr[j] = m_vf[base + i];
             }
         }
     }
@@ -139,19 +165,33 @@ __device__ __forceinline__ void factor_tile_state(
             double2 in;
             if constexpr (kMode == kVfState) in = vf_of(pr[j], zc[j], old[j]);
             else if constexpr (kMode == kVfConst) in = make_double2(c, c);
-            else in = m_vf[ix[j]];
-            sm.in[i] = in;
-            if (kDamp) sm.out[i] = old[j];
+            else in = pr[j];
+            sm.in[w[j] & 1023u] = in;
         }
     }
     __syncthreads();
-    factor_tile_compute<kDamp, N>(sm, w, n, f0, fac_mask, damp, keep);
-    __syncthreads();
-    /* phase 3: the new f->v messages, in place and coalesced */
-#pragma unroll
+    /* phase 2: each lane's outputs in work order, stored straight to its
+     * storage positions (coalesced: lane order is storage order) */
+#pragma unroll 1
     for (int j = 0; j < kPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
-        if (i < n) m_fv[base + i] = sm.out[i];
+        if (i >= n) break;
+        uint32_t wj = 0;
+        double2 oj = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int x = 0; x < kPerLane; x++) {
+            wj |= (j == x) ? w[x] : 0u;
+            if (kDamp) {
+                oj.x = (j == x) ? old[x].x : oj.x;
+                oj.y = (j == x) ? old[x].y : oj.y;
+            }
+        }
+        double2 r = factor_output<N>(sm, wj & 1023u, wj >> 10, f0, fac_mask);
+        if (kDamp) {
+            r.x = keep * r.x + damp * oj.x;
+            r.y = keep * r.y + damp * oj.y;
+        }
+        m_fv[base + i] = r;
     }
 }
 
@@ -261,7 +301,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
     const uint8_t *__restrict__ vzc, const double2 *m_vf, double2 *m_fv, double2 *scratch, double c, double damp,
     double keep, const unsigned long long *__restrict__ ctl, int k, double tol) {
     if (chunk_stopped(ctl, k, tol)) return;
-    __shared__ FactorSmem sm;
+    __shared__ StateFactorSmem sm;
     const uint32_t b = blockIdx.x;
     factor_tile_state<kDamp, N, kMode>(fac_off, fac_meta, fac_p, fac_q, fac_mask, idx, fac_work, vprod, vzc, m_vf,
                                        m_fv, scratch, c, damp, keep, blk[b], blk[b + 1], sm);
@@ -281,16 +321,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_state
     block_fold_diff(ctl, k, bits, sm.red);
 }
 
-/* the v->f messages of the state layout, materialized var-major (the pull
- * layout's order): vf[vpos[s]] = vf_msg(products of slot s's variable,
+/* the v->f messages of the state layout, materialized in its storage
+ * order: vf[s] = vf_msg(products of the variable at storage position s,
  * f->v[s]) */
 __global__ __attribute__((unused)) void vf_from_state(const double2 *__restrict__ vprod, const uint8_t *__restrict__ vzc,
-                              const uint32_t *__restrict__ fac_var, const uint32_t *__restrict__ fac_vpos,
-                              const double2 *__restrict__ m_fv, double2 *__restrict__ vf, uint64_t n) {
+                                                      const uint32_t *__restrict__ fac_var,
+                                                      const double2 *__restrict__ m_fv, double2 *__restrict__ vf,
+                                                      uint64_t n) {
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n) {
         const uint32_t v = fac_var[s];
-        vf[fac_vpos[s]] = vf_of(vprod[v], vzc[v], m_fv[s]);
+        vf[s] = vf_of(vprod[v], vzc[v], m_fv[s]);
     }
 }
 
