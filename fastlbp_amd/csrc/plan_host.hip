@@ -202,7 +202,7 @@ void cut_blocks(const uint32_t *off, uint64_t n_items, uint32_t max_items, uint3
 
 void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const std::vector<uint32_t> &fb,
                        const uint32_t *var_off, const std::vector<uint32_t> &vb, std::vector<uint32_t> &fac_work,
-                       std::vector<uint16_t> &var_work) {
+                       std::vector<uint16_t> &var_work, const uint32_t *slot_var, uint32_t reg_deg) {
     const uint64_t E = fac_off[fb.back()];
     fac_work.assign(E, 0);
     std::vector<std::pair<uint64_t, uint32_t>> keyed;
@@ -217,8 +217,9 @@ void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const 
             for (uint32_t t = 0; t < d; t++) {
                 const uint32_t kind = type == 0 ? 0u : (t == h ? 1u : 2u);
                 const uint32_t sl = fac_off[f] - base + t;
-                keyed.push_back({((uint64_t)kind << 40) | ((uint64_t)d << 20) | sl,
-                                 sl | ((f - f0) << 10)});
+                const bool by_var = slot_var && kind == 2 && d <= reg_deg;
+                const uint64_t cls = by_var ? 0u : d, sec = by_var ? slot_var[fac_off[f] + t] : sl;
+                keyed.push_back({((uint64_t)kind << 62) | (cls << 42) | (sec << 10) | sl, sl | ((f - f0) << 10)});
             }
         }
         std::sort(keyed.begin(), keyed.end());
@@ -345,7 +346,14 @@ int Plan::upload(const flbp_csr &c_ref) {
         HIP_TRY(hipMemcpy(var_blk, vb.data(), vb.size() * 4, hipMemcpyHostToDevice));
         std::vector<uint32_t> fw;
         std::vector<uint16_t> vw;
-        stage_work_orders(fo.data(), meta.data(), fb, dev_off.data(), vb, fw, vw);
+        std::vector<uint32_t> inv(V), sv(E);
+        for (uint64_t dv = 0; dv < V; dv++) inv[h_perm[dv]] = (uint32_t)dv;
+        uint64_t maxd = 0;
+        for (uint64_t I = 0; I < F; I++) maxd = std::max<uint64_t>(maxd, c.fac_off[I + 1] - c.fac_off[I]);
+        for (uint64_t s = 0; s < E; s++) sv[s] = inv[c.edge_var[s]];
+        /* the register degree bound the sweep launches pick (plan.hip) */
+        const uint32_t reg_deg = maxd <= (uint64_t)kFacRegDegSmall ? kFacRegDegSmall : kFacRegDeg;
+        stage_work_orders(fo.data(), meta.data(), fb, dev_off.data(), vb, fw, vw, sv.data(), reg_deg);
         TRY(alloc(fac_work, E));
         TRY(alloc(var_work, V));
         if (E) HIP_TRY(hipMemcpy(fac_work, fw.data(), E * 4, hipMemcpyHostToDevice));

@@ -69,10 +69,14 @@ void cut_blocks(const uint32_t *off, uint64_t n_items, uint32_t max_items, uint3
 /* work orders of the staged kernels (see staged_kernels.h): for each block
  * of at most kStageSlots slots, fac_work lists its slots (local slot | local
  * factor << 10) grouped by output kind (prior, head, body) and factor
- * degree; var_work lists its variables (local index) by degree */
+ * degree; var_work lists its variables (local index) by degree.  Given
+ * slot_var (device variable of each slot), the body outputs of factors of
+ * degree <= reg_deg (which all run the same register path) form one group,
+ * ordered by variable: the state layout stores messages in work order, and
+ * bodies of nearby variables then share lines. */
 void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const std::vector<uint32_t> &fb,
                        const uint32_t *var_off, const std::vector<uint32_t> &vb, std::vector<uint32_t> &fac_work,
-                       std::vector<uint16_t> &var_work);
+                       std::vector<uint16_t> &var_work, const uint32_t *slot_var = nullptr, uint32_t reg_deg = 0);
 void var_device_order(const flbp_csr &c, int order, std::vector<uint32_t> &perm,
                       std::vector<uint32_t> &dev_off, std::vector<uint32_t> &dev_edge,
                       std::vector<uint32_t> &vpos);
