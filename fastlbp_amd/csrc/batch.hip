@@ -61,7 +61,8 @@ struct Batch {
     double2 *masks = nullptr;
     uint32_t *var_off = nullptr, *var_edge = nullptr;
     uint16_t *var_work = nullptr;
-    double2 *m_fv = nullptr, *vprod = nullptr, *scratch = nullptr;
+    double2 *m_fv = nullptr, *scratch = nullptr;
+    double *vprod = nullptr;
     uint8_t *vzc = nullptr;
     double *belief = nullptr;
     uint8_t *vrun = nullptr, *vfresh = nullptr;
@@ -632,7 +633,7 @@ int flbp_batch_create_groups(uint64_t n_groups, const flbp_batch_group *gs, int 
     /* messages zero until init, old beliefs 0.0, every variant fresh */
     HIP_TRY(hipMemset(B->m_fv, 0, std::max<uint64_t>(M, 1) * 16));
     HIP_TRY(hipMemset(B->belief, 0, std::max<uint64_t>(S, 1) * 8));
-    HIP_TRY(hipMemset(B->vprod, 0, std::max<uint64_t>(S, 1) * 16));
+    HIP_TRY(hipMemset(B->vprod, 0, std::max<uint64_t>(S, 1) * 8));
     HIP_TRY(hipMemset(B->vzc, 0, std::max<uint64_t>(S, 1)));
     HIP_TRY(hipMemset(B->vfresh, 1, std::max<uint64_t>(NV, 1)));
     HIP_TRY(hipMemset(B->iters_dev, 0, std::max<uint64_t>(NV, 1) * 8));

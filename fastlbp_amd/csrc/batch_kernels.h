@@ -109,7 +109,7 @@ struct BatchArgs {
     const uint32_t *__restrict__ var_off, *__restrict__ var_edge;
     const uint16_t *__restrict__ var_work;
     double2 *m_fv;
-    double2 *__restrict__ vprod;
+    double *__restrict__ vprod;               /* product records (pack_prod) */
     uint8_t *__restrict__ vzc;
     double *__restrict__ belief;
     double2 *__restrict__ scratch;            /* wide factors' v->f inputs */
@@ -224,7 +224,7 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
             const uint32_t k = i & km;
             const uint64_t v = gr.vst_base + ((uint64_t)ix[j] << lk) + k;
             const double2 old = fv[i];
-            const double2 pr = A.vprod[v];
+            const double pr = A.vprod[v];
             const uint32_t zc = A.vzc[v];
             const bool fresh = A.vfresh[gr.vbase + k] != 0;
             s_in[i] = fresh ? make_double2(A.c, A.c) : vf_of(pr, zc, old);
@@ -374,17 +374,17 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
             const double b = b1 / (b0 + b1);
             const double diff = fabs(A.belief[vst + tid] - b);
             A.belief[vst + tid] = b;
-            A.vprod[vst + tid] = make_double2(prod_0, prod_1);
-            A.vzc[vst + tid] = pack_zc(z0, z1);
+            double rest;
+            A.vzc[vst + tid] = pack_prod(prod_0, prod_1, z0, z1, rest);
+            A.vprod[vst + tid] = rest;
             fold_diff(A, gr.vbase + tid, ti, (unsigned long long)__double_as_longlong(diff));
         }
         return;
     }
     /* staged results of a tile of K < kDirectK (nk <= kStage), in the upper
-     * half of the buffer: 3 * kBlock doubles and kBlock codes fit its 8 KiB */
-    double *const s_prod0 = reinterpret_cast<double *>(sm.buf + kStage);
-    double *const s_prod1 = s_prod0 + kBlock;
-    double *const s_bel = s_prod1 + kBlock;
+     * half of the buffer: 2 * kBlock doubles and kBlock codes fit its 8 KiB */
+    double *const s_prod = reinterpret_cast<double *>(sm.buf + kStage);
+    double *const s_bel = s_prod + kBlock;
     uint32_t *const s_z = reinterpret_cast<uint32_t *>(s_bel + kBlock);
     /* phase 1 */
     const uint32_t nv = v1 - v0, nvk = nv << lk;   /* nvk <= 2 * kBlock (direct), kBlock (staged) */
@@ -437,8 +437,9 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
             if (A.vrun[gr.vbase + k]) {
                 const double diff = fabs(A.belief[vst + o] - b);
                 A.belief[vst + o] = b;
-                A.vprod[vst + o] = make_double2(prod_0, prod_1);
-                A.vzc[vst + o] = pack_zc(z0, z1);
+                double rest;
+                A.vzc[vst + o] = pack_prod(prod_0, prod_1, z0, z1, rest);
+                A.vprod[vst + o] = rest;
                 const unsigned long long bits = (unsigned long long)__double_as_longlong(diff);
                 diff_bits = bits > diff_bits ? bits : diff_bits;
             }
@@ -446,9 +447,9 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
             const double diff = fabs(s_bel[o] - b);
             s_bel[o] = b;
             diff_bits = (unsigned long long)__double_as_longlong(diff);
-            s_prod0[o] = prod_0;
-            s_prod1[o] = prod_1;
-            s_z[o] = pack_zc(z0, z1);
+            double rest;
+            s_z[o] = pack_prod(prod_0, prod_1, z0, z1, rest);
+            s_prod[o] = rest;
         }
         if (!direct) break;
     }
@@ -459,7 +460,7 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
         __syncthreads();
         /* phase 3: the variants that run store products, codes and beliefs */
         if ((uint32_t)tid < nvk && A.vrun[gr.vbase + (tid & km)]) {
-            A.vprod[vst + tid] = make_double2(s_prod0[tid], s_prod1[tid]);
+            A.vprod[vst + tid] = s_prod[tid];
             A.vzc[vst + tid] = (uint8_t)s_z[tid];
             A.belief[vst + tid] = s_bel[tid];
         }

@@ -27,11 +27,43 @@ struct __align__(32) FacRec {
     double p, q, m0, m1;
 };
 /* the state layout (kernel variant 4, state_kernels.h) keeps each
- * variable's prod_fv_0/1 (calcProdAllKernel) as a double2 and its
- * num_zeros_fv_0/1 as one byte, min(z0, 2) | min(z1, 2) << 2: the
- * leave-one-out and the belief only ask whether a count is 0, 1 or more */
+ * variable's num_zeros_fv_0/1 (calcProdAllKernel) in one code byte,
+ * min(z0, 2) | min(z1, 2) << 2: the leave-one-out and the belief only ask
+ * whether a count is 0, 1 or more */
 __host__ __device__ inline uint8_t pack_zc(uint32_t z0, uint32_t z1) {
     return (uint8_t)((z0 < 2 ? z0 : 2u) | ((z1 < 2 ? z1 : 2u) << 2));
+}
+/* ... and its prod_fv_0/1 in 8 bytes.  calcProdAllKernel rescales the pair
+ * after every factor (causal_bp_parall.cu:14-18, 37-47; scale_pair): the
+ * larger value becomes the unit -- exactly 1.0, or the quiet NaN
+ * 0x7ff8000000000000 when the larger value was 0 or not finite -- and only
+ * the other value carries information.  A variable without edges keeps
+ * (1, 1).  So the record stores the other value, and two more bits of the
+ * code byte say which component is the unit and whether it is the NaN;
+ * unpack_prod returns the pair bit for bit. */
+constexpr uint32_t kProdUnitX = 16u, kProdUnitNan = 32u;
+constexpr uint64_t kUnitOneBits = 0x3ff0000000000000ull, kUnitNanBits = 0x7ff8000000000000ull;
+__host__ __device__ inline uint8_t pack_prod(double x, double y, uint32_t z0, uint32_t z1, double &rest) {
+    const uint64_t bx = __builtin_bit_cast(uint64_t, x), by = __builtin_bit_cast(uint64_t, y);
+    uint32_t f;
+    if (bx == kUnitOneBits) {
+        f = kProdUnitX;
+        rest = y;
+    } else if (by == kUnitOneBits) {
+        f = 0;
+        rest = x;
+    } else if (bx == kUnitNanBits) {
+        f = kProdUnitX | kProdUnitNan;
+        rest = y;
+    } else {
+        f = kProdUnitNan;
+        rest = x;
+    }
+    return (uint8_t)(pack_zc(z0, z1) | f);
+}
+__host__ __device__ inline double2 unpack_prod(double rest, uint32_t code) {
+    const double unit = (code & kProdUnitNan) ? __builtin_bit_cast(double, kUnitNanBits) : 1.0;
+    return (code & kProdUnitX) ? make_double2(unit, rest) : make_double2(rest, unit);
 }
 /* where the state layout's next factor pass takes its v->f inputs from:
  * the product records, the init value, or a var-major array */
@@ -116,8 +148,8 @@ struct Plan {
     uint16_t *var_work = nullptr;   /* per staged variable block: its variables in work order */
     uint64_t n_fac_blocks = 0, n_var_blocks = 0;
     int variant = 3;   /* 0 staged pull, 1 one lane per item, 2 staged push, 3 staged scatter, 4 staged state */
-    double2 *vprod = nullptr;       /* variant 4: per-variable products (device order) */
-    uint8_t *vzc = nullptr;         /* variant 4: per-variable zero-count codes (pack_zc) */
+    double *vprod = nullptr;        /* variant 4: per-variable products (device order, pack_prod) */
+    uint8_t *vzc = nullptr;         /* variant 4: per-variable codes (pack_prod) */
     uint32_t *fac_var = nullptr;    /* variant 4: device variable at each storage position */
     uint32_t *var_src = nullptr;    /* variant 4: storage position of each var-major edge */
     int vf_src = dev::kVfConst;     /* variant 4: v->f inputs of the next factor pass */

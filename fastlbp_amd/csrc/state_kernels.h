@@ -12,11 +12,12 @@
  *
  * This layout keeps exactly those two: f->v messages by factor tile (the
  * factor pass reads and rewrites them in place, contiguously) and, per
- * variable, its product pair (16 bytes) and a one-byte code of its zero
- * counts (pack_zc).  The variable pass forms each variable's product,
- * belief and |b - b_old| and stores the pair and code -- it writes no
- * per-edge data.  The factor pass derives every v->f input of its tile
- * from the pair and code of the slot's variable and the slot's own
+ * variable, its product pair in 8 bytes (the rescaled pair's non-unit
+ * value) and a one-byte code of its zero counts and unit (pack_prod).  The
+ * variable pass forms each variable's product, belief and |b - b_old| and
+ * stores the record and code -- it writes no per-edge data.  The factor
+ * pass derives every v->f input of its tile from the record and code of
+ * the slot's variable and the slot's own
  * previous f->v message (vf_msg, the reference's division-based
  * leave-one-out), then computes and stores the new f->v messages.  The
  * factor pass's random reads go to V-sized arrays (the codes of a million
@@ -59,8 +60,11 @@ struct ConstIn {
     __device__ __forceinline__ double2 pick(int) const { return make_double2(c, c); }
 };
 
-__device__ __forceinline__ double2 vf_of(double2 prod, uint32_t zc, double2 m) {
-    return vf_msg(prod.x, prod.y, zc & 3u, zc >> 2, m);
+/* the v->f message of an edge from its variable's product record (rest,
+ * code: pack_prod) and the edge's f->v message m */
+__device__ __forceinline__ double2 vf_of(double rest, uint32_t code, double2 m) {
+    const double2 prod = unpack_prod(rest, code);
+    return vf_msg(prod.x, prod.y, code & 3u, (code >> 2) & 3u, m);
 }
 
 /* LDS of one factor tile of the state layout: the v->f inputs of its slots
@@ -88,7 +92,7 @@ template <bool kDamp, int N, int kMode>
 __device__ __forceinline__ void factor_tile_state(
     const uint32_t *__restrict__ fac_off, const uint32_t *__restrict__ fac_meta,
     const double *__restrict__ fac_p, const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
-    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ fac_work, const double2 *__restrict__ vprod,
+    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ fac_work, const double *__restrict__ vprod,
     const uint8_t *__restrict__ vzc, const double2 *m_vf, double2 *m_fv, double2 *scratch, double c, double damp,
     double keep, uint32_t f0, uint32_t f1, StateFactorSmem &sm) {
     const uint32_t base = fac_off[f0];
@@ -142,7 +146,8 @@ __device__ __forceinline__ void factor_tile_state(
         sm.q[tid] = fac_q[f0 + tid];
     }
     if (tid == 0) sm.start[nf] = n;
-    double2 old[kPerLane], pr[kPerLane];
+    double2 old[kPerLane], vin[kPerLane];
+    double pr[kPerLane];
     uint32_t zc[kPerLane];
 #pragma unroll
     for (int j = 0; j < kPerLane; j++) {
@@ -154,7 +159,7 @@ __device__ __forceinline__ void factor_tile_state(
                 pr[j] = vprod[ix[j]];
                 zc[j] = vzc[ix[j]];
             } else if constexpr (kMode == kVfArray) {
-                pr[j] = m_vf[base + i];
+                vin[j] = m_vf[base + i];
             }
         }
     }
@@ -165,7 +170,7 @@ __device__ __forceinline__ void factor_tile_state(
             double2 in;
             if constexpr (kMode == kVfState) in = vf_of(pr[j], zc[j], old[j]);
             else if constexpr (kMode == kVfConst) in = make_double2(c, c);
-            else in = pr[j];
+            else in = vin[j];
             sm.in[w[j] & 1023u] = in;
         }
     }
@@ -201,7 +206,7 @@ __device__ __forceinline__ void factor_tile_state(
  * |b - b_old| as IEEE bits. */
 __device__ __forceinline__ unsigned long long var_tile_state(
     const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
-    const uint16_t *__restrict__ var_work, const double2 *__restrict__ m_fv, double2 *__restrict__ vprod,
+    const uint16_t *__restrict__ var_work, const double2 *__restrict__ m_fv, double *__restrict__ vprod,
     uint8_t *__restrict__ vzc, double *__restrict__ belief, uint32_t v0, uint32_t v1, VarSmem &sm) {
     const uint32_t base = var_off[v0];
     const uint32_t n = var_off[v1] - base;
@@ -232,8 +237,9 @@ __device__ __forceinline__ unsigned long long var_tile_state(
             const double b = b1 / (b0 + b1);
             const double diff = fabs(belief[v0] - b);
             belief[v0] = b;
-            vprod[v0] = make_double2(prod_0, prod_1);
-            vzc[v0] = pack_zc(z0, z1);
+            double rest;
+            vzc[v0] = pack_prod(prod_0, prod_1, z0, z1, rest);
+            vprod[v0] = rest;
             diff_bits = (unsigned long long)__double_as_longlong(diff);
         }
         return diff_bits;
@@ -278,14 +284,14 @@ __device__ __forceinline__ unsigned long long var_tile_state(
         const double diff = fabs(sm.bel[lv] - b);
         sm.bel[lv] = b;
         diff_bits = (unsigned long long)__double_as_longlong(diff);
-        sm.prod[0][lv] = prod_0;
-        sm.prod[1][lv] = prod_1;
-        sm.z[lv] = pack_zc(z0, z1);
+        double rest;
+        sm.z[lv] = pack_prod(prod_0, prod_1, z0, z1, rest);
+        sm.prod[0][lv] = rest;
     }
     __syncthreads();
     /* phase 3: products, codes and beliefs, coalesced */
     if ((uint32_t)tid < nv) {
-        vprod[v0 + tid] = make_double2(sm.prod[0][tid], sm.prod[1][tid]);
+        vprod[v0 + tid] = sm.prod[0][tid];
         vzc[v0 + tid] = (uint8_t)sm.z[tid];
         belief[v0 + tid] = sm.bel[tid];
     }
@@ -297,12 +303,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
     const uint32_t *__restrict__ blk, const uint32_t *__restrict__ fac_off,
     const uint32_t *__restrict__ fac_meta, const double *__restrict__ fac_p,
     const double *__restrict__ fac_q, const double2 *__restrict__ fac_mask,
-    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ fac_work, const double2 *__restrict__ vprod,
+    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ fac_work, const double *__restrict__ vprod,
     const uint8_t *__restrict__ vzc, const double2 *m_vf, double2 *m_fv, double2 *scratch, double c, double damp,
     double keep, const unsigned long long *__restrict__ ctl, int k, double tol) {
     if (chunk_stopped(ctl, k, tol)) return;
     __shared__ StateFactorSmem sm;
-    const uint32_t b = blockIdx.x;
+    /* tiles of neighbouring head variables read neighbouring product
+     * records: runs of kStateFacRun tiles per XCD share them in one L2 */
+    const uint32_t b = xcd_runs<kStateFacRun>(blockIdx.x, gridDim.x);
     factor_tile_state<kDamp, N, kMode>(fac_off, fac_meta, fac_p, fac_q, fac_mask, idx, fac_work, vprod, vzc, m_vf,
                                        m_fv, scratch, c, damp, keep, blk[b], blk[b + 1], sm);
 }
@@ -310,7 +318,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void factor_pass_st
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_state(
     const uint32_t *__restrict__ blk, const uint32_t *__restrict__ var_off,
     const uint32_t *__restrict__ var_edge, const uint16_t *__restrict__ var_work,
-    const double2 *__restrict__ m_fv, double2 *__restrict__ vprod, uint8_t *__restrict__ vzc,
+    const double2 *__restrict__ m_fv, double *__restrict__ vprod, uint8_t *__restrict__ vzc,
     double *__restrict__ belief, unsigned long long *__restrict__ ctl, int k, double tol) {
     if (chunk_stopped(ctl, k, tol)) return;
     __shared__ VarSmem sm;
@@ -324,7 +332,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_state
 /* the v->f messages of the state layout, materialized in its storage
  * order: vf[s] = vf_msg(products of the variable at storage position s,
  * f->v[s]) */
-__global__ __attribute__((unused)) void vf_from_state(const double2 *__restrict__ vprod, const uint8_t *__restrict__ vzc,
+__global__ __attribute__((unused)) void vf_from_state(const double *__restrict__ vprod, const uint8_t *__restrict__ vzc,
                                                       const uint32_t *__restrict__ fac_var,
                                                       const double2 *__restrict__ m_fv, double2 *__restrict__ vf,
                                                       uint64_t n) {
