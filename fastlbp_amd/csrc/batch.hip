@@ -53,6 +53,11 @@ struct Batch {
     uint64_t max_fac_deg = 0;
     std::vector<GroupHost> hg;
     std::vector<uint32_t> h_fac_off, h_fac_meta, h_var_off;   /* host copies for tiling */
+    /* per group, by its structure edges: device variable of each slot, slot
+     * of each var-major edge, and (once tiles are cut) the storage position
+     * of each slot -- a factor tile keeps its messages in its work order */
+    std::vector<uint32_t> h_slot_var, h_var_slot, h_store;
+    bool msgs_live = false;   /* messages set by init or a run: a new storage order moves them */
     std::vector<uint32_t> h_perm;
     std::vector<uint32_t> var_group;         /* variant -> group */
     GroupDev *groups = nullptr;
@@ -160,6 +165,7 @@ struct Batch {
                         act_n, act_group, act_fpre, act_vpre, tol};
     }
     int build_tiles();
+    int move_messages(const std::vector<uint32_t> &store);
     /* after batch_stop: the flat lists of the running groups' tiles */
     void expand() {
         const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n_ftiles + kBlock - 1) / kBlock, 1024));
@@ -187,12 +193,19 @@ struct Batch {
  * tile), twice that for K >= kDirectK, so each tile fills a block's LDS
  * slots (batch_tile_cap) with all K variants;
  * per tile the plain solve's work orders.  A factor wider than its bound
- * gets a tile of its own and a scratch range for its v->f inputs. */
+ * gets a tile of its own and a scratch range for its v->f inputs.  A
+ * factor tile stores its messages in its work order (a wide factor's in
+ * slot order), which fixes fac_var and var_edge; messages already set by
+ * init or a run move to the new order. */
 int Batch::build_tiles() {
     const uint32_t sf = opts.tile_factors, sv = opts.tile_vars;
     if (sf == cut_f && sv == cut_v && ftiles) return FLBP_OK;
     drop_exec();
     std::vector<std::vector<Tile>> ft(G), vt(G);
+    std::vector<uint32_t> store(Etot);
+    /* the body outputs that share one register path go in variable order,
+     * as in a plan (the register degree the sweep launches pick) */
+    const uint32_t reg_deg = max_fac_deg <= (uint64_t)kFacRegDegSmall ? kFacRegDegSmall : kFacRegDeg;
     std::vector<uint64_t> scr(G, 0);
     std::vector<uint32_t> fw(Etot);
     std::vector<uint16_t> vw(Vtot);
@@ -209,8 +222,15 @@ int Batch::build_tiles() {
         std::vector<uint16_t> gvw;
         cut_blocks(fo, d.n_fac, kStageItems, fs, fb);
         cut_blocks(vo, d.n_var, vi, vs, vb);
-        stage_work_orders(fo, h_fac_meta.data() + d.fac_base, fb, vo, vb, gfw, gvw);
+        stage_work_orders(fo, h_fac_meta.data() + d.fac_base, fb, vo, vb, gfw, gvw, h_slot_var.data() + d.edge_base,
+                          reg_deg);
         std::copy(gfw.begin(), gfw.end(), fw.begin() + d.edge_base);
+        for (size_t b = 0; b + 1 < fb.size(); b++) {
+            const uint32_t base = fo[fb[b]], n = fo[fb[b + 1]] - base;
+            const bool wide_tile = ((uint64_t)n << d.lk) > (uint64_t)batch_tile_cap(d.K);
+            for (uint32_t i = 0; i < n; i++)
+                store[d.edge_base + (wide_tile ? base + i : base + (gfw[base + i] & 1023u))] = base + i;
+        }
         std::copy(gvw.begin(), gvw.end(), vw.begin() + d.var_base);
         for (size_t b = 0; b + 1 < fb.size(); b++) {
             Tile t{(uint32_t)g, fb[b], fb[b + 1] - fb[b], 0u};
@@ -279,9 +299,54 @@ int Batch::build_tiles() {
     HIP_TRY(hipMemcpy(groups, gd.data(), G * sizeof(GroupDev), hipMemcpyHostToDevice));
     if (Etot) HIP_TRY(hipMemcpy(fac_work, fw.data(), Etot * 4, hipMemcpyHostToDevice));
     if (Vtot) HIP_TRY(hipMemcpy(var_work, vw.data(), Vtot * 2, hipMemcpyHostToDevice));
+    if (msgs_live && Etot && store != h_store) TRY(move_messages(store));
+    h_store.swap(store);
+    if (Etot) {
+        std::vector<uint32_t> fv(Etot), ve(Etot);
+        for (uint64_t g = 0; g < G; g++) {
+            const uint64_t eb = hg[g].d.edge_base, Eg = hg[g].n_edges;
+            for (uint64_t e = 0; e < Eg; e++) {
+                fv[eb + h_store[eb + e]] = h_slot_var[eb + e];
+                ve[eb + e] = h_store[eb + h_var_slot[eb + e]];
+            }
+        }
+        HIP_TRY(hipMemcpy(fac_var, fv.data(), Etot * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(var_edge, ve.data(), Etot * 4, hipMemcpyHostToDevice));
+    }
     cut_f = sf;
     cut_v = sv;
     return FLBP_OK;
+}
+
+/* messages from the storage order h_store to `store`, every group and
+ * variant (through a temporary copy of the message array) */
+int Batch::move_messages(const std::vector<uint32_t> &store) {
+    uint32_t *d_from = nullptr, *d_to = nullptr;
+    double2 *tmp = nullptr;
+    auto cleanup = [&]() {
+        for (void *q : {(void *)d_from, (void *)d_to, (void *)tmp})
+            if (q) (void)hipFree(q);
+    };
+    if (hipMalloc(&d_from, Etot * 4) != hipSuccess || hipMalloc(&d_to, Etot * 4) != hipSuccess ||
+        hipMalloc(&tmp, Mtot * 16) != hipSuccess) {
+        (void)hipGetLastError();
+        cleanup();
+        return fail(FLBP_ENOMEM, "batch: no room to move the messages to the new tiles' order");
+    }
+    hipError_t e = hipMemcpy(d_from, h_store.data(), Etot * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_to, store.data(), Etot * 4, hipMemcpyHostToDevice);
+    for (uint64_t g = 0; g < G && e == hipSuccess; g++) {
+        const GroupDev &d = hg[g].d;
+        const uint64_t n = hg[g].n_edges << d.lk;
+        if (!n) continue;
+        hipLaunchKernelGGL(bmove_msgs, dim3(grid_for(n, 256)), dim3(256), 0, stream, m_fv + d.msg_base,
+                           tmp + d.msg_base, d_from + d.edge_base, d_to + d.edge_base, n, d.lk);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(m_fv, tmp, Mtot * 16, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    cleanup();
+    return e == hipSuccess ? FLBP_OK : hip_fail(e, "batch: moving messages to the new tiles' order");
 }
 
 /* n sweeps: factor pass, variable pass and stop kernel each */
@@ -515,6 +580,8 @@ int flbp_batch_create_groups(uint64_t n_groups, const flbp_batch_group *gs, int 
     B->h_fac_off.resize(F + n_groups);
     B->h_fac_meta.resize(F);
     B->h_var_off.resize(V + n_groups);
+    B->h_slot_var.resize(E);
+    B->h_var_slot.resize(E);
     /* structure upload, group by group; head masks per clamped factor */
     std::vector<double2> mask_all;
     std::vector<uint32_t> crow(F, 0);
@@ -618,12 +685,11 @@ int flbp_batch_create_groups(uint64_t n_groups, const flbp_batch_group *gs, int 
         B->h_perm.insert(B->h_perm.end(), perm.begin(), perm.end());
         std::copy(dev_off.begin(), dev_off.end(), B->h_var_off.begin() + d.var_off_base);
         HIP_TRY(hipMemcpy(B->var_off + d.var_off_base, dev_off.data(), (Vg + 1) * 4, hipMemcpyHostToDevice));
-        if (Eg) {
-            HIP_TRY(hipMemcpy(B->var_edge + d.edge_base, dev_edge.data(), Eg * 4, hipMemcpyHostToDevice));
-            std::vector<uint32_t> inv(Vg), fv(Eg);
+        if (Eg) {   /* fac_var and var_edge go up with the tiles (build_tiles) */
+            std::copy(dev_edge.begin(), dev_edge.end(), B->h_var_slot.begin() + d.edge_base);
+            std::vector<uint32_t> inv(Vg);
             for (uint64_t dv = 0; dv < Vg; dv++) inv[perm[dv]] = (uint32_t)dv;
-            for (uint64_t s = 0; s < Eg; s++) fv[s] = inv[c.edge_var[s]];
-            HIP_TRY(hipMemcpy(B->fac_var + d.edge_base, fv.data(), Eg * 4, hipMemcpyHostToDevice));
+            for (uint64_t s = 0; s < Eg; s++) B->h_slot_var[d.edge_base + s] = inv[c.edge_var[s]];
         }
     }
     TRY(B->alloc(B->masks, mask_all.size()));
@@ -672,6 +738,7 @@ int flbp_batch_init(flbp_batch *b, int logdomain) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(B->stream));
     B->init_c = c;
+    B->msgs_live = false;   /* every message holds c: no storage order to keep */
     B->drop_exec();   /* the captured kernels carry the init value */
     return FLBP_OK;
 }
@@ -690,6 +757,7 @@ int flbp_batch_run(flbp_batch *b, const flbp_run_opts *o, uint64_t *iters_out, d
     for (uint64_t v = 0; v < NV; v++) most = std::max<uint64_t>(most, o->maxiter > B->iters[v] ? o->maxiter - B->iters[v] : 0);
     most = std::min<uint64_t>(most, 0xffffffffu);
     if (most) {
+        B->msgs_live = true;
         TRY(B->start_run(o->maxiter, o->tol));
         /* chunk 0: the whole solve as one replay; else replays of `chunk`
          * sweeps, enqueued back to back (at most two ahead of the host,

@@ -72,13 +72,14 @@ struct Tile {
 constexpr int kBatchSlots = 8;
 
 /* Tiles of groups with K >= kDirectK variants hold up to kBatchStage
- * message slots (all variants) and store their outputs straight from the
- * compute phase: lanes next to each other compute the same slot of
- * neighbouring variants, so each run of K outputs is one contiguous
- * K * 16-byte store, and no LDS staging of outputs is needed.  Tiles of
- * smaller groups hold kStage slots and stage their outputs in the upper
- * half of the same LDS buffer, as the plain kernels do (their outputs in
- * work order would be scattered stores). */
+ * message slots (all variants), tiles of smaller groups kStage.  Factor
+ * tiles store their messages in work order, so every lane stores its
+ * outputs straight from the compute phase to contiguous positions.  Variable
+ * tiles of K >= kDirectK store their records straight from the compute
+ * phase too (lanes next to each other handle one variable of neighbouring
+ * variants: each run of K records is one contiguous store); those of
+ * smaller groups stage them in the upper half of the LDS buffer, as the
+ * plain kernels do. */
 constexpr uint32_t kDirectK = 4;
 constexpr int kBatchStage = 2 * kStage;
 constexpr int kBatchPerLane = kBatchStage / kBlock;
@@ -194,13 +195,15 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
     const uint32_t nk = n << lk;
     const int tid = threadIdx.x;
     double2 *const fv = A.m_fv + gr.msg_base + ((uint64_t)base << lk);
-    const bool direct = K >= kDirectK;
     if (nk > batch_tile_cap(K)) {
         batch_wide_factor<kDamp>(A, gr, tile, damp, keep);
         return;
     }
-    double2 *const s_in = sm.buf;
-    double2 *const s_out = sm.buf + kStage;   /* staged outputs of a tile of K < kDirectK (nk <= kStage) */
+    /* the tile's messages are stored in its work order: position p of the
+     * tile holds work entry p (fac_var and fac_work are indexed by
+     * position), variant k of it at p * K + k.  Lane order is storage
+     * order, so every load and store of the tile is contiguous. */
+    double2 *const s_in = sm.buf;   /* v->f inputs by slot: slot s, variant k at s * K + k */
     /* phase 1 */
     const uint32_t nf = f1 - f0;
     uint32_t w[kBatchPerLane], ix[kBatchPerLane];
@@ -227,13 +230,12 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
             const double pr = A.vprod[v];
             const uint32_t zc = A.vzc[v];
             const bool fresh = A.vfresh[gr.vbase + k] != 0;
-            s_in[i] = fresh ? make_double2(A.c, A.c) : vf_of(pr, zc, old);
-            if (kDamp && !direct) s_out[i] = old;
+            s_in[((w[j] & 1023u) << lk) + k] = fresh ? make_double2(A.c, A.c) : vf_of(pr, zc, old);
         }
     }
     __syncthreads();
-    /* phase 2: position i of the tile's outputs is work entry i >> lk of
-     * variant i & (K - 1); messages of slot s, variant k at LDS s*K + k */
+    /* phase 2: lane position i computes work entry i >> lk of variant
+     * i & (K - 1) and stores it at its own position */
 #pragma unroll 1
     for (int j = 0; j < kBatchPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
@@ -261,18 +263,16 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
         const double2 *fin = s_in + (s << lk) + k;
         double2 r;
         if (d <= N) {
+            /* operand kk is slot kk + [kk >= a] + [kk + 1 >= b] (factor_output) */
             const bool head_target = (t == h);
-            const int skip_h = head_target ? -1 : h;
             const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
+            const int a = head_target ? t : (t < h ? t : h);
+            const int b = head_target ? N + 1 : (t < h ? h : t);
             double2 ops[N - 1];
-            int js = 0;
 #pragma unroll
             for (int kk = 0; kk < N - 1; kk++) {
-                js += (js == t || js == skip_h);
-                js += (js == t || js == skip_h);
-                const double2 v = fin[(uint32_t)(kk < cnt ? js : 0) << lk];   /* index kept inside the factor */
-                ops[kk] = kk < cnt ? v : make_double2(0.0, 0.0);
-                js++;
+                const int js = kk + (kk >= a) + (kk + 1 >= b);
+                ops[kk] = fin[(uint32_t)(js < d - 1 ? js : d - 1) << lk];
             }
             r = fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, fin[(uint32_t)h << lk], p, q, mask0, mask1);
         } else {
@@ -280,34 +280,16 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
             const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
             r = fv_msg<0>(in, type, clamped, d, t, h, mh, mo, p, q, mask0, mask1);
         }
-        const uint32_t o = (sl << lk) + k;
-        if (direct) {
-            /* the slot's old message is read back here, once every lane has
-             * passed phase 1; each output slot belongs to one lane */
-            if (A.vrun[gr.vbase + k]) {
-                if (kDamp) {
-                    const double2 old2 = fv[o];
-                    r.x = keep * r.x + damp * old2.x;
-                    r.y = keep * r.y + damp * old2.y;
-                }
-                fv[o] = r;
-            }
-        } else {
+        /* the old message of this position was read by this lane in phase
+         * 1 and is read back here; no other lane writes it */
+        if (A.vrun[gr.vbase + k]) {
             if (kDamp) {
-                const double2 old2 = s_out[o];
+                const double2 old2 = fv[i];
                 r.x = keep * r.x + damp * old2.x;
                 r.y = keep * r.y + damp * old2.y;
             }
-            s_out[o] = r;
+            fv[i] = r;
         }
-    }
-    if (direct) return;
-    __syncthreads();
-    /* phase 3: the variants that run store their new messages */
-#pragma unroll
-    for (int j = 0; j < kStage / kBlock; j++) {
-        const uint32_t i = tid + j * kBlock;
-        if (i < nk && A.vrun[gr.vbase + (i & km)]) fv[i] = s_out[i];
     }
 }
 
@@ -637,6 +619,18 @@ __global__ __attribute__((unused)) void bfill_pairs(double2 *p, uint64_t n, doub
 __global__ __attribute__((unused)) void bfill_u8(uint8_t *p, uint64_t n, uint8_t c) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = c;
+}
+
+/* one group's messages to a new storage order: variant k of slot e moves
+ * from position from[e] to position to[e] (n = slots << lk) */
+__global__ __attribute__((unused)) void bmove_msgs(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                   const uint32_t *__restrict__ from, const uint32_t *__restrict__ to,
+                                                   uint64_t n, uint32_t lk) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const uint64_t e = i >> lk, k = i & ((1u << lk) - 1u);
+        dst[((uint64_t)to[e] << lk) + k] = src[((uint64_t)from[e] << lk) + k];
+    }
 }
 
 /* out[i] = src[idx[i]] */

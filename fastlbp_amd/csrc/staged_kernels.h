@@ -235,21 +235,23 @@ __device__ __forceinline__ double2 factor_output(const Smem &sm, uint32_t sl, ui
         mask1 = mk.y;
     }
     if (d <= N) {
-        /* the other slots' messages, in slot order, from LDS */
+        /* the other slots' messages, in slot order, from LDS.  With the
+         * skipped slots a < b (the target and, for a body target, the
+         * head; b past the factor for a head target), operand k is slot
+         * k + [k >= a] + [k + 1 >= b].  Operands past cnt are never used:
+         * their reads are clamped to the factor's last slot. */
         const bool head_target = (t == h);
-        const int skip_h = head_target ? -1 : h;
         const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
+        const int a = head_target ? t : (t < h ? t : h);
+        const int b = head_target ? N + 1 : (t < h ? h : t);
+        const double2 *const in = sm.in + s;
         double2 ops[N - 1];
-        int js = 0;
 #pragma unroll
         for (int k = 0; k < N - 1; k++) {
-            js += (js == t || js == skip_h);
-            js += (js == t || js == skip_h);
-            const double2 v = sm.in[s + (k < cnt ? js : 0)];   /* index kept inside the factor */
-            ops[k] = k < cnt ? v : make_double2(0.0, 0.0);
-            js++;
+            const int js = k + (k >= a) + (k + 1 >= b);
+            ops[k] = in[js < d - 1 ? js : d - 1];
         }
-        return fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, sm.in[s + h], p, q, mask0, mask1);
+        return fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, in[h], p, q, mask0, mask1);
     }
     LdsIn in{sm.in + s};
     const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);

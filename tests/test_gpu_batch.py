@@ -107,6 +107,22 @@ def test_maxiter_cap_undamped_and_continued_runs():
             v += 1
 
 
+@pytest.mark.parametrize("K", [1, 2, 8])
+def test_new_tile_bounds_between_runs(K):
+    """New tile bounds between two runs re-cut the tiles, and with them the
+    order the factor tiles store their messages in: the stored messages
+    move along, so the continued run ends exactly where one run of the
+    same sweeps does (the oracle's 9 + rest sweeps)."""
+    gens, groups = mixed_batch(5, 4, K, 2)
+    b = Batch(groups)
+    b.init()
+    b.run(1e-14, 9, 0.3)
+    b.set_opts(tile_factors=128, tile_vars=256)
+    iters, md, _ = b.run(1e-9, 300, 0.3)
+    b.set_opts(tile_factors=512, tile_vars=512)
+    check_against_oracle(b, gens, 1e-9, 300, 0.3, iters, md)
+
+
 def test_logdomain_init_fixed_sweeps():
     """logdomain init and tol = -inf (fixed sweeps through NaN beliefs)"""
     gens, groups = mixed_batch(3, 3, 4, 1)
