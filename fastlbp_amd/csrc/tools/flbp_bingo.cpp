@@ -3,7 +3,7 @@
  * examples/bingo_wrapper.cu:25-195) on the MI355X engine, through the
  * drop-in CausalFactorGraph / CausalBP classes.
  *
- * usage: flbp_bingo <file.causal_fg> [seed] [--auto | --manual id] [--digits N]
+ * usage: flbp_bingo <file.causal_fg | file.fg> [seed] [--auto | --manual id] [--digits N]
  *
  * Commands, whitespace-separated tokens on stdin:
  *   O <var index> true|false                 clamp; echoes "O <index> <value>"
@@ -31,6 +31,12 @@
  * (its construct writes past its buffers).  With head clamps only the copy
  * is used as it is, as in the reference.
  * --auto takes device 0 (one process per GPU); --manual id takes device id.
+ *
+ * A file whose extension is not .causal_fg is read as a generic libDAI .fg
+ * graph, as the reference does (bingo_wrapper.cu:156-165), which then runs
+ * no inference: BP prints 1 (or, with a clamp set, ends the program with
+ * "We don't support clamp for BP", bingo_wrapper.cu:25-41) and Q prints 0
+ * (bingo_wrapper.cu:43-53).
  */
 #include <chrono>
 #include <cstdio>
@@ -41,6 +47,7 @@
 #include <string>
 
 #include <ext/causal_bp.h>
+#include <lbp/factorgraph.h>
 #include <utils/utils.h>
 
 using namespace lbp;
@@ -48,6 +55,8 @@ using namespace lbp;
 namespace {
 
 CausalFactorGraph g_fg;
+FactorGraph g_generic;
+bool g_is_causal = true;
 PropertySet g_opts;
 std::unique_ptr<CausalBP> g_bp;
 std::map<int, bool> g_clamps;
@@ -75,13 +84,13 @@ void init_bp() {
     g_bp->init();
 }
 
-bool valid_index(int i) { return i >= 0 && (size_t)i < g_fg.nrVars(); }
+bool valid_index(int i) { return i >= 0 && (size_t)i < (g_is_causal ? g_fg.nrVars() : g_generic.nrVars()); }
 
 }  // namespace
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::cerr << "usage: " << argv[0] << " <file.causal_fg> [seed] [--auto | --manual id] [--digits N]\n";
+        std::cerr << "usage: " << argv[0] << " <file.causal_fg | file.fg> [seed] [--auto | --manual id] [--digits N]\n";
         return 1;
     }
     int digits = 6;
@@ -108,13 +117,16 @@ int main(int argc, char **argv) {
     }
     const std::string file = argv[1];
     const size_t dot = file.find_last_of('.');
-    if (dot == std::string::npos || file.substr(dot + 1) != "causal_fg") {
-        std::cerr << "flbp_bingo: only .causal_fg graphs are supported\n";
-        return 1;
+    g_is_causal = dot != std::string::npos && file.substr(dot + 1) == "causal_fg";
+    if (g_is_causal) {
+        g_fg.ReadFromFile(file.c_str());
+        std::clog << "flbp_bingo: read " << g_fg.nrVars() << " variables, " << g_fg.nrFactors() << " factors"
+                  << std::endl;
+    } else {
+        g_generic.ReadFromFile(file.c_str());
+        std::clog << "flbp_bingo: read a generic graph of " << g_generic.nrVars() << " variables, "
+                  << g_generic.nrFactors() << " factors (no inference runs on it)" << std::endl;
     }
-    g_fg.ReadFromFile(file.c_str());
-    std::clog << "flbp_bingo: read " << g_fg.nrVars() << " variables, " << g_fg.nrFactors() << " factors"
-              << std::endl;
     g_opts.set("maxiter", static_cast<size_t>(10000000));
     g_opts.set("maxtime", Real(57600));
     g_opts.set("tol", Real(1e-6));
@@ -128,6 +140,10 @@ int main(int argc, char **argv) {
         if (cmd == "Q") {
             int i;
             std::cin >> i;
+            if (!g_is_causal) {
+                std::cout << Real(0) << std::endl;
+                continue;
+            }
             if (!g_bp) LOG_FATAL("Q before any BP");
             if (!valid_index(i)) LOG_FATAL("Q: no variable %d", i);
             std::cout << g_bp->newBelief((size_t)i) << std::endl;
@@ -136,6 +152,11 @@ int main(int argc, char **argv) {
             size_t min_iters, max_iters, hist;
             std::cin >> tol >> min_iters >> max_iters >> hist;
             std::clog << "flbp_bingo: BP " << tol << " " << min_iters << " " << max_iters << " " << hist << std::endl;
+            if (!g_is_causal) {
+                if (!g_clamps.empty()) LOG_FATAL("We don't support clamp for BP");
+                std::cout << Real(1) << std::endl;
+                continue;
+            }
             const auto t0 = std::chrono::steady_clock::now();
             init_bp();
             const auto t1 = std::chrono::steady_clock::now();

@@ -6,7 +6,13 @@
  * label up to the largest one, %.17g; clamped labels that left the graph
  * print their clamp value, absent labels 0.5.  The solve time goes to stderr.
  *
- * usage: flbp_fl_wrapper <file.causal_fg> <observations> [sweeps]
+ * A file whose extension is not .causal_fg is read as a generic libDAI .fg
+ * graph, as the reference does (fl_wrapper.cu:106-115); the reference runs
+ * no inference on it: an observation ends the program with "We don't
+ * support clamp for BP" (fl_wrapper.cu:39-42), and without one the empty
+ * causal graph's report is the single line "0 0.5" (fl_wrapper.cu:57-88).
+ *
+ * usage: flbp_fl_wrapper <file.causal_fg | file.fg> <observations> [sweeps]
  */
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +24,8 @@
 #include <vector>
 
 #include <ext/causal_bp.h>
+#include <lbp/factorgraph.h>
+#include <utils/log.h>
 
 using namespace lbp;
 
@@ -26,8 +34,13 @@ int main(int argc, char **argv) {
         std::cerr << "Incorrect number of arguments." << std::endl;
         return 1;
     }
+    const std::string file = argv[1];
+    const size_t dot = file.find_last_of('.');
+    const bool is_causal = dot != std::string::npos && file.substr(dot + 1) == "causal_fg";
     CausalFactorGraph causal_fg;
-    causal_fg.ReadFromFile(argv[1]);
+    FactorGraph fg;
+    if (is_causal) causal_fg.ReadFromFile(argv[1]);
+    else fg.ReadFromFile(argv[1]);
     std::ifstream observeFile(argv[2]);
     if (!observeFile.is_open()) {
         std::cerr << "Cannot open observation file!\n";
@@ -47,14 +60,19 @@ int main(int argc, char **argv) {
     opts.set("updates", std::string("PARALL"));
     opts.set("logdomain", false);
 
-    for (const auto &c : clamps)
+    for (const auto &c : clamps) {
+        if (!is_causal) LOG_FATAL("We don't support clamp for BP");
         for (size_t i = 0; i < causal_fg.nrVars(); i++)
             if (causal_fg.var(i).label() == (size_t)c.first) causal_fg.clamp(i, c.second ? 1 : 0);
+    }
     CausalFactorGraph new_causal_fg(causal_fg.factors());
     Real tic1 = toc();
-    std::unique_ptr<CausalBP> bp(new CausalBP(new_causal_fg, opts));
-    bp->init();
-    bp->run(1e-6, sweeps, 1);
+    std::unique_ptr<CausalBP> bp;
+    if (is_causal) {
+        bp.reset(new CausalBP(new_causal_fg, opts));
+        bp->init();
+        bp->run(1e-6, sweeps, 1);
+    }
     std::clog << toc() - tic1 << std::endl;
 
     size_t maxVarLabel = 0;

@@ -128,3 +128,49 @@ def test_index_past_the_table_keeps_the_digits_that_fit(tmp_path):
 
 def test_missing_file_exits_with_an_error(tmp_path):
     assert dump(str(tmp_path / "none.fg")).returncode != 0
+
+
+WRAP = os.path.join(ROOT, "fastlbp_amd", "bin", "flbp_fl_wrapper")
+BINGO = os.path.join(ROOT, "fastlbp_amd", "bin", "flbp_bingo")
+
+
+def small_fg(tmp_path):
+    path = str(tmp_path / "g.fg")
+    write_fg(path, [([3, 7], [2, 2], [(0, 0.25), (1, 0.5), (3, 0.125)]), ([7], [2], [(0, 0.6), (1, 0.4)])])
+    return path
+
+
+def test_fl_wrapper_reads_a_generic_graph_and_solves_nothing(tmp_path):
+    """A file that is not .causal_fg goes to the generic reader
+    (fl_wrapper.cu:106-115), and the reference runs no inference on it:
+    without observations the empty causal graph reports the single line
+    "0 0.5" (fl_wrapper.cu:57-88); an observation ends the program with
+    "We don't support clamp for BP" (fl_wrapper.cu:39-42).  No GPU call is
+    made on this path."""
+    path = small_fg(tmp_path)
+    obs = tmp_path / "none.obs"
+    obs.write_text("")
+    r = subprocess.run([WRAP, path, str(obs)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == "0 0.5\n"
+    obs.write_text("7 true\n")
+    r = subprocess.run([WRAP, path, str(obs)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "We don't support clamp for BP" in r.stderr
+    bad = tmp_path / "bad.fg"
+    bad.write_text("1\n\n2\n3 7\n2 2\nx\n")
+    r = subprocess.run([WRAP, str(bad), str(obs)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and r.stdout == ""
+
+
+def test_bingo_on_a_generic_graph(tmp_path):
+    """bingo_wrapper.cu:25-53 on a generic graph: BP prints the
+    yet-to-converge fraction 1 and Q prints 0, as no inference runs; O and
+    UC echo; a BP with a clamp set ends the program."""
+    path = small_fg(tmp_path)
+    r = subprocess.run([BINGO, path], input="BP 1e-6 1 10 1\nQ 0\nO 1 true\nUC 1\nBP 1e-6 1 10 1\nNL\nBYE\n",
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == "1\n0\nO 1 true\nUC 1\n1\n\n"
+    r = subprocess.run([BINGO, path], input="O 1 false\nBP 1e-6 1 10 1\n", capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0 and "We don't support clamp for BP" in r.stderr and r.stdout == "O 1 false\n"
