@@ -263,17 +263,9 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
         const double2 *fin = s_in + (s << lk) + k;
         double2 r;
         if (d <= N) {
-            /* operand kk is slot kk + [kk >= a] + [kk + 1 >= b] (factor_output) */
             const bool head_target = (t == h);
             const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
-            const int a = head_target ? t : (t < h ? t : h);
-            const int b = head_target ? N + 1 : (t < h ? h : t);
-            double2 ops[N - 1];
-#pragma unroll
-            for (int kk = 0; kk < N - 1; kk++) {
-                const int js = kk + (kk >= a) + (kk + 1 >= b);
-                ops[kk] = fin[(uint32_t)(js < d - 1 ? js : d - 1) << lk];
-            }
+            const SkipIn ops{fin, head_target ? t : (t < h ? t : h), head_target ? N + 1 : (t < h ? h : t), lk};
             r = fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, fin[(uint32_t)h << lk], p, q, mask0, mask1);
         } else {
             const StridedIn in{fin, lk};

@@ -431,9 +431,15 @@ int flbp_plan_init(flbp_plan *plan, int logdomain) {
     HIP_TRY(hipSetDevice(P->device));
     const double c = logdomain ? 0.0 : 1.0;
     if (P->E) {
-        hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_vf, P->E, c);
+        /* the state layout keeps only the f->v messages: its first factor
+         * pass takes the init value as its v->f input (vf_src), and the
+         * other arrays are written whenever the plan leaves the layout
+         * (flbp_plan_set_variant moves the messages) */
+        if (P->variant != 4)
+            hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_vf, P->E, c);
         hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_fv, P->E, c);
-        hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_fv_vm, P->E, c);
+        if (P->variant != 4)
+            hipLaunchKernelGGL(fill_pairs, dim3(grid_for(P->E, 256)), dim3(256), 0, P->stream, P->m_fv_vm, P->E, c);
     }
     if (int rc = flbp::seq_reset_state(P, c)) return rc;
     HIP_TRY(hipGetLastError());

@@ -41,15 +41,30 @@ struct LdsIn {
     __device__ __forceinline__ double2 pick(int j) const { return m[j]; }
 };
 
-/* One f->v output from a compact operand list: ops[0..cnt) are the
+/* The operands of one output of a factor whose slots' messages start at
+ * in[0] (stride 1 << lk): operand k is the k-th slot other than the
+ * skipped slots a < b (the target and, for a body target, the head; b past
+ * the factor for a head target), slot k + [k >= a] + [k + 1 >= b].  An
+ * operand is read only when the loop that uses it runs, so a wave whose
+ * outputs have few operands loads and indexes only those. */
+struct SkipIn {
+    const double2 *in;
+    int a, b;
+    uint32_t lk;
+    __device__ __forceinline__ double2 get(int k) const {
+        return in[(uint32_t)(k + (k >= a) + (k + 1 >= b)) << lk];
+    }
+};
+
+/* One f->v output from a compact operand list: ops.get(0..cnt) are the
  * messages of the factor's slots other than the target t (and, for a body
  * target, other than the head h), in slot order -- exactly the slots the
  * per-output loops of fv_msg visit, in the same order -- so the loops run
  * cnt steps with no per-slot tests and the arithmetic is fv_msg's.  mh is
- * the head slot's message; for a degree-2 head output ops[0] is the other
- * slot. */
-template <int M>
-__device__ __forceinline__ double2 fv_list(const double2 (&ops)[M], int cnt, uint32_t type, bool clamped, int d,
+ * the head slot's message; for a degree-2 head output ops.get(0) is the
+ * other slot.  M bounds cnt. */
+template <int M, class Ops>
+__device__ __forceinline__ double2 fv_list(const Ops &ops, int cnt, uint32_t type, bool clamped, int d,
                                            bool head_target, double2 mh, double p, double q, double mask0,
                                            double mask1) {
     double2 r;
@@ -70,7 +85,7 @@ __device__ __forceinline__ double2 fv_list(const double2 (&ops)[M], int cnt, uin
 #pragma unroll
             for (int k = 0; k < M; k++) {
                 if (k < cnt) {
-                    const double2 m = ops[k];
+                    const double2 m = ops.get(k);
                     Q *= m.x + m.y;
                     P *= is_and ? m.y : m.x;
                     scale_pair(P, Q);
@@ -95,7 +110,7 @@ __device__ __forceinline__ double2 fv_list(const double2 (&ops)[M], int cnt, uin
         u = p;
         w = 1 - p;
     } else if (d == 2) {
-        const double2 m = ops[0];
+        const double2 m = ops.get(0);
         const double x = is_and ? m.y : m.x, y = is_and ? m.x : m.y;
         u = p * x + q * y;
         w = (1 - p) * x + (1 - q) * y;
@@ -104,7 +119,7 @@ __device__ __forceinline__ double2 fv_list(const double2 (&ops)[M], int cnt, uin
 #pragma unroll
         for (int k = 0; k < M; k++) {
             if (k < cnt) {
-                const double2 m = ops[k];
+                const double2 m = ops.get(k);
                 const double a0 = m.x + m.y;
                 const double x = is_and ? m.y : m.x;
                 const double delta = is_and ? m.x : m.y;
@@ -235,23 +250,11 @@ __device__ __forceinline__ double2 factor_output(const Smem &sm, uint32_t sl, ui
         mask1 = mk.y;
     }
     if (d <= N) {
-        /* the other slots' messages, in slot order, from LDS.  With the
-         * skipped slots a < b (the target and, for a body target, the
-         * head; b past the factor for a head target), operand k is slot
-         * k + [k >= a] + [k + 1 >= b].  Operands past cnt are never used:
-         * their reads are clamped to the factor's last slot. */
+        /* the other slots' messages, in slot order, from LDS */
         const bool head_target = (t == h);
         const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
-        const int a = head_target ? t : (t < h ? t : h);
-        const int b = head_target ? N + 1 : (t < h ? h : t);
-        const double2 *const in = sm.in + s;
-        double2 ops[N - 1];
-#pragma unroll
-        for (int k = 0; k < N - 1; k++) {
-            const int js = k + (k >= a) + (k + 1 >= b);
-            ops[k] = in[js < d - 1 ? js : d - 1];
-        }
-        return fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, in[h], p, q, mask0, mask1);
+        const SkipIn ops{sm.in + s, head_target ? t : (t < h ? t : h), head_target ? N + 1 : (t < h ? h : t), 0u};
+        return fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, sm.in[s + h], p, q, mask0, mask1);
     }
     LdsIn in{sm.in + s};
     const double2 mh = in.pick(h), mo = in.pick(h == 0 ? 1 : 0);
