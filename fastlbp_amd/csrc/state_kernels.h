@@ -322,8 +322,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_state
     double *__restrict__ belief, unsigned long long *__restrict__ ctl, int k, double tol) {
     if (chunk_stopped(ctl, k, tol)) return;
     __shared__ VarSmem sm;
-    /* gathers from factor-major lines, as the pull layout's var pass */
-    const uint32_t b = xcd_runs<kPullVarRun>(blockIdx.x, gridDim.x);
+    /* gathers from the factor tiles' lines: runs of kStateVarRun tiles per XCD */
+    const uint32_t b = xcd_runs<kStateVarRun>(blockIdx.x, gridDim.x);
     const unsigned long long bits =
         var_tile_state(var_off, var_edge, var_work, m_fv, vprod, vzc, belief, blk[b], blk[b + 1], sm);
     block_fold_diff(ctl, k, bits, sm.red);
