@@ -1,6 +1,7 @@
 """Edge cases on the GPU against the oracle: the empty graph (no variables,
-no factors), a graph of prior factors only, and batches and variant batches
-that mix them with ordinary graphs."""
+no factors), a graph of prior factors only, batches and variant batches
+that mix them with ordinary graphs, and NaN probabilities spreading
+through messages, products and beliefs."""
 import numpy as np
 import pytest
 
@@ -94,3 +95,39 @@ def test_variant_batch_of_one_is_the_single_problem():
     g1 = prob.grad(2.0 * (b1 - y))
     assert same_bits(b[0], b1)
     assert np.allclose(g, g1, rtol=1e-12, atol=1e-14 * np.max(np.abs(g1)))
+
+
+@pytest.mark.parametrize("variant", [0, 3, 4])
+def test_nan_probabilities_spread_as_in_the_oracle(variant):
+    """Factors with p = NaN make NaN messages, which spread into products,
+    beliefs and the rescaled product pairs (a NaN non-unit value, or a NaN
+    unit once both values are NaN: the 8-byte product records of the state
+    layout keep both).  Every message and belief matches the oracle's
+    sweeps, NaN for NaN, and the finite ones bit for bit."""
+    print(f"[edges] NaN probabilities, variant {variant}", flush=True)
+    gen = fl.generate(6000, 71)
+    f = gen.factors
+    rng = np.random.default_rng(71)
+    p = f.p.copy()
+    p[rng.choice(f.n_factors, size=5, replace=False)] = np.nan
+    flist = fl.FactorList(f.type, f.head, f.body_off, f.body, p, f.q, f.head_clamped, f.mask0, f.mask1)
+    g = fl.Graph(flist)
+    plan = fl.Plan(g)
+    plan.set_variant(variant)
+    plan.init()
+    o = oracle.OracleGraph(flist)
+    o.init()
+    for n in (1, 1, 6):   # 14 NaN beliefs after one sweep, 1,799 after two, all after eight
+        plan.run_fixed(n, 0.3)
+        o.run_fixed(n, 0.3)
+        fv, vf = plan.messages()
+        ofv, ovf = o.messages()
+        assert same_bits(fv, ofv) and same_bits(vf, ovf), n
+        assert same_bits(plan.beliefs(), o.beliefs()), n
+        if n == 1:
+            b = plan.beliefs()
+            assert np.isnan(b).any() and np.isfinite(b).any()
+    r = plan.run(1e-9, 200, 0.3)   # the max change is NaN: the reference's loop stops
+    it, md = o.run(1e-9, 200, 0.3)
+    assert r.iters == it and same_bits([r.maxdiff], [md])
+    assert same_bits(plan.beliefs(), o.beliefs())
