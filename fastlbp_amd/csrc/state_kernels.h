@@ -176,8 +176,10 @@ __device__ __forceinline__ void factor_tile_state(
     }
     __syncthreads();
     /* phase 2: each lane's outputs in work order, stored straight to its
-     * storage positions (coalesced: lane order is storage order) */
-#pragma unroll 1
+     * storage positions (coalesced: lane order is storage order); unrolled,
+     * so each output's work entry and old message stay in their registers
+     * (57.3 against 58.0 us per launch with the loop rolled) */
+#pragma unroll
     for (int j = 0; j < kPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         if (i >= n) break;
