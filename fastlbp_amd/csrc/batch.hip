@@ -268,6 +268,9 @@ int Batch::build_tiles() {
         if (fall.size() >= (1ull << 31) || vall.size() >= (1ull << 31) || scr_tot >= (1ull << 32))
             return fail(FLBP_ERANGE, "batch: too many tiles");
     }
+    /* messages set by a run move to the new order first: a failure leaves
+     * the batch as it was */
+    if (msgs_live && Etot && store != h_store) TRY(move_messages(store));
     n_ftiles = (uint32_t)fall.size();
     n_vtiles = (uint32_t)vall.size();
     if (n_ftiles > cap_ftiles) {
@@ -299,7 +302,6 @@ int Batch::build_tiles() {
     HIP_TRY(hipMemcpy(groups, gd.data(), G * sizeof(GroupDev), hipMemcpyHostToDevice));
     if (Etot) HIP_TRY(hipMemcpy(fac_work, fw.data(), Etot * 4, hipMemcpyHostToDevice));
     if (Vtot) HIP_TRY(hipMemcpy(var_work, vw.data(), Vtot * 2, hipMemcpyHostToDevice));
-    if (msgs_live && Etot && store != h_store) TRY(move_messages(store));
     h_store.swap(store);
     if (Etot) {
         std::vector<uint32_t> fv(Etot), ve(Etot);

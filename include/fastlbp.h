@@ -468,6 +468,8 @@ int flbp_batch_beliefs(flbp_batch *b, uint64_t v, double *out, uint64_t n);
 int flbp_batch_beliefs_at(flbp_batch *b, uint64_t v, uint64_t n, const uint64_t *vars, double *out);
 /* p of every factor of group g's structure (shared by its variants) */
 int flbp_batch_set_p(flbp_batch *b, uint64_t g, const double *p, uint64_t n);
+/* new options; new tile bounds re-cut the tiles, and the messages a run
+ * has set move with them, so a solve may continue under the new bounds */
 int flbp_batch_set_opts(flbp_batch *b, const flbp_batch_opts *opts);
 int flbp_batch_get_opts(flbp_batch *b, flbp_batch_opts *opts);
 /* iteration counters and beliefs back to their creation values */
