@@ -463,12 +463,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_factor_p
 
 __global__ __launch_bounds__(kBlock) __attribute__((unused)) void batch_var_pass(BatchArgs A) {
     __shared__ BatchVarSmem sm;
-    /* the active tiles in runs of kStateVarRun per XCD, as the plain state
+    /* the active tiles in runs of kBatchVarRun per XCD, as the plain state
      * var pass gathers: the 768-graph batch 267.5 -> 247.1 ms (runs of 64:
      * 253.2 ms; runs of 16 in the factor pass: 269.0 ms) */
     const uint32_t na = A.act_count[1];
     if (blockIdx.x >= na) return;
-    const uint32_t ti = xcd_runs<kStateVarRun>(blockIdx.x, na);
+    const uint32_t ti = xcd_runs<kBatchVarRun>(blockIdx.x, na);
     const Tile tile = A.act_vtiles[ti];
     const GroupDev gr = A.groups[tile.group];
     batch_var_tile(A, gr, tile, ti, sm);

@@ -198,8 +198,8 @@ int Plan::launch_sweep_on(int k, double tol, double damping, const double2 *vf_i
         }
         if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
         if (V)
-            hipLaunchKernelGGL(var_pass_state, dim3((unsigned)n_var_blocks), dim3(kBlock), 0, stream, var_blk, var_off,
-                               var_src, var_work, m_fv, vprod, vzc, bel, ctl, k, tol);
+            hipLaunchKernelGGL(var_pass_state, dim3((unsigned)n_var_blocks4), dim3(kBlock), 0, stream, var_blk4,
+                               var_off, var_src, var_work4, m_fv, vprod, vzc, bel, ctl, k, tol);
         if (profiling) HIP_TRY(hipEventRecord(ev[2 * k + 2], stream));
         HIP_TRY(hipGetLastError());
         /* from the second sweep on, the v->f inputs come from the products

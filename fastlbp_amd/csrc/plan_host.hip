@@ -225,6 +225,10 @@ void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const 
         std::sort(keyed.begin(), keyed.end());
         for (uint32_t i = 0; i < n; i++) fac_work[base + i] = keyed[i].second;
     }
+    var_work_order(var_off, vb, var_work);
+}
+
+void var_work_order(const uint32_t *var_off, const std::vector<uint32_t> &vb, std::vector<uint16_t> &var_work) {
     const uint64_t V = vb.back();
     var_work.assign(V, 0);
     std::vector<std::pair<uint64_t, uint16_t>> vk;
@@ -356,6 +360,17 @@ int Plan::upload(const flbp_csr &c_ref) {
         stage_work_orders(fo.data(), meta.data(), fb, dev_off.data(), vb, fw, vw, sv.data(), reg_deg);
         TRY(alloc(fac_work, E));
         TRY(alloc(var_work, V));
+        {   /* the state layout's variable tiles */
+            std::vector<uint32_t> vb4;
+            std::vector<uint16_t> vw4;
+            cut_blocks(dev_off.data(), V, kStageItems, kStateVarSlots, vb4);
+            var_work_order(dev_off.data(), vb4, vw4);
+            n_var_blocks4 = vb4.size() - 1;
+            TRY(alloc(var_blk4, vb4.size()));
+            TRY(alloc(var_work4, V));
+            HIP_TRY(hipMemcpy(var_blk4, vb4.data(), vb4.size() * 4, hipMemcpyHostToDevice));
+            if (V) HIP_TRY(hipMemcpy(var_work4, vw4.data(), V * 2, hipMemcpyHostToDevice));
+        }
         if (E) HIP_TRY(hipMemcpy(fac_work, fw.data(), E * 4, hipMemcpyHostToDevice));
         if (V) HIP_TRY(hipMemcpy(var_work, vw.data(), V * 2, hipMemcpyHostToDevice));
         /* the state layout's storage order: a staged tile's slots in its

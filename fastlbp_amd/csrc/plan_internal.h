@@ -73,6 +73,11 @@ enum : int { kVfState = 0, kVfConst = 1, kVfArray = 2 };
 constexpr int kMaxChunk = 64;   /* sweeps enqueued between host checks */
 constexpr uint32_t kStageItems = 256;    /* factors / variables per staged block (== kBlock) */
 constexpr uint32_t kStageSlots = 512;    /* edge slots per staged block (== kStage) */
+/* edge slots per variable tile of the state layout: its var pass stages no
+ * outputs, so a tile can gather more messages at once (on the 1M-variable
+ * graph 36.0 us per launch against 38.1 with 512-slot tiles; 640: 37.2,
+ * 896: 38.2, 1,024: 40.4) */
+constexpr uint32_t kStateVarSlots = 768;
 /* a plan's default layout: the scatter layout up to this footprint, the
  * state layout above it (plan_host.hip has the measurements) */
 constexpr uint64_t kStateMinBytes = 64ull << 20;
@@ -109,6 +114,8 @@ void cut_blocks(const uint32_t *off, uint64_t n_items, uint32_t max_items, uint3
 void stage_work_orders(const uint32_t *fac_off, const uint32_t *fac_meta, const std::vector<uint32_t> &fb,
                        const uint32_t *var_off, const std::vector<uint32_t> &vb, std::vector<uint32_t> &fac_work,
                        std::vector<uint16_t> &var_work, const uint32_t *slot_var = nullptr, uint32_t reg_deg = 0);
+/* the variable half of stage_work_orders: per block, its variables by degree */
+void var_work_order(const uint32_t *var_off, const std::vector<uint32_t> &vb, std::vector<uint16_t> &var_work);
 void var_device_order(const flbp_csr &c, int order, std::vector<uint32_t> &perm,
                       std::vector<uint32_t> &dev_off, std::vector<uint32_t> &dev_edge,
                       std::vector<uint32_t> &vpos);
@@ -146,6 +153,9 @@ struct Plan {
     uint32_t *fac_blk = nullptr, *var_blk = nullptr;   /* staged block ranges */
     uint32_t *fac_work = nullptr;   /* per staged factor block: its output slots in work order */
     uint16_t *var_work = nullptr;   /* per staged variable block: its variables in work order */
+    uint32_t *var_blk4 = nullptr;   /* the state layout's variable tiles (kStateVarSlots) */
+    uint16_t *var_work4 = nullptr;  /* ... and their work orders */
+    uint64_t n_var_blocks4 = 0;
     uint64_t n_fac_blocks = 0, n_var_blocks = 0;
     int variant = 3;   /* 0 staged pull, 1 one lane per item, 2 staged push, 3 staged scatter, 4 staged state */
     double *vprod = nullptr;        /* variant 4: per-variable products (device order, pack_prod) */

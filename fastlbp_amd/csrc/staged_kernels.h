@@ -511,9 +511,12 @@ constexpr uint32_t kPullVarRun = 64;
  * 256 tiles: 64.1 and 63.5 us, one run per XCD 64.8 us) */
 constexpr uint32_t kStateFacRun = 16;
 /* tiles per XCD run in the state layout's var pass: on the 1M-variable
- * graph 38.4 us per launch against 40.6 with runs of 64 (16: 47.9, 32:
- * 44.6, 128: 38.7, 512: 39.2, 1,024: 40.3, one run per XCD: 41.1) */
-constexpr uint32_t kStateVarRun = 256;
+ * graph with 512-slot tiles 38.4 us per launch against 40.6 with runs of 64
+ * (16: 47.9, 32: 44.6, 128: 38.7, 512: 39.2, 1,024: 40.3, one run per XCD:
+ * 41.1); with 768-slot tiles runs of 512 36.0 us (128: 36.9, 256: 36.4).
+ * The batched var pass takes runs of 256. */
+constexpr uint32_t kStateVarRun = 512;
+constexpr uint32_t kBatchVarRun = 256;
 
 /* the factor pass of a plain solve: block b owns factors [blk[b], blk[b+1]) */
 template <bool kDamp, int N, int kLayout>

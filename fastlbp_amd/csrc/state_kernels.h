@@ -202,6 +202,20 @@ __device__ __forceinline__ void factor_tile_state(
     }
 }
 
+/* LDS of one variable tile of the state layout (kStateVarSlots slots:
+ * the pass stages no outputs, so its tiles hold more messages than the
+ * other layouts' 512) */
+constexpr int kVarStage = (int)flbp::kStateVarSlots;
+constexpr int kVarPerLane = (kVarStage + kBlock - 1) / kBlock;
+struct StateVarSmem {
+    double2 m[kVarStage];
+    double rest[kBlock];
+    uint32_t z[kBlock];
+    uint32_t start[kBlock + 1];
+    double bel[kBlock];
+    unsigned long long red[kBlock / 64];
+};
+
 /* The variable pass over variables [v0, v1) in the state layout: the f->v
  * messages gathered from their factor slots, then each variable's product
  * pair, zero-count code and belief stored.  Returns the lane's max
@@ -209,18 +223,18 @@ __device__ __forceinline__ void factor_tile_state(
 __device__ __forceinline__ unsigned long long var_tile_state(
     const uint32_t *__restrict__ var_off, const uint32_t *__restrict__ var_edge,
     const uint16_t *__restrict__ var_work, const double2 *__restrict__ m_fv, double *__restrict__ vprod,
-    uint8_t *__restrict__ vzc, double *__restrict__ belief, uint32_t v0, uint32_t v1, VarSmem &sm) {
+    uint8_t *__restrict__ vzc, double *__restrict__ belief, uint32_t v0, uint32_t v1, StateVarSmem &sm) {
     const uint32_t base = var_off[v0];
     const uint32_t n = var_off[v1] - base;
     const int tid = threadIdx.x;
     unsigned long long diff_bits = 0;
-    if (n > (uint32_t)kStage) {
+    if (n > (uint32_t)kVarStage) {
         /* one wide variable: the product in edge order on lane 0 over
-         * kStage-slot chunks gathered by the whole block */
+         * kVarStage-slot chunks gathered by the whole block */
         double prod_0 = 1.0, prod_1 = 1.0;
         uint32_t z0 = 0, z1 = 0;
-        for (uint32_t c = 0; c < n; c += kStage) {
-            const uint32_t m = n - c < (uint32_t)kStage ? n - c : (uint32_t)kStage;
+        for (uint32_t c = 0; c < n; c += kVarStage) {
+            const uint32_t m = n - c < (uint32_t)kVarStage ? n - c : (uint32_t)kVarStage;
             for (uint32_t e = tid; e < m; e += kBlock) sm.m[e] = m_fv[var_edge[base + c + e]];
             __syncthreads();
             if (tid == 0) {
@@ -250,9 +264,9 @@ __device__ __forceinline__ unsigned long long var_tile_state(
      * all loads independent of each other */
     const uint32_t nv = v1 - v0;
     uint32_t lv = 0;
-    uint32_t ve[kPerLane];
+    uint32_t ve[kVarPerLane];
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
+    for (int j = 0; j < kVarPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         ve[j] = i < n ? var_edge[base + i] : 0u;
     }
@@ -263,7 +277,7 @@ __device__ __forceinline__ unsigned long long var_tile_state(
     }
     if (tid == 0) sm.start[nv] = n;
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
+    for (int j = 0; j < kVarPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
         if (i < n) sm.m[i] = m_fv[ve[j]];
     }
@@ -288,12 +302,12 @@ __device__ __forceinline__ unsigned long long var_tile_state(
         diff_bits = (unsigned long long)__double_as_longlong(diff);
         double rest;
         sm.z[lv] = pack_prod(prod_0, prod_1, z0, z1, rest);
-        sm.prod[0][lv] = rest;
+        sm.rest[lv] = rest;
     }
     __syncthreads();
     /* phase 3: products, codes and beliefs, coalesced */
     if ((uint32_t)tid < nv) {
-        vprod[v0 + tid] = sm.prod[0][tid];
+        vprod[v0 + tid] = sm.rest[tid];
         vzc[v0 + tid] = (uint8_t)sm.z[tid];
         belief[v0 + tid] = sm.bel[tid];
     }
@@ -323,7 +337,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((unused)) void var_pass_state
     const double2 *__restrict__ m_fv, double *__restrict__ vprod, uint8_t *__restrict__ vzc,
     double *__restrict__ belief, unsigned long long *__restrict__ ctl, int k, double tol) {
     if (chunk_stopped(ctl, k, tol)) return;
-    __shared__ VarSmem sm;
+    __shared__ StateVarSmem sm;
     /* gathers from the factor tiles' lines: runs of kStateVarRun tiles per XCD */
     const uint32_t b = xcd_runs<kStateVarRun>(blockIdx.x, gridDim.x);
     const unsigned long long bits =
