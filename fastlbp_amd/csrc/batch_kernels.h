@@ -186,6 +186,7 @@ __device__ __forceinline__ void batch_wide_factor(const BatchArgs &A, const Grou
 template <bool kDamp, int N>
 __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const GroupDev &gr, const Tile &tile,
                                                   double damp, double keep, BatchFactorSmem &sm) {
+    static_assert(N <= 8, "SkipIn packs the operands of factors of degree 8 or less");
     const uint32_t *fac_off = A.fac_off + gr.fac_off_base;
     const uint64_t fb = gr.fac_base, eb = gr.edge_base;
     const uint32_t f0 = tile.first, f1 = tile.first + tile.count;
@@ -266,7 +267,8 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
         if (d <= N) {
             const bool head_target = (t == h);
             const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
-            const SkipIn ops{fin, head_target ? t : (t < h ? t : h), head_target ? N + 1 : (t < h ? h : t), lk};
+            const SkipIn ops =
+                SkipIn::make(fin, head_target ? t : (t < h ? t : h), head_target ? N + 1 : (t < h ? h : t), lk);
             r = fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, fin[(uint32_t)h << lk], p, q, mask0, mask1);
         } else {
             const StridedIn in{fin, lk};

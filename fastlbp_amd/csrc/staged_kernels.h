@@ -44,15 +44,24 @@ struct LdsIn {
 /* The operands of one output of a factor whose slots' messages start at
  * in[0] (stride 1 << lk): operand k is the k-th slot other than the
  * skipped slots a < b (the target and, for a body target, the head; b past
- * the factor for a head target), slot k + [k >= a] + [k + 1 >= b].  An
- * operand is read only when the loop that uses it runs, so a wave whose
- * outputs have few operands loads and indexes only those. */
+ * the factor for a head target), slot k + [k >= a] + [k + 1 >= b].  The
+ * skip counts of operands 0..7 are packed once per output as 2-bit fields
+ * of sk (field k: [k >= a] from 0x5555 << 2a, plus [k >= b - 1]; a field
+ * holds at most 2, so the two terms add without carries), so each operand
+ * costs one field extract.  An operand is read only when the loop that
+ * uses it runs, so a wave whose outputs have few operands loads and
+ * indexes only those. */
 struct SkipIn {
     const double2 *in;
-    int a, b;
+    uint32_t sk;
     uint32_t lk;
+    /* a < b, a <= 7, b <= 9 */
+    __device__ __forceinline__ static SkipIn make(const double2 *in, int a, int b, uint32_t lk) {
+        return SkipIn{in, (0x5555u << (2 * a)) + (0x5555u << (2 * (b - 1))), lk};
+    }
+    /* k <= 6 (operand lists of factors of degree 8 or less) */
     __device__ __forceinline__ double2 get(int k) const {
-        return in[(uint32_t)(k + (k >= a) + (k + 1 >= b)) << lk];
+        return in[(uint32_t)(k + ((sk >> (2 * k)) & 3u)) << lk];
     }
 };
 
@@ -234,6 +243,7 @@ struct VarSmem {
 template <int N, class Smem>
 __device__ __forceinline__ double2 factor_output(const Smem &sm, uint32_t sl, uint32_t lf, uint32_t f0,
                                                  const double2 *__restrict__ fac_mask) {
+    static_assert(N <= 8, "SkipIn packs the operands of factors of degree 8 or less");
     const uint32_t s = sm.start[lf];
     const int d = (int)(sm.start[lf + 1] - s);
     const int t = (int)(sl - s);
@@ -253,7 +263,8 @@ __device__ __forceinline__ double2 factor_output(const Smem &sm, uint32_t sl, ui
         /* the other slots' messages, in slot order, from LDS */
         const bool head_target = (t == h);
         const int cnt = type == 0 ? 0 : d - (head_target ? 1 : 2);
-        const SkipIn ops{sm.in + s, head_target ? t : (t < h ? t : h), head_target ? N + 1 : (t < h ? h : t), 0u};
+        const SkipIn ops =
+            SkipIn::make(sm.in + s, head_target ? t : (t < h ? t : h), head_target ? N + 1 : (t < h ? h : t), 0u);
         return fv_list<N - 1>(ops, cnt, type, clamped, d, head_target, sm.in[s + h], p, q, mask0, mask1);
     }
     LdsIn in{sm.in + s};
