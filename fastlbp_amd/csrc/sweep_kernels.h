@@ -270,7 +270,8 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
  * line apart: block b folds its maximum into slot b mod kCtlSlots, so the
  * grid's atomics spread over many lines instead of queueing on one address
  * (and are skipped when the slot already holds a larger value).  A reader
- * takes the max of the slots with one wave-wide load and a DPP reduction. */
+ * takes the max of the slots with one wave-wide load and a shuffle
+ * reduction; the stop test needs only two wave votes. */
 constexpr int kCtlSlots = 64;   /* == wavefront size */
 constexpr int kCtlPad = 8;      /* u64 per slot: one 64-byte line */
 constexpr int kCtlPerSweep = kCtlSlots * kCtlPad;
@@ -293,11 +294,19 @@ __device__ __forceinline__ unsigned long long ctl_read(const unsigned long long 
  * (a sweep that never ran leaves its slots 0 and stops every later sweep).
  * tol = -inf means a fixed number of sweeps (the 3-argument run): no sweep
  * stops, whatever max diff the one before left -- NaN included, which no
- * comparison with tol could let through. */
+ * comparison with tol could let through.
+ *
+ * The slots hold non-negative doubles or NaN, whose max as bit patterns
+ * (ctl_read) is NaN when any slot is NaN and otherwise their max as
+ * doubles; so !(max > tol) holds exactly when some slot is NaN or no slot
+ * exceeds tol: two votes over the wave's 64 slots, in place of the
+ * six-step 64-bit shuffle reduction.  Called with the whole wave active. */
 __device__ __forceinline__ bool chunk_stopped(const unsigned long long *ctl, int k, double tol) {
     if (k <= 0 || tol == -__builtin_inf()) return false;
-    const double prev = __longlong_as_double((long long)ctl_read(ctl, k - 1));
-    return !(prev > tol);
+    const int lane = threadIdx.x & 63;
+    const double v = __longlong_as_double(
+        (long long)__atomic_load_n(ctl + (uint64_t)(k - 1) * kCtlPerSweep + lane * kCtlPad, __ATOMIC_RELAXED));
+    return __ballot(v > tol) == 0 || __ballot(v != v) != 0;
 }
 
 
