@@ -132,18 +132,22 @@ __device__ __forceinline__ void factor_tile_state(
      * into LDS, and each slot's v->f input (derived from the products and
      * the old f->v message) into LDS at its slot */
     const uint32_t nf = f1 - f0;
+    /* the tile's arrays from wave-uniform base pointers: each lane then
+     * addresses them by its own small offset, in one 32-bit operation */
+    const uint32_t *t_work = fac_work + base, *t_idx = idx + base;
+    double2 *t_fv = m_fv + base;
     uint32_t w[kPerLane], ix[kPerLane];
 #pragma unroll
     for (int j = 0; j < kPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
-        w[j] = i < n ? fac_work[base + i] : 0u;
-        ix[j] = (kMode == kVfState && i < n) ? idx[base + i] : 0u;
+        w[j] = i < n ? t_work[i] : 0u;
+        ix[j] = (kMode == kVfState && i < n) ? t_idx[i] : 0u;
     }
     if ((uint32_t)tid < nf) {
-        sm.start[tid] = fac_off[f0 + tid] - base;
-        sm.meta[tid] = fac_meta[f0 + tid];
-        sm.p[tid] = fac_p[f0 + tid];
-        sm.q[tid] = fac_q[f0 + tid];
+        sm.start[tid] = (fac_off + f0)[tid] - base;
+        sm.meta[tid] = (fac_meta + f0)[tid];
+        sm.p[tid] = (fac_p + f0)[tid];
+        sm.q[tid] = (fac_q + f0)[tid];
     }
     if (tid == 0) sm.start[nf] = n;
     double2 old[kPerLane], vin[kPerLane];
@@ -154,12 +158,12 @@ __device__ __forceinline__ void factor_tile_state(
         const uint32_t i = tid + j * kBlock;
         old[j] = make_double2(0.0, 0.0);
         if (i < n) {
-            if (kDamp || kMode == kVfState) old[j] = m_fv[base + i];
+            if (kDamp || kMode == kVfState) old[j] = t_fv[i];
             if constexpr (kMode == kVfState) {
                 pr[j] = vprod[ix[j]];
                 zc[j] = vzc[ix[j]];
             } else if constexpr (kMode == kVfArray) {
-                vin[j] = m_vf[base + i];
+                vin[j] = (m_vf + base)[i];
             }
         }
     }
@@ -198,7 +202,7 @@ __device__ __forceinline__ void factor_tile_state(
             r.x = keep * r.x + damp * oj.x;
             r.y = keep * r.y + damp * oj.y;
         }
-        m_fv[base + i] = r;
+        t_fv[i] = r;
     }
 }
 
@@ -263,17 +267,21 @@ __device__ __forceinline__ unsigned long long var_tile_state(
     /* phase 1: offsets, old beliefs, work entries and the tile's messages,
      * all loads independent of each other */
     const uint32_t nv = v1 - v0;
+    /* the tile's arrays from wave-uniform base pointers (one 32-bit lane
+     * offset per access) */
+    double *t_bel = belief + v0, *t_prod = vprod + v0;
+    uint8_t *t_zc = vzc + v0;
     uint32_t lv = 0;
     uint32_t ve[kVarPerLane];
 #pragma unroll
     for (int j = 0; j < kVarPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
-        ve[j] = i < n ? var_edge[base + i] : 0u;
+        ve[j] = i < n ? (var_edge + base)[i] : 0u;
     }
     if ((uint32_t)tid < nv) {
-        lv = var_work[v0 + tid];
-        sm.start[tid] = var_off[v0 + tid] - base;
-        sm.bel[tid] = belief[v0 + tid];
+        lv = (var_work + v0)[tid];
+        sm.start[tid] = (var_off + v0)[tid] - base;
+        sm.bel[tid] = t_bel[tid];
     }
     if (tid == 0) sm.start[nv] = n;
 #pragma unroll
@@ -307,9 +315,9 @@ __device__ __forceinline__ unsigned long long var_tile_state(
     __syncthreads();
     /* phase 3: products, codes and beliefs, coalesced */
     if ((uint32_t)tid < nv) {
-        vprod[v0 + tid] = sm.rest[tid];
-        vzc[v0 + tid] = (uint8_t)sm.z[tid];
-        belief[v0 + tid] = sm.bel[tid];
+        t_prod[tid] = sm.rest[tid];
+        t_zc[tid] = (uint8_t)sm.z[tid];
+        t_bel[tid] = sm.bel[tid];
     }
     return diff_bits;
 }
