@@ -257,13 +257,33 @@ __device__ __forceinline__ double2 vf_msg(double prod_0, double prod_1, uint32_t
     return r;
 }
 
+/* one step of the wave max: each lane takes the max of its value and the
+ * value DPP control kCtrl routes to it (lanes without a source, or outside
+ * the row and bank masks, keep their own) */
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ unsigned long long dpp_max_step(unsigned long long v) {
+    const int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+    const uint32_t olo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, kCtrl, kRowMask, 0xf, false);
+    const uint32_t ohi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, kCtrl, kRowMask, 0xf, false);
+    const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
+    return o > v ? o : v;
+}
+
+/* max over the wave's 64 lanes, returned to every lane (the whole wave
+ * active): row_shr 1, 2, 4, 8 leave each 16-lane row's max in its lane 15;
+ * row_bcast 15 (rows 1 and 3) and row_bcast 31 (rows 2 and 3) carry those
+ * into lane 63, which is then read.  Register moves only, no LDS round
+ * trips as in a shuffle reduction. */
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
+    v = dpp_max_step<0x111, 0xf>(v);   /* row_shr:1 */
+    v = dpp_max_step<0x112, 0xf>(v);   /* row_shr:2 */
+    v = dpp_max_step<0x114, 0xf>(v);   /* row_shr:4 */
+    v = dpp_max_step<0x118, 0xf>(v);   /* row_shr:8 */
+    v = dpp_max_step<0x142, 0xa>(v);   /* row_bcast:15 */
+    v = dpp_max_step<0x143, 0xc>(v);   /* row_bcast:31 */
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
 }
 
 /* Per-sweep max |b - b_old| is kept in kCtlSlots partial maxima, one cache
