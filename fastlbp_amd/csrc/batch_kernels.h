@@ -234,6 +234,10 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
             s_in[((w[j] & 1023u) << lk) + k] = fresh ? make_double2(A.c, A.c) : vf_of(pr, zc, old);
         }
     }
+    /* the run flag of the lane's variant (every position of a lane,
+     * tid + j * kBlock, has variant tid & (K - 1)), read once here rather
+     * than before each store of phase 2 */
+    const bool run_k = A.vrun[gr.vbase + (tid & km)] != 0;
     __syncthreads();
     /* phase 2: lane position i computes work entry i >> lk of variant
      * i & (K - 1) and stores it at its own position (unrolled, so each
@@ -277,7 +281,7 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
         }
         /* the old message of this position was read by this lane in phase
          * 1 and is read back here; no other lane writes it */
-        if (A.vrun[gr.vbase + k]) {
+        if (run_k) {
             if (kDamp) {
                 const double2 old2 = fv[i];
                 r.x = keep * r.x + damp * old2.x;
