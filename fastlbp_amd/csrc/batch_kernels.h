@@ -389,6 +389,10 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
         const uint32_t i = tid + j * kBlock;
         if (i < nk) s_m[i] = fv[((uint64_t)ve[j] << lk) + (i & km)];
     }
+    /* the run flag of the lane's variant (both of a lane's entries, and on
+     * lanes 0..K-1 the fold below, belong to variant tid & (K - 1)), read
+     * once here */
+    const bool run_k = A.vrun[gr.vbase + (tid & km)] != 0;
     __syncthreads();
     /* phase 2: lane t (and t + kBlock in a direct tile) is work entry
      * t >> lk (a variable) of variant t & (K - 1); both entries of a lane
@@ -415,7 +419,7 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
         const double b = b1 / (b0 + b1);
         if (direct) {
             /* runs of K neighbouring lanes store K contiguous records */
-            if (A.vrun[gr.vbase + k]) {
+            if (run_k) {
                 const double diff = fabs(A.belief[vst + o] - b);
                 A.belief[vst + o] = b;
                 double rest;
@@ -440,14 +444,14 @@ __device__ __forceinline__ void batch_var_tile(const BatchArgs &A, const GroupDe
     if (!direct) {
         __syncthreads();
         /* phase 3: the variants that run store products, codes and beliefs */
-        if ((uint32_t)tid < nvk && A.vrun[gr.vbase + (tid & km)]) {
+        if ((uint32_t)tid < nvk && run_k) {
             A.vprod[vst + tid] = s_prod[tid];
             A.vzc[vst + tid] = (uint8_t)s_z[tid];
             A.belief[vst + tid] = s_bel[tid];
         }
     }
     /* per-variant fold: lanes 0..K-1 of each wave hold their variant's max */
-    if (lane < K && A.vrun[gr.vbase + lane]) fold_diff(A, gr.vbase + lane, ti, diff_bits);
+    if (lane < K && run_k) fold_diff(A, gr.vbase + lane, ti, diff_bits);
 }
 
 /* The sweep kernels launch one block per tile of the batch; block b takes
