@@ -221,6 +221,9 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
         sm.q[tid] = A.fac_q[fb + f0 + tid];
     }
     if (tid == 0) sm.start[nf] = n;
+    /* whether the lane's variant starts from the init constant (all of a
+     * lane's positions have variant tid & (K - 1)), read once */
+    const bool fresh = A.vfresh[gr.vbase + (tid & km)] != 0;
 #pragma unroll
     for (int j = 0; j < kBatchPerLane; j++) {
         const uint32_t i = tid + j * kBlock;
@@ -230,7 +233,6 @@ __device__ __forceinline__ void batch_factor_tile(const BatchArgs &A, const Grou
             const double2 old = fv[i];
             const double pr = A.vprod[v];
             const uint32_t zc = A.vzc[v];
-            const bool fresh = A.vfresh[gr.vbase + k] != 0;
             s_in[((w[j] & 1023u) << lk) + k] = fresh ? make_double2(A.c, A.c) : vf_of(pr, zc, old);
         }
     }
